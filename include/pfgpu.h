@@ -1,0 +1,156 @@
+/*
+ * pfgpu — MI355X-native nonlinear-elasticity assembly for PolyFEM.
+ *
+ * C-ABI of the GPU library.  Plain pointers and sizes only; no exceptions
+ * cross this boundary.  Every entry point returns a pfgpu_status and leaves a
+ * message for pfgpu_last_error() (thread-local) when it fails.
+ *
+ * The boundary replaces PolyFEM's in-process assembler API
+ * (src/polyfem/assembler/Assembler.hpp:53-376): an ElasticityNLAssembler
+ * adapter forwards its overrides to these entry points.  Conventions follow
+ * the reference:
+ *   - dof layout is node-major and interleaved, dof = node*3 + d
+ *     (Assembler.cpp:624,729,734);
+ *   - element-local ordering is a*3 + d (NeoHookeanElasticity.cpp:540-544);
+ *   - P2 local node order is v0..v3, e01, e12, e20, e03, e13, e23
+ *     (autogen/auto_p_bases.cpp:1437-1448);
+ *   - the Hessian is returned as sorted CSC with int32 indices, the
+ *     StiffnessMatrix type (utils/Types.hpp:21-25); the pattern is symmetric,
+ *     so outer/inner are also the CSR row pointers / column indices;
+ *   - the handle owns the sparsity pattern and the scatter maps that the
+ *     caller-owned, stateful MatrixCache carries in the reference
+ *     (utils/MatrixCache.hpp:43-134).
+ *
+ * Threading: one handle belongs to one HIP stream; calls on one handle are
+ * serialised by the caller; distinct handles are independent.  Host-pointer
+ * entry points are synchronous.  *_dev entry points take device pointers,
+ * enqueue on the handle's stream and return without synchronising.
+ */
+#ifndef PFGPU_H
+#define PFGPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    PFGPU_OK = 0,
+    PFGPU_ERR_INVALID_ARGUMENT = 1,
+    PFGPU_ERR_HIP = 2,
+    /* a non-finite energy, gradient or Hessian entry (an inverted element):
+     * outputs hold NaN, as the reference's do (ElasticForm.cpp:388-405) */
+    PFGPU_ERR_INVALID_STATE = 3,
+    PFGPU_ERR_UNSUPPORTED = 4,
+    PFGPU_ERR_OUT_OF_MEMORY = 5
+} pfgpu_status;
+
+typedef enum { PFGPU_NEOHOOKEAN = 0, PFGPU_LINEAR_ELASTICITY = 1 } pfgpu_material_kind;
+
+/* FP64: every kernel in double.  MIXED: element kernels in float; the energy
+ * sum, gradient accumulation, norms and CG dot products in double; Hessian
+ * values are float. */
+typedef enum { PFGPU_FP64 = 0, PFGPU_MIXED = 1 } pfgpu_precision;
+
+/* Assembly algorithm.  FUSED: patch-owned rows, on-chip deterministic
+ * reduction (the default).  PLAIN: element-local results written to HBM, then
+ * gathered into the global vector / CSR in a fixed order. */
+typedef enum { PFGPU_PATH_FUSED = 0, PFGPU_PATH_PLAIN = 1 } pfgpu_path;
+
+typedef struct {
+    int32_t order;            /* 1 (P1) or 2 (P2) */
+    int32_t quadrature_order; /* <= 0: max(2(p-1),1) (AssemblerUtils.cpp:225-228) */
+    int32_t n_elements;
+    int32_t n_nodes;             /* discretisation nodes; n_dofs = 3 n_nodes */
+    const int32_t *elem_nodes;   /* n_elements x (4|10), PolyFEM local order */
+    int32_t n_vertices;          /* geometric vertices */
+    const double *vertices;      /* n_vertices x 3 */
+    const int32_t *elem_vertices;/* n_elements x 4: the P1 geometry */
+} pfgpu_mesh_desc;
+
+typedef struct {
+    int32_t kind;          /* pfgpu_material_kind */
+    const double *lambda;  /* per-element Lame lambda (n_elements), or NULL */
+    const double *mu;      /* per-element Lame mu (n_elements), or NULL */
+    double young;          /* uniform E when lambda/mu are NULL */
+    double poisson;        /* uniform nu when lambda/mu are NULL */
+} pfgpu_material_desc;
+
+typedef struct {
+    int32_t order, n_local, n_quad, material, precision, path;
+    int32_t n_elements, n_nodes, n_vertices, n_dofs;
+    int64_t nnz;           /* Hessian non-zeros */
+    int64_t device_bytes;  /* device memory held by the handle */
+} pfgpu_info_t;
+
+typedef struct pfgpu_handle pfgpu_handle;
+
+/* ---- lifetime --------------------------------------------------------- */
+
+/* Builds the geometry cache (AssemblyValsCache::init, AssemblyValsCache.cpp:11-32),
+ * the Lame parameters (LameParameters::lambda_mu, MatParams.cpp:368-401; the
+ * per-element override of update_lame_params, NeoHookeanElasticity.hpp:58-62),
+ * the Hessian pattern (SparseMatrixCache first assembly, MatrixCache.cpp:134-212)
+ * and the scatter maps; uploads everything to the current device. */
+pfgpu_status pfgpu_create(const pfgpu_mesh_desc *mesh, const pfgpu_material_desc *material,
+                          int32_t precision, pfgpu_handle **out);
+pfgpu_status pfgpu_destroy(pfgpu_handle *h);
+const char *pfgpu_last_error(void);
+pfgpu_status pfgpu_info(const pfgpu_handle *h, pfgpu_info_t *info);
+/* hipStream_t passed as void*; NULL selects the default stream */
+pfgpu_status pfgpu_set_stream(pfgpu_handle *h, void *stream);
+pfgpu_status pfgpu_set_path(pfgpu_handle *h, int32_t path);
+/* replaces ElasticityAssembler::update_lame_params (Assembler.hpp:147-156) */
+pfgpu_status pfgpu_set_lame(pfgpu_handle *h, const double *lambda, const double *mu);
+
+/* ---- assembly, host pointers (synchronous) ------------------------------ */
+
+/* NLAssembler::assemble_energy (Assembler.cpp:495-531) */
+pfgpu_status pfgpu_value(pfgpu_handle *h, const double *x, double *energy);
+/* NLAssembler::assemble_energy_per_element (Assembler.cpp:533-572) */
+pfgpu_status pfgpu_value_per_element(pfgpu_handle *h, const double *x, double *out);
+/* NLAssembler::assemble_gradient (Assembler.cpp:574-643); g has n_dofs entries */
+pfgpu_status pfgpu_gradient(pfgpu_handle *h, const double *x, double *g);
+pfgpu_status pfgpu_hessian_nnz(const pfgpu_handle *h, int64_t *nnz);
+/* the CSC pattern of NLAssembler::assemble_hessian (Assembler.cpp:645-771) */
+pfgpu_status pfgpu_hessian_pattern(const pfgpu_handle *h, int32_t *outer, int32_t *inner);
+/* Hessian values in pattern order: double* for FP64, float* for MIXED.
+ * project_to_psd must be 0 (ipc::project_to_psd is not provided). */
+pfgpu_status pfgpu_hessian_values(pfgpu_handle *h, const double *x, int32_t project_to_psd, void *values);
+/* energy, gradient and Hessian values from one pass over the mesh; any
+ * output pointer may be NULL */
+pfgpu_status pfgpu_energy_gradient_hessian(pfgpu_handle *h, const double *x, double *energy,
+                                           double *g, void *values);
+/* y = H(x) v without forming H (no reference counterpart; the reference
+ * multiplies the assembled matrix) */
+pfgpu_status pfgpu_hvp(pfgpu_handle *h, const double *x, const double *v, double *y);
+/* ElasticForceDerivative::force_material_derivative (ElasticForceDerivative.cpp:35-132):
+ * out[e] = -sum_q da <dP/dlambda, grad p>, out[E+e] = -sum_q da <dP/dmu, grad p> */
+pfgpu_status pfgpu_material_vjp(pfgpu_handle *h, const double *x, const double *p, double *out);
+/* ElasticForceDerivative::force_shape_derivative (ElasticForceDerivative.cpp:135-264),
+ * out has 3 n_vertices entries, vertex*3 + d */
+pfgpu_status pfgpu_shape_vjp(pfgpu_handle *h, const double *x, const double *p, double *out);
+/* mixed derivative y = (d grad / d lambda) dlambda + (d grad / d mu) dmu,
+ * the forward counterpart of pfgpu_material_vjp (out = -(that)^T p) */
+pfgpu_status pfgpu_material_jvp(pfgpu_handle *h, const double *x, const double *dlambda,
+                                const double *dmu, double *y);
+
+/* ---- assembly, device pointers (asynchronous on the handle's stream) ---- */
+
+pfgpu_status pfgpu_value_dev(pfgpu_handle *h, const double *x, double *energy);
+pfgpu_status pfgpu_gradient_dev(pfgpu_handle *h, const double *x, double *g);
+pfgpu_status pfgpu_hessian_values_dev(pfgpu_handle *h, const double *x, void *values);
+pfgpu_status pfgpu_energy_gradient_hessian_dev(pfgpu_handle *h, const double *x, double *energy,
+                                               double *g, void *values);
+pfgpu_status pfgpu_hvp_dev(pfgpu_handle *h, const double *x, const double *v, double *y);
+pfgpu_status pfgpu_material_vjp_dev(pfgpu_handle *h, const double *x, const double *p, double *out);
+pfgpu_status pfgpu_shape_vjp_dev(pfgpu_handle *h, const double *x, const double *p, double *out);
+/* device copies of the pattern held by the handle (valid until destroy) */
+pfgpu_status pfgpu_hessian_pattern_dev(const pfgpu_handle *h, const int32_t **outer, const int32_t **inner);
+pfgpu_status pfgpu_synchronize(pfgpu_handle *h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,11 @@
+"""pfgpu: MI355X-native nonlinear-elasticity assembly for PolyFEM.
+
+The compute path is libpfgpu.so (hand-written HIP kernels for gfx950 behind
+the C-ABI in include/pfgpu.h); this package holds its Python binding, the
+mirror of PolyFEM's assembler/form API and the synthetic-mesh generator.
+"""
+from . import _ffi, mesh
+from .assembler import ElasticForm, NLAssembler, make_assembler
+from ._ffi import PfgpuError
+
+__all__ = ["ElasticForm", "NLAssembler", "make_assembler", "PfgpuError", "mesh", "_ffi"]

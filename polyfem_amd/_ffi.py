@@ -1,0 +1,100 @@
+"""ctypes binding of the libpfgpu C-ABI (include/pfgpu.h).
+
+The library must have been built (``python -m polyfem_amd.build_lib``); there
+is no fallback: loading fails loudly when it is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpfgpu.so")
+
+PFGPU_OK = 0
+STATUS_NAMES = {0: "OK", 1: "INVALID_ARGUMENT", 2: "HIP", 3: "INVALID_STATE", 4: "UNSUPPORTED", 5: "OUT_OF_MEMORY"}
+NEOHOOKEAN, LINEAR_ELASTICITY = 0, 1
+FP64, MIXED = 0, 1
+PATH_FUSED, PATH_PLAIN = 0, 1
+
+
+class MeshDesc(C.Structure):
+    _fields_ = [("order", C.c_int32), ("quadrature_order", C.c_int32), ("n_elements", C.c_int32),
+                ("n_nodes", C.c_int32), ("elem_nodes", C.c_void_p), ("n_vertices", C.c_int32),
+                ("vertices", C.c_void_p), ("elem_vertices", C.c_void_p)]
+
+
+class MaterialDesc(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("lam", C.c_void_p), ("mu", C.c_void_p),
+                ("young", C.c_double), ("poisson", C.c_double)]
+
+
+class Info(C.Structure):
+    _fields_ = [("order", C.c_int32), ("n_local", C.c_int32), ("n_quad", C.c_int32), ("material", C.c_int32),
+                ("precision", C.c_int32), ("path", C.c_int32), ("n_elements", C.c_int32), ("n_nodes", C.c_int32),
+                ("n_vertices", C.c_int32), ("n_dofs", C.c_int32), ("nnz", C.c_int64), ("device_bytes", C.c_int64)]
+
+
+class PfgpuError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"pfgpu {STATUS_NAMES.get(status, status)}: {message}")
+        self.status = status
+
+
+vp = C.c_void_p
+_SIGNATURES = {
+    "pfgpu_create": [C.POINTER(MeshDesc), C.POINTER(MaterialDesc), C.c_int32, C.POINTER(vp)],
+    "pfgpu_destroy": [vp],
+    "pfgpu_info": [vp, C.POINTER(Info)],
+    "pfgpu_set_stream": [vp, vp],
+    "pfgpu_set_path": [vp, C.c_int32],
+    "pfgpu_set_lame": [vp, vp, vp],
+    "pfgpu_value": [vp, vp, vp],
+    "pfgpu_value_per_element": [vp, vp, vp],
+    "pfgpu_gradient": [vp, vp, vp],
+    "pfgpu_hessian_nnz": [vp, vp],
+    "pfgpu_hessian_pattern": [vp, vp, vp],
+    "pfgpu_hessian_values": [vp, vp, C.c_int32, vp],
+    "pfgpu_energy_gradient_hessian": [vp, vp, vp, vp, vp],
+    "pfgpu_hvp": [vp, vp, vp, vp],
+    "pfgpu_material_vjp": [vp, vp, vp, vp],
+    "pfgpu_shape_vjp": [vp, vp, vp, vp],
+    "pfgpu_material_jvp": [vp, vp, vp, vp, vp],
+    "pfgpu_value_dev": [vp, vp, vp],
+    "pfgpu_gradient_dev": [vp, vp, vp],
+    "pfgpu_hessian_values_dev": [vp, vp, vp],
+    "pfgpu_energy_gradient_hessian_dev": [vp, vp, vp, vp, vp],
+    "pfgpu_hvp_dev": [vp, vp, vp, vp],
+    "pfgpu_material_vjp_dev": [vp, vp, vp, vp],
+    "pfgpu_shape_vjp_dev": [vp, vp, vp, vp],
+    "pfgpu_hessian_pattern_dev": [vp, vp, vp],
+    "pfgpu_synchronize": [vp],
+}
+
+_lib = None
+
+
+def exported_symbols():
+    return sorted(list(_SIGNATURES) + ["pfgpu_last_error"])
+
+
+def lib():
+    """Load libpfgpu.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise FileNotFoundError(f"{LIB_PATH} is missing: build it with `python -m polyfem_amd.build_lib`")
+        L = C.CDLL(LIB_PATH)
+        for name, args in _SIGNATURES.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = C.c_int
+        L.pfgpu_last_error.restype = C.c_char_p
+        L.pfgpu_last_error.argtypes = []
+        _lib = L
+    return _lib
+
+
+def check(status: int) -> None:
+    if status != PFGPU_OK:
+        raise PfgpuError(status, lib().pfgpu_last_error().decode())

@@ -1,0 +1,531 @@
+// The C-ABI of libpfgpu (include/pfgpu.h).  Exceptions never cross this
+// boundary: every entry point converts them to a status and a message.
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <new>
+
+#include "handle.hpp"
+
+using namespace pfgpu;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+template <typename F> pfgpu_status guard(F &&f)
+{
+    try {
+        f();
+        return PFGPU_OK;
+    } catch (const Error &e) {
+        g_last_error = e.what();
+        return e.status;
+    } catch (const std::bad_alloc &) {
+        g_last_error = "host allocation failed";
+        return PFGPU_ERR_OUT_OF_MEMORY;
+    } catch (const std::exception &e) {
+        g_last_error = e.what();
+        return PFGPU_ERR_HIP;
+    }
+}
+
+void require(bool ok, const std::string &msg)
+{
+    if (!ok) fail(PFGPU_ERR_INVALID_ARGUMENT, msg);
+}
+
+void check_handle(const pfgpu_handle *h) { require(h != nullptr, "null handle"); }
+
+// upload a host table element-major (E x k) as component-major (k x E)
+template <typename S, typename D> DevBuf upload_soa(const S *src, int E, int k, hipStream_t s)
+{
+    std::vector<D> v(size_t(E) * k);
+    for (int e = 0; e < E; ++e)
+        for (int c = 0; c < k; ++c) v[size_t(c) * E + e] = D(src[size_t(e) * k + c]);
+    return upload(v, s);
+}
+
+// ElementAssemblyValues::finalize3d for the P1 geometry (ElementAssemblyValues.cpp:73-103):
+// tmp.row(r) = sum_j dphi_j/dxi_r X_j, i.e. rows X1-X0, X2-X0, X3-X0; stores
+// J^{-T} row-major and det J.
+void geometry_host(const pfgpu_mesh_desc &m, std::vector<double> &g /* E x 10 */)
+{
+    g.resize(size_t(m.n_elements) * 10);
+    for (int e = 0; e < m.n_elements; ++e) {
+        const int32_t *v = m.elem_vertices + size_t(e) * 4;
+        for (int j = 0; j < 4; ++j)
+            require(v[j] >= 0 && v[j] < m.n_vertices, "elem_vertices references a vertex out of range");
+        const double *X0 = m.vertices + 3 * size_t(v[0]);
+        double t[9];
+        for (int r = 0; r < 3; ++r) {
+            const double *X = m.vertices + 3 * size_t(v[r + 1]);
+            for (int c = 0; c < 3; ++c) t[r * 3 + c] = X[c] - X0[c];
+        }
+        const double c00 = t[4] * t[8] - t[5] * t[7], c01 = t[5] * t[6] - t[3] * t[8], c02 = t[3] * t[7] - t[4] * t[6];
+        const double c10 = t[2] * t[7] - t[1] * t[8], c11 = t[0] * t[8] - t[2] * t[6], c12 = t[1] * t[6] - t[0] * t[7];
+        const double c20 = t[1] * t[5] - t[2] * t[4], c21 = t[2] * t[3] - t[0] * t[5], c22 = t[0] * t[4] - t[1] * t[3];
+        const double det = t[0] * c00 + t[1] * c01 + t[2] * c02;
+        require(std::isfinite(det) && det != 0.0, "element " + std::to_string(e) + " has a degenerate geometry");
+        const double id = 1.0 / det;
+        // inverse = adj / det with adj[r][c] = cof[c][r]; J^{-T}[r][c] = inverse[c][r] = cof[r][c] / det
+        const double jit[9] = {c00 * id, c01 * id, c02 * id, c10 * id, c11 * id, c12 * id, c20 * id, c21 * id, c22 * id};
+        double *out = g.data() + size_t(e) * 10;
+        for (int k = 0; k < 9; ++k) out[k] = jit[k];
+        out[9] = det;
+    }
+}
+
+void upload_lame(pfgpu_handle &h, const double *lambda, const double *mu)
+{
+    std::vector<double> lm(size_t(h.n_elements) * 2);
+    for (int e = 0; e < h.n_elements; ++e) {
+        require(std::isfinite(lambda[e]) && std::isfinite(mu[e]), "non-finite Lame parameter");
+        lm[size_t(e) * 2] = lambda[e];
+        lm[size_t(e) * 2 + 1] = mu[e];
+    }
+    if (h.precision == PFGPU_MIXED)
+        h.lame = upload_soa<double, float>(lm.data(), h.n_elements, 2, h.stream);
+    else
+        h.lame = upload_soa<double, double>(lm.data(), h.n_elements, 2, h.stream);
+}
+
+// staging of host arrays for the synchronous entry points
+const double *stage(pfgpu_handle &h, DevBuf &buf, const double *host, size_t n)
+{
+    buf.reserve(sizeof(double) * n);
+    PF_HIP(hipMemcpyAsync(buf.get(), host, sizeof(double) * n, hipMemcpyHostToDevice, h.stream));
+    return buf.as<double>();
+}
+
+void fetch(pfgpu_handle &h, void *host, const void *dev, size_t bytes)
+{
+    PF_HIP(hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, h.stream));
+    PF_HIP(hipStreamSynchronize(h.stream));
+}
+
+void gradient_dev(pfgpu_handle &h, const double *x, double *g, double *energy)
+{
+    if (h.path == PFGPU_PATH_FUSED)
+        fused_gradient(h, x, g, energy);
+    else
+        plain_gradient(h, x, g, energy);
+}
+
+void egh_dev(pfgpu_handle &h, const double *x, double *energy, double *g, void *values)
+{
+    if (h.path == PFGPU_PATH_FUSED) {
+        fused_energy_gradient_hessian(h, x, energy, g, values);
+        return;
+    }
+    if (g)
+        plain_gradient(h, x, g, energy);
+    else if (energy)
+        plain_energy(h, x, energy, nullptr);
+    if (values) plain_hessian(h, x, values);
+}
+
+void hvp_dev(pfgpu_handle &h, const double *x, const double *v, double *y)
+{
+    if (h.path == PFGPU_PATH_FUSED)
+        fused_hvp(h, x, v, y);
+    else
+        plain_hvp(h, x, v, y);
+}
+
+void check_finite(pfgpu_handle &h, const void *dev, int64_t n, bool is_float, const char *what)
+{
+    if (!all_finite(h, dev, n, is_float))
+        fail(PFGPU_ERR_INVALID_STATE, std::string(what) + " has non-finite entries (inverted element?)");
+}
+
+} // namespace
+
+int64_t pfgpu_handle::device_bytes() const
+{
+    const DevBuf *bufs[] = {&conn, &vconn, &geom, &lame, &node_ptr, &node_inc, &vert_ptr, &vert_inc,
+                            &brow_ptr, &block_row, &contrib_ptr, &contrib, &patch_desc, &patch_elems,
+                            &patch_rows, &patch_pos, &d_outer, &d_inner, &loc, &hloc, &partial, &red_out,
+                            &h_x, &h_v, &h_out, &h_out2, &h_vals};
+    int64_t s = 0;
+    for (const DevBuf *b : bufs) s += int64_t(b->bytes());
+    return s;
+}
+
+extern "C" {
+
+const char *pfgpu_last_error(void) { return g_last_error.c_str(); }
+
+pfgpu_status pfgpu_create(const pfgpu_mesh_desc *mesh, const pfgpu_material_desc *material, int32_t precision,
+                          pfgpu_handle **out)
+{
+    return guard([&] {
+        require(out != nullptr, "null output handle pointer");
+        *out = nullptr;
+        require(mesh != nullptr && material != nullptr, "null mesh or material descriptor");
+        const pfgpu_mesh_desc &m = *mesh;
+        require(m.order == 1 || m.order == 2, "order must be 1 or 2");
+        require(m.quadrature_order <= 2, "quadrature orders above 2 are not provided");
+        require(m.n_elements > 0 && m.n_nodes > 0 && m.n_vertices > 0, "empty mesh");
+        require(m.elem_nodes && m.vertices && m.elem_vertices, "null mesh array");
+        require(material->kind == PFGPU_NEOHOOKEAN || material->kind == PFGPU_LINEAR_ELASTICITY, "unknown material");
+        require(precision == PFGPU_FP64 || precision == PFGPU_MIXED, "unknown precision");
+        require((material->lambda == nullptr) == (material->mu == nullptr), "lambda and mu must both be given or both be NULL");
+
+        auto h = std::make_unique<pfgpu_handle>();
+        PF_HIP(hipGetDevice(&h->device));
+        h->order = m.order;
+        h->n_local = m.order == 1 ? 4 : 10;
+        const int qo = m.quadrature_order <= 0 ? (m.order == 1 ? 1 : 2) : m.quadrature_order;
+        h->n_quad = qo == 1 ? 1 : 4;
+        h->material = material->kind;
+        h->precision = precision;
+        h->path = PFGPU_PATH_PLAIN;
+        h->n_elements = m.n_elements;
+        h->n_nodes = m.n_nodes;
+        h->n_vertices = m.n_vertices;
+        h->n_dofs = 3 * m.n_nodes;
+
+        h->h_elem_nodes.assign(m.elem_nodes, m.elem_nodes + size_t(m.n_elements) * h->n_local);
+        h->pat = build_pattern(m.n_elements, h->n_local, m.elem_nodes, m.n_nodes, m.n_vertices, m.elem_vertices);
+
+        std::vector<double> g;
+        geometry_host(m, g);
+        if (precision == PFGPU_MIXED)
+            h->geom = upload_soa<double, float>(g.data(), m.n_elements, 10, h->stream);
+        else
+            h->geom = upload_soa<double, double>(g.data(), m.n_elements, 10, h->stream);
+
+        if (material->lambda) {
+            upload_lame(*h, material->lambda, material->mu);
+        } else {
+            const double E = material->young, nu = material->poisson;
+            require(std::isfinite(E) && E > 0 && nu > -1.0 && nu < 0.5, "invalid Young's modulus / Poisson ratio");
+            // LameParameters (MatParams.cpp:11-22), 3D
+            const double lam = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu)), mu = E / (2.0 * (1.0 + nu));
+            std::vector<double> la(m.n_elements, lam), mv(m.n_elements, mu);
+            upload_lame(*h, la.data(), mv.data());
+        }
+        h->conn = upload_soa<int32_t, int32_t>(m.elem_nodes, m.n_elements, h->n_local, h->stream);
+        h->vconn = upload_soa<int32_t, int32_t>(m.elem_vertices, m.n_elements, 4, h->stream);
+        h->node_ptr = upload(h->pat.node_ptr, h->stream);
+        h->node_inc = upload(h->pat.node_inc, h->stream);
+        h->vert_ptr = upload(h->pat.vert_ptr, h->stream);
+        h->vert_inc = upload(h->pat.vert_inc, h->stream);
+        *out = h.release();
+    });
+}
+
+pfgpu_status pfgpu_destroy(pfgpu_handle *h)
+{
+    return guard([&] {
+        if (!h) return;
+        (void)hipStreamSynchronize(h->stream);
+        delete h;
+    });
+}
+
+pfgpu_status pfgpu_info(const pfgpu_handle *h, pfgpu_info_t *info)
+{
+    return guard([&] {
+        check_handle(h);
+        require(info != nullptr, "null info");
+        info->order = h->order;
+        info->n_local = h->n_local;
+        info->n_quad = h->n_quad;
+        info->material = h->material;
+        info->precision = h->precision;
+        info->path = h->path;
+        info->n_elements = h->n_elements;
+        info->n_nodes = h->n_nodes;
+        info->n_vertices = h->n_vertices;
+        info->n_dofs = h->n_dofs;
+        info->nnz = h->pat.nnz;
+        info->device_bytes = h->device_bytes();
+    });
+}
+
+pfgpu_status pfgpu_set_stream(pfgpu_handle *h, void *stream)
+{
+    return guard([&] {
+        check_handle(h);
+        h->stream = static_cast<hipStream_t>(stream);
+    });
+}
+
+pfgpu_status pfgpu_set_path(pfgpu_handle *h, int32_t path)
+{
+    return guard([&] {
+        check_handle(h);
+        require(path == PFGPU_PATH_FUSED || path == PFGPU_PATH_PLAIN, "unknown path");
+        h->path = path;
+    });
+}
+
+pfgpu_status pfgpu_set_lame(pfgpu_handle *h, const double *lambda, const double *mu)
+{
+    return guard([&] {
+        check_handle(h);
+        require(lambda && mu, "null lambda/mu");
+        PF_HIP(hipStreamSynchronize(h->stream));
+        upload_lame(*h, lambda, mu);
+    });
+}
+
+pfgpu_status pfgpu_value(pfgpu_handle *h, const double *x, double *energy)
+{
+    return guard([&] {
+        check_handle(h);
+        require(x && energy, "null argument");
+        const double *dx = stage(*h, h->h_x, x, h->n_dofs);
+        h->h_out.reserve(sizeof(double));
+        plain_energy(*h, dx, h->h_out.as<double>(), nullptr);
+        fetch(*h, energy, h->h_out.get(), sizeof(double));
+        if (!std::isfinite(*energy)) fail(PFGPU_ERR_INVALID_STATE, "energy is not finite (inverted element?)");
+    });
+}
+
+pfgpu_status pfgpu_value_per_element(pfgpu_handle *h, const double *x, double *out)
+{
+    return guard([&] {
+        check_handle(h);
+        require(x && out, "null argument");
+        const double *dx = stage(*h, h->h_x, x, h->n_dofs);
+        h->h_out.reserve(sizeof(double) * size_t(h->n_elements));
+        plain_energy(*h, dx, nullptr, h->h_out.as<double>());
+        fetch(*h, out, h->h_out.get(), sizeof(double) * size_t(h->n_elements));
+        for (int e = 0; e < h->n_elements; ++e)
+            if (!std::isfinite(out[e])) fail(PFGPU_ERR_INVALID_STATE, "element energy is not finite (inverted element?)");
+    });
+}
+
+pfgpu_status pfgpu_gradient(pfgpu_handle *h, const double *x, double *g)
+{
+    return guard([&] {
+        check_handle(h);
+        require(x && g, "null argument");
+        const double *dx = stage(*h, h->h_x, x, h->n_dofs);
+        h->h_out.reserve(sizeof(double) * size_t(h->n_dofs));
+        gradient_dev(*h, dx, h->h_out.as<double>(), nullptr);
+        fetch(*h, g, h->h_out.get(), sizeof(double) * size_t(h->n_dofs));
+        for (int i = 0; i < h->n_dofs; ++i)
+            if (!std::isfinite(g[i])) fail(PFGPU_ERR_INVALID_STATE, "gradient is not finite (inverted element?)");
+    });
+}
+
+pfgpu_status pfgpu_hessian_nnz(const pfgpu_handle *h, int64_t *nnz)
+{
+    return guard([&] {
+        check_handle(h);
+        require(nnz != nullptr, "null argument");
+        *nnz = h->pat.nnz;
+    });
+}
+
+pfgpu_status pfgpu_hessian_pattern(const pfgpu_handle *h, int32_t *outer, int32_t *inner)
+{
+    return guard([&] {
+        check_handle(h);
+        require(outer && inner, "null argument");
+        const Pattern &p = h->pat;
+        std::memcpy(outer, p.outer.data(), sizeof(int32_t) * p.outer.size());
+        for (int i = 0; i < p.n_nodes; ++i) {
+            const int r0 = p.brow_ptr[i], deg = p.brow_ptr[i + 1] - r0;
+            for (int m = 0; m < 3; ++m)
+                for (int k = 0; k < deg; ++k)
+                    for (int n = 0; n < 3; ++n) inner[9 * int64_t(r0) + int64_t(m) * 3 * deg + 3 * k + n] = 3 * p.bcol[r0 + k] + n;
+        }
+    });
+}
+
+pfgpu_status pfgpu_hessian_values(pfgpu_handle *h, const double *x, int32_t project_to_psd, void *values)
+{
+    return guard([&] {
+        check_handle(h);
+        require(x && values, "null argument");
+        if (project_to_psd) fail(PFGPU_ERR_UNSUPPORTED, "project_to_psd is not provided");
+        const double *dx = stage(*h, h->h_x, x, h->n_dofs);
+        const size_t bytes = h->value_size() * size_t(h->pat.nnz);
+        h->h_vals.reserve(bytes);
+        egh_dev(*h, dx, nullptr, nullptr, h->h_vals.get());
+        check_finite(*h, h->h_vals.get(), h->pat.nnz, h->precision == PFGPU_MIXED, "Hessian");
+        fetch(*h, values, h->h_vals.get(), bytes);
+    });
+}
+
+pfgpu_status pfgpu_energy_gradient_hessian(pfgpu_handle *h, const double *x, double *energy, double *g, void *values)
+{
+    return guard([&] {
+        check_handle(h);
+        require(x != nullptr, "null x");
+        const double *dx = stage(*h, h->h_x, x, h->n_dofs);
+        double *de = nullptr, *dg = nullptr;
+        void *dv = nullptr;
+        if (energy) {
+            h->h_out2.reserve(sizeof(double));
+            de = h->h_out2.as<double>();
+        }
+        if (g) {
+            h->h_out.reserve(sizeof(double) * size_t(h->n_dofs));
+            dg = h->h_out.as<double>();
+        }
+        const size_t bytes = h->value_size() * size_t(h->pat.nnz);
+        if (values) {
+            h->h_vals.reserve(bytes);
+            dv = h->h_vals.get();
+        }
+        egh_dev(*h, dx, de, dg, dv);
+        if (energy) fetch(*h, energy, de, sizeof(double));
+        if (g) fetch(*h, g, dg, sizeof(double) * size_t(h->n_dofs));
+        if (values) fetch(*h, values, dv, bytes);
+        if (energy && !std::isfinite(*energy)) fail(PFGPU_ERR_INVALID_STATE, "energy is not finite (inverted element?)");
+        if (g)
+            for (int i = 0; i < h->n_dofs; ++i)
+                if (!std::isfinite(g[i])) fail(PFGPU_ERR_INVALID_STATE, "gradient is not finite (inverted element?)");
+        if (values) check_finite(*h, dv, h->pat.nnz, h->precision == PFGPU_MIXED, "Hessian");
+    });
+}
+
+pfgpu_status pfgpu_hvp(pfgpu_handle *h, const double *x, const double *v, double *y)
+{
+    return guard([&] {
+        check_handle(h);
+        require(x && v && y, "null argument");
+        const double *dx = stage(*h, h->h_x, x, h->n_dofs);
+        const double *dv = stage(*h, h->h_v, v, h->n_dofs);
+        h->h_out.reserve(sizeof(double) * size_t(h->n_dofs));
+        hvp_dev(*h, dx, dv, h->h_out.as<double>());
+        fetch(*h, y, h->h_out.get(), sizeof(double) * size_t(h->n_dofs));
+        for (int i = 0; i < h->n_dofs; ++i)
+            if (!std::isfinite(y[i])) fail(PFGPU_ERR_INVALID_STATE, "Hessian-vector product is not finite");
+    });
+}
+
+pfgpu_status pfgpu_material_vjp(pfgpu_handle *h, const double *x, const double *p, double *out)
+{
+    return guard([&] {
+        check_handle(h);
+        require(x && p && out, "null argument");
+        const double *dx = stage(*h, h->h_x, x, h->n_dofs);
+        const double *dp = stage(*h, h->h_v, p, h->n_dofs);
+        h->h_out.reserve(sizeof(double) * 2 * size_t(h->n_elements));
+        element_material_vjp(*h, dx, dp, h->h_out.as<double>());
+        fetch(*h, out, h->h_out.get(), sizeof(double) * 2 * size_t(h->n_elements));
+    });
+}
+
+pfgpu_status pfgpu_shape_vjp(pfgpu_handle *h, const double *x, const double *p, double *out)
+{
+    return guard([&] {
+        check_handle(h);
+        require(x && p && out, "null argument");
+        const double *dx = stage(*h, h->h_x, x, h->n_dofs);
+        const double *dp = stage(*h, h->h_v, p, h->n_dofs);
+        h->h_out.reserve(sizeof(double) * 3 * size_t(h->n_vertices));
+        plain_shape_vjp(*h, dx, dp, h->h_out.as<double>());
+        fetch(*h, out, h->h_out.get(), sizeof(double) * 3 * size_t(h->n_vertices));
+    });
+}
+
+pfgpu_status pfgpu_material_jvp(pfgpu_handle *h, const double *x, const double *dlambda, const double *dmu, double *y)
+{
+    return guard([&] {
+        check_handle(h);
+        require(x && dlambda && dmu && y, "null argument");
+        const double *dx = stage(*h, h->h_x, x, h->n_dofs);
+        h->h_v.reserve(sizeof(double) * 2 * size_t(h->n_elements));
+        double *dl = h->h_v.as<double>();
+        PF_HIP(hipMemcpyAsync(dl, dlambda, sizeof(double) * size_t(h->n_elements), hipMemcpyHostToDevice, h->stream));
+        PF_HIP(hipMemcpyAsync(dl + h->n_elements, dmu, sizeof(double) * size_t(h->n_elements), hipMemcpyHostToDevice, h->stream));
+        h->h_out.reserve(sizeof(double) * size_t(h->n_dofs));
+        plain_material_jvp(*h, dx, dl, dl + h->n_elements, h->h_out.as<double>());
+        fetch(*h, y, h->h_out.get(), sizeof(double) * size_t(h->n_dofs));
+    });
+}
+
+// ---- device-pointer entry points ----------------------------------------
+
+pfgpu_status pfgpu_value_dev(pfgpu_handle *h, const double *x, double *energy)
+{
+    return guard([&] {
+        check_handle(h);
+        require(x && energy, "null argument");
+        plain_energy(*h, x, energy, nullptr);
+    });
+}
+
+pfgpu_status pfgpu_gradient_dev(pfgpu_handle *h, const double *x, double *g)
+{
+    return guard([&] {
+        check_handle(h);
+        require(x && g, "null argument");
+        gradient_dev(*h, x, g, nullptr);
+    });
+}
+
+pfgpu_status pfgpu_hessian_values_dev(pfgpu_handle *h, const double *x, void *values)
+{
+    return guard([&] {
+        check_handle(h);
+        require(x && values, "null argument");
+        egh_dev(*h, x, nullptr, nullptr, values);
+    });
+}
+
+pfgpu_status pfgpu_energy_gradient_hessian_dev(pfgpu_handle *h, const double *x, double *energy, double *g, void *values)
+{
+    return guard([&] {
+        check_handle(h);
+        require(x != nullptr, "null x");
+        egh_dev(*h, x, energy, g, values);
+    });
+}
+
+pfgpu_status pfgpu_hvp_dev(pfgpu_handle *h, const double *x, const double *v, double *y)
+{
+    return guard([&] {
+        check_handle(h);
+        require(x && v && y, "null argument");
+        hvp_dev(*h, x, v, y);
+    });
+}
+
+pfgpu_status pfgpu_material_vjp_dev(pfgpu_handle *h, const double *x, const double *p, double *out)
+{
+    return guard([&] {
+        check_handle(h);
+        require(x && p && out, "null argument");
+        element_material_vjp(*h, x, p, out);
+    });
+}
+
+pfgpu_status pfgpu_shape_vjp_dev(pfgpu_handle *h, const double *x, const double *p, double *out)
+{
+    return guard([&] {
+        check_handle(h);
+        require(x && p && out, "null argument");
+        plain_shape_vjp(*h, x, p, out);
+    });
+}
+
+pfgpu_status pfgpu_hessian_pattern_dev(const pfgpu_handle *h, const int32_t **outer, const int32_t **inner)
+{
+    return guard([&] {
+        check_handle(h);
+        require(outer && inner, "null argument");
+        pfgpu_handle *mh = const_cast<pfgpu_handle *>(h);
+        build_device_pattern(*mh);
+        *outer = mh->d_outer.as<int32_t>();
+        *inner = mh->d_inner.as<int32_t>();
+    });
+}
+
+pfgpu_status pfgpu_synchronize(pfgpu_handle *h)
+{
+    return guard([&] {
+        check_handle(h);
+        PF_HIP(hipStreamSynchronize(h->stream));
+    });
+}
+
+} // extern "C"

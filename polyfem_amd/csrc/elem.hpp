@@ -1,0 +1,366 @@
+// Per-element arithmetic of the elasticity path, shared by every kernel.
+//
+// Conventions (reference: assembler/NeoHookeanElasticity.cpp, LinearElasticity.cpp):
+//   U[a][d]   displacement of local node a, component d
+//   jit       J^{-T} of the P1 geometry, row-major: jit[k*3+c]
+//             (ElementAssemblyValues.cpp:96-102)
+//   G_a(c)    = sum_k RG(q,a,k) jit[k][c], the physical basis gradient
+//   H = grad u, H[d][c] = sum_a U[a][d] G_a(c);  F = I + H
+//   da        = det J * w_q (Assembler.cpp:519), w already divided by 6
+//
+// All loops over local nodes, quadrature points and 3x3 indices are fully
+// unrolled; the reference-element gradients RG(q,a,k) are then compile-time
+// constants and the zero ones vanish from the code.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "common.hpp"
+
+namespace pfgpu {
+
+enum { MAT_NH = PFGPU_NEOHOOKEAN, MAT_LE = PFGPU_LINEAR_ELASTICITY };
+
+template <int ORDER> struct NLocal { static constexpr int value = ORDER == 1 ? 4 : 10; };
+
+// Hammer-Marlowe-Stroud points a = (5 - sqrt 5)/20, b = (5 + 3 sqrt 5)/20,
+// in the reference's order (a,a,b), (a,b,a), (b,a,a), (a,a,a)
+// (autogen/auto_tetrahedron.ipp:6-12).
+constexpr double HMS_A = 0.13819660112501053195188660538406111299991607666015625;
+constexpr double HMS_B = 0.58541019662496840414434018384781666100025177001953125;
+
+__host__ __device__ constexpr double quad_point(int nq, int q, int k)
+{
+    return nq == 1 ? 0.25 : (q == 3 ? HMS_A : (k == 2 - q ? HMS_B : HMS_A));
+}
+
+// weights sum to 1 and are divided by 6 (TetQuadrature.cpp:53)
+__host__ __device__ constexpr double quad_weight(int nq) { return nq == 1 ? 1.0 / 6.0 : 0.25 / 6.0; }
+
+// d phi_a / d xi_k of the P1/P2 Lagrange tet (autogen/auto_p_bases.cpp:1100-1123, 1260-1330)
+template <int ORDER> __host__ __device__ constexpr double ref_grad(double x, double y, double z, int a, int k)
+{
+    if (ORDER == 1) return a == 0 ? -1.0 : (k == a - 1 ? 1.0 : 0.0);
+    switch (a) {
+    case 0: return 4 * x + 4 * y + 4 * z - 3;
+    case 1: return k == 0 ? 4 * x - 1 : 0.0;
+    case 2: return k == 1 ? 4 * y - 1 : 0.0;
+    case 3: return k == 2 ? 4 * z - 1 : 0.0;
+    case 4: return k == 0 ? 4 * (-2 * x - y - z + 1) : -4 * x;
+    case 5: return k == 0 ? 4 * y : (k == 1 ? 4 * x : 0.0);
+    case 6: return k == 1 ? 4 * (-x - 2 * y - z + 1) : -4 * y;
+    case 7: return k == 2 ? 4 * (-x - y - 2 * z + 1) : -4 * z;
+    case 8: return k == 0 ? 4 * z : (k == 2 ? 4 * x : 0.0);
+    default: return k == 1 ? 4 * z : (k == 2 ? 4 * y : 0.0);
+    }
+}
+
+template <int ORDER, int NQ> __host__ __device__ constexpr double RG(int q, int a, int k)
+{
+    return ref_grad<ORDER>(quad_point(NQ, q, 0), quad_point(NQ, q, 1), quad_point(NQ, q, 2), a, k);
+}
+
+// sum_q RG(q,a,k) RG(q,b,l): the quadrature of the basis-gradient products
+template <int ORDER, int NQ> __host__ __device__ constexpr double RGG(int a, int b, int k, int l)
+{
+    double s = 0;
+    for (int q = 0; q < NQ; ++q) s += RG<ORDER, NQ>(q, a, k) * RG<ORDER, NQ>(q, b, l);
+    return s;
+}
+
+// ------------------------------------------------------------ 3x3 helpers
+
+template <typename T> __device__ __forceinline__ T det3(const T *m)
+{
+    return m[0] * (m[4] * m[8] - m[5] * m[7]) - m[1] * (m[3] * m[8] - m[5] * m[6]) +
+           m[2] * (m[3] * m[7] - m[4] * m[6]);
+}
+
+__device__ __forceinline__ float pf_log1p(float v) { return log1pf(v); }
+__device__ __forceinline__ double pf_log1p(double v) { return log1p(v); }
+
+// delta - log1p(delta) without cancellation: series below a threshold
+__device__ __forceinline__ float xmlog1p(float d)
+{
+    if (fabsf(d) < 0.1f) {
+        float s = 1.0f / 12.0f;
+#pragma unroll
+        for (int n = 11; n >= 2; --n) s = 1.0f / float(n) - d * s;
+        return d * d * s;
+    }
+    return d - log1pf(d);
+}
+__device__ __forceinline__ double xmlog1p(double d)
+{
+    if (fabs(d) < 1e-3) {
+        double s = 1.0 / 10.0;
+#pragma unroll
+        for (int n = 9; n >= 2; --n) s = 1.0 / double(n) - d * s;
+        return d * d * s;
+    }
+    return d - log1p(d);
+}
+
+// Kinematics of F = I + H, written so that nothing cancels when H is small:
+// J - 1 = tr H + (tr^2 H - tr H^2)/2 + det H, and
+// I - F^{-T} = (det H I + (1 + tr H) H^T - (H^T)^2) / J.
+template <typename T> struct Kin {
+    T H[9];
+    T J, lnJ, delta, s2, dH;
+    T FmT[9];  // F^{-T}
+    T D[9];    // I - F^{-T}
+};
+
+template <typename T> __device__ __forceinline__ void kinematics(Kin<T> &k)
+{
+    const T *H = k.H;
+    const T t = H[0] + H[4] + H[8];
+    k.s2 = H[0] * H[4] + H[4] * H[8] + H[0] * H[8] - H[1] * H[3] - H[5] * H[7] - H[2] * H[6];
+    k.dH = det3(H);
+    k.delta = t + k.s2 + k.dH;
+    k.J = T(1) + k.delta;
+    k.lnJ = pf_log1p(k.delta);
+    const T iJ = T(1) / k.J;
+    // cofactor of F: columns f1 x f2, f2 x f0, f0 x f1 (NeoHookeanElasticity.cpp:513-527)
+    const T F0 = T(1) + H[0], F4 = T(1) + H[4], F8 = T(1) + H[8];
+    const T F[9] = {F0, H[1], H[2], H[3], F4, H[5], H[6], H[7], F8};
+    k.FmT[0] = (F[4] * F[8] - F[5] * F[7]) * iJ;
+    k.FmT[1] = (F[5] * F[6] - F[3] * F[8]) * iJ;
+    k.FmT[2] = (F[3] * F[7] - F[4] * F[6]) * iJ;
+    k.FmT[3] = (F[2] * F[7] - F[1] * F[8]) * iJ;
+    k.FmT[4] = (F[0] * F[8] - F[2] * F[6]) * iJ;
+    k.FmT[5] = (F[1] * F[6] - F[0] * F[7]) * iJ;
+    k.FmT[6] = (F[1] * F[5] - F[2] * F[4]) * iJ;
+    k.FmT[7] = (F[2] * F[3] - F[0] * F[5]) * iJ;
+    k.FmT[8] = (F[0] * F[4] - F[1] * F[3]) * iJ;
+    const T tp1 = T(1) + t;
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            // (H^T)^2 [r][c] = sum_k H[k][r] H[c][k]
+            const T h2 = H[0 * 3 + r] * H[c * 3 + 0] + H[1 * 3 + r] * H[c * 3 + 1] + H[2 * 3 + r] * H[c * 3 + 2];
+            k.D[r * 3 + c] = ((r == c ? k.dH : T(0)) + tp1 * H[c * 3 + r] - h2) * iJ;
+        }
+}
+
+// ------------------------------------------------------------- element data
+
+template <typename T, int ORDER, int NQ> struct Elem {
+    static constexpr int NB = NLocal<ORDER>::value;
+    T U[NB][3];
+    T jit[9];
+    T det, lam, mu;
+};
+
+template <typename T, int ORDER, int NQ>
+__device__ __forceinline__ void load_geometry(const DevMesh<T> &m, int e, Elem<T, ORDER, NQ> &el)
+{
+    const int E = m.n_elements;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) el.jit[k] = m.geom[k * E + e];
+    el.det = m.geom[9 * E + e];
+    el.lam = m.lame[e];
+    el.mu = m.lame[E + e];
+}
+
+// gather of the element's nodal values from an interleaved dof vector
+template <typename T, int NB>
+__device__ __forceinline__ void gather(const DevMesh<T> &m, int e, const double *__restrict__ x, T (&U)[NB][3])
+{
+    const int E = m.n_elements;
+#pragma unroll
+    for (int a = 0; a < NB; ++a) {
+        const int64_t n3 = 3 * int64_t(m.conn[a * E + e]);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) U[a][d] = T(x[n3 + d]);
+    }
+}
+
+// grad of a nodal field at quadrature point q: out[d][c] = sum_a V[a][d] G_a(c)
+template <typename T, int ORDER, int NQ, int NB>
+__device__ __forceinline__ void field_grad(int q, const T (&V)[NB][3], const T *jit, T *out)
+{
+    T A[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) A[i] = T(0);
+#pragma unroll
+    for (int a = 0; a < NB; ++a)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const double r = RG<ORDER, NQ>(q, a, k);
+            if (r != 0.0) {
+#pragma unroll
+                for (int d = 0; d < 3; ++d) A[d * 3 + k] += V[a][d] * T(r);
+            }
+        }
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) out[d * 3 + c] = A[d * 3 + 0] * jit[0 * 3 + c] + A[d * 3 + 1] * jit[1 * 3 + c] + A[d * 3 + 2] * jit[2 * 3 + c];
+}
+
+// acc[a][d] += sum_c G_a(c) S[d][c] * da, as sum_k RG(q,a,k) (da S jit^T)[d][k]
+template <typename T, int ORDER, int NQ, int NB>
+__device__ __forceinline__ void contract_add(int q, const T *S, const T *jit, T da, T (&acc)[NB][3])
+{
+    T Q[9];
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) Q[d * 3 + k] = da * (S[d * 3 + 0] * jit[k * 3 + 0] + S[d * 3 + 1] * jit[k * 3 + 1] + S[d * 3 + 2] * jit[k * 3 + 2]);
+#pragma unroll
+    for (int a = 0; a < NB; ++a)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const double r = RG<ORDER, NQ>(q, a, k);
+            if (r != 0.0) {
+#pragma unroll
+                for (int d = 0; d < 3; ++d) acc[a][d] += T(r) * Q[d * 3 + k];
+            }
+        }
+}
+
+// ---------------------------------------------------------- constitutive
+
+// energy density (NeoHookeanElasticity.cpp:384; LinearElasticity.cpp:129)
+template <typename T, int MAT> __device__ __forceinline__ T energy_density(const Kin<T> &k, T lam, T mu)
+{
+    const T *H = k.H;
+    if (MAT == MAT_NH) {
+        // mu/2 (|F|^2 - 3 - 2 ln J) + lambda/2 ln^2 J with |F|^2 - 3 = 2 tr H + |H|^2
+        // and tr H - ln J = -(s2 + det H) + (delta - ln(1 + delta))
+        T hh = T(0);
+#pragma unroll
+        for (int i = 0; i < 9; ++i) hh += H[i] * H[i];
+        return mu * (xmlog1p(k.delta) - k.s2 - k.dH) + T(0.5) * mu * hh + T(0.5) * lam * k.lnJ * k.lnJ;
+    } else {
+        T ee = T(0);
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const T eps = T(0.5) * (H[r * 3 + c] + H[c * 3 + r]);
+                ee += eps * eps;
+            }
+        const T tr = H[0] + H[4] + H[8];
+        return mu * ee + T(0.5) * lam * tr * tr;
+    }
+}
+
+// first Piola-Kirchhoff stress (NeoHookeanElasticity.cpp:532: mu F - mu/J cof F + lambda ln J / J cof F)
+template <typename T, int MAT> __device__ __forceinline__ void pk1(const Kin<T> &k, T lam, T mu, T *P)
+{
+    const T *H = k.H;
+    if (MAT == MAT_NH) {
+        const T ll = lam * k.lnJ;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) P[i] = mu * (H[i] + k.D[i]) + ll * k.FmT[i];
+    } else {
+        const T tr = lam * (H[0] + H[4] + H[8]);
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) P[r * 3 + c] = mu * (H[r * 3 + c] + H[c * 3 + r]) + (r == c ? tr : T(0));
+    }
+}
+
+// linearised stress C : W (NeoHookeanElasticity.cpp:660-680 'result';
+// LinearElasticity.cpp:253-270)
+template <typename T, int MAT>
+__device__ __forceinline__ void dstress(const Kin<T> &k, T lam, T mu, const T *W, T *out)
+{
+    if (MAT == MAT_NH) {
+        const T *A = k.FmT;
+        T fw = T(0);
+#pragma unroll
+        for (int i = 0; i < 9; ++i) fw += A[i] * W[i];
+        T t1[9]; // A W^T
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) t1[r * 3 + c] = A[r * 3 + 0] * W[c * 3 + 0] + A[r * 3 + 1] * W[c * 3 + 1] + A[r * 3 + 2] * W[c * 3 + 2];
+        const T cc = mu - lam * k.lnJ, lf = lam * fw;
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const T t2 = t1[r * 3 + 0] * A[0 * 3 + c] + t1[r * 3 + 1] * A[1 * 3 + c] + t1[r * 3 + 2] * A[2 * 3 + c];
+                out[r * 3 + c] = mu * W[r * 3 + c] + cc * t2 + lf * A[r * 3 + c];
+            }
+    } else {
+        const T tr = lam * (W[0] + W[4] + W[8]);
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) out[r * 3 + c] = mu * (W[r * 3 + c] + W[c * 3 + r]) + (r == c ? tr : T(0));
+    }
+}
+
+// ------------------------------------------------------------- Hessian data
+//
+// With k_a = F^{-T} G_a the NeoHookean local Hessian block is
+//   H_ab = sum_q da [ mu (G_a . G_b) I + lambda k_a k_b^T + (mu - lambda ln J) k_b k_a^T ],
+// which equals D^T H9 D of NeoHookeanElasticity.cpp:636-652 (the d2J/dF2 term
+// contracts to (k_a k_b^T - k_b k_a^T) J / J).  Linear elasticity is the same
+// form with k_a = G_a, lambda and mu (LinearElasticity.cpp:30-63).
+// M maps reference gradients to k: k_a[m] = sum_k M[m][k] RG(q,a,k).
+template <typename T, int MAT>
+__device__ __forceinline__ void hess_qp(const Kin<T> &k, const T *jit, T lam, T mu, T da, T *M, T &cl, T &cc)
+{
+    if (MAT == MAT_NH) {
+#pragma unroll
+        for (int m = 0; m < 3; ++m)
+#pragma unroll
+            for (int kk = 0; kk < 3; ++kk)
+                M[m * 3 + kk] = k.FmT[m * 3 + 0] * jit[kk * 3 + 0] + k.FmT[m * 3 + 1] * jit[kk * 3 + 1] + k.FmT[m * 3 + 2] * jit[kk * 3 + 2];
+        cl = da * lam;
+        cc = da * (mu - lam * k.lnJ);
+    } else {
+#pragma unroll
+        for (int m = 0; m < 3; ++m)
+#pragma unroll
+            for (int kk = 0; kk < 3; ++kk) M[m * 3 + kk] = jit[kk * 3 + m];
+        cl = da * lam;
+        cc = da * mu;
+    }
+}
+
+template <typename T, int ORDER, int NQ>
+__device__ __forceinline__ void khat(int q, int a, const T *M, T *ka)
+{
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+        T s = T(0);
+#pragma unroll
+        for (int kk = 0; kk < 3; ++kk) {
+            const double r = RG<ORDER, NQ>(q, a, kk);
+            if (r != 0.0) s += M[m * 3 + kk] * T(r);
+        }
+        ka[m] = s;
+    }
+}
+
+// metric jit jit^T (symmetric, 6 entries: 00 11 22 01 02 12)
+template <typename T> __device__ __forceinline__ void metric(const T *jit, T *g6)
+{
+    auto dot = [&](int k, int l) { return jit[k * 3 + 0] * jit[l * 3 + 0] + jit[k * 3 + 1] * jit[l * 3 + 1] + jit[k * 3 + 2] * jit[l * 3 + 2]; };
+    g6[0] = dot(0, 0); g6[1] = dot(1, 1); g6[2] = dot(2, 2);
+    g6[3] = dot(0, 1); g6[4] = dot(0, 2); g6[5] = dot(1, 2);
+}
+
+// sum_q G_a(q).G_b(q) = sum_{k,l} metric[k][l] RGG(a,b,k,l)
+template <int ORDER, int NQ, typename T> __device__ __forceinline__ T grad_dot(int a, int b, const T *g6)
+{
+    T s = T(0);
+    const int kk[6] = {0, 1, 2, 0, 0, 1}, ll[6] = {0, 1, 2, 1, 2, 2};
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+        const double c = i < 3 ? RGG<ORDER, NQ>(a, b, kk[i], ll[i])
+                               : RGG<ORDER, NQ>(a, b, kk[i], ll[i]) + RGG<ORDER, NQ>(a, b, ll[i], kk[i]);
+        if (c != 0.0) s += T(c) * g6[i];
+    }
+    return s;
+}
+
+} // namespace pfgpu

@@ -1,0 +1,504 @@
+// Plain assembly path: one thread per element writes its element-local
+// result to HBM, then a gather kernel sums the contributions of every output
+// entry in element order.  Deterministic, no atomics.  This is the
+// straightforward GPU restatement of NLAssembler's element loop
+// (Assembler.cpp:495-771) and the baseline the fused path is measured against.
+#include "dispatch.hpp"
+
+namespace pfgpu {
+
+namespace {
+
+__device__ __forceinline__ double block_sum(double v)
+{
+    __shared__ double s[BLOCK];
+    const int t = threadIdx.x;
+    s[t] = v;
+    __syncthreads();
+#pragma unroll
+    for (int w = BLOCK / 2; w > 0; w >>= 1) {
+        if (t < w) s[t] += s[t + w];
+        __syncthreads();
+    }
+    return s[0];
+}
+
+// ------------------------------------------------------------ element kernels
+
+template <typename T, int ORDER, int NQ, int MAT>
+__global__ __launch_bounds__(BLOCK) void k_energy(DevMesh<T> m, const double *__restrict__ x,
+                                                  double *__restrict__ per_elem, double *__restrict__ partial)
+{
+    const int e = blockIdx.x * BLOCK + threadIdx.x;
+    double val = 0;
+    if (e < m.n_elements) {
+        Elem<T, ORDER, NQ> el;
+        load_geometry(m, e, el);
+        gather(m, e, x, el.U);
+        const T w = el.det * T(quad_weight(NQ));
+        T acc = T(0);
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            Kin<T> k;
+            field_grad<T, ORDER, NQ>(q, el.U, el.jit, k.H);
+            kinematics(k);
+            acc += energy_density<T, MAT>(k, el.lam, el.mu) * w;
+        }
+        val = double(acc);
+        if (per_elem) per_elem[e] = val;
+    }
+    const double s = block_sum(val);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+// local gradient (+ energy partials); gloc[(e*NB + a)*3 + d]
+template <typename T, int ORDER, int NQ, int MAT>
+__global__ __launch_bounds__(BLOCK) void k_gradient(DevMesh<T> m, const double *__restrict__ x, T *__restrict__ gloc,
+                                                    double *__restrict__ partial)
+{
+    constexpr int NB = NLocal<ORDER>::value;
+    const int e = blockIdx.x * BLOCK + threadIdx.x;
+    double val = 0;
+    if (e < m.n_elements) {
+        Elem<T, ORDER, NQ> el;
+        load_geometry(m, e, el);
+        gather(m, e, x, el.U);
+        const T w = el.det * T(quad_weight(NQ));
+        T g[NB][3];
+#pragma unroll
+        for (int a = 0; a < NB; ++a)
+#pragma unroll
+            for (int d = 0; d < 3; ++d) g[a][d] = T(0);
+        T acc = T(0);
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            Kin<T> k;
+            field_grad<T, ORDER, NQ>(q, el.U, el.jit, k.H);
+            kinematics(k);
+            acc += energy_density<T, MAT>(k, el.lam, el.mu) * w;
+            T P[9];
+            pk1<T, MAT>(k, el.lam, el.mu, P);
+            contract_add<T, ORDER, NQ>(q, P, el.jit, w, g);
+        }
+        val = double(acc);
+        T *out = gloc + int64_t(e) * NB * 3;
+#pragma unroll
+        for (int a = 0; a < NB; ++a)
+#pragma unroll
+            for (int d = 0; d < 3; ++d) out[a * 3 + d] = g[a][d];
+    }
+    if (partial) {
+        const double s = block_sum(val);
+        if (threadIdx.x == 0) partial[blockIdx.x] = s;
+    }
+}
+
+// upper-triangle local Hessian blocks; hloc[((e*NBLK) + s(a,b))*9 + m*3 + n], a <= b
+template <typename T, int ORDER, int NQ, int MAT>
+__global__ __launch_bounds__(BLOCK) void k_hessian(DevMesh<T> m, const double *__restrict__ x, T *__restrict__ hloc)
+{
+    constexpr int NB = NLocal<ORDER>::value, NBLK = NB * (NB + 1) / 2;
+    const int e = blockIdx.x * BLOCK + threadIdx.x;
+    if (e >= m.n_elements) return;
+    Elem<T, ORDER, NQ> el;
+    load_geometry(m, e, el);
+    if (MAT == MAT_NH) gather(m, e, x, el.U);
+    const T w = el.det * T(quad_weight(NQ));
+    T K[NQ][NB][3], cl[NQ], cc[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        Kin<T> k;
+        if (MAT == MAT_NH) {
+            field_grad<T, ORDER, NQ>(q, el.U, el.jit, k.H);
+            kinematics(k);
+        }
+        T M[9];
+        hess_qp<T, MAT>(k, el.jit, el.lam, el.mu, w, M, cl[q], cc[q]);
+#pragma unroll
+        for (int a = 0; a < NB; ++a) khat<T, ORDER, NQ>(q, a, M, K[q][a]);
+    }
+    T g6[6];
+    metric(el.jit, g6);
+    const T cm = w * el.mu;
+    T *out = hloc + int64_t(e) * NBLK * 9;
+#pragma unroll
+    for (int a = 0; a < NB; ++a)
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            if (b < a) continue;
+            const int s = a * NB - a * (a - 1) / 2 + (b - a);
+            const T gd = cm * grad_dot<ORDER, NQ>(a, b, g6);
+#pragma unroll
+            for (int mm = 0; mm < 3; ++mm)
+#pragma unroll
+                for (int nn = 0; nn < 3; ++nn) {
+                    T v = mm == nn ? gd : T(0);
+#pragma unroll
+                    for (int q = 0; q < NQ; ++q) v += cl[q] * K[q][a][mm] * K[q][b][nn] + cc[q] * K[q][b][mm] * K[q][a][nn];
+                    out[s * 9 + mm * 3 + nn] = v;
+                }
+        }
+}
+
+// local H(x) v; yloc[(e*NB + a)*3 + d]
+template <typename T, int ORDER, int NQ, int MAT>
+__global__ __launch_bounds__(BLOCK) void k_hvp(DevMesh<T> m, const double *__restrict__ x, const double *__restrict__ v,
+                                               T *__restrict__ yloc)
+{
+    constexpr int NB = NLocal<ORDER>::value;
+    const int e = blockIdx.x * BLOCK + threadIdx.x;
+    if (e >= m.n_elements) return;
+    Elem<T, ORDER, NQ> el;
+    load_geometry(m, e, el);
+    gather(m, e, x, el.U);
+    T V[NB][3];
+    gather(m, e, v, V);
+    const T w = el.det * T(quad_weight(NQ));
+    T y[NB][3];
+#pragma unroll
+    for (int a = 0; a < NB; ++a)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) y[a][d] = T(0);
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        Kin<T> k;
+        field_grad<T, ORDER, NQ>(q, el.U, el.jit, k.H);
+        if (MAT == MAT_NH) kinematics(k);
+        T Gv[9], A[9];
+        field_grad<T, ORDER, NQ>(q, V, el.jit, Gv);
+        dstress<T, MAT>(k, el.lam, el.mu, Gv, A);
+        contract_add<T, ORDER, NQ>(q, A, el.jit, w, y);
+    }
+    T *out = yloc + int64_t(e) * NB * 3;
+#pragma unroll
+    for (int a = 0; a < NB; ++a)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) out[a * 3 + d] = y[a][d];
+}
+
+// (d grad/d lambda) dl + (d grad/d mu) dm, element-local
+template <typename T, int ORDER, int NQ, int MAT>
+__global__ __launch_bounds__(BLOCK) void k_material_jvp(DevMesh<T> m, const double *__restrict__ x,
+                                                        const double *__restrict__ dl, const double *__restrict__ dm,
+                                                        T *__restrict__ yloc)
+{
+    constexpr int NB = NLocal<ORDER>::value;
+    const int e = blockIdx.x * BLOCK + threadIdx.x;
+    if (e >= m.n_elements) return;
+    Elem<T, ORDER, NQ> el;
+    load_geometry(m, e, el);
+    gather(m, e, x, el.U);
+    const T w = el.det * T(quad_weight(NQ));
+    const T sl = T(dl[e]), sm = T(dm[e]);
+    T y[NB][3];
+#pragma unroll
+    for (int a = 0; a < NB; ++a)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) y[a][d] = T(0);
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        Kin<T> k;
+        field_grad<T, ORDER, NQ>(q, el.U, el.jit, k.H);
+        T A[9];
+        if (MAT == MAT_NH) {
+            kinematics(k);
+            // dP/dmu = F - F^{-T}, dP/dlambda = ln J F^{-T} (NeoHookeanElasticity.cpp:747-762)
+#pragma unroll
+            for (int i = 0; i < 9; ++i) A[i] = sm * (k.H[i] + k.D[i]) + sl * k.lnJ * k.FmT[i];
+        } else {
+            // d sigma/d mu = grad u + grad u^T, d sigma/d lambda = tr(grad u) I (LinearElasticity.cpp:290-299)
+            const T tr = k.H[0] + k.H[4] + k.H[8];
+#pragma unroll
+            for (int r = 0; r < 3; ++r)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) A[r * 3 + c] = sm * (k.H[r * 3 + c] + k.H[c * 3 + r]) + (r == c ? sl * tr : T(0));
+        }
+        contract_add<T, ORDER, NQ>(q, A, el.jit, w, y);
+    }
+    T *out = yloc + int64_t(e) * NB * 3;
+#pragma unroll
+    for (int a = 0; a < NB; ++a)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) out[a * 3 + d] = y[a][d];
+}
+
+// force_material_derivative term (ElasticForceDerivative.cpp:97-129): out[e], out[E+e]
+template <typename T, int ORDER, int NQ, int MAT>
+__global__ __launch_bounds__(BLOCK) void k_material_vjp(DevMesh<T> m, const double *__restrict__ x,
+                                                        const double *__restrict__ p, double *__restrict__ out)
+{
+    constexpr int NB = NLocal<ORDER>::value;
+    const int e = blockIdx.x * BLOCK + threadIdx.x;
+    if (e >= m.n_elements) return;
+    Elem<T, ORDER, NQ> el;
+    load_geometry(m, e, el);
+    gather(m, e, x, el.U);
+    T Pn[NB][3];
+    gather(m, e, p, Pn);
+    const T w = el.det * T(quad_weight(NQ));
+    T tl = T(0), tm = T(0);
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        Kin<T> k;
+        field_grad<T, ORDER, NQ>(q, el.U, el.jit, k.H);
+        T Gp[9];
+        field_grad<T, ORDER, NQ>(q, Pn, el.jit, Gp);
+        T il = T(0), im = T(0);
+        if (MAT == MAT_NH) {
+            kinematics(k);
+#pragma unroll
+            for (int i = 0; i < 9; ++i) {
+                im += (k.H[i] + k.D[i]) * Gp[i];
+                il += k.FmT[i] * Gp[i];
+            }
+            il *= k.lnJ;
+        } else {
+#pragma unroll
+            for (int r = 0; r < 3; ++r)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) im += (k.H[r * 3 + c] + k.H[c * 3 + r]) * Gp[r * 3 + c];
+            il = (k.H[0] + k.H[4] + k.H[8]) * (Gp[0] + Gp[4] + Gp[8]);
+        }
+        tl -= il * w;
+        tm -= im * w;
+    }
+    out[e] = double(tl);
+    out[m.n_elements + e] = double(tm);
+}
+
+// force_shape_derivative term per geometric vertex (ElasticForceDerivative.cpp:208-259):
+// -da W g_v with W = grad u^T (C : grad p) + grad p^T P - <P, grad p> I;
+// sloc[(e*4 + v)*3 + d]
+template <typename T, int ORDER, int NQ, int MAT>
+__global__ __launch_bounds__(BLOCK) void k_shape_vjp(DevMesh<T> m, const double *__restrict__ x,
+                                                     const double *__restrict__ p, T *__restrict__ sloc)
+{
+    constexpr int NB = NLocal<ORDER>::value;
+    const int e = blockIdx.x * BLOCK + threadIdx.x;
+    if (e >= m.n_elements) return;
+    Elem<T, ORDER, NQ> el;
+    load_geometry(m, e, el);
+    gather(m, e, x, el.U);
+    T Pn[NB][3];
+    gather(m, e, p, Pn);
+    const T w = el.det * T(quad_weight(NQ));
+    T s[4][3];
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) s[v][d] = T(0);
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        Kin<T> k;
+        field_grad<T, ORDER, NQ>(q, el.U, el.jit, k.H);
+        if (MAT == MAT_NH) kinematics(k);
+        T Gp[9], P[9], B[9];
+        field_grad<T, ORDER, NQ>(q, Pn, el.jit, Gp);
+        pk1<T, MAT>(k, el.lam, el.mu, P);
+        dstress<T, MAT>(k, el.lam, el.mu, Gp, B);
+        T sp = T(0);
+#pragma unroll
+        for (int i = 0; i < 9; ++i) sp += P[i] * Gp[i];
+        T W[9];
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                T acc = d == c ? -sp : T(0);
+#pragma unroll
+                for (int r = 0; r < 3; ++r) acc += k.H[r * 3 + d] * B[r * 3 + c] + Gp[r * 3 + d] * P[r * 3 + c];
+                W[d * 3 + c] = acc;
+            }
+        // the P1 geometric basis at the same point
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            T gv[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                T g = T(0);
+#pragma unroll
+                for (int kk = 0; kk < 3; ++kk) {
+                    const double r = RG<1, NQ>(q, v, kk);
+                    if (r != 0.0) g += T(r) * el.jit[kk * 3 + c];
+                }
+                gv[c] = g;
+            }
+#pragma unroll
+            for (int d = 0; d < 3; ++d) s[v][d] -= w * (W[d * 3 + 0] * gv[0] + W[d * 3 + 1] * gv[1] + W[d * 3 + 2] * gv[2]);
+        }
+    }
+    T *out = sloc + int64_t(e) * 12;
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) out[v * 3 + d] = s[v][d];
+}
+
+// -------------------------------------------------------------- gathers
+
+// out[3i+d] = sum_{k in ptr[i]..ptr[i+1]} loc[inc[k]*3 + d], accumulated in double
+template <typename T>
+__global__ __launch_bounds__(BLOCK) void k_gather_rows(int n, const int32_t *__restrict__ ptr, const int32_t *__restrict__ inc,
+                                                       const T *__restrict__ loc, double *__restrict__ out)
+{
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    double s0 = 0, s1 = 0, s2 = 0;
+    for (int k = ptr[i]; k < ptr[i + 1]; ++k) {
+        const T *l = loc + int64_t(inc[k]) * 3;
+        s0 += double(l[0]);
+        s1 += double(l[1]);
+        s2 += double(l[2]);
+    }
+    out[3 * int64_t(i)] = s0;
+    out[3 * int64_t(i) + 1] = s1;
+    out[3 * int64_t(i) + 2] = s2;
+}
+
+// one thread per 3x3 block of the pattern; contributions in element order
+template <typename T, int NB>
+__global__ __launch_bounds__(BLOCK) void k_gather_blocks(int64_t n_blocks, const int32_t *__restrict__ brow_ptr,
+                                                         const int32_t *__restrict__ block_row, const int32_t *__restrict__ cptr,
+                                                         const int32_t *__restrict__ contrib, const T *__restrict__ hloc,
+                                                         T *__restrict__ values)
+{
+    constexpr int NBLK = NB * (NB + 1) / 2;
+    const int64_t b = int64_t(blockIdx.x) * BLOCK + threadIdx.x;
+    if (b >= n_blocks) return;
+    T acc[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) acc[i] = T(0);
+    for (int k = cptr[b]; k < cptr[b + 1]; ++k) {
+        const int32_t c = contrib[k];
+        const int e = c / (NB * NB), a = (c / NB) % NB, bb = c % NB;
+        if (a <= bb) {
+            const int s = a * NB - a * (a - 1) / 2 + (bb - a);
+            const T *h = hloc + (int64_t(e) * NBLK + s) * 9;
+#pragma unroll
+            for (int i = 0; i < 9; ++i) acc[i] += h[i];
+        } else {
+            const int s = bb * NB - bb * (bb - 1) / 2 + (a - bb);
+            const T *h = hloc + (int64_t(e) * NBLK + s) * 9;
+#pragma unroll
+            for (int mm = 0; mm < 3; ++mm)
+#pragma unroll
+                for (int nn = 0; nn < 3; ++nn) acc[mm * 3 + nn] += h[nn * 3 + mm];
+        }
+    }
+    const int row = block_row[b];
+    const int r0 = brow_ptr[row], deg = brow_ptr[row + 1] - r0;
+    T *out = values + 9 * int64_t(r0) + 3 * (b - r0);
+#pragma unroll
+    for (int mm = 0; mm < 3; ++mm)
+#pragma unroll
+        for (int nn = 0; nn < 3; ++nn) out[int64_t(mm) * 3 * deg + nn] = acc[mm * 3 + nn];
+}
+
+} // namespace
+
+// ------------------------------------------------------------------ launchers
+
+void plain_energy(Handle &h, const double *x, double *energy, double *per_element)
+{
+    const unsigned g = grid_for(h.n_elements);
+    h.partial.reserve(sizeof(double) * g);
+    dispatch(h, [&]<typename T, int O, int Q, int M>() {
+        k_energy<T, O, Q, M><<<g, BLOCK, 0, h.stream>>>(h.mesh<T>(), x, per_element, h.partial.as<double>());
+    });
+    PF_LAUNCH_CHECK();
+    if (energy) sum_doubles(h, h.partial.as<double>(), int(g), energy);
+}
+
+void plain_gradient(Handle &h, const double *x, double *g, double *energy)
+{
+    const unsigned grid = grid_for(h.n_elements);
+    h.loc.reserve(h.value_size() * size_t(h.n_elements) * h.n_local * 3);
+    if (energy) h.partial.reserve(sizeof(double) * grid);
+    dispatch(h, [&]<typename T, int O, int Q, int M>() {
+        k_gradient<T, O, Q, M><<<grid, BLOCK, 0, h.stream>>>(h.mesh<T>(), x, h.loc.as<T>(),
+                                                            energy ? h.partial.as<double>() : nullptr);
+        PF_LAUNCH_CHECK();
+        k_gather_rows<T><<<grid_for(h.n_nodes), BLOCK, 0, h.stream>>>(h.n_nodes, h.node_ptr.as<int32_t>(),
+                                                                        h.node_inc.as<int32_t>(), h.loc.as<T>(), g);
+    });
+    PF_LAUNCH_CHECK();
+    if (energy) sum_doubles(h, h.partial.as<double>(), int(grid), energy);
+}
+
+void ensure_plain_hessian_maps(Handle &h)
+{
+    if (h.plain_hess_ready) return;
+    std::vector<int64_t> ptr64;
+    std::vector<int32_t> contrib, block_row;
+    build_block_contributions(h.pat, h.n_local, h.h_elem_nodes.data(), ptr64, contrib, block_row);
+    std::vector<int32_t> ptr(ptr64.begin(), ptr64.end());
+    h.contrib_ptr = upload(ptr, h.stream);
+    h.contrib = upload(contrib, h.stream);
+    h.block_row = upload(block_row, h.stream);
+    h.brow_ptr = upload(h.pat.brow_ptr, h.stream);
+    h.plain_hess_ready = true;
+}
+
+void plain_hessian(Handle &h, const double *x, void *values)
+{
+    ensure_plain_hessian_maps(h);
+    const unsigned grid = grid_for(h.n_elements);
+    h.hloc.reserve(h.value_size() * size_t(h.n_elements) * h.n_blocks_sym() * 9);
+    dispatch(h, [&]<typename T, int O, int Q, int M>() {
+        k_hessian<T, O, Q, M><<<grid, BLOCK, 0, h.stream>>>(h.mesh<T>(), x, h.hloc.as<T>());
+        PF_LAUNCH_CHECK();
+        k_gather_blocks<T, NLocal<O>::value><<<grid_for(h.pat.n_blocks), BLOCK, 0, h.stream>>>(
+            h.pat.n_blocks, h.brow_ptr.as<int32_t>(), h.block_row.as<int32_t>(), h.contrib_ptr.as<int32_t>(),
+            h.contrib.as<int32_t>(), h.hloc.as<T>(), static_cast<T *>(values));
+    });
+    PF_LAUNCH_CHECK();
+}
+
+void plain_hvp(Handle &h, const double *x, const double *v, double *y)
+{
+    const unsigned grid = grid_for(h.n_elements);
+    h.loc.reserve(h.value_size() * size_t(h.n_elements) * h.n_local * 3);
+    dispatch(h, [&]<typename T, int O, int Q, int M>() {
+        k_hvp<T, O, Q, M><<<grid, BLOCK, 0, h.stream>>>(h.mesh<T>(), x, v, h.loc.as<T>());
+        PF_LAUNCH_CHECK();
+        k_gather_rows<T><<<grid_for(h.n_nodes), BLOCK, 0, h.stream>>>(h.n_nodes, h.node_ptr.as<int32_t>(),
+                                                                        h.node_inc.as<int32_t>(), h.loc.as<T>(), y);
+    });
+    PF_LAUNCH_CHECK();
+}
+
+void plain_material_jvp(Handle &h, const double *x, const double *dl, const double *dm, double *y)
+{
+    const unsigned grid = grid_for(h.n_elements);
+    h.loc.reserve(h.value_size() * size_t(h.n_elements) * h.n_local * 3);
+    dispatch(h, [&]<typename T, int O, int Q, int M>() {
+        k_material_jvp<T, O, Q, M><<<grid, BLOCK, 0, h.stream>>>(h.mesh<T>(), x, dl, dm, h.loc.as<T>());
+        PF_LAUNCH_CHECK();
+        k_gather_rows<T><<<grid_for(h.n_nodes), BLOCK, 0, h.stream>>>(h.n_nodes, h.node_ptr.as<int32_t>(),
+                                                                        h.node_inc.as<int32_t>(), h.loc.as<T>(), y);
+    });
+    PF_LAUNCH_CHECK();
+}
+
+void element_material_vjp(Handle &h, const double *x, const double *p, double *out)
+{
+    dispatch(h, [&]<typename T, int O, int Q, int M>() {
+        k_material_vjp<T, O, Q, M><<<grid_for(h.n_elements), BLOCK, 0, h.stream>>>(h.mesh<T>(), x, p, out);
+    });
+    PF_LAUNCH_CHECK();
+}
+
+void plain_shape_vjp(Handle &h, const double *x, const double *p, double *out)
+{
+    const unsigned grid = grid_for(h.n_elements);
+    h.loc.reserve(h.value_size() * size_t(h.n_elements) * 12);
+    dispatch(h, [&]<typename T, int O, int Q, int M>() {
+        k_shape_vjp<T, O, Q, M><<<grid, BLOCK, 0, h.stream>>>(h.mesh<T>(), x, p, h.loc.as<T>());
+        PF_LAUNCH_CHECK();
+        k_gather_rows<T><<<grid_for(h.n_vertices), BLOCK, 0, h.stream>>>(h.n_vertices, h.vert_ptr.as<int32_t>(),
+                                                                           h.vert_inc.as<int32_t>(), h.loc.as<T>(), out);
+    });
+    PF_LAUNCH_CHECK();
+}
+
+} // namespace pfgpu

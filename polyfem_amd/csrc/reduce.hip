@@ -1,0 +1,85 @@
+// Deterministic reductions and small utility kernels.
+#include "dispatch.hpp"
+
+namespace pfgpu {
+
+namespace {
+
+// Single block: thread t sums in[t], in[t+256], ... in order, then a fixed
+// tree.  The summation order depends only on n, never on scheduling.
+__global__ __launch_bounds__(BLOCK) void k_sum_doubles(const double *__restrict__ in, int n, double *__restrict__ out)
+{
+    __shared__ double s[BLOCK];
+    const int t = threadIdx.x;
+    double v = 0;
+    for (int i = t; i < n; i += BLOCK) v += in[i];
+    s[t] = v;
+    __syncthreads();
+    for (int w = BLOCK / 2; w > 0; w >>= 1) {
+        if (t < w) s[t] += s[t + w];
+        __syncthreads();
+    }
+    if (t == 0) out[0] = s[0];
+}
+
+template <typename T>
+__global__ __launch_bounds__(BLOCK) void k_count_nonfinite(const T *__restrict__ v, int64_t n, int *__restrict__ flag)
+{
+    bool bad = false;
+    for (int64_t i = int64_t(blockIdx.x) * BLOCK + threadIdx.x; i < n; i += int64_t(gridDim.x) * BLOCK)
+        bad |= !isfinite(v[i]);
+    // a plain store of the same value from any thread: no read-modify-write
+    if (bad) flag[0] = 1;
+}
+
+// scalar CSR column indices from the block pattern: row 3i+m lists 3j+n
+__global__ __launch_bounds__(BLOCK) void k_expand_inner(int n_nodes, const int32_t *__restrict__ brow_ptr,
+                                                        const int32_t *__restrict__ bcol, int32_t *__restrict__ inner)
+{
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n_nodes) return;
+    const int r0 = brow_ptr[i], deg = brow_ptr[i + 1] - r0;
+    for (int m = 0; m < 3; ++m)
+        for (int p = 0; p < deg; ++p)
+            for (int n = 0; n < 3; ++n) inner[9 * int64_t(r0) + int64_t(m) * 3 * deg + 3 * p + n] = 3 * bcol[r0 + p] + n;
+}
+
+} // namespace
+
+void sum_doubles(Handle &h, const double *in, int n, double *out)
+{
+    k_sum_doubles<<<1, BLOCK, 0, h.stream>>>(in, n, out);
+    PF_LAUNCH_CHECK();
+}
+
+bool all_finite(Handle &h, const void *dev, int64_t n, bool is_float)
+{
+    h.red_out.reserve(sizeof(int) * 4);
+    int *flag = h.red_out.as<int>();
+    PF_HIP(hipMemsetAsync(flag, 0, sizeof(int), h.stream));
+    const unsigned g = unsigned(std::min<int64_t>(4096, (n + BLOCK - 1) / BLOCK + 1));
+    if (is_float)
+        k_count_nonfinite<float><<<g, BLOCK, 0, h.stream>>>(static_cast<const float *>(dev), n, flag);
+    else
+        k_count_nonfinite<double><<<g, BLOCK, 0, h.stream>>>(static_cast<const double *>(dev), n, flag);
+    PF_LAUNCH_CHECK();
+    int host = 0;
+    PF_HIP(hipMemcpyAsync(&host, flag, sizeof(int), hipMemcpyDeviceToHost, h.stream));
+    PF_HIP(hipStreamSynchronize(h.stream));
+    return host == 0;
+}
+
+void build_device_pattern(Handle &h)
+{
+    if (h.d_outer.get()) return;
+    h.d_outer = upload(h.pat.outer, h.stream);
+    DevBuf bcol = upload(h.pat.bcol, h.stream);
+    DevBuf brow = upload(h.pat.brow_ptr, h.stream);
+    h.d_inner.alloc(sizeof(int32_t) * size_t(h.pat.nnz));
+    k_expand_inner<<<grid_for(h.n_nodes), BLOCK, 0, h.stream>>>(h.n_nodes, brow.as<int32_t>(), bcol.as<int32_t>(),
+                                                                 h.d_inner.as<int32_t>());
+    PF_LAUNCH_CHECK();
+    PF_HIP(hipStreamSynchronize(h.stream));
+}
+
+} // namespace pfgpu

@@ -1,0 +1,139 @@
+"""GPU parity of libpfgpu against the FP64 oracle (runs on an MI355X).
+
+Bars (BASELINE targets): sparsity indices bit-exact; FP64 energy, gradient and
+Hessian within 1e-10 relative; mixed precision within the stated FP32
+tolerances below.  Relative errors are max|a-b| / max|b| over the array.
+"""
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+from polyfem_amd import NLAssembler, PfgpuError
+from polyfem_amd.mesh import BASE_SEED, kuhn_box, lame_from_young, random_lame, smooth_displacement
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-10
+# FP32 element kernels: ~1e-7 per operation; sums of O(10) terms with mixed
+# signs and the conditioning of ln J leave ~1e-5 relative on the arrays.
+MIXED_TOL = {"energy": 1e-5, "gradient": 2e-5, "hessian": 2e-5, "hvp": 2e-5, "vjp": 5e-5}
+
+PATHS = ["plain"]
+CASES = [(1, "NeoHookean"), (2, "NeoHookean"), (1, "LinearElasticity"), (2, "LinearElasticity")]
+MAT = {"NeoHookean": po.NEOHOOKEAN, "LinearElasticity": po.LINEAR_ELASTICITY}
+
+
+def rel(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def setup_case(order, name, grid=(3, 2, 3), seed=0, precision="fp64", path="plain"):
+    m = kuhn_box(*grid, order=order)
+    lam, mu = random_lame(m.n_elements, seed=BASE_SEED + 100 + seed, E_mean=1e6, E_sigma=0.5)
+    x = smooth_displacement(m.nodes, 0.05, 1e-3, BASE_SEED + 200 + seed)
+    a = NLAssembler(name, m, lame=(lam, mu), precision=precision, path=path)
+    o = po.Oracle(m, lam, mu, material=MAT[name])
+    return m, x, a, o, lam, mu
+
+
+@pytest.mark.parametrize("path", PATHS)
+@pytest.mark.parametrize("order,name", CASES)
+def test_fp64_parity(order, name, path):
+    m, x, a, o, lam, mu = setup_case(order, name, path=path)
+    print(f"case order={order} material={name} path={path} E={m.n_elements}", flush=True)
+    assert rel(a.assemble_energy(x), o.energy(x)) < FP64_TOL
+    assert rel(a.assemble_energy_per_element(x), o.energy_per_element(x)) < FP64_TOL
+    assert rel(a.assemble_gradient(x), o.gradient(x)) < FP64_TOL
+    outer, inner, ref = o.hessian(x)
+    go, gi = a.hessian_pattern()
+    assert np.array_equal(go, outer) and np.array_equal(gi, inner)  # bit-exact sparsity
+    assert rel(a.hessian_values(x), ref) < FP64_TOL
+    e, g, vals = a.assemble_energy_gradient_hessian(x)
+    assert rel(e, o.energy(x)) < FP64_TOL and rel(g, o.gradient(x)) < FP64_TOL and rel(vals, ref) < FP64_TOL
+    rng = np.random.default_rng(BASE_SEED + 300)
+    v = rng.normal(size=m.n_dofs)
+    H = o.hessian_scipy(x)
+    assert rel(a.hessian_vector_product(x, v), H @ v) < FP64_TOL
+    p = rng.normal(size=m.n_dofs)
+    assert rel(a.force_material_derivative(x, p), o.material_vjp(x, p)) < FP64_TOL
+    assert rel(a.force_shape_derivative(x, p), o.shape_vjp(x, p)) < FP64_TOL
+    # forward mixed derivative is the transpose of the material term
+    dl, dm = rng.normal(size=m.n_elements), rng.normal(size=m.n_elements)
+    jvp = a.material_jvp(x, dl, dm)
+    term = o.material_vjp(x, p)
+    lhs = -p @ jvp
+    rhs = term[:m.n_elements] @ dl + term[m.n_elements:] @ dm
+    assert abs(lhs - rhs) <= 1e-10 * abs(rhs)
+
+
+@pytest.mark.parametrize("path", PATHS)
+@pytest.mark.parametrize("order,name", CASES)
+def test_mixed_precision_parity(order, name, path):
+    m, x, a, o, lam, mu = setup_case(order, name, precision="mixed", path=path, seed=1)
+    print(f"case mixed order={order} material={name} path={path}", flush=True)
+    assert rel(a.assemble_energy(x), o.energy(x)) < MIXED_TOL["energy"]
+    assert rel(a.assemble_gradient(x), o.gradient(x)) < MIXED_TOL["gradient"]
+    outer, inner, ref = o.hessian(x)
+    vals = a.hessian_values(x)
+    assert vals.dtype == np.float32
+    assert rel(vals, ref) < MIXED_TOL["hessian"]
+    v = np.random.default_rng(BASE_SEED + 301).normal(size=m.n_dofs)
+    assert rel(a.hessian_vector_product(x, v), o.hessian_scipy(x) @ v) < MIXED_TOL["hvp"]
+    p = np.random.default_rng(BASE_SEED + 302).normal(size=m.n_dofs)
+    assert rel(a.force_material_derivative(x, p), o.material_vjp(x, p)) < MIXED_TOL["vjp"]
+
+
+@pytest.mark.parametrize("path", PATHS)
+def test_config0_full_size_fp64(path):
+    """The 20^3 x 6 P1 NeoHookean state (48,000 tets, 27,783 dofs, nnz 1,157,229)."""
+    m = kuhn_box(20, 20, 20, order=1)
+    lam, mu = lame_from_young(1e6, 0.4)
+    x = smooth_displacement(m.nodes, 0.05, 1e-3, BASE_SEED)
+    a = NLAssembler("NeoHookean", m, young=1e6, poisson=0.4, path=path)
+    o = po.Oracle(m, lam, mu)
+    assert a.nnz == 1157229
+    e, g, vals = a.assemble_energy_gradient_hessian(x)
+    outer, inner, ref = o.hessian(x)
+    go, gi = a.hessian_pattern()
+    assert np.array_equal(go, outer) and np.array_equal(gi, inner)
+    assert rel(e, o.energy(x)) < FP64_TOL
+    assert rel(g, o.gradient(x)) < FP64_TOL
+    assert rel(vals, ref) < FP64_TOL
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("path", PATHS)
+def test_bitwise_deterministic(precision, path):
+    m, x, a, o, lam, mu = setup_case(2, "NeoHookean", grid=(4, 3, 3), precision=precision, path=path, seed=2)
+    r1 = a.assemble_energy_gradient_hessian(x)
+    r2 = a.assemble_energy_gradient_hessian(x)
+    assert r1[0] == r2[0]
+    assert np.array_equal(r1[1], r2[1]) and np.array_equal(r1[2], r2[2])
+    v = np.ones(m.n_dofs)
+    assert np.array_equal(a.hessian_vector_product(x, v), a.hessian_vector_product(x, v))
+
+
+def test_inverted_element_reports_invalid_state():
+    """Inverted elements give ln of a negative J -> NaN; the ABI returns
+    INVALID_STATE instead of crashing (ElasticForm.cpp:388-405)."""
+    m, x, a, o, lam, mu = setup_case(1, "NeoHookean", grid=(2, 2, 2), seed=3)
+    y = x.copy()
+    n0 = m.elem_nodes[0]
+    # collapse element 0 through its opposite face
+    y.reshape(-1, 3)[n0[3]] = m.nodes[n0[0]] - m.nodes[n0[3]] + (m.nodes[n0[0]] - m.nodes[n0[3]])
+    with pytest.raises(PfgpuError) as ei:
+        a.assemble_energy(y)
+    assert ei.value.status == 3
+    with pytest.raises(PfgpuError) as ei:
+        a.assemble_gradient(y)
+    assert ei.value.status == 3
+
+
+def test_update_lame_params():
+    m, x, a, o, lam, mu = setup_case(2, "NeoHookean", seed=4)
+    lam2, mu2 = lam * 1.3, mu * 0.7
+    a.update_lame_params(lam2, mu2)
+    o2 = po.Oracle(m, lam2, mu2)
+    assert rel(a.assemble_gradient(x), o2.gradient(x)) < FP64_TOL
