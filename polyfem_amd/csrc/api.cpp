@@ -144,8 +144,9 @@ void check_finite(pfgpu_handle &h, const void *dev, int64_t n, bool is_float, co
 int64_t pfgpu_handle::device_bytes() const
 {
     const DevBuf *bufs[] = {&conn, &vconn, &geom, &lame, &node_ptr, &node_inc, &vert_ptr, &vert_inc,
-                            &brow_ptr, &block_row, &contrib_ptr, &contrib, &patch_desc, &patch_elems,
-                            &patch_rows, &patch_pos, &d_outer, &d_inner, &loc, &hloc, &partial, &red_out,
+                            &brow_ptr, &block_row, &contrib_ptr, &contrib, &patch_desc, &patch_rows,
+                            &patch_row_acc, &patch_elems, &patch_cg, &patch_hdr, &patch_pos,
+                            &d_outer, &d_inner, &loc, &hloc, &partial, &red_out,
                             &h_x, &h_v, &h_out, &h_out2, &h_vals};
     int64_t s = 0;
     for (const DevBuf *b : bufs) s += int64_t(b->bytes());
@@ -180,13 +181,15 @@ pfgpu_status pfgpu_create(const pfgpu_mesh_desc *mesh, const pfgpu_material_desc
         h->n_quad = qo == 1 ? 1 : 4;
         h->material = material->kind;
         h->precision = precision;
-        h->path = PFGPU_PATH_PLAIN;
+        h->path = PFGPU_PATH_FUSED;
         h->n_elements = m.n_elements;
         h->n_nodes = m.n_nodes;
         h->n_vertices = m.n_vertices;
         h->n_dofs = 3 * m.n_nodes;
 
         h->h_elem_nodes.assign(m.elem_nodes, m.elem_nodes + size_t(m.n_elements) * h->n_local);
+        h->h_vertices.assign(m.vertices, m.vertices + size_t(m.n_vertices) * 3);
+        h->h_elem_vertices.assign(m.elem_vertices, m.elem_vertices + size_t(m.n_elements) * 4);
         h->pat = build_pattern(m.n_elements, h->n_local, m.elem_nodes, m.n_nodes, m.n_vertices, m.elem_vertices);
 
         std::vector<double> g;

@@ -350,7 +350,7 @@ template <typename T> __device__ __forceinline__ void metric(const T *jit, T *g6
 }
 
 // sum_q G_a(q).G_b(q) = sum_{k,l} metric[k][l] RGG(a,b,k,l)
-template <int ORDER, int NQ, typename T> __device__ __forceinline__ T grad_dot(int a, int b, const T *g6)
+template <int ORDER, int NQ, typename T> __host__ __device__ __forceinline__ T grad_dot(int a, int b, const T *g6)
 {
     T s = T(0);
     const int kk[6] = {0, 1, 2, 0, 0, 1}, ll[6] = {0, 1, 2, 1, 2, 2};

@@ -1,0 +1,42 @@
+// Patch tables of the fused assembly kernels (see fused_tables.cpp).
+#pragma once
+
+#include "common.hpp"
+
+namespace pfgpu {
+
+// Launch shape the tables are built for.
+struct FusedShape {
+    int threads;      // workgroup size
+    int groups;       // lane groups per workgroup (rows are dealt to groups)
+    int chunk;        // elements staged in LDS at a time
+    int acc_capacity; // accumulator entries (values of owned rows) per patch
+    int max_rows;     // owned rows per patch
+};
+
+struct FusedTables {
+    FusedShape shape{};
+    int n_patches = 0;
+    int max_rows = 0;
+    int max_acc = 0;
+    int64_t visits = 0;           // sum over patches of incident elements
+    std::vector<int32_t> desc;    // per patch: row0, n_rows, elem0, n_elems, cg0, n_chunks, acc, 0
+    std::vector<int32_t> rows;    // owned node ids
+    std::vector<int32_t> row_acc; // accumulator offset of each owned row
+    std::vector<int32_t> elems;   // incident element ids; bit 31: the patch sums its energy
+    std::vector<int32_t> cg;      // per patch, per (chunk, group): first item; +1 sentinel
+    std::vector<int32_t> hdr;     // item: element-in-chunk | a << 10 | row << 14
+    std::vector<uint8_t> pos;     // item x n_local: column-block position in the row
+};
+
+FusedTables build_fused_tables(const Pattern &pat, int n_local, const int32_t *elem_nodes, int n_elements,
+                               const double *vertices, const int32_t *elem_vertices, const FusedShape &shape);
+
+// Device view of the tables.
+struct FusedDev {
+    const int32_t *desc, *rows, *row_acc, *elems, *cg, *hdr;
+    const uint8_t *pos;
+    const int32_t *brow_ptr;
+};
+
+} // namespace pfgpu

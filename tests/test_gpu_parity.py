@@ -18,7 +18,7 @@ FP64_TOL = 1e-10
 # signs and the conditioning of ln J leave ~1e-5 relative on the arrays.
 MIXED_TOL = {"energy": 1e-5, "gradient": 2e-5, "hessian": 2e-5, "hvp": 2e-5, "vjp": 5e-5}
 
-PATHS = ["plain"]
+PATHS = ["fused", "plain"]
 CASES = [(1, "NeoHookean"), (2, "NeoHookean"), (1, "LinearElasticity"), (2, "LinearElasticity")]
 MAT = {"NeoHookean": po.NEOHOOKEAN, "LinearElasticity": po.LINEAR_ELASTICITY}
 
@@ -137,3 +137,21 @@ def test_update_lame_params():
     a.update_lame_params(lam2, mu2)
     o2 = po.Oracle(m, lam2, mu2)
     assert rel(a.assemble_gradient(x), o2.gradient(x)) < FP64_TOL
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("order", [1, 2])
+def test_fused_many_patches(order, precision):
+    """Meshes large enough for many patches and several staging chunks; fused
+    and plain paths against the oracle and against each other."""
+    grid = (8, 8, 8) if order == 2 else (16, 14, 12)
+    m, x, a, o, lam, mu = setup_case(order, "NeoHookean", grid=grid, precision=precision, path="fused", seed=5)
+    tol = FP64_TOL if precision == "fp64" else MIXED_TOL["hessian"]
+    print(f"case many patches order={order} precision={precision} E={m.n_elements}", flush=True)
+    e, g, vals = a.assemble_energy_gradient_hessian(x)
+    outer, inner, ref = o.hessian(x)
+    assert rel(vals, ref) < tol
+    assert rel(g, o.gradient(x)) < (FP64_TOL if precision == "fp64" else MIXED_TOL["gradient"])
+    assert rel(e, o.energy(x)) < (FP64_TOL if precision == "fp64" else MIXED_TOL["energy"])
+    v = np.random.default_rng(BASE_SEED + 303).normal(size=m.n_dofs)
+    assert rel(a.hessian_vector_product(x, v), o.hessian_scipy(x) @ v) < (FP64_TOL if precision == "fp64" else MIXED_TOL["hvp"])
