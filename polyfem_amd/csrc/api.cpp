@@ -145,7 +145,7 @@ int64_t pfgpu_handle::device_bytes() const
 {
     const DevBuf *bufs[] = {&conn, &vconn, &geom, &lame, &node_ptr, &node_inc, &vert_ptr, &vert_inc,
                             &brow_ptr, &block_row, &contrib_ptr, &contrib, &patch_desc, &patch_rows,
-                            &patch_row_acc, &patch_elems, &patch_cg, &patch_hdr, &patch_pos,
+                            &patch_row_acc, &patch_elems, &patch_cs, &patch_cg, &patch_hdr, &patch_pos,
                             &d_outer, &d_inner, &loc, &hloc, &partial, &red_out,
                             &h_x, &h_v, &h_out, &h_out2, &h_vals};
     int64_t s = 0;

@@ -1,4 +1,4 @@
-// Patch tables of the fused assembly kernels (see fused_tables.cpp).
+// Patch tables of the fused assembly kernel (see fused_tables.cpp).
 #pragma once
 
 #include "common.hpp"
@@ -7,24 +7,30 @@ namespace pfgpu {
 
 // Launch shape the tables are built for.
 struct FusedShape {
-    int threads;      // workgroup size
-    int groups;       // lane groups per workgroup (rows are dealt to groups)
-    int chunk;        // elements staged in LDS at a time
+    int groups;       // lane groups that accumulate rows (rows are dealt to groups)
+    int chunk;        // elements staged in LDS at a time (at most)
+    int item_cap;     // (row, element) items per chunk (at most)
     int acc_capacity; // accumulator entries (values of owned rows) per patch
     int max_rows;     // owned rows per patch
 };
+
+// desc fields per patch
+enum { PD_ROW0, PD_NROWS, PD_EL0, PD_NEL, PD_CG0, PD_NCH, PD_ACC, PD_CHUNK_BASE, PD_CS0, PD_STRIDE = 12 };
 
 struct FusedTables {
     FusedShape shape{};
     int n_patches = 0;
     int max_rows = 0;
     int max_acc = 0;
+    int max_unit_items = 0;
+    int64_t n_chunks = 0;         // chunks over all patches (one energy partial each)
     int64_t visits = 0;           // sum over patches of incident elements
-    std::vector<int32_t> desc;    // per patch: row0, n_rows, elem0, n_elems, cg0, n_chunks, acc, 0
+    std::vector<int32_t> desc;    // PD_STRIDE ints per patch
     std::vector<int32_t> rows;    // owned node ids
     std::vector<int32_t> row_acc; // accumulator offset of each owned row
     std::vector<int32_t> elems;   // incident element ids; bit 31: the patch sums its energy
-    std::vector<int32_t> cg;      // per patch, per (chunk, group): first item; +1 sentinel
+    std::vector<int32_t> chunk_start; // per patch n_chunks+1 element offsets (patch-local)
+    std::vector<int32_t> cg;      // per patch, per (chunk, group): first item; + sentinel
     std::vector<int32_t> hdr;     // item: element-in-chunk | a << 10 | row << 14
     std::vector<uint8_t> pos;     // item x n_local: column-block position in the row
 };
@@ -34,7 +40,7 @@ FusedTables build_fused_tables(const Pattern &pat, int n_local, const int32_t *e
 
 // Device view of the tables.
 struct FusedDev {
-    const int32_t *desc, *rows, *row_acc, *elems, *cg, *hdr;
+    const int32_t *desc, *rows, *row_acc, *elems, *chunk_start, *cg, *hdr;
     const uint8_t *pos;
     const int32_t *brow_ptr;
 };

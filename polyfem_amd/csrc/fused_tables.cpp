@@ -106,9 +106,9 @@ FusedTables build_fused_tables(const Pattern &pat, int n_local, const int32_t *e
 
     // per-patch construction, independent across patches
     struct Local {
-        std::vector<int32_t> rows, row_acc, elems, cg, hdr;
+        std::vector<int32_t> rows, row_acc, elems, chunk_start, cg, hdr;
         std::vector<uint8_t> pos;
-        int n_chunks = 0, acc = 0;
+        int n_chunks = 0, acc = 0, max_unit_items = 0;
     };
     std::vector<Local> loc(P);
     const int G = shape.groups;
@@ -134,14 +134,36 @@ FusedTables build_fused_tables(const Pattern &pat, int n_local, const int32_t *e
         std::sort(el.begin(), el.end());
         el.erase(std::unique(el.begin(), el.end()), el.end());
         const int nel = int(el.size());
-        L.n_chunks = (nel + CH - 1) / CH;
         L.elems.resize(nel);
+        std::vector<int> owned_rows_in(nel, 0);
+        for (int r = 0; r < R; ++r)
+            for (int k = pat.node_ptr[L.rows[r]]; k < pat.node_ptr[L.rows[r] + 1]; ++k) {
+                const int e = pat.node_inc[k] / n_local;
+                owned_rows_in[std::lower_bound(el.begin(), el.end(), e) - el.begin()]++;
+            }
         for (int k = 0; k < nel; ++k) {
             const int e = el[k];
             // the element's energy is summed by the patch owning its first node
             const bool own = patch_of[elem_nodes[size_t(e) * n_local]] == p;
             L.elems[k] = e | (own ? int32_t(0x80000000u) : 0);
         }
+        // chunks: at most CH elements and at most item_cap items each
+        std::vector<int> chunk_of(nel);
+        L.chunk_start.push_back(0);
+        for (int k = 0, cnt = 0, items = 0; k < nel; ++k) {
+            if (cnt == CH || items + owned_rows_in[k] > shape.item_cap) {
+                L.chunk_start.push_back(k);
+                L.max_unit_items = std::max(L.max_unit_items, items);
+                cnt = 0;
+                items = 0;
+            }
+            chunk_of[k] = int(L.chunk_start.size()) - 1;
+            cnt++;
+            items += owned_rows_in[k];
+            if (k == nel - 1) L.max_unit_items = std::max(L.max_unit_items, items);
+        }
+        L.chunk_start.push_back(nel);
+        L.n_chunks = int(L.chunk_start.size()) - 1;
         // rows -> groups, longest first onto the least loaded group
         std::vector<int> items_of_row(R), ridx(R);
         for (int r = 0; r < R; ++r) items_of_row[r] = pat.node_ptr[L.rows[r] + 1] - pat.node_ptr[L.rows[r]];
@@ -161,7 +183,7 @@ FusedTables build_fused_tables(const Pattern &pat, int n_local, const int32_t *e
             for (int k = pat.node_ptr[node]; k < pat.node_ptr[node + 1]; ++k) {
                 const int e = pat.node_inc[k] / n_local, a = pat.node_inc[k] % n_local;
                 const int ek = int(std::lower_bound(el.begin(), el.end(), e) - el.begin());
-                items.push_back({ek / CH, group_of[r], ek, a, r});
+                items.push_back({chunk_of[ek], group_of[r], ek, a, r});
             }
         }
         std::stable_sort(items.begin(), items.end(), [](const Item &x, const Item &y) {
@@ -176,7 +198,7 @@ FusedTables build_fused_tables(const Pattern &pat, int n_local, const int32_t *e
         L.pos.resize(items.size() * size_t(n_local));
         for (size_t i = 0; i < items.size(); ++i) {
             const Item &it = items[i];
-            const int kk = it.elem_k - it.chunk * CH;
+            const int kk = it.elem_k - L.chunk_start[it.chunk];
             L.hdr[i] = kk | (it.a << 10) | (it.r << 14);
             const int node = L.rows[it.r];
             const int32_t *c0 = pat.bcol.data() + pat.brow_ptr[node], *c1 = pat.bcol.data() + pat.brow_ptr[node + 1];
@@ -200,41 +222,47 @@ FusedTables build_fused_tables(const Pattern &pat, int n_local, const int32_t *e
     }
 
     // concatenate
-    t.desc.resize(size_t(P) * 8);
-    int64_t nrow = 0, nel = 0, ncg = 0, nit = 0;
+    t.desc.assign(size_t(P) * PD_STRIDE, 0);
+    int64_t nrow = 0, nel = 0, ncg = 0, nit = 0, ncs = 0;
     for (int p = 0; p < P; ++p) {
         nrow += loc[p].rows.size();
         nel += loc[p].elems.size();
         ncg += loc[p].cg.size();
         nit += loc[p].hdr.size();
+        ncs += loc[p].chunk_start.size();
     }
     if (nit >= (int64_t(1) << 31) || nel >= (int64_t(1) << 31)) fail(PFGPU_ERR_UNSUPPORTED, "patch tables exceed 32-bit offsets");
     t.rows.reserve(nrow);
     t.row_acc.reserve(nrow);
     t.elems.reserve(nel);
+    t.chunk_start.reserve(ncs);
     t.cg.reserve(ncg);
     t.hdr.reserve(nit);
     t.pos.reserve(nit * n_local);
     for (int p = 0; p < P; ++p) {
         Local &L = loc[p];
-        int32_t *d = t.desc.data() + size_t(p) * 8;
-        d[0] = int32_t(t.rows.size());
-        d[1] = int32_t(L.rows.size());
-        d[2] = int32_t(t.elems.size());
-        d[3] = int32_t(L.elems.size());
-        d[4] = int32_t(t.cg.size());
-        d[5] = L.n_chunks;
-        d[6] = L.acc;
-        d[7] = 0;
+        int32_t *d = t.desc.data() + size_t(p) * PD_STRIDE;
+        d[PD_ROW0] = int32_t(t.rows.size());
+        d[PD_NROWS] = int32_t(L.rows.size());
+        d[PD_EL0] = int32_t(t.elems.size());
+        d[PD_NEL] = int32_t(L.elems.size());
+        d[PD_CG0] = int32_t(t.cg.size());
+        d[PD_NCH] = L.n_chunks;
+        d[PD_ACC] = L.acc;
+        d[PD_CHUNK_BASE] = int32_t(t.n_chunks);
+        d[PD_CS0] = int32_t(t.chunk_start.size());
         const int32_t item_base = int32_t(t.hdr.size());
         t.rows.insert(t.rows.end(), L.rows.begin(), L.rows.end());
         t.row_acc.insert(t.row_acc.end(), L.row_acc.begin(), L.row_acc.end());
         t.elems.insert(t.elems.end(), L.elems.begin(), L.elems.end());
+        t.chunk_start.insert(t.chunk_start.end(), L.chunk_start.begin(), L.chunk_start.end());
         for (int32_t v : L.cg) t.cg.push_back(v + item_base);
         t.hdr.insert(t.hdr.end(), L.hdr.begin(), L.hdr.end());
         t.pos.insert(t.pos.end(), L.pos.begin(), L.pos.end());
         t.max_rows = std::max(t.max_rows, int(L.rows.size()));
         t.max_acc = std::max(t.max_acc, L.acc);
+        t.max_unit_items = std::max(t.max_unit_items, L.max_unit_items);
+        t.n_chunks += L.n_chunks;
         t.visits += int64_t(L.elems.size());
         L = Local();
     }

@@ -22,10 +22,10 @@ struct pfgpu_handle {
     pfgpu::DevBuf brow_ptr, block_row, contrib_ptr, contrib;
     // fused-path patch tables (built on first use)
     bool fused_ready = false;
-    pfgpu::DevBuf patch_desc, patch_rows, patch_row_acc, patch_elems, patch_cg, patch_hdr, patch_pos;
+    pfgpu::DevBuf patch_desc, patch_rows, patch_row_acc, patch_elems, patch_cs, patch_cg, patch_hdr, patch_pos;
     int n_patches = 0;
     int fused_max_acc = 0, fused_max_rows = 0;
-    int64_t fused_visits = 0;
+    int64_t fused_visits = 0, fused_chunks = 0;
     std::vector<double> h_vertices;       // host copy, n_vertices x 3
     std::vector<int32_t> h_elem_vertices; // host copy, E x 4
     // lazily built device pattern

@@ -159,23 +159,28 @@ static void direct(const Mesh &m, const Pattern &pat, std::vector<double> &vals,
 }
 
 template <int ORDER, int NQ>
-static int run(const Mesh &m, int groups_per_wave, int threads, int chunk, int acc_capacity, int max_rows)
+static int run(const Mesh &m, int groups, int chunk, int item_cap, int acc_capacity, int max_rows)
 {
     constexpr int NB = NLocal<ORDER>::value, SQ = 20, SE = SQ * NQ + 8;
     Pattern pat = build_pattern(m.E, NB, m.en.data(), m.N, m.Nv, m.ev.data());
-    FusedShape s{threads, (threads / 64) * groups_per_wave, chunk, acc_capacity, max_rows};
+    FusedShape s{groups, chunk, item_cap, acc_capacity, max_rows};
     FusedTables t = build_fused_tables(pat, NB, m.en.data(), m.E, m.vx.data(), m.ev.data(), s);
     const int G = s.groups;
-    std::printf("patches %d max_rows %d max_acc %d visits/E %.3f\n", t.n_patches, t.max_rows, t.max_acc,
-                double(t.visits) / m.E);
+    std::printf("patches %d max_rows %d max_acc %d max_unit_items %d chunks %ld visits/E %.3f\n", t.n_patches,
+                t.max_rows, t.max_acc, t.max_unit_items, long(t.n_chunks), double(t.visits) / m.E);
+    CHECK(t.max_unit_items <= item_cap, "chunk item cap");
+    int64_t chunk_sum = 0;
     std::vector<int> owner(m.N, -1);
     std::vector<double> vals(size_t(pat.nnz), -1e300), grad(size_t(m.N) * 3, -1e300);
     std::vector<long> seen_pairs(size_t(m.E) * NB, 0);
     for (int p = 0; p < t.n_patches; ++p) {
-        const int32_t *D = &t.desc[8 * size_t(p)];
-        const int row0 = D[0], nrows = D[1], el0 = D[2], nel = D[3], cg0 = D[4], nch = D[5], nacc = D[6];
+        const int32_t *D = &t.desc[PD_STRIDE * size_t(p)];
+        const int row0 = D[PD_ROW0], nrows = D[PD_NROWS], el0 = D[PD_EL0], nel = D[PD_NEL], cg0 = D[PD_CG0];
+        const int nch = D[PD_NCH], nacc = D[PD_ACC], cs0 = D[PD_CS0];
         CHECK(nrows <= t.max_rows && nacc <= t.max_acc && nacc <= acc_capacity, "patch %d size", p);
-        CHECK(nch == (nel + chunk - 1) / chunk, "patch %d chunks", p);
+        CHECK(D[PD_CHUNK_BASE] == chunk_sum, "chunk base");
+        chunk_sum += nch;
+        CHECK(t.chunk_start[cs0] == 0 && t.chunk_start[cs0 + nch] == nel, "patch %d chunk range", p);
         CHECK(cg0 + nch * G < int(t.cg.size()), "patch %d cg range", p);
         for (int r = 0; r < nrows; ++r) {
             const int node = t.rows[row0 + r];
@@ -190,7 +195,9 @@ static int run(const Mesh &m, int groups_per_wave, int threads, int chunk, int a
         // replay
         std::vector<double> acc(nacc, 0.0), gacc(3 * size_t(nrows), 0.0), stg(size_t(chunk) * SE, std::nan(""));
         for (int c = 0; c < nch; ++c) {
-            const int cbeg = c * chunk, cn = std::min(chunk, nel - cbeg);
+            const int cbeg = t.chunk_start[cs0 + c], cn = t.chunk_start[cs0 + c + 1] - cbeg;
+            CHECK(cn > 0 && cn <= chunk, "chunk size");
+            CHECK(t.cg[cg0 + (c + 1) * G] - t.cg[cg0 + c * G] <= item_cap, "chunk items");
             std::fill(stg.begin(), stg.end(), std::nan(""));
             for (int kk = 0; kk < cn; ++kk) {
                 const int e = t.elems[el0 + cbeg + kk] & 0x7fffffff;
@@ -267,11 +274,11 @@ static int run(const Mesh &m, int groups_per_wave, int threads, int chunk, int a
 
 int main(int argc, char **argv)
 {
-    if (argc < 7) { std::fprintf(stderr, "usage: fused_check mesh.bin gpw threads chunk acc_capacity max_rows\n"); return 2; }
+    if (argc < 7) { std::fprintf(stderr, "usage: fused_check mesh.bin groups chunk item_cap acc_capacity max_rows\n"); return 2; }
     Mesh m = read_mesh(argv[1]);
-    const int gpw = std::atoi(argv[2]), threads = std::atoi(argv[3]), chunk = std::atoi(argv[4]);
+    const int groups = std::atoi(argv[2]), chunk = std::atoi(argv[3]), icap = std::atoi(argv[4]);
     const int cap = std::atoi(argv[5]), max_rows = std::atoi(argv[6]);
-    int f = m.order == 1 ? run<1, 1>(m, gpw, threads, chunk, cap, max_rows) : run<2, 4>(m, gpw, threads, chunk, cap, max_rows);
+    int f = m.order == 1 ? run<1, 1>(m, groups, chunk, icap, cap, max_rows) : run<2, 4>(m, groups, chunk, icap, cap, max_rows);
     std::printf(f ? "FAIL (%d)\n" : "OK\n", f);
     return f ? 1 : 0;
 }

@@ -40,20 +40,20 @@ def write_mesh(path, m, x, lam, mu):
         mu.astype(np.float64).tofile(f)
 
 
-@pytest.mark.parametrize("order,grid,chunk,cap", [
-    (2, (3, 3, 4), 128, 4000),      # several patches, one chunk each
-    (2, (4, 3, 3), 64, 12000),      # several chunks per patch
-    (2, (10, 10, 10), 128, 27000),  # the FP32 P2 launch shape
-    (1, (6, 5, 4), 512, 3000),
-    (1, (12, 12, 12), 512, 24000),  # the FP32 P1 launch shape
+@pytest.mark.parametrize("order,grid,groups,chunk,icap,cap", [
+    (2, (3, 3, 4), 72, 64, 512, 4000),       # several patches
+    (2, (4, 3, 3), 72, 16, 40, 12000),       # several chunks per patch, item-capped chunks
+    (2, (10, 10, 10), 72, 64, 512, 23000),   # the FP32 P2 launch shape
+    (2, (6, 6, 6), 84, 32, 256, 11000),      # the FP64 P2 launch shape
+    (1, (6, 5, 4), 192, 256, 1024, 3000),
+    (1, (12, 12, 12), 192, 256, 1024, 20000),  # the FP32 P1 launch shape
 ])
-def test_fused_tables(exe, tmp_path, order, grid, chunk, cap):
+def test_fused_tables(exe, tmp_path, order, grid, groups, chunk, icap, cap):
     m = kuhn_box(*grid, order=order)
     lam, mu = random_lame(m.n_elements, seed=BASE_SEED + 1)
     x = smooth_displacement(m.nodes, 0.05, 1e-3, BASE_SEED)
     path = str(tmp_path / "mesh.bin")
     write_mesh(path, m, x, lam, mu)
-    gpw = 64 // (4 if order == 1 else 10)
-    r = subprocess.run([exe, path, str(gpw), "512", str(chunk), str(cap), "256"], capture_output=True, text=True)
+    r = subprocess.run([exe, path, str(groups), str(chunk), str(icap), str(cap), "256"], capture_output=True, text=True)
     print(r.stdout, r.stderr)
     assert r.returncode == 0 and "OK" in r.stdout
