@@ -55,23 +55,6 @@ template <int ORDER, int NQ> __device__ __forceinline__ const RGTab<ORDER, NQ> &
     else return c_tab24;
 }
 
-constexpr int NT = 1024;            // workgroup size
-constexpr int NWAVE = NT / 64;
-constexpr int LDS_LIMIT = 160 * 1024;
-constexpr int SQ = 20;              // staging per (element, q): M[9], cl, cc, Q[9]
-
-template <int NQ> constexpr int stage_stride() { return SQ * NQ + 8; } // + metric[6] + pad
-
-// elements per chunk: one producer thread per (element, q), 4 producer waves
-// in FP32 and 2 in FP64 (whose staging is twice as large)
-template <typename T, int NQ> constexpr int chunk_elems() { return (sizeof(T) == 4 ? 256 : 128) / NQ; }
-template <typename T, int NQ> constexpr int producer_waves() { return chunk_elems<T, NQ>() * NQ / 64; }
-template <typename T, int ORDER> constexpr int item_cap() { return (ORDER == 2 ? 512 : 1024) / (sizeof(T) == 4 ? 1 : 2); }
-template <typename T, int ORDER, int NQ> constexpr int n_groups()
-{
-    return (NWAVE - producer_waves<T, NQ>()) * (64 / NLocal<ORDER>::value);
-}
-
 // byte offsets of the LDS regions
 struct Lds {
     int acc, gacc, stg0, stg1, hdr0, hdr1, pos0, pos1, cg0, cg1, rg, rgg, racc, rdeg, red, total;
@@ -207,8 +190,8 @@ __global__ __launch_bounds__(NT) void k_egh(DevMesh<T> m, const double *__restri
         for (int i = pt; i < it1 - it0; i += WA * 64) lhdr[i] = fd.hdr[it0 + i];
         for (int i = pt; i < (it1 - it0) * NB; i += WA * 64) lpos[i] = fd.pos[int64_t(it0) * NB + i];
         double eacc = 0;
-        const int q = wave % NQ;
-        const int kk = (wave / NQ) * 64 + lane;
+        int q, kk;
+        producer_task<T, NQ>(tid, q, kk);
         if (kk < cn) {
             const int32_t ev = fd.elems[el0 + cbeg + kk];
             const int e = ev & 0x7fffffff;

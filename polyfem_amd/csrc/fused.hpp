@@ -1,9 +1,40 @@
 // Patch tables of the fused assembly kernel (see fused_tables.cpp).
 #pragma once
 
-#include "common.hpp"
+#include "elem.hpp"
 
 namespace pfgpu {
+
+// ---- launch shape of the fused kernel (shared with the host table builder
+// checks): one persistent workgroup of NT threads per CU; the first
+// producer_waves() wavefronts stage chunks, the others accumulate.
+constexpr int NT = 1024;            // workgroup size
+constexpr int NWAVE = NT / 64;
+constexpr int LDS_LIMIT = 160 * 1024;
+constexpr int SQ = 20;              // staging per (element, q): M[9], cl, cc, Q[9]
+
+template <int NQ> constexpr int stage_stride() { return SQ * NQ + 8; } // + metric[6] + pad
+
+// elements per chunk: one producer thread per (element, q), 4 producer waves
+// in FP32 and 2 in FP64 (whose staging is twice as large)
+template <typename T, int NQ> constexpr int chunk_elems() { return (sizeof(T) == 4 ? 256 : 128) / NQ; }
+template <typename T, int NQ> constexpr int producer_waves() { return chunk_elems<T, NQ>() * NQ / 64; }
+template <typename T, int ORDER> constexpr int item_cap() { return (ORDER == 2 ? 512 : 1024) / (sizeof(T) == 4 ? 1 : 2); }
+
+// producer thread t -> (quadrature point, element in chunk); the point is
+// uniform across a wavefront whenever the chunk is a multiple of 64 elements
+template <typename T, int NQ> __host__ __device__ constexpr void producer_task(int t, int &q, int &kk)
+{
+    constexpr int CH = chunk_elems<T, NQ>();
+    static_assert(CH * NQ == producer_waves<T, NQ>() * 64, "one producer thread per (element, point)");
+    q = t / CH;
+    kk = t - q * CH;
+}
+template <typename T, int ORDER, int NQ> constexpr int n_groups()
+{
+    return (NWAVE - producer_waves<T, NQ>()) * (64 / NLocal<ORDER>::value);
+}
+
 
 // Launch shape the tables are built for.
 struct FusedShape {

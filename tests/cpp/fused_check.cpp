@@ -272,8 +272,27 @@ static int run(const Mesh &m, int groups, int chunk, int item_cap, int acc_capac
     return n_fail;
 }
 
+// every producer thread maps to a distinct (point, element) and all pairs of a chunk are covered
+template <typename T, int NQ> static void check_producer_map()
+{
+    constexpr int CH = chunk_elems<T, NQ>(), WA = producer_waves<T, NQ>();
+    std::vector<int> seen(size_t(CH) * NQ, 0);
+    for (int t = 0; t < WA * 64; ++t) {
+        int q = -1, kk = -1;
+        producer_task<T, NQ>(t, q, kk);
+        CHECK(q >= 0 && q < NQ && kk >= 0 && kk < CH, "producer task %d -> (%d, %d) out of range", t, q, kk);
+        if (q >= 0 && q < NQ && kk >= 0 && kk < CH) seen[size_t(kk) * NQ + q]++;
+    }
+    for (size_t i = 0; i < seen.size(); ++i) CHECK(seen[i] == 1, "producer pair %zu covered %d times", i, seen[i]);
+    CHECK(WA < NWAVE, "consumers left");
+}
+
 int main(int argc, char **argv)
 {
+    check_producer_map<float, 1>();
+    check_producer_map<float, 4>();
+    check_producer_map<double, 1>();
+    check_producer_map<double, 4>();
     if (argc < 7) { std::fprintf(stderr, "usage: fused_check mesh.bin groups chunk item_cap acc_capacity max_rows\n"); return 2; }
     Mesh m = read_mesh(argv[1]);
     const int groups = std::atoi(argv[2]), chunk = std::atoi(argv[3]), icap = std::atoi(argv[4]);
