@@ -9,7 +9,7 @@ import ctypes as C
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libpfgpu.so")
+LIB_PATH = os.environ.get("PFGPU_LIB", os.path.join(HERE, "libpfgpu.so"))
 
 PFGPU_OK = 0
 STATUS_NAMES = {0: "OK", 1: "INVALID_ARGUMENT", 2: "HIP", 3: "INVALID_STATE", 4: "UNSUPPORTED", 5: "OUT_OF_MEMORY"}

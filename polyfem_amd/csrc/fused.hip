@@ -74,16 +74,16 @@ template <typename T, int ORDER, int NQ> Lds lds_layout(int max_acc, int max_row
     };
     L.acc = take(max_acc * int(sizeof(T)));
     L.gacc = take(3 * max_rows * int(sizeof(T)));
-    L.stg0 = take(CH * stage_stride<NQ>() * int(sizeof(T)));
-    L.stg1 = take(CH * stage_stride<NQ>() * int(sizeof(T)));
+    L.stg0 = take(CH * Stage<ORDER, NQ>::SIZE * int(sizeof(T)));
+    L.stg1 = take(CH * Stage<ORDER, NQ>::SIZE * int(sizeof(T)));
     L.hdr0 = take(IC * 4);
     L.hdr1 = take(IC * 4);
     L.pos0 = take(IC * NB);
     L.pos1 = take(IC * NB);
     L.cg0 = take((NG + 1) * 4);
     L.cg1 = take((NG + 1) * 4);
-    L.rg = take(NQ * NB * 3 * int(sizeof(T)));
-    L.rgg = take(NB * NB * 6 * int(sizeof(T)));
+    L.rg = take(NQ * NB * 4 * int(sizeof(T)));
+    L.rgg = take(NB * NB * 8 * int(sizeof(T)));
     L.racc = take(max_rows * 4);
     L.rdeg = take(max_rows * 4);
     L.red = take(2 * NWAVE * 8);
@@ -91,11 +91,14 @@ template <typename T, int ORDER, int NQ> Lds lds_layout(int max_acc, int max_row
     return L;
 }
 
-// producer work for one (element, quadrature point Q)
+// producer work for one (element, quadrature point Q): fills the staging
+// record s of the element (layout Stage<ORDER, NQ>)
 template <typename T, int ORDER, int NQ, int MAT, int Q>
 __device__ __forceinline__ void setup_point(const DevMesh<T> &m, int e, const double *__restrict__ x, T *__restrict__ s,
                                             bool own, double &eacc)
 {
+    using S = Stage<ORDER, NQ>;
+    constexpr int NB = NLocal<ORDER>::value;
     Elem<T, ORDER, NQ> el;
     load_geometry(m, e, el);
     gather(m, e, x, el.U);
@@ -106,26 +109,40 @@ __device__ __forceinline__ void setup_point(const DevMesh<T> &m, int e, const do
     if (own) eacc += double(energy_density<T, MAT>(k, el.lam, el.mu) * w);
     T P[9];
     pk1<T, MAT>(k, el.lam, el.mu, P);
-    T *sq = s + Q * SQ;
     // Q = w P jit^T: the gradient row of node a is sum_q sum_k RG(q,a,k) Q[d][k]
 #pragma unroll
     for (int d = 0; d < 3; ++d)
 #pragma unroll
         for (int kk = 0; kk < 3; ++kk)
-            sq[11 + d * 3 + kk] = w * (P[d * 3 + 0] * el.jit[kk * 3 + 0] + P[d * 3 + 1] * el.jit[kk * 3 + 1] + P[d * 3 + 2] * el.jit[kk * 3 + 2]);
+            s[S::Q + Q * 9 + d * 3 + kk] =
+                w * (P[d * 3 + 0] * el.jit[kk * 3 + 0] + P[d * 3 + 1] * el.jit[kk * 3 + 1] + P[d * 3 + 2] * el.jit[kk * 3 + 2]);
     T M[9], cl, cc;
     hess_qp<T, MAT>(k, el.jit, el.lam, el.mu, w, M, cl, cc);
 #pragma unroll
-    for (int i = 0; i < 9; ++i) sq[i] = M[i];
-    sq[9] = cl;
-    sq[10] = cc;
+    for (int b = 0; b < NB; ++b) khat<T, ORDER, NQ>(Q, b, M, s + S::K + (Q * NB + b) * 3);
+    s[S::C + Q] = cl;
+    s[S::C + NQ + Q] = cc;
     if (Q == 0) {
         T g6[6];
         metric(el.jit, g6);
         const T cm = w * el.mu;
 #pragma unroll
-        for (int i = 0; i < 6; ++i) s[NQ * SQ + i] = cm * g6[i];
+        for (int i = 0; i < 6; ++i) s[S::G + i] = cm * g6[i];
     }
+}
+
+template <typename T> struct Vec4 {
+    T x, y, z, w;
+};
+__device__ __forceinline__ Vec4<float> ld4(const float *p)
+{
+    const float4 v = *reinterpret_cast<const float4 *>(p);
+    return {v.x, v.y, v.z, v.w};
+}
+__device__ __forceinline__ Vec4<double> ld4(const double *p)
+{
+    const double2 a = *reinterpret_cast<const double2 *>(p), b = *reinterpret_cast<const double2 *>(p + 2);
+    return {a.x, a.y, b.x, b.y};
 }
 
 struct Unit {
@@ -140,7 +157,8 @@ __global__ __launch_bounds__(NT) void k_egh(DevMesh<T> m, const double *__restri
     constexpr int GPW = 64 / NB;
     constexpr int WA = producer_waves<T, NQ>();
     constexpr int NG = n_groups<T, ORDER, NQ>();
-    constexpr int SE = stage_stride<NQ>();
+    using S = Stage<ORDER, NQ>;
+    constexpr int SE = S::SIZE;
 
     extern __shared__ __align__(16) unsigned char smem[];
     T *acc = reinterpret_cast<T *>(smem + L.acc);
@@ -155,8 +173,9 @@ __global__ __launch_bounds__(NT) void k_egh(DevMesh<T> m, const double *__restri
     const bool producer = wave < WA;
 
     const RGTab<ORDER, NQ> &tb = tab<ORDER, NQ>();
-    for (int i = tid; i < NQ * NB * 3; i += NT) rg[i] = T(tb.rg[i]);
-    for (int i = tid; i < NB * NB * 6; i += NT) rgg[i] = T(tb.rgg[i]);
+    // reference-gradient tables padded to 4 and 8 entries for vector loads
+    for (int i = tid; i < NQ * NB * 4; i += NT) rg[i] = (i & 3) == 3 ? T(0) : T(tb.rg[(i >> 2) * 3 + (i & 3)]);
+    for (int i = tid; i < NB * NB * 8; i += NT) rgg[i] = (i & 7) >= 6 ? T(0) : T(tb.rgg[(i >> 3) * 6 + (i & 7)]);
 
     // per-patch bookkeeping (uniform)
     auto desc = [&](int p, int f) { return fd.desc[int64_t(p) * PD_STRIDE + f]; };
@@ -218,7 +237,6 @@ __global__ __launch_bounds__(NT) void k_egh(DevMesh<T> m, const double *__restri
     const int gi = lane / NB, b = lane - gi * NB;
     const bool in_group = !producer && gi < GPW;
     const int group = cw * GPW + gi;
-    T rgb[NQ][3];
     auto consume = [&](int slot) {
         const T *stg = reinterpret_cast<const T *>(smem + (slot ? L.stg1 : L.stg0));
         const int *lhdr = reinterpret_cast<const int *>(smem + (slot ? L.hdr1 : L.hdr0));
@@ -230,39 +248,52 @@ __global__ __launch_bounds__(NT) void k_egh(DevMesh<T> m, const double *__restri
             const int hdr = lhdr[it];
             const int kk = hdr & 1023, a = (hdr >> 10) & 15, r = hdr >> 14;
             const T *s = stg + kk * SE;
+            const int pos = lpos[it * NB + b];
             if (b < 3) {
+                // gradient entry (row a, component d = b)
                 T gs = T(0);
 #pragma unroll
-                for (int q = 0; q < NQ; ++q)
-#pragma unroll
-                    for (int k = 0; k < 3; ++k) gs += rg[(q * NB + a) * 3 + k] * s[q * SQ + 11 + b * 3 + k];
+                for (int q = 0; q < NQ; ++q) {
+                    const Vec4<T> g = ld4(rg + (q * NB + a) * 4);
+                    const T *qd = s + S::Q + q * 9 + b * 3;
+                    gs += g.x * qd[0] + g.y * qd[1] + g.z * qd[2];
+                }
                 gacc[3 * r + b] += gs;
             }
-            const int pos = lpos[it * NB + b];
+            T cl[NQ], cc[NQ];
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) {
+                cl[q] = s[S::C + q];
+                cc[q] = s[S::C + NQ + q];
+            }
             T blk[9];
 #pragma unroll
             for (int i = 0; i < 9; ++i) blk[i] = T(0);
 #pragma unroll
             for (int q = 0; q < NQ; ++q) {
-                const T *sq = s + q * SQ;
-                T kb[3], ka[3];
+                const T *kbp = s + S::K + (q * NB + b) * 3, *kap = s + S::K + (q * NB + a) * 3;
+                const T kb[3] = {kbp[0], kbp[1], kbp[2]};
+                const T ka[3] = {kap[0], kap[1], kap[2]};
+                T u[3], v[3];
+#pragma unroll
+                for (int mm = 0; mm < 3; ++mm) {
+                    u[mm] = cl[q] * ka[mm];
+                    v[mm] = cc[q] * ka[mm];
+                }
 #pragma unroll
                 for (int mm = 0; mm < 3; ++mm)
-                    kb[mm] = sq[mm * 3 + 0] * rgb[q][0] + sq[mm * 3 + 1] * rgb[q][1] + sq[mm * 3 + 2] * rgb[q][2];
 #pragma unroll
-                for (int mm = 0; mm < 3; ++mm) ka[mm] = __shfl(kb[mm], gi * NB + a);
-                const T cl = sq[9], cc = sq[10];
-#pragma unroll
-                for (int mm = 0; mm < 3; ++mm)
-#pragma unroll
-                    for (int nn = 0; nn < 3; ++nn) blk[mm * 3 + nn] += cl * ka[mm] * kb[nn] + cc * kb[mm] * ka[nn];
+                    for (int nn = 0; nn < 3; ++nn) blk[mm * 3 + nn] += u[mm] * kb[nn] + kb[mm] * v[nn];
             }
-            const T *g6 = s + NQ * SQ;
-            const T *cgf = rgg + (a * NB + b) * 6;
-            const T sab = g6[0] * cgf[0] + g6[1] * cgf[1] + g6[2] * cgf[2] + g6[3] * cgf[3] + g6[4] * cgf[4] + g6[5] * cgf[5];
-            blk[0] += sab;
-            blk[4] += sab;
-            blk[8] += sab;
+            {
+                const Vec4<T> g0 = ld4(s + S::G), c0 = ld4(rgg + (a * NB + b) * 8);
+                const T g4 = s[S::G + 4], g5 = s[S::G + 5];
+                const T c4 = rgg[(a * NB + b) * 8 + 4], c5 = rgg[(a * NB + b) * 8 + 5];
+                const T sab = g0.x * c0.x + g0.y * c0.y + g0.z * c0.z + g0.w * c0.w + g4 * c4 + g5 * c5;
+                blk[0] += sab;
+                blk[4] += sab;
+                blk[8] += sab;
+            }
             const int deg = rdeg[r];
             T *ar = acc + racc[r] + 3 * pos;
 #pragma unroll
@@ -276,10 +307,6 @@ __global__ __launch_bounds__(NT) void k_egh(DevMesh<T> m, const double *__restri
     if (cur.p >= n_patches) return; // uniform
     begin_patch(cur.p);
     __syncthreads(); // rg tables
-#pragma unroll
-    for (int q = 0; q < NQ; ++q)
-#pragma unroll
-        for (int k = 0; k < 3; ++k) rgb[q][k] = in_group ? rg[(q * NB + b) * 3 + k] : T(0);
     if (producer) produce(cur, 0);
     __syncthreads();
     if (tid == 0) {

@@ -11,9 +11,20 @@ namespace pfgpu {
 constexpr int NT = 1024;            // workgroup size
 constexpr int NWAVE = NT / 64;
 constexpr int LDS_LIMIT = 160 * 1024;
-constexpr int SQ = 20;              // staging per (element, q): M[9], cl, cc, Q[9]
-
-template <int NQ> constexpr int stage_stride() { return SQ * NQ + 8; } // + metric[6] + pad
+// Staging record of one element (value type T, 16-byte aligned sections):
+//   K[q][b][3]   k_b = F^{-T} G_b at point q (J^{-T} G_b for linear elasticity)
+//   C[2 NQ]      cl[q] = w lambda, cc[q] = w (mu - lambda ln J_q)
+//   Q[q][d][3]   w P_q J^{-1}: gradient row of node a = sum_q sum_k RG(q,a,k) Q[q][d][k]
+//   G[6]         w mu J^{-T}J^{-1} (00 11 22 01 02 12)
+constexpr int align4(int v) { return (v + 3) & ~3; }
+template <int ORDER, int NQ> struct Stage {
+    static constexpr int NB = NLocal<ORDER>::value;
+    static constexpr int K = 0;
+    static constexpr int C = align4(K + NQ * NB * 3);
+    static constexpr int Q = align4(C + 2 * NQ);
+    static constexpr int G = align4(Q + 9 * NQ);
+    static constexpr int SIZE = align4(G + 6);
+};
 
 // elements per chunk: one producer thread per (element, q), 4 producer waves
 // in FP32 and 2 in FP64 (whose staging is twice as large)

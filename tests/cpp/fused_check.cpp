@@ -70,11 +70,12 @@ static void geometry(const Mesh &m, int e, double *jit, double &det)
     for (int k = 0; k < 9; ++k) jit[k] = J[k] * id;
 }
 
-// the kernel's per-point staging (NeoHookean): M, cl, cc, Q; metric on q == 0
+// the kernel's per-point staging record (NeoHookean), layout Stage<ORDER, NQ>
 template <int ORDER, int NQ>
 static void stage_point(const Mesh &m, int e, int q, double *s)
 {
-    constexpr int NB = NLocal<ORDER>::value, SQ = 20;
+    using S = Stage<ORDER, NQ>;
+    constexpr int NB = NLocal<ORDER>::value;
     double jit[9], det;
     geometry(m, e, jit, det);
     const double w = det * quad_weight(NQ), lam = m.lam[e], mu = m.mu[e];
@@ -95,19 +96,48 @@ static void stage_point(const Mesh &m, int e, int q, double *s)
         FmT[i] = cof[i] / J;
         P[i] = mu * (F[i] - FmT[i]) + lam * lj * FmT[i];
     }
-    double *sq = s + q * SQ;
+    double M[9];
     for (int mm = 0; mm < 3; ++mm)
         for (int kk = 0; kk < 3; ++kk)
-            sq[mm * 3 + kk] = FmT[mm * 3] * jit[kk * 3] + FmT[mm * 3 + 1] * jit[kk * 3 + 1] + FmT[mm * 3 + 2] * jit[kk * 3 + 2];
-    sq[9] = w * lam;
-    sq[10] = w * (mu - lam * lj);
+            M[mm * 3 + kk] = FmT[mm * 3] * jit[kk * 3] + FmT[mm * 3 + 1] * jit[kk * 3 + 1] + FmT[mm * 3 + 2] * jit[kk * 3 + 2];
+    for (int b = 0; b < NB; ++b)
+        for (int mm = 0; mm < 3; ++mm) {
+            double v = 0;
+            for (int k = 0; k < 3; ++k) v += M[mm * 3 + k] * RG<ORDER, NQ>(q, b, k);
+            s[S::K + (q * NB + b) * 3 + mm] = v;
+        }
+    s[S::C + q] = w * lam;
+    s[S::C + NQ + q] = w * (mu - lam * lj);
     for (int d = 0; d < 3; ++d)
         for (int kk = 0; kk < 3; ++kk)
-            sq[11 + d * 3 + kk] = w * (P[d * 3] * jit[kk * 3] + P[d * 3 + 1] * jit[kk * 3 + 1] + P[d * 3 + 2] * jit[kk * 3 + 2]);
+            s[S::Q + q * 9 + d * 3 + kk] = w * (P[d * 3] * jit[kk * 3] + P[d * 3 + 1] * jit[kk * 3 + 1] + P[d * 3 + 2] * jit[kk * 3 + 2]);
     if (q == 0) {
         auto dot = [&](int k, int l) { return jit[k * 3] * jit[l * 3] + jit[k * 3 + 1] * jit[l * 3 + 1] + jit[k * 3 + 2] * jit[l * 3 + 2]; };
         const double g6[6] = {dot(0, 0), dot(1, 1), dot(2, 2), dot(0, 1), dot(0, 2), dot(1, 2)};
-        for (int i = 0; i < 6; ++i) s[NQ * SQ + i] = w * mu * g6[i];
+        for (int i = 0; i < 6; ++i) s[S::G + i] = w * mu * g6[i];
+    }
+}
+
+// block (a, b) and gradient row a of one staged element, as the kernel's consumer lanes form them
+template <int ORDER, int NQ>
+static void consume_item(const double *sp, int a, int b, double *blk, double *grow)
+{
+    using S = Stage<ORDER, NQ>;
+    constexpr int NB = NLocal<ORDER>::value;
+    for (int i = 0; i < 9; ++i) blk[i] = 0;
+    for (int q = 0; q < NQ; ++q) {
+        const double *ka = sp + S::K + (q * NB + a) * 3, *kb = sp + S::K + (q * NB + b) * 3;
+        const double cl = sp[S::C + q], cc = sp[S::C + NQ + q];
+        for (int mm = 0; mm < 3; ++mm)
+            for (int nn = 0; nn < 3; ++nn) blk[mm * 3 + nn] += cl * ka[mm] * kb[nn] + cc * kb[mm] * ka[nn];
+    }
+    const double sab = grad_dot<ORDER, NQ>(a, b, sp + S::G);
+    for (int d = 0; d < 3; ++d) blk[d * 4] += sab;
+    for (int d = 0; d < 3; ++d) {
+        double gs = 0;
+        for (int q = 0; q < NQ; ++q)
+            for (int k = 0; k < 3; ++k) gs += RG<ORDER, NQ>(q, a, k) * sp[S::Q + q * 9 + d * 3 + k];
+        grow[d] = gs;
     }
 }
 
@@ -115,42 +145,24 @@ static void stage_point(const Mesh &m, int e, int q, double *s)
 template <int ORDER, int NQ>
 static void direct(const Mesh &m, const Pattern &pat, std::vector<double> &vals, std::vector<double> &g)
 {
-    constexpr int NB = NLocal<ORDER>::value, SQ = 20;
+    constexpr int NB = NLocal<ORDER>::value;
     vals.assign(size_t(pat.nnz), 0.0);
     g.assign(size_t(m.N) * 3, 0.0);
-    std::vector<double> s(SQ * NQ + 8);
+    std::vector<double> s(Stage<ORDER, NQ>::SIZE);
     for (int e = 0; e < m.E; ++e) {
         for (int q = 0; q < NQ; ++q) stage_point<ORDER, NQ>(m, e, q, s.data());
         for (int a = 0; a < NB; ++a) {
             const int i = m.en[e * NB + a];
-            for (int d = 0; d < 3; ++d) {
-                double gs = 0;
-                for (int q = 0; q < NQ; ++q)
-                    for (int k = 0; k < 3; ++k) gs += RG<ORDER, NQ>(q, a, k) * s[q * SQ + 11 + d * 3 + k];
-                g[3 * size_t(i) + d] += gs;
-            }
+            const int deg = pat.brow_ptr[i + 1] - pat.brow_ptr[i];
             for (int b = 0; b < NB; ++b) {
                 const int j = m.en[e * NB + b];
-                double blk[9] = {0};
-                for (int q = 0; q < NQ; ++q) {
-                    double ka[3], kb[3];
-                    for (int mm = 0; mm < 3; ++mm) {
-                        ka[mm] = kb[mm] = 0;
-                        for (int k = 0; k < 3; ++k) {
-                            ka[mm] += s[q * SQ + mm * 3 + k] * RG<ORDER, NQ>(q, a, k);
-                            kb[mm] += s[q * SQ + mm * 3 + k] * RG<ORDER, NQ>(q, b, k);
-                        }
-                    }
-                    for (int mm = 0; mm < 3; ++mm)
-                        for (int nn = 0; nn < 3; ++nn)
-                            blk[mm * 3 + nn] += s[q * SQ + 9] * ka[mm] * kb[nn] + s[q * SQ + 10] * kb[mm] * ka[nn];
-                }
-                const double sab = grad_dot<ORDER, NQ>(a, b, &s[NQ * SQ]);
-                for (int d = 0; d < 3; ++d) blk[d * 4] += sab;
+                double blk[9], grow[3];
+                consume_item<ORDER, NQ>(s.data(), a, b, blk, grow);
+                if (b == 0)
+                    for (int d = 0; d < 3; ++d) g[3 * size_t(i) + d] += grow[d];
                 int pos = -1;
                 for (int k = pat.brow_ptr[i]; k < pat.brow_ptr[i + 1]; ++k)
                     if (pat.bcol[k] == j) pos = k - pat.brow_ptr[i];
-                const int deg = pat.brow_ptr[i + 1] - pat.brow_ptr[i];
                 for (int mm = 0; mm < 3; ++mm)
                     for (int nn = 0; nn < 3; ++nn) vals[9 * size_t(pat.brow_ptr[i]) + mm * 3 * deg + 3 * pos + nn] += blk[mm * 3 + nn];
             }
@@ -161,7 +173,7 @@ static void direct(const Mesh &m, const Pattern &pat, std::vector<double> &vals,
 template <int ORDER, int NQ>
 static int run(const Mesh &m, int groups, int chunk, int item_cap, int acc_capacity, int max_rows)
 {
-    constexpr int NB = NLocal<ORDER>::value, SQ = 20, SE = SQ * NQ + 8;
+    constexpr int NB = NLocal<ORDER>::value, SE = Stage<ORDER, NQ>::SIZE;
     Pattern pat = build_pattern(m.E, NB, m.en.data(), m.N, m.Nv, m.ev.data());
     FusedShape s{groups, chunk, item_cap, acc_capacity, max_rows};
     FusedTables t = build_fused_tables(pat, NB, m.en.data(), m.E, m.vx.data(), m.ev.data(), s);
@@ -220,36 +232,21 @@ static int run(const Mesh &m, int groups, int chunk, int item_cap, int acc_capac
                     for (int b = 0; b < NB; ++b) {
                         const int pos = t.pos[size_t(it) * NB + b];
                         CHECK(pos < deg && pat.bcol[pat.brow_ptr[node] + pos] == m.en[e * NB + b], "column position");
-                        double blk[9] = {0};
-                        for (int q = 0; q < NQ; ++q) {
-                            const double *sq = sp + q * SQ;
-                            double ka[3], kb[3];
-                            for (int mm = 0; mm < 3; ++mm) {
-                                ka[mm] = kb[mm] = 0;
-                                for (int k = 0; k < 3; ++k) {
-                                    ka[mm] += sq[mm * 3 + k] * RG<ORDER, NQ>(q, a, k);
-                                    kb[mm] += sq[mm * 3 + k] * RG<ORDER, NQ>(q, b, k);
-                                }
-                            }
-                            for (int mm = 0; mm < 3; ++mm)
-                                for (int nn = 0; nn < 3; ++nn) blk[mm * 3 + nn] += sq[9] * ka[mm] * kb[nn] + sq[10] * kb[mm] * ka[nn];
-                        }
-                        const double sab = grad_dot<ORDER, NQ>(a, b, sp + NQ * SQ);
-                        for (int d = 0; d < 3; ++d) blk[d * 4] += sab;
+                        double blk[9], grow[3];
+                        consume_item<ORDER, NQ>(sp, a, b, blk, grow);
                         for (int mm = 0; mm < 3; ++mm)
                             for (int nn = 0; nn < 3; ++nn) {
                                 const int ai = t.row_acc[row0 + r] + 3 * pos + mm * 3 * deg + nn;
                                 CHECK(ai >= t.row_acc[row0 + r] && ai < t.row_acc[row0 + r] + 9 * deg, "acc index in row segment");
                                 acc[ai] += blk[mm * 3 + nn];
                             }
-                        if (b < 3) {
-                            double gs = 0;
-                            for (int q = 0; q < NQ; ++q)
-                                for (int k = 0; k < 3; ++k) gs += RG<ORDER, NQ>(q, a, k) * sp[q * SQ + 11 + b * 3 + k];
-                            gacc[3 * size_t(r) + b] += gs;
-                        }
+                        if (b < 3) gacc[3 * size_t(r) + b] += grow[b];
                     }
-                    for (int i = 0; i < SE - 2; ++i) CHECK(!std::isnan(sp[i]) || (i >= NQ * SQ + 6), "staging slot read before written");
+                    for (int i = 0; i < Stage<ORDER, NQ>::G + 6; ++i)
+                        CHECK(!std::isnan(sp[i]) || (i >= Stage<ORDER, NQ>::C + 2 * NQ && i < Stage<ORDER, NQ>::Q) ||
+                                  (i >= Stage<ORDER, NQ>::Q + 9 * NQ && i < Stage<ORDER, NQ>::G) ||
+                                  (i >= NQ * NB * 3 && i < Stage<ORDER, NQ>::C),
+                              "staging slot read before written");
                 }
             }
         }
