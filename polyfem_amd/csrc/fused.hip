@@ -131,6 +131,9 @@ __device__ __forceinline__ void setup_point(const DevMesh<T> &m, int e, const do
     }
 }
 
+__device__ __forceinline__ float fmat(float a, float b, float c) { return fmaf(a, b, c); }
+__device__ __forceinline__ double fmat(double a, double b, double c) { return fma(a, b, c); }
+
 template <typename T> struct Vec4 {
     T x, y, z, w;
 };
@@ -256,7 +259,7 @@ __global__ __launch_bounds__(NT) void k_egh(DevMesh<T> m, const double *__restri
                 for (int q = 0; q < NQ; ++q) {
                     const Vec4<T> g = ld4(rg + (q * NB + a) * 4);
                     const T *qd = s + S::Q + q * 9 + b * 3;
-                    gs += g.x * qd[0] + g.y * qd[1] + g.z * qd[2];
+                    gs = fmat(g.x, qd[0], fmat(g.y, qd[1], fmat(g.z, qd[2], gs)));
                 }
                 gacc[3 * r + b] += gs;
             }
@@ -283,13 +286,13 @@ __global__ __launch_bounds__(NT) void k_egh(DevMesh<T> m, const double *__restri
 #pragma unroll
                 for (int mm = 0; mm < 3; ++mm)
 #pragma unroll
-                    for (int nn = 0; nn < 3; ++nn) blk[mm * 3 + nn] += u[mm] * kb[nn] + kb[mm] * v[nn];
+                    for (int nn = 0; nn < 3; ++nn) blk[mm * 3 + nn] = fmat(kb[mm], v[nn], fmat(u[mm], kb[nn], blk[mm * 3 + nn]));
             }
             {
                 const Vec4<T> g0 = ld4(s + S::G), c0 = ld4(rgg + (a * NB + b) * 8);
                 const T g4 = s[S::G + 4], g5 = s[S::G + 5];
                 const T c4 = rgg[(a * NB + b) * 8 + 4], c5 = rgg[(a * NB + b) * 8 + 5];
-                const T sab = g0.x * c0.x + g0.y * c0.y + g0.z * c0.z + g0.w * c0.w + g4 * c4 + g5 * c5;
+                const T sab = fmat(g0.x, c0.x, fmat(g0.y, c0.y, fmat(g0.z, c0.z, fmat(g0.w, c0.w, fmat(g4, c4, g5 * c5)))));
                 blk[0] += sab;
                 blk[4] += sab;
                 blk[8] += sab;

@@ -164,18 +164,7 @@ FusedTables build_fused_tables(const Pattern &pat, int n_local, const int32_t *e
         }
         L.chunk_start.push_back(nel);
         L.n_chunks = int(L.chunk_start.size()) - 1;
-        // rows -> groups, longest first onto the least loaded group
-        std::vector<int> items_of_row(R), ridx(R);
-        for (int r = 0; r < R; ++r) items_of_row[r] = pat.node_ptr[L.rows[r] + 1] - pat.node_ptr[L.rows[r]];
-        std::iota(ridx.begin(), ridx.end(), 0);
-        std::stable_sort(ridx.begin(), ridx.end(), [&](int a, int b) { return items_of_row[a] > items_of_row[b]; });
-        std::vector<int> load(G, 0), group_of(R);
-        for (int r : ridx) {
-            const int g = int(std::min_element(load.begin(), load.end()) - load.begin());
-            group_of[r] = g;
-            load[g] += items_of_row[r];
-        }
-        // items (row, element): bucketed by (chunk, group), element order inside
+        // items (row, element), each tagged with its chunk
         struct Item { int chunk, group, elem_k, a, r; };
         std::vector<Item> items;
         for (int r = 0; r < R; ++r) {
@@ -183,8 +172,36 @@ FusedTables build_fused_tables(const Pattern &pat, int n_local, const int32_t *e
             for (int k = pat.node_ptr[node]; k < pat.node_ptr[node + 1]; ++k) {
                 const int e = pat.node_inc[k] / n_local, a = pat.node_inc[k] % n_local;
                 const int ek = int(std::lower_bound(el.begin(), el.end(), e) - el.begin());
-                items.push_back({chunk_of[ek], group_of[r], ek, a, r});
+                items.push_back({chunk_of[ek], -1, ek, a, r});
             }
+        }
+        // Per chunk, rows -> groups: a row needs one owner only within a chunk
+        // (chunks are separated by barriers).  Longest row first onto the least
+        // loaded group, then groups renumbered by decreasing load so that the
+        // groups sharing a wavefront carry similar loads.
+        {
+            std::vector<std::vector<int>> row_items_in_chunk(L.n_chunks, std::vector<int>(R, 0));
+            for (const Item &it : items) row_items_in_chunk[it.chunk][it.r]++;
+            std::vector<std::vector<int>> group_of(L.n_chunks, std::vector<int>(R, -1));
+            for (int c = 0; c < L.n_chunks; ++c) {
+                const std::vector<int> &cnt = row_items_in_chunk[c];
+                std::vector<int> ridx;
+                for (int r = 0; r < R; ++r)
+                    if (cnt[r] > 0) ridx.push_back(r);
+                std::stable_sort(ridx.begin(), ridx.end(), [&](int a, int b) { return cnt[a] > cnt[b]; });
+                std::vector<int> load(G, 0);
+                for (int r : ridx) {
+                    const int g = int(std::min_element(load.begin(), load.end()) - load.begin());
+                    group_of[c][r] = g;
+                    load[g] += cnt[r];
+                }
+                std::vector<int> gorder(G), rank(G);
+                std::iota(gorder.begin(), gorder.end(), 0);
+                std::stable_sort(gorder.begin(), gorder.end(), [&](int a, int b) { return load[a] > load[b]; });
+                for (int k = 0; k < G; ++k) rank[gorder[k]] = k;
+                for (int r : ridx) group_of[c][r] = rank[group_of[c][r]];
+            }
+            for (Item &it : items) it.group = group_of[it.chunk][it.r];
         }
         std::stable_sort(items.begin(), items.end(), [](const Item &x, const Item &y) {
             if (x.chunk != y.chunk) return x.chunk < y.chunk;

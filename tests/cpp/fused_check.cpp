@@ -216,6 +216,16 @@ static int run(const Mesh &m, int groups, int chunk, int item_cap, int acc_capac
                 CHECK(e >= 0 && e < m.E, "element id");
                 for (int q = 0; q < NQ; ++q) stage_point<ORDER, NQ>(m, e, q, &stg[size_t(kk) * SE]);
             }
+            // within a chunk every row is accumulated by exactly one lane group
+            std::vector<int> row_group(nrows, -1);
+            for (int grp = 0; grp < G; ++grp)
+                for (int it = t.cg[cg0 + c * G + grp]; it < t.cg[cg0 + c * G + grp + 1]; ++it) {
+                    const int r = t.hdr[it] >> 14;
+                    if (r < nrows) {
+                        CHECK(row_group[r] == -1 || row_group[r] == grp, "row %d in two groups of one chunk", r);
+                        row_group[r] = grp;
+                    }
+                }
             for (int grp = 0; grp < G; ++grp) {
                 const int it0 = t.cg[cg0 + c * G + grp], it1 = t.cg[cg0 + c * G + grp + 1];
                 CHECK(it0 <= it1 && it1 <= int(t.hdr.size()), "item range");
