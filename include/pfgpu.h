@@ -150,6 +150,85 @@ pfgpu_status pfgpu_shape_vjp_dev(pfgpu_handle *h, const double *x, const double 
 pfgpu_status pfgpu_hessian_pattern_dev(const pfgpu_handle *h, const int32_t **outer, const int32_t **inner);
 pfgpu_status pfgpu_synchronize(pfgpu_handle *h);
 
+/* ---- solves on the free dofs (callers of the assembly) ------------------ */
+
+/* Newton-CG minimisation of the elastic energy over the free dofs, the
+ * reference's nonlinear solve of the elastic form: Dirichlet dofs removed
+ * from the unknowns (NLProblem::full_to_reduced / reduced_to_full,
+ * NLProblem.cpp:583-629; BCLagrangianForm::project_gradient / hessian,
+ * BCLagrangianForm.cpp:149-200), Newton direction H_ff d = -g_f from a
+ * matrix-free CG on the Hessian-vector product (polysolve Newton with a CG
+ * linear solver), backtracking Armijo line search on the energy, with
+ * non-finite energies (inverted elements) rejected as invalid steps
+ * (ElasticForm::is_step_valid, ElasticForm.cpp:388-405).  Every vector, dot
+ * product and energy difference is FP64 with a fixed reduction order. */
+typedef struct {
+    int32_t max_iterations;    /* Newton iterations */
+    double grad_norm;          /* stop when ||g_free||_2 <= grad_norm */
+    int32_t cg_max_iterations; /* per Newton step */
+    double cg_tolerance;       /* stop CG when ||r|| <= cg_tolerance ||g_free|| */
+    int32_t line_search;       /* 0: full steps (alpha = 1), 1: backtracking Armijo */
+    double armijo_c;           /* sufficient-decrease constant */
+    int32_t max_line_search;   /* step halvings before giving up */
+    int32_t use_graph;         /* 1: CG iterations replayed from a captured hipGraph */
+} pfgpu_newton_params;
+
+typedef struct {
+    int32_t iterations, cg_iterations, line_search_halvings, converged;
+    double energy, grad_norm, last_alpha;
+} pfgpu_newton_report;
+
+/* x: n_dofs values, the initial guess with the Dirichlet values already on
+ * the fixed dofs, overwritten by the result; fixed: n_dofs bytes, 1 = fixed */
+pfgpu_status pfgpu_newton_solve(pfgpu_handle *h, double *x, const uint8_t *fixed, const pfgpu_newton_params *params,
+                                pfgpu_newton_report *report);
+pfgpu_status pfgpu_newton_solve_dev(pfgpu_handle *h, double *x, const uint8_t *fixed,
+                                    const pfgpu_newton_params *params, pfgpu_newton_report *report);
+/* CG solve of H(x)_ff y_f = b_f with y = 0 on fixed dofs, matrix-free; the
+ * static adjoint solve (solve_static_adjoint, StateDiff.cpp:230-318; the
+ * Hessian is symmetric, so H^T = H).  Stops at ||r|| <= tolerance ||b_f||. */
+pfgpu_status pfgpu_cg_solve(pfgpu_handle *h, const double *x, const uint8_t *fixed, const double *b, double *y,
+                            int32_t max_iterations, double tolerance, int32_t *iterations, double *rel_residual);
+pfgpu_status pfgpu_cg_solve_dev(pfgpu_handle *h, const double *x, const uint8_t *fixed, const double *b, double *y,
+                                int32_t max_iterations, double tolerance, int32_t use_graph, int32_t *iterations,
+                                double *rel_residual);
+/* external load f (n_dofs, host pointer; NULL removes it): the Newton solve
+ * then minimises E(x) - f^T x (a body force such as gravity, assembled by
+ * the caller, e.g. f = M g with pfgpu_mass_matvec) */
+pfgpu_status pfgpu_set_load(pfgpu_handle *h, const double *f);
+
+/* ---- transient: mass, implicit Euler, checkpointed adjoint --------------- */
+
+/* consistent mass matrix M_ab = rho int phi_a phi_b (block = M_ab I_3), the
+ * reference's Mass assembler (assembler/Mass.cpp:4-20) integrated exactly;
+ * rho: per-element density (n_elements) or NULL for rho_uniform */
+pfgpu_status pfgpu_set_density(pfgpu_handle *h, const double *rho, double rho_uniform);
+/* y = M v */
+pfgpu_status pfgpu_mass_matvec(pfgpu_handle *h, const double *v, double *y);
+pfgpu_status pfgpu_mass_matvec_dev(pfgpu_handle *h, const double *v, double *y);
+
+typedef struct {
+    double dt;                /* time step */
+    int32_t time_steps;       /* T */
+    int32_t checkpoint_every; /* states kept every this many steps (1: all) */
+} pfgpu_transient_params;
+
+/* Implicit Euler (ImplicitEuler.cpp:13-31; InertiaForm.cpp:17-35): step i
+ * minimises 1/2 (x - xt)^T M (x - xt) + dt^2 (E(x) - f^T x) over the free
+ * dofs with xt = x_{i-1} + dt v_{i-1}, by the Newton solve above; fixed dofs
+ * keep their x0 values.  Host pointers; f may be NULL; xT, vT receive the
+ * final state.  The handle keeps the checkpoints for pfgpu_transient_adjoint. */
+pfgpu_status pfgpu_transient_forward(pfgpu_handle *h, const double *x0, const double *v0, const uint8_t *fixed,
+                                     const double *f, const pfgpu_transient_params *params,
+                                     const pfgpu_newton_params *newton, double *xT, double *vT);
+/* Gradient of J(x_T, v_T) through the last forward run, from dJ/dx_T and
+ * dJ/dv_T (either may be NULL): dtheta (2 n_elements, lambda block then mu
+ * block), dx0 and dv0 (n_dofs, zero on fixed dofs); any output may be NULL.
+ * The backward recurrence of solve_transient_adjoint (StateDiff.cpp:320-413)
+ * for BDF-1, recomputing the states between checkpoints. */
+pfgpu_status pfgpu_transient_adjoint(pfgpu_handle *h, const double *dJdx, const double *dJdv, int32_t cg_max_iterations,
+                                     double cg_tolerance, double *dtheta, double *dx0, double *dv0);
+
 #ifdef __cplusplus
 }
 #endif

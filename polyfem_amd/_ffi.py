@@ -35,6 +35,22 @@ class Info(C.Structure):
                 ("n_vertices", C.c_int32), ("n_dofs", C.c_int32), ("nnz", C.c_int64), ("device_bytes", C.c_int64)]
 
 
+class NewtonParams(C.Structure):
+    _fields_ = [("max_iterations", C.c_int32), ("grad_norm", C.c_double), ("cg_max_iterations", C.c_int32),
+                ("cg_tolerance", C.c_double), ("line_search", C.c_int32), ("armijo_c", C.c_double),
+                ("max_line_search", C.c_int32), ("use_graph", C.c_int32)]
+
+
+class NewtonReport(C.Structure):
+    _fields_ = [("iterations", C.c_int32), ("cg_iterations", C.c_int32), ("line_search_halvings", C.c_int32),
+                ("converged", C.c_int32), ("energy", C.c_double), ("grad_norm", C.c_double),
+                ("last_alpha", C.c_double)]
+
+
+class TransientParams(C.Structure):
+    _fields_ = [("dt", C.c_double), ("time_steps", C.c_int32), ("checkpoint_every", C.c_int32)]
+
+
 class PfgpuError(RuntimeError):
     def __init__(self, status: int, message: str):
         super().__init__(f"pfgpu {STATUS_NAMES.get(status, status)}: {message}")
@@ -69,6 +85,17 @@ _SIGNATURES = {
     "pfgpu_shape_vjp_dev": [vp, vp, vp, vp],
     "pfgpu_hessian_pattern_dev": [vp, vp, vp],
     "pfgpu_synchronize": [vp],
+    "pfgpu_newton_solve": [vp, vp, vp, C.POINTER(NewtonParams), C.POINTER(NewtonReport)],
+    "pfgpu_newton_solve_dev": [vp, vp, vp, C.POINTER(NewtonParams), C.POINTER(NewtonReport)],
+    "pfgpu_cg_solve": [vp, vp, vp, vp, vp, C.c_int32, C.c_double, C.POINTER(C.c_int32), C.POINTER(C.c_double)],
+    "pfgpu_cg_solve_dev": [vp, vp, vp, vp, vp, C.c_int32, C.c_double, C.c_int32, C.POINTER(C.c_int32),
+                           C.POINTER(C.c_double)],
+    "pfgpu_set_load": [vp, vp],
+    "pfgpu_set_density": [vp, vp, C.c_double],
+    "pfgpu_mass_matvec": [vp, vp, vp],
+    "pfgpu_mass_matvec_dev": [vp, vp, vp],
+    "pfgpu_transient_forward": [vp, vp, vp, vp, vp, C.POINTER(TransientParams), C.POINTER(NewtonParams), vp, vp],
+    "pfgpu_transient_adjoint": [vp, vp, vp, C.c_int32, C.c_double, vp, vp, vp],
 }
 
 _lib = None

@@ -141,15 +141,22 @@ void check_finite(pfgpu_handle &h, const void *dev, int64_t n, bool is_float, co
 
 } // namespace
 
+pfgpu_handle::~pfgpu_handle()
+{
+    pfgpu::solver_release(*this);
+    if (own_stream) (void)hipStreamDestroy(own_stream);
+}
+
 int64_t pfgpu_handle::device_bytes() const
 {
     const DevBuf *bufs[] = {&conn, &vconn, &geom, &lame, &node_ptr, &node_inc, &vert_ptr, &vert_inc,
                             &brow_ptr, &block_row, &contrib_ptr, &contrib, &patch_desc, &patch_rows,
-                            &patch_row_acc, &patch_elems, &patch_cs, &patch_cg, &patch_hdr, &patch_pos,
+                            &patch_row_acc, &patch_elems, &patch_cs, &patch_cg, &patch_hdr, &patch_pos, &patch_vnodes,
                             &d_outer, &d_inner, &loc, &hloc, &partial, &red_out,
-                            &h_x, &h_v, &h_out, &h_out2, &h_vals};
+                            &h_x, &h_v, &h_out, &h_out2, &h_vals, &sv, &h_mask, &mass, &pat_bcol, &tr_fixed, &tr_f, &tr_seg, &load};
     int64_t s = 0;
     for (const DevBuf *b : bufs) s += int64_t(b->bytes());
+    for (const DevBuf &b : ckpt) s += int64_t(b.bytes());
     return s;
 }
 
@@ -528,6 +535,221 @@ pfgpu_status pfgpu_synchronize(pfgpu_handle *h)
     return guard([&] {
         check_handle(h);
         PF_HIP(hipStreamSynchronize(h->stream));
+    });
+}
+
+} // extern "C"
+
+namespace {
+
+// Graph capture needs a stream other than the legacy default one: a solve on
+// a handle bound to the default stream runs on the handle's own (blocking)
+// stream, which orders itself after the default stream's earlier work.
+struct SolverStream {
+    pfgpu_handle &h;
+    hipStream_t saved;
+    explicit SolverStream(pfgpu_handle &hh) : h(hh), saved(hh.stream)
+    {
+        if (h.stream == nullptr) {
+            if (!h.own_stream) PF_HIP(hipStreamCreate(&h.own_stream));
+            h.stream = h.own_stream;
+        }
+    }
+    ~SolverStream()
+    {
+        if (saved == nullptr) (void)hipStreamSynchronize(h.stream);
+        h.stream = saved;
+    }
+};
+
+void check_newton_params(const pfgpu_newton_params *p)
+{
+    require(p != nullptr, "null Newton parameters");
+    require(p->max_iterations >= 0 && p->cg_max_iterations >= 0 && p->max_line_search >= 0, "negative iteration count");
+    require(p->grad_norm >= 0 && p->cg_tolerance >= 0 && p->armijo_c >= 0 && p->armijo_c < 1, "invalid tolerance");
+    require(p->line_search == 0 || p->line_search == 1, "line_search must be 0 or 1");
+}
+
+const uint8_t *stage_mask(pfgpu_handle &h, const uint8_t *fixed)
+{
+    if (!fixed) return nullptr;
+    h.h_mask.reserve(size_t(h.n_dofs));
+    PF_HIP(hipMemcpyAsync(h.h_mask.get(), fixed, size_t(h.n_dofs), hipMemcpyHostToDevice, h.stream));
+    return h.h_mask.as<uint8_t>();
+}
+
+} // namespace
+
+extern "C" {
+
+pfgpu_status pfgpu_newton_solve_dev(pfgpu_handle *h, double *x, const uint8_t *fixed, const pfgpu_newton_params *params,
+                                    pfgpu_newton_report *report)
+{
+    return guard([&] {
+        check_handle(h);
+        require(x != nullptr && report != nullptr, "null argument");
+        check_newton_params(params);
+        SolverStream ss(*h);
+        newton_solve(*h, x, fixed, *params, *report);
+    });
+}
+
+pfgpu_status pfgpu_newton_solve(pfgpu_handle *h, double *x, const uint8_t *fixed, const pfgpu_newton_params *params,
+                                pfgpu_newton_report *report)
+{
+    return guard([&] {
+        check_handle(h);
+        require(x != nullptr && report != nullptr, "null argument");
+        check_newton_params(params);
+        SolverStream ss(*h);
+        h->h_out2.reserve(sizeof(double) * size_t(h->n_dofs));
+        double *dx = h->h_out2.as<double>();
+        PF_HIP(hipMemcpyAsync(dx, x, sizeof(double) * size_t(h->n_dofs), hipMemcpyHostToDevice, h->stream));
+        const uint8_t *dm = stage_mask(*h, fixed);
+        newton_solve(*h, dx, dm, *params, *report);
+        fetch(*h, x, dx, sizeof(double) * size_t(h->n_dofs));
+    });
+}
+
+pfgpu_status pfgpu_cg_solve_dev(pfgpu_handle *h, const double *x, const uint8_t *fixed, const double *b, double *y,
+                                int32_t max_iterations, double tolerance, int32_t use_graph, int32_t *iterations,
+                                double *rel_residual)
+{
+    return guard([&] {
+        check_handle(h);
+        require(x && b && y, "null argument");
+        require(max_iterations >= 0 && tolerance >= 0, "invalid CG limits");
+        SolverStream ss(*h);
+        double rel = 0;
+        const int it = cg_solve_ws(*h, x, fixed, b, max_iterations, tolerance, use_graph != 0, &rel);
+        PF_HIP(hipMemcpyAsync(y, cg_solution(*h), sizeof(double) * size_t(h->n_dofs), hipMemcpyDeviceToDevice, h->stream));
+        if (iterations) *iterations = it;
+        if (rel_residual) *rel_residual = rel;
+    });
+}
+
+pfgpu_status pfgpu_set_load(pfgpu_handle *h, const double *f)
+{
+    return guard([&] {
+        check_handle(h);
+        if (!f) {
+            h->has_load = false;
+            h->obj_f = nullptr;
+            h->obj_generation++;
+            return;
+        }
+        for (int i = 0; i < h->n_dofs; ++i) require(std::isfinite(f[i]), "non-finite load");
+        h->load.reserve(sizeof(double) * size_t(h->n_dofs));
+        PF_HIP(hipMemcpyAsync(h->load.get(), f, sizeof(double) * size_t(h->n_dofs), hipMemcpyHostToDevice, h->stream));
+        PF_HIP(hipStreamSynchronize(h->stream));
+        h->has_load = true;
+        h->obj_f = h->load.as<double>();
+        h->obj_generation++;
+    });
+}
+
+pfgpu_status pfgpu_set_density(pfgpu_handle *h, const double *rho, double rho_uniform)
+{
+    return guard([&] {
+        check_handle(h);
+        if (rho)
+            for (int e = 0; e < h->n_elements; ++e) require(std::isfinite(rho[e]) && rho[e] > 0, "density must be positive");
+        else
+            require(std::isfinite(rho_uniform) && rho_uniform > 0, "density must be positive");
+        build_mass(*h, rho, rho_uniform);
+        PF_HIP(hipStreamSynchronize(h->stream));
+    });
+}
+
+pfgpu_status pfgpu_mass_matvec_dev(pfgpu_handle *h, const double *v, double *y)
+{
+    return guard([&] {
+        check_handle(h);
+        require(v && y, "null argument");
+        mass_apply(*h, v, y);
+    });
+}
+
+pfgpu_status pfgpu_mass_matvec(pfgpu_handle *h, const double *v, double *y)
+{
+    return guard([&] {
+        check_handle(h);
+        require(v && y, "null argument");
+        const double *dv = stage(*h, h->h_v, v, h->n_dofs);
+        h->h_out.reserve(sizeof(double) * size_t(h->n_dofs));
+        mass_apply(*h, dv, h->h_out.as<double>());
+        fetch(*h, y, h->h_out.get(), sizeof(double) * size_t(h->n_dofs));
+    });
+}
+
+pfgpu_status pfgpu_transient_forward(pfgpu_handle *h, const double *x0, const double *v0, const uint8_t *fixed,
+                                     const double *f, const pfgpu_transient_params *params,
+                                     const pfgpu_newton_params *newton, double *xT, double *vT)
+{
+    return guard([&] {
+        check_handle(h);
+        require(x0 && v0 && params, "null argument");
+        check_newton_params(newton);
+        require(params->dt > 0 && std::isfinite(params->dt), "dt must be positive");
+        require(params->time_steps >= 1 && params->checkpoint_every >= 1, "time_steps and checkpoint_every must be >= 1");
+        SolverStream ss(*h);
+        const size_t n = size_t(h->n_dofs), bytes = sizeof(double) * n;
+        DevBuf buf(4 * bytes + n);
+        double *dx0 = buf.as<double>(), *dv0 = dx0 + n, *dxT = dv0 + n, *dvT = dxT + n, *df = nullptr;
+        uint8_t *dm = reinterpret_cast<uint8_t *>(dvT + n);
+        PF_HIP(hipMemcpyAsync(dx0, x0, bytes, hipMemcpyHostToDevice, h->stream));
+        PF_HIP(hipMemcpyAsync(dv0, v0, bytes, hipMemcpyHostToDevice, h->stream));
+        if (fixed) PF_HIP(hipMemcpyAsync(dm, fixed, n, hipMemcpyHostToDevice, h->stream));
+        DevBuf fb;
+        if (f) {
+            fb.alloc(bytes);
+            df = fb.as<double>();
+            PF_HIP(hipMemcpyAsync(df, f, bytes, hipMemcpyHostToDevice, h->stream));
+        }
+        transient_forward(*h, dx0, dv0, fixed ? dm : nullptr, df, params->dt, params->time_steps,
+                          params->checkpoint_every, *newton, dxT, dvT);
+        if (xT) PF_HIP(hipMemcpyAsync(xT, dxT, bytes, hipMemcpyDeviceToHost, h->stream));
+        if (vT) PF_HIP(hipMemcpyAsync(vT, dvT, bytes, hipMemcpyDeviceToHost, h->stream));
+        PF_HIP(hipStreamSynchronize(h->stream));
+    });
+}
+
+pfgpu_status pfgpu_transient_adjoint(pfgpu_handle *h, const double *dJdx, const double *dJdv, int32_t cg_max_iterations,
+                                     double cg_tolerance, double *dtheta, double *dx0, double *dv0)
+{
+    return guard([&] {
+        check_handle(h);
+        require(cg_max_iterations > 0 && cg_tolerance >= 0, "invalid CG limits");
+        SolverStream ss(*h);
+        const size_t n = size_t(h->n_dofs), bytes = sizeof(double) * n, E = size_t(h->n_elements);
+        DevBuf buf(4 * bytes + 2 * E * sizeof(double));
+        double *djx = buf.as<double>(), *djv = djx + n, *ox = djv + n, *ov = ox + n, *ot = ov + n;
+        if (dJdx) PF_HIP(hipMemcpyAsync(djx, dJdx, bytes, hipMemcpyHostToDevice, h->stream));
+        if (dJdv) PF_HIP(hipMemcpyAsync(djv, dJdv, bytes, hipMemcpyHostToDevice, h->stream));
+        transient_adjoint(*h, dJdx ? djx : nullptr, dJdv ? djv : nullptr, cg_max_iterations, cg_tolerance, ot, ox, ov);
+        if (dtheta) PF_HIP(hipMemcpyAsync(dtheta, ot, 2 * E * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+        if (dx0) PF_HIP(hipMemcpyAsync(dx0, ox, bytes, hipMemcpyDeviceToHost, h->stream));
+        if (dv0) PF_HIP(hipMemcpyAsync(dv0, ov, bytes, hipMemcpyDeviceToHost, h->stream));
+        PF_HIP(hipStreamSynchronize(h->stream));
+    });
+}
+
+pfgpu_status pfgpu_cg_solve(pfgpu_handle *h, const double *x, const uint8_t *fixed, const double *b, double *y,
+                            int32_t max_iterations, double tolerance, int32_t *iterations, double *rel_residual)
+{
+    return guard([&] {
+        check_handle(h);
+        require(x && b && y, "null argument");
+        require(max_iterations >= 0 && tolerance >= 0, "invalid CG limits");
+        SolverStream ss(*h);
+        const double *dx = stage(*h, h->h_x, x, h->n_dofs);
+        const double *db = stage(*h, h->h_v, b, h->n_dofs);
+        const uint8_t *dm = stage_mask(*h, fixed);
+        double rel = 0;
+        const int it = cg_solve_ws(*h, dx, dm, db, max_iterations, tolerance, true, &rel);
+        fetch(*h, y, cg_solution(*h), sizeof(double) * size_t(h->n_dofs));
+        if (iterations) *iterations = it;
+        if (rel_residual) *rel_residual = rel;
     });
 }
 

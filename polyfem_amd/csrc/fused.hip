@@ -7,16 +7,18 @@
 //            kinematics, the energy density, the stress and the Hessian
 //            factors into an LDS staging buffer, and copies the chunk's
 //            (row, element) items into LDS;
-//   consumers (the other waves): lane groups of n_local lanes, each group
-//            owning whole rows of the patch; for an item (row node = local
-//            node a of element e) lane b computes the 3x3 block H_e(a, b)
-//            and adds it to the row's LDS accumulator at the column
-//            position of node_e(b); lanes 0..2 add the gradient entries.
+//   consumers (the other waves): lane groups of 3 n_local lanes, each group
+//            owning whole rows of the chunk; for an item (row node = local
+//            node a of element e) lane (b, m) computes row m of the 3x3
+//            block H_e(a, b) and adds it to the row's LDS accumulator at the
+//            column position of node_e(b); lanes (0, m) add the gradient.
 // Staging and items are double-buffered, so the loads of chunk u+1 overlap
 // the accumulation of chunk u.  When a patch is complete its rows are
 // streamed to HBM, each as one contiguous segment.  Nothing is atomic: a row
 // has exactly one writer and its contributions arrive in element order, so
 // results are bitwise reproducible.
+#include <type_traits>
+
 #include "dispatch.hpp"
 #include "fused.hpp"
 
@@ -55,6 +57,9 @@ template <int ORDER, int NQ> __device__ __forceinline__ const RGTab<ORDER, NQ> &
     else return c_tab24;
 }
 
+__device__ __forceinline__ float fmat(float a, float b, float c) { return fmaf(a, b, c); }
+__device__ __forceinline__ double fmat(double a, double b, double c) { return fma(a, b, c); }
+
 // byte offsets of the LDS regions
 struct Lds {
     int acc, gacc, stg0, stg1, hdr0, hdr1, pos0, pos1, cg0, cg1, rg, rgg, racc, rdeg, red, total;
@@ -62,8 +67,8 @@ struct Lds {
 
 template <typename T, int ORDER, int NQ> Lds lds_layout(int max_acc, int max_rows)
 {
-    constexpr int NB = NLocal<ORDER>::value;
-    constexpr int CH = chunk_elems<T, NQ>(), IC = item_cap<T, ORDER>(), NG = n_groups<T, ORDER, NQ>();
+    constexpr int NB = NLocal<ORDER>::value, PS = pos_stride(NB);
+    constexpr int CH = chunk_elems<T, NQ>(), IC = item_cap<T, ORDER, NQ>(), NG = n_groups<T, ORDER, NQ>();
     auto al = [](int v) { return (v + 15) & ~15; };
     Lds L{};
     int off = 0;
@@ -74,12 +79,12 @@ template <typename T, int ORDER, int NQ> Lds lds_layout(int max_acc, int max_row
     };
     L.acc = take(max_acc * int(sizeof(T)));
     L.gacc = take(3 * max_rows * int(sizeof(T)));
-    L.stg0 = take(CH * Stage<ORDER, NQ>::SIZE * int(sizeof(T)));
-    L.stg1 = take(CH * Stage<ORDER, NQ>::SIZE * int(sizeof(T)));
+    L.stg0 = take(CH * Stage<T, ORDER, NQ>::SIZE * int(sizeof(T)));
+    L.stg1 = take(CH * Stage<T, ORDER, NQ>::SIZE * int(sizeof(T)));
     L.hdr0 = take(IC * 4);
     L.hdr1 = take(IC * 4);
-    L.pos0 = take(IC * NB);
-    L.pos1 = take(IC * NB);
+    L.pos0 = take(IC * PS);
+    L.pos1 = take(IC * PS);
     L.cg0 = take((NG + 1) * 4);
     L.cg1 = take((NG + 1) * 4);
     L.rg = take(NQ * NB * 4 * int(sizeof(T)));
@@ -91,17 +96,22 @@ template <typename T, int ORDER, int NQ> Lds lds_layout(int max_acc, int max_row
     return L;
 }
 
-// producer work for one (element, quadrature point Q): fills the staging
-// record s of the element (layout Stage<ORDER, NQ>)
+// producer work for one (element, quadrature point Q): fills the point's
+// parts of the staging record s of the element (layout Stage<T, ORDER, NQ>)
+// and returns the point's contribution to the element gradient in gq
 template <typename T, int ORDER, int NQ, int MAT, int Q>
-__device__ __forceinline__ void setup_point(const DevMesh<T> &m, int e, const double *__restrict__ x, T *__restrict__ s,
-                                            bool own, double &eacc)
+__device__ __forceinline__ void setup_point(const DevMesh<T> &m, int e, const int32_t (&nodes)[NLocal<ORDER>::value],
+                                            const double *__restrict__ x, T *__restrict__ s, bool own, double &eacc,
+                                            T (&gq)[NLocal<ORDER>::value][3])
 {
-    using S = Stage<ORDER, NQ>;
+    using S = Stage<T, ORDER, NQ>;
     constexpr int NB = NLocal<ORDER>::value;
     Elem<T, ORDER, NQ> el;
     load_geometry(m, e, el);
-    gather(m, e, x, el.U);
+#pragma unroll
+    for (int a = 0; a < NB; ++a)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) el.U[a][d] = T(x[3 * int64_t(nodes[a]) + d]);
     const T w = el.det * T(quad_weight(NQ));
     Kin<T> k;
     field_grad<T, ORDER, NQ>(Q, el.U, el.jit, k.H);
@@ -109,30 +119,50 @@ __device__ __forceinline__ void setup_point(const DevMesh<T> &m, int e, const do
     if (own) eacc += double(energy_density<T, MAT>(k, el.lam, el.mu) * w);
     T P[9];
     pk1<T, MAT>(k, el.lam, el.mu, P);
-    // Q = w P jit^T: the gradient row of node a is sum_q sum_k RG(q,a,k) Q[d][k]
+    // gradient row of node a at this point: sum_k RG(Q,a,k) Qm[d][k], Qm = w P jit^T
+    {
+        T Qm[9];
 #pragma unroll
-    for (int d = 0; d < 3; ++d)
+        for (int d = 0; d < 3; ++d)
 #pragma unroll
-        for (int kk = 0; kk < 3; ++kk)
-            s[S::Q + Q * 9 + d * 3 + kk] =
-                w * (P[d * 3 + 0] * el.jit[kk * 3 + 0] + P[d * 3 + 1] * el.jit[kk * 3 + 1] + P[d * 3 + 2] * el.jit[kk * 3 + 2]);
+            for (int kk = 0; kk < 3; ++kk)
+                Qm[d * 3 + kk] = w * (P[d * 3 + 0] * el.jit[kk * 3 + 0] + P[d * 3 + 1] * el.jit[kk * 3 + 1] +
+                                      P[d * 3 + 2] * el.jit[kk * 3 + 2]);
+#pragma unroll
+        for (int a = 0; a < NB; ++a)
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                T g = T(0);
+#pragma unroll
+                for (int kk = 0; kk < 3; ++kk) {
+                    const double r = RG<ORDER, NQ>(Q, a, kk);
+                    if (r != 0.0) g = fmat(Qm[d * 3 + kk], T(r), g);
+                }
+                gq[a][d] = g;
+            }
+    }
     T M[9], cl, cc;
     hess_qp<T, MAT>(k, el.jit, el.lam, el.mu, w, M, cl, cc);
 #pragma unroll
-    for (int b = 0; b < NB; ++b) khat<T, ORDER, NQ>(Q, b, M, s + S::K + (Q * NB + b) * 3);
-    s[S::C + Q] = cl;
-    s[S::C + NQ + Q] = cc;
+    for (int b = 0; b < NB; ++b) {
+        T kb[4];
+        khat<T, ORDER, NQ>(Q, b, M, kb);
+        kb[3] = T(0);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) s[S::K + (Q * NB + b) * 4 + i] = kb[i];
+    }
+    s[S::C + 2 * Q] = cl;
+    s[S::C + 2 * Q + 1] = cc;
     if (Q == 0) {
         T g6[6];
         metric(el.jit, g6);
         const T cm = w * el.mu;
 #pragma unroll
         for (int i = 0; i < 6; ++i) s[S::G + i] = cm * g6[i];
+        s[S::G + 6] = T(0);
+        s[S::G + 7] = T(0);
     }
 }
-
-__device__ __forceinline__ float fmat(float a, float b, float c) { return fmaf(a, b, c); }
-__device__ __forceinline__ double fmat(double a, double b, double c) { return fma(a, b, c); }
 
 template <typename T> struct Vec4 {
     T x, y, z, w;
@@ -147,6 +177,19 @@ __device__ __forceinline__ Vec4<double> ld4(const double *p)
     const double2 a = *reinterpret_cast<const double2 *>(p), b = *reinterpret_cast<const double2 *>(p + 2);
     return {a.x, a.y, b.x, b.y};
 }
+template <typename T> struct Vec2 {
+    T x, y;
+};
+__device__ __forceinline__ Vec2<float> ld2(const float *p)
+{
+    const float2 v = *reinterpret_cast<const float2 *>(p);
+    return {v.x, v.y};
+}
+__device__ __forceinline__ Vec2<double> ld2(const double *p)
+{
+    const double2 v = *reinterpret_cast<const double2 *>(p);
+    return {v.x, v.y};
+}
 
 struct Unit {
     int p, c; // patch, chunk; p < 0: none
@@ -157,11 +200,17 @@ __global__ __launch_bounds__(NT) void k_egh(DevMesh<T> m, const double *__restri
                                             T *__restrict__ values, double *__restrict__ grad, double *__restrict__ epart)
 {
     constexpr int NB = NLocal<ORDER>::value;
-    constexpr int GPW = 64 / NB;
+    constexpr int GL = group_lanes<ORDER>();
+    constexpr int GPW = groups_per_wave<ORDER>();
     constexpr int WA = producer_waves<T, NQ>();
     constexpr int NG = n_groups<T, ORDER, NQ>();
-    using S = Stage<ORDER, NQ>;
+    constexpr int IC = item_cap<T, ORDER, NQ>();
+    constexpr int PS = pos_stride(NB);
+    constexpr int NP = WA * 64; // producer threads
+    using S = Stage<T, ORDER, NQ>;
     constexpr int SE = S::SIZE;
+    constexpr int CH = chunk_elems<T, NQ>();
+    constexpr int NPART = grad_parts<T, NQ>();
 
     extern __shared__ __align__(16) unsigned char smem[];
     T *acc = reinterpret_cast<T *>(smem + L.acc);
@@ -198,36 +247,96 @@ __global__ __launch_bounds__(NT) void k_egh(DevMesh<T> m, const double *__restri
         return np < n_patches ? Unit{np, 0} : Unit{-1, 0};
     };
 
-    // producers: stage unit u into buffer slot
-    auto produce = [&](Unit u, int slot) {
+    // producers.  prefetch(u) issues the loads of unit u's item tables and
+    // element node lists into registers; they stay in flight across the
+    // barrier and produce(u) stores the tables to LDS and stages the elements.
+    using PU = std::conditional_t<PS == 16, uint4, uint32_t>; // column positions of one item
+    constexpr int NH = (IC + NP - 1) / NP;
+    constexpr int NC = (NG + 1 + NP - 1) / NP;
+    struct Pref {
+        int nit, cn;
+        int32_t ev;
+        int32_t nodes[NB];
+        int hv[NH], cv[NC];
+        PU pv[NH];
+    };
+    int pq, pkk;
+    producer_task<T, NQ>(tid, pq, pkk);
+    auto prefetch = [&](Unit u, Pref &f) {
+        f.nit = 0;
+        f.cn = 0;
+        if (u.p < 0) return;
+        const int el0 = desc(u.p, PD_EL0), cs0 = desc(u.p, PD_CS0), cgb = desc(u.p, PD_CG0);
+        const int cbeg = fd.chunk_start[cs0 + u.c];
+        f.cn = fd.chunk_start[cs0 + u.c + 1] - cbeg;
+        const int it0 = fd.cg[cgb + u.c * NG];
+        f.nit = fd.cg[cgb + (u.c + 1) * NG] - it0;
+        const PU *gpos = reinterpret_cast<const PU *>(fd.pos) + it0;
+#pragma unroll
+        for (int j = 0; j < NH; ++j)
+            if (tid + j * NP < f.nit) {
+                f.hv[j] = fd.hdr[it0 + tid + j * NP];
+                f.pv[j] = gpos[tid + j * NP];
+            }
+#pragma unroll
+        for (int j = 0; j < NC; ++j) f.cv[j] = tid + j * NP <= NG ? fd.cg[cgb + u.c * NG + tid + j * NP] - it0 : 0;
+        if (pkk < f.cn) {
+            const int64_t v = int64_t(el0) + cbeg + pkk;
+            f.ev = fd.elems[v];
+#pragma unroll
+            for (int a = 0; a < NB; ++a) f.nodes[a] = fd.vnodes[v * NB + a];
+        }
+    };
+    auto produce = [&](int slot, const Pref &f) {
         T *stg = reinterpret_cast<T *>(smem + (slot ? L.stg1 : L.stg0));
         int *lhdr = reinterpret_cast<int *>(smem + (slot ? L.hdr1 : L.hdr0));
-        uint8_t *lpos = reinterpret_cast<uint8_t *>(smem + (slot ? L.pos1 : L.pos0));
+        PU *lpu = reinterpret_cast<PU *>(smem + (slot ? L.pos1 : L.pos0));
         int *lcg = reinterpret_cast<int *>(smem + (slot ? L.cg1 : L.cg0));
-        const int el0 = desc(u.p, PD_EL0), cs0 = desc(u.p, PD_CS0), cgb = desc(u.p, PD_CG0);
-        const int cbeg = fd.chunk_start[cs0 + u.c], cn = fd.chunk_start[cs0 + u.c + 1] - cbeg;
-        const int it0 = fd.cg[cgb + u.c * NG], it1 = fd.cg[cgb + (u.c + 1) * NG];
-        const int pt = tid; // 0 .. WA*64-1
-        for (int g = pt; g <= NG; g += WA * 64) lcg[g] = fd.cg[cgb + u.c * NG + g] - it0;
-        for (int i = pt; i < it1 - it0; i += WA * 64) lhdr[i] = fd.hdr[it0 + i];
-        for (int i = pt; i < (it1 - it0) * NB; i += WA * 64) lpos[i] = fd.pos[int64_t(it0) * NB + i];
+#pragma unroll
+        for (int j = 0; j < NH; ++j)
+            if (tid + j * NP < f.nit) {
+                lhdr[tid + j * NP] = f.hv[j];
+                lpu[tid + j * NP] = f.pv[j];
+            }
+#pragma unroll
+        for (int j = 0; j < NC; ++j)
+            if (tid + j * NP <= NG) lcg[tid + j * NP] = f.cv[j];
         double eacc = 0;
-        int q, kk;
-        producer_task<T, NQ>(tid, q, kk);
-        if (kk < cn) {
-            const int32_t ev = fd.elems[el0 + cbeg + kk];
-            const int e = ev & 0x7fffffff;
-            const bool own = ev < 0;
-            T *s = stg + kk * SE;
+        T gq[NB][3];
+#pragma unroll
+        for (int a = 0; a < NB; ++a) gq[a][0] = gq[a][1] = gq[a][2] = T(0);
+        T *s = stg + pkk * SE;
+        if (pkk < f.cn) {
+            const int e = f.ev & 0x7fffffff;
+            const bool own = f.ev < 0;
             if constexpr (NQ == 1) {
-                setup_point<T, ORDER, NQ, MAT, 0>(m, e, x, s, own, eacc);
+                setup_point<T, ORDER, NQ, MAT, 0>(m, e, f.nodes, x, s, own, eacc, gq);
             } else {
-                switch (q) {
-                case 0: setup_point<T, ORDER, NQ, MAT, 0>(m, e, x, s, own, eacc); break;
-                case 1: setup_point<T, ORDER, NQ, MAT, 1>(m, e, x, s, own, eacc); break;
-                case 2: setup_point<T, ORDER, NQ, MAT, 2>(m, e, x, s, own, eacc); break;
-                default: setup_point<T, ORDER, NQ, MAT, 3>(m, e, x, s, own, eacc); break;
+                switch (pq) {
+                case 0: setup_point<T, ORDER, NQ, MAT, 0>(m, e, f.nodes, x, s, own, eacc, gq); break;
+                case 1: setup_point<T, ORDER, NQ, MAT, 1>(m, e, f.nodes, x, s, own, eacc, gq); break;
+                case 2: setup_point<T, ORDER, NQ, MAT, 2>(m, e, f.nodes, x, s, own, eacc, gq); break;
+                default: setup_point<T, ORDER, NQ, MAT, 3>(m, e, f.nodes, x, s, own, eacc, gq); break;
                 }
+            }
+        }
+        // element gradient: sum over the points held by this wavefront (lanes
+        // CH apart), written by the lane of the wavefront's first point
+#pragma unroll
+        for (int off = CH; off < 64; off <<= 1)
+#pragma unroll
+            for (int a = 0; a < NB; ++a)
+#pragma unroll
+                for (int d = 0; d < 3; ++d) gq[a][d] += __shfl_xor(gq[a][d], off);
+        if (pkk < f.cn && lane < CH) {
+            const int h = CH >= 64 ? pq : wave;
+#pragma unroll
+            for (int a = 0; a < NB; ++a) {
+                T *r = s + S::R + (h * NB + a) * 4;
+                r[0] = gq[a][0];
+                r[1] = gq[a][1];
+                r[2] = gq[a][2];
+                r[3] = T(0);
             }
         }
 #pragma unroll
@@ -235,9 +344,12 @@ __global__ __launch_bounds__(NT) void k_egh(DevMesh<T> m, const double *__restri
         if (lane == 0) red[slot * NWAVE + wave] = eacc;
     };
 
-    // consumers: accumulate unit u from buffer slot
+    // consumers: accumulate unit u from buffer slot.  Group lane (b, mr)
+    // forms row mr of the 3x3 block (a, b) of each item; lanes with b = 0 add
+    // gradient component mr of row a.
     const int cw = wave - WA;
-    const int gi = lane / NB, b = lane - gi * NB;
+    const int gi = lane / GL, gl = lane - gi * GL;
+    const int b = gl / 3, mr = gl - 3 * (gl / 3);
     const bool in_group = !producer && gi < GPW;
     const int group = cw * GPW + gi;
     auto consume = [&](int slot) {
@@ -251,66 +363,49 @@ __global__ __launch_bounds__(NT) void k_egh(DevMesh<T> m, const double *__restri
             const int hdr = lhdr[it];
             const int kk = hdr & 1023, a = (hdr >> 10) & 15, r = hdr >> 14;
             const T *s = stg + kk * SE;
-            const int pos = lpos[it * NB + b];
-            if (b < 3) {
-                // gradient entry (row a, component d = b)
-                T gs = T(0);
+            const int pos = lpos[it * PS + b];
+            if (b == 0) {
+                // gradient entry (row a, component mr)
+                T gs = s[S::R + a * 4 + mr];
 #pragma unroll
-                for (int q = 0; q < NQ; ++q) {
-                    const Vec4<T> g = ld4(rg + (q * NB + a) * 4);
-                    const T *qd = s + S::Q + q * 9 + b * 3;
-                    gs = fmat(g.x, qd[0], fmat(g.y, qd[1], fmat(g.z, qd[2], gs)));
-                }
-                gacc[3 * r + b] += gs;
+                for (int h = 1; h < NPART; ++h) gs += s[S::R + (h * NB + a) * 4 + mr];
+                gacc[3 * r + mr] += gs;
             }
-            T cl[NQ], cc[NQ];
+            T o0 = T(0), o1 = T(0), o2 = T(0);
 #pragma unroll
             for (int q = 0; q < NQ; ++q) {
-                cl[q] = s[S::C + q];
-                cc[q] = s[S::C + NQ + q];
-            }
-            T blk[9];
-#pragma unroll
-            for (int i = 0; i < 9; ++i) blk[i] = T(0);
-#pragma unroll
-            for (int q = 0; q < NQ; ++q) {
-                const T *kbp = s + S::K + (q * NB + b) * 3, *kap = s + S::K + (q * NB + a) * 3;
-                const T kb[3] = {kbp[0], kbp[1], kbp[2]};
-                const T ka[3] = {kap[0], kap[1], kap[2]};
-                T u[3], v[3];
-#pragma unroll
-                for (int mm = 0; mm < 3; ++mm) {
-                    u[mm] = cl[q] * ka[mm];
-                    v[mm] = cc[q] * ka[mm];
-                }
-#pragma unroll
-                for (int mm = 0; mm < 3; ++mm)
-#pragma unroll
-                    for (int nn = 0; nn < 3; ++nn) blk[mm * 3 + nn] = fmat(kb[mm], v[nn], fmat(u[mm], kb[nn], blk[mm * 3 + nn]));
+                const Vec4<T> ka = ld4(s + S::K + (q * NB + a) * 4), kb = ld4(s + S::K + (q * NB + b) * 4);
+                const Vec2<T> c = ld2(s + S::C + 2 * q);
+                const T u = c.x * s[S::K + (q * NB + a) * 4 + mr], v = c.y * s[S::K + (q * NB + b) * 4 + mr];
+                o0 = fmat(u, kb.x, fmat(v, ka.x, o0));
+                o1 = fmat(u, kb.y, fmat(v, ka.y, o1));
+                o2 = fmat(u, kb.z, fmat(v, ka.z, o2));
             }
             {
                 const Vec4<T> g0 = ld4(s + S::G), c0 = ld4(rgg + (a * NB + b) * 8);
-                const T g4 = s[S::G + 4], g5 = s[S::G + 5];
-                const T c4 = rgg[(a * NB + b) * 8 + 4], c5 = rgg[(a * NB + b) * 8 + 5];
-                const T sab = fmat(g0.x, c0.x, fmat(g0.y, c0.y, fmat(g0.z, c0.z, fmat(g0.w, c0.w, fmat(g4, c4, g5 * c5)))));
-                blk[0] += sab;
-                blk[4] += sab;
-                blk[8] += sab;
+                const Vec2<T> g4 = ld2(s + S::G + 4), c4 = ld2(rgg + (a * NB + b) * 8 + 4);
+                const T sab = fmat(g0.x, c0.x, fmat(g0.y, c0.y, fmat(g0.z, c0.z, fmat(g0.w, c0.w, fmat(g4.x, c4.x, g4.y * c4.y)))));
+                o0 += mr == 0 ? sab : T(0);
+                o1 += mr == 1 ? sab : T(0);
+                o2 += mr == 2 ? sab : T(0);
             }
-            const int deg = rdeg[r];
-            T *ar = acc + racc[r] + 3 * pos;
-#pragma unroll
-            for (int mm = 0; mm < 3; ++mm)
-#pragma unroll
-                for (int nn = 0; nn < 3; ++nn) ar[mm * 3 * deg + nn] += blk[mm * 3 + nn];
+            T *ar = acc + racc[r] + mr * 3 * rdeg[r] + 3 * pos;
+            ar[0] += o0;
+            ar[1] += o1;
+            ar[2] += o2;
         }
     };
 
     Unit cur{int(blockIdx.x), 0};
     if (cur.p >= n_patches) return; // uniform
     begin_patch(cur.p);
+    Pref pf;
+    if (producer) prefetch(cur, pf);
     __syncthreads(); // rg tables
-    if (producer) produce(cur, 0);
+    if (producer) {
+        produce(0, pf);
+        prefetch(next_unit(cur), pf);
+    }
     __syncthreads();
     if (tid == 0) {
         double t = 0;
@@ -321,7 +416,10 @@ __global__ __launch_bounds__(NT) void k_egh(DevMesh<T> m, const double *__restri
     while (true) {
         const Unit nxt = next_unit(cur);
         if (producer) {
-            if (nxt.p >= 0) produce(nxt, slot ^ 1);
+            if (nxt.p >= 0) {
+                produce(slot ^ 1, pf);
+                prefetch(next_unit(nxt), pf);
+            }
         } else {
             consume(slot);
         }
@@ -354,7 +452,7 @@ __global__ __launch_bounds__(NT) void k_egh(DevMesh<T> m, const double *__restri
 FusedDev device_view(const Handle &h)
 {
     return FusedDev{h.patch_desc.as<int32_t>(), h.patch_rows.as<int32_t>(), h.patch_row_acc.as<int32_t>(),
-                    h.patch_elems.as<int32_t>(), h.patch_cs.as<int32_t>(), h.patch_cg.as<int32_t>(),
+                    h.patch_elems.as<int32_t>(), h.patch_vnodes.as<int32_t>(), h.patch_cs.as<int32_t>(), h.patch_cg.as<int32_t>(),
                     h.patch_hdr.as<int32_t>(), h.patch_pos.as<uint8_t>(), h.brow_ptr.as<int32_t>()};
 }
 
@@ -363,7 +461,7 @@ template <typename T, int ORDER, int NQ> FusedShape shape_for()
     FusedShape s{};
     s.groups = n_groups<T, ORDER, NQ>();
     s.chunk = chunk_elems<T, NQ>();
-    s.item_cap = item_cap<T, ORDER>();
+    s.item_cap = item_cap<T, ORDER, NQ>();
     s.max_rows = 256;
     const Lds L = lds_layout<T, ORDER, NQ>(0, s.max_rows);
     s.acc_capacity = (LDS_LIMIT - L.total) / int(sizeof(T));
@@ -401,6 +499,7 @@ void ensure_fused_tables(Handle &h)
     h.patch_rows = upload(t.rows, h.stream);
     h.patch_row_acc = upload(t.row_acc, h.stream);
     h.patch_elems = upload(t.elems, h.stream);
+    h.patch_vnodes = upload(t.vnodes, h.stream);
     h.patch_cs = upload(t.chunk_start, h.stream);
     h.patch_cg = upload(t.cg, h.stream);
     h.patch_hdr = upload(t.hdr, h.stream);

@@ -8,9 +8,10 @@
 //
 // Nodes are grouped in Morton order of their positions so that patches are
 // compact and the border (re-visited) elements few.  Inside a patch the rows
-// are dealt to lane groups (one group = n_local lanes, one lane per column
-// block of an element) so that each group owns whole rows; a group processes
-// its (row, element) items in element order, which fixes the summation order.
+// are dealt to lane groups (one group = 3 n_local lanes, one lane per row of
+// each 3x3 column block of an element) so that each group owns whole rows of a
+// chunk; a group processes its (row, element) items in a fixed order, which
+// fixes the summation order.
 #include <algorithm>
 #include <array>
 #include <numeric>
@@ -113,6 +114,7 @@ FusedTables build_fused_tables(const Pattern &pat, int n_local, const int32_t *e
     std::vector<Local> loc(P);
     const int G = shape.groups;
     const int CH = shape.chunk;
+    const int PS = pos_stride(n_local);
     auto build_one = [&](int p) {
         Local &L = loc[p];
         const int k0 = patch_start[p], k1 = patch_start[p + 1];
@@ -134,45 +136,88 @@ FusedTables build_fused_tables(const Pattern &pat, int n_local, const int32_t *e
         std::sort(el.begin(), el.end());
         el.erase(std::unique(el.begin(), el.end()), el.end());
         const int nel = int(el.size());
-        L.elems.resize(nel);
-        std::vector<int> owned_rows_in(nel, 0);
+        // owned rows of each element
+        std::vector<std::vector<int>> erows(nel);
         for (int r = 0; r < R; ++r)
             for (int k = pat.node_ptr[L.rows[r]]; k < pat.node_ptr[L.rows[r] + 1]; ++k) {
                 const int e = pat.node_inc[k] / n_local;
-                owned_rows_in[std::lower_bound(el.begin(), el.end(), e) - el.begin()]++;
+                erows[std::lower_bound(el.begin(), el.end(), e) - el.begin()].push_back(r);
             }
+        int n_items = 0;
+        for (const auto &v : erows) n_items += int(v.size());
+        // Chunks: at most CH elements and item_cap items each.  A row's items
+        // in one chunk run serially on one lane group, so elements are dealt
+        // to chunks (in element order, each to the chunk where its busiest row
+        // has the fewest items so far, then the fewest items overall) to
+        // spread every row's items over the chunks.
+        std::vector<int> chunk_of(nel);
+        int nch = std::max((nel + CH - 1) / CH, (n_items + shape.item_cap - 1) / shape.item_cap);
+        nch = std::max(nch, 1);
+        for (;; ++nch) {
+            std::vector<int> load(size_t(nch) * R, 0), cnt(nch, 0), itc(nch, 0);
+            bool ok = true;
+            for (int k = 0; k < nel && ok; ++k) {
+                int best = -1, bkey0 = 0, bkey1 = 0;
+                for (int c = 0; c < nch; ++c) {
+                    if (cnt[c] >= CH || itc[c] + int(erows[k].size()) > shape.item_cap) continue;
+                    int key0 = 0;
+                    for (int r : erows[k]) key0 = std::max(key0, load[size_t(c) * R + r]);
+                    if (best < 0 || key0 < bkey0 || (key0 == bkey0 && itc[c] < bkey1)) {
+                        best = c;
+                        bkey0 = key0;
+                        bkey1 = itc[c];
+                    }
+                }
+                if (best < 0) {
+                    ok = false;
+                    break;
+                }
+                chunk_of[k] = best;
+                cnt[best]++;
+                itc[best] += int(erows[k].size());
+                for (int r : erows[k]) load[size_t(best) * R + r]++;
+            }
+            if (ok) {
+                for (int c = 0; c < nch; ++c) L.max_unit_items = std::max(L.max_unit_items, itc[c]);
+                break;
+            }
+        }
+        // drop empty chunks, then lay the elements out chunk by chunk
+        {
+            std::vector<int> used(nch, 0), remap(nch, -1);
+            for (int k = 0; k < nel; ++k) used[chunk_of[k]] = 1;
+            int n = 0;
+            for (int c = 0; c < nch; ++c)
+                if (used[c]) remap[c] = n++;
+            for (int k = 0; k < nel; ++k) chunk_of[k] = remap[chunk_of[k]];
+            nch = n;
+        }
+        std::vector<int> perm(nel);
+        std::iota(perm.begin(), perm.end(), 0);
+        std::stable_sort(perm.begin(), perm.end(), [&](int x, int y) { return chunk_of[x] < chunk_of[y]; });
+        std::vector<int> slot_of(nel); // element index in the sorted list -> position in the chunked list
+        for (int k = 0; k < nel; ++k) slot_of[perm[k]] = k;
+        L.elems.resize(nel);
+        L.chunk_start.assign(nch + 1, 0);
         for (int k = 0; k < nel; ++k) {
-            const int e = el[k];
+            const int e = el[perm[k]];
             // the element's energy is summed by the patch owning its first node
             const bool own = patch_of[elem_nodes[size_t(e) * n_local]] == p;
             L.elems[k] = e | (own ? int32_t(0x80000000u) : 0);
+            L.chunk_start[chunk_of[perm[k]] + 1]++;
         }
-        // chunks: at most CH elements and at most item_cap items each
-        std::vector<int> chunk_of(nel);
-        L.chunk_start.push_back(0);
-        for (int k = 0, cnt = 0, items = 0; k < nel; ++k) {
-            if (cnt == CH || items + owned_rows_in[k] > shape.item_cap) {
-                L.chunk_start.push_back(k);
-                L.max_unit_items = std::max(L.max_unit_items, items);
-                cnt = 0;
-                items = 0;
-            }
-            chunk_of[k] = int(L.chunk_start.size()) - 1;
-            cnt++;
-            items += owned_rows_in[k];
-            if (k == nel - 1) L.max_unit_items = std::max(L.max_unit_items, items);
-        }
-        L.chunk_start.push_back(nel);
-        L.n_chunks = int(L.chunk_start.size()) - 1;
+        for (int c = 0; c < nch; ++c) L.chunk_start[c + 1] += L.chunk_start[c];
+        L.n_chunks = nch;
         // items (row, element), each tagged with its chunk
         struct Item { int chunk, group, elem_k, a, r; };
         std::vector<Item> items;
+        items.reserve(n_items);
         for (int r = 0; r < R; ++r) {
             const int node = L.rows[r];
             for (int k = pat.node_ptr[node]; k < pat.node_ptr[node + 1]; ++k) {
                 const int e = pat.node_inc[k] / n_local, a = pat.node_inc[k] % n_local;
                 const int ek = int(std::lower_bound(el.begin(), el.end(), e) - el.begin());
-                items.push_back({chunk_of[ek], -1, ek, a, r});
+                items.push_back({chunk_of[ek], -1, slot_of[ek], a, r});
             }
         }
         // Per chunk, rows -> groups: a row needs one owner only within a chunk
@@ -212,19 +257,19 @@ FusedTables build_fused_tables(const Pattern &pat, int n_local, const int32_t *e
         for (const Item &it : items) L.cg[size_t(it.chunk) * G + it.group + 1]++;
         for (size_t i = 1; i < L.cg.size(); ++i) L.cg[i] += L.cg[i - 1];
         L.hdr.resize(items.size());
-        L.pos.resize(items.size() * size_t(n_local));
+        L.pos.assign(items.size() * size_t(PS), 0);
         for (size_t i = 0; i < items.size(); ++i) {
             const Item &it = items[i];
             const int kk = it.elem_k - L.chunk_start[it.chunk];
             L.hdr[i] = kk | (it.a << 10) | (it.r << 14);
             const int node = L.rows[it.r];
             const int32_t *c0 = pat.bcol.data() + pat.brow_ptr[node], *c1 = pat.bcol.data() + pat.brow_ptr[node + 1];
-            const int e = el[it.elem_k];
+            const int e = el[perm[it.elem_k]];
             for (int b = 0; b < n_local; ++b) {
                 const int j = elem_nodes[size_t(e) * n_local + b];
                 const int64_t ps = std::lower_bound(c0, c1, j) - c0;
                 if (ps > 255) fail(PFGPU_ERR_UNSUPPORTED, "a node has more than 256 block neighbours");
-                L.pos[i * n_local + b] = uint8_t(ps);
+                L.pos[i * PS + b] = uint8_t(ps);
             }
         }
     };
@@ -252,10 +297,11 @@ FusedTables build_fused_tables(const Pattern &pat, int n_local, const int32_t *e
     t.rows.reserve(nrow);
     t.row_acc.reserve(nrow);
     t.elems.reserve(nel);
+    t.vnodes.reserve(nel * n_local);
     t.chunk_start.reserve(ncs);
     t.cg.reserve(ncg);
     t.hdr.reserve(nit);
-    t.pos.reserve(nit * n_local);
+    t.pos.reserve(nit * pos_stride(n_local));
     for (int p = 0; p < P; ++p) {
         Local &L = loc[p];
         int32_t *d = t.desc.data() + size_t(p) * PD_STRIDE;
@@ -272,6 +318,10 @@ FusedTables build_fused_tables(const Pattern &pat, int n_local, const int32_t *e
         t.rows.insert(t.rows.end(), L.rows.begin(), L.rows.end());
         t.row_acc.insert(t.row_acc.end(), L.row_acc.begin(), L.row_acc.end());
         t.elems.insert(t.elems.end(), L.elems.begin(), L.elems.end());
+        for (int32_t ev : L.elems) {
+            const int32_t *en = elem_nodes + size_t(ev & 0x7fffffff) * n_local;
+            t.vnodes.insert(t.vnodes.end(), en, en + n_local);
+        }
         t.chunk_start.insert(t.chunk_start.end(), L.chunk_start.begin(), L.chunk_start.end());
         for (int32_t v : L.cg) t.cg.push_back(v + item_base);
         t.hdr.insert(t.hdr.end(), L.hdr.begin(), L.hdr.end());

@@ -1,6 +1,8 @@
 // The handle behind the C-ABI and the launch functions that operate on it.
 #pragma once
 
+#include <array>
+
 #include "common.hpp"
 
 struct pfgpu_handle {
@@ -22,7 +24,7 @@ struct pfgpu_handle {
     pfgpu::DevBuf brow_ptr, block_row, contrib_ptr, contrib;
     // fused-path patch tables (built on first use)
     bool fused_ready = false;
-    pfgpu::DevBuf patch_desc, patch_rows, patch_row_acc, patch_elems, patch_cs, patch_cg, patch_hdr, patch_pos;
+    pfgpu::DevBuf patch_desc, patch_rows, patch_row_acc, patch_elems, patch_cs, patch_cg, patch_hdr, patch_pos, patch_vnodes;
     int n_patches = 0;
     int fused_max_acc = 0, fused_max_rows = 0;
     int64_t fused_visits = 0, fused_chunks = 0;
@@ -34,6 +36,28 @@ struct pfgpu_handle {
     // scratch
     pfgpu::DevBuf loc, hloc, partial, red_out;
     pfgpu::DevBuf h_x, h_v, h_out, h_out2, h_vals; // staging for host-pointer calls
+    // consistent mass matrix: one value per 3x3 block (block = m I_3)
+    pfgpu::DevBuf mass, pat_bcol;
+    bool has_mass = false;
+    // objective of the solves: w E(x) + 1/2 (x - xt)^T M (x - xt) - w f^T x
+    // (xt, f: device vectors or null; the mass term needs xt)
+    double obj_w = 1.0;
+    const double *obj_xt = nullptr, *obj_f = nullptr;
+    int obj_generation = 0;
+    // transient run: checkpoints of (x, v) every ckpt_every steps
+    std::vector<pfgpu::DevBuf> ckpt;
+    pfgpu::DevBuf tr_fixed, tr_f, tr_seg, load;
+    bool has_load = false;
+    double tr_dt = 0;
+    int tr_steps = 0, tr_every = 0;
+    bool tr_has_f = false, tr_has_fixed = false;
+    pfgpu_newton_params tr_newton{};
+    // solver workspace: vectors, scalars, the captured CG iteration
+    pfgpu::DevBuf sv, h_mask;
+    void *cg_graph = nullptr; // hipGraphExec_t
+    std::array<const void *, 4> cg_graph_key{};
+    hipStream_t own_stream = nullptr; // used when the caller's stream is the legacy default
+    ~pfgpu_handle();
 
     size_t value_size() const { return precision == PFGPU_MIXED ? sizeof(float) : sizeof(double); }
     int n_blocks_sym() const { return n_local * (n_local + 1) / 2; }
@@ -67,6 +91,22 @@ void ensure_fused_tables(Handle &h);
 void fused_gradient(Handle &h, const double *x, double *g, double *energy);
 void fused_energy_gradient_hessian(Handle &h, const double *x, double *energy, double *g, void *values);
 void fused_hvp(Handle &h, const double *x, const double *v, double *y);
+
+// ---- mass (mass.cpp) ---------------------------------------------------
+std::vector<double> reference_mass(int order);
+void build_mass(Handle &h, const double *rho, double rho_uniform);
+
+// ---- solver (solver.hip) ------------------------------------------------
+void mass_apply(Handle &h, const double *v, double *y);
+void newton_solve(Handle &h, double *x, const uint8_t *fixed, const pfgpu_newton_params &p, pfgpu_newton_report &r);
+void transient_forward(Handle &h, const double *x0, const double *v0, const uint8_t *fixed, const double *f,
+                       double dt, int steps, int every, const pfgpu_newton_params &np, double *xT, double *vT);
+void transient_adjoint(Handle &h, const double *dJdx, const double *dJdv, int cg_max, double cg_tol, double *dtheta,
+                       double *dx0, double *dv0);
+int cg_solve_ws(Handle &h, const double *x, const uint8_t *fixed, const double *b, int max_it, double tol, bool use_graph,
+                double *rel);
+double *cg_solution(Handle &h);
+void solver_release(Handle &h);
 
 // ---- misc kernels (reduce.hip) ----------------------------------------
 void build_device_pattern(Handle &h);

@@ -93,18 +93,32 @@ __global__ __launch_bounds__(BLOCK) void k_gradient(DevMesh<T> m, const double *
     }
 }
 
-// upper-triangle local Hessian blocks; hloc[((e*NBLK) + s(a,b))*9 + m*3 + n], a <= b
+// Upper-triangle local Hessian blocks (a <= b, s(a,b) = a NB - a(a-1)/2 + b - a)
+// in tiles of 64 elements: block s of element e is the 9 values at
+// hloc[(((e / 64) NBLK + s) 64 + e % 64) 9], row-major.  A wavefront computes
+// the blocks of its 64 elements and writes each block index s as one
+// contiguous 2304-byte run, staged through LDS.
 template <typename T, int ORDER, int NQ, int MAT>
 __global__ __launch_bounds__(BLOCK) void k_hessian(DevMesh<T> m, const double *__restrict__ x, T *__restrict__ hloc)
 {
     constexpr int NB = NLocal<ORDER>::value, NBLK = NB * (NB + 1) / 2;
+    __shared__ T stage[BLOCK / 64][64 * 9];
     const int e = blockIdx.x * BLOCK + threadIdx.x;
-    if (e >= m.n_elements) return;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    // every lane of a wavefront takes part in the staged stores; lanes past
+    // the last element compute on element 0 and their blocks are never read
+    const bool valid = e < m.n_elements;
+    const int ee = valid ? e : 0;
+    const int64_t tile = (int64_t(blockIdx.x) * BLOCK + wv * 64) / 64;
+    if (tile * 64 >= m.n_elements) return; // whole wavefront past the end (uniform)
     Elem<T, ORDER, NQ> el;
-    load_geometry(m, e, el);
-    if (MAT == MAT_NH) gather(m, e, x, el.U);
+    load_geometry(m, ee, el);
+    if (MAT == MAT_NH) gather(m, ee, x, el.U);
     const T w = el.det * T(quad_weight(NQ));
-    T K[NQ][NB][3], cl[NQ], cc[NQ];
+    // per point: M = F^{-T} J^{-T} (k_a = M RG(q,a)) and the two curvature
+    // weights; the k vectors are formed per block row and column to keep the
+    // register footprint small
+    T Mq[NQ][9], cl[NQ], cc[NQ];
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
         Kin<T> k;
@@ -112,20 +126,24 @@ __global__ __launch_bounds__(BLOCK) void k_hessian(DevMesh<T> m, const double *_
             field_grad<T, ORDER, NQ>(q, el.U, el.jit, k.H);
             kinematics(k);
         }
-        T M[9];
-        hess_qp<T, MAT>(k, el.jit, el.lam, el.mu, w, M, cl[q], cc[q]);
-#pragma unroll
-        for (int a = 0; a < NB; ++a) khat<T, ORDER, NQ>(q, a, M, K[q][a]);
+        hess_qp<T, MAT>(k, el.jit, el.lam, el.mu, w, Mq[q], cl[q], cc[q]);
     }
     T g6[6];
     metric(el.jit, g6);
     const T cm = w * el.mu;
-    T *out = hloc + int64_t(e) * NBLK * 9;
+    T *st = stage[wv];
+    T *out = hloc + tile * NBLK * 576;
+#pragma unroll 1
+    for (int a = 0; a < NB; ++a) {
+        T Ka[NQ][3];
 #pragma unroll
-    for (int a = 0; a < NB; ++a)
+        for (int q = 0; q < NQ; ++q) khat<T, ORDER, NQ>(q, a, Mq[q], Ka[q]);
+        // not unrolled: the k_b of all columns at once would not fit in registers
+#pragma unroll 1
+        for (int b = a; b < NB; ++b) {
+            T Kb[NQ][3];
 #pragma unroll
-        for (int b = 0; b < NB; ++b) {
-            if (b < a) continue;
+            for (int q = 0; q < NQ; ++q) khat<T, ORDER, NQ>(q, b, Mq[q], Kb[q]);
             const int s = a * NB - a * (a - 1) / 2 + (b - a);
             const T gd = cm * grad_dot<ORDER, NQ>(a, b, g6);
 #pragma unroll
@@ -134,10 +152,16 @@ __global__ __launch_bounds__(BLOCK) void k_hessian(DevMesh<T> m, const double *_
                 for (int nn = 0; nn < 3; ++nn) {
                     T v = mm == nn ? gd : T(0);
 #pragma unroll
-                    for (int q = 0; q < NQ; ++q) v += cl[q] * K[q][a][mm] * K[q][b][nn] + cc[q] * K[q][b][mm] * K[q][a][nn];
-                    out[s * 9 + mm * 3 + nn] = v;
+                    for (int q = 0; q < NQ; ++q) v += cl[q] * Ka[q][mm] * Kb[q][nn] + cc[q] * Kb[q][mm] * Ka[q][nn];
+                    st[lane * 9 + mm * 3 + nn] = v;
                 }
+            // the wavefront's 64 blocks s are 576 consecutive values
+            __builtin_amdgcn_wave_barrier();
+#pragma unroll
+            for (int j = 0; j < 9; ++j) out[s * 576 + j * 64 + lane] = st[j * 64 + lane];
+            __builtin_amdgcn_wave_barrier();
         }
+    }
 }
 
 // local H(x) v; yloc[(e*NB + a)*3 + d]
@@ -355,35 +379,46 @@ __global__ __launch_bounds__(BLOCK) void k_gather_rows(int n, const int32_t *__r
     out[3 * int64_t(i) + 2] = s2;
 }
 
-// one thread per 3x3 block of the pattern; contributions in element order
-template <typename T, int NB>
+// one thread per 3x3 block of the pattern; contributions in element order.
+// src[k]: index of the contribution's block in hloc (bit 31: stored
+// transposed); contributions are fetched four at a time.
+template <typename T>
 __global__ __launch_bounds__(BLOCK) void k_gather_blocks(int64_t n_blocks, const int32_t *__restrict__ brow_ptr,
                                                          const int32_t *__restrict__ block_row, const int32_t *__restrict__ cptr,
-                                                         const int32_t *__restrict__ contrib, const T *__restrict__ hloc,
+                                                         const int32_t *__restrict__ src, const T *__restrict__ hloc,
                                                          T *__restrict__ values)
 {
-    constexpr int NBLK = NB * (NB + 1) / 2;
     const int64_t b = int64_t(blockIdx.x) * BLOCK + threadIdx.x;
     if (b >= n_blocks) return;
     T acc[9];
 #pragma unroll
     for (int i = 0; i < 9; ++i) acc[i] = T(0);
-    for (int k = cptr[b]; k < cptr[b + 1]; ++k) {
-        const int32_t c = contrib[k];
-        const int e = c / (NB * NB), a = (c / NB) % NB, bb = c % NB;
-        if (a <= bb) {
-            const int s = a * NB - a * (a - 1) / 2 + (bb - a);
-            const T *h = hloc + (int64_t(e) * NBLK + s) * 9;
+    const int k0 = cptr[b], k1 = cptr[b + 1];
+    for (int k = k0; k < k1; k += 4) {
+        int32_t c[4];
 #pragma unroll
-            for (int i = 0; i < 9; ++i) acc[i] += h[i];
-        } else {
-            const int s = bb * NB - bb * (bb - 1) / 2 + (a - bb);
-            const T *h = hloc + (int64_t(e) * NBLK + s) * 9;
+        for (int j = 0; j < 4; ++j) c[j] = k + j < k1 ? src[k + j] : 0;
+        T h[4][9];
 #pragma unroll
-            for (int mm = 0; mm < 3; ++mm)
+        for (int j = 0; j < 4; ++j)
+            if (k + j < k1) {
+                const T *p = hloc + int64_t(c[j] & 0x7fffffff) * 9;
 #pragma unroll
-                for (int nn = 0; nn < 3; ++nn) acc[mm * 3 + nn] += h[nn * 3 + mm];
-        }
+                for (int i = 0; i < 9; ++i) h[j][i] = p[i];
+            }
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            if (k + j < k1) {
+                if (c[j] >= 0) {
+#pragma unroll
+                    for (int i = 0; i < 9; ++i) acc[i] += h[j][i];
+                } else {
+#pragma unroll
+                    for (int mm = 0; mm < 3; ++mm)
+#pragma unroll
+                        for (int nn = 0; nn < 3; ++nn) acc[mm * 3 + nn] += h[j][nn * 3 + mm];
+                }
+            }
     }
     const int row = block_row[b];
     const int r0 = brow_ptr[row], deg = brow_ptr[row + 1] - r0;
@@ -432,6 +467,21 @@ void ensure_plain_hessian_maps(Handle &h)
     std::vector<int32_t> contrib, block_row;
     build_block_contributions(h.pat, h.n_local, h.h_elem_nodes.data(), ptr64, contrib, block_row);
     std::vector<int32_t> ptr(ptr64.begin(), ptr64.end());
+    // contribution (e, a, b) -> index of block s(min(a,b), max(a,b)) of element
+    // e in the tiled hloc layout of k_hessian; bit 31 when a > b (transposed)
+    {
+        const int NB = h.n_local, NBLK = h.n_blocks_sym();
+        if (int64_t((h.n_elements + 63) / 64) * NBLK * 64 >= (int64_t(1) << 31))
+            fail(PFGPU_ERR_UNSUPPORTED, "too many local Hessian blocks for 31-bit indices");
+        for (size_t k = 0; k < contrib.size(); ++k) {
+            const int32_t c = contrib[k];
+            const int e = c / (NB * NB), a = (c / NB) % NB, b = c % NB;
+            const int lo = std::min(a, b), hi = std::max(a, b);
+            const int s = lo * NB - lo * (lo - 1) / 2 + (hi - lo);
+            const int32_t idx = int32_t((int64_t(e / 64) * NBLK + s) * 64 + e % 64);
+            contrib[k] = a > b ? int32_t(uint32_t(idx) | 0x80000000u) : idx;
+        }
+    }
     h.contrib_ptr = upload(ptr, h.stream);
     h.contrib = upload(contrib, h.stream);
     h.block_row = upload(block_row, h.stream);
@@ -443,11 +493,11 @@ void plain_hessian(Handle &h, const double *x, void *values)
 {
     ensure_plain_hessian_maps(h);
     const unsigned grid = grid_for(h.n_elements);
-    h.hloc.reserve(h.value_size() * size_t(h.n_elements) * h.n_blocks_sym() * 9);
+    h.hloc.reserve(h.value_size() * size_t((h.n_elements + 63) / 64) * 64 * h.n_blocks_sym() * 9);
     dispatch(h, [&]<typename T, int O, int Q, int M>() {
         k_hessian<T, O, Q, M><<<grid, BLOCK, 0, h.stream>>>(h.mesh<T>(), x, h.hloc.as<T>());
         PF_LAUNCH_CHECK();
-        k_gather_blocks<T, NLocal<O>::value><<<grid_for(h.pat.n_blocks), BLOCK, 0, h.stream>>>(
+        k_gather_blocks<T><<<grid_for(h.pat.n_blocks), BLOCK, 0, h.stream>>>(
             h.pat.n_blocks, h.brow_ptr.as<int32_t>(), h.block_row.as<int32_t>(), h.contrib_ptr.as<int32_t>(),
             h.contrib.as<int32_t>(), h.hloc.as<T>(), static_cast<T *>(values));
     });

@@ -1,0 +1,654 @@
+// Newton-CG on the free dofs, device-resident.
+//
+// Follows the reference's nonlinear solve of the elastic form: Dirichlet dofs
+// are removed from the unknowns (NLProblem::full_to_reduced /
+// reduced_to_full, NLProblem.cpp:583-629; the gradient and Hessian rows and
+// columns of fixed dofs are dropped, BCLagrangianForm.cpp:149-200), the
+// Newton direction solves H_ff d = -g_f (polysolve Newton with a CG linear
+// solver, here matrix-free through the Hessian-vector product) and the step
+// length comes from a backtracking Armijo line search on the energy.
+//
+// Fixed dofs are handled with a mask instead of compacted vectors: the
+// reduced system is the full system with fixed rows and columns zeroed,
+// which keeps every vector in the mesh's dof layout.
+//
+// Precision: whatever the assembler's element precision, every solver
+// vector, dot product, norm and energy difference is FP64, and every
+// reduction has a fixed order (per-block partials over a fixed grid, then
+// one fixed tree), so a solve is bitwise reproducible.
+//
+// The objective is w E(x) + 1/2 (x - xt)^T M (x - xt) - w f^T x: the elastic
+// form alone (w = 1, no xt, no f), with a body load f, or the implicit-Euler
+// step objective (w = dt^2, xt = x_prev + dt v_prev; InertiaForm.cpp:17-35,
+// ImplicitEuler.cpp:13-31, the elastic form weighted by dt^2).
+//
+// One CG iteration is a fixed sequence of kernels whose scalars (alpha,
+// beta, r.r) live in device memory, so it can be captured once in a hipGraph
+// and replayed; the host reads the residual only every CHECK iterations.
+#include <cmath>
+
+#include "dispatch.hpp"
+
+namespace pfgpu {
+
+namespace {
+
+constexpr int DOT_BLOCKS = 256; // fixed grid of the dot-product kernels
+constexpr int CHECK = 8;        // CG iterations between host convergence checks
+
+// scalar slots of the CG workspace
+enum { S_RR, S_PQ, S_ALPHA, S_BETA, S_RR_NEW, S_FLAG, S_B2, S_GD, S_E0, S_E1, S_M, S_F, S_COUNT };
+
+// per-block partial of sum_i a_i b_i over free dofs (mask: 1 = fixed)
+__global__ __launch_bounds__(BLOCK) void k_dot_partials(const double *__restrict__ a, const double *__restrict__ b,
+                                                        const uint8_t *__restrict__ fixed, int n, double *__restrict__ part)
+{
+    __shared__ double s[BLOCK];
+    double v = 0;
+    for (int i = blockIdx.x * BLOCK + threadIdx.x; i < n; i += DOT_BLOCKS * BLOCK)
+        if (!fixed || !fixed[i]) v = fma(a[i], b[i], v);
+    s[threadIdx.x] = v;
+    __syncthreads();
+    for (int w = BLOCK / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = s[0];
+}
+
+// one block: fixed-order sum of the DOT_BLOCKS partials into scal[slot], then
+// the scalar bookkeeping of the step that follows
+enum { AFTER_NONE, AFTER_ALPHA, AFTER_BETA };
+__global__ __launch_bounds__(BLOCK) void k_finish_dot(const double *__restrict__ part, double *__restrict__ scal, int slot,
+                                                      int after)
+{
+    __shared__ double s[BLOCK];
+    const int t = threadIdx.x;
+    s[t] = t < DOT_BLOCKS ? part[t] : 0.0;
+    __syncthreads();
+    for (int w = BLOCK / 2; w > 0; w >>= 1) {
+        if (t < w) s[t] += s[t + w];
+        __syncthreads();
+    }
+    if (t != 0) return;
+    const double v = s[0];
+    scal[slot] = v;
+    if (after == AFTER_ALPHA) {
+        // alpha = r.r / p.Hp; a non-positive curvature stops the iteration
+        const double rr = scal[S_RR];
+        const bool ok = v > 0.0 && rr > 0.0 && scal[S_FLAG] == 0.0;
+        if (!(v > 0.0) && rr > 0.0) scal[S_FLAG] = 1.0;
+        scal[S_ALPHA] = ok ? rr / v : 0.0;
+    } else if (after == AFTER_BETA) {
+        const double rr = scal[S_RR];
+        scal[S_BETA] = rr > 0.0 && scal[S_FLAG] == 0.0 ? v / rr : 0.0;
+        scal[S_RR] = scal[S_FLAG] == 0.0 ? v : rr;
+    }
+}
+
+// q := 0 on fixed dofs
+__global__ __launch_bounds__(BLOCK) void k_mask(double *__restrict__ q, const uint8_t *__restrict__ fixed, int n)
+{
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i < n && fixed && fixed[i]) q[i] = 0.0;
+}
+
+// y += alpha p, r -= alpha q
+__global__ __launch_bounds__(BLOCK) void k_cg_update(double *__restrict__ y, double *__restrict__ r, const double *__restrict__ p,
+                                                     const double *__restrict__ q, const double *__restrict__ scal, int n)
+{
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    const double a = scal[S_ALPHA];
+    y[i] = fma(a, p[i], y[i]);
+    r[i] = fma(-a, q[i], r[i]);
+}
+
+// p = r + beta p
+__global__ __launch_bounds__(BLOCK) void k_cg_direction(double *__restrict__ p, const double *__restrict__ r,
+                                                        const double *__restrict__ scal, int n)
+{
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    p[i] = fma(scal[S_BETA], p[i], r[i]);
+}
+
+// CG start: y = 0, r = b (free dofs), p = r
+__global__ __launch_bounds__(BLOCK) void k_cg_start(double *__restrict__ y, double *__restrict__ r, double *__restrict__ p,
+                                                    const double *__restrict__ b, const uint8_t *__restrict__ fixed, int n)
+{
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    const double v = fixed && fixed[i] ? 0.0 : b[i];
+    y[i] = 0.0;
+    r[i] = v;
+    p[i] = v;
+}
+
+__global__ void k_set_scalar(double *__restrict__ scal, int slot, double v) { scal[slot] = v; }
+
+// out = x + a d (a read from the host argument), or -v on free dofs
+__global__ __launch_bounds__(BLOCK) void k_axpy_to(double *__restrict__ out, const double *__restrict__ x, double a,
+                                                   const double *__restrict__ d, int n)
+{
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i < n) out[i] = fma(a, d[i], x[i]);
+}
+__global__ __launch_bounds__(BLOCK) void k_neg_free(double *__restrict__ out, const double *__restrict__ v,
+                                                    const uint8_t *__restrict__ fixed, int n)
+{
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i < n) out[i] = fixed && fixed[i] ? 0.0 : -v[i];
+}
+
+// y = a y + M v (M: one value per 3x3 block of the node pattern, block = m I_3),
+// one thread per node, columns in pattern order
+__global__ __launch_bounds__(BLOCK) void k_mass_apply(int n_nodes, const int32_t *__restrict__ brow_ptr,
+                                                      const int32_t *__restrict__ bcol, const double *__restrict__ m,
+                                                      const double *__restrict__ v, double a, double *__restrict__ y)
+{
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n_nodes) return;
+    double s0 = 0, s1 = 0, s2 = 0;
+    for (int k = brow_ptr[i]; k < brow_ptr[i + 1]; ++k) {
+        const double w = m[k];
+        const int64_t j = 3 * int64_t(bcol[k]);
+        s0 = fma(w, v[j], s0);
+        s1 = fma(w, v[j + 1], s1);
+        s2 = fma(w, v[j + 2], s2);
+    }
+    const int64_t o = 3 * int64_t(i);
+    y[o] = fma(a, y[o], s0);
+    y[o + 1] = fma(a, y[o + 1], s1);
+    y[o + 2] = fma(a, y[o + 2], s2);
+}
+
+// out = x - xt
+__global__ __launch_bounds__(BLOCK) void k_sub(double *__restrict__ out, const double *__restrict__ x,
+                                               const double *__restrict__ xt, int n)
+{
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i < n) out[i] = x[i] - xt[i];
+}
+
+// g = w g + mt - w f   (mt, f may be null)
+__global__ __launch_bounds__(BLOCK) void k_combine_gradient(double *__restrict__ g, double w, const double *__restrict__ mt,
+                                                            const double *__restrict__ f, int n)
+{
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    double v = w * g[i];
+    if (mt) v += mt[i];
+    if (f) v = fma(-w, f[i], v);
+    g[i] = v;
+}
+
+// y = (x - x_prev) / dt  and  xt = x + dt y (the next step's predictor)
+__global__ __launch_bounds__(BLOCK) void k_euler_update(const double *__restrict__ x, const double *__restrict__ xp,
+                                                        double dt, double *__restrict__ v, int n)
+{
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i < n) v[i] = (x[i] - xp[i]) / dt;
+}
+
+// out = a u + b v (either may be null)
+__global__ __launch_bounds__(BLOCK) void k_lincomb(double *__restrict__ out, double a, const double *__restrict__ u, double b,
+                                                   const double *__restrict__ v, int n)
+{
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    double s = 0;
+    if (u) s = a * u[i];
+    if (v) s = fma(b, v[i], s);
+    out[i] = s;
+}
+
+// out += a v
+__global__ __launch_bounds__(BLOCK) void k_axpy(double *__restrict__ out, double a, const double *__restrict__ v, int n)
+{
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i < n) out[i] = fma(a, v[i], out[i]);
+}
+
+struct Work {
+    double *r, *p, *q, *y, *g, *xt, *d, *t, *mt, *scal, *part;
+};
+
+Work workspace(Handle &h)
+{
+    const size_t n = size_t(h.n_dofs);
+    h.sv.reserve(sizeof(double) * (9 * n + S_COUNT + DOT_BLOCKS));
+    double *b = h.sv.as<double>();
+    return Work{b, b + n, b + 2 * n, b + 3 * n, b + 4 * n, b + 5 * n, b + 6 * n, b + 7 * n, b + 8 * n,
+                b + 9 * n, b + 9 * n + S_COUNT};
+}
+
+void dot(Handle &h, const Work &w, const double *a, const double *b, const uint8_t *fixed, int slot, int after)
+{
+    k_dot_partials<<<DOT_BLOCKS, BLOCK, 0, h.stream>>>(a, b, fixed, h.n_dofs, w.part);
+    k_finish_dot<<<1, BLOCK, 0, h.stream>>>(w.part, w.scal, slot, after);
+}
+
+double read_scalar(Handle &h, const Work &w, int slot)
+{
+    double v = 0;
+    PF_HIP(hipMemcpyAsync(&v, w.scal + slot, sizeof(double), hipMemcpyDeviceToHost, h.stream));
+    PF_HIP(hipStreamSynchronize(h.stream));
+    return v;
+}
+
+bool inertia(const Handle &h) { return h.obj_xt != nullptr; }
+
+// y = (w H(x) + [M]) v
+void hvp_any(Handle &h, const double *x, const double *v, double *y)
+{
+    if (h.path == PFGPU_PATH_FUSED)
+        fused_hvp(h, x, v, y);
+    else
+        plain_hvp(h, x, v, y);
+    if (inertia(h))
+        k_mass_apply<<<grid_for(h.n_nodes), BLOCK, 0, h.stream>>>(h.n_nodes, h.brow_ptr.as<int32_t>(),
+                                                                   h.pat_bcol.as<int32_t>(), h.mass.as<double>(), v,
+                                                                   h.obj_w, y);
+    else if (h.obj_w != 1.0)
+        k_combine_gradient<<<grid_for(h.n_dofs), BLOCK, 0, h.stream>>>(y, h.obj_w, nullptr, nullptr, h.n_dofs);
+    PF_LAUNCH_CHECK();
+}
+
+// objective gradient into g and its value into scal[slot] (device)
+void objective_gradient(Handle &h, const Work &w, const double *x, double *g, int slot);
+// objective value into scal[slot] (device)
+void objective_energy(Handle &h, const Work &w, const double *x, int slot);
+
+// one CG iteration (a fixed kernel sequence)
+void cg_iteration(Handle &h, const Work &w, const double *x, const uint8_t *fixed)
+{
+    const unsigned g = grid_for(h.n_dofs);
+    hvp_any(h, x, w.p, w.q);
+    k_mask<<<g, BLOCK, 0, h.stream>>>(w.q, fixed, h.n_dofs);
+    dot(h, w, w.p, w.q, fixed, S_PQ, AFTER_ALPHA);
+    k_cg_update<<<g, BLOCK, 0, h.stream>>>(w.y, w.r, w.p, w.q, w.scal, h.n_dofs);
+    dot(h, w, w.r, w.r, fixed, S_RR_NEW, AFTER_BETA);
+    k_cg_direction<<<g, BLOCK, 0, h.stream>>>(w.p, w.r, w.scal, h.n_dofs);
+    PF_LAUNCH_CHECK();
+}
+
+void objective_energy(Handle &h, const Work &w, const double *x, int slot)
+{
+    // elastic part into scal[slot]; the host combines the parts
+    plain_energy(h, x, w.scal + slot, nullptr);
+    if (inertia(h)) {
+        const unsigned g = grid_for(h.n_dofs);
+        k_sub<<<g, BLOCK, 0, h.stream>>>(w.t, x, h.obj_xt, h.n_dofs);
+        k_mass_apply<<<grid_for(h.n_nodes), BLOCK, 0, h.stream>>>(h.n_nodes, h.brow_ptr.as<int32_t>(),
+                                                                   h.pat_bcol.as<int32_t>(), h.mass.as<double>(), w.t,
+                                                                   0.0, w.mt);
+        dot(h, w, w.t, w.mt, nullptr, S_M, AFTER_NONE);
+    }
+    if (h.obj_f) dot(h, w, h.obj_f, x, nullptr, S_F, AFTER_NONE);
+    PF_LAUNCH_CHECK();
+}
+
+void objective_gradient(Handle &h, const Work &w, const double *x, double *g, int slot)
+{
+    if (h.path == PFGPU_PATH_FUSED)
+        fused_gradient(h, x, g, w.scal + slot);
+    else
+        plain_gradient(h, x, g, w.scal + slot);
+    const double *mt = nullptr;
+    if (inertia(h)) {
+        const unsigned gr = grid_for(h.n_dofs);
+        k_sub<<<gr, BLOCK, 0, h.stream>>>(w.t, x, h.obj_xt, h.n_dofs);
+        k_mass_apply<<<grid_for(h.n_nodes), BLOCK, 0, h.stream>>>(h.n_nodes, h.brow_ptr.as<int32_t>(),
+                                                                   h.pat_bcol.as<int32_t>(), h.mass.as<double>(), w.t,
+                                                                   0.0, w.mt);
+        dot(h, w, w.t, w.mt, nullptr, S_M, AFTER_NONE);
+        mt = w.mt;
+    }
+    if (h.obj_f) dot(h, w, h.obj_f, x, nullptr, S_F, AFTER_NONE);
+    if (mt || h.obj_f || h.obj_w != 1.0)
+        k_combine_gradient<<<grid_for(h.n_dofs), BLOCK, 0, h.stream>>>(g, h.obj_w, mt, h.obj_f, h.n_dofs);
+    PF_LAUNCH_CHECK();
+}
+
+// host value of the objective whose parts objective_energy/gradient left in scal
+double objective_value(Handle &h, const Work &w, int slot)
+{
+    double e = h.obj_w * read_scalar(h, w, slot);
+    if (inertia(h)) e += 0.5 * read_scalar(h, w, S_M);
+    if (h.obj_f) e -= h.obj_w * read_scalar(h, w, S_F);
+    return e;
+}
+
+} // namespace
+
+void mass_apply(Handle &h, const double *v, double *y)
+{
+    if (!h.has_mass) fail(PFGPU_ERR_INVALID_STATE, "no mass matrix: call pfgpu_set_density first");
+    k_mass_apply<<<grid_for(h.n_nodes), BLOCK, 0, h.stream>>>(h.n_nodes, h.brow_ptr.as<int32_t>(), h.pat_bcol.as<int32_t>(),
+                                                               h.mass.as<double>(), v, 0.0, y);
+    PF_LAUNCH_CHECK();
+}
+
+void solver_release(Handle &h)
+{
+    if (h.cg_graph) (void)hipGraphExecDestroy(static_cast<hipGraphExec_t>(h.cg_graph));
+    h.cg_graph = nullptr;
+    h.cg_graph_key = {};
+}
+
+// CG on H(x)_ff y = b_f starting from y = 0; the solution is left in the
+// workspace's y.  Returns the iterations taken; *rel = ||r|| / ||b_f||.
+int cg_solve_ws(Handle &h, const double *x, const uint8_t *fixed, const double *b, int max_it, double tol,
+                bool use_graph, double *rel)
+{
+    const Work w = workspace(h);
+    const unsigned g = grid_for(h.n_dofs);
+    k_cg_start<<<g, BLOCK, 0, h.stream>>>(w.y, w.r, w.p, b, fixed, h.n_dofs);
+    k_set_scalar<<<1, 1, 0, h.stream>>>(w.scal, S_FLAG, 0.0);
+    dot(h, w, w.r, w.r, fixed, S_RR, AFTER_NONE);
+    PF_LAUNCH_CHECK();
+    const double b2 = read_scalar(h, w, S_RR);
+    if (rel) *rel = 0.0;
+    if (b2 == 0.0 || max_it <= 0) return 0;
+    if (!std::isfinite(b2)) fail(PFGPU_ERR_INVALID_STATE, "non-finite right-hand side in the CG solve");
+    // the graph replays one iteration; it is rebuilt when its arguments change
+    const std::array<const void *, 4> key{x, fixed, w.r, reinterpret_cast<const void *>(intptr_t(h.obj_generation))};
+    if (use_graph && (h.cg_graph == nullptr || h.cg_graph_key != key)) {
+        solver_release(h);
+        cg_iteration(h, w, x, fixed); // sizes every scratch buffer before capture
+        k_cg_start<<<g, BLOCK, 0, h.stream>>>(w.y, w.r, w.p, b, fixed, h.n_dofs);
+        k_set_scalar<<<1, 1, 0, h.stream>>>(w.scal, S_FLAG, 0.0);
+        dot(h, w, w.r, w.r, fixed, S_RR, AFTER_NONE);
+        hipGraph_t graph = nullptr;
+        PF_HIP(hipStreamBeginCapture(h.stream, hipStreamCaptureModeThreadLocal));
+        try {
+            cg_iteration(h, w, x, fixed);
+        } catch (...) {
+            (void)hipStreamEndCapture(h.stream, &graph);
+            if (graph) (void)hipGraphDestroy(graph);
+            throw;
+        }
+        PF_HIP(hipStreamEndCapture(h.stream, &graph));
+        hipGraphExec_t exec = nullptr;
+        const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        PF_HIP(e);
+        h.cg_graph = exec;
+        h.cg_graph_key = key;
+    }
+    int it = 0;
+    double rr = b2;
+    while (it < max_it) {
+        const int n = std::min(CHECK, max_it - it);
+        for (int k = 0; k < n; ++k) {
+            if (use_graph)
+                PF_HIP(hipGraphLaunch(static_cast<hipGraphExec_t>(h.cg_graph), h.stream));
+            else
+                cg_iteration(h, w, x, fixed);
+        }
+        it += n;
+        rr = read_scalar(h, w, S_RR);
+        if (!std::isfinite(rr)) fail(PFGPU_ERR_INVALID_STATE, "non-finite residual in the CG solve (inverted element?)");
+        if (std::sqrt(rr) <= tol * std::sqrt(b2) || read_scalar(h, w, S_FLAG) != 0.0) break;
+    }
+    if (rel) *rel = std::sqrt(rr / b2);
+    return it;
+}
+
+double *cg_solution(Handle &h) { return workspace(h).y; }
+
+void newton_solve(Handle &h, double *x, const uint8_t *fixed, const pfgpu_newton_params &prm, pfgpu_newton_report &rep)
+{
+    const Work w = workspace(h);
+    const unsigned gr = grid_for(h.n_dofs);
+    rep = pfgpu_newton_report{};
+    if (inertia(h) && !h.has_mass) fail(PFGPU_ERR_INVALID_STATE, "inertia term without a mass matrix");
+    auto energy_at = [&](const double *xx) {
+        objective_energy(h, w, xx, S_E1);
+        return objective_value(h, w, S_E1);
+    };
+    for (int it = 0; it < prm.max_iterations; ++it) {
+        // energy and gradient at x; the reduced gradient norm
+        objective_gradient(h, w, x, w.g, S_E0);
+        dot(h, w, w.g, w.g, fixed, S_B2, AFTER_NONE);
+        const double gn = std::sqrt(read_scalar(h, w, S_B2));
+        const double e0 = objective_value(h, w, S_E0);
+        rep.energy = e0;
+        rep.grad_norm = gn;
+        if (!std::isfinite(gn) || !std::isfinite(e0)) fail(PFGPU_ERR_INVALID_STATE, "non-finite energy or gradient at the Newton iterate");
+        if (gn <= prm.grad_norm) {
+            rep.converged = 1;
+            return;
+        }
+        // Newton direction: H_ff d = -g_f
+        k_neg_free<<<gr, BLOCK, 0, h.stream>>>(w.d, w.g, fixed, h.n_dofs);
+        PF_LAUNCH_CHECK();
+        double rel = 0;
+        const int cg_it = cg_solve_ws(h, x, fixed, w.d, prm.cg_max_iterations, prm.cg_tolerance, prm.use_graph != 0, &rel);
+        rep.cg_iterations += cg_it;
+        // direction in w.y; g.d for the Armijo condition
+        dot(h, w, w.g, w.y, fixed, S_GD, AFTER_NONE);
+        double gd = read_scalar(h, w, S_GD);
+        const double *dir = w.y;
+        if (!(gd < 0.0)) {
+            // not a descent direction (indefinite Hessian): steepest descent
+            k_neg_free<<<gr, BLOCK, 0, h.stream>>>(w.y, w.g, fixed, h.n_dofs);
+            PF_LAUNCH_CHECK();
+            gd = -gn * gn;
+        }
+        double alpha = 1.0;
+        bool accepted = false;
+        for (int ls = 0; ls <= prm.max_line_search; ++ls) {
+            k_axpy_to<<<gr, BLOCK, 0, h.stream>>>(w.xt, x, alpha, dir, h.n_dofs);
+            PF_LAUNCH_CHECK();
+            if (prm.line_search == 0) {
+                accepted = true;
+                break;
+            }
+            const double e1 = energy_at(w.xt);
+            // a NaN energy is an inverted element: rejected like an invalid step
+            if (std::isfinite(e1) && e1 - e0 <= prm.armijo_c * alpha * gd) {
+                accepted = true;
+                break;
+            }
+            alpha *= 0.5;
+            rep.line_search_halvings++;
+        }
+        if (!accepted) fail(PFGPU_ERR_INVALID_STATE, "line search found no acceptable step");
+        PF_HIP(hipMemcpyAsync(x, w.xt, sizeof(double) * size_t(h.n_dofs), hipMemcpyDeviceToDevice, h.stream));
+        rep.iterations = it + 1;
+        rep.last_alpha = alpha;
+    }
+    // final state
+    objective_gradient(h, w, x, w.g, S_E0);
+    dot(h, w, w.g, w.g, fixed, S_B2, AFTER_NONE);
+    rep.grad_norm = std::sqrt(read_scalar(h, w, S_B2));
+    rep.energy = objective_value(h, w, S_E0);
+    rep.converged = rep.grad_norm <= prm.grad_norm;
+}
+
+// ---- implicit Euler with checkpointed adjoint --------------------------
+//
+// Forward (ImplicitEuler.cpp:13-31; InertiaForm.cpp:17-35): with
+// xt_i = x_{i-1} + dt v_{i-1}, x_i minimises
+//   1/2 (x - xt_i)^T M (x - xt_i) + dt^2 (E(x) - f^T x)
+// over the free dofs (fixed dofs keep their x0 values), and
+// v_i = (x_i - x_{i-1}) / dt.  (x_i, v_i) is kept for i = 0 and every
+// `every` steps; the backward pass recomputes the states in between.
+//
+// Backward: the step residuals R_i = M (x_i - xt_i) + dt^2 (g(x_i) - f)
+// vanish at every step; with J = J(x_T, v_T) and multipliers p_i the
+// stationarity conditions give, on the free dofs,
+//   (M + dt^2 H(x_i)) p_i = dJ/dx_i + M (2 p_{i+1} - p_{i+2}),
+//   dJ/dx_T += (1/dt) dJ/dv_T,  dJ/dx_{T-1} = -(1/dt) dJ/dv_T,
+// and the total derivatives
+//   dJ/dtheta = sum_i dt^2 (-p_i^T dg(x_i)/dtheta)   (material VJP at x_i),
+//   dJ/dv_0 = dt M p_1,   dJ/dx_0 = M (p_1 - p_2),
+// zero on fixed dofs.  This is the BDF-1 case of the reference's backward
+// recurrence (StateDiff.cpp:320-413 with dJ_material_transient_adjoint_term,
+// AdjointTools.cpp:852-887, and dJ_initial_condition_adjoint_term,
+// AdjointTools.cpp:992-1010), written in one multiplier per step.
+
+namespace {
+
+struct ObjectiveScope {
+    Handle &h;
+    double w;
+    const double *xt, *f;
+    ObjectiveScope(Handle &hh, double ww, const double *x_tilde, const double *ff)
+        : h(hh), w(hh.obj_w), xt(hh.obj_xt), f(hh.obj_f)
+    {
+        h.obj_w = ww;
+        h.obj_xt = x_tilde;
+        h.obj_f = ff;
+        h.obj_generation++;
+    }
+    ~ObjectiveScope()
+    {
+        h.obj_w = w;
+        h.obj_xt = xt;
+        h.obj_f = f;
+        h.obj_generation++;
+    }
+};
+
+// one forward step: x_out minimises the step objective from (x_prev, v_prev);
+// v_out = (x_out - x_prev) / dt.  xt is a scratch vector.
+void euler_step(Handle &h, const double *xp, const double *vp, double *xt, double *x_out, double *v_out)
+{
+    const int n = h.n_dofs;
+    const unsigned g = grid_for(n);
+    const uint8_t *fixed = h.tr_has_fixed ? h.tr_fixed.as<uint8_t>() : nullptr;
+    k_lincomb<<<g, BLOCK, 0, h.stream>>>(xt, 1.0, xp, h.tr_dt, vp, n);
+    PF_HIP(hipMemcpyAsync(x_out, xp, sizeof(double) * size_t(n), hipMemcpyDeviceToDevice, h.stream));
+    {
+        ObjectiveScope os(h, h.tr_dt * h.tr_dt, xt, h.tr_has_f ? h.tr_f.as<double>() : nullptr);
+        pfgpu_newton_report rep{};
+        newton_solve(h, x_out, fixed, h.tr_newton, rep);
+        if (!rep.converged) fail(PFGPU_ERR_INVALID_STATE, "a time step's Newton solve did not converge");
+    }
+    k_euler_update<<<g, BLOCK, 0, h.stream>>>(x_out, xp, h.tr_dt, v_out, n);
+    PF_LAUNCH_CHECK();
+}
+
+} // namespace
+
+void transient_forward(Handle &h, const double *x0, const double *v0, const uint8_t *fixed, const double *f, double dt,
+                       int steps, int every, const pfgpu_newton_params &np, double *xT, double *vT)
+{
+    if (!h.has_mass) fail(PFGPU_ERR_INVALID_STATE, "no mass matrix: call pfgpu_set_density first");
+    const size_t n = size_t(h.n_dofs), bytes = sizeof(double) * n;
+    h.tr_dt = dt;
+    h.tr_steps = steps;
+    h.tr_every = every;
+    h.tr_newton = np;
+    h.tr_has_fixed = fixed != nullptr;
+    if (fixed) {
+        h.tr_fixed.reserve(n);
+        PF_HIP(hipMemcpyAsync(h.tr_fixed.get(), fixed, n, hipMemcpyDeviceToDevice, h.stream));
+    }
+    h.tr_has_f = f != nullptr;
+    if (f) {
+        h.tr_f.reserve(bytes);
+        PF_HIP(hipMemcpyAsync(h.tr_f.get(), f, bytes, hipMemcpyDeviceToDevice, h.stream));
+    }
+    const int n_ck = steps / every + 1;
+    h.ckpt.clear();
+    h.ckpt.resize(size_t(n_ck));
+    for (auto &c : h.ckpt) c.reserve(2 * bytes);
+    PF_HIP(hipMemcpyAsync(h.ckpt[0].as<double>(), x0, bytes, hipMemcpyDeviceToDevice, h.stream));
+    PF_HIP(hipMemcpyAsync(h.ckpt[0].as<double>() + n, v0, bytes, hipMemcpyDeviceToDevice, h.stream));
+    // two ping-pong states and the predictor
+    h.tr_seg.reserve(5 * bytes);
+    double *S = h.tr_seg.as<double>();
+    double *xa = S, *va = S + n, *xb = S + 2 * n, *vb = S + 3 * n, *xt = S + 4 * n;
+    PF_HIP(hipMemcpyAsync(xa, x0, bytes, hipMemcpyDeviceToDevice, h.stream));
+    PF_HIP(hipMemcpyAsync(va, v0, bytes, hipMemcpyDeviceToDevice, h.stream));
+    for (int i = 1; i <= steps; ++i) {
+        euler_step(h, xa, va, xt, xb, vb);
+        std::swap(xa, xb);
+        std::swap(va, vb);
+        if (i % every == 0) {
+            PF_HIP(hipMemcpyAsync(h.ckpt[i / every].as<double>(), xa, bytes, hipMemcpyDeviceToDevice, h.stream));
+            PF_HIP(hipMemcpyAsync(h.ckpt[i / every].as<double>() + n, va, bytes, hipMemcpyDeviceToDevice, h.stream));
+        }
+    }
+    if (xT) PF_HIP(hipMemcpyAsync(xT, xa, bytes, hipMemcpyDeviceToDevice, h.stream));
+    if (vT) PF_HIP(hipMemcpyAsync(vT, va, bytes, hipMemcpyDeviceToDevice, h.stream));
+}
+
+void transient_adjoint(Handle &h, const double *dJdx, const double *dJdv, int cg_max, double cg_tol, double *dtheta,
+                       double *dx0, double *dv0)
+{
+    if (h.tr_steps <= 0 || h.ckpt.empty()) fail(PFGPU_ERR_INVALID_STATE, "no forward run to differentiate");
+    const int T = h.tr_steps, K = h.tr_every, n = h.n_dofs, E = h.n_elements;
+    const size_t bytes = sizeof(double) * size_t(n);
+    const unsigned g = grid_for(n);
+    const double dt = h.tr_dt;
+    const uint8_t *fixed = h.tr_has_fixed ? h.tr_fixed.as<uint8_t>() : nullptr;
+    // segment states x_{c..c+K}, v_{c..c+K}; multipliers p_{i+1}, p_{i+2}; rhs;
+    // VJP of one step; running sum of the material terms
+    DevBuf seg(size_t(2 * (K + 1)) * bytes), pv(size_t(4) * bytes), acc(sizeof(double) * 4 * size_t(E));
+    double *SX = seg.as<double>(), *SV = SX + size_t(K + 1) * n;
+    double *p1 = pv.as<double>(), *p2 = p1 + n, *rhs = p2 + n, *tmp = rhs + n;
+    double *vjp = acc.as<double>(), *sum = vjp + 2 * size_t(E);
+    k_lincomb<<<g, BLOCK, 0, h.stream>>>(p1, 0.0, nullptr, 0.0, nullptr, n);
+    k_lincomb<<<g, BLOCK, 0, h.stream>>>(p2, 0.0, nullptr, 0.0, nullptr, n);
+    k_lincomb<<<grid_for(2 * E), BLOCK, 0, h.stream>>>(sum, 0.0, nullptr, 0.0, nullptr, 2 * E);
+    PF_LAUNCH_CHECK();
+    DevBuf xt_buf(bytes);
+    for (int c = ((T - 1) / K) * K; c >= 0; c -= K) {
+        // recompute the states of segment [c, min(c + K, T)] from checkpoint c
+        const int e = std::min(c + K, T);
+        const double *ck = h.ckpt[c / K].as<double>();
+        PF_HIP(hipMemcpyAsync(SX, ck, bytes, hipMemcpyDeviceToDevice, h.stream));
+        PF_HIP(hipMemcpyAsync(SV, ck + n, bytes, hipMemcpyDeviceToDevice, h.stream));
+        for (int i = c + 1; i <= e; ++i)
+            euler_step(h, SX + size_t(i - 1 - c) * n, SV + size_t(i - 1 - c) * n, xt_buf.as<double>(),
+                       SX + size_t(i - c) * n, SV + size_t(i - c) * n);
+        for (int i = e; i > c; --i) {
+            // rhs = dJ/dx_i + M (2 p_{i+1} - p_{i+2})
+            k_lincomb<<<g, BLOCK, 0, h.stream>>>(tmp, 2.0, p1, -1.0, p2, n);
+            mass_apply(h, tmp, rhs);
+            if (i == T) {
+                if (dJdx) k_axpy<<<g, BLOCK, 0, h.stream>>>(rhs, 1.0, dJdx, n);
+                if (dJdv) k_axpy<<<g, BLOCK, 0, h.stream>>>(rhs, 1.0 / dt, dJdv, n);
+            }
+            if (i == T - 1 && dJdv) k_axpy<<<g, BLOCK, 0, h.stream>>>(rhs, -1.0 / dt, dJdv, n);
+            PF_LAUNCH_CHECK();
+            const double *xi = SX + size_t(i - c) * n;
+            {
+                // (M + dt^2 H(x_i)) p_i = rhs on the free dofs
+                ObjectiveScope os(h, dt * dt, xi, nullptr);
+                double rel = 0;
+                cg_solve_ws(h, xi, fixed, rhs, cg_max, cg_tol, true, &rel);
+                if (!(rel <= std::max(cg_tol, 1e-300) * 10.0) && rel > 1e-6)
+                    fail(PFGPU_ERR_INVALID_STATE, "adjoint CG solve did not converge");
+            }
+            // shift: p_{i+2} <- p_{i+1}, p_{i+1} <- p_i
+            std::swap(p1, p2);
+            PF_HIP(hipMemcpyAsync(p1, cg_solution(h), bytes, hipMemcpyDeviceToDevice, h.stream));
+            // material term of step i
+            element_material_vjp(h, xi, p1, vjp);
+            k_axpy<<<grid_for(2 * E), BLOCK, 0, h.stream>>>(sum, dt * dt, vjp, 2 * E);
+            PF_LAUNCH_CHECK();
+        }
+    }
+    // p1 = p_1, p2 = p_2
+    if (dtheta) PF_HIP(hipMemcpyAsync(dtheta, sum, sizeof(double) * 2 * size_t(E), hipMemcpyDeviceToDevice, h.stream));
+    if (dv0) {
+        mass_apply(h, p1, dv0);
+        k_lincomb<<<g, BLOCK, 0, h.stream>>>(dv0, dt, dv0, 0.0, nullptr, n);
+        if (fixed) k_mask<<<g, BLOCK, 0, h.stream>>>(dv0, fixed, n);
+    }
+    if (dx0) {
+        k_lincomb<<<g, BLOCK, 0, h.stream>>>(tmp, 1.0, p1, -1.0, p2, n);
+        mass_apply(h, tmp, dx0);
+        if (fixed) k_mask<<<g, BLOCK, 0, h.stream>>>(dx0, fixed, n);
+    }
+    PF_LAUNCH_CHECK();
+    PF_HIP(hipStreamSynchronize(h.stream));
+}
+
+} // namespace pfgpu

@@ -1,0 +1,176 @@
+"""Host-side mirror of the callers of the assembly: the nonlinear solve of the
+elastic form and the static adjoint solve, backed by libpfgpu.
+
+* ``NLProblem`` -- ``solver::NLProblem`` (solver/NLProblem.cpp:583-629):
+  ``full_to_reduced`` / ``reduced_to_full`` over the Dirichlet dofs.
+* ``NewtonSolver`` -- polysolve's Newton solver with a CG linear solver and a
+  backtracking (Armijo) line search, as configured by the reference's
+  ``"nonlinear": {"solver": "Newton", "line_search": {"method": "Backtracking"}}``;
+  runs entirely on the GPU (``pfgpu_newton_solve``).
+* ``solve_static_adjoint`` -- ``solve_static_adjoint`` (solver/StateDiff.cpp:230-318):
+  H(u)^T p = dJ/du with zero adjoint on Dirichlet dofs (``pfgpu_cg_solve``).
+
+Every computation runs in the HIP library; there is no CPU path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _ffi
+from ._ffi import check, lib
+
+
+def dirichlet_mask(n_dofs: int, fixed_dofs) -> np.ndarray:
+    """uint8 mask over the dofs, 1 on Dirichlet dofs."""
+    m = np.zeros(n_dofs, dtype=np.uint8)
+    idx = np.asarray(fixed_dofs, dtype=np.int64).ravel()
+    if idx.size and (idx.min() < 0 or idx.max() >= n_dofs):
+        raise ValueError("Dirichlet dof out of range")
+    m[idx] = 1
+    return m
+
+
+class NLProblem:
+    """Reduced / full dof maps of a problem with Dirichlet dofs (NLProblem.cpp:583-629)."""
+
+    def __init__(self, n_dofs: int, fixed_dofs, fixed_values=None):
+        self.n_dofs = n_dofs
+        self.mask = dirichlet_mask(n_dofs, fixed_dofs)
+        self.fixed = np.flatnonzero(self.mask)
+        self.free = np.flatnonzero(self.mask == 0)
+        vals = np.zeros(self.fixed.size) if fixed_values is None else np.asarray(fixed_values, dtype=np.float64)
+        if vals.size != self.fixed.size:
+            raise ValueError("one value per Dirichlet dof expected")
+        self.fixed_values = vals
+
+    def full_to_reduced(self, full):
+        return np.asarray(full, dtype=np.float64)[self.free]
+
+    def reduced_to_full(self, reduced):
+        full = np.zeros(self.n_dofs)
+        full[self.free] = reduced
+        full[self.fixed] = self.fixed_values
+        return full
+
+
+@dataclass
+class NewtonReport:
+    iterations: int
+    cg_iterations: int
+    line_search_halvings: int
+    converged: bool
+    energy: float
+    grad_norm: float
+    last_alpha: float
+
+
+class NewtonSolver:
+    """Newton-CG on the free dofs with a backtracking line search (GPU-resident)."""
+
+    def __init__(self, max_iterations: int = 10, grad_norm: float = 1e-8, cg_max_iterations: int = 1000,
+                 cg_tolerance: float = 1e-6, line_search: str = "Backtracking", armijo_c: float = 1e-4,
+                 max_line_search: int = 30, use_graph: bool = True):
+        if line_search not in ("Backtracking", "None"):
+            raise ValueError("line_search must be 'Backtracking' or 'None'")
+        self.params = _ffi.NewtonParams(max_iterations, grad_norm, cg_max_iterations, cg_tolerance,
+                                        1 if line_search == "Backtracking" else 0, armijo_c, max_line_search,
+                                        1 if use_graph else 0)
+
+    def minimize(self, assembler, x, problem: NLProblem):
+        """Returns (x*, report); x holds the initial guess on the free dofs and
+        is overwritten with the problem's Dirichlet values on the fixed dofs."""
+        xx = np.array(x, dtype=np.float64, copy=True).ravel()
+        if xx.size != assembler.mesh.n_dofs:
+            raise ValueError("x has the wrong size")
+        xx[problem.fixed] = problem.fixed_values
+        rep = _ffi.NewtonReport()
+        check(lib().pfgpu_newton_solve(assembler._h, xx.ctypes.data_as(C.c_void_p),
+                                       problem.mask.ctypes.data_as(C.c_void_p), C.byref(self.params), C.byref(rep)))
+        return xx, NewtonReport(rep.iterations, rep.cg_iterations, rep.line_search_halvings, bool(rep.converged),
+                                rep.energy, rep.grad_norm, rep.last_alpha)
+
+
+def cg_solve(assembler, x, rhs, fixed_mask=None, max_iterations: int = 1000, tolerance: float = 1e-8):
+    """y with H(x)_ff y_f = rhs_f and y = 0 on fixed dofs (matrix-free CG).
+    Returns (y, iterations, relative residual)."""
+    n = assembler.mesh.n_dofs
+    xx = np.ascontiguousarray(x, dtype=np.float64)
+    b = np.ascontiguousarray(rhs, dtype=np.float64)
+    if xx.size != n or b.size != n:
+        raise ValueError("wrong vector size")
+    y = np.zeros(n)
+    it = C.c_int32(0)
+    rel = C.c_double(0)
+    m = None if fixed_mask is None else np.ascontiguousarray(fixed_mask, dtype=np.uint8)
+    check(lib().pfgpu_cg_solve(assembler._h, xx.ctypes.data_as(C.c_void_p),
+                               None if m is None else m.ctypes.data_as(C.c_void_p), b.ctypes.data_as(C.c_void_p),
+                               y.ctypes.data_as(C.c_void_p), max_iterations, tolerance, C.byref(it), C.byref(rel)))
+    return y, it.value, rel.value
+
+
+def solve_static_adjoint(assembler, u, dJdu, problem: NLProblem, max_iterations: int = 2000, tolerance: float = 1e-10):
+    """Adjoint p of a static solve: H(u) p = dJ/du on the free dofs, p = 0 on
+    Dirichlet dofs (StateDiff.cpp:230-318; the Hessian is symmetric)."""
+    p, it, rel = cg_solve(assembler, u, dJdu, problem.mask, max_iterations, tolerance)
+    return p
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+class TransientSimulation:
+    """Implicit-Euler elastodynamics with a checkpointed discrete adjoint.
+
+    Mirrors the reference's transient solve (time_integrator/ImplicitEuler.cpp:13-31,
+    solver/forms/InertiaForm.cpp:17-35 with the elastic form weighted by dt^2)
+    and its backward recurrence (optimization/StateDiff.cpp:320-413,
+    AdjointTools.cpp:852-887 and 992-1010).  The mass matrix is the
+    consistent one (assembler/Mass.cpp:4-20) with density ``rho``."""
+
+    def __init__(self, assembler, rho=1000.0, dt=1e-3, time_steps=10, checkpoint_every=1,
+                 newton: NewtonSolver = None):
+        self.a = assembler
+        E = assembler.mesh.n_elements
+        if np.ndim(rho) == 0:
+            check(lib().pfgpu_set_density(assembler._h, None, float(rho)))
+        else:
+            r = np.ascontiguousarray(rho, dtype=np.float64)
+            if r.size != E:
+                raise ValueError("one density per element expected")
+            check(lib().pfgpu_set_density(assembler._h, _p(r), 0.0))
+        self.params = _ffi.TransientParams(dt, time_steps, checkpoint_every)
+        self.newton = newton or NewtonSolver(max_iterations=50, grad_norm=1e-10, cg_tolerance=1e-12)
+
+    def mass_matvec(self, v):
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        y = np.zeros_like(v)
+        check(lib().pfgpu_mass_matvec(self.a._h, _p(v), _p(y)))
+        return y
+
+    def forward(self, x0, v0, problem: NLProblem = None, load=None):
+        n = self.a.mesh.n_dofs
+        x0 = np.array(x0, dtype=np.float64).ravel()
+        v0 = np.ascontiguousarray(v0, dtype=np.float64).ravel()
+        mask = None
+        if problem is not None:
+            x0[problem.fixed] = problem.fixed_values
+            mask = problem.mask
+        f = None if load is None else np.ascontiguousarray(load, dtype=np.float64)
+        xT, vT = np.zeros(n), np.zeros(n)
+        check(lib().pfgpu_transient_forward(self.a._h, _p(x0), _p(v0), _p(mask), _p(f), C.byref(self.params),
+                                            C.byref(self.newton.params), _p(xT), _p(vT)))
+        return xT, vT
+
+    def adjoint(self, dJdx=None, dJdv=None, cg_max_iterations=5000, cg_tolerance=1e-12):
+        """(dJ/d(lambda, mu), dJ/dx0, dJ/dv0) of J(x_T, v_T) through the last forward run."""
+        n, E = self.a.mesh.n_dofs, self.a.mesh.n_elements
+        gx = None if dJdx is None else np.ascontiguousarray(dJdx, dtype=np.float64)
+        gv = None if dJdv is None else np.ascontiguousarray(dJdv, dtype=np.float64)
+        dth, dx0, dv0 = np.zeros(2 * E), np.zeros(n), np.zeros(n)
+        check(lib().pfgpu_transient_adjoint(self.a._h, _p(gx), _p(gv), cg_max_iterations, cg_tolerance,
+                                            _p(dth), _p(dx0), _p(dv0)))
+        return dth, dx0, dv0

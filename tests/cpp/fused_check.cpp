@@ -14,6 +14,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <algorithm>
 #include <vector>
 
 #include "../../polyfem_amd/csrc/elem.hpp"
@@ -70,11 +71,12 @@ static void geometry(const Mesh &m, int e, double *jit, double &det)
     for (int k = 0; k < 9; ++k) jit[k] = J[k] * id;
 }
 
-// the kernel's per-point staging record (NeoHookean), layout Stage<ORDER, NQ>
-template <int ORDER, int NQ>
+// the kernel's per-point staging record (NeoHookean), layout StageT<ORDER, NQ, NPART>;
+// the gradient partial sums R must be zeroed before the first point
+template <int ORDER, int NQ, int NPART>
 static void stage_point(const Mesh &m, int e, int q, double *s)
 {
-    using S = Stage<ORDER, NQ>;
+    using S = StageT<ORDER, NQ, NPART>;
     constexpr int NB = NLocal<ORDER>::value;
     double jit[9], det;
     geometry(m, e, jit, det);
@@ -100,34 +102,47 @@ static void stage_point(const Mesh &m, int e, int q, double *s)
     for (int mm = 0; mm < 3; ++mm)
         for (int kk = 0; kk < 3; ++kk)
             M[mm * 3 + kk] = FmT[mm * 3] * jit[kk * 3] + FmT[mm * 3 + 1] * jit[kk * 3 + 1] + FmT[mm * 3 + 2] * jit[kk * 3 + 2];
-    for (int b = 0; b < NB; ++b)
+    for (int b = 0; b < NB; ++b) {
         for (int mm = 0; mm < 3; ++mm) {
             double v = 0;
             for (int k = 0; k < 3; ++k) v += M[mm * 3 + k] * RG<ORDER, NQ>(q, b, k);
-            s[S::K + (q * NB + b) * 3 + mm] = v;
+            s[S::K + (q * NB + b) * 4 + mm] = v;
         }
-    s[S::C + q] = w * lam;
-    s[S::C + NQ + q] = w * (mu - lam * lj);
-    for (int d = 0; d < 3; ++d)
+        s[S::K + (q * NB + b) * 4 + 3] = 0;
+    }
+    s[S::C + 2 * q] = w * lam;
+    s[S::C + 2 * q + 1] = w * (mu - lam * lj);
+    // point q adds to partial sum h = q / (NQ / NPART) of the element gradient
+    const int h = q / (NQ / NPART);
+    for (int d = 0; d < 3; ++d) {
+        double Qm[3];
         for (int kk = 0; kk < 3; ++kk)
-            s[S::Q + q * 9 + d * 3 + kk] = w * (P[d * 3] * jit[kk * 3] + P[d * 3 + 1] * jit[kk * 3 + 1] + P[d * 3 + 2] * jit[kk * 3 + 2]);
+            Qm[kk] = w * (P[d * 3] * jit[kk * 3] + P[d * 3 + 1] * jit[kk * 3 + 1] + P[d * 3 + 2] * jit[kk * 3 + 2]);
+        for (int a = 0; a < NB; ++a) {
+            double g = 0;
+            for (int kk = 0; kk < 3; ++kk) g += RG<ORDER, NQ>(q, a, kk) * Qm[kk];
+            s[S::R + (h * NB + a) * 4 + d] += g;
+        }
+    }
     if (q == 0) {
         auto dot = [&](int k, int l) { return jit[k * 3] * jit[l * 3] + jit[k * 3 + 1] * jit[l * 3 + 1] + jit[k * 3 + 2] * jit[l * 3 + 2]; };
         const double g6[6] = {dot(0, 0), dot(1, 1), dot(2, 2), dot(0, 1), dot(0, 2), dot(1, 2)};
         for (int i = 0; i < 6; ++i) s[S::G + i] = w * mu * g6[i];
+        s[S::G + 6] = s[S::G + 7] = 0;
     }
 }
 
-// block (a, b) and gradient row a of one staged element, as the kernel's consumer lanes form them
-template <int ORDER, int NQ>
+// block (a, b) and gradient row a of one staged element, as the kernel's
+// consumer lanes form them (lane (b, mm) forms block row mm)
+template <int ORDER, int NQ, int NPART>
 static void consume_item(const double *sp, int a, int b, double *blk, double *grow)
 {
-    using S = Stage<ORDER, NQ>;
+    using S = StageT<ORDER, NQ, NPART>;
     constexpr int NB = NLocal<ORDER>::value;
     for (int i = 0; i < 9; ++i) blk[i] = 0;
     for (int q = 0; q < NQ; ++q) {
-        const double *ka = sp + S::K + (q * NB + a) * 3, *kb = sp + S::K + (q * NB + b) * 3;
-        const double cl = sp[S::C + q], cc = sp[S::C + NQ + q];
+        const double *ka = sp + S::K + (q * NB + a) * 4, *kb = sp + S::K + (q * NB + b) * 4;
+        const double cl = sp[S::C + 2 * q], cc = sp[S::C + 2 * q + 1];
         for (int mm = 0; mm < 3; ++mm)
             for (int nn = 0; nn < 3; ++nn) blk[mm * 3 + nn] += cl * ka[mm] * kb[nn] + cc * kb[mm] * ka[nn];
     }
@@ -135,29 +150,36 @@ static void consume_item(const double *sp, int a, int b, double *blk, double *gr
     for (int d = 0; d < 3; ++d) blk[d * 4] += sab;
     for (int d = 0; d < 3; ++d) {
         double gs = 0;
-        for (int q = 0; q < NQ; ++q)
-            for (int k = 0; k < 3; ++k) gs += RG<ORDER, NQ>(q, a, k) * sp[S::Q + q * 9 + d * 3 + k];
+        for (int h = 0; h < NPART; ++h) gs += sp[S::R + (h * NB + a) * 4 + d];
         grow[d] = gs;
     }
 }
 
+// a staging record before its points are staged: NaN except the gradient sums
+template <int ORDER, int NQ, int NPART> static void clear_record(double *s)
+{
+    using S = StageT<ORDER, NQ, NPART>;
+    for (int i = 0; i < S::SIZE; ++i) s[i] = i >= S::R && i < S::G ? 0.0 : std::nan("");
+}
+
 // direct assembly of the same quantities, element by element
-template <int ORDER, int NQ>
+template <int ORDER, int NQ, int NPART>
 static void direct(const Mesh &m, const Pattern &pat, std::vector<double> &vals, std::vector<double> &g)
 {
     constexpr int NB = NLocal<ORDER>::value;
     vals.assign(size_t(pat.nnz), 0.0);
     g.assign(size_t(m.N) * 3, 0.0);
-    std::vector<double> s(Stage<ORDER, NQ>::SIZE);
+    std::vector<double> s(StageT<ORDER, NQ, NPART>::SIZE);
     for (int e = 0; e < m.E; ++e) {
-        for (int q = 0; q < NQ; ++q) stage_point<ORDER, NQ>(m, e, q, s.data());
+        clear_record<ORDER, NQ, NPART>(s.data());
+        for (int q = 0; q < NQ; ++q) stage_point<ORDER, NQ, NPART>(m, e, q, s.data());
         for (int a = 0; a < NB; ++a) {
             const int i = m.en[e * NB + a];
             const int deg = pat.brow_ptr[i + 1] - pat.brow_ptr[i];
             for (int b = 0; b < NB; ++b) {
                 const int j = m.en[e * NB + b];
                 double blk[9], grow[3];
-                consume_item<ORDER, NQ>(s.data(), a, b, blk, grow);
+                consume_item<ORDER, NQ, NPART>(s.data(), a, b, blk, grow);
                 if (b == 0)
                     for (int d = 0; d < 3; ++d) g[3 * size_t(i) + d] += grow[d];
                 int pos = -1;
@@ -170,10 +192,12 @@ static void direct(const Mesh &m, const Pattern &pat, std::vector<double> &vals,
     }
 }
 
-template <int ORDER, int NQ>
+template <int ORDER, int NQ, int NPART>
 static int run(const Mesh &m, int groups, int chunk, int item_cap, int acc_capacity, int max_rows)
 {
-    constexpr int NB = NLocal<ORDER>::value, SE = Stage<ORDER, NQ>::SIZE;
+    using S = StageT<ORDER, NQ, NPART>;
+    constexpr int NB = NLocal<ORDER>::value, SE = S::SIZE;
+    CHECK(SE % 8 == 4, "record length 4 mod 8 words");
     Pattern pat = build_pattern(m.E, NB, m.en.data(), m.N, m.Nv, m.ev.data());
     FusedShape s{groups, chunk, item_cap, acc_capacity, max_rows};
     FusedTables t = build_fused_tables(pat, NB, m.en.data(), m.E, m.vx.data(), m.ev.data(), s);
@@ -203,7 +227,12 @@ static int run(const Mesh &m, int groups, int chunk, int item_cap, int acc_capac
             CHECK(t.row_acc[row0 + r] + 9 * deg <= nacc, "row acc bound");
             if (r > 0) CHECK(t.row_acc[row0 + r] == t.row_acc[row0 + r - 1] + 9 * (pat.brow_ptr[t.rows[row0 + r - 1] + 1] - pat.brow_ptr[t.rows[row0 + r - 1]]), "row acc contiguous");
         }
-        for (int k = 1; k < nel; ++k) CHECK((t.elems[el0 + k] & 0x7fffffff) > (t.elems[el0 + k - 1] & 0x7fffffff), "elements sorted");
+        {
+            std::vector<int> pe;
+            for (int k = 0; k < nel; ++k) pe.push_back(t.elems[el0 + k] & 0x7fffffff);
+            std::sort(pe.begin(), pe.end());
+            for (int k = 1; k < nel; ++k) CHECK(pe[k] > pe[k - 1], "patch %d lists an element twice", p);
+        }
         // replay
         std::vector<double> acc(nacc, 0.0), gacc(3 * size_t(nrows), 0.0), stg(size_t(chunk) * SE, std::nan(""));
         for (int c = 0; c < nch; ++c) {
@@ -214,7 +243,9 @@ static int run(const Mesh &m, int groups, int chunk, int item_cap, int acc_capac
             for (int kk = 0; kk < cn; ++kk) {
                 const int e = t.elems[el0 + cbeg + kk] & 0x7fffffff;
                 CHECK(e >= 0 && e < m.E, "element id");
-                for (int q = 0; q < NQ; ++q) stage_point<ORDER, NQ>(m, e, q, &stg[size_t(kk) * SE]);
+                for (int a = 0; a < NB; ++a) CHECK(t.vnodes[size_t(el0 + cbeg + kk) * NB + a] == m.en[size_t(e) * NB + a], "visit node list");
+                clear_record<ORDER, NQ, NPART>(&stg[size_t(kk) * SE]);
+                for (int q = 0; q < NQ; ++q) stage_point<ORDER, NQ, NPART>(m, e, q, &stg[size_t(kk) * SE]);
             }
             // within a chunk every row is accumulated by exactly one lane group
             std::vector<int> row_group(nrows, -1);
@@ -240,23 +271,21 @@ static int run(const Mesh &m, int groups, int chunk, int item_cap, int acc_capac
                     const int deg = pat.brow_ptr[node + 1] - pat.brow_ptr[node];
                     const double *sp = &stg[size_t(kk) * SE];
                     for (int b = 0; b < NB; ++b) {
-                        const int pos = t.pos[size_t(it) * NB + b];
+                        const int pos = t.pos[size_t(it) * pos_stride(NB) + b];
                         CHECK(pos < deg && pat.bcol[pat.brow_ptr[node] + pos] == m.en[e * NB + b], "column position");
                         double blk[9], grow[3];
-                        consume_item<ORDER, NQ>(sp, a, b, blk, grow);
+                        consume_item<ORDER, NQ, NPART>(sp, a, b, blk, grow);
                         for (int mm = 0; mm < 3; ++mm)
                             for (int nn = 0; nn < 3; ++nn) {
                                 const int ai = t.row_acc[row0 + r] + 3 * pos + mm * 3 * deg + nn;
                                 CHECK(ai >= t.row_acc[row0 + r] && ai < t.row_acc[row0 + r] + 9 * deg, "acc index in row segment");
                                 acc[ai] += blk[mm * 3 + nn];
                             }
-                        if (b < 3) gacc[3 * size_t(r) + b] += grow[b];
+                        if (b == 0)
+                            for (int d = 0; d < 3; ++d) gacc[3 * size_t(r) + d] += grow[d];
                     }
-                    for (int i = 0; i < Stage<ORDER, NQ>::G + 6; ++i)
-                        CHECK(!std::isnan(sp[i]) || (i >= Stage<ORDER, NQ>::C + 2 * NQ && i < Stage<ORDER, NQ>::Q) ||
-                                  (i >= Stage<ORDER, NQ>::Q + 9 * NQ && i < Stage<ORDER, NQ>::G) ||
-                                  (i >= NQ * NB * 3 && i < Stage<ORDER, NQ>::C),
-                              "staging slot read before written");
+                    for (int i = 0; i < S::G + 8; ++i)
+                        CHECK(!std::isnan(sp[i]) || (i >= S::C + 2 * NQ && i < S::R), "staging slot read before written");
                 }
             }
         }
@@ -270,7 +299,7 @@ static int run(const Mesh &m, int groups, int chunk, int item_cap, int acc_capac
     for (int i = 0; i < m.N; ++i) CHECK(owner[i] >= 0, "node %d not owned", i);
     for (size_t k = 0; k < seen_pairs.size(); ++k) CHECK(seen_pairs[k] == 1, "pair (e,a)=%zu scheduled %ld times", k, seen_pairs[k]);
     std::vector<double> rv, rg;
-    direct<ORDER, NQ>(m, pat, rv, rg);
+    direct<ORDER, NQ, NPART>(m, pat, rv, rg);
     double ev = 0, sv = 0, eg = 0, sg = 0;
     for (size_t i = 0; i < rv.size(); ++i) { ev = std::max(ev, std::fabs(vals[i] - rv[i])); sv = std::max(sv, std::fabs(rv[i])); }
     for (size_t i = 0; i < rg.size(); ++i) { eg = std::max(eg, std::fabs(grad[i] - rg[i])); sg = std::max(sg, std::fabs(rg[i])); }
@@ -294,17 +323,42 @@ template <typename T, int NQ> static void check_producer_map()
     CHECK(WA < NWAVE, "consumers left");
 }
 
+// consumer lane groups fit a wavefront and the LDS copy of the item tables is
+// whole words
+template <typename T, int ORDER, int NQ> static void check_groups()
+{
+    constexpr int NB = NLocal<ORDER>::value, GPW = groups_per_wave<ORDER>(), GL = group_lanes<ORDER>();
+    constexpr int CH = chunk_elems<T, NQ>(), IC = item_cap<T, ORDER, NQ>();
+    using S = Stage<T, ORDER, NQ>;
+    CHECK(GPW * GL <= 64 && GPW >= 1, "groups per wave");
+    CHECK(pos_stride(NB) >= NB && (pos_stride(NB) == 4 || pos_stride(NB) == 16), "pos stride");
+    CHECK(CH <= 1024, "element-in-chunk field is 10 bits");
+    CHECK(IC >= CH * NB, "item cap below chunk x n_local");
+    CHECK(S::SIZE % 8 == 4 && S::C % 4 == 0 && S::R % 4 == 0 && S::G % 4 == 0, "staging sections 16-byte aligned");
+    constexpr int NPART = grad_parts<T, NQ>();
+    CHECK(NPART >= 1 && NQ % NPART == 0, "gradient partial sums");
+}
+
 int main(int argc, char **argv)
 {
     check_producer_map<float, 1>();
     check_producer_map<float, 4>();
     check_producer_map<double, 1>();
     check_producer_map<double, 4>();
+    check_groups<float, 1, 1>();
+    check_groups<float, 2, 4>();
+    check_groups<double, 1, 1>();
+    check_groups<double, 2, 4>();
     if (argc < 7) { std::fprintf(stderr, "usage: fused_check mesh.bin groups chunk item_cap acc_capacity max_rows\n"); return 2; }
     Mesh m = read_mesh(argv[1]);
     const int groups = std::atoi(argv[2]), chunk = std::atoi(argv[3]), icap = std::atoi(argv[4]);
     const int cap = std::atoi(argv[5]), max_rows = std::atoi(argv[6]);
-    int f = m.order == 1 ? run<1, 1>(m, groups, chunk, icap, cap, max_rows) : run<2, 4>(m, groups, chunk, icap, cap, max_rows);
+    // partial gradient sums per element, as the kernel's producers leave them
+    const int np = chunk >= 64 ? 4 : std::max(1, 4 * chunk / 64);
+    int f = m.order == 1 ? run<1, 1, 1>(m, groups, chunk, icap, cap, max_rows)
+            : np == 1    ? run<2, 4, 1>(m, groups, chunk, icap, cap, max_rows)
+            : np == 2    ? run<2, 4, 2>(m, groups, chunk, icap, cap, max_rows)
+                         : run<2, 4, 4>(m, groups, chunk, icap, cap, max_rows);
     std::printf(f ? "FAIL (%d)\n" : "OK\n", f);
     return f ? 1 : 0;
 }

@@ -41,12 +41,12 @@ def write_mesh(path, m, x, lam, mu):
 
 
 @pytest.mark.parametrize("order,grid,groups,chunk,icap,cap", [
-    (2, (3, 3, 4), 72, 64, 512, 4000),       # several patches
-    (2, (4, 3, 3), 72, 16, 40, 12000),       # several chunks per patch, item-capped chunks
-    (2, (10, 10, 10), 72, 64, 512, 23000),   # the FP32 P2 launch shape
-    (2, (6, 6, 6), 84, 32, 256, 11000),      # the FP64 P2 launch shape
-    (1, (6, 5, 4), 192, 256, 1024, 3000),
-    (1, (12, 12, 12), 192, 256, 1024, 20000),  # the FP32 P1 launch shape
+    (2, (3, 3, 4), 28, 32, 320, 4000),       # several patches
+    (2, (4, 3, 3), 28, 8, 40, 12000),        # several chunks per patch, item-capped chunks
+    (2, (10, 10, 10), 28, 32, 320, 21000),   # the FP32 P2 launch shape
+    (2, (6, 6, 6), 30, 16, 160, 10400),      # the FP64 P2 launch shape
+    (1, (6, 5, 4), 70, 128, 512, 3000),
+    (1, (12, 12, 12), 70, 128, 512, 27000),  # the FP32 P1 launch shape
 ])
 def test_fused_tables(exe, tmp_path, order, grid, groups, chunk, icap, cap):
     m = kuhn_box(*grid, order=order)

@@ -1,0 +1,165 @@
+"""GPU tests of the solves that call the assembly: Newton-CG on the free dofs
+and the static adjoint (runs on an MI355X).
+
+The exact Newton step of the oracle (a sparse direct solve of the FP64
+oracle Hessian restricted to the free dofs) pins the Newton-CG step; central
+finite differences of the objective through full nonlinear solves pin the
+adjoint gradient.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+import scipy.sparse.linalg as spla
+
+from oracle import pyoracle as po
+from polyfem_amd import NewtonSolver, NLAssembler, NLProblem, cg_solve, solve_static_adjoint
+from polyfem_amd.mesh import BASE_SEED, face_nodes, kuhn_box, random_lame, smooth_displacement
+
+pytestmark = pytest.mark.gpu
+MAT = {"NeoHookean": po.NEOHOOKEAN, "LinearElasticity": po.LINEAR_ELASTICITY}
+
+
+def clamped_box(order, grid=(3, 3, 3), pull=0.1):
+    """Bottom face fixed, top face moved by (0, 0, pull) (the small-box Dirichlet setup)."""
+    m = kuhn_box(*grid, order=order)
+    bot, top = face_nodes(m, 2, 0.0), face_nodes(m, 2, 1.0)
+    fixed = np.concatenate([3 * bot[:, None] + np.arange(3), 3 * top[:, None] + np.arange(3)]).ravel()
+    vals = np.concatenate([np.zeros(3 * bot.size), np.tile([0.0, 0.0, pull], top.size)])
+    return m, NLProblem(m.n_dofs, fixed, vals)
+
+
+def rel(a, b):
+    return float(np.max(np.abs(np.asarray(a) - np.asarray(b))) / max(np.max(np.abs(np.asarray(b))), 1e-300))
+
+
+@pytest.mark.parametrize("order,name", [(1, "NeoHookean"), (2, "NeoHookean"), (1, "LinearElasticity")])
+def test_newton_step_is_exact_newton_step(order, name):
+    m, prob = clamped_box(order)
+    lam, mu = random_lame(m.n_elements, seed=BASE_SEED + 3)
+    x0 = smooth_displacement(m.nodes, 0.02, 1e-3, BASE_SEED + 4)
+    x0[prob.fixed] = prob.fixed_values
+    print(f"newton step order={order} {name} dofs={m.n_dofs}", flush=True)
+    a = NLAssembler(name, m, lame=(lam, mu), precision="fp64")
+    x1, rep = NewtonSolver(max_iterations=1, grad_norm=0.0, cg_max_iterations=20000, cg_tolerance=1e-13,
+                           line_search="None").minimize(a, x0, prob)
+    assert rep.iterations == 1 and rep.last_alpha == 1.0
+    o = po.Oracle(m, lam, mu, material=MAT[name])
+    H = o.hessian_scipy(x0).tocsr()[prob.free][:, prob.free]
+    g = o.gradient(x0)[prob.free]
+    ref = x0.copy()
+    ref[prob.free] += spla.spsolve(H.tocsc(), -g)
+    assert np.array_equal(x1[prob.fixed], prob.fixed_values)
+    assert rel(x1 - x0, ref - x0) < 1e-8
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("path", ["fused", "plain"])
+def test_newton_converges(precision, path):
+    m, prob = clamped_box(2, (2, 2, 3))
+    lam, mu = random_lame(m.n_elements, seed=BASE_SEED + 5)
+    x0 = prob.reduced_to_full(np.zeros(prob.free.size))
+    o = po.Oracle(m, lam, mu)
+    g0 = np.linalg.norm(o.gradient(x0)[prob.free])
+    # FP32 element kernels leave ~1e-6 relative noise in the gradient
+    rtol = 1e-10 if precision == "fp64" else 1e-5
+    print(f"newton converge {precision} {path} |g0|={g0:.3e}", flush=True)
+    a = NLAssembler("NeoHookean", m, lame=(lam, mu), precision=precision, path=path)
+    x, rep = NewtonSolver(max_iterations=25, grad_norm=rtol * g0, cg_max_iterations=4000,
+                          cg_tolerance=1e-10 if precision == "fp64" else 1e-6).minimize(a, x0, prob)
+    assert rep.converged, rep
+    assert np.linalg.norm(o.gradient(x)[prob.free]) <= (rtol if precision == "fp64" else 10 * rtol) * g0
+    assert o.energy(x) < o.energy(x0)
+
+
+def test_newton_rejects_inverting_steps():
+    # a large pull makes full Newton steps invert elements; the line search
+    # must halve them (non-finite energies are invalid steps)
+    m, prob = clamped_box(1, (2, 2, 2), pull=0.6)
+    a = NLAssembler("NeoHookean", m, young=1e6, poisson=0.4, precision="fp64")
+    x, rep = NewtonSolver(max_iterations=40, grad_norm=1e-6, cg_tolerance=1e-10).minimize(a, np.zeros(m.n_dofs), prob)
+    assert np.isfinite(rep.energy) and rep.converged
+    assert np.isfinite(a.assemble_energy(x))
+
+
+def test_cg_graph_replays_match_eager():
+    torch = pytest.importorskip("torch")
+    from polyfem_amd import _ffi
+    m, prob = clamped_box(2, (2, 2, 2))
+    lam, mu = random_lame(m.n_elements, seed=BASE_SEED + 6)
+    x = smooth_displacement(m.nodes, 0.02, 1e-3, BASE_SEED + 7)
+    a = NLAssembler("NeoHookean", m, lame=(lam, mu), precision="mixed")
+    rng = np.random.default_rng(BASE_SEED + 8)
+    dev = torch.device("cuda:0")
+    dx = torch.tensor(x, device=dev)
+    dm = torch.tensor(prob.mask, device=dev)
+    out = {}
+    for trial in range(3):
+        b = rng.normal(size=m.n_dofs)
+        db = torch.tensor(b, device=dev)
+        for graph in (0, 1, 1, 0, 1):
+            y = torch.zeros(m.n_dofs, dtype=torch.float64, device=dev)
+            it, rr = C.c_int32(0), C.c_double(0)
+            _ffi.check(_ffi.lib().pfgpu_cg_solve_dev(a._h, C.c_void_p(dx.data_ptr()), C.c_void_p(dm.data_ptr()),
+                                                     C.c_void_p(db.data_ptr()), C.c_void_p(y.data_ptr()), 300, 0.0,
+                                                     graph, C.byref(it), C.byref(rr)))
+            _ffi.check(_ffi.lib().pfgpu_synchronize(a._h))
+            yy = y.cpu().numpy()
+            dx.add_(0.0)  # eager work of the program between replays
+            torch.cuda.synchronize()
+            if graph == 0 and (trial, 0) not in out:
+                out[(trial, 0)] = yy
+            assert it.value == 300
+            assert np.array_equal(yy, out[(trial, 0)]), f"trial {trial} graph={graph} differs from eager"
+
+
+def test_static_adjoint_residual():
+    m, prob = clamped_box(2, (2, 2, 3))
+    lam, mu = random_lame(m.n_elements, seed=BASE_SEED + 9)
+    u = smooth_displacement(m.nodes, 0.02, 1e-3, BASE_SEED + 10)
+    u[prob.fixed] = prob.fixed_values
+    a = NLAssembler("NeoHookean", m, lame=(lam, mu), precision="fp64")
+    rhs = np.random.default_rng(BASE_SEED + 11).normal(size=m.n_dofs)
+    p = solve_static_adjoint(a, u, rhs, prob, tolerance=1e-12)
+    assert np.all(p[prob.fixed] == 0.0)
+    H = po.Oracle(m, lam, mu).hessian_scipy(u).tocsr()[prob.free][:, prob.free]
+    r = H @ p[prob.free] - rhs[prob.free]
+    assert np.linalg.norm(r) <= 1e-9 * np.linalg.norm(rhs[prob.free])
+
+
+def test_adjoint_material_gradient_matches_finite_differences():
+    """dJ/d(lambda, mu) of J(u(theta)) = 1/2 |u_S - u*|^2 with u(theta) the
+    static equilibrium: adjoint H p = dJ/du, dJ/dtheta = -p^T dg/dtheta
+    (the material VJP), checked by central differences of J through full
+    Newton solves."""
+    m, prob = clamped_box(1, (2, 2, 3), pull=0.05)
+    lam, mu = random_lame(m.n_elements, seed=BASE_SEED + 12)
+    S = np.flatnonzero(np.abs(m.nodes[:, 2] - 1.0 / 3.0) < 1e-9)
+    Sd = (3 * S[:, None] + np.arange(3)).ravel()
+    target = np.random.default_rng(BASE_SEED + 13).normal(size=Sd.size) * 1e-3
+    solver = NewtonSolver(max_iterations=30, grad_norm=1e-9, cg_max_iterations=5000, cg_tolerance=1e-12)
+
+    def solve(lm, mm):
+        a = NLAssembler("NeoHookean", m, lame=(lm, mm), precision="fp64")
+        u, rep = solver.minimize(a, np.zeros(m.n_dofs), prob)
+        assert rep.converged
+        return a, u
+
+    def J(u):
+        return 0.5 * np.sum((u[Sd] - target) ** 2)
+
+    a, u = solve(lam, mu)
+    dJdu = np.zeros(m.n_dofs)
+    dJdu[Sd] = u[Sd] - target
+    p = solve_static_adjoint(a, u, dJdu, prob, tolerance=1e-13)
+    grad = a.force_material_derivative(u, p)
+    rng = np.random.default_rng(BASE_SEED + 14)
+    for _ in range(3):
+        dl = rng.normal(size=m.n_elements) * 0.05 * lam
+        dm = rng.normal(size=m.n_elements) * 0.05 * mu
+        h = 1e-4
+        jp = J(solve(lam + h * dl, mu + h * dm)[1])
+        jm = J(solve(lam - h * dl, mu - h * dm)[1])
+        fd = (jp - jm) / (2 * h)
+        lin = grad[:m.n_elements] @ dl + grad[m.n_elements:] @ dm
+        assert abs(fd - lin) <= 1e-5 * max(abs(lin), 1e-12) + 1e-14, (fd, lin)
