@@ -53,9 +53,11 @@ typedef enum { PFGPU_NEOHOOKEAN = 0, PFGPU_LINEAR_ELASTICITY = 1 } pfgpu_materia
  * values are float. */
 typedef enum { PFGPU_FP64 = 0, PFGPU_MIXED = 1 } pfgpu_precision;
 
-/* Assembly algorithm.  FUSED: patch-owned rows, on-chip deterministic
- * reduction (the default).  PLAIN: element-local results written to HBM, then
- * gathered into the global vector / CSR in a fixed order. */
+/* Assembly algorithm.  PLAIN (the default): element-local results written
+ * to HBM, then gathered into the global vector / CSR in a fixed order.
+ * FUSED: patch-owned rows accumulated on chip (LDS) in a fixed order, one
+ * persistent launch.  Both are deterministic and give the same results to
+ * rounding; PLAIN is the faster one on MI355X at the benchmarked sizes. */
 typedef enum { PFGPU_PATH_FUSED = 0, PFGPU_PATH_PLAIN = 1 } pfgpu_path;
 
 typedef struct {
@@ -131,6 +133,13 @@ pfgpu_status pfgpu_material_vjp(pfgpu_handle *h, const double *x, const double *
 /* ElasticForceDerivative::force_shape_derivative (ElasticForceDerivative.cpp:135-264),
  * out has 3 n_vertices entries, vertex*3 + d */
 pfgpu_status pfgpu_shape_vjp(pfgpu_handle *h, const double *x, const double *p, double *out);
+/* explicit derivative of the elastic energy w.r.t. the per-element Lame
+ * parameters, out[e] = dE/dlambda_e, out[E+e] = dE/dmu_e (the energy density
+ * is linear in (lambda, mu): NeoHookean 1/2 ln^2 J and
+ * 1/2 (|F|^2 - 3 - 2 ln J), NeoHookeanElasticity.cpp:384; the term a
+ * material objective that contains the elastic energy adds to the adjoint
+ * terms) */
+pfgpu_status pfgpu_energy_lame_gradient(pfgpu_handle *h, const double *x, double *out);
 /* mixed derivative y = (d grad / d lambda) dlambda + (d grad / d mu) dmu,
  * the forward counterpart of pfgpu_material_vjp (out = -(that)^T p) */
 pfgpu_status pfgpu_material_jvp(pfgpu_handle *h, const double *x, const double *dlambda,
@@ -176,6 +185,9 @@ typedef struct {
 typedef struct {
     int32_t iterations, cg_iterations, line_search_halvings, converged;
     double energy, grad_norm, last_alpha;
+    /* steps that fell back to steepest descent (Cauchy step) after the
+     * Newton direction was rejected by the line search */
+    int32_t gradient_steps;
 } pfgpu_newton_report;
 
 /* x: n_dofs values, the initial guess with the Dirichlet values already on

@@ -44,7 +44,7 @@ class NewtonParams(C.Structure):
 class NewtonReport(C.Structure):
     _fields_ = [("iterations", C.c_int32), ("cg_iterations", C.c_int32), ("line_search_halvings", C.c_int32),
                 ("converged", C.c_int32), ("energy", C.c_double), ("grad_norm", C.c_double),
-                ("last_alpha", C.c_double)]
+                ("last_alpha", C.c_double), ("gradient_steps", C.c_int32)]
 
 
 class TransientParams(C.Structure):
@@ -91,6 +91,7 @@ _SIGNATURES = {
     "pfgpu_cg_solve_dev": [vp, vp, vp, vp, vp, C.c_int32, C.c_double, C.c_int32, C.POINTER(C.c_int32),
                            C.POINTER(C.c_double)],
     "pfgpu_set_load": [vp, vp],
+    "pfgpu_energy_lame_gradient": [vp, vp, vp],
     "pfgpu_set_density": [vp, vp, C.c_double],
     "pfgpu_mass_matvec": [vp, vp, vp],
     "pfgpu_mass_matvec_dev": [vp, vp, vp],

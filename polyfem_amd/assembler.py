@@ -42,7 +42,7 @@ class NLAssembler:
     """Elasticity assembler on one tet mesh (P1 or P2, P1 geometry)."""
 
     def __init__(self, name: str, mesh, lame=None, young: float = None, poisson: float = None,
-                 precision: str = "fp64", path: str = "fused", quadrature_order: int = 0):
+                 precision: str = "fp64", path: str = "plain", quadrature_order: int = 0):
         if name not in _NAMES:
             raise ValueError(f"unknown material {name!r}; expected one of {sorted(_NAMES)}")
         self._name = name
@@ -172,6 +172,14 @@ class NLAssembler:
         p = _f64(adjoint, self.n_dofs)
         out = np.empty(2 * self.n_elements)
         check(lib().pfgpu_material_vjp(self._h, _ptr(x), _ptr(p), _ptr(out)))
+        return out
+
+    def energy_lame_gradient(self, x) -> np.ndarray:
+        """dE/dlambda_e (first E entries) and dE/dmu_e (last E), the explicit
+        dependence of the elastic energy on the per-element Lame parameters."""
+        x = _f64(x, self.n_dofs)
+        out = np.empty(2 * self.n_elements)
+        check(lib().pfgpu_energy_lame_gradient(self._h, _ptr(x), _ptr(out)))
         return out
 
     def force_shape_derivative(self, x, adjoint) -> np.ndarray:

@@ -150,7 +150,7 @@ pfgpu_handle::~pfgpu_handle()
 int64_t pfgpu_handle::device_bytes() const
 {
     const DevBuf *bufs[] = {&conn, &vconn, &geom, &lame, &node_ptr, &node_inc, &vert_ptr, &vert_inc,
-                            &brow_ptr, &block_row, &contrib_ptr, &contrib, &patch_desc, &patch_rows,
+                            &brow_ptr, &block_row, &contrib_ptr, &contrib, &row_chunk, &blk_chunk, &patch_desc, &patch_rows,
                             &patch_row_acc, &patch_elems, &patch_cs, &patch_cg, &patch_hdr, &patch_pos, &patch_vnodes,
                             &d_outer, &d_inner, &loc, &hloc, &partial, &red_out,
                             &h_x, &h_v, &h_out, &h_out2, &h_vals, &sv, &h_mask, &mass, &pat_bcol, &tr_fixed, &tr_f, &tr_seg, &load};
@@ -188,7 +188,7 @@ pfgpu_status pfgpu_create(const pfgpu_mesh_desc *mesh, const pfgpu_material_desc
         h->n_quad = qo == 1 ? 1 : 4;
         h->material = material->kind;
         h->precision = precision;
-        h->path = PFGPU_PATH_FUSED;
+        h->path = PFGPU_PATH_PLAIN;
         h->n_elements = m.n_elements;
         h->n_nodes = m.n_nodes;
         h->n_vertices = m.n_vertices;
@@ -420,6 +420,18 @@ pfgpu_status pfgpu_material_vjp(pfgpu_handle *h, const double *x, const double *
         const double *dp = stage(*h, h->h_v, p, h->n_dofs);
         h->h_out.reserve(sizeof(double) * 2 * size_t(h->n_elements));
         element_material_vjp(*h, dx, dp, h->h_out.as<double>());
+        fetch(*h, out, h->h_out.get(), sizeof(double) * 2 * size_t(h->n_elements));
+    });
+}
+
+pfgpu_status pfgpu_energy_lame_gradient(pfgpu_handle *h, const double *x, double *out)
+{
+    return guard([&] {
+        check_handle(h);
+        require(x && out, "null argument");
+        const double *dx = stage(*h, h->h_x, x, h->n_dofs);
+        h->h_out.reserve(sizeof(double) * 2 * size_t(h->n_elements));
+        energy_lame_gradient(*h, dx, h->h_out.as<double>());
         fetch(*h, out, h->h_out.get(), sizeof(double) * 2 * size_t(h->n_elements));
     });
 }

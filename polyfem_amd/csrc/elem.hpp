@@ -68,6 +68,30 @@ template <int ORDER, int NQ> __host__ __device__ constexpr double RGG(int a, int
     return s;
 }
 
+// The same reference quantities as tables, for loops whose node indices are
+// runtime values uniform over a wavefront (then the loads are scalar):
+//   rg[(q NB + a) 3 + k] = RG(q, a, k)
+//   rgg[(a NB + b) 6 + i] = coefficient of metric entry i (00 11 22 01 02 12)
+//                           in sum_q G_a(q).G_b(q)
+template <typename T, int ORDER, int NQ> struct RefTab {
+    static constexpr int NB = NLocal<ORDER>::value;
+    T rg[NQ * NB * 3];
+    T rgg[NB * NB * 6];
+    constexpr RefTab() : rg(), rgg()
+    {
+        for (int q = 0; q < NQ; ++q)
+            for (int a = 0; a < NB; ++a)
+                for (int k = 0; k < 3; ++k) rg[(q * NB + a) * 3 + k] = T(RG<ORDER, NQ>(q, a, k));
+        const int kk[6] = {0, 1, 2, 0, 0, 1}, ll[6] = {0, 1, 2, 1, 2, 2};
+        for (int a = 0; a < NB; ++a)
+            for (int b = 0; b < NB; ++b)
+                for (int i = 0; i < 6; ++i)
+                    rgg[(a * NB + b) * 6 + i] =
+                        T(i < 3 ? RGG<ORDER, NQ>(a, b, kk[i], ll[i])
+                                : RGG<ORDER, NQ>(a, b, kk[i], ll[i]) + RGG<ORDER, NQ>(a, b, ll[i], kk[i]));
+    }
+};
+
 // ------------------------------------------------------------ 3x3 helpers
 
 template <typename T> __device__ __forceinline__ T det3(const T *m)

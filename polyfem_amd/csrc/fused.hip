@@ -223,6 +223,9 @@ __global__ __launch_bounds__(NT) void k_egh(DevMesh<T> m, const double *__restri
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const bool producer = wave < WA;
+    // the producers are the critical path of every chunk and share their
+    // SIMDs with consumer waves: they issue first
+    if (producer) __builtin_amdgcn_s_setprio(2);
 
     const RGTab<ORDER, NQ> &tb = tab<ORDER, NQ>();
     // reference-gradient tables padded to 4 and 8 entries for vector loads

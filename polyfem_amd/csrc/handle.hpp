@@ -7,7 +7,7 @@
 
 struct pfgpu_handle {
     int order = 1, n_local = 4, n_quad = 1, material = PFGPU_NEOHOOKEAN;
-    int precision = PFGPU_FP64, path = PFGPU_PATH_FUSED;
+    int precision = PFGPU_FP64, path = PFGPU_PATH_PLAIN;
     int n_elements = 0, n_nodes = 0, n_vertices = 0, n_dofs = 0;
     hipStream_t stream = nullptr;
     int device = 0;
@@ -22,6 +22,9 @@ struct pfgpu_handle {
     // plain-path Hessian maps
     bool plain_hess_ready = false;
     pfgpu::DevBuf brow_ptr, block_row, contrib_ptr, contrib;
+    // chunks of the segmented gathers (node rows; pattern blocks)
+    pfgpu::DevBuf row_chunk, blk_chunk;
+    int n_row_chunks = 0, n_blk_chunks = 0;
     // fused-path patch tables (built on first use)
     bool fused_ready = false;
     pfgpu::DevBuf patch_desc, patch_rows, patch_row_acc, patch_elems, patch_cs, patch_cg, patch_hdr, patch_pos, patch_vnodes;
@@ -83,6 +86,7 @@ void plain_hessian(Handle &h, const double *x, void *values);
 void plain_hvp(Handle &h, const double *x, const double *v, double *y);
 void plain_material_jvp(Handle &h, const double *x, const double *dl, const double *dm, double *y);
 void element_material_vjp(Handle &h, const double *x, const double *p, double *out);
+void energy_lame_gradient(Handle &h, const double *x, double *out);
 void plain_shape_vjp(Handle &h, const double *x, const double *p, double *out);
 void ensure_plain_hessian_maps(Handle &h);
 

@@ -23,6 +23,31 @@ __device__ __forceinline__ double block_sum(double v)
     return s[0];
 }
 
+// reference tables in constant memory, one per (value type, order, points)
+__constant__ RefTab<float, 1, 1> c_rf11 = RefTab<float, 1, 1>();
+__constant__ RefTab<float, 1, 4> c_rf14 = RefTab<float, 1, 4>();
+__constant__ RefTab<float, 2, 1> c_rf21 = RefTab<float, 2, 1>();
+__constant__ RefTab<float, 2, 4> c_rf24 = RefTab<float, 2, 4>();
+__constant__ RefTab<double, 1, 1> c_rd11 = RefTab<double, 1, 1>();
+__constant__ RefTab<double, 1, 4> c_rd14 = RefTab<double, 1, 4>();
+__constant__ RefTab<double, 2, 1> c_rd21 = RefTab<double, 2, 1>();
+__constant__ RefTab<double, 2, 4> c_rd24 = RefTab<double, 2, 4>();
+
+template <typename T, int ORDER, int NQ> __device__ __forceinline__ const RefTab<T, ORDER, NQ> &ref_tab()
+{
+    if constexpr (sizeof(T) == 4) {
+        if constexpr (ORDER == 1 && NQ == 1) return c_rf11;
+        else if constexpr (ORDER == 1) return c_rf14;
+        else if constexpr (NQ == 1) return c_rf21;
+        else return c_rf24;
+    } else {
+        if constexpr (ORDER == 1 && NQ == 1) return c_rd11;
+        else if constexpr (ORDER == 1) return c_rd14;
+        else if constexpr (NQ == 1) return c_rd21;
+        else return c_rd24;
+    }
+}
+
 // ------------------------------------------------------------ element kernels
 
 template <typename T, int ORDER, int NQ, int MAT>
@@ -49,6 +74,32 @@ __global__ __launch_bounds__(BLOCK) void k_energy(DevMesh<T> m, const double *__
     }
     const double s = block_sum(val);
     if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+// explicit derivative of the element energy w.r.t. its Lame pair: the energy
+// density is linear in (lambda, mu), so out[e] = sum_q w psi(F; 1, 0) and
+// out[E + e] = sum_q w psi(F; 0, 1)
+template <typename T, int ORDER, int NQ, int MAT>
+__global__ __launch_bounds__(BLOCK) void k_energy_lame_grad(DevMesh<T> m, const double *__restrict__ x,
+                                                            double *__restrict__ out)
+{
+    const int e = blockIdx.x * BLOCK + threadIdx.x;
+    if (e >= m.n_elements) return;
+    Elem<T, ORDER, NQ> el;
+    load_geometry(m, e, el);
+    gather(m, e, x, el.U);
+    const T w = el.det * T(quad_weight(NQ));
+    T al = T(0), am = T(0);
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        Kin<T> k;
+        field_grad<T, ORDER, NQ>(q, el.U, el.jit, k.H);
+        kinematics(k);
+        al += energy_density<T, MAT>(k, T(1), T(0)) * w;
+        am += energy_density<T, MAT>(k, T(0), T(1)) * w;
+    }
+    out[e] = double(al);
+    out[m.n_elements + e] = double(am);
 }
 
 // local gradient (+ energy partials); gloc[(e*NB + a)*3 + d]
@@ -133,19 +184,32 @@ __global__ __launch_bounds__(BLOCK) void k_hessian(DevMesh<T> m, const double *_
     const T cm = w * el.mu;
     T *st = stage[wv];
     T *out = hloc + tile * NBLK * 576;
+    const RefTab<T, ORDER, NQ> &tb = ref_tab<T, ORDER, NQ>();
+    // a and b are uniform over the wavefront: the table reads are scalar
 #pragma unroll 1
     for (int a = 0; a < NB; ++a) {
         T Ka[NQ][3];
 #pragma unroll
-        for (int q = 0; q < NQ; ++q) khat<T, ORDER, NQ>(q, a, Mq[q], Ka[q]);
+        for (int q = 0; q < NQ; ++q)
+#pragma unroll
+            for (int mm = 0; mm < 3; ++mm) {
+                const T *r = tb.rg + (q * NB + a) * 3;
+                Ka[q][mm] = Mq[q][mm * 3] * r[0] + Mq[q][mm * 3 + 1] * r[1] + Mq[q][mm * 3 + 2] * r[2];
+            }
         // not unrolled: the k_b of all columns at once would not fit in registers
 #pragma unroll 1
         for (int b = a; b < NB; ++b) {
             T Kb[NQ][3];
 #pragma unroll
-            for (int q = 0; q < NQ; ++q) khat<T, ORDER, NQ>(q, b, Mq[q], Kb[q]);
+            for (int q = 0; q < NQ; ++q)
+#pragma unroll
+                for (int mm = 0; mm < 3; ++mm) {
+                    const T *r = tb.rg + (q * NB + b) * 3;
+                    Kb[q][mm] = Mq[q][mm * 3] * r[0] + Mq[q][mm * 3 + 1] * r[1] + Mq[q][mm * 3 + 2] * r[2];
+                }
             const int s = a * NB - a * (a - 1) / 2 + (b - a);
-            const T gd = cm * grad_dot<ORDER, NQ>(a, b, g6);
+            const T *c = tb.rgg + (a * NB + b) * 6;
+            const T gd = cm * (g6[0] * c[0] + g6[1] * c[1] + g6[2] * c[2] + g6[3] * c[3] + g6[4] * c[4] + g6[5] * c[5]);
 #pragma unroll
             for (int mm = 0; mm < 3; ++mm)
 #pragma unroll
@@ -379,6 +443,106 @@ __global__ __launch_bounds__(BLOCK) void k_gather_rows(int n, const int32_t *__r
     out[3 * int64_t(i) + 2] = s2;
 }
 
+// Segmented gathers.  The output items (node rows, or 3x3 blocks of the
+// pattern) are cut into chunks of consecutive items holding at most SEG
+// contributions (chunk[c] = first item of chunk c).  A workgroup takes one
+// chunk: thread t fetches contribution t of the chunk into LDS, so all of the
+// chunk's loads are in flight at once; then one thread per item sums its
+// contributions in their stored (element) order and writes the item.
+constexpr int SEG = BLOCK;
+
+// Workgroups are dealt round-robin to the 8 XCDs (each with its own L2).
+// Chunk c of n is given to workgroup (c mod per) * 8 + c / per, per =
+// ceil(n / 8), so that each XCD walks one contiguous eighth of the chunks and
+// its L2 sees the element blocks that neighbouring rows share.  The grid is
+// 8 per workgroups; those past the last chunk exit.
+constexpr int N_XCD = 8;
+__device__ __forceinline__ int xcd_chunk(int n_chunks)
+{
+    const int per = (n_chunks + N_XCD - 1) / N_XCD;
+    return int(blockIdx.x % N_XCD) * per + int(blockIdx.x / N_XCD);
+}
+inline unsigned xcd_grid(int n_chunks) { return unsigned((n_chunks + N_XCD - 1) / N_XCD * N_XCD); }
+
+template <typename T>
+__global__ __launch_bounds__(BLOCK) void k_gather_rows_seg(int n_chunks, const int32_t *__restrict__ chunk,
+                                                           const int32_t *__restrict__ ptr, const int32_t *__restrict__ inc,
+                                                           const T *__restrict__ loc, double *__restrict__ out)
+{
+    __shared__ double sv[SEG * 3];
+    const int c = xcd_chunk(n_chunks);
+    if (c >= n_chunks) return;
+    const int i0 = chunk[c], i1 = chunk[c + 1];
+    const int k0 = ptr[i0], k1 = ptr[i1];
+    const int t = threadIdx.x;
+    if (k0 + t < k1) {
+        const T *l = loc + int64_t(inc[k0 + t]) * 3;
+        sv[t * 3] = double(l[0]);
+        sv[t * 3 + 1] = double(l[1]);
+        sv[t * 3 + 2] = double(l[2]);
+    }
+    __syncthreads();
+    for (int i = i0 + t; i < i1; i += BLOCK) {
+        double s0 = 0, s1 = 0, s2 = 0;
+        for (int k = ptr[i] - k0; k < ptr[i + 1] - k0; ++k) {
+            s0 += sv[k * 3];
+            s1 += sv[k * 3 + 1];
+            s2 += sv[k * 3 + 2];
+        }
+        out[3 * int64_t(i)] = s0;
+        out[3 * int64_t(i) + 1] = s1;
+        out[3 * int64_t(i) + 2] = s2;
+    }
+}
+
+// src[k]: index of contribution k's block in hloc (bit 31: stored transposed)
+template <typename T>
+__global__ __launch_bounds__(BLOCK) void k_gather_blocks_seg(int n_chunks, const int32_t *__restrict__ chunk,
+                                                             const int32_t *__restrict__ brow_ptr,
+                                                             const int32_t *__restrict__ block_row,
+                                                             const int32_t *__restrict__ cptr, const int32_t *__restrict__ src,
+                                                             const T *__restrict__ hloc, T *__restrict__ values)
+{
+    __shared__ T sv[SEG * 9];
+    const int c = xcd_chunk(n_chunks);
+    if (c >= n_chunks) return;
+    const int b0 = chunk[c], b1 = chunk[c + 1];
+    const int k0 = cptr[b0], k1 = cptr[b1];
+    const int t = threadIdx.x;
+    if (k0 + t < k1) {
+        const int32_t c = src[k0 + t];
+        const T *p = hloc + int64_t(c & 0x7fffffff) * 9;
+        T h[9];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) h[i] = p[i];
+        if (c >= 0) {
+#pragma unroll
+            for (int i = 0; i < 9; ++i) sv[t * 9 + i] = h[i];
+        } else {
+#pragma unroll
+            for (int mm = 0; mm < 3; ++mm)
+#pragma unroll
+                for (int nn = 0; nn < 3; ++nn) sv[t * 9 + mm * 3 + nn] = h[nn * 3 + mm];
+        }
+    }
+    __syncthreads();
+    for (int b = b0 + t; b < b1; b += BLOCK) {
+        T acc[9];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) acc[i] = T(0);
+        for (int k = cptr[b] - k0; k < cptr[b + 1] - k0; ++k)
+#pragma unroll
+            for (int i = 0; i < 9; ++i) acc[i] += sv[k * 9 + i];
+        const int row = block_row[b];
+        const int r0 = brow_ptr[row], deg = brow_ptr[row + 1] - r0;
+        T *out = values + 9 * int64_t(r0) + 3 * (b - r0);
+#pragma unroll
+        for (int mm = 0; mm < 3; ++mm)
+#pragma unroll
+            for (int nn = 0; nn < 3; ++nn) out[int64_t(mm) * 3 * deg + nn] = acc[mm * 3 + nn];
+    }
+}
+
 // one thread per 3x3 block of the pattern; contributions in element order.
 // src[k]: index of the contribution's block in hloc (bit 31: stored
 // transposed); contributions are fetched four at a time.
@@ -433,6 +597,9 @@ __global__ __launch_bounds__(BLOCK) void k_gather_blocks(int64_t n_blocks, const
 
 // ------------------------------------------------------------------ launchers
 
+// node-row gather of element-local 3-vectors (defined below)
+void gather_rows(Handle &h, const void *loc, double *out);
+
 void plain_energy(Handle &h, const double *x, double *energy, double *per_element)
 {
     const unsigned g = grid_for(h.n_elements);
@@ -453,11 +620,49 @@ void plain_gradient(Handle &h, const double *x, double *g, double *energy)
         k_gradient<T, O, Q, M><<<grid, BLOCK, 0, h.stream>>>(h.mesh<T>(), x, h.loc.as<T>(),
                                                             energy ? h.partial.as<double>() : nullptr);
         PF_LAUNCH_CHECK();
-        k_gather_rows<T><<<grid_for(h.n_nodes), BLOCK, 0, h.stream>>>(h.n_nodes, h.node_ptr.as<int32_t>(),
-                                                                        h.node_inc.as<int32_t>(), h.loc.as<T>(), g);
+        gather_rows(h, h.loc.get(), g);
     });
     PF_LAUNCH_CHECK();
     if (energy) sum_doubles(h, h.partial.as<double>(), int(grid), energy);
+}
+
+// chunk starts over items 0..n-1 with contribution offsets ptr: consecutive
+// items, at most SEG contributions per chunk
+template <typename P> std::vector<int32_t> segment_chunks(const P *ptr, int64_t n)
+{
+    std::vector<int32_t> c{0};
+    int64_t k0 = ptr[0];
+    for (int64_t i = 0; i < n; ++i) {
+        if (ptr[i + 1] - ptr[i] > SEG) fail(PFGPU_ERR_UNSUPPORTED, "an output gathers more than 256 contributions");
+        if (ptr[i + 1] - k0 > SEG) {
+            c.push_back(int32_t(i));
+            k0 = ptr[i];
+        }
+    }
+    c.push_back(int32_t(n));
+    return c;
+}
+
+void ensure_row_chunks(Handle &h)
+{
+    if (h.row_chunk.get()) return;
+    std::vector<int32_t> ptr(size_t(h.n_nodes) + 1);
+    PF_HIP(hipMemcpy(ptr.data(), h.node_ptr.get(), sizeof(int32_t) * ptr.size(), hipMemcpyDeviceToHost));
+    const std::vector<int32_t> c = segment_chunks(ptr.data(), h.n_nodes);
+    h.row_chunk = upload(c, h.stream);
+    h.n_row_chunks = int(c.size()) - 1;
+}
+
+void gather_rows(Handle &h, const void *loc, double *out)
+{
+    ensure_row_chunks(h);
+    if (h.precision == PFGPU_MIXED)
+        k_gather_rows_seg<float><<<xcd_grid(h.n_row_chunks), BLOCK, 0, h.stream>>>(
+            h.n_row_chunks, h.row_chunk.as<int32_t>(), h.node_ptr.as<int32_t>(), h.node_inc.as<int32_t>(), static_cast<const float *>(loc), out);
+    else
+        k_gather_rows_seg<double><<<xcd_grid(h.n_row_chunks), BLOCK, 0, h.stream>>>(
+            h.n_row_chunks, h.row_chunk.as<int32_t>(), h.node_ptr.as<int32_t>(), h.node_inc.as<int32_t>(), static_cast<const double *>(loc), out);
+    PF_LAUNCH_CHECK();
 }
 
 void ensure_plain_hessian_maps(Handle &h)
@@ -484,6 +689,11 @@ void ensure_plain_hessian_maps(Handle &h)
     }
     h.contrib_ptr = upload(ptr, h.stream);
     h.contrib = upload(contrib, h.stream);
+    {
+        const std::vector<int32_t> c = segment_chunks(ptr.data(), h.pat.n_blocks);
+        h.blk_chunk = upload(c, h.stream);
+        h.n_blk_chunks = int(c.size()) - 1;
+    }
     h.block_row = upload(block_row, h.stream);
     h.brow_ptr = upload(h.pat.brow_ptr, h.stream);
     h.plain_hess_ready = true;
@@ -497,8 +707,8 @@ void plain_hessian(Handle &h, const double *x, void *values)
     dispatch(h, [&]<typename T, int O, int Q, int M>() {
         k_hessian<T, O, Q, M><<<grid, BLOCK, 0, h.stream>>>(h.mesh<T>(), x, h.hloc.as<T>());
         PF_LAUNCH_CHECK();
-        k_gather_blocks<T><<<grid_for(h.pat.n_blocks), BLOCK, 0, h.stream>>>(
-            h.pat.n_blocks, h.brow_ptr.as<int32_t>(), h.block_row.as<int32_t>(), h.contrib_ptr.as<int32_t>(),
+        k_gather_blocks_seg<T><<<xcd_grid(h.n_blk_chunks), BLOCK, 0, h.stream>>>(
+            h.n_blk_chunks, h.blk_chunk.as<int32_t>(), h.brow_ptr.as<int32_t>(), h.block_row.as<int32_t>(), h.contrib_ptr.as<int32_t>(),
             h.contrib.as<int32_t>(), h.hloc.as<T>(), static_cast<T *>(values));
     });
     PF_LAUNCH_CHECK();
@@ -511,8 +721,7 @@ void plain_hvp(Handle &h, const double *x, const double *v, double *y)
     dispatch(h, [&]<typename T, int O, int Q, int M>() {
         k_hvp<T, O, Q, M><<<grid, BLOCK, 0, h.stream>>>(h.mesh<T>(), x, v, h.loc.as<T>());
         PF_LAUNCH_CHECK();
-        k_gather_rows<T><<<grid_for(h.n_nodes), BLOCK, 0, h.stream>>>(h.n_nodes, h.node_ptr.as<int32_t>(),
-                                                                        h.node_inc.as<int32_t>(), h.loc.as<T>(), y);
+        gather_rows(h, h.loc.get(), y);
     });
     PF_LAUNCH_CHECK();
 }
@@ -524,8 +733,7 @@ void plain_material_jvp(Handle &h, const double *x, const double *dl, const doub
     dispatch(h, [&]<typename T, int O, int Q, int M>() {
         k_material_jvp<T, O, Q, M><<<grid, BLOCK, 0, h.stream>>>(h.mesh<T>(), x, dl, dm, h.loc.as<T>());
         PF_LAUNCH_CHECK();
-        k_gather_rows<T><<<grid_for(h.n_nodes), BLOCK, 0, h.stream>>>(h.n_nodes, h.node_ptr.as<int32_t>(),
-                                                                        h.node_inc.as<int32_t>(), h.loc.as<T>(), y);
+        gather_rows(h, h.loc.get(), y);
     });
     PF_LAUNCH_CHECK();
 }
@@ -534,6 +742,14 @@ void element_material_vjp(Handle &h, const double *x, const double *p, double *o
 {
     dispatch(h, [&]<typename T, int O, int Q, int M>() {
         k_material_vjp<T, O, Q, M><<<grid_for(h.n_elements), BLOCK, 0, h.stream>>>(h.mesh<T>(), x, p, out);
+    });
+    PF_LAUNCH_CHECK();
+}
+
+void energy_lame_gradient(Handle &h, const double *x, double *out)
+{
+    dispatch(h, [&]<typename T, int O, int Q, int M>() {
+        k_energy_lame_grad<T, O, Q, M><<<grid_for(h.n_elements), BLOCK, 0, h.stream>>>(h.mesh<T>(), x, out);
     });
     PF_LAUNCH_CHECK();
 }

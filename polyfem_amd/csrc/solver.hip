@@ -437,23 +437,47 @@ void newton_solve(Handle &h, double *x, const uint8_t *fixed, const pfgpu_newton
             PF_LAUNCH_CHECK();
             gd = -gn * gn;
         }
+        // backtracking from alpha0 along dir; true when a step was accepted
         double alpha = 1.0;
+        auto search = [&](const double *dd, double slope, double alpha0) {
+            alpha = alpha0;
+            for (int ls = 0; ls <= prm.max_line_search; ++ls) {
+                k_axpy_to<<<gr, BLOCK, 0, h.stream>>>(w.xt, x, alpha, dd, h.n_dofs);
+                PF_LAUNCH_CHECK();
+                if (prm.line_search == 0) return true;
+                const double e1 = energy_at(w.xt);
+                // a NaN energy is an inverted element: rejected like an invalid step
+                if (std::isfinite(e1) && e1 - e0 <= prm.armijo_c * alpha * slope) return true;
+                alpha *= 0.5;
+                rep.line_search_halvings++;
+            }
+            return false;
+        };
+        // Near the minimum the predicted decrease g.d falls below the rounding
+        // of the FP64 energy sum and the Armijo test cannot resolve it: the
+        // full Newton step is taken (the quadratic model is exact to that
+        // precision) if it keeps every element valid.
+        const double noise = 64.0 * 2.2e-16 * std::fabs(e0);
         bool accepted = false;
-        for (int ls = 0; ls <= prm.max_line_search; ++ls) {
-            k_axpy_to<<<gr, BLOCK, 0, h.stream>>>(w.xt, x, alpha, dir, h.n_dofs);
+        if (prm.line_search != 0 && -gd <= noise) {
+            k_axpy_to<<<gr, BLOCK, 0, h.stream>>>(w.xt, x, 1.0, dir, h.n_dofs);
             PF_LAUNCH_CHECK();
-            if (prm.line_search == 0) {
-                accepted = true;
-                break;
-            }
-            const double e1 = energy_at(w.xt);
-            // a NaN energy is an inverted element: rejected like an invalid step
-            if (std::isfinite(e1) && e1 - e0 <= prm.armijo_c * alpha * gd) {
-                accepted = true;
-                break;
-            }
-            alpha *= 0.5;
-            rep.line_search_halvings++;
+            alpha = 1.0;
+            accepted = std::isfinite(energy_at(w.xt));
+        }
+        if (!accepted) accepted = search(dir, gd, 1.0);
+        if (!accepted) {
+            // Newton direction rejected (near-singular or indefinite Hessian):
+            // steepest descent from the Cauchy step g^T g / g^T H g of the
+            // quadratic model, or a unit-length step when the curvature along
+            // g is not positive
+            k_neg_free<<<gr, BLOCK, 0, h.stream>>>(w.y, w.g, fixed, h.n_dofs);
+            hvp_any(h, x, w.y, w.q);
+            dot(h, w, w.y, w.q, fixed, S_GD, AFTER_NONE);
+            const double c = read_scalar(h, w, S_GD);
+            const double a0 = c > 0.0 ? gn * gn / c : 1.0 / gn;
+            accepted = search(w.y, -gn * gn, a0);
+            rep.gradient_steps++;
         }
         if (!accepted) fail(PFGPU_ERR_INVALID_STATE, "line search found no acceptable step");
         PF_HIP(hipMemcpyAsync(x, w.xt, sizeof(double) * size_t(h.n_dofs), hipMemcpyDeviceToDevice, h.stream));

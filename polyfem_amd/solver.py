@@ -65,6 +65,7 @@ class NewtonReport:
     energy: float
     grad_norm: float
     last_alpha: float
+    gradient_steps: int
 
 
 class NewtonSolver:
@@ -90,7 +91,7 @@ class NewtonSolver:
         check(lib().pfgpu_newton_solve(assembler._h, xx.ctypes.data_as(C.c_void_p),
                                        problem.mask.ctypes.data_as(C.c_void_p), C.byref(self.params), C.byref(rep)))
         return xx, NewtonReport(rep.iterations, rep.cg_iterations, rep.line_search_halvings, bool(rep.converged),
-                                rep.energy, rep.grad_norm, rep.last_alpha)
+                                rep.energy, rep.grad_norm, rep.last_alpha, rep.gradient_steps)
 
 
 def cg_solve(assembler, x, rhs, fixed_mask=None, max_iterations: int = 1000, tolerance: float = 1e-8):

@@ -155,3 +155,17 @@ def test_fused_many_patches(order, precision):
     assert rel(e, o.energy(x)) < (FP64_TOL if precision == "fp64" else MIXED_TOL["energy"])
     v = np.random.default_rng(BASE_SEED + 303).normal(size=m.n_dofs)
     assert rel(a.hessian_vector_product(x, v), o.hessian_scipy(x) @ v) < (FP64_TOL if precision == "fp64" else MIXED_TOL["hvp"])
+
+
+@pytest.mark.parametrize("order,name", CASES)
+def test_energy_lame_gradient(order, name):
+    """dE/d(lambda, mu) per element: the energy is linear in the Lame pair, so
+    the derivative equals the per-element energy with unit (1, 0) / (0, 1)."""
+    m, x, a, o, lam, mu = setup_case(order, name)
+    d = a.energy_lame_gradient(x)
+    E = m.n_elements
+    o_l = po.Oracle(m, np.ones(E), np.zeros(E), material=MAT[name])
+    o_m = po.Oracle(m, np.zeros(E), np.ones(E), material=MAT[name])
+    assert rel(d[:E], o_l.energy_per_element(x)) < FP64_TOL
+    assert rel(d[E:], o_m.energy_per_element(x)) < FP64_TOL
+    assert abs(lam @ d[:E] + mu @ d[E:] - o.energy(x)) <= FP64_TOL * abs(o.energy(x))

@@ -12,7 +12,7 @@ import pytest
 
 from oracle import pyoracle as po
 from polyfem_amd import NewtonSolver, NLAssembler, NLProblem, TransientSimulation
-from polyfem_amd.mesh import BASE_SEED, face_nodes, kuhn_box, random_lame, smooth_displacement
+from polyfem_amd.mesh import BASE_SEED, face_nodes, kuhn_box, random_lame
 
 pytestmark = pytest.mark.gpu
 
@@ -75,7 +75,13 @@ def setup(every, lam=None, mu=None, material="NeoHookean"):
     a = NLAssembler(material, m, lame=(lam, mu), precision="fp64")
     newton = NewtonSolver(max_iterations=50, grad_norm=1e-13, cg_max_iterations=5000, cg_tolerance=1e-14)
     sim = TransientSimulation(a, rho=1000.0, dt=2e-3, time_steps=6, checkpoint_every=every, newton=newton)
-    v0 = smooth_displacement(m.nodes, 0.5, 0.0, BASE_SEED + 23)
+    # seeded initial velocity (the sine field of the box configurations
+    # vanishes at every node of a 2x2x2 grid)
+    X = m.nodes
+    v0 = 0.5 * np.stack([np.sin(np.pi * X[:, 2]) * np.cos(np.pi * X[:, 1]),
+                         np.sin(np.pi * X[:, 2]) * X[:, 0],
+                         0.3 * np.sin(np.pi * X[:, 2])], axis=1).ravel()
+    v0 += 0.05 * np.random.default_rng(BASE_SEED + 23).normal(size=v0.size)
     v0[prob.fixed] = 0.0
     return m, prob, a, sim, np.zeros(m.n_dofs), v0, lam, mu
 
@@ -93,7 +99,9 @@ def test_checkpointed_adjoint_equals_store_all():
         print(f"transient checkpoint_every={every}", flush=True)
         xT, vT = sim.forward(x0, v0, prob)
         J, gx, gv = objective(a, sim, xT, vT)
+        assert J > 0 and np.abs(vT).max() > 0.05
         out.append(sim.adjoint(gx, gv))
+        assert np.abs(out[-1][0]).max() > 0 and np.abs(out[-1][2]).max() > 0
     for k in range(1, len(out)):
         for u, w in zip(out[0], out[k]):
             assert np.array_equal(u, w)
@@ -105,6 +113,7 @@ def test_transient_adjoint_matches_finite_differences():
     J0, gx, gv = objective(a, sim, xT, vT)
     dth, dx0, dv0 = sim.adjoint(gx, gv)
     assert np.all(dv0[prob.fixed] == 0) and np.all(dx0[prob.fixed] == 0)
+    assert J0 > 0 and np.abs(dv0).max() > 0 and np.abs(dth).max() > 0
     rng = np.random.default_rng(BASE_SEED + 24)
 
     def J_of(lm, mm, vv):
@@ -117,7 +126,9 @@ def test_transient_adjoint_matches_finite_differences():
     dl = rng.normal(size=m.n_elements) * lam
     dm = rng.normal(size=m.n_elements) * mu
     fd = (J_of(lam + h * dl, mu + h * dm, v0) - J_of(lam - h * dl, mu - h * dm, v0)) / (2 * h)
-    lin = dth[:m.n_elements] @ dl + dth[m.n_elements:] @ dm
+    # J contains E_el(x_T): its explicit Lame derivative adds to the adjoint terms
+    tot = dth + a.energy_lame_gradient(xT)
+    lin = tot[:m.n_elements] @ dl + tot[m.n_elements:] @ dm
     assert abs(fd - lin) <= 1e-5 * abs(lin), (fd, lin)
     # initial velocity direction (free dofs)
     dv = rng.normal(size=m.n_dofs)
