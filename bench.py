@@ -178,6 +178,9 @@ def main():
     ap.add_argument("--no-extras", action="store_true", help="skip the error, adjoint and Newton measurements")
     ap.add_argument("--newton-iterations", type=int, default=10)
     ap.add_argument("--cg-iterations", type=int, default=100)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only copy 1 GiB device to device five times (the reference kernel for the "
+                         "FETCH_SIZE / WRITE_SIZE counter scale) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the path shards by "
@@ -186,6 +189,15 @@ def main():
 
     import numpy as np
     import torch
+
+    if args.calibrate_copy:
+        src = torch.ones(1 << 28, dtype=torch.float32, device="cuda:0")  # 1 GiB
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_each_way": src.numel() * 4, "copies": 5}))
+        return
 
     from polyfem_amd import NLAssembler
     from polyfem_amd.mesh import BASE_SEED, kuhn_box, random_lame, smooth_displacement
