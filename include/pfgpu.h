@@ -196,6 +196,18 @@ pfgpu_status pfgpu_newton_solve(pfgpu_handle *h, double *x, const uint8_t *fixed
                                 pfgpu_newton_report *report);
 pfgpu_status pfgpu_newton_solve_dev(pfgpu_handle *h, double *x, const uint8_t *fixed,
                                     const pfgpu_newton_params *params, pfgpu_newton_report *report);
+/* Batched Newton-CG over independent problems held by one handle as one
+ * mesh with disjoint node ranges (same order and material; build the handle
+ * from the concatenated meshes): problem s owns elements
+ * [elem_offsets[s], elem_offsets[s+1]) and nodes [node_offsets[s],
+ * node_offsets[s+1]).  Every assembly kernel is one launch for the whole
+ * batch; one CG on the block-diagonal system gives all Newton directions;
+ * line search, step length and convergence are per problem (reports has
+ * n_problems entries).  Host pointers; the elastic energy only (no load, no
+ * inertia). */
+pfgpu_status pfgpu_newton_solve_batch(pfgpu_handle *h, double *x, const uint8_t *fixed, int32_t n_problems,
+                                      const int32_t *elem_offsets, const int32_t *node_offsets,
+                                      const pfgpu_newton_params *params, pfgpu_newton_report *reports);
 /* CG solve of H(x)_ff y_f = b_f with y = 0 on fixed dofs, matrix-free; the
  * static adjoint solve (solve_static_adjoint, StateDiff.cpp:230-318; the
  * Hessian is symmetric, so H^T = H).  Stops at ||r|| <= tolerance ||b_f||. */

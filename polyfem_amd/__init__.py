@@ -7,7 +7,8 @@ mirror of PolyFEM's assembler/form API and the synthetic-mesh generator.
 from . import _ffi, mesh
 from .assembler import ElasticForm, NLAssembler, make_assembler
 from ._ffi import PfgpuError
-from .solver import NewtonSolver, NLProblem, TransientSimulation, cg_solve, solve_static_adjoint
+from .solver import (BatchSolver, NewtonSolver, NLProblem, TransientSimulation, cg_solve, concatenate_meshes,
+                     solve_static_adjoint)
 
 __all__ = ["ElasticForm", "NLAssembler", "make_assembler", "PfgpuError", "mesh", "_ffi", "NewtonSolver",
-           "NLProblem", "TransientSimulation", "cg_solve", "solve_static_adjoint"]
+           "NLProblem", "TransientSimulation", "cg_solve", "solve_static_adjoint", "BatchSolver", "concatenate_meshes"]

@@ -91,6 +91,7 @@ _SIGNATURES = {
     "pfgpu_cg_solve_dev": [vp, vp, vp, vp, vp, C.c_int32, C.c_double, C.c_int32, C.POINTER(C.c_int32),
                            C.POINTER(C.c_double)],
     "pfgpu_set_load": [vp, vp],
+    "pfgpu_newton_solve_batch": [vp, vp, vp, C.c_int32, vp, vp, C.POINTER(NewtonParams), vp],
     "pfgpu_energy_lame_gradient": [vp, vp, vp],
     "pfgpu_set_density": [vp, vp, C.c_double],
     "pfgpu_mass_matvec": [vp, vp, vp],

@@ -152,7 +152,7 @@ int64_t pfgpu_handle::device_bytes() const
     const DevBuf *bufs[] = {&conn, &vconn, &geom, &lame, &node_ptr, &node_inc, &vert_ptr, &vert_inc,
                             &brow_ptr, &block_row, &contrib_ptr, &contrib, &row_chunk, &blk_chunk, &patch_desc, &patch_rows,
                             &patch_row_acc, &patch_elems, &patch_cs, &patch_cg, &patch_hdr, &patch_pos, &patch_vnodes,
-                            &d_outer, &d_inner, &loc, &hloc, &partial, &red_out,
+                            &d_outer, &d_inner, &loc, &hloc, &partial, &red_out, &sum_part,
                             &h_x, &h_v, &h_out, &h_out2, &h_vals, &sv, &h_mask, &mass, &pat_bcol, &tr_fixed, &tr_f, &tr_seg, &load};
     int64_t s = 0;
     for (const DevBuf *b : bufs) s += int64_t(b->bytes());
@@ -222,6 +222,10 @@ pfgpu_status pfgpu_create(const pfgpu_mesh_desc *mesh, const pfgpu_material_desc
         h->node_inc = upload(h->pat.node_inc, h->stream);
         h->vert_ptr = upload(h->pat.vert_ptr, h->stream);
         h->vert_inc = upload(h->pat.vert_inc, h->stream);
+        // fixed-size scratch of the reductions, allocated here so that *_dev
+        // calls enqueue no allocation (they may be captured in a graph)
+        h->sum_part.reserve(sizeof(double) * 128);
+        h->red_out.reserve(sizeof(int) * 4);
         *out = h.release();
     });
 }
@@ -619,6 +623,37 @@ pfgpu_status pfgpu_newton_solve(pfgpu_handle *h, double *x, const uint8_t *fixed
         PF_HIP(hipMemcpyAsync(dx, x, sizeof(double) * size_t(h->n_dofs), hipMemcpyHostToDevice, h->stream));
         const uint8_t *dm = stage_mask(*h, fixed);
         newton_solve(*h, dx, dm, *params, *report);
+        fetch(*h, x, dx, sizeof(double) * size_t(h->n_dofs));
+    });
+}
+
+pfgpu_status pfgpu_newton_solve_batch(pfgpu_handle *h, double *x, const uint8_t *fixed, int32_t n_problems,
+                                      const int32_t *elem_offsets, const int32_t *node_offsets,
+                                      const pfgpu_newton_params *params, pfgpu_newton_report *reports)
+{
+    return guard([&] {
+        check_handle(h);
+        require(x && elem_offsets && node_offsets && reports, "null argument");
+        check_newton_params(params);
+        require(n_problems >= 1, "empty batch");
+        require(elem_offsets[0] == 0 && elem_offsets[n_problems] == h->n_elements && node_offsets[0] == 0 &&
+                    node_offsets[n_problems] == h->n_nodes,
+                "batch offsets must cover the handle's elements and nodes");
+        for (int s = 0; s < n_problems; ++s) {
+            require(elem_offsets[s + 1] > elem_offsets[s] && node_offsets[s + 1] > node_offsets[s], "empty problem in the batch");
+            // every element of problem s must use nodes of problem s only
+            for (int e = elem_offsets[s]; e < elem_offsets[s + 1]; ++e)
+                for (int a = 0; a < h->n_local; ++a) {
+                    const int32_t nd = h->h_elem_nodes[size_t(e) * h->n_local + a];
+                    require(nd >= node_offsets[s] && nd < node_offsets[s + 1], "a batch problem's element uses another problem's node");
+                }
+        }
+        SolverStream ss(*h);
+        h->h_out2.reserve(sizeof(double) * size_t(h->n_dofs));
+        double *dx = h->h_out2.as<double>();
+        PF_HIP(hipMemcpyAsync(dx, x, sizeof(double) * size_t(h->n_dofs), hipMemcpyHostToDevice, h->stream));
+        const uint8_t *dm = stage_mask(*h, fixed);
+        newton_solve_batch(*h, dx, dm, n_problems, elem_offsets, node_offsets, *params, reports);
         fetch(*h, x, dx, sizeof(double) * size_t(h->n_dofs));
     });
 }

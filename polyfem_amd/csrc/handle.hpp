@@ -37,7 +37,7 @@ struct pfgpu_handle {
     pfgpu::DevBuf d_outer, d_inner;
 
     // scratch
-    pfgpu::DevBuf loc, hloc, partial, red_out;
+    pfgpu::DevBuf loc, hloc, partial, red_out, sum_part;
     pfgpu::DevBuf h_x, h_v, h_out, h_out2, h_vals; // staging for host-pointer calls
     // consistent mass matrix: one value per 3x3 block (block = m I_3)
     pfgpu::DevBuf mass, pat_bcol;
@@ -103,6 +103,8 @@ void build_mass(Handle &h, const double *rho, double rho_uniform);
 // ---- solver (solver.hip) ------------------------------------------------
 void mass_apply(Handle &h, const double *v, double *y);
 void newton_solve(Handle &h, double *x, const uint8_t *fixed, const pfgpu_newton_params &p, pfgpu_newton_report &r);
+void newton_solve_batch(Handle &h, double *x, const uint8_t *fixed, int nseg, const int32_t *eo, const int32_t *no,
+                        const pfgpu_newton_params &p, pfgpu_newton_report *reps);
 void transient_forward(Handle &h, const double *x0, const double *v0, const uint8_t *fixed, const double *f,
                        double dt, int steps, int every, const pfgpu_newton_params &np, double *xT, double *vT);
 void transient_adjoint(Handle &h, const double *dJdx, const double *dJdv, int cg_max, double cg_tol, double *dtheta,

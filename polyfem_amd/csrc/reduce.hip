@@ -22,6 +22,25 @@ __global__ __launch_bounds__(BLOCK) void k_sum_doubles(const double *__restrict_
     if (t == 0) out[0] = s[0];
 }
 
+// First level for long inputs: block b sums the contiguous range
+// [b seg, (b+1) seg) (thread-strided, then a fixed tree) into part[b].
+__global__ __launch_bounds__(BLOCK) void k_sum_doubles_blocks(const double *__restrict__ in, int n, int seg,
+                                                              double *__restrict__ part)
+{
+    __shared__ double s[BLOCK];
+    const int t = threadIdx.x;
+    const int i0 = blockIdx.x * seg, i1 = min(n, i0 + seg);
+    double v = 0;
+    for (int i = i0 + t; i < i1; i += BLOCK) v += in[i];
+    s[t] = v;
+    __syncthreads();
+    for (int w = BLOCK / 2; w > 0; w >>= 1) {
+        if (t < w) s[t] += s[t + w];
+        __syncthreads();
+    }
+    if (t == 0) part[blockIdx.x] = s[0];
+}
+
 template <typename T>
 __global__ __launch_bounds__(BLOCK) void k_count_nonfinite(const T *__restrict__ v, int64_t n, int *__restrict__ flag)
 {
@@ -48,7 +67,18 @@ __global__ __launch_bounds__(BLOCK) void k_expand_inner(int n_nodes, const int32
 
 void sum_doubles(Handle &h, const double *in, int n, double *out)
 {
-    k_sum_doubles<<<1, BLOCK, 0, h.stream>>>(in, n, out);
+    // one block for short inputs; above that a fixed grid of SUM_BLOCKS
+    // contiguous ranges, then one block over their sums.  The order depends
+    // on n only.
+    constexpr int SUM_BLOCKS = 128;
+    if (n <= 8 * BLOCK) {
+        k_sum_doubles<<<1, BLOCK, 0, h.stream>>>(in, n, out);
+    } else {
+        if (h.sum_part.bytes() < sizeof(double) * SUM_BLOCKS) h.sum_part.reserve(sizeof(double) * SUM_BLOCKS);
+        const int seg = (n + SUM_BLOCKS - 1) / SUM_BLOCKS;
+        k_sum_doubles_blocks<<<SUM_BLOCKS, BLOCK, 0, h.stream>>>(in, n, seg, h.sum_part.as<double>());
+        k_sum_doubles<<<1, BLOCK, 0, h.stream>>>(h.sum_part.as<double>(), SUM_BLOCKS, out);
+    }
     PF_LAUNCH_CHECK();
 }
 

@@ -675,4 +675,193 @@ void transient_adjoint(Handle &h, const double *dJdx, const double *dJdv, int cg
     PF_HIP(hipStreamSynchronize(h.stream));
 }
 
+// ---- batches: independent problems concatenated in one handle ---------
+//
+// A batch handle holds several meshes of the same order and material as one
+// mesh with disjoint node ranges, so every assembly kernel is one launch for
+// the whole batch and the Hessian is block diagonal.  Problem s owns elements
+// [eo[s], eo[s+1]) and nodes [no[s], no[s+1]).  The Newton iteration runs
+// on all problems at once: one CG on the block-diagonal system (its solution
+// is every problem's Newton direction), then a line search, convergence test
+// and step length per problem, from per-problem FP64 sums.  A converged
+// problem is frozen (its dofs join the fixed mask).
+
+namespace {
+
+// out[s] = sum of v[off[s] .. off[s+1]) (one block per segment, fixed order)
+__global__ __launch_bounds__(BLOCK) void k_seg_sum(const double *__restrict__ v, const int32_t *__restrict__ off,
+                                                   double *__restrict__ out)
+{
+    __shared__ double sh[BLOCK];
+    const int sgm = blockIdx.x, t = threadIdx.x;
+    double a = 0;
+    for (int i = off[sgm] + t; i < off[sgm + 1]; i += BLOCK) a += v[i];
+    sh[t] = a;
+    __syncthreads();
+    for (int w = BLOCK / 2; w > 0; w >>= 1) {
+        if (t < w) sh[t] += sh[t + w];
+        __syncthreads();
+    }
+    if (t == 0) out[sgm] = sh[0];
+}
+
+// out[s] = sum over dofs of segment s (nodes no[s]..no[s+1]) of a b, free dofs only
+__global__ __launch_bounds__(BLOCK) void k_seg_dot(const double *__restrict__ a, const double *__restrict__ b,
+                                                   const uint8_t *__restrict__ fixed, const int32_t *__restrict__ no,
+                                                   double *__restrict__ out)
+{
+    __shared__ double sh[BLOCK];
+    const int sgm = blockIdx.x, t = threadIdx.x;
+    double acc = 0;
+    for (int i = 3 * no[sgm] + t; i < 3 * no[sgm + 1]; i += BLOCK)
+        if (!fixed || !fixed[i]) acc = fma(a[i], b[i], acc);
+    sh[t] = acc;
+    __syncthreads();
+    for (int w = BLOCK / 2; w > 0; w >>= 1) {
+        if (t < w) sh[t] += sh[t + w];
+        __syncthreads();
+    }
+    if (t == 0) out[sgm] = sh[0];
+}
+
+// out = x + alpha[seg(i)] d over all dofs; seg: node -> segment
+__global__ __launch_bounds__(BLOCK) void k_axpy_seg(double *__restrict__ out, const double *__restrict__ x,
+                                                    const double *__restrict__ alpha, const double *__restrict__ d,
+                                                    const int32_t *__restrict__ seg, int n)
+{
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i < n) out[i] = fma(alpha[seg[i / 3]], d[i], x[i]);
+}
+
+// working mask: fixed dofs plus every dof of a frozen segment
+__global__ __launch_bounds__(BLOCK) void k_seg_mask(uint8_t *__restrict__ mask, const uint8_t *__restrict__ fixed,
+                                                    const uint8_t *__restrict__ frozen, const int32_t *__restrict__ seg, int n)
+{
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i < n) mask[i] = (fixed && fixed[i]) || frozen[seg[i / 3]] ? 1 : 0;
+}
+
+} // namespace
+
+void newton_solve_batch(Handle &h, double *x, const uint8_t *fixed, int nseg, const int32_t *eo_h, const int32_t *no_h,
+                        const pfgpu_newton_params &prm, pfgpu_newton_report *reps)
+{
+    if (h.obj_xt || h.obj_f || h.obj_w != 1.0) fail(PFGPU_ERR_UNSUPPORTED, "batched solves take the elastic energy only");
+    const int n = h.n_dofs, E = h.n_elements;
+    const unsigned gr = grid_for(n);
+    const Work w = workspace(h);
+    // per-batch device tables
+    std::vector<int32_t> seg_of(size_t(h.n_nodes));
+    for (int sg = 0; sg < nseg; ++sg)
+        for (int i = no_h[sg]; i < no_h[sg + 1]; ++i) seg_of[size_t(i)] = sg;
+    DevBuf d_eo = upload(std::vector<int32_t>(eo_h, eo_h + nseg + 1), h.stream);
+    DevBuf d_no = upload(std::vector<int32_t>(no_h, no_h + nseg + 1), h.stream);
+    DevBuf d_seg = upload(seg_of, h.stream);
+    DevBuf scal{sizeof(double) * size_t(4 * nseg) + size_t(nseg)}, pe{sizeof(double) * size_t(E)}, mask{size_t(n)};
+    double *s_e = scal.as<double>(), *s_g = s_e + nseg, *s_gd = s_g + nseg, *s_a = s_gd + nseg;
+    uint8_t *frozen = reinterpret_cast<uint8_t *>(s_a + nseg);
+    std::vector<double> e0(nseg), e1(nseg), gn(nseg), gd(nseg), alpha(nseg);
+    std::vector<uint8_t> fr(nseg, 0);
+    std::vector<pfgpu_newton_report> rep(nseg);
+    auto read = [&](const double *dev, std::vector<double> &host) {
+        PF_HIP(hipMemcpyAsync(host.data(), dev, sizeof(double) * size_t(nseg), hipMemcpyDeviceToHost, h.stream));
+        PF_HIP(hipStreamSynchronize(h.stream));
+    };
+    auto seg_energy = [&](const double *xx, std::vector<double> &host) {
+        plain_energy(h, xx, w.scal + S_E1, pe.as<double>());
+        k_seg_sum<<<nseg, BLOCK, 0, h.stream>>>(pe.as<double>(), d_eo.as<int32_t>(), s_e);
+        PF_LAUNCH_CHECK();
+        read(s_e, host);
+    };
+    auto seg_grad = [&]() {
+        if (h.path == PFGPU_PATH_FUSED)
+            fused_gradient(h, x, w.g, nullptr);
+        else
+            plain_gradient(h, x, w.g, nullptr);
+        k_seg_dot<<<nseg, BLOCK, 0, h.stream>>>(w.g, w.g, fixed, d_no.as<int32_t>(), s_g);
+        PF_LAUNCH_CHECK();
+        read(s_g, gn);
+        for (double &v : gn) v = std::sqrt(v);
+    };
+    for (int it = 0; it < prm.max_iterations; ++it) {
+        seg_energy(x, e0);
+        seg_grad();
+        bool any = false;
+        for (int sg = 0; sg < nseg; ++sg) {
+            if (!std::isfinite(e0[sg]) || !std::isfinite(gn[sg]))
+                fail(PFGPU_ERR_INVALID_STATE, "non-finite energy or gradient at a batched Newton iterate");
+            rep[sg].energy = e0[sg];
+            rep[sg].grad_norm = gn[sg];
+            if (!fr[sg] && gn[sg] <= prm.grad_norm) {
+                fr[sg] = 1;
+                rep[sg].converged = 1;
+            }
+            any |= !fr[sg];
+        }
+        if (!any) break;
+        PF_HIP(hipMemcpyAsync(frozen, fr.data(), size_t(nseg), hipMemcpyHostToDevice, h.stream));
+        k_seg_mask<<<gr, BLOCK, 0, h.stream>>>(mask.as<uint8_t>(), fixed, frozen, d_seg.as<int32_t>(), n);
+        // Newton directions of all active problems: one CG on the block-diagonal system
+        k_neg_free<<<gr, BLOCK, 0, h.stream>>>(w.d, w.g, mask.as<uint8_t>(), n);
+        PF_LAUNCH_CHECK();
+        double rel = 0;
+        const int cg_it = cg_solve_ws(h, x, mask.as<uint8_t>(), w.d, prm.cg_max_iterations, prm.cg_tolerance,
+                                      prm.use_graph != 0, &rel);
+        k_seg_dot<<<nseg, BLOCK, 0, h.stream>>>(w.g, w.y, mask.as<uint8_t>(), d_no.as<int32_t>(), s_gd);
+        PF_LAUNCH_CHECK();
+        read(s_gd, gd);
+        // a problem whose direction is not a descent direction takes -g
+        bool steepest = false;
+        for (int sg = 0; sg < nseg; ++sg)
+            if (!fr[sg] && !(gd[sg] < 0.0)) steepest = true;
+        if (steepest) fail(PFGPU_ERR_INVALID_STATE, "batched Newton direction is not a descent direction");
+        // per-problem backtracking
+        std::vector<uint8_t> done(nseg, 0);
+        for (int sg = 0; sg < nseg; ++sg) {
+            alpha[sg] = fr[sg] ? 0.0 : 1.0;
+            done[sg] = fr[sg];
+            if (!fr[sg]) rep[sg].cg_iterations += cg_it;
+        }
+        for (int ls = 0; ls <= prm.max_line_search; ++ls) {
+            PF_HIP(hipMemcpyAsync(s_a, alpha.data(), sizeof(double) * size_t(nseg), hipMemcpyHostToDevice, h.stream));
+            k_axpy_seg<<<gr, BLOCK, 0, h.stream>>>(w.xt, x, s_a, w.y, d_seg.as<int32_t>(), n);
+            PF_LAUNCH_CHECK();
+            if (prm.line_search == 0) {
+                std::fill(done.begin(), done.end(), 1);
+                break;
+            }
+            seg_energy(w.xt, e1);
+            bool all = true;
+            for (int sg = 0; sg < nseg; ++sg) {
+                if (done[sg]) continue;
+                const double noise = 64.0 * 2.2e-16 * std::fabs(e0[sg]);
+                if (std::isfinite(e1[sg]) && (e1[sg] - e0[sg] <= prm.armijo_c * alpha[sg] * gd[sg] || -gd[sg] <= noise)) {
+                    done[sg] = 1;
+                } else {
+                    alpha[sg] *= 0.5;
+                    rep[sg].line_search_halvings++;
+                    all = false;
+                }
+            }
+            if (all) break;
+        }
+        for (int sg = 0; sg < nseg; ++sg)
+            if (!done[sg]) fail(PFGPU_ERR_INVALID_STATE, "batched line search found no acceptable step");
+        PF_HIP(hipMemcpyAsync(x, w.xt, sizeof(double) * size_t(n), hipMemcpyDeviceToDevice, h.stream));
+        for (int sg = 0; sg < nseg; ++sg)
+            if (!fr[sg]) {
+                rep[sg].iterations = it + 1;
+                rep[sg].last_alpha = alpha[sg];
+            }
+    }
+    seg_energy(x, e0);
+    seg_grad();
+    for (int sg = 0; sg < nseg; ++sg) {
+        rep[sg].energy = e0[sg];
+        rep[sg].grad_norm = gn[sg];
+        rep[sg].converged = gn[sg] <= prm.grad_norm;
+        reps[sg] = rep[sg];
+    }
+}
+
 } // namespace pfgpu

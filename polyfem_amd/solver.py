@@ -175,3 +175,74 @@ class TransientSimulation:
         check(lib().pfgpu_transient_adjoint(self.a._h, _p(gx), _p(gv), cg_max_iterations, cg_tolerance,
                                             _p(dth), _p(dx0), _p(dv0)))
         return dth, dx0, dv0
+
+
+def concatenate_meshes(meshes):
+    """One mesh holding several meshes with disjoint node and vertex ranges;
+    returns (mesh, element offsets, node offsets)."""
+    from .mesh import TetMesh
+    order = meshes[0].order
+    if any(m.order != order for m in meshes):
+        raise ValueError("a batch holds meshes of one order")
+    eo, no, vo = [0], [0], [0]
+    for m in meshes:
+        eo.append(eo[-1] + m.n_elements)
+        no.append(no[-1] + m.n_nodes)
+        vo.append(vo[-1] + m.vertices.shape[0])
+    big = TetMesh(order=order,
+                  vertices=np.concatenate([m.vertices for m in meshes]),
+                  elem_vertices=np.concatenate([m.elem_vertices + vo[i] for i, m in enumerate(meshes)]).astype(np.int32),
+                  nodes=np.concatenate([m.nodes for m in meshes]),
+                  elem_nodes=np.concatenate([m.elem_nodes + no[i] for i, m in enumerate(meshes)]).astype(np.int32))
+    return big, np.array(eo, dtype=np.int32), np.array(no, dtype=np.int32)
+
+
+class BatchSolver:
+    """Independent static problems solved together (many meshes of uneven
+    size; one handle, one launch per kernel, per group of equal order and
+    material).  ``problems`` is a list of dicts with keys ``mesh``,
+    ``material`` ("NeoHookean" | "LinearElasticity"), ``lame`` (lambda, mu),
+    ``fixed_dofs``, ``fixed_values`` and optionally ``x0``."""
+
+    def __init__(self, problems, precision="mixed", newton: NewtonSolver = None):
+        from .assembler import NLAssembler
+        self.problems = problems
+        self.newton = newton or NewtonSolver(max_iterations=20, grad_norm=1e-8)
+        groups = {}
+        for i, p in enumerate(problems):
+            groups.setdefault((p["mesh"].order, p["material"]), []).append(i)
+        self.groups = []
+        for (order, material), idx in sorted(groups.items()):
+            big, eo, no = concatenate_meshes([problems[i]["mesh"] for i in idx])
+            lam = np.concatenate([np.broadcast_to(np.asarray(problems[i]["lame"][0], dtype=np.float64),
+                                                  (problems[i]["mesh"].n_elements,)) for i in idx])
+            mu = np.concatenate([np.broadcast_to(np.asarray(problems[i]["lame"][1], dtype=np.float64),
+                                                 (problems[i]["mesh"].n_elements,)) for i in idx])
+            a = NLAssembler(material, big, lame=(lam, mu), precision=precision)
+            fixed = np.concatenate([np.asarray(problems[i]["fixed_dofs"], dtype=np.int64) + 3 * no[k]
+                                    for k, i in enumerate(idx)])
+            vals = np.concatenate([np.asarray(problems[i]["fixed_values"], dtype=np.float64) for i in idx])
+            self.groups.append(dict(idx=idx, a=a, eo=eo, no=no, problem=NLProblem(big.n_dofs, fixed, vals)))
+
+    def solve(self):
+        """Returns (list of x per problem, list of NewtonReport per problem)."""
+        xs, reps = [None] * len(self.problems), [None] * len(self.problems)
+        for g in self.groups:
+            a, eo, no, prob = g["a"], g["eo"], g["no"], g["problem"]
+            x = np.zeros(a.mesh.n_dofs)
+            for k, i in enumerate(g["idx"]):
+                if "x0" in self.problems[i]:
+                    x[3 * no[k]:3 * no[k + 1]] = self.problems[i]["x0"]
+            x[prob.fixed] = prob.fixed_values
+            n = len(g["idx"])
+            r = (_ffi.NewtonReport * n)()
+            check(lib().pfgpu_newton_solve_batch(a._h, x.ctypes.data_as(C.c_void_p),
+                                                 prob.mask.ctypes.data_as(C.c_void_p), n,
+                                                 eo.ctypes.data_as(C.c_void_p), no.ctypes.data_as(C.c_void_p),
+                                                 C.byref(self.newton.params), r))
+            for k, i in enumerate(g["idx"]):
+                xs[i] = x[3 * no[k]:3 * no[k + 1]].copy()
+                rr = r[k]
+                reps[i] = NewtonReport(rr.iterations, rr.cg_iterations, rr.line_search_halvings, bool(rr.converged),
+                                       rr.energy, rr.grad_norm, rr.last_alpha, rr.gradient_steps)
+        return xs, reps
