@@ -171,7 +171,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--path", default="plain", choices=["fused", "plain"])
+    ap.add_argument("--path", default="auto", choices=["auto", "fused", "plain"],
+                    help="auto: the launch-shape autotuner picks path and workgroup sizes (untimed setup)")
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
     ap.add_argument("--grid", type=int, default=40)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -206,7 +207,8 @@ def main():
     m = kuhn_box(n, n, n, order=2)
     lam, mu = random_lame(m.n_elements, seed=BASE_SEED + 1)
     x_host = smooth_displacement(m.nodes, 0.05, 1e-3, BASE_SEED)
-    a = NLAssembler("NeoHookean", m, lame=(lam, mu), precision=args.precision, path=args.path)
+    a = NLAssembler("NeoHookean", m, lame=(lam, mu), precision=args.precision,
+                    path="plain" if args.path == "auto" else args.path)
     dev = torch.device("cuda:0")
     stream = torch.cuda.current_stream(dev)
     a.set_stream(stream.cuda_stream)
@@ -216,6 +218,8 @@ def main():
     e = torch.empty(1, dtype=torch.float64, device=dev)
     vals = torch.empty(a.nnz, dtype=vdt, device=dev)
     torch.cuda.synchronize()
+
+    tuning = a.autotune_dev(x.data_ptr()) if args.path == "auto" else None
 
     def step():
         a.energy_gradient_hessian_dev(x.data_ptr(), e.data_ptr(), g.data_ptr(), vals.data_ptr())
@@ -241,7 +245,7 @@ def main():
     alg = algorithmic_bytes(a.n_dofs, m.n_elements, 10, m.vertices.shape[0], a.nnz, vb, vb)
     achieved = alg / (kernel_ms / 1000.0) / 1e9
     traffic = None
-    tfile = os.path.join(ROOT, "profiles", f"traffic_egh_p2_{n}_{args.precision}_{args.path}.json")
+    tfile = os.path.join(ROOT, "profiles", f"traffic_egh_p2_{n}_{args.precision}_{a.path}.json")
     if os.path.exists(tfile):
         traffic = json.load(open(tfile)).get("bytes_per_launch")
 
@@ -273,7 +277,8 @@ def main():
         "dtype": "f32" if args.precision == "mixed" else "f64",
         "data": "synthetic (Kuhn box mesh, seeded displacement and Lame parameters)",
         "config": {"workload": f"P2 NeoHookean E+G+H assembly, {n}^3x6 tets ({m.n_elements}), 4 qp, "
-                               f"{args.precision} precision, {args.path} path",
+                               f"{args.precision} precision, {a.path} path",
+                   "tuning": tuning,
                    "n_elements": m.n_elements, "n_dofs": a.n_dofs, "nnz": a.nnz, "quadrature_points": a.n_quad},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,

@@ -103,6 +103,24 @@ pfgpu_status pfgpu_info(const pfgpu_handle *h, pfgpu_info_t *info);
 /* hipStream_t passed as void*; NULL selects the default stream */
 pfgpu_status pfgpu_set_stream(pfgpu_handle *h, void *stream);
 pfgpu_status pfgpu_set_path(pfgpu_handle *h, int32_t path);
+/* Launch-shape autotuner (no reference counterpart): times every candidate
+ * configuration of the energy + gradient + Hessian assembly on this handle's
+ * mesh at state x (device pointer; 3 timed runs each, after one warm-up) and
+ * keeps the fastest.  Candidates: the fused path, and the plain path with
+ * block-gather workgroups of 128/256/512 threads and Hessian-kernel
+ * workgroups of 64/128/256 threads.  All candidates give identical results
+ * up to the summation order of the path. */
+typedef struct {
+    int32_t path, gather_threads, hessian_threads; /* chosen configuration */
+    int32_t candidates;                            /* configurations timed */
+    double ms;                                     /* best time, E+G+H */
+    double ms_plain_default;                       /* plain, 256/256 */
+    double ms_fused;                               /* fused (negative: not available) */
+} pfgpu_tune_report;
+pfgpu_status pfgpu_autotune(pfgpu_handle *h, const double *x, pfgpu_tune_report *report);
+/* sets a configuration directly (gather_threads in {128,256,512},
+ * hessian_threads in {64,128,256}) */
+pfgpu_status pfgpu_set_tuning(pfgpu_handle *h, int32_t path, int32_t gather_threads, int32_t hessian_threads);
 /* replaces ElasticityAssembler::update_lame_params (Assembler.hpp:147-156) */
 pfgpu_status pfgpu_set_lame(pfgpu_handle *h, const double *lambda, const double *mu);
 

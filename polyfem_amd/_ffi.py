@@ -51,6 +51,12 @@ class TransientParams(C.Structure):
     _fields_ = [("dt", C.c_double), ("time_steps", C.c_int32), ("checkpoint_every", C.c_int32)]
 
 
+class TuneReport(C.Structure):
+    _fields_ = [("path", C.c_int32), ("gather_threads", C.c_int32), ("hessian_threads", C.c_int32),
+                ("candidates", C.c_int32), ("ms", C.c_double), ("ms_plain_default", C.c_double),
+                ("ms_fused", C.c_double)]
+
+
 class PfgpuError(RuntimeError):
     def __init__(self, status: int, message: str):
         super().__init__(f"pfgpu {STATUS_NAMES.get(status, status)}: {message}")
@@ -91,6 +97,8 @@ _SIGNATURES = {
     "pfgpu_cg_solve_dev": [vp, vp, vp, vp, vp, C.c_int32, C.c_double, C.c_int32, C.POINTER(C.c_int32),
                            C.POINTER(C.c_double)],
     "pfgpu_set_load": [vp, vp],
+    "pfgpu_autotune": [vp, vp, C.POINTER(TuneReport)],
+    "pfgpu_set_tuning": [vp, C.c_int32, C.c_int32, C.c_int32],
     "pfgpu_newton_solve_batch": [vp, vp, vp, C.c_int32, vp, vp, C.POINTER(NewtonParams), vp],
     "pfgpu_energy_lame_gradient": [vp, vp, vp],
     "pfgpu_set_density": [vp, vp, C.c_double],

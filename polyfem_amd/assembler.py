@@ -105,6 +105,20 @@ class NLAssembler:
         check(lib().pfgpu_set_path(self._h, _PATHS[path]))
         self.path = path
 
+    def autotune_dev(self, x_ptr: int) -> dict:
+        """Times the launch-shape candidates of the E+G+H assembly at the
+        device state x and keeps the fastest (pfgpu_autotune)."""
+        r = _ffi.TuneReport()
+        check(lib().pfgpu_autotune(self._h, C.c_void_p(x_ptr), C.byref(r)))
+        self.path = {v: k for k, v in _PATHS.items()}[r.path]
+        return {"path": self.path, "gather_threads": r.gather_threads, "hessian_threads": r.hessian_threads,
+                "candidates": r.candidates, "ms": r.ms, "ms_plain_default": r.ms_plain_default,
+                "ms_fused": r.ms_fused if r.ms_fused >= 0 else None}
+
+    def set_tuning(self, path: str, gather_threads: int = 256, hessian_threads: int = 256):
+        check(lib().pfgpu_set_tuning(self._h, _PATHS[path], gather_threads, hessian_threads))
+        self.path = path
+
     def set_stream(self, stream_handle: int):
         check(lib().pfgpu_set_stream(self._h, C.c_void_p(stream_handle)))
 

@@ -267,6 +267,91 @@ pfgpu_status pfgpu_set_stream(pfgpu_handle *h, void *stream)
     });
 }
 
+pfgpu_status pfgpu_set_tuning(pfgpu_handle *h, int32_t path, int32_t gather_threads, int32_t hessian_threads)
+{
+    return guard([&] {
+        check_handle(h);
+        require(path == PFGPU_PATH_FUSED || path == PFGPU_PATH_PLAIN, "unknown path");
+        require(gather_threads == 128 || gather_threads == 256 || gather_threads == 512, "gather_threads must be 128, 256 or 512");
+        require(hessian_threads == 64 || hessian_threads == 128 || hessian_threads == 256,
+                "hessian_threads must be 64, 128 or 256");
+        h->path = path;
+        h->tune_gather_threads = gather_threads;
+        h->tune_hessian_threads = hessian_threads;
+    });
+}
+
+pfgpu_status pfgpu_autotune(pfgpu_handle *h, const double *x, pfgpu_tune_report *report)
+{
+    return guard([&] {
+        check_handle(h);
+        require(x && report, "null argument");
+        const size_t nnz = size_t(h->pat.nnz);
+        DevBuf out(sizeof(double) * (size_t(h->n_dofs) + 1) + h->value_size() * nnz);
+        double *de = out.as<double>(), *dg = de + 1;
+        void *dv = dg + h->n_dofs;
+        hipEvent_t t0, t1;
+        PF_HIP(hipEventCreate(&t0));
+        PF_HIP(hipEventCreate(&t1));
+        auto time_config = [&]() {
+            egh_dev(*h, x, de, dg, dv); // warm-up; builds any table of the configuration
+            float best = 1e30f;
+            for (int r = 0; r < 3; ++r) {
+                PF_HIP(hipEventRecord(t0, h->stream));
+                egh_dev(*h, x, de, dg, dv);
+                PF_HIP(hipEventRecord(t1, h->stream));
+                PF_HIP(hipEventSynchronize(t1));
+                float ms = 0;
+                PF_HIP(hipEventElapsedTime(&ms, t0, t1));
+                best = std::min(best, ms);
+            }
+            return double(best);
+        };
+        pfgpu_tune_report r{};
+        r.ms = 1e300;
+        r.ms_fused = -1;
+        const int G[3] = {128, 256, 512}, HB[3] = {64, 128, 256};
+        try {
+            for (int gi = 0; gi < 3; ++gi)
+                for (int hi = 0; hi < 3; ++hi) {
+                    h->path = PFGPU_PATH_PLAIN;
+                    h->tune_gather_threads = G[gi];
+                    h->tune_hessian_threads = HB[hi];
+                    const double ms = time_config();
+                    r.candidates++;
+                    if (G[gi] == 256 && HB[hi] == 256) r.ms_plain_default = ms;
+                    if (ms < r.ms) {
+                        r.ms = ms;
+                        r.path = PFGPU_PATH_PLAIN;
+                        r.gather_threads = G[gi];
+                        r.hessian_threads = HB[hi];
+                    }
+                }
+            h->path = PFGPU_PATH_FUSED;
+            try {
+                r.ms_fused = time_config();
+                r.candidates++;
+                if (r.ms_fused < r.ms) {
+                    r.ms = r.ms_fused;
+                    r.path = PFGPU_PATH_FUSED;
+                }
+            } catch (const Error &e) {
+                if (e.status != PFGPU_ERR_UNSUPPORTED) throw;
+            }
+        } catch (...) {
+            (void)hipEventDestroy(t0);
+            (void)hipEventDestroy(t1);
+            throw;
+        }
+        (void)hipEventDestroy(t0);
+        (void)hipEventDestroy(t1);
+        h->path = r.path;
+        h->tune_gather_threads = r.gather_threads ? r.gather_threads : 256;
+        h->tune_hessian_threads = r.hessian_threads ? r.hessian_threads : 256;
+        *report = r;
+    });
+}
+
 pfgpu_status pfgpu_set_path(pfgpu_handle *h, int32_t path)
 {
     return guard([&] {

@@ -24,7 +24,10 @@ struct pfgpu_handle {
     pfgpu::DevBuf brow_ptr, block_row, contrib_ptr, contrib;
     // chunks of the segmented gathers (node rows; pattern blocks)
     pfgpu::DevBuf row_chunk, blk_chunk;
-    int n_row_chunks = 0, n_blk_chunks = 0;
+    int n_row_chunks = 0, n_blk_chunks = 0, blk_chunk_threads = 0;
+    std::vector<int32_t> h_contrib_ptr; // host copy: re-chunking for another gather size
+    // launch shapes of the plain path (set by pfgpu_autotune / pfgpu_set_tuning)
+    int tune_gather_threads = 256, tune_hessian_threads = 256;
     // fused-path patch tables (built on first use)
     bool fused_ready = false;
     pfgpu::DevBuf patch_desc, patch_rows, patch_row_acc, patch_elems, patch_cs, patch_cg, patch_hdr, patch_pos, patch_vnodes;

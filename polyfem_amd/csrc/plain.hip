@@ -149,18 +149,18 @@ __global__ __launch_bounds__(BLOCK) void k_gradient(DevMesh<T> m, const double *
 // hloc[(((e / 64) NBLK + s) 64 + e % 64) 9], row-major.  A wavefront computes
 // the blocks of its 64 elements and writes each block index s as one
 // contiguous 2304-byte run, staged through LDS.
-template <typename T, int ORDER, int NQ, int MAT>
-__global__ __launch_bounds__(BLOCK) void k_hessian(DevMesh<T> m, const double *__restrict__ x, T *__restrict__ hloc)
+template <typename T, int ORDER, int NQ, int MAT, int HB>
+__global__ __launch_bounds__(HB) void k_hessian(DevMesh<T> m, const double *__restrict__ x, T *__restrict__ hloc)
 {
     constexpr int NB = NLocal<ORDER>::value, NBLK = NB * (NB + 1) / 2;
-    __shared__ T stage[BLOCK / 64][64 * 9];
-    const int e = blockIdx.x * BLOCK + threadIdx.x;
+    __shared__ T stage[HB / 64][64 * 9];
+    const int e = blockIdx.x * HB + threadIdx.x;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     // every lane of a wavefront takes part in the staged stores; lanes past
     // the last element compute on element 0 and their blocks are never read
     const bool valid = e < m.n_elements;
     const int ee = valid ? e : 0;
-    const int64_t tile = (int64_t(blockIdx.x) * BLOCK + wv * 64) / 64;
+    const int64_t tile = (int64_t(blockIdx.x) * HB + wv * 64) / 64;
     if (tile * 64 >= m.n_elements) return; // whole wavefront past the end (uniform)
     Elem<T, ORDER, NQ> el;
     load_geometry(m, ee, el);
@@ -496,14 +496,14 @@ __global__ __launch_bounds__(BLOCK) void k_gather_rows_seg(int n_chunks, const i
 }
 
 // src[k]: index of contribution k's block in hloc (bit 31: stored transposed)
-template <typename T>
-__global__ __launch_bounds__(BLOCK) void k_gather_blocks_seg(int n_chunks, const int32_t *__restrict__ chunk,
+template <typename T, int GS>
+__global__ __launch_bounds__(GS) void k_gather_blocks_seg(int n_chunks, const int32_t *__restrict__ chunk,
                                                              const int32_t *__restrict__ brow_ptr,
                                                              const int32_t *__restrict__ block_row,
                                                              const int32_t *__restrict__ cptr, const int32_t *__restrict__ src,
                                                              const T *__restrict__ hloc, T *__restrict__ values)
 {
-    __shared__ T sv[SEG * 9];
+    __shared__ T sv[GS * 9];
     const int c = xcd_chunk(n_chunks);
     if (c >= n_chunks) return;
     const int b0 = chunk[c], b1 = chunk[c + 1];
@@ -526,7 +526,7 @@ __global__ __launch_bounds__(BLOCK) void k_gather_blocks_seg(int n_chunks, const
         }
     }
     __syncthreads();
-    for (int b = b0 + t; b < b1; b += BLOCK) {
+    for (int b = b0 + t; b < b1; b += GS) {
         T acc[9];
 #pragma unroll
         for (int i = 0; i < 9; ++i) acc[i] = T(0);
@@ -628,13 +628,13 @@ void plain_gradient(Handle &h, const double *x, double *g, double *energy)
 
 // chunk starts over items 0..n-1 with contribution offsets ptr: consecutive
 // items, at most SEG contributions per chunk
-template <typename P> std::vector<int32_t> segment_chunks(const P *ptr, int64_t n)
+template <typename P> std::vector<int32_t> segment_chunks(const P *ptr, int64_t n, int cap = SEG)
 {
     std::vector<int32_t> c{0};
     int64_t k0 = ptr[0];
     for (int64_t i = 0; i < n; ++i) {
-        if (ptr[i + 1] - ptr[i] > SEG) fail(PFGPU_ERR_UNSUPPORTED, "an output gathers more than 256 contributions");
-        if (ptr[i + 1] - k0 > SEG) {
+        if (ptr[i + 1] - ptr[i] > cap) fail(PFGPU_ERR_UNSUPPORTED, "an output gathers more contributions than a gather workgroup holds");
+        if (ptr[i + 1] - k0 > cap) {
             c.push_back(int32_t(i));
             k0 = ptr[i];
         }
@@ -689,27 +689,50 @@ void ensure_plain_hessian_maps(Handle &h)
     }
     h.contrib_ptr = upload(ptr, h.stream);
     h.contrib = upload(contrib, h.stream);
-    {
-        const std::vector<int32_t> c = segment_chunks(ptr.data(), h.pat.n_blocks);
-        h.blk_chunk = upload(c, h.stream);
-        h.n_blk_chunks = int(c.size()) - 1;
-    }
+    h.h_contrib_ptr = std::move(ptr);
+    h.blk_chunk_threads = 0;
     h.block_row = upload(block_row, h.stream);
     h.brow_ptr = upload(h.pat.brow_ptr, h.stream);
     h.plain_hess_ready = true;
 }
 
+// chunks of the block gather for its workgroup size (rebuilt when the tuning changes)
+void ensure_block_chunks(Handle &h)
+{
+    if (h.blk_chunk_threads == h.tune_gather_threads) return;
+    const std::vector<int32_t> c = segment_chunks(h.h_contrib_ptr.data(), h.pat.n_blocks, h.tune_gather_threads);
+    h.blk_chunk = upload(c, h.stream);
+    h.n_blk_chunks = int(c.size()) - 1;
+    h.blk_chunk_threads = h.tune_gather_threads;
+}
+
 void plain_hessian(Handle &h, const double *x, void *values)
 {
     ensure_plain_hessian_maps(h);
-    const unsigned grid = grid_for(h.n_elements);
+    ensure_block_chunks(h);
     h.hloc.reserve(h.value_size() * size_t((h.n_elements + 63) / 64) * 64 * h.n_blocks_sym() * 9);
     dispatch(h, [&]<typename T, int O, int Q, int M>() {
-        k_hessian<T, O, Q, M><<<grid, BLOCK, 0, h.stream>>>(h.mesh<T>(), x, h.hloc.as<T>());
+        auto hess = [&]<int HB>() {
+            k_hessian<T, O, Q, M, HB><<<grid_for(h.n_elements, HB), HB, 0, h.stream>>>(h.mesh<T>(), x, h.hloc.as<T>());
+        };
+        if (h.tune_hessian_threads == 64)
+            hess.template operator()<64>();
+        else if (h.tune_hessian_threads == 128)
+            hess.template operator()<128>();
+        else
+            hess.template operator()<256>();
         PF_LAUNCH_CHECK();
-        k_gather_blocks_seg<T><<<xcd_grid(h.n_blk_chunks), BLOCK, 0, h.stream>>>(
-            h.n_blk_chunks, h.blk_chunk.as<int32_t>(), h.brow_ptr.as<int32_t>(), h.block_row.as<int32_t>(), h.contrib_ptr.as<int32_t>(),
-            h.contrib.as<int32_t>(), h.hloc.as<T>(), static_cast<T *>(values));
+        auto gath = [&]<int GS>() {
+            k_gather_blocks_seg<T, GS><<<xcd_grid(h.n_blk_chunks), GS, 0, h.stream>>>(
+                h.n_blk_chunks, h.blk_chunk.as<int32_t>(), h.brow_ptr.as<int32_t>(), h.block_row.as<int32_t>(),
+                h.contrib_ptr.as<int32_t>(), h.contrib.as<int32_t>(), h.hloc.as<T>(), static_cast<T *>(values));
+        };
+        if (h.tune_gather_threads == 128)
+            gath.template operator()<128>();
+        else if (h.tune_gather_threads == 512)
+            gath.template operator()<512>();
+        else
+            gath.template operator()<256>();
     });
     PF_LAUNCH_CHECK();
 }

@@ -169,3 +169,26 @@ def test_energy_lame_gradient(order, name):
     assert rel(d[:E], o_l.energy_per_element(x)) < FP64_TOL
     assert rel(d[E:], o_m.energy_per_element(x)) < FP64_TOL
     assert abs(lam @ d[:E] + mu @ d[E:] - o.energy(x)) <= FP64_TOL * abs(o.energy(x))
+
+
+def test_autotune_and_tunings_agree():
+    """Every plain launch shape gives bit-identical results (the summation
+    order does not depend on the workgroup sizes); the autotuner's choice
+    matches the oracle."""
+    torch = pytest.importorskip("torch")
+    m, x, a, o, lam, mu = setup_case(2, "NeoHookean", grid=(4, 3, 3), precision="mixed", path="plain")
+    ref = None
+    for gt in (128, 256, 512):
+        for ht in (64, 128, 256):
+            a.set_tuning("plain", gt, ht)
+            e, g, vals = a.assemble_energy_gradient_hessian(x)
+            if ref is None:
+                ref = (e, g, vals)
+            assert e == ref[0] and np.array_equal(g, ref[1]) and np.array_equal(vals, ref[2]), (gt, ht)
+    xd = torch.tensor(x, device="cuda:0")
+    rep = a.autotune_dev(xd.data_ptr())
+    print("autotune:", rep, flush=True)
+    assert rep["candidates"] >= 9 and rep["ms"] > 0
+    e, g, vals = a.assemble_energy_gradient_hessian(x)
+    _, _, hv = o.hessian(x)
+    assert rel(vals, hv) < MIXED_TOL["hessian"] and rel(g, o.gradient(x)) < MIXED_TOL["gradient"]
