@@ -543,56 +543,6 @@ __global__ __launch_bounds__(GS) void k_gather_blocks_seg(int n_chunks, const in
     }
 }
 
-// one thread per 3x3 block of the pattern; contributions in element order.
-// src[k]: index of the contribution's block in hloc (bit 31: stored
-// transposed); contributions are fetched four at a time.
-template <typename T>
-__global__ __launch_bounds__(BLOCK) void k_gather_blocks(int64_t n_blocks, const int32_t *__restrict__ brow_ptr,
-                                                         const int32_t *__restrict__ block_row, const int32_t *__restrict__ cptr,
-                                                         const int32_t *__restrict__ src, const T *__restrict__ hloc,
-                                                         T *__restrict__ values)
-{
-    const int64_t b = int64_t(blockIdx.x) * BLOCK + threadIdx.x;
-    if (b >= n_blocks) return;
-    T acc[9];
-#pragma unroll
-    for (int i = 0; i < 9; ++i) acc[i] = T(0);
-    const int k0 = cptr[b], k1 = cptr[b + 1];
-    for (int k = k0; k < k1; k += 4) {
-        int32_t c[4];
-#pragma unroll
-        for (int j = 0; j < 4; ++j) c[j] = k + j < k1 ? src[k + j] : 0;
-        T h[4][9];
-#pragma unroll
-        for (int j = 0; j < 4; ++j)
-            if (k + j < k1) {
-                const T *p = hloc + int64_t(c[j] & 0x7fffffff) * 9;
-#pragma unroll
-                for (int i = 0; i < 9; ++i) h[j][i] = p[i];
-            }
-#pragma unroll
-        for (int j = 0; j < 4; ++j)
-            if (k + j < k1) {
-                if (c[j] >= 0) {
-#pragma unroll
-                    for (int i = 0; i < 9; ++i) acc[i] += h[j][i];
-                } else {
-#pragma unroll
-                    for (int mm = 0; mm < 3; ++mm)
-#pragma unroll
-                        for (int nn = 0; nn < 3; ++nn) acc[mm * 3 + nn] += h[j][nn * 3 + mm];
-                }
-            }
-    }
-    const int row = block_row[b];
-    const int r0 = brow_ptr[row], deg = brow_ptr[row + 1] - r0;
-    T *out = values + 9 * int64_t(r0) + 3 * (b - r0);
-#pragma unroll
-    for (int mm = 0; mm < 3; ++mm)
-#pragma unroll
-        for (int nn = 0; nn < 3; ++nn) out[int64_t(mm) * 3 * deg + nn] = acc[mm * 3 + nn];
-}
-
 } // namespace
 
 // ------------------------------------------------------------------ launchers
