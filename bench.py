@@ -166,6 +166,277 @@ def extras(args, m, lam, mu, x_host, a, x, g, vals, y, eqp):
     return res
 
 
+def suite(args):
+    """Further workloads at full size, one JSON line each (timings in seconds
+    unless named otherwise)."""
+    import numpy as np
+    import torch
+
+    from polyfem_amd import BatchSolver, NewtonSolver, NLAssembler, NLProblem, TransientSimulation, cg_solve
+    from polyfem_amd.mesh import (BASE_SEED, face_nodes, kuhn_box, lame_from_young, random_lame,
+                                  random_smooth_field, smooth_displacement)
+
+    torch.cuda.is_available()
+
+    def clock():
+        torch.cuda.synchronize()
+        return time.perf_counter()
+
+    def log(msg):
+        print(msg, file=sys.stderr, flush=True)
+
+    # ---- P1 20^3 x 6, FP64, E = 1e6, nu = 0.4: E+G+H, then one Newton step on the
+    # free dofs (bottom clamped, top moved by +0.1 z) with alpha = 1, and the same
+    # step with the backtracking line search
+    m = kuhn_box(20, 20, 20, order=1)
+    a = NLAssembler("NeoHookean", m, young=1e6, poisson=0.4, precision="fp64")
+    x = smooth_displacement(m.nodes, 0.05, 1e-3, BASE_SEED)
+    bot, top = face_nodes(m, 2, 0.0), face_nodes(m, 2, 1.0)
+    fixed = np.concatenate([3 * bot[:, None] + np.arange(3), 3 * top[:, None] + np.arange(3)]).ravel()
+    prob = NLProblem(m.n_dofs, fixed, np.concatenate([np.zeros(3 * bot.size), np.tile([0.0, 0.0, 0.1], top.size)]))
+    for _ in range(3):
+        a.assemble_energy_gradient_hessian(x)
+    ts = []
+    for _ in range(20):
+        t0 = clock()
+        a.assemble_energy_gradient_hessian(x)
+        ts.append(clock() - t0)
+    x0 = x.copy()
+    x0[prob.fixed] = prob.fixed_values
+    g0 = float(np.linalg.norm(a.assemble_gradient(x0)[prob.free]))
+    steps = {}
+    for name, ls in (("alpha_1", "None"), ("backtracking", "Backtracking")):
+        t0 = clock()
+        x1, rep = NewtonSolver(max_iterations=1, grad_norm=0.0, cg_max_iterations=20000, cg_tolerance=1e-10,
+                               line_search=ls).minimize(a, x0, prob)
+        tn = clock() - t0
+        e1 = a.assemble_energy_per_element(x1) if name == "backtracking" else None
+        steps[name] = {"seconds": tn, "cg_iterations": rep.cg_iterations, "step_length": rep.last_alpha,
+                       "grad_norm_after": rep.grad_norm if np.isfinite(rep.grad_norm) else None,
+                       "valid_after": bool(np.isfinite(rep.grad_norm))}
+        if e1 is not None:
+            steps[name]["energy_after"] = float(e1.sum())
+    print(json.dumps({"workload": "P1 NeoHookean 20^3x6 (48,000 tets, 27,783 dofs), FP64, 1 qp",
+                      "e_g_h_ms_host_api": 1e3 * statistics.median(ts),
+                      "elem_qp_per_s_host_api": m.n_elements / statistics.median(ts),
+                      "grad_norm_before": g0, "newton_step": steps}), flush=True)
+    a.close()
+
+    # ---- batch: 256 box meshes with tet counts log-uniform in [1k, 200k], problem i P1
+    # if i is even, LinearElasticity if i // 2 is even; per problem E log-normal (mean
+    # ln 1e6, sigma 0.5), nu uniform [0.3, 0.45], bottom clamped and top pulled by 2-8 %
+    # of the height; mixed precision, autotuned launch shapes, graph-replayed CG
+    rng = np.random.default_rng(BASE_SEED + 21)
+    probs = []
+    t_setup = time.perf_counter()
+    for i in range(args.batch_problems):
+        order = 1 + (i % 2)
+        material = "LinearElasticity" if (i // 2) % 2 == 0 else "NeoHookean"
+        cells = np.exp(rng.uniform(np.log(1000), np.log(200000))) / 6
+        aspect = rng.uniform(1.0, 3.0)
+        nx = max(1, int(round((cells / aspect) ** (1 / 3))))
+        nz = max(1, int(round(cells / (nx * nx))))
+        mb = kuhn_box(nx, nx, nz, order=order, size=(1.0, 1.0, nz / nx))
+        lam_i, mu_i = lame_from_young(np.exp(rng.normal(np.log(1e6), 0.5)), rng.uniform(0.3, 0.45))
+        pull = rng.uniform(0.02, 0.08) * (nz / nx)
+        bot, top = face_nodes(mb, 2, 0.0), face_nodes(mb, 2, nz / nx)
+        fixed = np.concatenate([3 * bot[:, None] + np.arange(3), 3 * top[:, None] + np.arange(3)]).ravel()
+        vals = np.concatenate([np.zeros(3 * bot.size), np.tile([0.0, 0.0, pull], top.size)])
+        x0 = np.zeros(mb.n_dofs)
+        x0[2::3] = pull * mb.nodes[:, 2] / (nz / nx)
+        probs.append(dict(mesh=mb, material=material, lame=(float(lam_i), float(mu_i)), fixed_dofs=fixed,
+                          fixed_values=vals, x0=x0))
+    n_tets = sum(p["mesh"].n_elements for p in probs)
+    newton = NewtonSolver(max_iterations=args.batch_newton, grad_norm=0.0, cg_max_iterations=args.batch_cg,
+                          cg_tolerance=1e-6)
+    bs = BatchSolver(probs, precision="mixed", newton=newton)
+    tuning, g0 = [], np.zeros(len(probs))
+    for grp in bs.groups:
+        ga = grp["a"]
+        xg = torch.zeros(ga.mesh.n_dofs, dtype=torch.float64, device="cuda")
+        tuning.append(ga.autotune_dev(xg.data_ptr()))
+        # initial reduced gradient norm of every problem of the group (untimed)
+        x0g = np.concatenate([probs[i]["x0"] for i in grp["idx"]])
+        x0g[grp["problem"].fixed] = grp["problem"].fixed_values
+        gg = ga.assemble_gradient(x0g)
+        gg[grp["problem"].fixed] = 0.0
+        for k, i in enumerate(grp["idx"]):
+            g0[i] = np.linalg.norm(gg[3 * grp["no"][k]:3 * grp["no"][k + 1]])
+    t_setup = time.perf_counter() - t_setup
+    log(f"batch: {len(probs)} problems, {n_tets} tets, setup {t_setup:.1f} s")
+    t0 = clock()
+    xs, reps = bs.solve()
+    t_batch = clock() - t0
+    red = np.array([r.grad_norm for r in reps]) / g0
+    print(json.dumps({
+        "workload": f"batch of {len(probs)} box meshes, {n_tets} tets in all (1k-200k each, half P1 / half P2, "
+                    "half LinearElasticity / half NeoHookean), mixed precision",
+        "newton": {"iterations": args.batch_newton, "cg_max_iterations": args.batch_cg, "cg_tolerance": 1e-6},
+        "solve_s": t_batch, "problems_per_s": len(probs) / t_batch,
+        "cg_iterations_total": int(sum(r.cg_iterations for r in reps)),
+        "grad_reduction": {"median": float(np.median(red)), "max": float(red.max()),
+                           "below_1e-6": int((red <= 1e-6).sum())},
+        "groups": [{"order": int(grp["a"].mesh.order), "material": probs[grp["idx"][0]]["material"],
+                    "problems": len(grp["idx"]), "tets": int(grp["a"].mesh.n_elements), "tuning": tn}
+                   for grp, tn in zip(bs.groups, tuning)]}), flush=True)
+    for grp in bs.groups:
+        grp["a"].close()
+    del bs, probs
+
+    # ---- cantilever: 160 x 40 x 40 cells over [0,4]x[0,1]x[0,1], P1 NeoHookean, FP64,
+    # clamped at x = 0, gravity rho g (rho = 1000, g = 0.2 m/s^2 along -z, a dead load:
+    # the nodal vector f = M g of the initial geometry); Lame parameters log-normal
+    # (sigma 0.3); J = sum over top-face vertices of |X + u - X*|^2 with X* = X + u* and u*
+    # the equilibrium of a second Lame draw.  dJ/d(lambda, mu) and dJ/dX by the static
+    # adjoint, checked by central differences (h = 1e-6 relative for the Lame
+    # parameters, 1e-6 absolute for the vertices) through full Newton solves.
+    m = kuhn_box(160, 40, 40, order=1, size=(4.0, 1.0, 1.0))
+    E = m.n_elements
+    lam, mu = random_lame(E, seed=BASE_SEED + 31, E_sigma=0.3)
+    lam2, mu2 = random_lame(E, seed=BASE_SEED + 32, E_sigma=0.3)
+    clamp = face_nodes(m, 0, 0.0)
+    prob = NLProblem(m.n_dofs, (3 * clamp[:, None] + np.arange(3)).ravel())
+    v2n = np.empty(m.vertices.shape[0], dtype=np.int64)
+    v2n[m.elem_vertices.ravel()] = m.elem_nodes[:, :4].ravel()
+    S = np.flatnonzero(np.abs(m.vertices[:, 2] - 1.0) < 1e-12)
+    Sd = (3 * v2n[S][:, None] + np.arange(3)).ravel()
+    newton = NewtonSolver(max_iterations=25, grad_norm=1e-8, cg_max_iterations=20000, cg_tolerance=1e-10)
+    a = NLAssembler("NeoHookean", m, lame=(lam2, mu2), precision="fp64")
+    g = np.zeros(m.n_dofs)
+    g[2::3] = -0.2
+    f = TransientSimulation(a, rho=1000.0).mass_matvec(g)
+    V0 = m.vertices.copy()
+
+    def equilibrium(u0):
+        a.set_load(f)
+        t0 = clock()
+        u, rep = newton.minimize(a, u0, prob)
+        if not rep.converged:
+            raise RuntimeError(f"cantilever Newton did not converge: {rep}")
+        return u, rep, clock() - t0
+
+    ustar = equilibrium(np.zeros(m.n_dofs))[0]
+    Xstar = V0[S] + ustar[Sd].reshape(-1, 3)
+    a.update_lame_params(lam, mu)
+    u, rep, t_fwd = equilibrium(np.zeros(m.n_dofs))
+    log(f"cantilever forward: {t_fwd:.2f} s, {rep.iterations} Newton, {rep.cg_iterations} CG")
+
+    def J(V, uu):
+        return float(np.sum((V[S] + uu[Sd].reshape(-1, 3) - Xstar) ** 2))
+
+    r = V0[S] + u[Sd].reshape(-1, 3) - Xstar
+    dJdu = np.zeros(m.n_dofs)
+    dJdu[Sd] = 2 * r.ravel()
+    t0 = clock()
+    p, adj_it, adj_rel = cg_solve(a, u, dJdu, prob.mask, max_iterations=20000, tolerance=1e-10)
+    t_adj = clock() - t0
+    t0 = clock()
+    dtheta = a.force_material_derivative(u, p)
+    shape_term = a.force_shape_derivative(u, p)
+    t_vjp = clock() - t0
+    explicit = np.zeros_like(V0)
+    explicit[S] = 2 * r
+    dX = explicit.ravel() - shape_term
+    rng = np.random.default_rng(BASE_SEED + 33)
+    h = 1e-6
+    fd_m, fd_s = [], []
+    t_fd = time.perf_counter()
+    for _ in range(args.fd_directions):
+        dl, dm = rng.normal(size=E) * lam, rng.normal(size=E) * mu
+        a.update_lame_params(lam + h * dl, mu + h * dm)
+        jp = J(V0, equilibrium(u)[0])
+        a.update_lame_params(lam - h * dl, mu - h * dm)
+        jm = J(V0, equilibrium(u)[0])
+        lin = float(dtheta[:E] @ dl + dtheta[E:] @ dm)
+        fd_m.append(abs((jp - jm) / (2 * h) - lin) / abs(lin))
+    a.update_lame_params(lam, mu)
+    for _ in range(args.fd_shape_directions):
+        dV = random_smooth_field(V0, 1.0, int(rng.integers(1 << 62))).reshape(-1, 3)
+        dV[m.vertices[:, 0] < 1e-12] = 0.0  # the clamped face stays where it is
+        a.set_vertices(V0 + h * dV)
+        jp = J(V0 + h * dV, equilibrium(u)[0])
+        a.set_vertices(V0 - h * dV)
+        jm = J(V0 - h * dV, equilibrium(u)[0])
+        lin = float(dX @ dV.ravel())
+        fd_s.append(abs((jp - jm) / (2 * h) - lin) / abs(lin))
+    a.set_vertices(V0)
+    t_fd = time.perf_counter() - t_fd
+    log(f"cantilever FD: material max rel err {max(fd_m, default=0):.2e}, shape {max(fd_s, default=0):.2e}")
+    print(json.dumps({
+        "workload": f"P1 NeoHookean cantilever 160x40x40 cells ({E} tets, {m.n_dofs} dofs), FP64, gravity",
+        "forward_newton_s": t_fwd, "newton_iterations": rep.iterations, "newton_cg_iterations": rep.cg_iterations,
+        "objective": J(V0, u), "adjoint_cg_s": t_adj, "adjoint_cg_iterations": adj_it,
+        "adjoint_rel_residual": adj_rel, "material_and_shape_vjp_s": t_vjp,
+        "grad_norms": {"dJ_dtheta": float(np.linalg.norm(dtheta)), "dJ_dX": float(np.linalg.norm(dX))},
+        "fd_check": {"h": h, "material_directions": len(fd_m), "material_max_rel_err": max(fd_m, default=None),
+                     "shape_directions": len(fd_s), "shape_max_rel_err": max(fd_s, default=None),
+                     "seconds": t_fd}}), flush=True)
+    a.close()
+
+    # ---- implicit Euler: 80^3 x 6 P1 tets, rho = 1000, dt = 1e-3, 100 steps, checkpoints
+    # every 10, v0 a seeded random smooth field of amplitude 0.5; J = 1/2 v_T^T M v_T +
+    # E(u_T); dJ/dv0 and dJ/dmu (the adjoint's implicit part plus the explicit
+    # dE(u_T)/dmu).  Timed in mixed precision and FP64; the FP64 gradient is checked
+    # by central differences (h = 1e-6) on a few random directions.
+    m = kuhn_box(80, 80, 80, order=1)
+    E = m.n_elements
+    lam, mu = random_lame(E, seed=BASE_SEED + 41)
+    v0 = random_smooth_field(m.nodes, 0.5, BASE_SEED + 42)
+
+    def run(asm, sim, vv):
+        xT, vT = sim.forward(np.zeros(m.n_dofs), vv)
+        MvT = sim.mass_matvec(vT)
+        return xT, vT, MvT, float(0.5 * vT @ MvT + asm.assemble_energy(xT))
+
+    out = {"workload": f"implicit Euler, P1 NeoHookean 80^3x6 ({E} tets, {m.n_dofs} dofs), dt 1e-3, 100 steps, "
+                       "checkpoints every 10"}
+    for precision, gn in (("mixed", 5e-8), ("fp64", 1e-11)):
+        asm = NLAssembler("NeoHookean", m, lame=(lam, mu), precision=precision)
+        sim = TransientSimulation(asm, rho=1000.0, dt=1e-3, time_steps=100, checkpoint_every=10,
+                                  newton=NewtonSolver(max_iterations=20, grad_norm=gn, cg_max_iterations=5000,
+                                                      cg_tolerance=1e-10))
+        t0 = clock()
+        xT, vT, MvT, Jv = run(asm, sim, v0)
+        t_fwd = clock() - t0
+        t0 = clock()
+        dth, _, dv0 = sim.adjoint(asm.assemble_gradient(xT), MvT, cg_max_iterations=5000, cg_tolerance=1e-10)
+        t_adj = clock() - t0
+        dmu = dth[E:] + asm.energy_lame_gradient(xT)[E:]
+        log(f"transient {precision}: forward {t_fwd:.2f} s, adjoint {t_adj:.2f} s")
+        out[precision] = {"forward_s": t_fwd, "adjoint_s_with_recomputation": t_adj, "objective": Jv,
+                          "work": sim.stats(),
+                          "grad_norms": {"dJ_dv0": float(np.linalg.norm(dv0)), "dJ_dmu": float(np.linalg.norm(dmu))}}
+        if precision == "mixed":
+            g_mixed = (dv0, dmu)
+            asm.close()
+            continue
+        out["mixed_vs_fp64_gradient_rel_err"] = {
+            "dJ_dv0": float(np.linalg.norm(g_mixed[0] - dv0) / np.linalg.norm(dv0)),
+            "dJ_dmu": float(np.linalg.norm(g_mixed[1] - dmu) / np.linalg.norm(dmu))}
+        rng = np.random.default_rng(BASE_SEED + 43)
+        h = 1e-6
+        fd = []
+        for k in range(args.fd_transient_directions):
+            if k % 2 == 0:
+                dv = random_smooth_field(m.nodes, 0.5, int(rng.integers(1 << 62)))
+                jp, jm = run(asm, sim, v0 + h * dv)[3], run(asm, sim, v0 - h * dv)[3]
+                lin, what = float(dv0 @ dv), "v0"
+            else:
+                dm = rng.normal(size=E) * mu
+                asm.update_lame_params(lam, mu + h * dm)
+                jp = run(asm, sim, v0)[3]
+                asm.update_lame_params(lam, mu - h * dm)
+                jm = run(asm, sim, v0)[3]
+                asm.update_lame_params(lam, mu)
+                lin, what = float(dmu @ dm), "mu"
+            fd.append({"wrt": what, "fd": (jp - jm) / (2 * h), "adjoint": lin,
+                       "rel_err": abs((jp - jm) / (2 * h) - lin) / abs(lin)})
+            log(f"transient FD direction: {fd[-1]}")
+        out["fd_check_fp64"] = {"h": h, "directions": fd}
+        asm.close()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -179,6 +450,19 @@ def main():
     ap.add_argument("--no-extras", action="store_true", help="skip the error, adjoint and Newton measurements")
     ap.add_argument("--newton-iterations", type=int, default=10)
     ap.add_argument("--cg-iterations", type=int, default=100)
+    ap.add_argument("--suite", action="store_true",
+                    help="instead of the headline line: one JSON line per further configuration "
+                         "(P1 20^3 FP64 E+G+H and Newton step; 1.5M-tet cantilever static adjoint; "
+                         "3M-tet implicit-Euler adjoint), timed at full size")
+    ap.add_argument("--batch-problems", type=int, default=256, help="--suite: problems in the batch")
+    ap.add_argument("--batch-newton", type=int, default=10, help="--suite: Newton iterations of the batch")
+    ap.add_argument("--batch-cg", type=int, default=2000, help="--suite: CG iteration cap of the batch")
+    ap.add_argument("--fd-directions", type=int, default=64,
+                    help="--suite: random Lame directions of the cantilever's finite-difference check")
+    ap.add_argument("--fd-shape-directions", type=int, default=4,
+                    help="--suite: random vertex directions of the cantilever's finite-difference check")
+    ap.add_argument("--fd-transient-directions", type=int, default=4,
+                    help="--suite: random directions of the transient finite-difference check")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only copy 1 GiB device to device five times (the reference kernel for the "
                          "FETCH_SIZE / WRITE_SIZE counter scale) and exit")
@@ -198,6 +482,10 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes_each_way": src.numel() * 4, "copies": 5}))
+        return
+
+    if args.suite:
+        suite(args)
         return
 
     from polyfem_amd import NLAssembler

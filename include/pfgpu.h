@@ -123,6 +123,11 @@ pfgpu_status pfgpu_autotune(pfgpu_handle *h, const double *x, pfgpu_tune_report 
 pfgpu_status pfgpu_set_tuning(pfgpu_handle *h, int32_t path, int32_t gather_threads, int32_t hessian_threads);
 /* replaces ElasticityAssembler::update_lame_params (Assembler.hpp:147-156) */
 pfgpu_status pfgpu_set_lame(pfgpu_handle *h, const double *lambda, const double *mu);
+/* new rest positions of the mesh vertices (n_vertices x 3, same topology):
+ * replaces Mesh::set_point over the vertices followed by the rebuild of the
+ * assembly-values cache (optimization/var2sims/ShapeVariableToSimulation.cpp:84-95,
+ * State.cpp:998-1000); the mass matrix, if set, is rebuilt with the same density */
+pfgpu_status pfgpu_set_vertices(pfgpu_handle *h, const double *vertices);
 
 /* ---- assembly, host pointers (synchronous) ------------------------------ */
 
@@ -270,6 +275,20 @@ pfgpu_status pfgpu_transient_forward(pfgpu_handle *h, const double *x0, const do
  * for BDF-1, recomputing the states between checkpoints. */
 pfgpu_status pfgpu_transient_adjoint(pfgpu_handle *h, const double *dJdx, const double *dJdv, int32_t cg_max_iterations,
                                      double cg_tolerance, double *dtheta, double *dx0, double *dv0);
+
+/* work counts of the last pfgpu_transient_forward and pfgpu_transient_adjoint
+ * (the adjoint's recomputed forward steps are counted in recompute_*), the
+ * per-step solver statistics the reference logs */
+typedef struct pfgpu_transient_stats {
+    int64_t steps;
+    int64_t newton_iterations;
+    int64_t cg_iterations;
+    int64_t line_search_halvings;
+    int64_t recompute_newton_iterations;
+    int64_t recompute_cg_iterations;
+    int64_t adjoint_cg_iterations;
+} pfgpu_transient_stats;
+pfgpu_status pfgpu_get_transient_stats(pfgpu_handle *h, pfgpu_transient_stats *stats);
 
 #ifdef __cplusplus
 }

@@ -51,6 +51,12 @@ class TransientParams(C.Structure):
     _fields_ = [("dt", C.c_double), ("time_steps", C.c_int32), ("checkpoint_every", C.c_int32)]
 
 
+class TransientStats(C.Structure):
+    _fields_ = [(f, C.c_int64) for f in ("steps", "newton_iterations", "cg_iterations", "line_search_halvings",
+                                         "recompute_newton_iterations", "recompute_cg_iterations",
+                                         "adjoint_cg_iterations")]
+
+
 class TuneReport(C.Structure):
     _fields_ = [("path", C.c_int32), ("gather_threads", C.c_int32), ("hessian_threads", C.c_int32),
                 ("candidates", C.c_int32), ("ms", C.c_double), ("ms_plain_default", C.c_double),
@@ -97,6 +103,8 @@ _SIGNATURES = {
     "pfgpu_cg_solve_dev": [vp, vp, vp, vp, vp, C.c_int32, C.c_double, C.c_int32, C.POINTER(C.c_int32),
                            C.POINTER(C.c_double)],
     "pfgpu_set_load": [vp, vp],
+    "pfgpu_set_vertices": [vp, vp],
+    "pfgpu_get_transient_stats": [vp, vp],
     "pfgpu_autotune": [vp, vp, C.POINTER(TuneReport)],
     "pfgpu_set_tuning": [vp, C.c_int32, C.c_int32, C.c_int32],
     "pfgpu_newton_solve_batch": [vp, vp, vp, C.c_int32, vp, vp, C.POINTER(NewtonParams), vp],

@@ -196,6 +196,24 @@ class NLAssembler:
         check(lib().pfgpu_energy_lame_gradient(self._h, _ptr(x), _ptr(out)))
         return out
 
+    def set_vertices(self, vertices):
+        """New rest positions of the mesh vertices (n_vertices x 3), same
+        topology: the element geometry (and the mass matrix, if set) follow."""
+        v = np.ascontiguousarray(vertices, dtype=np.float64).reshape(-1, 3)
+        if v.shape[0] != self.n_vertices:
+            raise ValueError("one position per mesh vertex expected")
+        check(lib().pfgpu_set_vertices(self._h, _ptr(v)))
+        self._vx = v
+
+    def set_load(self, f):
+        """External load f (n_dofs) for the Newton solve, which then minimises
+        E(x) - f^T x; None removes it."""
+        if f is None:
+            check(lib().pfgpu_set_load(self._h, None))
+            return
+        f = _f64(f, self.n_dofs)
+        check(lib().pfgpu_set_load(self._h, _ptr(f)))
+
     def force_shape_derivative(self, x, adjoint) -> np.ndarray:
         """term[vertex*3 + d], ElasticForceDerivative.cpp:135-264."""
         x = _f64(x, self.n_dofs)

@@ -37,14 +37,20 @@ void require(bool ok, const std::string &msg)
 
 void check_handle(const pfgpu_handle *h) { require(h != nullptr, "null handle"); }
 
-// upload a host table element-major (E x k) as component-major (k x E)
-template <typename S, typename D> DevBuf upload_soa(const S *src, int E, int k, hipStream_t s)
+// a host table element-major (E x k) as component-major (k x E)
+template <typename S, typename D> std::vector<D> soa(const S *src, int E, int k)
 {
     std::vector<D> v(size_t(E) * k);
     for (int e = 0; e < E; ++e)
         for (int c = 0; c < k; ++c) v[size_t(c) * E + e] = D(src[size_t(e) * k + c]);
-    return upload(v, s);
+    return v;
 }
+template <typename S, typename D> DevBuf upload_soa(const S *src, int E, int k, hipStream_t s)
+{
+    return upload(soa<S, D>(src, E, k), s);
+}
+// element geometry (J^{-T}, det J) of the handle's vertices, in place
+void upload_geometry(pfgpu_handle &h, const pfgpu_mesh_desc &m);
 
 // ElementAssemblyValues::finalize3d for the P1 geometry (ElementAssemblyValues.cpp:73-103):
 // tmp.row(r) = sum_j dphi_j/dxi_r X_j, i.e. rows X1-X0, X2-X0, X3-X0; stores
@@ -84,10 +90,21 @@ void upload_lame(pfgpu_handle &h, const double *lambda, const double *mu)
         lm[size_t(e) * 2] = lambda[e];
         lm[size_t(e) * 2 + 1] = mu[e];
     }
+    // in place: graphs captured over the handle's kernels keep reading the current values
     if (h.precision == PFGPU_MIXED)
-        h.lame = upload_soa<double, float>(lm.data(), h.n_elements, 2, h.stream);
+        upload_into(h.lame, soa<double, float>(lm.data(), h.n_elements, 2), h.stream);
     else
-        h.lame = upload_soa<double, double>(lm.data(), h.n_elements, 2, h.stream);
+        upload_into(h.lame, soa<double, double>(lm.data(), h.n_elements, 2), h.stream);
+}
+
+void upload_geometry(pfgpu_handle &h, const pfgpu_mesh_desc &m)
+{
+    std::vector<double> g;
+    geometry_host(m, g);
+    if (h.precision == PFGPU_MIXED)
+        upload_into(h.geom, soa<double, float>(g.data(), m.n_elements, 10), h.stream);
+    else
+        upload_into(h.geom, soa<double, double>(g.data(), m.n_elements, 10), h.stream);
 }
 
 // staging of host arrays for the synchronous entry points
@@ -199,12 +216,7 @@ pfgpu_status pfgpu_create(const pfgpu_mesh_desc *mesh, const pfgpu_material_desc
         h->h_elem_vertices.assign(m.elem_vertices, m.elem_vertices + size_t(m.n_elements) * 4);
         h->pat = build_pattern(m.n_elements, h->n_local, m.elem_nodes, m.n_nodes, m.n_vertices, m.elem_vertices);
 
-        std::vector<double> g;
-        geometry_host(m, g);
-        if (precision == PFGPU_MIXED)
-            h->geom = upload_soa<double, float>(g.data(), m.n_elements, 10, h->stream);
-        else
-            h->geom = upload_soa<double, double>(g.data(), m.n_elements, 10, h->stream);
+        upload_geometry(*h, m);
 
         if (material->lambda) {
             upload_lame(*h, material->lambda, material->mu);
@@ -368,6 +380,25 @@ pfgpu_status pfgpu_set_lame(pfgpu_handle *h, const double *lambda, const double 
         require(lambda && mu, "null lambda/mu");
         PF_HIP(hipStreamSynchronize(h->stream));
         upload_lame(*h, lambda, mu);
+        solver_release(*h);
+    });
+}
+
+pfgpu_status pfgpu_set_vertices(pfgpu_handle *h, const double *vertices)
+{
+    return guard([&] {
+        check_handle(h);
+        require(vertices, "null vertices");
+        const size_t n = size_t(h->n_vertices) * 3;
+        for (size_t i = 0; i < n; ++i) require(std::isfinite(vertices[i]), "non-finite vertex coordinate");
+        PF_HIP(hipStreamSynchronize(h->stream));
+        pfgpu_mesh_desc m{h->order, 0, h->n_elements, h->n_nodes, h->h_elem_nodes.data(), h->n_vertices, vertices,
+                          h->h_elem_vertices.data()};
+        upload_geometry(*h, m); // validates every element before anything changes
+        h->h_vertices.assign(vertices, vertices + n);
+        if (h->has_mass) build_mass(*h, h->h_rho.empty() ? nullptr : h->h_rho.data(), h->rho_uniform);
+        solver_release(*h);
+        PF_HIP(hipStreamSynchronize(h->stream));
     });
 }
 
@@ -760,6 +791,15 @@ pfgpu_status pfgpu_cg_solve_dev(pfgpu_handle *h, const double *x, const uint8_t 
     });
 }
 
+pfgpu_status pfgpu_get_transient_stats(pfgpu_handle *h, pfgpu_transient_stats *stats)
+{
+    return guard([&] {
+        check_handle(h);
+        require(stats, "null argument");
+        *stats = h->tr_stats;
+    });
+}
+
 pfgpu_status pfgpu_set_load(pfgpu_handle *h, const double *f)
 {
     return guard([&] {
@@ -789,6 +829,12 @@ pfgpu_status pfgpu_set_density(pfgpu_handle *h, const double *rho, double rho_un
         else
             require(std::isfinite(rho_uniform) && rho_uniform > 0, "density must be positive");
         build_mass(*h, rho, rho_uniform);
+        if (rho)
+            h->h_rho.assign(rho, rho + h->n_elements);
+        else
+            h->h_rho.clear();
+        h->rho_uniform = rho_uniform;
+        solver_release(*h);
         PF_HIP(hipStreamSynchronize(h->stream));
     });
 }

@@ -75,6 +75,10 @@ class DevBuf {
     size_t n_ = 0;
 };
 
+// v into dst: in place when the size matches (device pointers held elsewhere,
+// e.g. by captured graphs, stay valid), else into a new allocation
+template <typename T> void upload_into(DevBuf &dst, const std::vector<T> &v, hipStream_t s);
+
 template <typename T> DevBuf upload(const std::vector<T> &v, hipStream_t s)
 {
     DevBuf b(v.size() * sizeof(T));
@@ -83,6 +87,17 @@ template <typename T> DevBuf upload(const std::vector<T> &v, hipStream_t s)
         PF_HIP(hipStreamSynchronize(s));
     }
     return b;
+}
+
+template <typename T> void upload_into(DevBuf &dst, const std::vector<T> &v, hipStream_t s)
+{
+    if (dst.get() && dst.bytes() == v.size() * sizeof(T)) {
+        PF_HIP(hipStreamSynchronize(s));
+        PF_HIP(hipMemcpyAsync(dst.get(), v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
+        PF_HIP(hipStreamSynchronize(s));
+    } else {
+        dst = upload(v, s);
+    }
 }
 
 // Element data as the kernels see it.  Every per-element array is SoA

@@ -45,6 +45,8 @@ struct pfgpu_handle {
     // consistent mass matrix: one value per 3x3 block (block = m I_3)
     pfgpu::DevBuf mass, pat_bcol;
     bool has_mass = false;
+    std::vector<double> h_rho; // per-element density of the mass matrix (empty: rho_uniform)
+    double rho_uniform = 0;
     // objective of the solves: w E(x) + 1/2 (x - xt)^T M (x - xt) - w f^T x
     // (xt, f: device vectors or null; the mass term needs xt)
     double obj_w = 1.0;
@@ -58,10 +60,12 @@ struct pfgpu_handle {
     int tr_steps = 0, tr_every = 0;
     bool tr_has_f = false, tr_has_fixed = false;
     pfgpu_newton_params tr_newton{};
+    pfgpu_transient_stats tr_stats{};
+    bool tr_recompute = false; // forward steps run by the adjoint count as recomputation
     // solver workspace: vectors, scalars, the captured CG iteration
     pfgpu::DevBuf sv, h_mask;
     void *cg_graph = nullptr; // hipGraphExec_t
-    std::array<const void *, 4> cg_graph_key{};
+    std::array<const void *, 5> cg_graph_key{};
     hipStream_t own_stream = nullptr; // used when the caller's stream is the legacy default
     ~pfgpu_handle();
 

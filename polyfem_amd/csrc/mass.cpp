@@ -111,7 +111,7 @@ void build_mass(Handle &h, const double *rho, double rho_uniform)
             }
         }
     }
-    h.mass = upload(vals, h.stream);
+    upload_into(h.mass, vals, h.stream);
     if (!h.brow_ptr.get()) h.brow_ptr = upload(h.pat.brow_ptr, h.stream);
     if (!h.pat_bcol.get()) h.pat_bcol = upload(h.pat.bcol, h.stream);
     h.has_mass = true;

@@ -311,6 +311,16 @@ void objective_gradient(Handle &h, const Work &w, const double *x, double *g, in
     PF_LAUNCH_CHECK();
 }
 
+// Rounding level of an objective value e whose elastic part is e_el: the FP64
+// sums contribute a few ulps of |e|; FP32 element energies (mixed precision)
+// contribute a few FP32 ulps of the weighted elastic part.  Energy differences
+// below this are not resolved by the line search.
+double roundoff_noise(const Handle &h, double e, double e_el)
+{
+    const double fp32 = h.precision == PFGPU_MIXED ? 16.0 * 5.96e-8 * std::fabs(h.obj_w * e_el) : 0.0;
+    return 64.0 * 2.2e-16 * std::fabs(e) + fp32;
+}
+
 // host value of the objective whose parts objective_energy/gradient left in scal
 double objective_value(Handle &h, const Work &w, int slot)
 {
@@ -353,7 +363,9 @@ int cg_solve_ws(Handle &h, const double *x, const uint8_t *fixed, const double *
     if (b2 == 0.0 || max_it <= 0) return 0;
     if (!std::isfinite(b2)) fail(PFGPU_ERR_INVALID_STATE, "non-finite right-hand side in the CG solve");
     // the graph replays one iteration; it is rebuilt when its arguments change
-    const std::array<const void *, 4> key{x, fixed, w.r, reinterpret_cast<const void *>(intptr_t(h.obj_generation))};
+    // (the element scratch is grow-only: a reallocation by another call rebuilds the graph)
+    const std::array<const void *, 5> key{x, fixed, w.r, reinterpret_cast<const void *>(intptr_t(h.obj_generation)),
+                                          h.loc.get()};
     if (use_graph && (h.cg_graph == nullptr || h.cg_graph_key != key)) {
         solver_release(h);
         cg_iteration(h, w, x, fixed); // sizes every scratch buffer before capture
@@ -376,6 +388,7 @@ int cg_solve_ws(Handle &h, const double *x, const uint8_t *fixed, const double *
         PF_HIP(e);
         h.cg_graph = exec;
         h.cg_graph_key = key;
+        h.cg_graph_key[4] = h.loc.get(); // as sized by the eager pass
     }
     int it = 0;
     double rr = b2;
@@ -454,10 +467,10 @@ void newton_solve(Handle &h, double *x, const uint8_t *fixed, const pfgpu_newton
             return false;
         };
         // Near the minimum the predicted decrease g.d falls below the rounding
-        // of the FP64 energy sum and the Armijo test cannot resolve it: the
-        // full Newton step is taken (the quadratic model is exact to that
-        // precision) if it keeps every element valid.
-        const double noise = 64.0 * 2.2e-16 * std::fabs(e0);
+        // of the energy and the Armijo test cannot resolve it: the full Newton
+        // step is taken (the quadratic model is exact to that precision) if it
+        // keeps every element valid.
+        const double noise = roundoff_noise(h, e0, read_scalar(h, w, S_E0));
         bool accepted = false;
         if (prm.line_search != 0 && -gd <= noise) {
             k_axpy_to<<<gr, BLOCK, 0, h.stream>>>(w.xt, x, 1.0, dir, h.n_dofs);
@@ -550,6 +563,15 @@ void euler_step(Handle &h, const double *xp, const double *vp, double *xt, doubl
         ObjectiveScope os(h, h.tr_dt * h.tr_dt, xt, h.tr_has_f ? h.tr_f.as<double>() : nullptr);
         pfgpu_newton_report rep{};
         newton_solve(h, x_out, fixed, h.tr_newton, rep);
+        if (h.tr_recompute) {
+            h.tr_stats.recompute_newton_iterations += rep.iterations;
+            h.tr_stats.recompute_cg_iterations += rep.cg_iterations;
+        } else {
+            h.tr_stats.steps++;
+            h.tr_stats.newton_iterations += rep.iterations;
+            h.tr_stats.cg_iterations += rep.cg_iterations;
+            h.tr_stats.line_search_halvings += rep.line_search_halvings;
+        }
         if (!rep.converged) fail(PFGPU_ERR_INVALID_STATE, "a time step's Newton solve did not converge");
     }
     k_euler_update<<<g, BLOCK, 0, h.stream>>>(x_out, xp, h.tr_dt, v_out, n);
@@ -563,6 +585,8 @@ void transient_forward(Handle &h, const double *x0, const double *v0, const uint
 {
     if (!h.has_mass) fail(PFGPU_ERR_INVALID_STATE, "no mass matrix: call pfgpu_set_density first");
     const size_t n = size_t(h.n_dofs), bytes = sizeof(double) * n;
+    h.tr_stats = pfgpu_transient_stats{};
+    h.tr_recompute = false;
     h.tr_dt = dt;
     h.tr_steps = steps;
     h.tr_every = every;
@@ -622,6 +646,12 @@ void transient_adjoint(Handle &h, const double *dJdx, const double *dJdv, int cg
     k_lincomb<<<grid_for(2 * E), BLOCK, 0, h.stream>>>(sum, 0.0, nullptr, 0.0, nullptr, 2 * E);
     PF_LAUNCH_CHECK();
     DevBuf xt_buf(bytes);
+    h.tr_stats.recompute_newton_iterations = h.tr_stats.recompute_cg_iterations = h.tr_stats.adjoint_cg_iterations = 0;
+    struct Recompute {
+        Handle &h;
+        ~Recompute() { h.tr_recompute = false; }
+    } recompute_scope{h};
+    h.tr_recompute = true;
     for (int c = ((T - 1) / K) * K; c >= 0; c -= K) {
         // recompute the states of segment [c, min(c + K, T)] from checkpoint c
         const int e = std::min(c + K, T);
@@ -646,7 +676,7 @@ void transient_adjoint(Handle &h, const double *dJdx, const double *dJdv, int cg
                 // (M + dt^2 H(x_i)) p_i = rhs on the free dofs
                 ObjectiveScope os(h, dt * dt, xi, nullptr);
                 double rel = 0;
-                cg_solve_ws(h, xi, fixed, rhs, cg_max, cg_tol, true, &rel);
+                h.tr_stats.adjoint_cg_iterations += cg_solve_ws(h, xi, fixed, rhs, cg_max, cg_tol, true, &rel);
                 if (!(rel <= std::max(cg_tol, 1e-300) * 10.0) && rel > 1e-6)
                     fail(PFGPU_ERR_INVALID_STATE, "adjoint CG solve did not converge");
             }
@@ -681,47 +711,170 @@ void transient_adjoint(Handle &h, const double *dJdx, const double *dJdv, int cg
 // mesh with disjoint node ranges, so every assembly kernel is one launch for
 // the whole batch and the Hessian is block diagonal.  Problem s owns elements
 // [eo[s], eo[s+1]) and nodes [no[s], no[s+1]).  The Newton iteration runs
-// on all problems at once: one CG on the block-diagonal system (its solution
-// is every problem's Newton direction), then a line search, convergence test
-// and step length per problem, from per-problem FP64 sums.  A converged
-// problem is frozen (its dofs join the fixed mask).
+// on all problems at once, and each problem keeps the iteration it would
+// have alone: the CG on the block-diagonal system carries one (alpha, beta,
+// r.r) per problem and stops each problem at its own relative tolerance;
+// line search, fallbacks, convergence test and step length are per problem
+// too.  A converged problem is frozen (its dofs join the fixed mask).
+//
+// Per-problem sums run over fixed chunks of at most SEG_CHUNK dofs that never
+// straddle two problems: one block per chunk writes a partial, then one
+// thread per problem adds its chunks' partials in order, so a problem of a
+// million dofs is summed by hundreds of blocks and every sum is reproducible.
 
 namespace {
+
+constexpr int SEG_CHUNK = 4096;
+
+// per-problem CG state (device), one entry per problem
+struct SegState {
+    double *rr, *b2, *alpha, *beta, *done, *iters, *active;
+};
+
+__device__ double block_sum(double v, double *sh)
+{
+    const int t = threadIdx.x;
+    sh[t] = v;
+    __syncthreads();
+    for (int w = BLOCK / 2; w > 0; w >>= 1) {
+        if (t < w) sh[t] += sh[t + w];
+        __syncthreads();
+    }
+    return sh[0];
+}
+
+// part[c] = sum over chunk c's free dofs of a b
+__global__ __launch_bounds__(BLOCK) void k_chunk_dot(const double *__restrict__ a, const double *__restrict__ b,
+                                                     const uint8_t *__restrict__ mask, const int32_t *__restrict__ cb,
+                                                     double *__restrict__ part)
+{
+    __shared__ double sh[BLOCK];
+    const int c = blockIdx.x;
+    double acc = 0;
+    for (int i = cb[c] + threadIdx.x; i < cb[c + 1]; i += BLOCK)
+        if (!mask || !mask[i]) acc = fma(a[i], b[i], acc);
+    const double v = block_sum(acc, sh);
+    if (threadIdx.x == 0) part[c] = v;
+}
+
+// per-problem totals of the chunk partials, in chunk order
+__device__ double seg_total(const double *__restrict__ part, const int32_t *__restrict__ scp, int s)
+{
+    double v = 0;
+    for (int c = scp[s]; c < scp[s + 1]; ++c) v += part[c];
+    return v;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_seg_total(const double *__restrict__ part, const int32_t *__restrict__ scp,
+                                                     int nseg, double *__restrict__ out)
+{
+    for (int s = threadIdx.x; s < nseg; s += BLOCK) out[s] = seg_total(part, scp, s);
+}
+
+// CG start from the r.r partials: b2 = rr; a problem with b = 0 is done
+__global__ __launch_bounds__(BLOCK) void k_seg_cg_start(const double *__restrict__ part, const int32_t *__restrict__ scp,
+                                                        int nseg, SegState st)
+{
+    __shared__ double sh[BLOCK];
+    double act = 0;
+    for (int s = threadIdx.x; s < nseg; s += BLOCK) {
+        const double v = seg_total(part, scp, s);
+        st.rr[s] = v;
+        st.b2[s] = v;
+        st.done[s] = v > 0.0 ? 0.0 : 1.0;
+        st.iters[s] = 0.0;
+        act += st.done[s] == 0.0 ? 1.0 : 0.0;
+    }
+    const double a = block_sum(act, sh);
+    if (threadIdx.x == 0) *st.active = a;
+}
+
+// alpha = r.r / p.Hp per problem; a non-positive curvature ends that problem
+__global__ __launch_bounds__(BLOCK) void k_seg_alpha(const double *__restrict__ part, const int32_t *__restrict__ scp,
+                                                     int nseg, SegState st)
+{
+    for (int s = threadIdx.x; s < nseg; s += BLOCK) {
+        const double pq = seg_total(part, scp, s);
+        if (st.done[s] != 0.0) {
+            st.alpha[s] = 0.0;
+        } else if (!(pq > 0.0)) {
+            st.alpha[s] = 0.0;
+            st.done[s] = 1.0;
+        } else {
+            st.alpha[s] = st.rr[s] / pq;
+        }
+    }
+}
+
+// y += alpha p, r -= alpha q on free dofs; partial of the new r.r
+__global__ __launch_bounds__(BLOCK) void k_chunk_cg_update(double *__restrict__ y, double *__restrict__ r,
+                                                           const double *__restrict__ p, const double *__restrict__ q,
+                                                           const uint8_t *__restrict__ mask, const int32_t *__restrict__ cb,
+                                                           const int32_t *__restrict__ cs, const double *__restrict__ alpha,
+                                                           double *__restrict__ part)
+{
+    __shared__ double sh[BLOCK];
+    const int c = blockIdx.x;
+    const double a = alpha[cs[c]];
+    double acc = 0;
+    for (int i = cb[c] + threadIdx.x; i < cb[c + 1]; i += BLOCK) {
+        if (mask && mask[i]) continue;
+        double ri = r[i];
+        if (a != 0.0) {
+            y[i] = fma(a, p[i], y[i]);
+            ri = fma(-a, q[i], ri);
+            r[i] = ri;
+        }
+        acc = fma(ri, ri, acc);
+    }
+    const double v = block_sum(acc, sh);
+    if (threadIdx.x == 0) part[c] = v;
+}
+
+// beta = r'.r' / r.r per problem; convergence at ||r|| <= tol ||b||
+__global__ __launch_bounds__(BLOCK) void k_seg_beta(const double *__restrict__ part, const int32_t *__restrict__ scp,
+                                                    int nseg, double tol2, SegState st)
+{
+    __shared__ double sh[BLOCK];
+    double act = 0;
+    for (int s = threadIdx.x; s < nseg; s += BLOCK) {
+        const double rn = seg_total(part, scp, s);
+        if (!isfinite(rn)) act += __builtin_nan(""); // reported to the host as a failed solve
+        if (st.done[s] == 0.0) {
+            st.iters[s] += 1.0;
+            st.beta[s] = rn / st.rr[s];
+            st.rr[s] = rn;
+            if (rn <= tol2 * st.b2[s]) st.done[s] = 1.0;
+        } else {
+            st.beta[s] = 0.0;
+        }
+        act += st.done[s] == 0.0 ? 1.0 : 0.0;
+    }
+    const double a = block_sum(act, sh);
+    if (threadIdx.x == 0) *st.active = a;
+}
+
+// p = r + beta p for the problems still iterating
+__global__ __launch_bounds__(BLOCK) void k_chunk_cg_direction(double *__restrict__ p, const double *__restrict__ r,
+                                                              const int32_t *__restrict__ cb, const int32_t *__restrict__ cs,
+                                                              const double *__restrict__ beta, const double *__restrict__ done)
+{
+    const int c = blockIdx.x, s = cs[c];
+    if (done[s] != 0.0) return;
+    const double b = beta[s];
+    for (int i = cb[c] + threadIdx.x; i < cb[c + 1]; i += BLOCK) p[i] = fma(b, p[i], r[i]);
+}
 
 // out[s] = sum of v[off[s] .. off[s+1]) (one block per segment, fixed order)
 __global__ __launch_bounds__(BLOCK) void k_seg_sum(const double *__restrict__ v, const int32_t *__restrict__ off,
                                                    double *__restrict__ out)
 {
     __shared__ double sh[BLOCK];
-    const int sgm = blockIdx.x, t = threadIdx.x;
+    const int sgm = blockIdx.x;
     double a = 0;
-    for (int i = off[sgm] + t; i < off[sgm + 1]; i += BLOCK) a += v[i];
-    sh[t] = a;
-    __syncthreads();
-    for (int w = BLOCK / 2; w > 0; w >>= 1) {
-        if (t < w) sh[t] += sh[t + w];
-        __syncthreads();
-    }
-    if (t == 0) out[sgm] = sh[0];
-}
-
-// out[s] = sum over dofs of segment s (nodes no[s]..no[s+1]) of a b, free dofs only
-__global__ __launch_bounds__(BLOCK) void k_seg_dot(const double *__restrict__ a, const double *__restrict__ b,
-                                                   const uint8_t *__restrict__ fixed, const int32_t *__restrict__ no,
-                                                   double *__restrict__ out)
-{
-    __shared__ double sh[BLOCK];
-    const int sgm = blockIdx.x, t = threadIdx.x;
-    double acc = 0;
-    for (int i = 3 * no[sgm] + t; i < 3 * no[sgm + 1]; i += BLOCK)
-        if (!fixed || !fixed[i]) acc = fma(a[i], b[i], acc);
-    sh[t] = acc;
-    __syncthreads();
-    for (int w = BLOCK / 2; w > 0; w >>= 1) {
-        if (t < w) sh[t] += sh[t + w];
-        __syncthreads();
-    }
-    if (t == 0) out[sgm] = sh[0];
+    for (int i = off[sgm] + threadIdx.x; i < off[sgm + 1]; i += BLOCK) a += v[i];
+    const double tot = block_sum(a, sh);
+    if (threadIdx.x == 0) out[sgm] = tot;
 }
 
 // out = x + alpha[seg(i)] d over all dofs; seg: node -> segment
@@ -741,6 +894,56 @@ __global__ __launch_bounds__(BLOCK) void k_seg_mask(uint8_t *__restrict__ mask, 
     if (i < n) mask[i] = (fixed && fixed[i]) || frozen[seg[i / 3]] ? 1 : 0;
 }
 
+// d = -g on the free dofs of the flagged segments (others keep d)
+__global__ __launch_bounds__(BLOCK) void k_seg_steepest(double *__restrict__ d, const double *__restrict__ g,
+                                                        const uint8_t *__restrict__ mask, const uint8_t *__restrict__ flag,
+                                                        const int32_t *__restrict__ seg, int n)
+{
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i < n && flag[seg[i / 3]]) d[i] = mask[i] ? 0.0 : -g[i];
+}
+
+// chunks of at most SEG_CHUNK dofs inside each problem's dof range
+struct SegChunks {
+    DevBuf cb, cs, scp, part;
+    int n = 0;
+};
+
+SegChunks seg_chunks(const int32_t *no, int nseg, hipStream_t st)
+{
+    std::vector<int32_t> cb{0}, cs, scp{0};
+    for (int s = 0; s < nseg; ++s) {
+        const int32_t b = 3 * no[s + 1];
+        for (int32_t i = 3 * no[s]; i < b; i += SEG_CHUNK) {
+            cb.push_back(std::min(b, i + SEG_CHUNK));
+            cs.push_back(s);
+        }
+        scp.push_back(int32_t(cs.size()));
+    }
+    SegChunks c;
+    c.n = int(cs.size());
+    c.cb = upload(cb, st);
+    c.cs = upload(cs, st);
+    c.scp = upload(scp, st);
+    c.part = DevBuf{sizeof(double) * size_t(c.n)};
+    return c;
+}
+
+// One CG iteration on every active problem at once (a fixed kernel sequence).
+void seg_cg_iteration(Handle &h, const Work &w, const double *x, const uint8_t *mask, const SegChunks &ch, int nseg,
+                      double tol2, const SegState &st)
+{
+    hvp_any(h, x, w.p, w.q);
+    k_chunk_dot<<<ch.n, BLOCK, 0, h.stream>>>(w.p, w.q, mask, ch.cb.as<int32_t>(), ch.part.as<double>());
+    k_seg_alpha<<<1, BLOCK, 0, h.stream>>>(ch.part.as<double>(), ch.scp.as<int32_t>(), nseg, st);
+    k_chunk_cg_update<<<ch.n, BLOCK, 0, h.stream>>>(w.y, w.r, w.p, w.q, mask, ch.cb.as<int32_t>(), ch.cs.as<int32_t>(),
+                                                    st.alpha, ch.part.as<double>());
+    k_seg_beta<<<1, BLOCK, 0, h.stream>>>(ch.part.as<double>(), ch.scp.as<int32_t>(), nseg, tol2, st);
+    k_chunk_cg_direction<<<ch.n, BLOCK, 0, h.stream>>>(w.p, w.r, ch.cb.as<int32_t>(), ch.cs.as<int32_t>(), st.beta,
+                                                       st.done);
+    PF_LAUNCH_CHECK();
+}
+
 } // namespace
 
 void newton_solve_batch(Handle &h, double *x, const uint8_t *fixed, int nseg, const int32_t *eo_h, const int32_t *no_h,
@@ -755,17 +958,25 @@ void newton_solve_batch(Handle &h, double *x, const uint8_t *fixed, int nseg, co
     for (int sg = 0; sg < nseg; ++sg)
         for (int i = no_h[sg]; i < no_h[sg + 1]; ++i) seg_of[size_t(i)] = sg;
     DevBuf d_eo = upload(std::vector<int32_t>(eo_h, eo_h + nseg + 1), h.stream);
-    DevBuf d_no = upload(std::vector<int32_t>(no_h, no_h + nseg + 1), h.stream);
     DevBuf d_seg = upload(seg_of, h.stream);
-    DevBuf scal{sizeof(double) * size_t(4 * nseg) + size_t(nseg)}, pe{sizeof(double) * size_t(E)}, mask{size_t(n)};
-    double *s_e = scal.as<double>(), *s_g = s_e + nseg, *s_gd = s_g + nseg, *s_a = s_gd + nseg;
-    uint8_t *frozen = reinterpret_cast<uint8_t *>(s_a + nseg);
-    std::vector<double> e0(nseg), e1(nseg), gn(nseg), gd(nseg), alpha(nseg);
-    std::vector<uint8_t> fr(nseg, 0);
-    std::vector<pfgpu_newton_report> rep(nseg);
+    const SegChunks ch = seg_chunks(no_h, nseg, h.stream);
+    const size_t ns = size_t(nseg);
+    // doubles: energy, gradient norm, g.d, step (4 ns), CG state (6 ns + 1); then two byte flags per problem
+    DevBuf scal{sizeof(double) * (10 * ns + 1) + 2 * ns}, pe{sizeof(double) * size_t(E)}, mask{size_t(n)};
+    double *s_e = scal.as<double>(), *s_g = s_e + ns, *s_gd = s_g + ns, *s_a = s_gd + ns;
+    const SegState st{s_a + ns, s_a + 2 * ns, s_a + 3 * ns, s_a + 4 * ns, s_a + 5 * ns, s_a + 6 * ns, s_a + 7 * ns};
+    uint8_t *frozen = reinterpret_cast<uint8_t *>(s_a + 7 * ns + 1), *flag = frozen + ns;
+    std::vector<double> e0(ns), e1(ns), gn(ns), gd(ns), alpha(ns), it_cg(ns);
+    std::vector<uint8_t> fr(ns, 0), fl(ns, 0);
+    std::vector<pfgpu_newton_report> rep(ns);
     auto read = [&](const double *dev, std::vector<double> &host) {
-        PF_HIP(hipMemcpyAsync(host.data(), dev, sizeof(double) * size_t(nseg), hipMemcpyDeviceToHost, h.stream));
+        PF_HIP(hipMemcpyAsync(host.data(), dev, sizeof(double) * ns, hipMemcpyDeviceToHost, h.stream));
         PF_HIP(hipStreamSynchronize(h.stream));
+    };
+    auto seg_dot = [&](const double *a, const double *b, const uint8_t *m, double *out) {
+        k_chunk_dot<<<ch.n, BLOCK, 0, h.stream>>>(a, b, m, ch.cb.as<int32_t>(), ch.part.as<double>());
+        k_seg_total<<<1, BLOCK, 0, h.stream>>>(ch.part.as<double>(), ch.scp.as<int32_t>(), nseg, out);
+        PF_LAUNCH_CHECK();
     };
     auto seg_energy = [&](const double *xx, std::vector<double> &host) {
         plain_energy(h, xx, w.scal + S_E1, pe.as<double>());
@@ -778,10 +989,104 @@ void newton_solve_batch(Handle &h, double *x, const uint8_t *fixed, int nseg, co
             fused_gradient(h, x, w.g, nullptr);
         else
             plain_gradient(h, x, w.g, nullptr);
-        k_seg_dot<<<nseg, BLOCK, 0, h.stream>>>(w.g, w.g, fixed, d_no.as<int32_t>(), s_g);
-        PF_LAUNCH_CHECK();
+        seg_dot(w.g, w.g, fixed, s_g);
         read(s_g, gn);
         for (double &v : gn) v = std::sqrt(v);
+    };
+    auto upload_flags = [&](uint8_t *dev, const std::vector<uint8_t> &host) {
+        PF_HIP(hipMemcpyAsync(dev, host.data(), ns, hipMemcpyHostToDevice, h.stream));
+    };
+    // the CG iteration, captured once per call: every pointer it reads is fixed
+    // for the call (x, the working mask and the workspace change contents only)
+    hipGraphExec_t graph = nullptr;
+    const void *graph_loc = nullptr; // the element scratch the captured Hv writes
+    struct GraphGuard {
+        hipGraphExec_t &g;
+        ~GraphGuard()
+        {
+            if (g) (void)hipGraphExecDestroy(g);
+        }
+    } guard_graph{graph};
+    const double tol2 = prm.cg_tolerance * prm.cg_tolerance;
+    // per-problem CG on H_ff y = d over the active problems; iterations per problem into it_cg
+    auto seg_cg = [&]() {
+        k_cg_start<<<gr, BLOCK, 0, h.stream>>>(w.y, w.r, w.p, w.d, mask.as<uint8_t>(), n);
+        k_chunk_dot<<<ch.n, BLOCK, 0, h.stream>>>(w.r, w.r, mask.as<uint8_t>(), ch.cb.as<int32_t>(), ch.part.as<double>());
+        k_seg_cg_start<<<1, BLOCK, 0, h.stream>>>(ch.part.as<double>(), ch.scp.as<int32_t>(), nseg, st);
+        PF_LAUNCH_CHECK();
+        if (graph && graph_loc != h.loc.get()) {
+            (void)hipGraphExecDestroy(graph);
+            graph = nullptr;
+        }
+        if (prm.use_graph && !graph) {
+            // an eager iteration sizes every scratch buffer; then restart and capture
+            seg_cg_iteration(h, w, x, mask.as<uint8_t>(), ch, nseg, tol2, st);
+            k_cg_start<<<gr, BLOCK, 0, h.stream>>>(w.y, w.r, w.p, w.d, mask.as<uint8_t>(), n);
+            k_chunk_dot<<<ch.n, BLOCK, 0, h.stream>>>(w.r, w.r, mask.as<uint8_t>(), ch.cb.as<int32_t>(),
+                                                      ch.part.as<double>());
+            k_seg_cg_start<<<1, BLOCK, 0, h.stream>>>(ch.part.as<double>(), ch.scp.as<int32_t>(), nseg, st);
+            hipGraph_t g = nullptr;
+            PF_HIP(hipStreamBeginCapture(h.stream, hipStreamCaptureModeThreadLocal));
+            try {
+                seg_cg_iteration(h, w, x, mask.as<uint8_t>(), ch, nseg, tol2, st);
+            } catch (...) {
+                (void)hipStreamEndCapture(h.stream, &g);
+                if (g) (void)hipGraphDestroy(g);
+                throw;
+            }
+            PF_HIP(hipStreamEndCapture(h.stream, &g));
+            const hipError_t e = hipGraphInstantiate(&graph, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            PF_HIP(e);
+            graph_loc = h.loc.get();
+        }
+        int it = 0;
+        std::vector<double> act(1);
+        while (it < prm.cg_max_iterations) {
+            const int k = std::min(CHECK, prm.cg_max_iterations - it);
+            for (int j = 0; j < k; ++j) {
+                if (graph)
+                    PF_HIP(hipGraphLaunch(graph, h.stream));
+                else
+                    seg_cg_iteration(h, w, x, mask.as<uint8_t>(), ch, nseg, tol2, st);
+            }
+            it += k;
+            PF_HIP(hipMemcpyAsync(act.data(), st.active, sizeof(double), hipMemcpyDeviceToHost, h.stream));
+            PF_HIP(hipStreamSynchronize(h.stream));
+            if (!std::isfinite(act[0])) fail(PFGPU_ERR_INVALID_STATE, "non-finite residual in the batched CG solve");
+            if (act[0] == 0.0) break;
+        }
+        read(st.iters, it_cg);
+    };
+    // per-problem backtracking over the problems flagged in `todo`; alpha holds
+    // each flagged problem's first trial step, slope its g.d
+    auto search = [&](std::vector<uint8_t> &todo, const std::vector<double> &slope) {
+        for (int ls = 0; ls <= prm.max_line_search; ++ls) {
+            PF_HIP(hipMemcpyAsync(s_a, alpha.data(), sizeof(double) * ns, hipMemcpyHostToDevice, h.stream));
+            k_axpy_seg<<<gr, BLOCK, 0, h.stream>>>(w.xt, x, s_a, w.y, d_seg.as<int32_t>(), n);
+            PF_LAUNCH_CHECK();
+            if (prm.line_search == 0) {
+                std::fill(todo.begin(), todo.end(), 0);
+                return;
+            }
+            seg_energy(w.xt, e1);
+            bool all = true;
+            for (int sg = 0; sg < nseg; ++sg) {
+                if (!todo[sg]) continue;
+                // Armijo, or the roundoff regime: a predicted decrease below the
+                // rounding of the FP64 energy sum is taken if every element stays valid
+                const double noise = roundoff_noise(h, e0[sg], e0[sg]);
+                if (std::isfinite(e1[sg]) &&
+                    (e1[sg] - e0[sg] <= prm.armijo_c * alpha[sg] * slope[sg] || (-slope[sg] <= noise && alpha[sg] == 1.0))) {
+                    todo[sg] = 0;
+                } else {
+                    alpha[sg] *= 0.5;
+                    rep[sg].line_search_halvings++;
+                    all = false;
+                }
+            }
+            if (all) return;
+        }
     };
     for (int it = 0; it < prm.max_iterations; ++it) {
         seg_energy(x, e0);
@@ -799,54 +1104,61 @@ void newton_solve_batch(Handle &h, double *x, const uint8_t *fixed, int nseg, co
             any |= !fr[sg];
         }
         if (!any) break;
-        PF_HIP(hipMemcpyAsync(frozen, fr.data(), size_t(nseg), hipMemcpyHostToDevice, h.stream));
+        upload_flags(frozen, fr);
         k_seg_mask<<<gr, BLOCK, 0, h.stream>>>(mask.as<uint8_t>(), fixed, frozen, d_seg.as<int32_t>(), n);
-        // Newton directions of all active problems: one CG on the block-diagonal system
+        // Newton directions of all active problems: H_ff y = -g_f, per-problem CG
         k_neg_free<<<gr, BLOCK, 0, h.stream>>>(w.d, w.g, mask.as<uint8_t>(), n);
         PF_LAUNCH_CHECK();
-        double rel = 0;
-        const int cg_it = cg_solve_ws(h, x, mask.as<uint8_t>(), w.d, prm.cg_max_iterations, prm.cg_tolerance,
-                                      prm.use_graph != 0, &rel);
-        k_seg_dot<<<nseg, BLOCK, 0, h.stream>>>(w.g, w.y, mask.as<uint8_t>(), d_no.as<int32_t>(), s_gd);
-        PF_LAUNCH_CHECK();
+        seg_cg();
+        seg_dot(w.g, w.y, mask.as<uint8_t>(), s_gd);
         read(s_gd, gd);
         // a problem whose direction is not a descent direction takes -g
-        bool steepest = false;
-        for (int sg = 0; sg < nseg; ++sg)
-            if (!fr[sg] && !(gd[sg] < 0.0)) steepest = true;
-        if (steepest) fail(PFGPU_ERR_INVALID_STATE, "batched Newton direction is not a descent direction");
-        // per-problem backtracking
-        std::vector<uint8_t> done(nseg, 0);
+        bool steep = false;
         for (int sg = 0; sg < nseg; ++sg) {
-            alpha[sg] = fr[sg] ? 0.0 : 1.0;
-            done[sg] = fr[sg];
-            if (!fr[sg]) rep[sg].cg_iterations += cg_it;
-        }
-        for (int ls = 0; ls <= prm.max_line_search; ++ls) {
-            PF_HIP(hipMemcpyAsync(s_a, alpha.data(), sizeof(double) * size_t(nseg), hipMemcpyHostToDevice, h.stream));
-            k_axpy_seg<<<gr, BLOCK, 0, h.stream>>>(w.xt, x, s_a, w.y, d_seg.as<int32_t>(), n);
-            PF_LAUNCH_CHECK();
-            if (prm.line_search == 0) {
-                std::fill(done.begin(), done.end(), 1);
-                break;
+            fl[sg] = !fr[sg] && !(gd[sg] < 0.0);
+            if (fl[sg]) {
+                gd[sg] = -gn[sg] * gn[sg];
+                steep = true;
             }
-            seg_energy(w.xt, e1);
-            bool all = true;
+            if (!fr[sg]) rep[sg].cg_iterations += int(it_cg[sg]);
+        }
+        if (steep) {
+            upload_flags(flag, fl);
+            k_seg_steepest<<<gr, BLOCK, 0, h.stream>>>(w.y, w.g, mask.as<uint8_t>(), flag, d_seg.as<int32_t>(), n);
+            PF_LAUNCH_CHECK();
+        }
+        std::vector<uint8_t> todo(ns);
+        for (int sg = 0; sg < nseg; ++sg) {
+            todo[sg] = !fr[sg];
+            alpha[sg] = fr[sg] ? 0.0 : 1.0;
+        }
+        search(todo, gd);
+        // Newton direction rejected (near-singular or indefinite Hessian):
+        // steepest descent from the Cauchy step g^T g / g^T H g of each problem's
+        // quadratic model, or a unit-length step when its curvature along g is not positive
+        bool cauchy = false;
+        for (int sg = 0; sg < nseg; ++sg) {
+            fl[sg] = todo[sg];
+            cauchy |= todo[sg] != 0;
+        }
+        if (cauchy) {
+            upload_flags(flag, fl);
+            k_seg_steepest<<<gr, BLOCK, 0, h.stream>>>(w.y, w.g, mask.as<uint8_t>(), flag, d_seg.as<int32_t>(), n);
+            hvp_any(h, x, w.y, w.q);
+            seg_dot(w.y, w.q, mask.as<uint8_t>(), s_gd);
+            std::vector<double> c(ns), slope(ns);
+            read(s_gd, c);
             for (int sg = 0; sg < nseg; ++sg) {
-                if (done[sg]) continue;
-                const double noise = 64.0 * 2.2e-16 * std::fabs(e0[sg]);
-                if (std::isfinite(e1[sg]) && (e1[sg] - e0[sg] <= prm.armijo_c * alpha[sg] * gd[sg] || -gd[sg] <= noise)) {
-                    done[sg] = 1;
-                } else {
-                    alpha[sg] *= 0.5;
-                    rep[sg].line_search_halvings++;
-                    all = false;
+                slope[sg] = -gn[sg] * gn[sg];
+                if (todo[sg]) {
+                    alpha[sg] = c[sg] > 0.0 ? gn[sg] * gn[sg] / c[sg] : 1.0 / gn[sg];
+                    rep[sg].gradient_steps++;
                 }
             }
-            if (all) break;
+            search(todo, slope);
+            for (int sg = 0; sg < nseg; ++sg)
+                if (todo[sg]) fail(PFGPU_ERR_INVALID_STATE, "batched line search found no acceptable step");
         }
-        for (int sg = 0; sg < nseg; ++sg)
-            if (!done[sg]) fail(PFGPU_ERR_INVALID_STATE, "batched line search found no acceptable step");
         PF_HIP(hipMemcpyAsync(x, w.xt, sizeof(double) * size_t(n), hipMemcpyDeviceToDevice, h.stream));
         for (int sg = 0; sg < nseg; ++sg)
             if (!fr[sg]) {

@@ -171,6 +171,24 @@ def smooth_displacement(nodes: np.ndarray, amplitude: float = 0.05, noise: float
     return np.ascontiguousarray(u.reshape(-1))
 
 
+def random_smooth_field(nodes: np.ndarray, amplitude: float = 0.5, seed: int = BASE_SEED, modes: int = 6,
+                        max_wavenumber: int = 2) -> np.ndarray:
+    """A seeded smooth random vector field: per component, a sum of ``modes``
+    sines sin(2 pi k.X + phase) with integer wave vectors |k_i| <= max_wavenumber
+    and N(0, 1) weights, scaled so that max |u| = amplitude.  Dof layout
+    ``node * 3 + d``."""
+    rng = np.random.default_rng(seed)
+    u = np.zeros((nodes.shape[0], 3))
+    for d in range(3):
+        for _ in range(modes):
+            k = rng.integers(-max_wavenumber, max_wavenumber + 1, size=3)
+            if not k.any():
+                k[rng.integers(3)] = 1
+            u[:, d] += rng.normal() * np.sin(2 * np.pi * nodes @ k + rng.uniform(0, 2 * np.pi))
+    u *= amplitude / max(np.abs(u).max(), 1e-300)
+    return np.ascontiguousarray(u.reshape(-1))
+
+
 def random_lame(n_elements: int, seed: int = BASE_SEED + 1, E_mean: float = 1e6, E_sigma: float = 0.5,
                 nu_range=(0.3, 0.45)):
     """Per-element E ~ logNormal(ln E_mean, E_sigma), nu ~ U(nu_range); returns (lambda, mu)."""

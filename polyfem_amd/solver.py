@@ -166,6 +166,12 @@ class TransientSimulation:
                                             C.byref(self.newton.params), _p(xT), _p(vT)))
         return xT, vT
 
+    def stats(self) -> dict:
+        """Newton and CG work of the last forward run and adjoint."""
+        st = _ffi.TransientStats()
+        check(lib().pfgpu_get_transient_stats(self.a._h, C.byref(st)))
+        return {f: int(getattr(st, f)) for f, _ in st._fields_}
+
     def adjoint(self, dJdx=None, dJdv=None, cg_max_iterations=5000, cg_tolerance=1e-12):
         """(dJ/d(lambda, mu), dJ/dx0, dJ/dv0) of J(x_T, v_T) through the last forward run."""
         n, E = self.a.mesh.n_dofs, self.a.mesh.n_elements

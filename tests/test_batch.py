@@ -67,3 +67,32 @@ def test_batch_matches_individual_solves(precision):
         tol = 1e-7 if precision == "fp64" else 1e-3
         assert np.max(np.abs(x - xi)) <= tol * np.max(np.abs(xi)), (p["material"], p["mesh"].order)
         assert np.array_equal(x[pr.fixed], pr.fixed_values)
+
+
+@pytest.mark.gpu
+def test_batch_problems_of_different_stiffness():
+    """Stiffness spread of 100x across the batch: each problem's CG runs to its
+    own relative tolerance, so every problem takes the Newton iterations it
+    takes alone and reaches the same minimiser."""
+    from polyfem_amd import BatchSolver, NewtonSolver, NLAssembler, NLProblem
+    probs = make_problems(seed=BASE_SEED + 50)
+    for i, p in enumerate(probs):
+        s = 10.0 ** (2.0 * i / (len(probs) - 1))
+        p["lame"] = (p["lame"][0] * s, p["lame"][1] * s)
+    g0 = []
+    for p in probs:
+        a = NLAssembler(p["material"], p["mesh"], lame=p["lame"], precision="fp64")
+        pr = NLProblem(p["mesh"].n_dofs, p["fixed_dofs"], p["fixed_values"])
+        x0 = p["x0"].copy()
+        x0[pr.fixed] = pr.fixed_values
+        g0.append(np.linalg.norm(a.assemble_gradient(x0)[pr.free]))
+    newton = NewtonSolver(max_iterations=30, grad_norm=1e-8 * min(g0), cg_max_iterations=5000, cg_tolerance=1e-10)
+    print(f"batch of {len(probs)} problems, stiffness spread 100x", flush=True)
+    xs, reps = BatchSolver(probs, precision="fp64", newton=newton).solve()
+    for p, x, rep in zip(probs, xs, reps):
+        a = NLAssembler(p["material"], p["mesh"], lame=p["lame"], precision="fp64")
+        pr = NLProblem(p["mesh"].n_dofs, p["fixed_dofs"], p["fixed_values"])
+        xi, ri = newton.minimize(a, p["x0"], pr)
+        assert rep.converged and ri.converged, (rep, ri)
+        assert abs(rep.iterations - ri.iterations) <= 1, (rep, ri)
+        assert np.max(np.abs(x - xi)) <= 1e-7 * np.max(np.abs(xi))

@@ -24,7 +24,7 @@ import numpy as np
 import pytest
 
 from oracle import pyoracle as po
-from polyfem_amd.mesh import BASE_SEED, kuhn_box, lame_from_young, smooth_displacement
+from polyfem_amd.mesh import BASE_SEED, face_nodes, kuhn_box, lame_from_young, smooth_displacement
 
 LAM, MU = lame_from_young(2e4, 0.3)  # test_form_derivatives.cpp:41-165
 
@@ -377,3 +377,58 @@ def test_baseline_matches_single_thread():
     b.hessian(x, v2)  # later call: slot map + per-thread arrays
     assert np.allclose(v1, ref, rtol=0, atol=1e-12 * np.abs(ref).max())
     assert np.allclose(v2, ref, rtol=0, atol=1e-10 * np.abs(ref).max())
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_static_shape_gradient_through_solves(order):
+    """dJ/dX = dJ/dX|explicit - shape_vjp(u, p) with H p = dJ/du, u the static
+    equilibrium (bottom clamped, top pulled) and J = 1/2 sum_S |X + u - X*|^2
+    over the vertices of a mid plane: against central differences of J
+    through oracle Newton solves (sparse direct) on moved meshes."""
+    import scipy.sparse.linalg as spla
+    m = kuhn_box(2, 2, 3, order=order)
+    bot, top = face_nodes(m, 2, 0.0), face_nodes(m, 2, 1.0)
+    fixed = np.concatenate([3 * bot[:, None] + np.arange(3), 3 * top[:, None] + np.arange(3)]).ravel()
+    vals = np.concatenate([np.zeros(3 * bot.size), np.tile([0.0, 0.0, 0.05], top.size)])
+    free = np.setdiff1d(np.arange(m.n_dofs), fixed)
+    rng = np.random.default_rng(BASE_SEED + 14)
+    lam = LAM * np.exp(rng.normal(0, 0.3, m.n_elements))
+    mu = MU * np.exp(rng.normal(0, 0.3, m.n_elements))
+    v2n = np.empty(m.vertices.shape[0], dtype=np.int64)
+    v2n[m.elem_vertices.ravel()] = m.elem_nodes[:, :4].ravel()
+    S = np.flatnonzero(np.abs(m.vertices[:, 2] - 1.0 / 3.0) < 1e-9)
+    Sd = (3 * v2n[S][:, None] + np.arange(3)).ravel()
+    Xstar = m.vertices[S] + rng.normal(size=(S.size, 3)) * 1e-3
+
+    def solve(V, u0):
+        m2 = kuhn_box(2, 2, 3, order=order)
+        m2.vertices = V
+        o = po.Oracle(m2, lam, mu)
+        u = u0.copy()
+        u[fixed] = vals
+        for _ in range(30):
+            g = o.gradient(u)
+            if np.linalg.norm(g[free]) < 1e-10:
+                break
+            H = o.hessian_scipy(u).tocsr()[free][:, free].tocsc()
+            u[free] -= spla.spsolve(H, g[free])
+        return u, o
+
+    def J(V, u):
+        return 0.5 * np.sum((V[S] + u[Sd].reshape(-1, 3) - Xstar) ** 2)
+
+    V0 = m.vertices.copy()
+    u, o = solve(V0, np.zeros(m.n_dofs))
+    r = V0[S] + u[Sd].reshape(-1, 3) - Xstar
+    dJdu = np.zeros(m.n_dofs)
+    dJdu[Sd] = r.ravel()
+    p = np.zeros(m.n_dofs)
+    p[free] = spla.spsolve(o.hessian_scipy(u).tocsr()[free][:, free].tocsc(), dJdu[free])
+    explicit = np.zeros_like(V0)
+    explicit[S] = r
+    grad = explicit.ravel() - o.shape_vjp(u, p)
+    h = 1e-5
+    for _ in range(2):
+        dV = rng.normal(size=V0.shape) * 0.02
+        fd = (J(V0 + h * dV, solve(V0 + h * dV, u)[0]) - J(V0 - h * dV, solve(V0 - h * dV, u)[0])) / (2 * h)
+        assert abs(grad @ dV.ravel() - fd) <= 1e-6 * abs(fd)
