@@ -239,6 +239,17 @@ pfgpu_status pfgpu_cg_solve(pfgpu_handle *h, const double *x, const uint8_t *fix
 pfgpu_status pfgpu_cg_solve_dev(pfgpu_handle *h, const double *x, const uint8_t *fixed, const double *b, double *y,
                                 int32_t max_iterations, double tolerance, int32_t use_graph, int32_t *iterations,
                                 double *rel_residual);
+/* Linear algebra of the CG solves (the reference picks polysolve's linear
+ * solver and preconditioner from the "solver.linear" options passed where the
+ * nonlinear solver is built, StateSolveNonlinear.cpp:47-50; here the CG is
+ * the solver).  hessian: the objective Hessian as values assembled once per
+ * Newton iteration and applied by a block-sparse product, or the matrix-free
+ * element product; AUTO assembles for P1 and stays matrix-free for P2.
+ * preconditioner: none, or the inverse 3x3 diagonal blocks (block Jacobi). */
+enum { PFGPU_SOLVER_HESSIAN_AUTO = 0, PFGPU_SOLVER_HESSIAN_ASSEMBLED = 1, PFGPU_SOLVER_HESSIAN_MATRIX_FREE = 2 };
+enum { PFGPU_PRECOND_NONE = 0, PFGPU_PRECOND_BLOCK_JACOBI = 1 };
+pfgpu_status pfgpu_set_solver_options(pfgpu_handle *h, int32_t hessian, int32_t preconditioner);
+
 /* external load f (n_dofs, host pointer; NULL removes it): the Newton solve
  * then minimises E(x) - f^T x (a body force such as gravity, assembled by
  * the caller, e.g. f = M g with pfgpu_mass_matvec) */

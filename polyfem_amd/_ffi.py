@@ -69,6 +69,9 @@ class PfgpuError(RuntimeError):
         self.status = status
 
 
+SOLVER_HESSIAN_AUTO, SOLVER_HESSIAN_ASSEMBLED, SOLVER_HESSIAN_MATRIX_FREE = 0, 1, 2
+PRECOND_NONE, PRECOND_BLOCK_JACOBI = 0, 1
+
 vp = C.c_void_p
 _SIGNATURES = {
     "pfgpu_create": [C.POINTER(MeshDesc), C.POINTER(MaterialDesc), C.c_int32, C.POINTER(vp)],
@@ -105,6 +108,7 @@ _SIGNATURES = {
     "pfgpu_set_load": [vp, vp],
     "pfgpu_set_vertices": [vp, vp],
     "pfgpu_get_transient_stats": [vp, vp],
+    "pfgpu_set_solver_options": [vp, C.c_int32, C.c_int32],
     "pfgpu_autotune": [vp, vp, C.POINTER(TuneReport)],
     "pfgpu_set_tuning": [vp, C.c_int32, C.c_int32, C.c_int32],
     "pfgpu_newton_solve_batch": [vp, vp, vp, C.c_int32, vp, vp, C.POINTER(NewtonParams), vp],

@@ -119,6 +119,17 @@ class NLAssembler:
         check(lib().pfgpu_set_tuning(self._h, _PATHS[path], gather_threads, hessian_threads))
         self.path = path
 
+    def set_solver_options(self, hessian: str = "auto", preconditioner: str = "block_jacobi"):
+        """Linear algebra of the solves' CG: hessian "auto" (assembled for P1,
+        matrix-free for P2), "assembled" or "matrix_free"; preconditioner
+        "block_jacobi" (inverse 3x3 diagonal blocks) or "none"."""
+        hm = {"auto": _ffi.SOLVER_HESSIAN_AUTO, "assembled": _ffi.SOLVER_HESSIAN_ASSEMBLED,
+              "matrix_free": _ffi.SOLVER_HESSIAN_MATRIX_FREE}
+        pm = {"none": _ffi.PRECOND_NONE, "block_jacobi": _ffi.PRECOND_BLOCK_JACOBI}
+        if hessian not in hm or preconditioner not in pm:
+            raise ValueError("unknown solver option")
+        check(lib().pfgpu_set_solver_options(self._h, hm[hessian], pm[preconditioner]))
+
     def set_stream(self, stream_handle: int):
         check(lib().pfgpu_set_stream(self._h, C.c_void_p(stream_handle)))
 

@@ -784,10 +784,24 @@ pfgpu_status pfgpu_cg_solve_dev(pfgpu_handle *h, const double *x, const uint8_t 
         require(max_iterations >= 0 && tolerance >= 0, "invalid CG limits");
         SolverStream ss(*h);
         double rel = 0;
+        h->op_valid = false; // the operator is built at this x
         const int it = cg_solve_ws(*h, x, fixed, b, max_iterations, tolerance, use_graph != 0, &rel);
         PF_HIP(hipMemcpyAsync(y, cg_solution(*h), sizeof(double) * size_t(h->n_dofs), hipMemcpyDeviceToDevice, h->stream));
         if (iterations) *iterations = it;
         if (rel_residual) *rel_residual = rel;
+    });
+}
+
+pfgpu_status pfgpu_set_solver_options(pfgpu_handle *h, int32_t hessian, int32_t preconditioner)
+{
+    return guard([&] {
+        check_handle(h);
+        require(hessian >= PFGPU_SOLVER_HESSIAN_AUTO && hessian <= PFGPU_SOLVER_HESSIAN_MATRIX_FREE, "unknown hessian mode");
+        require(preconditioner == PFGPU_PRECOND_NONE || preconditioner == PFGPU_PRECOND_BLOCK_JACOBI,
+                "unknown preconditioner");
+        h->solver_hessian = hessian;
+        h->solver_precond = preconditioner;
+        solver_release(*h);
     });
 }
 
@@ -924,6 +938,7 @@ pfgpu_status pfgpu_cg_solve(pfgpu_handle *h, const double *x, const uint8_t *fix
         const double *db = stage(*h, h->h_v, b, h->n_dofs);
         const uint8_t *dm = stage_mask(*h, fixed);
         double rel = 0;
+        h->op_valid = false; // the operator is built at this x
         const int it = cg_solve_ws(*h, dx, dm, db, max_iterations, tolerance, true, &rel);
         fetch(*h, y, cg_solution(*h), sizeof(double) * size_t(h->n_dofs));
         if (iterations) *iterations = it;

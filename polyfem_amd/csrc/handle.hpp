@@ -65,7 +65,11 @@ struct pfgpu_handle {
     // solver workspace: vectors, scalars, the captured CG iteration
     pfgpu::DevBuf sv, h_mask;
     void *cg_graph = nullptr; // hipGraphExec_t
-    std::array<const void *, 5> cg_graph_key{};
+    std::array<const void *, 8> cg_graph_key{};
+    // CG operator: assembled block-sparse values or matrix-free, block-Jacobi preconditioner
+    int solver_hessian = PFGPU_SOLVER_HESSIAN_AUTO, solver_precond = PFGPU_PRECOND_BLOCK_JACOBI;
+    bool op_valid = false, op_assembled = false, op_precond = false;
+    pfgpu::DevBuf solver_vals, pinv, diag_pos;
     hipStream_t own_stream = nullptr; // used when the caller's stream is the legacy default
     ~pfgpu_handle();
 

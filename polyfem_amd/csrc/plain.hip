@@ -642,7 +642,7 @@ void ensure_plain_hessian_maps(Handle &h)
     h.h_contrib_ptr = std::move(ptr);
     h.blk_chunk_threads = 0;
     h.block_row = upload(block_row, h.stream);
-    h.brow_ptr = upload(h.pat.brow_ptr, h.stream);
+    if (!h.brow_ptr.get()) h.brow_ptr = upload(h.pat.brow_ptr, h.stream);
     h.plain_hess_ready = true;
 }
 

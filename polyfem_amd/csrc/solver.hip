@@ -33,11 +33,26 @@ namespace pfgpu {
 
 namespace {
 
-constexpr int DOT_BLOCKS = 256; // fixed grid of the dot-product kernels
-constexpr int CHECK = 8;        // CG iterations between host convergence checks
+constexpr int DOT_BLOCKS = 1024; // fixed grid of the dot-product kernels
+constexpr int CHECK = 8;         // CG iterations between host convergence checks
+constexpr int SPMV_LANES = 16;   // lanes per block row of the block-sparse product
 
-// scalar slots of the CG workspace
-enum { S_RR, S_PQ, S_ALPHA, S_BETA, S_RR_NEW, S_FLAG, S_B2, S_GD, S_E0, S_E1, S_M, S_F, S_COUNT };
+// scalar slots of the CG workspace (r.z: preconditioned residual product)
+enum { S_RR, S_PQ, S_ALPHA, S_BETA, S_RZ, S_FLAG, S_B2, S_GD, S_E0, S_E1, S_M, S_F, S_COUNT };
+
+__device__ double block_reduce(double v, double *sh)
+{
+    const int t = threadIdx.x;
+    sh[t] = v;
+    __syncthreads();
+    for (int w = BLOCK / 2; w > 0; w >>= 1) {
+        if (t < w) sh[t] += sh[t + w];
+        __syncthreads();
+    }
+    const double r = sh[0];
+    __syncthreads();
+    return r;
+}
 
 // per-block partial of sum_i a_i b_i over free dofs (mask: 1 = fixed)
 __global__ __launch_bounds__(BLOCK) void k_dot_partials(const double *__restrict__ a, const double *__restrict__ b,
@@ -58,31 +73,221 @@ __global__ __launch_bounds__(BLOCK) void k_dot_partials(const double *__restrict
 
 // one block: fixed-order sum of the DOT_BLOCKS partials into scal[slot], then
 // the scalar bookkeeping of the step that follows
-enum { AFTER_NONE, AFTER_ALPHA, AFTER_BETA };
+enum { AFTER_NONE, AFTER_ALPHA };
+__device__ double sum_partials(const double *__restrict__ part, double *sh)
+{
+    double a = 0;
+    for (int k = threadIdx.x; k < DOT_BLOCKS; k += BLOCK) a += part[k];
+    return block_reduce(a, sh);
+}
+
 __global__ __launch_bounds__(BLOCK) void k_finish_dot(const double *__restrict__ part, double *__restrict__ scal, int slot,
                                                       int after)
 {
     __shared__ double s[BLOCK];
-    const int t = threadIdx.x;
-    s[t] = t < DOT_BLOCKS ? part[t] : 0.0;
-    __syncthreads();
-    for (int w = BLOCK / 2; w > 0; w >>= 1) {
-        if (t < w) s[t] += s[t + w];
-        __syncthreads();
-    }
-    if (t != 0) return;
-    const double v = s[0];
+    const double v = sum_partials(part, s);
+    if (threadIdx.x != 0) return;
     scal[slot] = v;
     if (after == AFTER_ALPHA) {
-        // alpha = r.r / p.Hp; a non-positive curvature stops the iteration
-        const double rr = scal[S_RR];
-        const bool ok = v > 0.0 && rr > 0.0 && scal[S_FLAG] == 0.0;
-        if (!(v > 0.0) && rr > 0.0) scal[S_FLAG] = 1.0;
-        scal[S_ALPHA] = ok ? rr / v : 0.0;
-    } else if (after == AFTER_BETA) {
-        const double rr = scal[S_RR];
-        scal[S_BETA] = rr > 0.0 && scal[S_FLAG] == 0.0 ? v / rr : 0.0;
-        scal[S_RR] = scal[S_FLAG] == 0.0 ? v : rr;
+        // alpha = r.z / p.Hp; a non-positive curvature stops the iteration
+        const double rz = scal[S_RZ];
+        const bool ok = v > 0.0 && rz > 0.0 && scal[S_FLAG] == 0.0;
+        if (!(v > 0.0) && rz > 0.0) scal[S_FLAG] = 1.0;
+        scal[S_ALPHA] = ok ? rz / v : 0.0;
+    }
+}
+
+// z = P r for one node: P the inverse of the node's 3x3 diagonal block (symmetric,
+// stored as its upper triangle P00 P01 P02 P11 P12 P22 in FP32), or the identity
+constexpr int PINV = 6;
+__device__ inline void precondition(const float *__restrict__ pinv, int64_t i, const double r[3], double z[3])
+{
+    if (!pinv) {
+        z[0] = r[0], z[1] = r[1], z[2] = r[2];
+        return;
+    }
+    const float *P = pinv + PINV * i;
+    const double a = P[0], b = P[1], c = P[2], d = P[3], e = P[4], f = P[5];
+    z[0] = fma(a, r[0], fma(b, r[1], c * r[2]));
+    z[1] = fma(b, r[0], fma(d, r[1], e * r[2]));
+    z[2] = fma(c, r[0], fma(e, r[1], f * r[2]));
+}
+
+// PCG start: y = 0, r = b on free dofs, z = P r, p = z; partials of r.z and r.r
+__global__ __launch_bounds__(BLOCK) void k_pcg_start(int n_nodes, const double *__restrict__ b,
+                                                     const uint8_t *__restrict__ fixed, const float *__restrict__ pinv,
+                                                     double *__restrict__ y, double *__restrict__ r,
+                                                     double *__restrict__ z, double *__restrict__ p,
+                                                     double *__restrict__ part_rz, double *__restrict__ part_rr)
+{
+    __shared__ double sh[BLOCK];
+    double rz = 0, rr = 0;
+    for (int i = blockIdx.x * BLOCK + threadIdx.x; i < n_nodes; i += DOT_BLOCKS * BLOCK) {
+        const int64_t o = 3 * int64_t(i);
+        double ri[3], zi[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) ri[d] = fixed && fixed[o + d] ? 0.0 : b[o + d];
+        precondition(pinv, i, ri, zi);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            y[o + d] = 0.0;
+            r[o + d] = ri[d];
+            z[o + d] = zi[d];
+            p[o + d] = zi[d];
+            rz = fma(ri[d], zi[d], rz);
+            rr = fma(ri[d], ri[d], rr);
+        }
+    }
+    const double a = block_reduce(rz, sh), c = block_reduce(rr, sh);
+    if (threadIdx.x == 0) {
+        part_rz[blockIdx.x] = a;
+        part_rr[blockIdx.x] = c;
+    }
+}
+
+// y += alpha p, r -= alpha q on free dofs; z = P r; partials of r.z and r.r
+__global__ __launch_bounds__(BLOCK) void k_pcg_update(int n_nodes, double *__restrict__ y, double *__restrict__ r,
+                                                      double *__restrict__ z, const double *__restrict__ p,
+                                                      const double *__restrict__ q, const uint8_t *__restrict__ fixed,
+                                                      const float *__restrict__ pinv, const double *__restrict__ scal,
+                                                      double *__restrict__ part_rz, double *__restrict__ part_rr)
+{
+    __shared__ double sh[BLOCK];
+    const double a = scal[S_ALPHA];
+    double rz = 0, rr = 0;
+    for (int i = blockIdx.x * BLOCK + threadIdx.x; i < n_nodes; i += DOT_BLOCKS * BLOCK) {
+        const int64_t o = 3 * int64_t(i);
+        double ri[3], zi[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            ri[d] = r[o + d];
+            if (a != 0.0 && !(fixed && fixed[o + d])) {
+                y[o + d] = fma(a, p[o + d], y[o + d]);
+                ri[d] = fma(-a, q[o + d], ri[d]);
+                r[o + d] = ri[d];
+            }
+        }
+        precondition(pinv, i, ri, zi);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            z[o + d] = zi[d];
+            rz = fma(ri[d], zi[d], rz);
+            rr = fma(ri[d], ri[d], rr);
+        }
+    }
+    const double c = block_reduce(rz, sh), e = block_reduce(rr, sh);
+    if (threadIdx.x == 0) {
+        part_rz[blockIdx.x] = c;
+        part_rr[blockIdx.x] = e;
+    }
+}
+
+// sums of the r.z and r.r partials; at the start they set r.z and r.r, after an
+// update they also give beta = r'.z' / r.z (zero once the iteration has stopped)
+__global__ __launch_bounds__(BLOCK) void k_finish_pcg(const double *__restrict__ part_rz, const double *__restrict__ part_rr,
+                                                      double *__restrict__ scal, int start)
+{
+    __shared__ double s[BLOCK];
+    const double rz = sum_partials(part_rz, s), rr = sum_partials(part_rr, s);
+    if (threadIdx.x != 0) return;
+    if (start) {
+        scal[S_RZ] = rz;
+        scal[S_RR] = rr;
+        scal[S_FLAG] = 0.0;
+        return;
+    }
+    const double old = scal[S_RZ];
+    const bool live = scal[S_FLAG] == 0.0 && old > 0.0;
+    scal[S_BETA] = live ? rz / old : 0.0;
+    if (scal[S_FLAG] == 0.0) {
+        scal[S_RZ] = rz;
+        scal[S_RR] = rr;
+    }
+}
+
+// y = w H v + M v over the block-sparse pattern: H in the assembled value
+// layout (block (i, pos) entry (m, n) at 9 brow_ptr[i] + 3 m deg + 3 pos + n),
+// M one value per block (block = m I_3) or null.  SPMV_LANES lanes per block
+// row; each row's sums are reduced by a fixed butterfly, so y is reproducible.
+template <typename T>
+__global__ __launch_bounds__(BLOCK) void k_bsr_spmv(int n_nodes, const int32_t *__restrict__ brow_ptr,
+                                                    const int32_t *__restrict__ bcol, const T *__restrict__ vals,
+                                                    const double *__restrict__ mass, double w,
+                                                    const double *__restrict__ v, double *__restrict__ y)
+{
+    const int lane = threadIdx.x % SPMV_LANES;
+    const int i = blockIdx.x * (BLOCK / SPMV_LANES) + threadIdx.x / SPMV_LANES;
+    double h0 = 0, h1 = 0, h2 = 0, m0 = 0, m1 = 0, m2 = 0;
+    if (i < n_nodes) {
+        const int b0 = brow_ptr[i], deg = brow_ptr[i + 1] - b0;
+        const T *row = vals + 9 * int64_t(b0);
+        for (int pos = lane; pos < deg; pos += SPMV_LANES) {
+            const int64_t c = 3 * int64_t(bcol[b0 + pos]);
+            const double v0 = v[c], v1 = v[c + 1], v2 = v[c + 2];
+            const T *r0 = row + 3 * pos, *r1 = r0 + 3 * deg, *r2 = r1 + 3 * deg;
+            h0 = fma(double(r0[0]), v0, fma(double(r0[1]), v1, fma(double(r0[2]), v2, h0)));
+            h1 = fma(double(r1[0]), v0, fma(double(r1[1]), v1, fma(double(r1[2]), v2, h1)));
+            h2 = fma(double(r2[0]), v0, fma(double(r2[1]), v1, fma(double(r2[2]), v2, h2)));
+            if (mass) {
+                const double mm = mass[b0 + pos];
+                m0 = fma(mm, v0, m0);
+                m1 = fma(mm, v1, m1);
+                m2 = fma(mm, v2, m2);
+            }
+        }
+    }
+#pragma unroll
+    for (int o = SPMV_LANES / 2; o > 0; o >>= 1) {
+        h0 += __shfl_xor(h0, o, SPMV_LANES);
+        h1 += __shfl_xor(h1, o, SPMV_LANES);
+        h2 += __shfl_xor(h2, o, SPMV_LANES);
+        m0 += __shfl_xor(m0, o, SPMV_LANES);
+        m1 += __shfl_xor(m1, o, SPMV_LANES);
+        m2 += __shfl_xor(m2, o, SPMV_LANES);
+    }
+    if (i < n_nodes && lane < 3) {
+        const double hv = lane == 0 ? h0 : lane == 1 ? h1 : h2, mv = lane == 0 ? m0 : lane == 1 ? m1 : m2;
+        y[3 * int64_t(i) + lane] = fma(w, hv, mv);
+    }
+}
+
+// inverse of every node's 3x3 diagonal block of w H + M, with the rows and
+// columns of fixed dofs replaced by the identity; a block that is not
+// positive definite (an indefinite tangent) falls back to 1 / |diagonal|
+template <typename T>
+__global__ __launch_bounds__(BLOCK) void k_block_jacobi(int n_nodes, const int32_t *__restrict__ brow_ptr,
+                                                        const int32_t *__restrict__ diag_pos, const T *__restrict__ vals,
+                                                        const double *__restrict__ mass, double w,
+                                                        const uint8_t *__restrict__ fixed, float *__restrict__ pinv)
+{
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n_nodes) return;
+    const int b0 = brow_ptr[i], deg = brow_ptr[i + 1] - b0, dp = diag_pos[i];
+    const T *row = vals + 9 * int64_t(b0) + 3 * dp;
+    double B[3][3];
+#pragma unroll
+    for (int m = 0; m < 3; ++m)
+#pragma unroll
+        for (int n = 0; n < 3; ++n) B[m][n] = w * double(row[m * 3 * deg + n]) + (mass && m == n ? mass[b0 + dp] : 0.0);
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+        if (fixed && fixed[3 * int64_t(i) + d])
+#pragma unroll
+            for (int k = 0; k < 3; ++k) B[d][k] = B[k][d] = d == k ? 1.0 : 0.0;
+    // symmetric part (the exact block is symmetric; rounding is not)
+    const double a = B[0][0], b = 0.5 * (B[0][1] + B[1][0]), c = 0.5 * (B[0][2] + B[2][0]);
+    const double d = B[1][1], e = 0.5 * (B[1][2] + B[2][1]), f = B[2][2];
+    const double c00 = d * f - e * e, c01 = c * e - b * f, c02 = b * e - c * d;
+    const double det = a * c00 + b * c01 + c * c02;
+    float *P = pinv + PINV * int64_t(i);
+    if (a > 0.0 && a * d - b * b > 0.0 && det > 0.0 && isfinite(det)) {
+        const double id = 1.0 / det;
+        const double c11 = a * f - c * c, c12 = b * c - a * e, c22 = a * d - b * b;
+        P[0] = float(c00 * id), P[1] = float(c01 * id), P[2] = float(c02 * id);
+        P[3] = float(c11 * id), P[4] = float(c12 * id), P[5] = float(c22 * id);
+    } else {
+        auto inv = [](double v) { return std::fabs(v) > 0.0 && isfinite(v) ? 1.0 / std::fabs(v) : 1.0; };
+        P[0] = float(inv(a)), P[1] = 0.0f, P[2] = 0.0f, P[3] = float(inv(d)), P[4] = 0.0f, P[5] = float(inv(f));
     }
 }
 
@@ -93,36 +298,13 @@ __global__ __launch_bounds__(BLOCK) void k_mask(double *__restrict__ q, const ui
     if (i < n && fixed && fixed[i]) q[i] = 0.0;
 }
 
-// y += alpha p, r -= alpha q
-__global__ __launch_bounds__(BLOCK) void k_cg_update(double *__restrict__ y, double *__restrict__ r, const double *__restrict__ p,
-                                                     const double *__restrict__ q, const double *__restrict__ scal, int n)
-{
-    const int i = blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= n) return;
-    const double a = scal[S_ALPHA];
-    y[i] = fma(a, p[i], y[i]);
-    r[i] = fma(-a, q[i], r[i]);
-}
-
-// p = r + beta p
-__global__ __launch_bounds__(BLOCK) void k_cg_direction(double *__restrict__ p, const double *__restrict__ r,
+// p = z + beta p
+__global__ __launch_bounds__(BLOCK) void k_cg_direction(double *__restrict__ p, const double *__restrict__ z,
                                                         const double *__restrict__ scal, int n)
 {
     const int i = blockIdx.x * BLOCK + threadIdx.x;
     if (i >= n) return;
-    p[i] = fma(scal[S_BETA], p[i], r[i]);
-}
-
-// CG start: y = 0, r = b (free dofs), p = r
-__global__ __launch_bounds__(BLOCK) void k_cg_start(double *__restrict__ y, double *__restrict__ r, double *__restrict__ p,
-                                                    const double *__restrict__ b, const uint8_t *__restrict__ fixed, int n)
-{
-    const int i = blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= n) return;
-    const double v = fixed && fixed[i] ? 0.0 : b[i];
-    y[i] = 0.0;
-    r[i] = v;
-    p[i] = v;
+    p[i] = fma(scal[S_BETA], p[i], z[i]);
 }
 
 __global__ void k_set_scalar(double *__restrict__ scal, int slot, double v) { scal[slot] = v; }
@@ -141,26 +323,57 @@ __global__ __launch_bounds__(BLOCK) void k_neg_free(double *__restrict__ out, co
     if (i < n) out[i] = fixed && fixed[i] ? 0.0 : -v[i];
 }
 
-// y = a y + M v (M: one value per 3x3 block of the node pattern, block = m I_3),
-// one thread per node, columns in pattern order
+// y = a y + M v (M: one value per 3x3 block of the node pattern, block = m I_3);
+// SPMV_LANES lanes per node row, reduced by a fixed butterfly
 __global__ __launch_bounds__(BLOCK) void k_mass_apply(int n_nodes, const int32_t *__restrict__ brow_ptr,
                                                       const int32_t *__restrict__ bcol, const double *__restrict__ m,
                                                       const double *__restrict__ v, double a, double *__restrict__ y)
 {
-    const int i = blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= n_nodes) return;
+    const int lane = threadIdx.x % SPMV_LANES;
+    const int i = blockIdx.x * (BLOCK / SPMV_LANES) + threadIdx.x / SPMV_LANES;
     double s0 = 0, s1 = 0, s2 = 0;
-    for (int k = brow_ptr[i]; k < brow_ptr[i + 1]; ++k) {
-        const double w = m[k];
-        const int64_t j = 3 * int64_t(bcol[k]);
-        s0 = fma(w, v[j], s0);
-        s1 = fma(w, v[j + 1], s1);
-        s2 = fma(w, v[j + 2], s2);
+    if (i < n_nodes)
+        for (int k = brow_ptr[i] + lane; k < brow_ptr[i + 1]; k += SPMV_LANES) {
+            const double w = m[k];
+            const int64_t j = 3 * int64_t(bcol[k]);
+            s0 = fma(w, v[j], s0);
+            s1 = fma(w, v[j + 1], s1);
+            s2 = fma(w, v[j + 2], s2);
+        }
+#pragma unroll
+    for (int o = SPMV_LANES / 2; o > 0; o >>= 1) {
+        s0 += __shfl_xor(s0, o, SPMV_LANES);
+        s1 += __shfl_xor(s1, o, SPMV_LANES);
+        s2 += __shfl_xor(s2, o, SPMV_LANES);
     }
-    const int64_t o = 3 * int64_t(i);
-    y[o] = fma(a, y[o], s0);
-    y[o + 1] = fma(a, y[o + 1], s1);
-    y[o + 2] = fma(a, y[o + 2], s2);
+    if (i < n_nodes && lane < 3) {
+        const int64_t o = 3 * int64_t(i) + lane;
+        const double sv = lane == 0 ? s0 : lane == 1 ? s1 : s2;
+        y[o] = a == 0.0 ? sv : fma(a, y[o], sv);
+    }
+}
+
+// vals = w vals + M over the block-sparse layout (the objective's Hessian w H + M
+// in place of H); SPMV_LANES lanes per node row
+template <typename T>
+__global__ __launch_bounds__(BLOCK) void k_fold_operator(int n_nodes, const int32_t *__restrict__ brow_ptr,
+                                                         const double *__restrict__ mass, double w, T *__restrict__ vals)
+{
+    const int lane = threadIdx.x % SPMV_LANES;
+    const int i = blockIdx.x * (BLOCK / SPMV_LANES) + threadIdx.x / SPMV_LANES;
+    if (i >= n_nodes) return;
+    const int b0 = brow_ptr[i], deg = brow_ptr[i + 1] - b0;
+    T *row = vals + 9 * int64_t(b0);
+    for (int pos = lane; pos < deg; pos += SPMV_LANES) {
+        const double mm = mass ? mass[b0 + pos] : 0.0;
+#pragma unroll
+        for (int m = 0; m < 3; ++m)
+#pragma unroll
+            for (int n = 0; n < 3; ++n) {
+                T &e = row[m * 3 * deg + 3 * pos + n];
+                e = T(fma(w, double(e), m == n ? mm : 0.0));
+            }
+    }
 }
 
 // out = x - xt
@@ -211,16 +424,16 @@ __global__ __launch_bounds__(BLOCK) void k_axpy(double *__restrict__ out, double
 }
 
 struct Work {
-    double *r, *p, *q, *y, *g, *xt, *d, *t, *mt, *scal, *part;
+    double *r, *p, *q, *y, *g, *xt, *d, *t, *mt, *z, *scal, *part, *part2;
 };
 
 Work workspace(Handle &h)
 {
     const size_t n = size_t(h.n_dofs);
-    h.sv.reserve(sizeof(double) * (9 * n + S_COUNT + DOT_BLOCKS));
+    h.sv.reserve(sizeof(double) * (10 * n + S_COUNT + 2 * DOT_BLOCKS));
     double *b = h.sv.as<double>();
-    return Work{b, b + n, b + 2 * n, b + 3 * n, b + 4 * n, b + 5 * n, b + 6 * n, b + 7 * n, b + 8 * n,
-                b + 9 * n, b + 9 * n + S_COUNT};
+    return Work{b,         b + n,     b + 2 * n,           b + 3 * n, b + 4 * n, b + 5 * n, b + 6 * n, b + 7 * n, b + 8 * n,
+                b + 9 * n, b + 10 * n, b + 10 * n + S_COUNT, b + 10 * n + S_COUNT + DOT_BLOCKS};
 }
 
 void dot(Handle &h, const Work &w, const double *a, const double *b, const uint8_t *fixed, int slot, int after)
@@ -239,15 +452,36 @@ double read_scalar(Handle &h, const Work &w, int slot)
 
 bool inertia(const Handle &h) { return h.obj_xt != nullptr; }
 
-// y = (w H(x) + [M]) v
+bool assembled_operator(const Handle &h)
+{
+    return h.solver_hessian == PFGPU_SOLVER_HESSIAN_ASSEMBLED ||
+           (h.solver_hessian == PFGPU_SOLVER_HESSIAN_AUTO && h.order == 1);
+}
+
+// y = (w H(x) + [M]) v: the block-sparse product with the values assembled by
+// prepare_operator at x, or the matrix-free product
 void hvp_any(Handle &h, const double *x, const double *v, double *y)
 {
+    if (h.op_valid && h.op_assembled) {
+        // the values hold w H + M (folded by prepare_operator)
+        const unsigned g = grid_for(h.n_nodes, BLOCK / SPMV_LANES);
+        auto spmv = [&]<typename T>() {
+            k_bsr_spmv<T><<<g, BLOCK, 0, h.stream>>>(h.n_nodes, h.brow_ptr.as<int32_t>(), h.pat_bcol.as<int32_t>(),
+                                                     h.solver_vals.as<T>(), nullptr, 1.0, v, y);
+        };
+        if (h.precision == PFGPU_MIXED)
+            spmv.template operator()<float>();
+        else
+            spmv.template operator()<double>();
+        PF_LAUNCH_CHECK();
+        return;
+    }
     if (h.path == PFGPU_PATH_FUSED)
         fused_hvp(h, x, v, y);
     else
         plain_hvp(h, x, v, y);
     if (inertia(h))
-        k_mass_apply<<<grid_for(h.n_nodes), BLOCK, 0, h.stream>>>(h.n_nodes, h.brow_ptr.as<int32_t>(),
+        k_mass_apply<<<grid_for(h.n_nodes, BLOCK / SPMV_LANES), BLOCK, 0, h.stream>>>(h.n_nodes, h.brow_ptr.as<int32_t>(),
                                                                    h.pat_bcol.as<int32_t>(), h.mass.as<double>(), v,
                                                                    h.obj_w, y);
     else if (h.obj_w != 1.0)
@@ -255,21 +489,88 @@ void hvp_any(Handle &h, const double *x, const double *v, double *y)
     PF_LAUNCH_CHECK();
 }
 
+// The CG operator at x: with an assembled operator or the block-Jacobi
+// preconditioner, the Hessian values at x (plain or fused assembly) into the
+// solver's value buffer and, for the preconditioner, the inverse diagonal
+// blocks of w H + M with the fixed dofs as identity.  Valid until x, the
+// objective or the mask change (op_valid is cleared then).
+void prepare_operator(Handle &h, const double *x, const uint8_t *fixed)
+{
+    h.op_assembled = assembled_operator(h);
+    h.op_precond = h.solver_precond == PFGPU_PRECOND_BLOCK_JACOBI;
+    if (h.op_assembled || h.op_precond) {
+        if (!h.brow_ptr.get()) h.brow_ptr = upload(h.pat.brow_ptr, h.stream);
+        if (!h.pat_bcol.get()) h.pat_bcol = upload(h.pat.bcol, h.stream);
+        h.solver_vals.reserve(h.value_size() * size_t(h.pat.nnz));
+        if (h.path == PFGPU_PATH_FUSED)
+            fused_energy_gradient_hessian(h, x, nullptr, nullptr, h.solver_vals.get());
+        else
+            plain_hessian(h, x, h.solver_vals.get());
+    }
+    const double *mass = inertia(h) ? h.mass.as<double>() : nullptr;
+    const bool fold = h.op_assembled && (mass || h.obj_w != 1.0);
+    if (fold) {
+        auto f = [&]<typename T>() {
+            k_fold_operator<T><<<grid_for(h.n_nodes, BLOCK / SPMV_LANES), BLOCK, 0, h.stream>>>(
+                h.n_nodes, h.brow_ptr.as<int32_t>(), mass, h.obj_w, h.solver_vals.as<T>());
+        };
+        if (h.precision == PFGPU_MIXED)
+            f.template operator()<float>();
+        else
+            f.template operator()<double>();
+        PF_LAUNCH_CHECK();
+    }
+    if (h.op_precond) {
+        if (!h.diag_pos.get()) {
+            std::vector<int32_t> dp(size_t(h.n_nodes));
+            for (int i = 0; i < h.n_nodes; ++i) {
+                const int32_t *c0 = h.pat.bcol.data() + h.pat.brow_ptr[i], *c1 = h.pat.bcol.data() + h.pat.brow_ptr[i + 1];
+                const int32_t *f = std::lower_bound(c0, c1, i);
+                if (f == c1 || *f != i) fail(PFGPU_ERR_INVALID_STATE, "pattern without a diagonal block");
+                dp[size_t(i)] = int32_t(f - c0);
+            }
+            h.diag_pos = upload(dp, h.stream);
+        }
+        h.pinv.reserve(sizeof(float) * PINV * size_t(h.n_nodes));
+        auto jac = [&]<typename T>() {
+            k_block_jacobi<T><<<grid_for(h.n_nodes), BLOCK, 0, h.stream>>>(
+                h.n_nodes, h.brow_ptr.as<int32_t>(), h.diag_pos.as<int32_t>(), h.solver_vals.as<T>(),
+                fold ? nullptr : mass, fold ? 1.0 : h.obj_w, fixed, h.pinv.as<float>());
+        };
+        if (h.precision == PFGPU_MIXED)
+            jac.template operator()<float>();
+        else
+            jac.template operator()<double>();
+        PF_LAUNCH_CHECK();
+    }
+    h.op_valid = true;
+}
+
 // objective gradient into g and its value into scal[slot] (device)
 void objective_gradient(Handle &h, const Work &w, const double *x, double *g, int slot);
 // objective value into scal[slot] (device)
 void objective_energy(Handle &h, const Work &w, const double *x, int slot);
 
-// one CG iteration (a fixed kernel sequence)
+const float *precond(const Handle &h) { return h.op_valid && h.op_precond ? h.pinv.as<float>() : nullptr; }
+
+// PCG start from b (a fixed kernel sequence)
+void cg_start(Handle &h, const Work &w, const uint8_t *fixed, const double *b)
+{
+    k_pcg_start<<<DOT_BLOCKS, BLOCK, 0, h.stream>>>(h.n_nodes, b, fixed, precond(h), w.y, w.r, w.z, w.p, w.part,
+                                                    w.part2);
+    k_finish_pcg<<<1, BLOCK, 0, h.stream>>>(w.part, w.part2, w.scal, 1);
+    PF_LAUNCH_CHECK();
+}
+
+// one PCG iteration (a fixed kernel sequence)
 void cg_iteration(Handle &h, const Work &w, const double *x, const uint8_t *fixed)
 {
-    const unsigned g = grid_for(h.n_dofs);
     hvp_any(h, x, w.p, w.q);
-    k_mask<<<g, BLOCK, 0, h.stream>>>(w.q, fixed, h.n_dofs);
     dot(h, w, w.p, w.q, fixed, S_PQ, AFTER_ALPHA);
-    k_cg_update<<<g, BLOCK, 0, h.stream>>>(w.y, w.r, w.p, w.q, w.scal, h.n_dofs);
-    dot(h, w, w.r, w.r, fixed, S_RR_NEW, AFTER_BETA);
-    k_cg_direction<<<g, BLOCK, 0, h.stream>>>(w.p, w.r, w.scal, h.n_dofs);
+    k_pcg_update<<<DOT_BLOCKS, BLOCK, 0, h.stream>>>(h.n_nodes, w.y, w.r, w.z, w.p, w.q, fixed, precond(h), w.scal,
+                                                     w.part, w.part2);
+    k_finish_pcg<<<1, BLOCK, 0, h.stream>>>(w.part, w.part2, w.scal, 0);
+    k_cg_direction<<<grid_for(h.n_dofs), BLOCK, 0, h.stream>>>(w.p, w.z, w.scal, h.n_dofs);
     PF_LAUNCH_CHECK();
 }
 
@@ -280,7 +581,7 @@ void objective_energy(Handle &h, const Work &w, const double *x, int slot)
     if (inertia(h)) {
         const unsigned g = grid_for(h.n_dofs);
         k_sub<<<g, BLOCK, 0, h.stream>>>(w.t, x, h.obj_xt, h.n_dofs);
-        k_mass_apply<<<grid_for(h.n_nodes), BLOCK, 0, h.stream>>>(h.n_nodes, h.brow_ptr.as<int32_t>(),
+        k_mass_apply<<<grid_for(h.n_nodes, BLOCK / SPMV_LANES), BLOCK, 0, h.stream>>>(h.n_nodes, h.brow_ptr.as<int32_t>(),
                                                                    h.pat_bcol.as<int32_t>(), h.mass.as<double>(), w.t,
                                                                    0.0, w.mt);
         dot(h, w, w.t, w.mt, nullptr, S_M, AFTER_NONE);
@@ -299,7 +600,7 @@ void objective_gradient(Handle &h, const Work &w, const double *x, double *g, in
     if (inertia(h)) {
         const unsigned gr = grid_for(h.n_dofs);
         k_sub<<<gr, BLOCK, 0, h.stream>>>(w.t, x, h.obj_xt, h.n_dofs);
-        k_mass_apply<<<grid_for(h.n_nodes), BLOCK, 0, h.stream>>>(h.n_nodes, h.brow_ptr.as<int32_t>(),
+        k_mass_apply<<<grid_for(h.n_nodes, BLOCK / SPMV_LANES), BLOCK, 0, h.stream>>>(h.n_nodes, h.brow_ptr.as<int32_t>(),
                                                                    h.pat_bcol.as<int32_t>(), h.mass.as<double>(), w.t,
                                                                    0.0, w.mt);
         dot(h, w, w.t, w.mt, nullptr, S_M, AFTER_NONE);
@@ -335,16 +636,24 @@ double objective_value(Handle &h, const Work &w, int slot)
 void mass_apply(Handle &h, const double *v, double *y)
 {
     if (!h.has_mass) fail(PFGPU_ERR_INVALID_STATE, "no mass matrix: call pfgpu_set_density first");
-    k_mass_apply<<<grid_for(h.n_nodes), BLOCK, 0, h.stream>>>(h.n_nodes, h.brow_ptr.as<int32_t>(), h.pat_bcol.as<int32_t>(),
+    k_mass_apply<<<grid_for(h.n_nodes, BLOCK / SPMV_LANES), BLOCK, 0, h.stream>>>(h.n_nodes, h.brow_ptr.as<int32_t>(), h.pat_bcol.as<int32_t>(),
                                                                h.mass.as<double>(), v, 0.0, y);
     PF_LAUNCH_CHECK();
 }
 
-void solver_release(Handle &h)
+namespace {
+void release_graph(Handle &h)
 {
     if (h.cg_graph) (void)hipGraphExecDestroy(static_cast<hipGraphExec_t>(h.cg_graph));
     h.cg_graph = nullptr;
     h.cg_graph_key = {};
+}
+} // namespace
+
+void solver_release(Handle &h)
+{
+    h.op_valid = false;
+    release_graph(h);
 }
 
 // CG on H(x)_ff y = b_f starting from y = 0; the solution is left in the
@@ -353,29 +662,31 @@ int cg_solve_ws(Handle &h, const double *x, const uint8_t *fixed, const double *
                 bool use_graph, double *rel)
 {
     const Work w = workspace(h);
-    const unsigned g = grid_for(h.n_dofs);
-    k_cg_start<<<g, BLOCK, 0, h.stream>>>(w.y, w.r, w.p, b, fixed, h.n_dofs);
-    k_set_scalar<<<1, 1, 0, h.stream>>>(w.scal, S_FLAG, 0.0);
-    dot(h, w, w.r, w.r, fixed, S_RR, AFTER_NONE);
-    PF_LAUNCH_CHECK();
+    if (!h.op_valid) prepare_operator(h, x, fixed);
+    cg_start(h, w, fixed, b);
     const double b2 = read_scalar(h, w, S_RR);
     if (rel) *rel = 0.0;
     if (b2 == 0.0 || max_it <= 0) return 0;
     if (!std::isfinite(b2)) fail(PFGPU_ERR_INVALID_STATE, "non-finite right-hand side in the CG solve");
     // the graph replays one iteration; it is rebuilt when its arguments change
-    // (the element scratch is grow-only: a reallocation by another call rebuilds the graph)
-    const std::array<const void *, 5> key{x, fixed, w.r, reinterpret_cast<const void *>(intptr_t(h.obj_generation)),
-                                          h.loc.get()};
+    // (scratch buffers are grow-only: a reallocation by another call rebuilds the graph)
+    const std::array<const void *, 8> key{x,
+                                          fixed,
+                                          w.r,
+                                          reinterpret_cast<const void *>(intptr_t(h.obj_generation)),
+                                          h.loc.get(),
+                                          reinterpret_cast<const void *>(intptr_t(h.op_assembled ? 1 : 0) |
+                                                                         intptr_t(precond(h) ? 2 : 0)),
+                                          h.solver_vals.get(),
+                                          h.pinv.get()};
     if (use_graph && (h.cg_graph == nullptr || h.cg_graph_key != key)) {
-        solver_release(h);
+        release_graph(h); // (the operator prepared above stays valid)
         cg_iteration(h, w, x, fixed); // sizes every scratch buffer before capture
-        k_cg_start<<<g, BLOCK, 0, h.stream>>>(w.y, w.r, w.p, b, fixed, h.n_dofs);
-        k_set_scalar<<<1, 1, 0, h.stream>>>(w.scal, S_FLAG, 0.0);
-        dot(h, w, w.r, w.r, fixed, S_RR, AFTER_NONE);
+        cg_start(h, w, fixed, b);
         hipGraph_t graph = nullptr;
         PF_HIP(hipStreamBeginCapture(h.stream, hipStreamCaptureModeThreadLocal));
         try {
-            cg_iteration(h, w, x, fixed);
+            for (int k = 0; k < CHECK; ++k) cg_iteration(h, w, x, fixed);
         } catch (...) {
             (void)hipStreamEndCapture(h.stream, &graph);
             if (graph) (void)hipGraphDestroy(graph);
@@ -393,17 +704,19 @@ int cg_solve_ws(Handle &h, const double *x, const uint8_t *fixed, const double *
     int it = 0;
     double rr = b2;
     while (it < max_it) {
+        // the graph holds CHECK iterations; a shorter remainder runs eagerly
         const int n = std::min(CHECK, max_it - it);
-        for (int k = 0; k < n; ++k) {
-            if (use_graph)
-                PF_HIP(hipGraphLaunch(static_cast<hipGraphExec_t>(h.cg_graph), h.stream));
-            else
-                cg_iteration(h, w, x, fixed);
-        }
+        if (use_graph && n == CHECK)
+            PF_HIP(hipGraphLaunch(static_cast<hipGraphExec_t>(h.cg_graph), h.stream));
+        else
+            for (int k = 0; k < n; ++k) cg_iteration(h, w, x, fixed);
         it += n;
-        rr = read_scalar(h, w, S_RR);
+        double sc[S_COUNT];
+        PF_HIP(hipMemcpyAsync(sc, w.scal, sizeof(sc), hipMemcpyDeviceToHost, h.stream));
+        PF_HIP(hipStreamSynchronize(h.stream));
+        rr = sc[S_RR];
         if (!std::isfinite(rr)) fail(PFGPU_ERR_INVALID_STATE, "non-finite residual in the CG solve (inverted element?)");
-        if (std::sqrt(rr) <= tol * std::sqrt(b2) || read_scalar(h, w, S_FLAG) != 0.0) break;
+        if (std::sqrt(rr) <= tol * std::sqrt(b2) || sc[S_FLAG] != 0.0) break;
     }
     if (rel) *rel = std::sqrt(rr / b2);
     return it;
@@ -422,6 +735,7 @@ void newton_solve(Handle &h, double *x, const uint8_t *fixed, const pfgpu_newton
         return objective_value(h, w, S_E1);
     };
     for (int it = 0; it < prm.max_iterations; ++it) {
+        h.op_valid = false; // a new iterate: the CG operator is rebuilt at x
         // energy and gradient at x; the reduced gradient norm
         objective_gradient(h, w, x, w.g, S_E0);
         dot(h, w, w.g, w.g, fixed, S_B2, AFTER_NONE);
@@ -536,6 +850,7 @@ struct ObjectiveScope {
     ObjectiveScope(Handle &hh, double ww, const double *x_tilde, const double *ff)
         : h(hh), w(hh.obj_w), xt(hh.obj_xt), f(hh.obj_f)
     {
+        h.op_valid = false;
         h.obj_w = ww;
         h.obj_xt = x_tilde;
         h.obj_f = ff;
@@ -543,6 +858,7 @@ struct ObjectiveScope {
     }
     ~ObjectiveScope()
     {
+        h.op_valid = false;
         h.obj_w = w;
         h.obj_xt = xt;
         h.obj_f = f;
@@ -724,11 +1040,11 @@ void transient_adjoint(Handle &h, const double *dJdx, const double *dJdv, int cg
 
 namespace {
 
-constexpr int SEG_CHUNK = 4096;
+constexpr int SEG_CHUNK = 4095; // dofs: whole nodes
 
-// per-problem CG state (device), one entry per problem
+// per-problem PCG state (device), one entry per problem
 struct SegState {
-    double *rr, *b2, *alpha, *beta, *done, *iters, *active;
+    double *rr, *rz, *b2, *alpha, *beta, *done, *iters, *active;
 };
 
 __device__ double block_sum(double v, double *sh)
@@ -771,15 +1087,51 @@ __global__ __launch_bounds__(BLOCK) void k_seg_total(const double *__restrict__ 
     for (int s = threadIdx.x; s < nseg; s += BLOCK) out[s] = seg_total(part, scp, s);
 }
 
-// CG start from the r.r partials: b2 = rr; a problem with b = 0 is done
-__global__ __launch_bounds__(BLOCK) void k_seg_cg_start(const double *__restrict__ part, const int32_t *__restrict__ scp,
+// PCG start over chunk c's nodes: y = 0, r = b on free dofs, z = P r, p = z;
+// partials of r.z and r.r
+__global__ __launch_bounds__(BLOCK) void k_chunk_pcg_start(const double *__restrict__ b, const uint8_t *__restrict__ mask,
+                                                           const float *__restrict__ pinv, const int32_t *__restrict__ cb,
+                                                           double *__restrict__ y, double *__restrict__ r,
+                                                           double *__restrict__ z, double *__restrict__ p,
+                                                           double *__restrict__ part_rz, double *__restrict__ part_rr)
+{
+    __shared__ double sh[BLOCK];
+    const int c = blockIdx.x;
+    double rz = 0, rr = 0;
+    for (int i = cb[c] / 3 + threadIdx.x; i < cb[c + 1] / 3; i += BLOCK) {
+        const int64_t o = 3 * int64_t(i);
+        double ri[3], zi[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) ri[d] = mask[o + d] ? 0.0 : b[o + d];
+        precondition(pinv, i, ri, zi);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            y[o + d] = 0.0;
+            r[o + d] = ri[d];
+            z[o + d] = zi[d];
+            p[o + d] = zi[d];
+            rz = fma(ri[d], zi[d], rz);
+            rr = fma(ri[d], ri[d], rr);
+        }
+    }
+    const double a = block_sum(rz, sh), e = block_sum(rr, sh);
+    if (threadIdx.x == 0) {
+        part_rz[c] = a;
+        part_rr[c] = e;
+    }
+}
+
+// PCG start from the r.z and r.r partials: b2 = r.r; a problem with b = 0 is done
+__global__ __launch_bounds__(BLOCK) void k_seg_cg_start(const double *__restrict__ part_rz,
+                                                        const double *__restrict__ part_rr, const int32_t *__restrict__ scp,
                                                         int nseg, SegState st)
 {
     __shared__ double sh[BLOCK];
     double act = 0;
     for (int s = threadIdx.x; s < nseg; s += BLOCK) {
-        const double v = seg_total(part, scp, s);
+        const double v = seg_total(part_rr, scp, s);
         st.rr[s] = v;
+        st.rz[s] = seg_total(part_rz, scp, s);
         st.b2[s] = v;
         st.done[s] = v > 0.0 ? 0.0 : 1.0;
         st.iters[s] = 0.0;
@@ -789,7 +1141,7 @@ __global__ __launch_bounds__(BLOCK) void k_seg_cg_start(const double *__restrict
     if (threadIdx.x == 0) *st.active = a;
 }
 
-// alpha = r.r / p.Hp per problem; a non-positive curvature ends that problem
+// alpha = r.z / p.Hp per problem; a non-positive curvature ends that problem
 __global__ __launch_bounds__(BLOCK) void k_seg_alpha(const double *__restrict__ part, const int32_t *__restrict__ scp,
                                                      int nseg, SegState st)
 {
@@ -801,49 +1153,65 @@ __global__ __launch_bounds__(BLOCK) void k_seg_alpha(const double *__restrict__ 
             st.alpha[s] = 0.0;
             st.done[s] = 1.0;
         } else {
-            st.alpha[s] = st.rr[s] / pq;
+            st.alpha[s] = st.rz[s] / pq;
         }
     }
 }
 
-// y += alpha p, r -= alpha q on free dofs; partial of the new r.r
-__global__ __launch_bounds__(BLOCK) void k_chunk_cg_update(double *__restrict__ y, double *__restrict__ r,
-                                                           const double *__restrict__ p, const double *__restrict__ q,
-                                                           const uint8_t *__restrict__ mask, const int32_t *__restrict__ cb,
-                                                           const int32_t *__restrict__ cs, const double *__restrict__ alpha,
-                                                           double *__restrict__ part)
+// y += alpha p, r -= alpha q on free dofs, z = P r over chunk c's nodes;
+// partials of the new r.z and r.r
+__global__ __launch_bounds__(BLOCK) void k_chunk_pcg_update(double *__restrict__ y, double *__restrict__ r,
+                                                            double *__restrict__ z, const double *__restrict__ p,
+                                                            const double *__restrict__ q, const uint8_t *__restrict__ mask,
+                                                            const float *__restrict__ pinv, const int32_t *__restrict__ cb,
+                                                            const int32_t *__restrict__ cs, const double *__restrict__ alpha,
+                                                            double *__restrict__ part_rz, double *__restrict__ part_rr)
 {
     __shared__ double sh[BLOCK];
     const int c = blockIdx.x;
     const double a = alpha[cs[c]];
-    double acc = 0;
-    for (int i = cb[c] + threadIdx.x; i < cb[c + 1]; i += BLOCK) {
-        if (mask && mask[i]) continue;
-        double ri = r[i];
-        if (a != 0.0) {
-            y[i] = fma(a, p[i], y[i]);
-            ri = fma(-a, q[i], ri);
-            r[i] = ri;
+    double rz = 0, rr = 0;
+    for (int i = cb[c] / 3 + threadIdx.x; i < cb[c + 1] / 3; i += BLOCK) {
+        const int64_t o = 3 * int64_t(i);
+        double ri[3], zi[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            ri[d] = r[o + d];
+            if (a != 0.0 && !mask[o + d]) {
+                y[o + d] = fma(a, p[o + d], y[o + d]);
+                ri[d] = fma(-a, q[o + d], ri[d]);
+                r[o + d] = ri[d];
+            }
         }
-        acc = fma(ri, ri, acc);
+        precondition(pinv, i, ri, zi);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            z[o + d] = zi[d];
+            rz = fma(ri[d], zi[d], rz);
+            rr = fma(ri[d], ri[d], rr);
+        }
     }
-    const double v = block_sum(acc, sh);
-    if (threadIdx.x == 0) part[c] = v;
+    const double e = block_sum(rz, sh), f = block_sum(rr, sh);
+    if (threadIdx.x == 0) {
+        part_rz[c] = e;
+        part_rr[c] = f;
+    }
 }
 
-// beta = r'.r' / r.r per problem; convergence at ||r|| <= tol ||b||
-__global__ __launch_bounds__(BLOCK) void k_seg_beta(const double *__restrict__ part, const int32_t *__restrict__ scp,
-                                                    int nseg, double tol2, SegState st)
+// beta = r'.z' / r.z per problem; convergence at ||r|| <= tol ||b||
+__global__ __launch_bounds__(BLOCK) void k_seg_beta(const double *__restrict__ part_rz, const double *__restrict__ part_rr,
+                                                    const int32_t *__restrict__ scp, int nseg, double tol2, SegState st)
 {
     __shared__ double sh[BLOCK];
     double act = 0;
     for (int s = threadIdx.x; s < nseg; s += BLOCK) {
-        const double rn = seg_total(part, scp, s);
-        if (!isfinite(rn)) act += __builtin_nan(""); // reported to the host as a failed solve
+        const double rn = seg_total(part_rr, scp, s), zn = seg_total(part_rz, scp, s);
+        if (!isfinite(rn) || !isfinite(zn)) act += __builtin_nan(""); // reported to the host as a failed solve
         if (st.done[s] == 0.0) {
             st.iters[s] += 1.0;
-            st.beta[s] = rn / st.rr[s];
+            st.beta[s] = st.rz[s] > 0.0 ? zn / st.rz[s] : 0.0;
             st.rr[s] = rn;
+            st.rz[s] = zn;
             if (rn <= tol2 * st.b2[s]) st.done[s] = 1.0;
         } else {
             st.beta[s] = 0.0;
@@ -854,15 +1222,15 @@ __global__ __launch_bounds__(BLOCK) void k_seg_beta(const double *__restrict__ p
     if (threadIdx.x == 0) *st.active = a;
 }
 
-// p = r + beta p for the problems still iterating
-__global__ __launch_bounds__(BLOCK) void k_chunk_cg_direction(double *__restrict__ p, const double *__restrict__ r,
+// p = z + beta p for the problems still iterating
+__global__ __launch_bounds__(BLOCK) void k_chunk_cg_direction(double *__restrict__ p, const double *__restrict__ z,
                                                               const int32_t *__restrict__ cb, const int32_t *__restrict__ cs,
                                                               const double *__restrict__ beta, const double *__restrict__ done)
 {
     const int c = blockIdx.x, s = cs[c];
     if (done[s] != 0.0) return;
     const double b = beta[s];
-    for (int i = cb[c] + threadIdx.x; i < cb[c + 1]; i += BLOCK) p[i] = fma(b, p[i], r[i]);
+    for (int i = cb[c] + threadIdx.x; i < cb[c + 1]; i += BLOCK) p[i] = fma(b, p[i], z[i]);
 }
 
 // out[s] = sum of v[off[s] .. off[s+1]) (one block per segment, fixed order)
@@ -905,7 +1273,7 @@ __global__ __launch_bounds__(BLOCK) void k_seg_steepest(double *__restrict__ d, 
 
 // chunks of at most SEG_CHUNK dofs inside each problem's dof range
 struct SegChunks {
-    DevBuf cb, cs, scp, part;
+    DevBuf cb, cs, scp, part, part2;
     int n = 0;
 };
 
@@ -926,20 +1294,32 @@ SegChunks seg_chunks(const int32_t *no, int nseg, hipStream_t st)
     c.cs = upload(cs, st);
     c.scp = upload(scp, st);
     c.part = DevBuf{sizeof(double) * size_t(c.n)};
+    c.part2 = DevBuf{sizeof(double) * size_t(c.n)};
     return c;
 }
 
-// One CG iteration on every active problem at once (a fixed kernel sequence).
+// PCG start of every active problem (a fixed kernel sequence)
+void seg_cg_start(Handle &h, const Work &w, const uint8_t *mask, const SegChunks &ch, int nseg, const SegState &st)
+{
+    k_chunk_pcg_start<<<ch.n, BLOCK, 0, h.stream>>>(w.d, mask, precond(h), ch.cb.as<int32_t>(), w.y, w.r, w.z, w.p,
+                                                    ch.part.as<double>(), ch.part2.as<double>());
+    k_seg_cg_start<<<1, BLOCK, 0, h.stream>>>(ch.part.as<double>(), ch.part2.as<double>(), ch.scp.as<int32_t>(), nseg, st);
+    PF_LAUNCH_CHECK();
+}
+
+// One PCG iteration on every active problem at once (a fixed kernel sequence).
 void seg_cg_iteration(Handle &h, const Work &w, const double *x, const uint8_t *mask, const SegChunks &ch, int nseg,
                       double tol2, const SegState &st)
 {
     hvp_any(h, x, w.p, w.q);
     k_chunk_dot<<<ch.n, BLOCK, 0, h.stream>>>(w.p, w.q, mask, ch.cb.as<int32_t>(), ch.part.as<double>());
     k_seg_alpha<<<1, BLOCK, 0, h.stream>>>(ch.part.as<double>(), ch.scp.as<int32_t>(), nseg, st);
-    k_chunk_cg_update<<<ch.n, BLOCK, 0, h.stream>>>(w.y, w.r, w.p, w.q, mask, ch.cb.as<int32_t>(), ch.cs.as<int32_t>(),
-                                                    st.alpha, ch.part.as<double>());
-    k_seg_beta<<<1, BLOCK, 0, h.stream>>>(ch.part.as<double>(), ch.scp.as<int32_t>(), nseg, tol2, st);
-    k_chunk_cg_direction<<<ch.n, BLOCK, 0, h.stream>>>(w.p, w.r, ch.cb.as<int32_t>(), ch.cs.as<int32_t>(), st.beta,
+    k_chunk_pcg_update<<<ch.n, BLOCK, 0, h.stream>>>(w.y, w.r, w.z, w.p, w.q, mask, precond(h), ch.cb.as<int32_t>(),
+                                                     ch.cs.as<int32_t>(), st.alpha, ch.part.as<double>(),
+                                                     ch.part2.as<double>());
+    k_seg_beta<<<1, BLOCK, 0, h.stream>>>(ch.part.as<double>(), ch.part2.as<double>(), ch.scp.as<int32_t>(), nseg, tol2,
+                                          st);
+    k_chunk_cg_direction<<<ch.n, BLOCK, 0, h.stream>>>(w.p, w.z, ch.cb.as<int32_t>(), ch.cs.as<int32_t>(), st.beta,
                                                        st.done);
     PF_LAUNCH_CHECK();
 }
@@ -961,11 +1341,12 @@ void newton_solve_batch(Handle &h, double *x, const uint8_t *fixed, int nseg, co
     DevBuf d_seg = upload(seg_of, h.stream);
     const SegChunks ch = seg_chunks(no_h, nseg, h.stream);
     const size_t ns = size_t(nseg);
-    // doubles: energy, gradient norm, g.d, step (4 ns), CG state (6 ns + 1); then two byte flags per problem
-    DevBuf scal{sizeof(double) * (10 * ns + 1) + 2 * ns}, pe{sizeof(double) * size_t(E)}, mask{size_t(n)};
+    // doubles: energy, gradient norm, g.d, step (4 ns), PCG state (7 ns + 1); then two byte flags per problem
+    DevBuf scal{sizeof(double) * (11 * ns + 1) + 2 * ns}, pe{sizeof(double) * size_t(E)}, mask{size_t(n)};
     double *s_e = scal.as<double>(), *s_g = s_e + ns, *s_gd = s_g + ns, *s_a = s_gd + ns;
-    const SegState st{s_a + ns, s_a + 2 * ns, s_a + 3 * ns, s_a + 4 * ns, s_a + 5 * ns, s_a + 6 * ns, s_a + 7 * ns};
-    uint8_t *frozen = reinterpret_cast<uint8_t *>(s_a + 7 * ns + 1), *flag = frozen + ns;
+    const SegState st{s_a + ns,     s_a + 2 * ns, s_a + 3 * ns, s_a + 4 * ns,
+                      s_a + 5 * ns, s_a + 6 * ns, s_a + 7 * ns, s_a + 8 * ns};
+    uint8_t *frozen = reinterpret_cast<uint8_t *>(s_a + 8 * ns + 1), *flag = frozen + ns;
     std::vector<double> e0(ns), e1(ns), gn(ns), gd(ns), alpha(ns), it_cg(ns);
     std::vector<uint8_t> fr(ns, 0), fl(ns, 0);
     std::vector<pfgpu_newton_report> rep(ns);
@@ -999,7 +1380,8 @@ void newton_solve_batch(Handle &h, double *x, const uint8_t *fixed, int nseg, co
     // the CG iteration, captured once per call: every pointer it reads is fixed
     // for the call (x, the working mask and the workspace change contents only)
     hipGraphExec_t graph = nullptr;
-    const void *graph_loc = nullptr; // the element scratch the captured Hv writes
+    // the scratch and operator buffers the captured iteration uses
+    const void *graph_loc = nullptr, *graph_vals = nullptr, *graph_pinv = nullptr;
     struct GraphGuard {
         hipGraphExec_t &g;
         ~GraphGuard()
@@ -1010,25 +1392,19 @@ void newton_solve_batch(Handle &h, double *x, const uint8_t *fixed, int nseg, co
     const double tol2 = prm.cg_tolerance * prm.cg_tolerance;
     // per-problem CG on H_ff y = d over the active problems; iterations per problem into it_cg
     auto seg_cg = [&]() {
-        k_cg_start<<<gr, BLOCK, 0, h.stream>>>(w.y, w.r, w.p, w.d, mask.as<uint8_t>(), n);
-        k_chunk_dot<<<ch.n, BLOCK, 0, h.stream>>>(w.r, w.r, mask.as<uint8_t>(), ch.cb.as<int32_t>(), ch.part.as<double>());
-        k_seg_cg_start<<<1, BLOCK, 0, h.stream>>>(ch.part.as<double>(), ch.scp.as<int32_t>(), nseg, st);
-        PF_LAUNCH_CHECK();
-        if (graph && graph_loc != h.loc.get()) {
+        seg_cg_start(h, w, mask.as<uint8_t>(), ch, nseg, st);
+        if (graph && (graph_loc != h.loc.get() || graph_vals != h.solver_vals.get() || graph_pinv != h.pinv.get())) {
             (void)hipGraphExecDestroy(graph);
             graph = nullptr;
         }
         if (prm.use_graph && !graph) {
             // an eager iteration sizes every scratch buffer; then restart and capture
             seg_cg_iteration(h, w, x, mask.as<uint8_t>(), ch, nseg, tol2, st);
-            k_cg_start<<<gr, BLOCK, 0, h.stream>>>(w.y, w.r, w.p, w.d, mask.as<uint8_t>(), n);
-            k_chunk_dot<<<ch.n, BLOCK, 0, h.stream>>>(w.r, w.r, mask.as<uint8_t>(), ch.cb.as<int32_t>(),
-                                                      ch.part.as<double>());
-            k_seg_cg_start<<<1, BLOCK, 0, h.stream>>>(ch.part.as<double>(), ch.scp.as<int32_t>(), nseg, st);
+            seg_cg_start(h, w, mask.as<uint8_t>(), ch, nseg, st);
             hipGraph_t g = nullptr;
             PF_HIP(hipStreamBeginCapture(h.stream, hipStreamCaptureModeThreadLocal));
             try {
-                seg_cg_iteration(h, w, x, mask.as<uint8_t>(), ch, nseg, tol2, st);
+                for (int k = 0; k < CHECK; ++k) seg_cg_iteration(h, w, x, mask.as<uint8_t>(), ch, nseg, tol2, st);
             } catch (...) {
                 (void)hipStreamEndCapture(h.stream, &g);
                 if (g) (void)hipGraphDestroy(g);
@@ -1039,17 +1415,18 @@ void newton_solve_batch(Handle &h, double *x, const uint8_t *fixed, int nseg, co
             (void)hipGraphDestroy(g);
             PF_HIP(e);
             graph_loc = h.loc.get();
+            graph_vals = h.solver_vals.get();
+            graph_pinv = h.pinv.get();
         }
         int it = 0;
         std::vector<double> act(1);
         while (it < prm.cg_max_iterations) {
+            // the graph holds CHECK iterations; a shorter remainder runs eagerly
             const int k = std::min(CHECK, prm.cg_max_iterations - it);
-            for (int j = 0; j < k; ++j) {
-                if (graph)
-                    PF_HIP(hipGraphLaunch(graph, h.stream));
-                else
-                    seg_cg_iteration(h, w, x, mask.as<uint8_t>(), ch, nseg, tol2, st);
-            }
+            if (graph && k == CHECK)
+                PF_HIP(hipGraphLaunch(graph, h.stream));
+            else
+                for (int j = 0; j < k; ++j) seg_cg_iteration(h, w, x, mask.as<uint8_t>(), ch, nseg, tol2, st);
             it += k;
             PF_HIP(hipMemcpyAsync(act.data(), st.active, sizeof(double), hipMemcpyDeviceToHost, h.stream));
             PF_HIP(hipStreamSynchronize(h.stream));
@@ -1106,7 +1483,10 @@ void newton_solve_batch(Handle &h, double *x, const uint8_t *fixed, int nseg, co
         if (!any) break;
         upload_flags(frozen, fr);
         k_seg_mask<<<gr, BLOCK, 0, h.stream>>>(mask.as<uint8_t>(), fixed, frozen, d_seg.as<int32_t>(), n);
-        // Newton directions of all active problems: H_ff y = -g_f, per-problem CG
+        PF_LAUNCH_CHECK();
+        h.op_valid = false;
+        prepare_operator(h, x, mask.as<uint8_t>());
+        // Newton directions of all active problems: H_ff y = -g_f, per-problem PCG
         k_neg_free<<<gr, BLOCK, 0, h.stream>>>(w.d, w.g, mask.as<uint8_t>(), n);
         PF_LAUNCH_CHECK();
         seg_cg();

@@ -167,3 +167,37 @@ def test_adjoint_material_gradient_matches_finite_differences():
         fd = (jp - jm) / (2 * h)
         lin = grad[:m.n_elements] @ dl + grad[m.n_elements:] @ dm
         assert abs(fd - lin) <= 1e-5 * max(abs(lin), 1e-12) + 1e-14, (fd, lin)
+
+
+@pytest.mark.parametrize("order,precision", [(1, "fp64"), (2, "fp64"), (1, "mixed")])
+def test_assembled_and_matrix_free_operators_agree(order, precision):
+    """One unpreconditioned CG step from 0 is y = (b.b / b.Ab) b: the
+    block-sparse product of the assembled values and the matrix-free element
+    product give the same step; full solves agree with and without the
+    block-Jacobi preconditioner."""
+    m, prob = clamped_box(order, (2, 3, 2))
+    lam, mu = random_lame(m.n_elements, seed=BASE_SEED + 70)
+    x = smooth_displacement(m.nodes, 0.02, 1e-3, BASE_SEED + 71)
+    x[prob.fixed] = prob.fixed_values
+    b = np.random.default_rng(BASE_SEED + 72).normal(size=m.n_dofs)
+    print(f"operators order={order} {precision}", flush=True)
+    a = NLAssembler("NeoHookean", m, lame=(lam, mu), precision=precision)
+    one, full = {}, {}
+    for hess in ("assembled", "matrix_free"):
+        for pre in ("none", "block_jacobi"):
+            a.set_solver_options(hess, pre)
+            if pre == "none":
+                one[hess] = cg_solve(a, x, b, prob.mask, max_iterations=1, tolerance=0.0)[0]
+            full[(hess, pre)], it, r = cg_solve(a, x, b, prob.mask, max_iterations=2000, tolerance=1e-11)
+            assert r <= 1e-11 or precision == "mixed", (hess, pre, r)
+    tol1 = 1e-12 if precision == "fp64" else 1e-5
+    assert rel(one["assembled"], one["matrix_free"]) < tol1
+    ref = full[("matrix_free", "none")]
+    tol = 1e-9 if precision == "fp64" else 1e-3
+    for k, y in full.items():
+        assert rel(y, ref) < tol, k
+    # the solve: H_ff y_f = b_f (FP64 oracle Hessian)
+    if precision == "fp64":
+        H = po.Oracle(m, lam, mu).hessian_scipy(x).tocsr()[prob.free][:, prob.free]
+        res = H @ full[("assembled", "block_jacobi")][prob.free] - b[prob.free]
+        assert np.linalg.norm(res) <= 1e-9 * np.linalg.norm(b[prob.free])
