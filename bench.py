@@ -166,24 +166,42 @@ def extras(args, m, lam, mu, x_host, a, x, g, vals, y, eqp):
     return res
 
 
+_ARGS = None
+
+
+def _clock():
+    import torch
+    torch.cuda.synchronize()
+    return time.perf_counter()
+
+
+def _log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
 def suite(args):
     """Further workloads at full size, one JSON line each (timings in seconds
     unless named otherwise)."""
-    import numpy as np
+    global _ARGS
     import torch
-
-    from polyfem_amd import BatchSolver, NewtonSolver, NLAssembler, NLProblem, TransientSimulation, cg_solve
-    from polyfem_amd.mesh import (BASE_SEED, face_nodes, kuhn_box, lame_from_young, random_lame,
-                                  random_smooth_field, smooth_displacement)
-
     torch.cuda.is_available()
+    _ARGS = args
+    parts = set(args.suite_parts.split(","))
+    if "step" in parts:
+        suite_step()
+    if "batch" in parts:
+        suite_batch()
+    if "cantilever" in parts:
+        suite_cantilever()
+    if "transient" in parts:
+        suite_transient()
 
-    def clock():
-        torch.cuda.synchronize()
-        return time.perf_counter()
 
-    def log(msg):
-        print(msg, file=sys.stderr, flush=True)
+def suite_step():
+    import numpy as np
+    from polyfem_amd import NewtonSolver, NLAssembler, NLProblem
+    from polyfem_amd.mesh import BASE_SEED, face_nodes, kuhn_box, smooth_displacement
+    clock = _clock
 
     # ---- P1 20^3 x 6, FP64, E = 1e6, nu = 0.4: E+G+H, then one Newton step on the
     # free dofs (bottom clamped, top moved by +0.1 z) with alpha = 1, and the same
@@ -222,6 +240,14 @@ def suite(args):
                       "grad_norm_before": g0, "newton_step": steps}), flush=True)
     a.close()
 
+
+def suite_batch():
+    import numpy as np
+    import torch
+    from polyfem_amd import BatchSolver, NewtonSolver
+    from polyfem_amd.mesh import BASE_SEED, face_nodes, kuhn_box, lame_from_young
+    args, clock, log = _ARGS, _clock, _log
+
     # ---- batch: 256 box meshes with tet counts log-uniform in [1k, 200k], problem i P1
     # if i is even, LinearElasticity if i // 2 is even; per problem E log-normal (mean
     # ln 1e6, sigma 0.5), nu uniform [0.3, 0.45], bottom clamped and top pulled by 2-8 %
@@ -247,8 +273,8 @@ def suite(args):
         probs.append(dict(mesh=mb, material=material, lame=(float(lam_i), float(mu_i)), fixed_dofs=fixed,
                           fixed_values=vals, x0=x0))
     n_tets = sum(p["mesh"].n_elements for p in probs)
-    newton = NewtonSolver(max_iterations=args.batch_newton, grad_norm=0.0, cg_max_iterations=args.batch_cg,
-                          cg_tolerance=1e-6)
+    newton = NewtonSolver(max_iterations=args.batch_newton, grad_norm=0.0, relative_grad_norm=args.batch_rtol,
+                          cg_max_iterations=args.batch_cg, cg_tolerance=1e-6)
     bs = BatchSolver(probs, precision="mixed", newton=newton)
     tuning, g0 = [], np.zeros(len(probs))
     for grp in bs.groups:
@@ -271,7 +297,9 @@ def suite(args):
     print(json.dumps({
         "workload": f"batch of {len(probs)} box meshes, {n_tets} tets in all (1k-200k each, half P1 / half P2, "
                     "half LinearElasticity / half NeoHookean), mixed precision",
-        "newton": {"iterations": args.batch_newton, "cg_max_iterations": args.batch_cg, "cg_tolerance": 1e-6},
+        "newton": {"max_iterations": args.batch_newton, "relative_grad_norm": args.batch_rtol,
+                   "cg_max_iterations": args.batch_cg, "cg_tolerance": 1e-6},
+        "converged": int(sum(r.converged for r in reps)), "newton_iterations_max": max(r.iterations for r in reps),
         "solve_s": t_batch, "problems_per_s": len(probs) / t_batch,
         "cg_iterations_total": int(sum(r.cg_iterations for r in reps)),
         "grad_reduction": {"median": float(np.median(red)), "max": float(red.max()),
@@ -282,6 +310,13 @@ def suite(args):
     for grp in bs.groups:
         grp["a"].close()
     del bs, probs
+
+
+def suite_cantilever():
+    import numpy as np
+    from polyfem_amd import NewtonSolver, NLAssembler, NLProblem, TransientSimulation, cg_solve
+    from polyfem_amd.mesh import BASE_SEED, face_nodes, kuhn_box, random_lame, random_smooth_field
+    args, clock, log = _ARGS, _clock, _log
 
     # ---- cantilever: 160 x 40 x 40 cells over [0,4]x[0,1]x[0,1], P1 NeoHookean, FP64,
     # clamped at x = 0, gravity rho g (rho = 1000, g = 0.2 m/s^2 along -z, a dead load:
@@ -373,6 +408,13 @@ def suite(args):
                      "seconds": t_fd}}), flush=True)
     a.close()
 
+
+def suite_transient():
+    import numpy as np
+    from polyfem_amd import NewtonSolver, NLAssembler, TransientSimulation
+    from polyfem_amd.mesh import BASE_SEED, kuhn_box, random_lame, random_smooth_field
+    args, clock, log = _ARGS, _clock, _log
+
     # ---- implicit Euler: 80^3 x 6 P1 tets, rho = 1000, dt = 1e-3, 100 steps, checkpoints
     # every 10, v0 a seeded random smooth field of amplitude 0.5; J = 1/2 v_T^T M v_T +
     # E(u_T); dJ/dv0 and dJ/dmu (the adjoint's implicit part plus the explicit
@@ -454,8 +496,12 @@ def main():
                     help="instead of the headline line: one JSON line per further configuration "
                          "(P1 20^3 FP64 E+G+H and Newton step; 1.5M-tet cantilever static adjoint; "
                          "3M-tet implicit-Euler adjoint), timed at full size")
+    ap.add_argument("--suite-parts", default="step,batch,cantilever,transient",
+                    help="--suite: comma-separated subset of step, batch, cantilever, transient")
     ap.add_argument("--batch-problems", type=int, default=256, help="--suite: problems in the batch")
-    ap.add_argument("--batch-newton", type=int, default=10, help="--suite: Newton iterations of the batch")
+    ap.add_argument("--batch-newton", type=int, default=20, help="--suite: Newton iteration cap of the batch")
+    ap.add_argument("--batch-rtol", type=float, default=1e-4,
+                    help="--suite: per-problem gradient reduction at which a batch problem has converged")
     ap.add_argument("--batch-cg", type=int, default=2000, help="--suite: CG iteration cap of the batch")
     ap.add_argument("--fd-directions", type=int, default=64,
                     help="--suite: random Lame directions of the cantilever's finite-difference check")

@@ -203,6 +203,9 @@ typedef struct {
     double armijo_c;           /* sufficient-decrease constant */
     int32_t max_line_search;   /* step halvings before giving up */
     int32_t use_graph;         /* 1: CG iterations replayed from a captured hipGraph */
+    /* also converged once ||g_f|| <= relative_grad_norm ||g_f(x0)|| (per
+     * problem in a batch); 0 disables */
+    double relative_grad_norm;
 } pfgpu_newton_params;
 
 typedef struct {
@@ -226,7 +229,8 @@ pfgpu_status pfgpu_newton_solve_dev(pfgpu_handle *h, double *x, const uint8_t *f
  * node_offsets[s+1]).  Every assembly kernel is one launch for the whole
  * batch; one CG on the block-diagonal system gives all Newton directions;
  * line search, step length and convergence are per problem (reports has
- * n_problems entries).  Host pointers; the elastic energy only (no load, no
+ * n_problems entries); each problem's CG stops at its own tolerance and a
+ * problem that has converged costs no further element work.  Host pointers; the elastic energy only (no load, no
  * inertia). */
 pfgpu_status pfgpu_newton_solve_batch(pfgpu_handle *h, double *x, const uint8_t *fixed, int32_t n_problems,
                                       const int32_t *elem_offsets, const int32_t *node_offsets,

@@ -38,7 +38,7 @@ class Info(C.Structure):
 class NewtonParams(C.Structure):
     _fields_ = [("max_iterations", C.c_int32), ("grad_norm", C.c_double), ("cg_max_iterations", C.c_int32),
                 ("cg_tolerance", C.c_double), ("line_search", C.c_int32), ("armijo_c", C.c_double),
-                ("max_line_search", C.c_int32), ("use_graph", C.c_int32)]
+                ("max_line_search", C.c_int32), ("use_graph", C.c_int32), ("relative_grad_norm", C.c_double)]
 
 
 class NewtonReport(C.Structure):

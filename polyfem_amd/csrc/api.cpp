@@ -698,7 +698,9 @@ void check_newton_params(const pfgpu_newton_params *p)
 {
     require(p != nullptr, "null Newton parameters");
     require(p->max_iterations >= 0 && p->cg_max_iterations >= 0 && p->max_line_search >= 0, "negative iteration count");
-    require(p->grad_norm >= 0 && p->cg_tolerance >= 0 && p->armijo_c >= 0 && p->armijo_c < 1, "invalid tolerance");
+    require(p->grad_norm >= 0 && p->cg_tolerance >= 0 && p->armijo_c >= 0 && p->armijo_c < 1 &&
+                p->relative_grad_norm >= 0,
+            "invalid tolerance");
     require(p->line_search == 0 || p->line_search == 1, "line_search must be 0 or 1");
 }
 

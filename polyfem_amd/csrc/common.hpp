@@ -100,6 +100,16 @@ template <typename T> void upload_into(DevBuf &dst, const std::vector<T> &v, hip
     }
 }
 
+// Problems of a batch whose element work is skipped (their CG has finished):
+// flag[s] != 0 skips problem s; elem_seg / node_seg map elements / nodes to
+// problems.  Empty (flag null) outside a batched solve.
+struct SegSkip {
+    const int32_t *elem_seg = nullptr, *node_seg = nullptr;
+    const double *flag = nullptr;
+    __host__ __device__ bool element(int e) const { return flag && flag[elem_seg[e]] != 0.0; }
+    __host__ __device__ bool node(int i) const { return flag && flag[node_seg[i]] != 0.0; }
+};
+
 // Element data as the kernels see it.  Every per-element array is SoA
 // (component-major, element-minor) so that one wavefront reading 64
 // consecutive elements issues 64 consecutive addresses.

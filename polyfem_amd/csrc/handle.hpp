@@ -69,6 +69,7 @@ struct pfgpu_handle {
     // CG operator: assembled block-sparse values or matrix-free, block-Jacobi preconditioner
     int solver_hessian = PFGPU_SOLVER_HESSIAN_AUTO, solver_precond = PFGPU_PRECOND_BLOCK_JACOBI;
     bool op_valid = false, op_assembled = false, op_precond = false;
+    pfgpu::SegSkip skip; // set only inside a batched CG
     pfgpu::DevBuf solver_vals, pinv, diag_pos;
     hipStream_t own_stream = nullptr; // used when the caller's stream is the legacy default
     ~pfgpu_handle();

@@ -231,11 +231,11 @@ __global__ __launch_bounds__(HB) void k_hessian(DevMesh<T> m, const double *__re
 // local H(x) v; yloc[(e*NB + a)*3 + d]
 template <typename T, int ORDER, int NQ, int MAT>
 __global__ __launch_bounds__(BLOCK) void k_hvp(DevMesh<T> m, const double *__restrict__ x, const double *__restrict__ v,
-                                               T *__restrict__ yloc)
+                                               T *__restrict__ yloc, SegSkip skip)
 {
     constexpr int NB = NLocal<ORDER>::value;
     const int e = blockIdx.x * BLOCK + threadIdx.x;
-    if (e >= m.n_elements) return;
+    if (e >= m.n_elements || skip.element(e)) return;
     Elem<T, ORDER, NQ> el;
     load_geometry(m, e, el);
     gather(m, e, x, el.U);
@@ -467,7 +467,8 @@ inline unsigned xcd_grid(int n_chunks) { return unsigned((n_chunks + N_XCD - 1) 
 template <typename T>
 __global__ __launch_bounds__(BLOCK) void k_gather_rows_seg(int n_chunks, const int32_t *__restrict__ chunk,
                                                            const int32_t *__restrict__ ptr, const int32_t *__restrict__ inc,
-                                                           const T *__restrict__ loc, double *__restrict__ out)
+                                                           const T *__restrict__ loc, double *__restrict__ out,
+                                                           SegSkip skip)
 {
     __shared__ double sv[SEG * 3];
     const int c = xcd_chunk(n_chunks);
@@ -475,6 +476,15 @@ __global__ __launch_bounds__(BLOCK) void k_gather_rows_seg(int n_chunks, const i
     const int i0 = chunk[c], i1 = chunk[c + 1];
     const int k0 = ptr[i0], k1 = ptr[i1];
     const int t = threadIdx.x;
+    // rows of skipped problems are written as zero; a chunk lies in at most a
+    // few problems, so a fully skipped chunk reads nothing
+    if (skip.node(i0) && skip.node(i1 - 1)) {
+        for (int i = i0 + t; i < i1; i += BLOCK)
+            if (skip.node(i)) out[3 * int64_t(i)] = out[3 * int64_t(i) + 1] = out[3 * int64_t(i) + 2] = 0.0;
+        bool all = true;
+        for (int i = i0; i < i1 && all; ++i) all = skip.node(i);
+        if (all) return;
+    }
     if (k0 + t < k1) {
         const T *l = loc + int64_t(inc[k0 + t]) * 3;
         sv[t * 3] = double(l[0]);
@@ -484,6 +494,10 @@ __global__ __launch_bounds__(BLOCK) void k_gather_rows_seg(int n_chunks, const i
     __syncthreads();
     for (int i = i0 + t; i < i1; i += BLOCK) {
         double s0 = 0, s1 = 0, s2 = 0;
+        if (skip.node(i)) {
+            out[3 * int64_t(i)] = out[3 * int64_t(i) + 1] = out[3 * int64_t(i) + 2] = 0.0;
+            continue;
+        }
         for (int k = ptr[i] - k0; k < ptr[i + 1] - k0; ++k) {
             s0 += sv[k * 3];
             s1 += sv[k * 3 + 1];
@@ -608,10 +622,12 @@ void gather_rows(Handle &h, const void *loc, double *out)
     ensure_row_chunks(h);
     if (h.precision == PFGPU_MIXED)
         k_gather_rows_seg<float><<<xcd_grid(h.n_row_chunks), BLOCK, 0, h.stream>>>(
-            h.n_row_chunks, h.row_chunk.as<int32_t>(), h.node_ptr.as<int32_t>(), h.node_inc.as<int32_t>(), static_cast<const float *>(loc), out);
+            h.n_row_chunks, h.row_chunk.as<int32_t>(), h.node_ptr.as<int32_t>(), h.node_inc.as<int32_t>(), static_cast<const float *>(loc), out,
+            h.skip);
     else
         k_gather_rows_seg<double><<<xcd_grid(h.n_row_chunks), BLOCK, 0, h.stream>>>(
-            h.n_row_chunks, h.row_chunk.as<int32_t>(), h.node_ptr.as<int32_t>(), h.node_inc.as<int32_t>(), static_cast<const double *>(loc), out);
+            h.n_row_chunks, h.row_chunk.as<int32_t>(), h.node_ptr.as<int32_t>(), h.node_inc.as<int32_t>(), static_cast<const double *>(loc), out,
+            h.skip);
     PF_LAUNCH_CHECK();
 }
 
@@ -692,7 +708,7 @@ void plain_hvp(Handle &h, const double *x, const double *v, double *y)
     const unsigned grid = grid_for(h.n_elements);
     h.loc.reserve(h.value_size() * size_t(h.n_elements) * h.n_local * 3);
     dispatch(h, [&]<typename T, int O, int Q, int M>() {
-        k_hvp<T, O, Q, M><<<grid, BLOCK, 0, h.stream>>>(h.mesh<T>(), x, v, h.loc.as<T>());
+        k_hvp<T, O, Q, M><<<grid, BLOCK, 0, h.stream>>>(h.mesh<T>(), x, v, h.loc.as<T>(), h.skip);
         PF_LAUNCH_CHECK();
         gather_rows(h, h.loc.get(), y);
     });

@@ -213,12 +213,12 @@ template <typename T>
 __global__ __launch_bounds__(BLOCK) void k_bsr_spmv(int n_nodes, const int32_t *__restrict__ brow_ptr,
                                                     const int32_t *__restrict__ bcol, const T *__restrict__ vals,
                                                     const double *__restrict__ mass, double w,
-                                                    const double *__restrict__ v, double *__restrict__ y)
+                                                    const double *__restrict__ v, double *__restrict__ y, SegSkip skip)
 {
     const int lane = threadIdx.x % SPMV_LANES;
     const int i = blockIdx.x * (BLOCK / SPMV_LANES) + threadIdx.x / SPMV_LANES;
     double h0 = 0, h1 = 0, h2 = 0, m0 = 0, m1 = 0, m2 = 0;
-    if (i < n_nodes) {
+    if (i < n_nodes && !skip.node(i)) {
         const int b0 = brow_ptr[i], deg = brow_ptr[i + 1] - b0;
         const T *row = vals + 9 * int64_t(b0);
         for (int pos = lane; pos < deg; pos += SPMV_LANES) {
@@ -467,7 +467,7 @@ void hvp_any(Handle &h, const double *x, const double *v, double *y)
         const unsigned g = grid_for(h.n_nodes, BLOCK / SPMV_LANES);
         auto spmv = [&]<typename T>() {
             k_bsr_spmv<T><<<g, BLOCK, 0, h.stream>>>(h.n_nodes, h.brow_ptr.as<int32_t>(), h.pat_bcol.as<int32_t>(),
-                                                     h.solver_vals.as<T>(), nullptr, 1.0, v, y);
+                                                     h.solver_vals.as<T>(), nullptr, 1.0, v, y, h.skip);
         };
         if (h.precision == PFGPU_MIXED)
             spmv.template operator()<float>();
@@ -730,6 +730,7 @@ void newton_solve(Handle &h, double *x, const uint8_t *fixed, const pfgpu_newton
     const unsigned gr = grid_for(h.n_dofs);
     rep = pfgpu_newton_report{};
     if (inertia(h) && !h.has_mass) fail(PFGPU_ERR_INVALID_STATE, "inertia term without a mass matrix");
+    double gn0 = 0; // gradient norm at the initial guess
     auto energy_at = [&](const double *xx) {
         objective_energy(h, w, xx, S_E1);
         return objective_value(h, w, S_E1);
@@ -744,7 +745,8 @@ void newton_solve(Handle &h, double *x, const uint8_t *fixed, const pfgpu_newton
         rep.energy = e0;
         rep.grad_norm = gn;
         if (!std::isfinite(gn) || !std::isfinite(e0)) fail(PFGPU_ERR_INVALID_STATE, "non-finite energy or gradient at the Newton iterate");
-        if (gn <= prm.grad_norm) {
+        if (it == 0) gn0 = gn;
+        if (gn <= prm.grad_norm || gn <= prm.relative_grad_norm * gn0) {
             rep.converged = 1;
             return;
         }
@@ -816,7 +818,7 @@ void newton_solve(Handle &h, double *x, const uint8_t *fixed, const pfgpu_newton
     dot(h, w, w.g, w.g, fixed, S_B2, AFTER_NONE);
     rep.grad_norm = std::sqrt(read_scalar(h, w, S_B2));
     rep.energy = objective_value(h, w, S_E0);
-    rep.converged = rep.grad_norm <= prm.grad_norm;
+    rep.converged = rep.grad_norm <= prm.grad_norm || rep.grad_norm <= prm.relative_grad_norm * gn0;
 }
 
 // ---- implicit Euler with checkpointed adjoint --------------------------
@@ -1339,6 +1341,10 @@ void newton_solve_batch(Handle &h, double *x, const uint8_t *fixed, int nseg, co
         for (int i = no_h[sg]; i < no_h[sg + 1]; ++i) seg_of[size_t(i)] = sg;
     DevBuf d_eo = upload(std::vector<int32_t>(eo_h, eo_h + nseg + 1), h.stream);
     DevBuf d_seg = upload(seg_of, h.stream);
+    std::vector<int32_t> eseg_of(static_cast<size_t>(E));
+    for (int sg = 0; sg < nseg; ++sg)
+        for (int e = eo_h[sg]; e < eo_h[sg + 1]; ++e) eseg_of[size_t(e)] = sg;
+    DevBuf d_eseg = upload(eseg_of, h.stream);
     const SegChunks ch = seg_chunks(no_h, nseg, h.stream);
     const size_t ns = size_t(nseg);
     // doubles: energy, gradient norm, g.d, step (4 ns), PCG state (7 ns + 1); then two byte flags per problem
@@ -1347,7 +1353,7 @@ void newton_solve_batch(Handle &h, double *x, const uint8_t *fixed, int nseg, co
     const SegState st{s_a + ns,     s_a + 2 * ns, s_a + 3 * ns, s_a + 4 * ns,
                       s_a + 5 * ns, s_a + 6 * ns, s_a + 7 * ns, s_a + 8 * ns};
     uint8_t *frozen = reinterpret_cast<uint8_t *>(s_a + 8 * ns + 1), *flag = frozen + ns;
-    std::vector<double> e0(ns), e1(ns), gn(ns), gd(ns), alpha(ns), it_cg(ns);
+    std::vector<double> e0(ns), e1(ns), gn(ns), gn0(ns), gd(ns), alpha(ns), it_cg(ns);
     std::vector<uint8_t> fr(ns, 0), fl(ns, 0);
     std::vector<pfgpu_newton_report> rep(ns);
     auto read = [&](const double *dev, std::vector<double> &host) {
@@ -1392,6 +1398,12 @@ void newton_solve_batch(Handle &h, double *x, const uint8_t *fixed, int nseg, co
     const double tol2 = prm.cg_tolerance * prm.cg_tolerance;
     // per-problem CG on H_ff y = d over the active problems; iterations per problem into it_cg
     auto seg_cg = [&]() {
+        // problems whose CG has finished (or that are frozen) skip their element work
+        struct SkipScope {
+            Handle &h;
+            ~SkipScope() { h.skip = SegSkip{}; }
+        } skip_scope{h};
+        h.skip = SegSkip{d_eseg.as<int32_t>(), d_seg.as<int32_t>(), st.done};
         seg_cg_start(h, w, mask.as<uint8_t>(), ch, nseg, st);
         if (graph && (graph_loc != h.loc.get() || graph_vals != h.solver_vals.get() || graph_pinv != h.pinv.get())) {
             (void)hipGraphExecDestroy(graph);
@@ -1474,7 +1486,8 @@ void newton_solve_batch(Handle &h, double *x, const uint8_t *fixed, int nseg, co
                 fail(PFGPU_ERR_INVALID_STATE, "non-finite energy or gradient at a batched Newton iterate");
             rep[sg].energy = e0[sg];
             rep[sg].grad_norm = gn[sg];
-            if (!fr[sg] && gn[sg] <= prm.grad_norm) {
+            if (it == 0) gn0[sg] = gn[sg];
+            if (!fr[sg] && (gn[sg] <= prm.grad_norm || gn[sg] <= prm.relative_grad_norm * gn0[sg])) {
                 fr[sg] = 1;
                 rep[sg].converged = 1;
             }
@@ -1551,7 +1564,7 @@ void newton_solve_batch(Handle &h, double *x, const uint8_t *fixed, int nseg, co
     for (int sg = 0; sg < nseg; ++sg) {
         rep[sg].energy = e0[sg];
         rep[sg].grad_norm = gn[sg];
-        rep[sg].converged = gn[sg] <= prm.grad_norm;
+        rep[sg].converged = gn[sg] <= prm.grad_norm || gn[sg] <= prm.relative_grad_norm * gn0[sg];
         reps[sg] = rep[sg];
     }
 }

@@ -73,12 +73,12 @@ class NewtonSolver:
 
     def __init__(self, max_iterations: int = 10, grad_norm: float = 1e-8, cg_max_iterations: int = 1000,
                  cg_tolerance: float = 1e-6, line_search: str = "Backtracking", armijo_c: float = 1e-4,
-                 max_line_search: int = 30, use_graph: bool = True):
+                 max_line_search: int = 30, use_graph: bool = True, relative_grad_norm: float = 0.0):
         if line_search not in ("Backtracking", "None"):
             raise ValueError("line_search must be 'Backtracking' or 'None'")
         self.params = _ffi.NewtonParams(max_iterations, grad_norm, cg_max_iterations, cg_tolerance,
                                         1 if line_search == "Backtracking" else 0, armijo_c, max_line_search,
-                                        1 if use_graph else 0)
+                                        1 if use_graph else 0, relative_grad_norm)
 
     def minimize(self, assembler, x, problem: NLProblem):
         """Returns (x*, report); x holds the initial guess on the free dofs and
