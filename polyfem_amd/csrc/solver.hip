@@ -35,7 +35,10 @@ namespace {
 
 constexpr int DOT_BLOCKS = 1024; // fixed grid of the dot-product kernels
 constexpr int CHECK = 8;         // CG iterations between host convergence checks
-constexpr int SPMV_LANES = 16;   // lanes per block row of the block-sparse product
+#ifndef PF_SPMV_LANES
+#define PF_SPMV_LANES 16
+#endif
+constexpr int SPMV_LANES = PF_SPMV_LANES; // lanes per block row of the block-sparse product
 
 // scalar slots of the CG workspace (r.z: preconditioned residual product)
 enum { S_RR, S_PQ, S_ALPHA, S_BETA, S_RZ, S_FLAG, S_B2, S_GD, S_E0, S_E1, S_M, S_F, S_COUNT };
@@ -1075,18 +1078,38 @@ __global__ __launch_bounds__(BLOCK) void k_chunk_dot(const double *__restrict__ 
     if (threadIdx.x == 0) part[c] = v;
 }
 
-// per-problem totals of the chunk partials, in chunk order
+// The per-problem scalar kernels run as one workgroup of SEGW threads, one
+// wavefront per problem at a time: lane l adds the chunk partials l, l+64, ...
+// of the problem in order, then a fixed butterfly leaves the total in every lane.
+constexpr int SEGW = 1024, WAVE = 64;
+
 __device__ double seg_total(const double *__restrict__ part, const int32_t *__restrict__ scp, int s)
 {
     double v = 0;
-    for (int c = scp[s]; c < scp[s + 1]; ++c) v += part[c];
+    for (int c = scp[s] + int(threadIdx.x % WAVE); c < scp[s + 1]; c += WAVE) v += part[c];
+#pragma unroll
+    for (int o = WAVE / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
     return v;
 }
 
-__global__ __launch_bounds__(BLOCK) void k_seg_total(const double *__restrict__ part, const int32_t *__restrict__ scp,
-                                                     int nseg, double *__restrict__ out)
+// sum over the workgroup's wavefronts of a per-wavefront value, in wave order
+__device__ double wave_values_sum(double v, double *sh)
 {
-    for (int s = threadIdx.x; s < nseg; s += BLOCK) out[s] = seg_total(part, scp, s);
+    if (threadIdx.x % WAVE == 0) sh[threadIdx.x / WAVE] = v;
+    __syncthreads();
+    double t = 0;
+    if (threadIdx.x == 0)
+        for (int k = 0; k < SEGW / WAVE; ++k) t += sh[k];
+    return t;
+}
+
+__global__ __launch_bounds__(SEGW) void k_seg_total(const double *__restrict__ part, const int32_t *__restrict__ scp,
+                                                    int nseg, double *__restrict__ out)
+{
+    for (int s = int(threadIdx.x / WAVE); s < nseg; s += SEGW / WAVE) {
+        const double v = seg_total(part, scp, s);
+        if (threadIdx.x % WAVE == 0) out[s] = v;
+    }
 }
 
 // PCG start over chunk c's nodes: y = 0, r = b on free dofs, z = P r, p = z;
@@ -1124,31 +1147,34 @@ __global__ __launch_bounds__(BLOCK) void k_chunk_pcg_start(const double *__restr
 }
 
 // PCG start from the r.z and r.r partials: b2 = r.r; a problem with b = 0 is done
-__global__ __launch_bounds__(BLOCK) void k_seg_cg_start(const double *__restrict__ part_rz,
-                                                        const double *__restrict__ part_rr, const int32_t *__restrict__ scp,
-                                                        int nseg, SegState st)
+__global__ __launch_bounds__(SEGW) void k_seg_cg_start(const double *__restrict__ part_rz,
+                                                       const double *__restrict__ part_rr, const int32_t *__restrict__ scp,
+                                                       int nseg, SegState st)
 {
-    __shared__ double sh[BLOCK];
+    __shared__ double sh[SEGW / WAVE];
     double act = 0;
-    for (int s = threadIdx.x; s < nseg; s += BLOCK) {
-        const double v = seg_total(part_rr, scp, s);
-        st.rr[s] = v;
-        st.rz[s] = seg_total(part_rz, scp, s);
-        st.b2[s] = v;
-        st.done[s] = v > 0.0 ? 0.0 : 1.0;
-        st.iters[s] = 0.0;
-        act += st.done[s] == 0.0 ? 1.0 : 0.0;
+    for (int s = int(threadIdx.x / WAVE); s < nseg; s += SEGW / WAVE) {
+        const double v = seg_total(part_rr, scp, s), z = seg_total(part_rz, scp, s);
+        if (threadIdx.x % WAVE == 0) {
+            st.rr[s] = v;
+            st.rz[s] = z;
+            st.b2[s] = v;
+            st.done[s] = v > 0.0 ? 0.0 : 1.0;
+            st.iters[s] = 0.0;
+        }
+        act += v > 0.0 ? 1.0 : 0.0;
     }
-    const double a = block_sum(act, sh);
+    const double a = wave_values_sum(act, sh);
     if (threadIdx.x == 0) *st.active = a;
 }
 
 // alpha = r.z / p.Hp per problem; a non-positive curvature ends that problem
-__global__ __launch_bounds__(BLOCK) void k_seg_alpha(const double *__restrict__ part, const int32_t *__restrict__ scp,
-                                                     int nseg, SegState st)
+__global__ __launch_bounds__(SEGW) void k_seg_alpha(const double *__restrict__ part, const int32_t *__restrict__ scp,
+                                                    int nseg, SegState st)
 {
-    for (int s = threadIdx.x; s < nseg; s += BLOCK) {
+    for (int s = int(threadIdx.x / WAVE); s < nseg; s += SEGW / WAVE) {
         const double pq = seg_total(part, scp, s);
+        if (threadIdx.x % WAVE != 0) continue;
         if (st.done[s] != 0.0) {
             st.alpha[s] = 0.0;
         } else if (!(pq > 0.0)) {
@@ -1201,13 +1227,14 @@ __global__ __launch_bounds__(BLOCK) void k_chunk_pcg_update(double *__restrict__
 }
 
 // beta = r'.z' / r.z per problem; convergence at ||r|| <= tol ||b||
-__global__ __launch_bounds__(BLOCK) void k_seg_beta(const double *__restrict__ part_rz, const double *__restrict__ part_rr,
-                                                    const int32_t *__restrict__ scp, int nseg, double tol2, SegState st)
+__global__ __launch_bounds__(SEGW) void k_seg_beta(const double *__restrict__ part_rz, const double *__restrict__ part_rr,
+                                                   const int32_t *__restrict__ scp, int nseg, double tol2, SegState st)
 {
-    __shared__ double sh[BLOCK];
+    __shared__ double sh[SEGW / WAVE];
     double act = 0;
-    for (int s = threadIdx.x; s < nseg; s += BLOCK) {
+    for (int s = int(threadIdx.x / WAVE); s < nseg; s += SEGW / WAVE) {
         const double rn = seg_total(part_rr, scp, s), zn = seg_total(part_rz, scp, s);
+        if (threadIdx.x % WAVE != 0) continue;
         if (!isfinite(rn) || !isfinite(zn)) act += __builtin_nan(""); // reported to the host as a failed solve
         if (st.done[s] == 0.0) {
             st.iters[s] += 1.0;
@@ -1220,7 +1247,7 @@ __global__ __launch_bounds__(BLOCK) void k_seg_beta(const double *__restrict__ p
         }
         act += st.done[s] == 0.0 ? 1.0 : 0.0;
     }
-    const double a = block_sum(act, sh);
+    const double a = wave_values_sum(act, sh);
     if (threadIdx.x == 0) *st.active = a;
 }
 
@@ -1305,7 +1332,7 @@ void seg_cg_start(Handle &h, const Work &w, const uint8_t *mask, const SegChunks
 {
     k_chunk_pcg_start<<<ch.n, BLOCK, 0, h.stream>>>(w.d, mask, precond(h), ch.cb.as<int32_t>(), w.y, w.r, w.z, w.p,
                                                     ch.part.as<double>(), ch.part2.as<double>());
-    k_seg_cg_start<<<1, BLOCK, 0, h.stream>>>(ch.part.as<double>(), ch.part2.as<double>(), ch.scp.as<int32_t>(), nseg, st);
+    k_seg_cg_start<<<1, SEGW, 0, h.stream>>>(ch.part.as<double>(), ch.part2.as<double>(), ch.scp.as<int32_t>(), nseg, st);
     PF_LAUNCH_CHECK();
 }
 
@@ -1315,11 +1342,11 @@ void seg_cg_iteration(Handle &h, const Work &w, const double *x, const uint8_t *
 {
     hvp_any(h, x, w.p, w.q);
     k_chunk_dot<<<ch.n, BLOCK, 0, h.stream>>>(w.p, w.q, mask, ch.cb.as<int32_t>(), ch.part.as<double>());
-    k_seg_alpha<<<1, BLOCK, 0, h.stream>>>(ch.part.as<double>(), ch.scp.as<int32_t>(), nseg, st);
+    k_seg_alpha<<<1, SEGW, 0, h.stream>>>(ch.part.as<double>(), ch.scp.as<int32_t>(), nseg, st);
     k_chunk_pcg_update<<<ch.n, BLOCK, 0, h.stream>>>(w.y, w.r, w.z, w.p, w.q, mask, precond(h), ch.cb.as<int32_t>(),
                                                      ch.cs.as<int32_t>(), st.alpha, ch.part.as<double>(),
                                                      ch.part2.as<double>());
-    k_seg_beta<<<1, BLOCK, 0, h.stream>>>(ch.part.as<double>(), ch.part2.as<double>(), ch.scp.as<int32_t>(), nseg, tol2,
+    k_seg_beta<<<1, SEGW, 0, h.stream>>>(ch.part.as<double>(), ch.part2.as<double>(), ch.scp.as<int32_t>(), nseg, tol2,
                                           st);
     k_chunk_cg_direction<<<ch.n, BLOCK, 0, h.stream>>>(w.p, w.z, ch.cb.as<int32_t>(), ch.cs.as<int32_t>(), st.beta,
                                                        st.done);
@@ -1362,7 +1389,7 @@ void newton_solve_batch(Handle &h, double *x, const uint8_t *fixed, int nseg, co
     };
     auto seg_dot = [&](const double *a, const double *b, const uint8_t *m, double *out) {
         k_chunk_dot<<<ch.n, BLOCK, 0, h.stream>>>(a, b, m, ch.cb.as<int32_t>(), ch.part.as<double>());
-        k_seg_total<<<1, BLOCK, 0, h.stream>>>(ch.part.as<double>(), ch.scp.as<int32_t>(), nseg, out);
+        k_seg_total<<<1, SEGW, 0, h.stream>>>(ch.part.as<double>(), ch.scp.as<int32_t>(), nseg, out);
         PF_LAUNCH_CHECK();
     };
     auto seg_energy = [&](const double *xx, std::vector<double> &host) {
