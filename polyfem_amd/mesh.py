@@ -117,6 +117,36 @@ def kuhn_box(nx: int, ny: int, nz: int, order: int = 1, size=(1.0, 1.0, 1.0),
             v1, v2 = v2, v1
         ev[:, p, 0], ev[:, p, 1], ev[:, p, 2], ev[:, p, 3] = v0, v1, v2, v3
     elem_vertices = ev.reshape(-1, 4)
+    m = tet_mesh(vertices, elem_vertices, order=order, numbering=numbering, reorient=False)
+    m.grid = (nx, ny, nz)
+    m.size = tuple(float(s) for s in size)
+    return m
+
+
+def tet_mesh(vertices, tets, order: int = 1, numbering: str = "first_encounter", reorient: bool = True) -> TetMesh:
+    """A P1 or P2 mesh over the tets ``tets`` (E x 4 vertex ids) of ``vertices``.
+
+    Nodes are numbered by PolyFEM's first-encounter rule (or ``"natural"``:
+    vertex nodes keep their vertex ids, edge nodes follow in edge order); P2
+    edge nodes sit at the edge midpoints (P1 geometry).  With ``reorient``, a
+    negatively oriented tet has its vertices 1 and 2 swapped (the reference
+    expects positive orientation); a degenerate tet is an error.  Vertices no
+    tet uses get no node.
+    """
+    if order not in (1, 2):
+        raise ValueError("order must be 1 or 2")
+    vertices = np.ascontiguousarray(vertices, dtype=np.float64).reshape(-1, 3)
+    elem_vertices = np.array(tets, dtype=np.int64).reshape(-1, 4)
+    nv = vertices.shape[0]
+    if elem_vertices.size and (elem_vertices.min() < 0 or elem_vertices.max() >= nv):
+        raise ValueError("a tet references a vertex out of range")
+    if reorient:
+        X = vertices[elem_vertices]
+        det = np.einsum("ij,ij->i", np.cross(X[:, 1] - X[:, 0], X[:, 2] - X[:, 0]), X[:, 3] - X[:, 0])
+        if np.any(det == 0) or not np.all(np.isfinite(det)):
+            raise ValueError("degenerate tet")
+        neg = det < 0
+        elem_vertices[neg, 1], elem_vertices[neg, 2] = elem_vertices[neg, 2].copy(), elem_vertices[neg, 1].copy()
 
     if order == 1:
         prims = elem_vertices
@@ -135,17 +165,17 @@ def kuhn_box(nx: int, ny: int, nz: int, order: int = 1, size=(1.0, 1.0, 1.0),
 
     if numbering == "first_encounter":
         node_of_prim = _first_encounter(prims, n_prims)
+        used = node_of_prim >= 0
+        nodes = np.empty((int(used.sum()), 3), dtype=np.float64)
+        nodes[node_of_prim[used]] = prim_coords[used]
     elif numbering == "natural":
         node_of_prim = np.arange(n_prims, dtype=np.int64)
+        nodes = np.array(prim_coords, dtype=np.float64)
     else:
         raise ValueError(f"unknown numbering {numbering!r}")
     elem_nodes = node_of_prim[prims]
-    nodes = np.empty((n_prims, 3), dtype=np.float64)
-    nodes[node_of_prim] = prim_coords
-
     return TetMesh(order=order, vertices=vertices, elem_vertices=elem_vertices.astype(np.int32),
-                   nodes=nodes, elem_nodes=elem_nodes.astype(np.int32), grid=(nx, ny, nz),
-                   size=tuple(float(s) for s in size))
+                   nodes=nodes, elem_nodes=elem_nodes.astype(np.int32))
 
 
 def lame_from_young(E, nu):
