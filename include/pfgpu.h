@@ -46,7 +46,10 @@ typedef enum {
     PFGPU_ERR_OUT_OF_MEMORY = 5
 } pfgpu_status;
 
-typedef enum { PFGPU_NEOHOOKEAN = 0, PFGPU_LINEAR_ELASTICITY = 1 } pfgpu_material_kind;
+/* SaintVenant: psi = lambda/2 tr(E)^2 + mu E:E with E = (F^T F - I)/2, the
+ * isotropic case of SaintVenantElasticity (SaintVenantElasticity.cpp:235-278);
+ * the plain path only */
+typedef enum { PFGPU_NEOHOOKEAN = 0, PFGPU_LINEAR_ELASTICITY = 1, PFGPU_SAINT_VENANT = 2 } pfgpu_material_kind;
 
 /* FP64: every kernel in double.  MIXED: element kernels in float; the energy
  * sum, gradient accumulation, norms and CG dot products in double; Hessian

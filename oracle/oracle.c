@@ -473,10 +473,136 @@ static void le_hessian(const orc_problem *p, int e, double *H)
     }
 }
 
+/* --------------------------------------------------------- SaintVenant */
+
+/* SaintVenantElasticity::compute_energy_aux (SaintVenantElasticity.cpp:235-278):
+ * strain = (grad u^T grad u + grad u + grad u^T) / 2 (:9-20), Voigt strain
+ * (E00, E11, E22, 2E12, 2E02, 2E01), stress = C strain with the isotropic C of
+ * ElasticityTensor::set_from_lambda_mu (MatParams.cpp:211-254), stress tensor
+ * [[s0 s5 s4] [s5 s1 s3] [s4 s3 s2]], psi = tr(stress strain) / 2. */
+static void sv_strain(const double *D, double *Es)
+{
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) {
+            double s = D[r * 3 + c] + D[c * 3 + r];
+            for (int k = 0; k < 3; ++k) s += D[k * 3 + r] * D[k * 3 + c];
+            Es[r * 3 + c] = 0.5 * s;
+        }
+}
+
+static void sv_stress_voigt(const double *Es, double lambda, double mu, double *S)
+{
+    const double eps[6] = {Es[0], Es[4], Es[8], 2 * Es[5], 2 * Es[2], 2 * Es[1]};
+    double C[36] = {0};
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) C[i * 6 + j] = lambda + (i == j ? 2 * mu : 0.0);
+    for (int i = 3; i < 6; ++i) C[i * 6 + i] = mu;
+    double s[6];
+    for (int i = 0; i < 6; ++i) {
+        s[i] = 0;
+        for (int j = 0; j < 6; ++j) s[i] += C[i * 6 + j] * eps[j];
+    }
+    const double T[9] = {s[0], s[5], s[4], s[5], s[1], s[3], s[4], s[3], s[2]};
+    memcpy(S, T, sizeof(T));
+}
+
+static double sv_energy(const orc_problem *p, int e, const double *x)
+{
+    double U[30], D[9], Es[9], S[9];
+    local_disp(p, e, x, U);
+    double energy = 0;
+    for (int q = 0; q < p->n_quad; ++q) {
+        le_disp_grad(p, e, q, U, D);
+        sv_strain(D, Es);
+        sv_stress_voigt(Es, p->lambda[e], p->mu[e], S);
+        double tr = 0; /* tr(S Es) */
+        for (int r = 0; r < 3; ++r)
+            for (int k = 0; k < 3; ++k) tr += S[r * 3 + k] * Es[k * 3 + r];
+        energy += tr * da_of(p, e, q);
+    }
+    return 0.5 * energy;
+}
+
+/* The reference differentiates the energy by forward autodiff
+ * (SaintVenantElasticity.cpp:89-131); restated: dpsi/dF = F S (S the stress
+ * tensor above), contracted with grad_t_m. */
+static void sv_gradient(const orc_problem *p, int e, const double *x, double *out)
+{
+    const int n = p->n_local;
+    double U[30], D[9], G[30], Es[9], S[9];
+    local_disp(p, e, x, U);
+    memset(out, 0, sizeof(double) * 3 * n);
+    for (int q = 0; q < p->n_quad; ++q) {
+        le_disp_grad(p, e, q, U, D);
+        phys_grad(p, e, q, G);
+        sv_strain(D, Es);
+        sv_stress_voigt(Es, p->lambda[e], p->mu[e], S);
+        double P[9];
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c) {
+                double s = S[r * 3 + c];
+                for (int k = 0; k < 3; ++k) s += D[r * 3 + k] * S[k * 3 + c];
+                P[r * 3 + c] = s;
+            }
+        const double da = da_of(p, e, q);
+        for (int a = 0; a < n; ++a)
+            for (int d = 0; d < 3; ++d) {
+                double s = 0;
+                for (int c = 0; c < 3; ++c) s += P[d * 3 + c] * G[a * 3 + c];
+                out[a * 3 + d] += s * da;
+            }
+    }
+}
+
+/* Second derivative of the energy above: d2psi/dF_jc dF_kl =
+ * delta_jk S_cl + lambda F_jc F_kl + mu delta_cl (F F^T)_jk + mu F_jl F_kc,
+ * assembled as D^T A D with D(c*3+j, a*3+j) = grad_t_m(a,c). */
+static void sv_hessian(const orc_problem *p, int e, const double *x, double *H)
+{
+    const int n = p->n_local, N = 3 * n;
+    double U[30], D[9], G[30], Es[9], S[9], F[9], FF[9];
+    local_disp(p, e, x, U);
+    const double lambda = p->lambda[e], mu = p->mu[e];
+    memset(H, 0, sizeof(double) * N * N);
+    for (int q = 0; q < p->n_quad; ++q) {
+        le_disp_grad(p, e, q, U, D);
+        phys_grad(p, e, q, G);
+        sv_strain(D, Es);
+        sv_stress_voigt(Es, lambda, mu, S);
+        for (int i = 0; i < 9; ++i) F[i] = D[i] + (i % 4 == 0 ? 1.0 : 0.0);
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c) FF[r * 3 + c] = F[r * 3 + 0] * F[c * 3 + 0] + F[r * 3 + 1] * F[c * 3 + 1] + F[r * 3 + 2] * F[c * 3 + 2];
+        double A[81];
+        for (int c = 0; c < 3; ++c)
+            for (int j = 0; j < 3; ++j)
+                for (int l = 0; l < 3; ++l)
+                    for (int k = 0; k < 3; ++k)
+                        A[(c * 3 + j) * 9 + l * 3 + k] = (j == k ? S[c * 3 + l] : 0.0) + lambda * F[j * 3 + c] * F[k * 3 + l] +
+                                                         (c == l ? mu * FF[j * 3 + k] : 0.0) + mu * F[j * 3 + l] * F[k * 3 + c];
+        double T[9 * 30];
+        for (int i = 0; i < 9; ++i)
+            for (int a = 0; a < n; ++a)
+                for (int j = 0; j < 3; ++j) {
+                    double s = 0;
+                    for (int c = 0; c < 3; ++c) s += A[i * 9 + c * 3 + j] * G[a * 3 + c];
+                    T[i * N + a * 3 + j] = s;
+                }
+        const double da = da_of(p, e, q);
+        for (int a = 0; a < n; ++a)
+            for (int j = 0; j < 3; ++j)
+                for (int k = 0; k < N; ++k) {
+                    double s = 0;
+                    for (int c = 0; c < 3; ++c) s += G[a * 3 + c] * T[(c * 3 + j) * N + k];
+                    H[(a * 3 + j) * N + k] += s * da;
+                }
+    }
+}
+
 /* ------------------------------------------------------ public element API */
 
 double orc_element_energy(const orc_problem *p, int e, const double *x)
 {
+    if (p->material == ORC_SAINT_VENANT) return sv_energy(p, e, x);
     return p->material == ORC_NEOHOOKEAN ? nh_energy(p, e, x) : le_energy(p, e, x);
 }
 
@@ -484,6 +610,8 @@ void orc_element_gradient(const orc_problem *p, int e, const double *x, double *
 {
     if (p->material == ORC_NEOHOOKEAN)
         nh_gradient(p, e, x, g);
+    else if (p->material == ORC_SAINT_VENANT)
+        sv_gradient(p, e, x, g);
     else
         le_gradient(p, e, x, g);
 }
@@ -492,6 +620,8 @@ void orc_element_hessian(const orc_problem *p, int e, const double *x, double *H
 {
     if (p->material == ORC_NEOHOOKEAN)
         nh_hessian(p, e, x, H);
+    else if (p->material == ORC_SAINT_VENANT)
+        sv_hessian(p, e, x, H);
     else
         le_hessian(p, e, H);
 }
@@ -663,6 +793,20 @@ void orc_material_vjp(const orc_problem *p, const double *x, const double *adjoi
                     dmu[i] = F[i] - FmT[i];
                     dla[i] = ld * FmT[i];
                 }
+            } else if (p->material == ORC_SAINT_VENANT) {
+                /* no counterpart in the reference: the derivatives of P = F S
+                 * in lambda and mu, dP/dlambda = tr(E) F, dP/dmu = 2 F E */
+                double Es[9], F[9];
+                sv_strain(gu, Es);
+                for (int i = 0; i < 9; ++i) F[i] = gu[i] + (i % 4 == 0 ? 1.0 : 0.0);
+                const double tr = Es[0] + Es[4] + Es[8];
+                for (int r = 0; r < 3; ++r)
+                    for (int c = 0; c < 3; ++c) {
+                        double fe = 0;
+                        for (int k = 0; k < 3; ++k) fe += F[r * 3 + k] * Es[k * 3 + c];
+                        dmu[r * 3 + c] = 2 * fe;
+                        dla[r * 3 + c] = tr * F[r * 3 + c];
+                    }
             } else {
                 const double tr = gu[0] + gu[4] + gu[8];
                 for (int r = 0; r < 3; ++r)
@@ -713,6 +857,31 @@ static void stress_grad_mul(const orc_problem *p, int e, const double *gu, const
                 t2[r * 3 + c] = s;
             }
         for (int i = 0; i < 9; ++i) res[i] = mu * mat[i] + t2[i] * (mu - lambda * ld) + lambda * fm * FmT[i];
+    } else if (p->material == ORC_SAINT_VENANT) {
+        /* no counterpart in the reference: P = F S and its derivative along
+         * mat, mat S + F (lambda tr(dE) I + 2 mu dE), dE = (mat^T F + F^T mat) / 2 */
+        double Es[9], S[9], F[9], dE[9];
+        sv_strain(gu, Es);
+        sv_stress_voigt(Es, lambda, mu, S);
+        for (int i = 0; i < 9; ++i) F[i] = gu[i] + (i % 4 == 0 ? 1.0 : 0.0);
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c) {
+                double s = 0;
+                for (int k = 0; k < 3; ++k) s += mat[k * 3 + r] * F[k * 3 + c] + F[k * 3 + r] * mat[k * 3 + c];
+                dE[r * 3 + c] = 0.5 * s;
+            }
+        const double trd = dE[0] + dE[4] + dE[8];
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c) {
+                double ps = 0, ms = 0, fd = 0;
+                for (int k = 0; k < 3; ++k) {
+                    ps += F[r * 3 + k] * S[k * 3 + c];
+                    ms += mat[r * 3 + k] * S[k * 3 + c];
+                    fd += F[r * 3 + k] * (2 * mu * dE[k * 3 + c] + (k == c ? lambda * trd : 0.0));
+                }
+                stress[r * 3 + c] = ps;
+                res[r * 3 + c] = ms + fd;
+            }
     } else {
         const double tr = gu[0] + gu[4] + gu[8], tm = mat[0] + mat[4] + mat[8];
         for (int r = 0; r < 3; ++r)

@@ -20,7 +20,7 @@
 extern "C" {
 #endif
 
-enum { ORC_NEOHOOKEAN = 0, ORC_LINEAR_ELASTICITY = 1 };
+enum { ORC_NEOHOOKEAN = 0, ORC_LINEAR_ELASTICITY = 1, ORC_SAINT_VENANT = 2 };
 
 typedef struct orc_problem orc_problem;
 

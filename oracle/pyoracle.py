@@ -15,6 +15,7 @@ LIB_PATH = os.path.join(_HERE, "build", "liboracle.so")
 
 NEOHOOKEAN = 0
 LINEAR_ELASTICITY = 1
+SAINT_VENANT = 2
 
 _lib = None
 

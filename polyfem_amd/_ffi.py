@@ -13,7 +13,8 @@ LIB_PATH = os.environ.get("PFGPU_LIB", os.path.join(HERE, "libpfgpu.so"))
 
 PFGPU_OK = 0
 STATUS_NAMES = {0: "OK", 1: "INVALID_ARGUMENT", 2: "HIP", 3: "INVALID_STATE", 4: "UNSUPPORTED", 5: "OUT_OF_MEMORY"}
-NEOHOOKEAN, LINEAR_ELASTICITY = 0, 1
+OK, ERR_INVALID_ARGUMENT, ERR_HIP, ERR_INVALID_STATE, ERR_UNSUPPORTED, ERR_OUT_OF_MEMORY = range(6)
+NEOHOOKEAN, LINEAR_ELASTICITY, SAINT_VENANT = 0, 1, 2
 FP64, MIXED = 0, 1
 PATH_FUSED, PATH_PLAIN = 0, 1
 

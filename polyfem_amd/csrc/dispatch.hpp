@@ -16,6 +16,8 @@ template <typename F> void dispatch(const Handle &h, F &&f)
     auto mat = [&]<typename T, int O, int Q>() {
         if (h.material == MAT_NH)
             f.template operator()<T, O, Q, MAT_NH>();
+        else if (h.material == MAT_SV)
+            f.template operator()<T, O, Q, MAT_SV>();
         else
             f.template operator()<T, O, Q, MAT_LE>();
     };

@@ -19,7 +19,7 @@
 
 namespace pfgpu {
 
-enum { MAT_NH = PFGPU_NEOHOOKEAN, MAT_LE = PFGPU_LINEAR_ELASTICITY };
+enum { MAT_NH = PFGPU_NEOHOOKEAN, MAT_LE = PFGPU_LINEAR_ELASTICITY, MAT_SV = PFGPU_SAINT_VENANT };
 
 template <int ORDER> struct NLocal { static constexpr int value = ORDER == 1 ? 4 : 10; };
 
@@ -247,6 +247,38 @@ __device__ __forceinline__ void contract_add(int q, const T *S, const T *jit, T 
 
 // ---------------------------------------------------------- constitutive
 
+// SaintVenant kinematics from H = grad u: F = I + H, the Green strain
+// E = (H + H^T + H^T H) / 2 (no cancellation for small H) and the second
+// Piola-Kirchhoff stress S = lambda tr(E) I + 2 mu E
+template <typename T> struct SvKin {
+    T F[9], E[9], trE;
+};
+template <typename T> __device__ __forceinline__ void sv_kinematics(const T *H, SvKin<T> &s)
+{
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const T hh = H[0 * 3 + r] * H[0 * 3 + c] + H[1 * 3 + r] * H[1 * 3 + c] + H[2 * 3 + r] * H[2 * 3 + c];
+            s.E[r * 3 + c] = T(0.5) * (H[r * 3 + c] + H[c * 3 + r] + hh);
+            s.F[r * 3 + c] = H[r * 3 + c] + (r == c ? T(1) : T(0));
+        }
+    s.trE = s.E[0] + s.E[4] + s.E[8];
+}
+template <typename T> __device__ __forceinline__ void sv_stress(const SvKin<T> &s, T lam, T mu, T *S)
+{
+#pragma unroll
+    for (int i = 0; i < 9; ++i) S[i] = T(2) * mu * s.E[i] + (i % 4 == 0 ? lam * s.trE : T(0));
+}
+// out = A B (3x3)
+template <typename T> __device__ __forceinline__ void mul3(const T *A, const T *B, T *out)
+{
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) out[r * 3 + c] = A[r * 3 + 0] * B[0 * 3 + c] + A[r * 3 + 1] * B[1 * 3 + c] + A[r * 3 + 2] * B[2 * 3 + c];
+}
+
 // energy density (NeoHookeanElasticity.cpp:384; LinearElasticity.cpp:129)
 template <typename T, int MAT> __device__ __forceinline__ T energy_density(const Kin<T> &k, T lam, T mu)
 {
@@ -258,6 +290,14 @@ template <typename T, int MAT> __device__ __forceinline__ T energy_density(const
 #pragma unroll
         for (int i = 0; i < 9; ++i) hh += H[i] * H[i];
         return mu * (xmlog1p(k.delta) - k.s2 - k.dH) + T(0.5) * mu * hh + T(0.5) * lam * k.lnJ * k.lnJ;
+    } else if (MAT == MAT_SV) {
+        // (SaintVenantElasticity.cpp:235-278: 1/2 sigma : E with sigma = C : E, C isotropic)
+        SvKin<T> s;
+        sv_kinematics(H, s);
+        T ee = T(0);
+#pragma unroll
+        for (int i = 0; i < 9; ++i) ee += s.E[i] * s.E[i];
+        return mu * ee + T(0.5) * lam * s.trE * s.trE;
     } else {
         T ee = T(0);
 #pragma unroll
@@ -280,6 +320,13 @@ template <typename T, int MAT> __device__ __forceinline__ void pk1(const Kin<T> 
         const T ll = lam * k.lnJ;
 #pragma unroll
         for (int i = 0; i < 9; ++i) P[i] = mu * (H[i] + k.D[i]) + ll * k.FmT[i];
+    } else if (MAT == MAT_SV) {
+        // P = F S
+        SvKin<T> s;
+        sv_kinematics(H, s);
+        T S[9];
+        sv_stress(s, lam, mu, S);
+        mul3(s.F, S, P);
     } else {
         const T tr = lam * (H[0] + H[4] + H[8]);
 #pragma unroll
@@ -312,6 +359,26 @@ __device__ __forceinline__ void dstress(const Kin<T> &k, T lam, T mu, const T *W
                 const T t2 = t1[r * 3 + 0] * A[0 * 3 + c] + t1[r * 3 + 1] * A[1 * 3 + c] + t1[r * 3 + 2] * A[2 * 3 + c];
                 out[r * 3 + c] = mu * W[r * 3 + c] + cc * t2 + lf * A[r * 3 + c];
             }
+    } else if (MAT == MAT_SV) {
+        // dP = W S + F dS, dS = lambda tr(dE) I + 2 mu dE, dE = (W^T F + F^T W) / 2
+        SvKin<T> s;
+        sv_kinematics(k.H, s);
+        T S[9], FtW[9], dS[9], WS[9], FdS[9];
+        sv_stress(s, lam, mu, S);
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+                FtW[r * 3 + c] = s.F[0 * 3 + r] * W[0 * 3 + c] + s.F[1 * 3 + r] * W[1 * 3 + c] + s.F[2 * 3 + r] * W[2 * 3 + c];
+        const T trd = FtW[0] + FtW[4] + FtW[8];
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) dS[r * 3 + c] = mu * (FtW[r * 3 + c] + FtW[c * 3 + r]) + (r == c ? lam * trd : T(0));
+        mul3(W, S, WS);
+        mul3(s.F, dS, FdS);
+#pragma unroll
+        for (int i = 0; i < 9; ++i) out[i] = WS[i] + FdS[i];
     } else {
         const T tr = lam * (W[0] + W[4] + W[8]);
 #pragma unroll
@@ -319,6 +386,36 @@ __device__ __forceinline__ void dstress(const Kin<T> &k, T lam, T mu, const T *W
 #pragma unroll
             for (int c = 0; c < 3; ++c) out[r * 3 + c] = mu * (W[r * 3 + c] + W[c * 3 + r]) + (r == c ? tr : T(0));
     }
+}
+
+// SaintVenant local Hessian (derived from dP = W S + F dS): with k_a = F G_a,
+//   H_ab = sum_q da [ (G_a^T S G_b) I + mu (G_a . G_b) F F^T + lambda k_a k_b^T + mu k_b k_a^T ].
+// The last two terms have the NeoHookean form (hess_qp); sv_extra gives the
+// per-point data of the first two: the S-weighted reference metric
+// da jit S jit^T and mu da F F^T (symmetric, entries 00 11 22 01 02 12).
+template <typename T>
+__device__ __forceinline__ void sv_extra(const T *H, const T *jit, T lam, T mu, T da, T *sg6, T *ff6)
+{
+    SvKin<T> s;
+    sv_kinematics(H, s);
+    T S[9], JS[9];
+    sv_stress(s, lam, mu, S);
+    // JS[k][j] = sum_i jit[k][i] S[i][j]; sg[k][l] = sum_j JS[k][j] jit[l][j]
+#pragma unroll
+    for (int kk = 0; kk < 3; ++kk)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) JS[kk * 3 + j] = jit[kk * 3 + 0] * S[0 * 3 + j] + jit[kk * 3 + 1] * S[1 * 3 + j] + jit[kk * 3 + 2] * S[2 * 3 + j];
+    auto sg = [&](int kk, int l) { return da * (JS[kk * 3 + 0] * jit[l * 3 + 0] + JS[kk * 3 + 1] * jit[l * 3 + 1] + JS[kk * 3 + 2] * jit[l * 3 + 2]); };
+    auto ff = [&](int m, int n) { return mu * da * (s.F[m * 3 + 0] * s.F[n * 3 + 0] + s.F[m * 3 + 1] * s.F[n * 3 + 1] + s.F[m * 3 + 2] * s.F[n * 3 + 2]); };
+    sg6[0] = sg(0, 0), sg6[1] = sg(1, 1), sg6[2] = sg(2, 2), sg6[3] = sg(0, 1), sg6[4] = sg(0, 2), sg6[5] = sg(1, 2);
+    ff6[0] = ff(0, 0), ff6[1] = ff(1, 1), ff6[2] = ff(2, 2), ff6[3] = ff(0, 1), ff6[4] = ff(0, 2), ff6[5] = ff(1, 2);
+}
+
+// u^T A v for a symmetric A in the 6-entry form
+template <typename T> __device__ __forceinline__ T sym_form(const T *u, const T *a6, const T *v)
+{
+    return u[0] * (a6[0] * v[0] + a6[3] * v[1] + a6[4] * v[2]) + u[1] * (a6[3] * v[0] + a6[1] * v[1] + a6[5] * v[2]) +
+           u[2] * (a6[4] * v[0] + a6[5] * v[1] + a6[2] * v[2]);
 }
 
 // ------------------------------------------------------------- Hessian data
@@ -340,6 +437,16 @@ __device__ __forceinline__ void hess_qp(const Kin<T> &k, const T *jit, T lam, T 
                 M[m * 3 + kk] = k.FmT[m * 3 + 0] * jit[kk * 3 + 0] + k.FmT[m * 3 + 1] * jit[kk * 3 + 1] + k.FmT[m * 3 + 2] * jit[kk * 3 + 2];
         cl = da * lam;
         cc = da * (mu - lam * k.lnJ);
+    } else if (MAT == MAT_SV) {
+        // k_a = F G_a; the two further SaintVenant terms are added by the caller
+        const T F[9] = {T(1) + k.H[0], k.H[1], k.H[2], k.H[3], T(1) + k.H[4], k.H[5], k.H[6], k.H[7], T(1) + k.H[8]};
+#pragma unroll
+        for (int m = 0; m < 3; ++m)
+#pragma unroll
+            for (int kk = 0; kk < 3; ++kk)
+                M[m * 3 + kk] = F[m * 3 + 0] * jit[kk * 3 + 0] + F[m * 3 + 1] * jit[kk * 3 + 1] + F[m * 3 + 2] * jit[kk * 3 + 2];
+        cl = da * lam;
+        cc = da * mu;
     } else {
 #pragma unroll
         for (int m = 0; m < 3; ++m)

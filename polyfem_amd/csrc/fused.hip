@@ -527,13 +527,16 @@ void fused_energy_gradient_hessian(Handle &h, const double *x, double *energy, d
             plain_energy(h, x, energy, nullptr);
         return;
     }
+    if (h.material == MAT_SV) fail(PFGPU_ERR_UNSUPPORTED, "the fused path covers NeoHookean and linear elasticity");
     ensure_fused_tables(h);
     double *gv = g;
     if (!gv) {
         h.loc.reserve(sizeof(double) * size_t(h.n_dofs));
         gv = h.loc.as<double>();
     }
-    dispatch(h, [&]<typename T, int O, int Q, int M>() { launch_egh<T, O, Q, M>(h, x, values, gv, energy); });
+    dispatch(h, [&]<typename T, int O, int Q, int M>() {
+        if constexpr (M != MAT_SV) launch_egh<T, O, Q, M>(h, x, values, gv, energy);
+    });
 }
 
 void fused_hvp(Handle &h, const double *x, const double *v, double *y) { plain_hvp(h, x, v, y); }

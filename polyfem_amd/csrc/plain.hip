@@ -164,20 +164,21 @@ __global__ __launch_bounds__(HB) void k_hessian(DevMesh<T> m, const double *__re
     if (tile * 64 >= m.n_elements) return; // whole wavefront past the end (uniform)
     Elem<T, ORDER, NQ> el;
     load_geometry(m, ee, el);
-    if (MAT == MAT_NH) gather(m, ee, x, el.U);
+    if (MAT != MAT_LE) gather(m, ee, x, el.U);
     const T w = el.det * T(quad_weight(NQ));
     // per point: M = F^{-T} J^{-T} (k_a = M RG(q,a)) and the two curvature
     // weights; the k vectors are formed per block row and column to keep the
-    // register footprint small
-    T Mq[NQ][9], cl[NQ], cc[NQ];
+    // register footprint small.  SaintVenant adds the S-weighted metric and
+    // mu F F^T per point.
+    constexpr int NX = MAT == MAT_SV ? NQ : 1;
+    T Mq[NQ][9], cl[NQ], cc[NQ], sg6[NX][6], ff6[NX][6];
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
         Kin<T> k;
-        if (MAT == MAT_NH) {
-            field_grad<T, ORDER, NQ>(q, el.U, el.jit, k.H);
-            kinematics(k);
-        }
+        if (MAT != MAT_LE) field_grad<T, ORDER, NQ>(q, el.U, el.jit, k.H);
+        if (MAT == MAT_NH) kinematics(k);
         hess_qp<T, MAT>(k, el.jit, el.lam, el.mu, w, Mq[q], cl[q], cc[q]);
+        if constexpr (MAT == MAT_SV) sv_extra(k.H, el.jit, el.lam, el.mu, w, sg6[q], ff6[q]);
     }
     T g6[6];
     metric(el.jit, g6);
@@ -209,12 +210,27 @@ __global__ __launch_bounds__(HB) void k_hessian(DevMesh<T> m, const double *__re
                 }
             const int s = a * NB - a * (a - 1) / 2 + (b - a);
             const T *c = tb.rgg + (a * NB + b) * 6;
-            const T gd = cm * (g6[0] * c[0] + g6[1] * c[1] + g6[2] * c[2] + g6[3] * c[3] + g6[4] * c[4] + g6[5] * c[5]);
+            T gd = T(0), xf[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
+            if constexpr (MAT == MAT_SV) {
+                // sum_q [ (G_a^T S G_b) I + mu (G_a . G_b) F F^T ] da
+#pragma unroll
+                for (int q = 0; q < NQ; ++q) {
+                    const T *ra = tb.rg + (q * NB + a) * 3, *rb = tb.rg + (q * NB + b) * 3;
+                    gd += sym_form(ra, sg6[q], rb);
+                    const T gq = sym_form(ra, g6, rb);
+#pragma unroll
+                    for (int i = 0; i < 6; ++i) xf[i] += gq * ff6[q][i];
+                }
+            } else {
+                gd = cm * (g6[0] * c[0] + g6[1] * c[1] + g6[2] * c[2] + g6[3] * c[3] + g6[4] * c[4] + g6[5] * c[5]);
+            }
+            const int sym[9] = {0, 3, 4, 3, 1, 5, 4, 5, 2};
 #pragma unroll
             for (int mm = 0; mm < 3; ++mm)
 #pragma unroll
                 for (int nn = 0; nn < 3; ++nn) {
                     T v = mm == nn ? gd : T(0);
+                    if constexpr (MAT == MAT_SV) v += xf[sym[mm * 3 + nn]];
 #pragma unroll
                     for (int q = 0; q < NQ; ++q) v += cl[q] * Ka[q][mm] * Kb[q][nn] + cc[q] * Kb[q][mm] * Ka[q][nn];
                     st[lane * 9 + mm * 3 + nn] = v;
@@ -293,6 +309,14 @@ __global__ __launch_bounds__(BLOCK) void k_material_jvp(DevMesh<T> m, const doub
             // dP/dmu = F - F^{-T}, dP/dlambda = ln J F^{-T} (NeoHookeanElasticity.cpp:747-762)
 #pragma unroll
             for (int i = 0; i < 9; ++i) A[i] = sm * (k.H[i] + k.D[i]) + sl * k.lnJ * k.FmT[i];
+        } else if (MAT == MAT_SV) {
+            // dP/dmu = 2 F E, dP/dlambda = tr(E) F
+            SvKin<T> s;
+            sv_kinematics(k.H, s);
+            T FE[9];
+            mul3(s.F, s.E, FE);
+#pragma unroll
+            for (int i = 0; i < 9; ++i) A[i] = sm * T(2) * FE[i] + sl * s.trE * s.F[i];
         } else {
             // d sigma/d mu = grad u + grad u^T, d sigma/d lambda = tr(grad u) I (LinearElasticity.cpp:290-299)
             const T tr = k.H[0] + k.H[4] + k.H[8];
@@ -340,6 +364,17 @@ __global__ __launch_bounds__(BLOCK) void k_material_vjp(DevMesh<T> m, const doub
                 il += k.FmT[i] * Gp[i];
             }
             il *= k.lnJ;
+        } else if (MAT == MAT_SV) {
+            SvKin<T> s;
+            sv_kinematics(k.H, s);
+            T FE[9];
+            mul3(s.F, s.E, FE);
+#pragma unroll
+            for (int i = 0; i < 9; ++i) {
+                im += T(2) * FE[i] * Gp[i];
+                il += s.F[i] * Gp[i];
+            }
+            il *= s.trE;
         } else {
 #pragma unroll
             for (int r = 0; r < 3; ++r)

@@ -20,7 +20,10 @@ MIXED_TOL = {"energy": 1e-5, "gradient": 2e-5, "hessian": 2e-5, "hvp": 2e-5, "vj
 
 PATHS = ["fused", "plain"]
 CASES = [(1, "NeoHookean"), (2, "NeoHookean"), (1, "LinearElasticity"), (2, "LinearElasticity")]
-MAT = {"NeoHookean": po.NEOHOOKEAN, "LinearElasticity": po.LINEAR_ELASTICITY}
+# SaintVenant runs on the plain path only
+SV_CASES = [(1, "SaintVenant"), (2, "SaintVenant")]
+PATH_CASES = [(p, o, n) for p in PATHS for o, n in CASES] + [("plain", o, n) for o, n in SV_CASES]
+MAT = {"NeoHookean": po.NEOHOOKEAN, "LinearElasticity": po.LINEAR_ELASTICITY, "SaintVenant": po.SAINT_VENANT}
 
 
 def rel(a, b):
@@ -38,8 +41,7 @@ def setup_case(order, name, grid=(3, 2, 3), seed=0, precision="fp64", path="plai
     return m, x, a, o, lam, mu
 
 
-@pytest.mark.parametrize("path", PATHS)
-@pytest.mark.parametrize("order,name", CASES)
+@pytest.mark.parametrize("path,order,name", PATH_CASES)
 def test_fp64_parity(order, name, path):
     m, x, a, o, lam, mu = setup_case(order, name, path=path)
     print(f"case order={order} material={name} path={path} E={m.n_elements}", flush=True)
@@ -68,8 +70,7 @@ def test_fp64_parity(order, name, path):
     assert abs(lhs - rhs) <= 1e-10 * abs(rhs)
 
 
-@pytest.mark.parametrize("path", PATHS)
-@pytest.mark.parametrize("order,name", CASES)
+@pytest.mark.parametrize("path,order,name", PATH_CASES)
 def test_mixed_precision_parity(order, name, path):
     m, x, a, o, lam, mu = setup_case(order, name, precision="mixed", path=path, seed=1)
     print(f"case mixed order={order} material={name} path={path}", flush=True)
@@ -157,7 +158,7 @@ def test_fused_many_patches(order, precision):
     assert rel(a.hessian_vector_product(x, v), o.hessian_scipy(x) @ v) < (FP64_TOL if precision == "fp64" else MIXED_TOL["hvp"])
 
 
-@pytest.mark.parametrize("order,name", CASES)
+@pytest.mark.parametrize("order,name", CASES + SV_CASES)
 def test_energy_lame_gradient(order, name):
     """dE/d(lambda, mu) per element: the energy is linear in the Lame pair, so
     the derivative equals the per-element energy with unit (1, 0) / (0, 1)."""
@@ -192,3 +193,18 @@ def test_autotune_and_tunings_agree():
     e, g, vals = a.assemble_energy_gradient_hessian(x)
     _, _, hv = o.hessian(x)
     assert rel(vals, hv) < MIXED_TOL["hessian"] and rel(g, o.gradient(x)) < MIXED_TOL["gradient"]
+
+
+def test_saint_venant_is_plain_path_only():
+    """The fused kernel carries the NeoHookean/LE tangent only: choosing it for
+    SaintVenant is an error, and the autotuner keeps the plain path."""
+    import torch
+    from polyfem_amd import _ffi
+    m = kuhn_box(2, 2, 2, order=2)
+    a = NLAssembler("SaintVenant", m, young=1e6, poisson=0.35, precision="mixed")
+    with pytest.raises(PfgpuError) as err:
+        a.set_path("fused")
+    assert err.value.status == _ffi.ERR_UNSUPPORTED
+    x = torch.zeros(m.n_dofs, dtype=torch.float64, device="cuda")
+    rep = a.autotune_dev(x.data_ptr())
+    assert rep["path"] == "plain" and rep["ms_fused"] is None

@@ -201,3 +201,21 @@ def test_assembled_and_matrix_free_operators_agree(order, precision):
         H = po.Oracle(m, lam, mu).hessian_scipy(x).tocsr()[prob.free][:, prob.free]
         res = H @ full[("assembled", "block_jacobi")][prob.free] - b[prob.free]
         assert np.linalg.norm(res) <= 1e-9 * np.linalg.norm(b[prob.free])
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_saint_venant_newton_converges(order):
+    """Newton on SaintVenant (plain path): quadratic convergence to the
+    minimiser; the final gradient agrees with the oracle's."""
+    m, prob = clamped_box(order, (2, 2, 3), pull=0.1)
+    lam, mu = random_lame(m.n_elements, seed=BASE_SEED + 90)
+    a = NLAssembler("SaintVenant", m, lame=(lam, mu), precision="fp64")
+    print(f"SaintVenant Newton order={order}", flush=True)
+    x0 = np.zeros(m.n_dofs)
+    x0[prob.fixed] = prob.fixed_values
+    g0 = np.linalg.norm(a.assemble_gradient(x0)[prob.free])
+    x, rep = NewtonSolver(max_iterations=30, grad_norm=1e-9 * g0, cg_max_iterations=5000,
+                          cg_tolerance=1e-12).minimize(a, np.zeros(m.n_dofs), prob)
+    assert rep.converged and rep.iterations <= 10, rep
+    go = po.Oracle(m, lam, mu, material=po.SAINT_VENANT).gradient(x)
+    assert np.linalg.norm(go[prob.free]) <= 1e-8 * g0

@@ -132,7 +132,8 @@ def test_geometry_volume_and_linear_reproduction():
 
 def _energy_cs(mesh, e, x, lam, mu, order, material):
     """Energy of one element written from NeoHookeanElasticity.cpp:337-386 /
-    LinearElasticity.cpp:105-134, analytic in x so complex-step differentiates it."""
+    LinearElasticity.cpp:105-134 / SaintVenantElasticity.cpp:235-278 (isotropic
+    C), analytic in x so complex-step differentiates it."""
     pts, w = po.quadrature(1 if order == 1 else 2)
     V = mesh.vertices[mesh.elem_vertices[e]]
     T = V[1:] - V[0]
@@ -150,6 +151,9 @@ def _energy_cs(mesh, e, x, lam, mu, order, material):
                  + F[0, 2] * (F[1, 0] * F[2, 1] - F[1, 1] * F[2, 0]))
             lj = np.log(J)
             val = mu / 2 * (np.sum(F * F) - 3 - 2 * lj) + lam / 2 * lj * lj
+        elif material == po.SAINT_VENANT:
+            Eg = (Du.T @ Du + Du + Du.T) / 2
+            val = mu * np.sum(Eg * Eg) + lam / 2 * np.trace(Eg) ** 2
         else:
             eps = (Du + Du.T) / 2
             val = mu * np.sum(eps * eps) + lam / 2 * np.trace(eps) ** 2
@@ -177,7 +181,7 @@ def _cs_gradient(mesh, e, x, lam, mu, order, material):
 
 
 @pytest.mark.parametrize("order", [1, 2])
-@pytest.mark.parametrize("material", [po.NEOHOOKEAN, po.LINEAR_ELASTICITY])
+@pytest.mark.parametrize("material", [po.NEOHOOKEAN, po.LINEAR_ELASTICITY, po.SAINT_VENANT])
 def test_hand_derivatives_vs_complex_step(order, material):
     """Margin 1e-12 as generic_elastic_assembler (test_assembler.cpp:191,206,222)."""
     m = kuhn_box(1, 1, 1, order=order)
@@ -215,6 +219,9 @@ def _cs_hessian(mesh, e, x, order, material, oracle):
                 cof = np.array([np.cross(F[:, 1], F[:, 2]), np.cross(F[:, 2], F[:, 0]), np.cross(F[:, 0], F[:, 1])]).T
                 J = np.sum(F[:, 0] * cof[:, 0])
                 P = MU * F - MU / J * cof + LAM * np.log(J) / J * cof
+            elif material == po.SAINT_VENANT:
+                Eg = (Du.T @ Du + Du + Du.T) / 2
+                P = (np.eye(3) + Du) @ (LAM * np.trace(Eg) * np.eye(3) + 2 * MU * Eg)
             else:
                 P = MU * (Du + Du.T) + LAM * np.trace(Du) * np.eye(3)
             g += det * w[q] * (G @ P.T)
@@ -229,7 +236,7 @@ def _cs_hessian(mesh, e, x, order, material, oracle):
 
 
 @pytest.mark.parametrize("order", [1, 2])
-@pytest.mark.parametrize("material", [po.NEOHOOKEAN, po.LINEAR_ELASTICITY])
+@pytest.mark.parametrize("material", [po.NEOHOOKEAN, po.LINEAR_ELASTICITY, po.SAINT_VENANT])
 def test_hessian_vs_complex_step(order, material):
     m = kuhn_box(1, 1, 1, order=order)
     o = po.Oracle(m, LAM, MU, material=material)
@@ -256,6 +263,35 @@ def test_nh_hessian_at_rest_equals_linear_stiffness(order):
 
 
 @pytest.mark.parametrize("order", [1, 2])
+def test_sv_hessian_at_rest_equals_linear_stiffness(order):
+    """SaintVenant linearises to linear elasticity: equal Hessians at u = 0."""
+    m = kuhn_box(2, 2, 2, order=order)
+    lam = np.linspace(1e3, 3e3, m.n_elements)
+    mu = np.linspace(5e2, 9e2, m.n_elements)
+    x0 = np.zeros(m.n_dofs)
+    Hs = po.Oracle(m, lam, mu, material=po.SAINT_VENANT).hessian_scipy(x0)
+    Hl = po.Oracle(m, lam, mu, material=po.LINEAR_ELASTICITY).hessian_scipy(x0)
+    assert abs(Hs - Hl).max() < 1e-8
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_sv_uniaxial_stretch_closed_form(order):
+    """u = (s - 1) x e_x on the unit cube: E = diag(e, 0, 0), e = (s^2 - 1)/2,
+    psi = (lambda/2 + mu) e^2; the gradient at interior nodes vanishes."""
+    m = kuhn_box(2, 2, 2, order=order)
+    o = po.Oracle(m, LAM, MU, material=po.SAINT_VENANT)
+    s = 1.3
+    x = np.zeros((m.n_nodes, 3))
+    x[:, 0] = (s - 1) * m.nodes[:, 0]
+    x = x.reshape(-1)
+    e = (s * s - 1) / 2
+    assert abs(o.energy(x) - (LAM / 2 + MU) * e * e) < 1e-12 * (LAM / 2 + MU) * e * e
+    inner = np.all((m.nodes > 1e-9) & (m.nodes < 1 - 1e-9), axis=1)
+    g = o.gradient(x).reshape(-1, 3)
+    assert np.abs(g[inner]).max() < 1e-9 * LAM
+
+
+@pytest.mark.parametrize("order", [1, 2])
 def test_linear_elasticity_is_quadratic(order):
     m = kuhn_box(2, 1, 2, order=order)
     o = po.Oracle(m, LAM, MU, material=po.LINEAR_ELASTICITY)
@@ -267,9 +303,10 @@ def test_linear_elasticity_is_quadratic(order):
 
 
 @pytest.mark.parametrize("order", [1, 2])
-def test_rigid_motion_and_frame_indifference(order):
+@pytest.mark.parametrize("material", [po.NEOHOOKEAN, po.SAINT_VENANT])
+def test_rigid_motion_and_frame_indifference(order, material):
     m = kuhn_box(2, 2, 1, order=order)
-    o = po.Oracle(m, LAM, MU)
+    o = po.Oracle(m, LAM, MU, material=material)
     th = 0.7
     R = np.array([[np.cos(th), -np.sin(th), 0], [np.sin(th), np.cos(th), 0], [0, 0, 1]])
     X = m.nodes
@@ -320,7 +357,7 @@ def test_pattern_counts_match_box_closed_form():
 # -------------------------------------------------------------- adjoint
 
 @pytest.mark.parametrize("order", [1, 2])
-@pytest.mark.parametrize("material", [po.NEOHOOKEAN, po.LINEAR_ELASTICITY])
+@pytest.mark.parametrize("material", [po.NEOHOOKEAN, po.LINEAR_ELASTICITY, po.SAINT_VENANT])
 def test_material_term_vs_fd(order, material):
     """<term, dtheta> = -p^T (dg/dtheta) dtheta (ElasticForceDerivative.cpp:123-124)."""
     m = kuhn_box(2, 2, 1, order=order)
@@ -341,7 +378,7 @@ def test_material_term_vs_fd(order, material):
 
 
 @pytest.mark.parametrize("order", [1, 2])
-@pytest.mark.parametrize("material", [po.NEOHOOKEAN, po.LINEAR_ELASTICITY])
+@pytest.mark.parametrize("material", [po.NEOHOOKEAN, po.LINEAR_ELASTICITY, po.SAINT_VENANT])
 def test_shape_term_vs_fd(order, material):
     """<term, dX> = + d(p^T g)/dX dX with the displacement dofs held fixed."""
     m = kuhn_box(2, 1, 2, order=order)
