@@ -99,12 +99,17 @@ void upload_lame(pfgpu_handle &h, const double *lambda, const double *mu)
 
 void upload_geometry(pfgpu_handle &h, const pfgpu_mesh_desc &m)
 {
+    // computed on the device; a degenerate element is located by the host
+    // computation, which raises with its index
+    for (int e = 0; e < m.n_elements; ++e)
+        for (int j = 0; j < 4; ++j) {
+            const int32_t v = m.elem_vertices[size_t(e) * 4 + j];
+            require(v >= 0 && v < m.n_vertices, "elem_vertices references a vertex out of range");
+        }
+    if (device_geometry(h, m.vertices)) return;
     std::vector<double> g;
     geometry_host(m, g);
-    if (h.precision == PFGPU_MIXED)
-        upload_into(h.geom, soa<double, float>(g.data(), m.n_elements, 10), h.stream);
-    else
-        upload_into(h.geom, soa<double, double>(g.data(), m.n_elements, 10), h.stream);
+    fail(PFGPU_ERR_INVALID_ARGUMENT, "an element has a non-finite geometry");
 }
 
 // staging of host arrays for the synchronous entry points
@@ -218,6 +223,7 @@ pfgpu_status pfgpu_create(const pfgpu_mesh_desc *mesh, const pfgpu_material_desc
         h->h_elem_vertices.assign(m.elem_vertices, m.elem_vertices + size_t(m.n_elements) * 4);
         h->pat = build_pattern(m.n_elements, h->n_local, m.elem_nodes, m.n_nodes, m.n_vertices, m.elem_vertices);
 
+        h->vconn = upload_soa<int32_t, int32_t>(m.elem_vertices, m.n_elements, 4, h->stream);
         upload_geometry(*h, m);
 
         if (material->lambda) {
@@ -231,7 +237,6 @@ pfgpu_status pfgpu_create(const pfgpu_mesh_desc *mesh, const pfgpu_material_desc
             upload_lame(*h, la.data(), mv.data());
         }
         h->conn = upload_soa<int32_t, int32_t>(m.elem_nodes, m.n_elements, h->n_local, h->stream);
-        h->vconn = upload_soa<int32_t, int32_t>(m.elem_vertices, m.n_elements, 4, h->stream);
         h->node_ptr = upload(h->pat.node_ptr, h->stream);
         h->node_inc = upload(h->pat.node_inc, h->stream);
         h->vert_ptr = upload(h->pat.vert_ptr, h->stream);
@@ -400,7 +405,12 @@ pfgpu_status pfgpu_set_vertices(pfgpu_handle *h, const double *vertices)
         PF_HIP(hipStreamSynchronize(h->stream));
         pfgpu_mesh_desc m{h->order, 0, h->n_elements, h->n_nodes, h->h_elem_nodes.data(), h->n_vertices, vertices,
                           h->h_elem_vertices.data()};
-        upload_geometry(*h, m); // validates every element before anything changes
+        try {
+            upload_geometry(*h, m);
+        } catch (...) {
+            (void)device_geometry(*h, h->h_vertices.data()); // the previous geometry stays
+            throw;
+        }
         h->h_vertices.assign(vertices, vertices + n);
         if (h->has_mass) build_mass(*h, h->h_rho.empty() ? nullptr : h->h_rho.data(), h->rho_uniform);
         solver_release(*h);

@@ -38,6 +38,7 @@ struct pfgpu_handle {
     std::vector<int32_t> h_elem_vertices; // host copy, E x 4
     // lazily built device pattern
     pfgpu::DevBuf d_outer, d_inner;
+    pfgpu::DevBuf d_vertices; // vertex positions (FP64) of the device geometry
 
     // scratch
     pfgpu::DevBuf loc, hloc, partial, red_out, sum_part;
@@ -111,6 +112,9 @@ void fused_hvp(Handle &h, const double *x, const double *v, double *y);
 // ---- mass (mass.cpp) ---------------------------------------------------
 std::vector<double> reference_mass(int order);
 void build_mass(Handle &h, const double *rho, double rho_uniform);
+
+// ---- geometry (geometry.hip) -------------------------------------------
+bool device_geometry(Handle &h, const double *vertices);
 
 // ---- solver (solver.hip) ------------------------------------------------
 void mass_apply(Handle &h, const double *v, double *y);

@@ -116,3 +116,21 @@ def test_shape_gradient_matches_finite_differences(order):
         fd = (jp - jm) / (2 * h)
         lin = grad @ dV.ravel()
         assert abs(fd - lin) <= 1e-5 * abs(lin) + 1e-14, (fd, lin)
+
+
+def test_degenerate_vertices_are_rejected_and_leave_the_handle_unchanged():
+    from polyfem_amd import PfgpuError
+    m = kuhn_box(2, 2, 2, order=1)
+    lam, mu = random_lame(m.n_elements, seed=BASE_SEED + 69)
+    x = smooth_displacement(m.nodes, 0.02, 1e-3, BASE_SEED + 70)
+    print("degenerate vertex update", flush=True)
+    a = NLAssembler("NeoHookean", m, lame=(lam, mu), precision="fp64")
+    e0, g0 = a.assemble_energy(x), a.assemble_gradient(x)
+    V = m.vertices.copy()
+    ev = m.elem_vertices[0]
+    V[ev[3]] = V[ev[0]]  # collapses element 0 (and its neighbours through that vertex)
+    with pytest.raises(PfgpuError):
+        a.set_vertices(V)
+    assert a.assemble_energy(x) == e0 and np.array_equal(a.assemble_gradient(x), g0)
+    with pytest.raises(PfgpuError):
+        NLAssembler("NeoHookean", dataclasses.replace(m, vertices=V), lame=(lam, mu), precision="fp64")
