@@ -144,7 +144,9 @@ pfgpu_status pfgpu_hessian_nnz(const pfgpu_handle *h, int64_t *nnz);
 /* the CSC pattern of NLAssembler::assemble_hessian (Assembler.cpp:645-771) */
 pfgpu_status pfgpu_hessian_pattern(const pfgpu_handle *h, int32_t *outer, int32_t *inner);
 /* Hessian values in pattern order: double* for FP64, float* for MIXED.
- * project_to_psd must be 0 (ipc::project_to_psd is not provided). */
+ * project_to_psd != 0: every element's local Hessian is projected onto the
+ * positive semidefinite cone before the sum, as NLAssembler::assemble_hessian
+ * does with ipc::project_to_psd (eigenvalues <= 0 clamped to 0). */
 pfgpu_status pfgpu_hessian_values(pfgpu_handle *h, const double *x, int32_t project_to_psd, void *values);
 /* energy, gradient and Hessian values from one pass over the mesh; any
  * output pointer may be NULL */
@@ -253,7 +255,14 @@ pfgpu_status pfgpu_cg_solve_dev(pfgpu_handle *h, const double *x, const uint8_t 
  * Newton iteration and applied by a block-sparse product, or the matrix-free
  * element product; AUTO assembles for P1 and stays matrix-free for P2.
  * preconditioner: none, or the inverse 3x3 diagonal blocks (block Jacobi). */
-enum { PFGPU_SOLVER_HESSIAN_AUTO = 0, PFGPU_SOLVER_HESSIAN_ASSEMBLED = 1, PFGPU_SOLVER_HESSIAN_MATRIX_FREE = 2 };
+enum {
+    PFGPU_SOLVER_HESSIAN_AUTO = 0,
+    PFGPU_SOLVER_HESSIAN_ASSEMBLED = 1,
+    PFGPU_SOLVER_HESSIAN_MATRIX_FREE = 2,
+    /* assembled from element Hessians projected onto the PSD cone (the
+     * reference's project_to_psd solver option) */
+    PFGPU_SOLVER_HESSIAN_ASSEMBLED_PSD = 3
+};
 enum { PFGPU_PRECOND_NONE = 0, PFGPU_PRECOND_BLOCK_JACOBI = 1 };
 pfgpu_status pfgpu_set_solver_options(pfgpu_handle *h, int32_t hessian, int32_t preconditioner);
 

@@ -177,6 +177,26 @@ class Oracle:
         lib().orc_hessian(self.p, self._x(x), outer, inner, values)
         return outer, inner, values
 
+    def hessian_psd(self, x):
+        """(outer, inner, values) of the Hessian summed from element Hessians
+        projected onto the PSD cone, as NLAssembler::assemble_hessian with
+        project_to_psd (Assembler.cpp:645-771) calling ipc::project_to_psd of
+        ipc-toolkit@b40e9c0 (CLAMP): A = V diag(d) V^T, eigenvalues <= 0 set to
+        0, and A unchanged when its smallest eigenvalue is >= 0.  The
+        eigen-decomposition is numpy.linalg.eigh (LAPACK)."""
+        n = self.n_local
+        rows, cols, vals = [], [], []
+        for e in range(self.n_elements):
+            H = self.element_hessian(e, x)
+            d, V = np.linalg.eigh(H)
+            if d[0] < 0.0:
+                H = (V * np.where(d > 0.0, d, 0.0)) @ V.T
+            g = (3 * self.mesh.elem_nodes[e][:, None] + np.arange(3)).reshape(-1)
+            rows.append(np.repeat(g, 3 * n))
+            cols.append(np.tile(g, 3 * n))
+            vals.append(H.reshape(-1))
+        return triplets_to_csc(self.n_dofs, np.concatenate(rows), np.concatenate(cols), np.concatenate(vals))
+
     def hessian_scipy(self, x):
         import scipy.sparse as sp
         outer, inner, values = self.hessian(x)

@@ -70,7 +70,7 @@ class PfgpuError(RuntimeError):
         self.status = status
 
 
-SOLVER_HESSIAN_AUTO, SOLVER_HESSIAN_ASSEMBLED, SOLVER_HESSIAN_MATRIX_FREE = 0, 1, 2
+SOLVER_HESSIAN_AUTO, SOLVER_HESSIAN_ASSEMBLED, SOLVER_HESSIAN_MATRIX_FREE, SOLVER_HESSIAN_ASSEMBLED_PSD = 0, 1, 2, 3
 PRECOND_NONE, PRECOND_BLOCK_JACOBI = 0, 1
 
 vp = C.c_void_p

@@ -121,10 +121,11 @@ class NLAssembler:
 
     def set_solver_options(self, hessian: str = "auto", preconditioner: str = "block_jacobi"):
         """Linear algebra of the solves' CG: hessian "auto" (assembled for P1,
-        matrix-free for P2), "assembled" or "matrix_free"; preconditioner
+        matrix-free for P2), "assembled", "matrix_free" or "assembled_psd"
+        (element Hessians projected onto the PSD cone); preconditioner
         "block_jacobi" (inverse 3x3 diagonal blocks) or "none"."""
         hm = {"auto": _ffi.SOLVER_HESSIAN_AUTO, "assembled": _ffi.SOLVER_HESSIAN_ASSEMBLED,
-              "matrix_free": _ffi.SOLVER_HESSIAN_MATRIX_FREE}
+              "matrix_free": _ffi.SOLVER_HESSIAN_MATRIX_FREE, "assembled_psd": _ffi.SOLVER_HESSIAN_ASSEMBLED_PSD}
         pm = {"none": _ffi.PRECOND_NONE, "block_jacobi": _ffi.PRECOND_BLOCK_JACOBI}
         if hessian not in hm or preconditioner not in pm:
             raise ValueError("unknown solver option")

@@ -489,11 +489,13 @@ pfgpu_status pfgpu_hessian_values(pfgpu_handle *h, const double *x, int32_t proj
     return guard([&] {
         check_handle(h);
         require(x && values, "null argument");
-        if (project_to_psd) fail(PFGPU_ERR_UNSUPPORTED, "project_to_psd is not provided");
         const double *dx = stage(*h, h->h_x, x, h->n_dofs);
         const size_t bytes = h->value_size() * size_t(h->pat.nnz);
         h->h_vals.reserve(bytes);
-        egh_dev(*h, dx, nullptr, nullptr, h->h_vals.get());
+        if (project_to_psd)
+            plain_hessian(*h, dx, h->h_vals.get(), true); // the projection runs on the element tiles of the plain path
+        else
+            egh_dev(*h, dx, nullptr, nullptr, h->h_vals.get());
         check_finite(*h, h->h_vals.get(), h->pat.nnz, h->precision == PFGPU_MIXED, "Hessian");
         fetch(*h, values, h->h_vals.get(), bytes);
     });
@@ -814,7 +816,7 @@ pfgpu_status pfgpu_set_solver_options(pfgpu_handle *h, int32_t hessian, int32_t 
 {
     return guard([&] {
         check_handle(h);
-        require(hessian >= PFGPU_SOLVER_HESSIAN_AUTO && hessian <= PFGPU_SOLVER_HESSIAN_MATRIX_FREE, "unknown hessian mode");
+        require(hessian >= PFGPU_SOLVER_HESSIAN_AUTO && hessian <= PFGPU_SOLVER_HESSIAN_ASSEMBLED_PSD, "unknown hessian mode");
         require(preconditioner == PFGPU_PRECOND_NONE || preconditioner == PFGPU_PRECOND_BLOCK_JACOBI,
                 "unknown preconditioner");
         h->solver_hessian = hessian;

@@ -95,7 +95,9 @@ void sum_doubles(Handle &h, const double *in, int n, double *out);
 // ---- plain path (plain.hip) -------------------------------------------
 void plain_energy(Handle &h, const double *x, double *energy, double *per_element);
 void plain_gradient(Handle &h, const double *x, double *g, double *energy);
-void plain_hessian(Handle &h, const double *x, void *values);
+void plain_hessian(Handle &h, const double *x, void *values, bool project_to_psd = false);
+// every element's local Hessian in the block tiles projected onto the PSD cone (psd.hip)
+void psd_project_blocks(Handle &h, void *hloc);
 void plain_hvp(Handle &h, const double *x, const double *v, double *y);
 void plain_material_jvp(Handle &h, const double *x, const double *dl, const double *dm, double *y);
 void element_material_vjp(Handle &h, const double *x, const double *p, double *out);

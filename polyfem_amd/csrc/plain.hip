@@ -707,7 +707,7 @@ void ensure_block_chunks(Handle &h)
     h.blk_chunk_threads = h.tune_gather_threads;
 }
 
-void plain_hessian(Handle &h, const double *x, void *values)
+void plain_hessian(Handle &h, const double *x, void *values, bool project_to_psd)
 {
     ensure_plain_hessian_maps(h);
     ensure_block_chunks(h);
@@ -723,6 +723,7 @@ void plain_hessian(Handle &h, const double *x, void *values)
         else
             hess.template operator()<256>();
         PF_LAUNCH_CHECK();
+        if (project_to_psd) psd_project_blocks(h, h.hloc.get());
         auto gath = [&]<int GS>() {
             k_gather_blocks_seg<T, GS><<<xcd_grid(h.n_blk_chunks), GS, 0, h.stream>>>(
                 h.n_blk_chunks, h.blk_chunk.as<int32_t>(), h.brow_ptr.as<int32_t>(), h.block_row.as<int32_t>(),

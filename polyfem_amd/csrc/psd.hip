@@ -1,0 +1,189 @@
+// Projection of every element's local Hessian onto the positive semidefinite
+// cone: the reference's project_to_psd option (NLAssembler::assemble_hessian,
+// Assembler.cpp:645-771, calling ipc::project_to_psd of ipc-toolkit@b40e9c0
+// with its CLAMP method): with A = V diag(d) V^T, eigenvalues <= 0 become 0;
+// a matrix whose eigenvalues are all >= 0 is left as it is.
+//
+// One wavefront per element.  The symmetric 3n x 3n matrix (n = 4 or 10
+// nodes) is read from the element-block tiles k_hessian writes, diagonalised
+// in LDS by the cyclic Jacobi method with the round-robin ordering (n*3/2
+// disjoint rotations per round, all applied at once, n*3-1 rounds per sweep)
+// until the off-diagonal part is at rounding level, and written back only
+// when some eigenvalue is negative.
+#include "dispatch.hpp"
+
+namespace pfgpu {
+
+namespace {
+
+// round-robin pairing of N indices: round r pairs pos[k] with pos[N-1-k]
+template <int N> struct JacobiRounds {
+    int8_t p[(N - 1) * (N / 2)], q[(N - 1) * (N / 2)];
+    constexpr JacobiRounds() : p(), q()
+    {
+        int pos[N] = {};
+        for (int i = 0; i < N; ++i) pos[i] = i;
+        for (int r = 0; r < N - 1; ++r) {
+            for (int k = 0; k < N / 2; ++k) {
+                const int a = pos[k], b = pos[N - 1 - k];
+                p[r * (N / 2) + k] = int8_t(a < b ? a : b);
+                q[r * (N / 2) + k] = int8_t(a < b ? b : a);
+            }
+            const int last = pos[N - 1];
+            for (int i = N - 1; i > 1; --i) pos[i] = pos[i - 1];
+            pos[1] = last;
+        }
+    }
+};
+__constant__ JacobiRounds<12> c_jr12 = JacobiRounds<12>();
+__constant__ JacobiRounds<30> c_jr30 = JacobiRounds<30>();
+
+template <int N> __device__ __forceinline__ const JacobiRounds<N> &rounds()
+{
+    if constexpr (N == 12) return c_jr12;
+    else return c_jr30;
+}
+
+// element-local node pair of upper block s (a <= b), in k_hessian's order
+template <int NB> struct BlockPairs {
+    int8_t a[NB * (NB + 1) / 2], b[NB * (NB + 1) / 2];
+    constexpr BlockPairs() : a(), b()
+    {
+        int s = 0;
+        for (int i = 0; i < NB; ++i)
+            for (int j = i; j < NB; ++j) {
+                a[s] = int8_t(i);
+                b[s] = int8_t(j);
+                ++s;
+            }
+    }
+};
+__constant__ BlockPairs<4> c_bp4 = BlockPairs<4>();
+__constant__ BlockPairs<10> c_bp10 = BlockPairs<10>();
+
+template <int NB> __device__ __forceinline__ const BlockPairs<NB> &block_pairs()
+{
+    if constexpr (NB == 4) return c_bp4;
+    else return c_bp10;
+}
+
+constexpr int PSD_WAVE = 64;
+
+template <typename T> __device__ __forceinline__ T wave_sum(T v)
+{
+#pragma unroll
+    for (int o = PSD_WAVE / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, PSD_WAVE);
+    return v;
+}
+
+template <typename T, int ORDER>
+__global__ __launch_bounds__(PSD_WAVE) void k_psd_project(int n_elements, T *__restrict__ hloc)
+{
+    constexpr int NB = ORDER == 1 ? 4 : 10, N = 3 * NB, NP = N / 2, NBLK = NB * (NB + 1) / 2;
+    constexpr int MAX_SWEEPS = 20;
+    // squared relative off-diagonal size at which the iteration stops
+    // (the rounding floor of the off-diagonal part is about (N eps)^2 of the diagonal)
+    constexpr double TOL2 = sizeof(T) == 8 ? 1e-26 : 1e-10;
+    __shared__ T A[N][N + 1], V[N][N + 1], CS[NP][2];
+    const int e = blockIdx.x, lane = threadIdx.x;
+    if (e >= n_elements) return; // uniform over the workgroup
+    // tile layout [tile][s][element in tile][9]: 64 elements per tile
+    T *base = hloc + (int64_t(e) / 64) * NBLK * 576 + (e % 64) * 9;
+    const BlockPairs<NB> &bp = block_pairs<NB>();
+    for (int i = lane; i < NBLK * 9; i += PSD_WAVE) {
+        const int s = i / 9, j = i % 9, a = bp.a[s], b = bp.b[s];
+        const T v = base[int64_t(s) * 576 + j];
+        A[3 * a + j / 3][3 * b + j % 3] = v;
+        A[3 * b + j % 3][3 * a + j / 3] = v;
+    }
+    for (int i = lane; i < N * N; i += PSD_WAVE) V[i / N][i % N] = i / N == i % N ? T(1) : T(0);
+    __syncthreads();
+    const JacobiRounds<N> &jr = rounds<N>();
+    for (int sweep = 0; sweep < MAX_SWEEPS; ++sweep) {
+        T off = T(0), dia = T(0);
+        for (int i = lane; i < N * N; i += PSD_WAVE) {
+            const T v = A[i / N][i % N];
+            if (i / N == i % N)
+                dia += v * v;
+            else
+                off += v * v;
+        }
+        off = wave_sum(off);
+        dia = wave_sum(dia);
+        // the same values in every lane: a uniform decision
+        if (!(off > T(TOL2) * dia)) break;
+        for (int r = 0; r < N - 1; ++r) {
+            if (lane < NP) {
+                const int p = jr.p[r * NP + lane], q = jr.q[r * NP + lane];
+                const T apq = A[p][q];
+                T c = T(1), s = T(0);
+                if (apq != T(0)) {
+                    // the rotation that zeroes A[p][q] (the classical Jacobi step)
+                    const T th = (A[q][q] - A[p][p]) / (T(2) * apq);
+                    const T t = (th >= T(0) ? T(1) : T(-1)) / (std::fabs(th) + std::sqrt(th * th + T(1)));
+                    c = T(1) / std::sqrt(t * t + T(1));
+                    s = t * c;
+                }
+                CS[lane][0] = c;
+                CS[lane][1] = s;
+            }
+            __syncthreads();
+            // rows p, q of every pair (the pairs are disjoint)
+            for (int i = lane; i < NP * N; i += PSD_WAVE) {
+                const int k = i / N, j = i % N, p = jr.p[r * NP + k], q = jr.q[r * NP + k];
+                const T c = CS[k][0], s = CS[k][1], ap = A[p][j], aq = A[q][j];
+                A[p][j] = c * ap - s * aq;
+                A[q][j] = s * ap + c * aq;
+            }
+            __syncthreads();
+            // columns p, q, and the eigenvector matrix; the entry the rotation
+            // annihilates is set to zero rather than left at rounding level
+            for (int i = lane; i < NP * N; i += PSD_WAVE) {
+                const int k = i / N, j = i % N, p = jr.p[r * NP + k], q = jr.q[r * NP + k];
+                const T c = CS[k][0], s = CS[k][1];
+                const T ap = A[j][p], aq = A[j][q];
+                A[j][p] = j == q ? T(0) : c * ap - s * aq;
+                A[j][q] = j == p ? T(0) : s * ap + c * aq;
+                const T vp = V[j][p], vq = V[j][q];
+                V[j][p] = c * vp - s * vq;
+                V[j][q] = s * vp + c * vq;
+            }
+            __syncthreads();
+        }
+    }
+    // eigenvalues on the diagonal; nothing to do when none is negative
+    bool neg = false;
+    for (int k = lane; k < N; k += PSD_WAVE) neg |= A[k][k] < T(0);
+    if (!__any(neg)) return;
+    for (int i = lane; i < NBLK * 9; i += PSD_WAVE) {
+        const int s = i / 9, j = i % 9, r = 3 * bp.a[s] + j / 3, c = 3 * bp.b[s] + j % 3;
+        T acc = T(0);
+#pragma unroll 6
+        for (int k = 0; k < N; ++k) {
+            const T d = A[k][k];
+            if (d > T(0)) acc += V[r][k] * d * V[c][k];
+        }
+        base[int64_t(s) * 576 + j] = acc;
+    }
+}
+
+} // namespace
+
+// hloc: the element-block tiles of k_hessian for every element of the handle
+void psd_project_blocks(Handle &h, void *hloc)
+{
+    if (h.n_elements == 0) return;
+    auto go = [&]<typename T>() {
+        if (h.order == 1)
+            k_psd_project<T, 1><<<h.n_elements, PSD_WAVE, 0, h.stream>>>(h.n_elements, static_cast<T *>(hloc));
+        else
+            k_psd_project<T, 2><<<h.n_elements, PSD_WAVE, 0, h.stream>>>(h.n_elements, static_cast<T *>(hloc));
+    };
+    if (h.precision == PFGPU_MIXED)
+        go.template operator()<float>();
+    else
+        go.template operator()<double>();
+    PF_LAUNCH_CHECK();
+}
+
+} // namespace pfgpu

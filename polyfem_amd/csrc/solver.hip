@@ -457,7 +457,7 @@ bool inertia(const Handle &h) { return h.obj_xt != nullptr; }
 
 bool assembled_operator(const Handle &h)
 {
-    return h.solver_hessian == PFGPU_SOLVER_HESSIAN_ASSEMBLED ||
+    return h.solver_hessian == PFGPU_SOLVER_HESSIAN_ASSEMBLED || h.solver_hessian == PFGPU_SOLVER_HESSIAN_ASSEMBLED_PSD ||
            (h.solver_hessian == PFGPU_SOLVER_HESSIAN_AUTO && h.order == 1);
 }
 
@@ -505,7 +505,9 @@ void prepare_operator(Handle &h, const double *x, const uint8_t *fixed)
         if (!h.brow_ptr.get()) h.brow_ptr = upload(h.pat.brow_ptr, h.stream);
         if (!h.pat_bcol.get()) h.pat_bcol = upload(h.pat.bcol, h.stream);
         h.solver_vals.reserve(h.value_size() * size_t(h.pat.nnz));
-        if (h.path == PFGPU_PATH_FUSED)
+        if (h.solver_hessian == PFGPU_SOLVER_HESSIAN_ASSEMBLED_PSD)
+            plain_hessian(h, x, h.solver_vals.get(), true);
+        else if (h.path == PFGPU_PATH_FUSED)
             fused_energy_gradient_hessian(h, x, nullptr, nullptr, h.solver_vals.get());
         else
             plain_hessian(h, x, h.solver_vals.get());

@@ -208,3 +208,27 @@ def test_saint_venant_is_plain_path_only():
     x = torch.zeros(m.n_dofs, dtype=torch.float64, device="cuda")
     rep = a.autotune_dev(x.data_ptr())
     assert rep["path"] == "plain" and rep["ms_fused"] is None
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("order,name", [(1, "NeoHookean"), (2, "NeoHookean"), (2, "SaintVenant")])
+def test_project_to_psd(order, name, precision):
+    """Element Hessians projected onto the PSD cone (eigenvalues <= 0 clamped)
+    against the oracle's projection through LAPACK eigh, on a strongly
+    compressed state whose element Hessians are indefinite."""
+    m = kuhn_box(2, 2, 2, order=order)
+    lam, mu = random_lame(m.n_elements, seed=BASE_SEED + 110)
+    x = smooth_displacement(m.nodes, 0.05, 1e-3, BASE_SEED + 111)
+    x[2::3] -= 0.45 * m.nodes[:, 2]
+    print(f"psd order={order} {name} {precision}", flush=True)
+    a = NLAssembler(name, m, lame=(lam, mu), precision=precision)
+    o = po.Oracle(m, lam, mu, material=MAT[name])
+    outer, inner, ref = o.hessian_psd(x)
+    _, _, plain = o.hessian(x)
+    assert rel(plain, ref) > 1e-6  # the projection changes this state
+    go, gi = a.hessian_pattern()
+    assert np.array_equal(go, outer) and np.array_equal(gi, inner)
+    tol = FP64_TOL if precision == "fp64" else 1e-4
+    assert rel(a.hessian_values(x, project_to_psd=True), ref) < tol
+    # the plain values are unaffected by an earlier projected call
+    assert rel(a.hessian_values(x), plain) < (FP64_TOL if precision == "fp64" else MIXED_TOL["hessian"])

@@ -219,3 +219,25 @@ def test_saint_venant_newton_converges(order):
     assert rep.converged and rep.iterations <= 10, rep
     go = po.Oracle(m, lam, mu, material=po.SAINT_VENANT).gradient(x)
     assert np.linalg.norm(go[prob.free]) <= 1e-8 * g0
+
+
+def test_newton_with_psd_projected_operator():
+    """The assembled operator from PSD-projected element Hessians (the
+    reference's project_to_psd option): Newton still converges to the same
+    minimiser from a compressed start."""
+    m, prob = clamped_box(1, (2, 2, 3), pull=-0.2)
+    lam, mu = random_lame(m.n_elements, seed=BASE_SEED + 95)
+    print("Newton with the PSD-projected operator", flush=True)
+    sols = []
+    for hess in ("assembled", "assembled_psd"):
+        a = NLAssembler("NeoHookean", m, lame=(lam, mu), precision="fp64")
+        a.set_solver_options(hess, "block_jacobi")
+        x0 = np.zeros(m.n_dofs)
+        x0[2::3] = -0.2 * m.nodes[:, 2]
+        x0[prob.fixed] = prob.fixed_values
+        g0 = np.linalg.norm(a.assemble_gradient(x0)[prob.free])
+        x, rep = NewtonSolver(max_iterations=60, grad_norm=1e-9 * g0, cg_max_iterations=5000,
+                              cg_tolerance=1e-12).minimize(a, x0, prob)
+        assert rep.converged, (hess, rep)
+        sols.append(x)
+    assert rel(sols[1], sols[0]) < 1e-7

@@ -469,3 +469,30 @@ def test_static_shape_gradient_through_solves(order):
         dV = rng.normal(size=V0.shape) * 0.02
         fd = (J(V0 + h * dV, solve(V0 + h * dV, u)[0]) - J(V0 - h * dV, solve(V0 - h * dV, u)[0])) / (2 * h)
         assert abs(grad @ dV.ravel() - fd) <= 1e-6 * abs(fd)
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_psd_projection_oracle(order):
+    """The projected Hessian: same pattern as the plain one; every element
+    matrix after projection is PSD, so the sum is PSD; PSD element matrices
+    (linear elasticity) are left unchanged; projecting a compressed NeoHookean
+    state changes it; projection of the projected element matrices is a no-op."""
+    import scipy.sparse as sp
+    m = kuhn_box(2, 2, 1, order=order)
+    lam, mu = np.full(m.n_elements, LAM), np.full(m.n_elements, MU)
+    x = smooth_displacement(m.nodes, 0.05, 1e-3, BASE_SEED + 15)
+    ol = po.Oracle(m, lam, mu, material=po.LINEAR_ELASTICITY)
+    o1, i1, v1 = ol.hessian(x)
+    o2, i2, v2 = ol.hessian_psd(x)
+    assert np.array_equal(o1, o2) and np.array_equal(i1, i2)
+    assert np.abs(v1 - v2).max() <= 1e-9 * np.abs(v1).max()
+    # a strongly compressed NeoHookean state has indefinite element Hessians
+    xc = x.copy()
+    xc[2::3] -= 0.45 * m.nodes[:, 2]
+    on = po.Oracle(m, lam, mu, material=po.NEOHOOKEAN)
+    H = on.hessian_scipy(xc).toarray()
+    outer, inner, vals = on.hessian_psd(xc)
+    P = sp.csc_matrix((vals, inner, outer), shape=H.shape).toarray()
+    assert np.linalg.eigvalsh(H).min() < -1e-6 * np.abs(H).max()
+    assert np.linalg.eigvalsh(P).min() > -1e-9 * np.abs(P).max()
+    assert np.abs(P - P.T).max() <= 1e-12 * np.abs(P).max()
