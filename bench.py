@@ -108,6 +108,17 @@ def extras(args, m, lam, mu, x_host, a, x, g, vals, y, eqp):
 
     res["mixed_vs_fp64_rel_error"] = {"energy": rel(e32, e64), "gradient": rel(g, g64), "hessian": rel(vals, v64),
                                       "hvp": rel(y, y64)}
+    # PSD projection of the element Hessians (host entry point, so both timings
+    # include the same staging and copy of the values)
+    xh = x.cpu().numpy()
+    tp = []
+    for psd in (False, True, False, True):
+        t0 = time.perf_counter()
+        a.hessian_values(xh, project_to_psd=psd)
+        tp.append(time.perf_counter() - t0)
+    res["project_to_psd"] = {"extra_ms": 1e3 * (tp[3] - tp[2]),
+                             "what": "one E+G+H-sized Hessian with every element matrix projected onto the PSD cone, "
+                                     "minus the same call without projection (host API)"}
     # Dirichlet setup of the Newton and adjoint solves
     bot, top = face_nodes(m, 2, 0.0), face_nodes(m, 2, 1.0)
     fixed = np.concatenate([3 * bot[:, None] + np.arange(3), 3 * top[:, None] + np.arange(3)]).ravel()
