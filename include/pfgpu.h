@@ -49,7 +49,15 @@ typedef enum {
 /* SaintVenant: psi = lambda/2 tr(E)^2 + mu E:E with E = (F^T F - I)/2, the
  * isotropic case of SaintVenantElasticity (SaintVenantElasticity.cpp:235-278);
  * the plain path only */
-typedef enum { PFGPU_NEOHOOKEAN = 0, PFGPU_LINEAR_ELASTICITY = 1, PFGPU_SAINT_VENANT = 2 } pfgpu_material_kind;
+/* FixedCorotational: psi = mu |F - R|^2 + lambda/2 (J - 1)^2 with R the
+ * rotation of F (FixedCorotational.cpp:589-706); elements with det F <= 0
+ * are reported like inverted NeoHookean elements; the plain path only */
+typedef enum {
+    PFGPU_NEOHOOKEAN = 0,
+    PFGPU_LINEAR_ELASTICITY = 1,
+    PFGPU_SAINT_VENANT = 2,
+    PFGPU_FIXED_COROTATIONAL = 3
+} pfgpu_material_kind;
 
 /* FP64: every kernel in double.  MIXED: element kernels in float; the energy
  * sum, gradient accumulation, norms and CG dot products in double; Hessian

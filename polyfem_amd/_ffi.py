@@ -14,7 +14,7 @@ LIB_PATH = os.environ.get("PFGPU_LIB", os.path.join(HERE, "libpfgpu.so"))
 PFGPU_OK = 0
 STATUS_NAMES = {0: "OK", 1: "INVALID_ARGUMENT", 2: "HIP", 3: "INVALID_STATE", 4: "UNSUPPORTED", 5: "OUT_OF_MEMORY"}
 OK, ERR_INVALID_ARGUMENT, ERR_HIP, ERR_INVALID_STATE, ERR_UNSUPPORTED, ERR_OUT_OF_MEMORY = range(6)
-NEOHOOKEAN, LINEAR_ELASTICITY, SAINT_VENANT = 0, 1, 2
+NEOHOOKEAN, LINEAR_ELASTICITY, SAINT_VENANT, FIXED_COROTATIONAL = 0, 1, 2, 3
 FP64, MIXED = 0, 1
 PATH_FUSED, PATH_PLAIN = 0, 1
 

@@ -22,7 +22,8 @@ import numpy as np
 from . import _ffi
 from ._ffi import check, lib
 
-_NAMES = {"NeoHookean": _ffi.NEOHOOKEAN, "LinearElasticity": _ffi.LINEAR_ELASTICITY, "SaintVenant": _ffi.SAINT_VENANT}
+_NAMES = {"NeoHookean": _ffi.NEOHOOKEAN, "LinearElasticity": _ffi.LINEAR_ELASTICITY, "SaintVenant": _ffi.SAINT_VENANT,
+          "FixedCorotational": _ffi.FIXED_COROTATIONAL}
 _PRECISIONS = {"fp64": _ffi.FP64, "mixed": _ffi.MIXED}
 _PATHS = {"fused": _ffi.PATH_FUSED, "plain": _ffi.PATH_PLAIN}
 

@@ -199,7 +199,7 @@ pfgpu_status pfgpu_create(const pfgpu_mesh_desc *mesh, const pfgpu_material_desc
         require(m.n_elements > 0 && m.n_nodes > 0 && m.n_vertices > 0, "empty mesh");
         require(m.elem_nodes && m.vertices && m.elem_vertices, "null mesh array");
         require(material->kind == PFGPU_NEOHOOKEAN || material->kind == PFGPU_LINEAR_ELASTICITY ||
-                    material->kind == PFGPU_SAINT_VENANT,
+                    material->kind == PFGPU_SAINT_VENANT || material->kind == PFGPU_FIXED_COROTATIONAL,
                 "unknown material");
         require(precision == PFGPU_FP64 || precision == PFGPU_MIXED, "unknown precision");
         require((material->lambda == nullptr) == (material->mu == nullptr), "lambda and mu must both be given or both be NULL");
@@ -294,7 +294,7 @@ pfgpu_status pfgpu_set_tuning(pfgpu_handle *h, int32_t path, int32_t gather_thre
         require(gather_threads == 128 || gather_threads == 256 || gather_threads == 512, "gather_threads must be 128, 256 or 512");
         require(hessian_threads == 64 || hessian_threads == 128 || hessian_threads == 256,
                 "hessian_threads must be 64, 128 or 256");
-        if (path == PFGPU_PATH_FUSED && h->material == PFGPU_SAINT_VENANT)
+        if (path == PFGPU_PATH_FUSED && (h->material == PFGPU_SAINT_VENANT || h->material == PFGPU_FIXED_COROTATIONAL))
             fail(PFGPU_ERR_UNSUPPORTED, "the fused path covers NeoHookean and linear elasticity");
         h->path = path;
         h->tune_gather_threads = gather_threads;
@@ -378,7 +378,7 @@ pfgpu_status pfgpu_set_path(pfgpu_handle *h, int32_t path)
     return guard([&] {
         check_handle(h);
         require(path == PFGPU_PATH_FUSED || path == PFGPU_PATH_PLAIN, "unknown path");
-        if (path == PFGPU_PATH_FUSED && h->material == PFGPU_SAINT_VENANT)
+        if (path == PFGPU_PATH_FUSED && (h->material == PFGPU_SAINT_VENANT || h->material == PFGPU_FIXED_COROTATIONAL))
             fail(PFGPU_ERR_UNSUPPORTED, "the fused path covers NeoHookean and linear elasticity");
         h->path = path;
     });

@@ -18,6 +18,8 @@ template <typename F> void dispatch(const Handle &h, F &&f)
             f.template operator()<T, O, Q, MAT_NH>();
         else if (h.material == MAT_SV)
             f.template operator()<T, O, Q, MAT_SV>();
+        else if (h.material == MAT_FC)
+            f.template operator()<T, O, Q, MAT_FC>();
         else
             f.template operator()<T, O, Q, MAT_LE>();
     };

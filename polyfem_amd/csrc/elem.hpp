@@ -19,7 +19,12 @@
 
 namespace pfgpu {
 
-enum { MAT_NH = PFGPU_NEOHOOKEAN, MAT_LE = PFGPU_LINEAR_ELASTICITY, MAT_SV = PFGPU_SAINT_VENANT };
+enum {
+    MAT_NH = PFGPU_NEOHOOKEAN,
+    MAT_LE = PFGPU_LINEAR_ELASTICITY,
+    MAT_SV = PFGPU_SAINT_VENANT,
+    MAT_FC = PFGPU_FIXED_COROTATIONAL
+};
 
 template <int ORDER> struct NLocal { static constexpr int value = ORDER == 1 ? 4 : 10; };
 
@@ -270,6 +275,66 @@ template <typename T> __device__ __forceinline__ void sv_stress(const SvKin<T> &
 #pragma unroll
     for (int i = 0; i < 9; ++i) S[i] = T(2) * mu * s.E[i] + (i % 4 == 0 ? lam * s.trE : T(0));
 }
+// FixedCorotational kinematics (det F > 0): the rotation R = I + Q of the polar
+// decomposition F = R S, by Newton's iteration X <- (X + X^{-T}) / 2 from
+// X = F written for Q = X - I, Q <- (Q - X^{-T} Q^T) / 2, so that F - R = H - Q
+// keeps its precision for small strains; and K^{-1} for K = tr(S) I - S,
+// which gives the rotation's derivative dR[W] = R [K^{-1} a(W)]x with
+// a(W) the axial vector of R^T W - W^T R.
+constexpr int FC_POLAR_ITERATIONS = 12;
+template <typename T> struct FcKin {
+    T Q[9], Ki6[6]; // Ki6: K^{-1}, entries 00 11 22 01 02 12
+};
+template <typename T> __device__ __forceinline__ void fc_kinematics(const T *H, FcKin<T> &f)
+{
+#pragma unroll
+    for (int i = 0; i < 9; ++i) f.Q[i] = H[i];
+#pragma unroll 1
+    for (int it = 0; it < FC_POLAR_ITERATIONS; ++it) {
+        const T X[9] = {T(1) + f.Q[0], f.Q[1], f.Q[2], f.Q[3], T(1) + f.Q[4], f.Q[5], f.Q[6], f.Q[7], T(1) + f.Q[8]};
+        const T C[9] = {X[4] * X[8] - X[5] * X[7], X[5] * X[6] - X[3] * X[8], X[3] * X[7] - X[4] * X[6],
+                        X[2] * X[7] - X[1] * X[8], X[0] * X[8] - X[2] * X[6], X[1] * X[6] - X[0] * X[7],
+                        X[1] * X[5] - X[2] * X[4], X[2] * X[3] - X[0] * X[5], X[0] * X[4] - X[1] * X[3]};
+        const T id = T(1) / (X[0] * C[0] + X[1] * C[1] + X[2] * C[2]);
+        T N[9];
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+                N[r * 3 + c] = T(0.5) * (f.Q[r * 3 + c] - id * (C[r * 3 + 0] * f.Q[c * 3 + 0] + C[r * 3 + 1] * f.Q[c * 3 + 1] +
+                                                                 C[r * 3 + 2] * f.Q[c * 3 + 2]));
+#pragma unroll
+        for (int i = 0; i < 9; ++i) f.Q[i] = N[i];
+    }
+    // S - I = R^T F - I = H + Q^T + Q^T H (symmetrised); K = tr(S) I - S
+    T Sm[9];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            Sm[r * 3 + c] = H[r * 3 + c] + f.Q[c * 3 + r] + f.Q[0 * 3 + r] * H[0 * 3 + c] + f.Q[1 * 3 + r] * H[1 * 3 + c] +
+                            f.Q[2 * 3 + r] * H[2 * 3 + c];
+    const T tr = T(3) + Sm[0] + Sm[4] + Sm[8];
+    const T k00 = tr - T(1) - Sm[0], k11 = tr - T(1) - Sm[4], k22 = tr - T(1) - Sm[8];
+    const T k01 = -T(0.5) * (Sm[1] + Sm[3]), k02 = -T(0.5) * (Sm[2] + Sm[6]), k12 = -T(0.5) * (Sm[5] + Sm[7]);
+    const T c00 = k11 * k22 - k12 * k12, c01 = k02 * k12 - k01 * k22, c02 = k01 * k12 - k02 * k11;
+    const T c11 = k00 * k22 - k02 * k02, c12 = k01 * k02 - k00 * k12, c22 = k00 * k11 - k01 * k01;
+    const T idet = T(1) / (k00 * c00 + k01 * c01 + k02 * c02);
+    f.Ki6[0] = c00 * idet, f.Ki6[1] = c11 * idet, f.Ki6[2] = c22 * idet;
+    f.Ki6[3] = c01 * idet, f.Ki6[4] = c02 * idet, f.Ki6[5] = c12 * idet;
+    // an inverted or degenerate F (det F <= 0) poisons every output: the
+    // orthogonal polar factor is then a reflection, not the reference's
+    // sign-flipped rotation, so the stress, tangent and adjoint terms report
+    // non-finite values (the energy is NaN there as well)
+    const T detF = (T(1) + H[0]) * ((T(1) + H[4]) * (T(1) + H[8]) - H[5] * H[7]) -
+                   H[1] * (H[3] * (T(1) + H[8]) - H[5] * H[6]) + H[2] * (H[3] * H[7] - (T(1) + H[4]) * H[6]);
+    const T poison = detF > T(0) ? T(0) : T(NAN);
+#pragma unroll
+    for (int i = 0; i < 9; ++i) f.Q[i] += poison;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) f.Ki6[i] += poison;
+}
+
 // out = A B (3x3)
 template <typename T> __device__ __forceinline__ void mul3(const T *A, const T *B, T *out)
 {
@@ -290,6 +355,16 @@ template <typename T, int MAT> __device__ __forceinline__ T energy_density(const
 #pragma unroll
         for (int i = 0; i < 9; ++i) hh += H[i] * H[i];
         return mu * (xmlog1p(k.delta) - k.s2 - k.dH) + T(0.5) * mu * hh + T(0.5) * lam * k.lnJ * k.lnJ;
+    } else if (MAT == MAT_FC) {
+        // mu |F - R|^2 + lambda/2 (J - 1)^2 (FixedCorotational.cpp:589-596: the
+        // sum of (sigma_i - 1)^2 is |F - R|^2); NaN for det F <= 0
+        FcKin<T> f;
+        fc_kinematics(H, f);
+        T dd = T(0);
+#pragma unroll
+        for (int i = 0; i < 9; ++i) dd += (H[i] - f.Q[i]) * (H[i] - f.Q[i]);
+        const T v = mu * dd + T(0.5) * lam * k.delta * k.delta;
+        return k.J > T(0) ? v : T(NAN);
     } else if (MAT == MAT_SV) {
         // (SaintVenantElasticity.cpp:235-278: 1/2 sigma : E with sigma = C : E, C isotropic)
         SvKin<T> s;
@@ -320,6 +395,13 @@ template <typename T, int MAT> __device__ __forceinline__ void pk1(const Kin<T> 
         const T ll = lam * k.lnJ;
 #pragma unroll
         for (int i = 0; i < 9; ++i) P[i] = mu * (H[i] + k.D[i]) + ll * k.FmT[i];
+    } else if (MAT == MAT_FC) {
+        // P = 2 mu (F - R) + lambda (J - 1) cof F (FixedCorotational.cpp:683-706)
+        FcKin<T> f;
+        fc_kinematics(H, f);
+        const T lj = lam * k.delta * k.J;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) P[i] = T(2) * mu * (H[i] - f.Q[i]) + lj * k.FmT[i];
     } else if (MAT == MAT_SV) {
         // P = F S
         SvKin<T> s;
@@ -358,6 +440,40 @@ __device__ __forceinline__ void dstress(const Kin<T> &k, T lam, T mu, const T *W
             for (int c = 0; c < 3; ++c) {
                 const T t2 = t1[r * 3 + 0] * A[0 * 3 + c] + t1[r * 3 + 1] * A[1 * 3 + c] + t1[r * 3 + 2] * A[2 * 3 + c];
                 out[r * 3 + c] = mu * W[r * 3 + c] + cc * t2 + lf * A[r * 3 + c];
+            }
+    } else if (MAT == MAT_FC) {
+        // dP = 2 mu (W - R [K^{-1} a(W)]x) + lambda J (2J - 1) (F^{-T}:W) F^{-T}
+        //      - lambda J (J - 1) F^{-T} W^T F^{-T}
+        FcKin<T> f;
+        fc_kinematics(k.H, f);
+        const T *A = k.FmT;
+        T fw = T(0);
+#pragma unroll
+        for (int i = 0; i < 9; ++i) fw += A[i] * W[i];
+        T t1[9];
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) t1[r * 3 + c] = A[r * 3 + 0] * W[c * 3 + 0] + A[r * 3 + 1] * W[c * 3 + 1] + A[r * 3 + 2] * W[c * 3 + 2];
+        const T R[9] = {T(1) + f.Q[0], f.Q[1], f.Q[2], f.Q[3], T(1) + f.Q[4], f.Q[5], f.Q[6], f.Q[7], T(1) + f.Q[8]};
+        T B[9]; // R^T W
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) B[r * 3 + c] = R[0 * 3 + r] * W[0 * 3 + c] + R[1 * 3 + r] * W[1 * 3 + c] + R[2 * 3 + r] * W[2 * 3 + c];
+        const T a0 = B[7] - B[5], a1 = B[2] - B[6], a2 = B[3] - B[1];
+        const T *K = f.Ki6;
+        const T w0 = K[0] * a0 + K[3] * a1 + K[4] * a2, w1 = K[3] * a0 + K[1] * a1 + K[5] * a2, w2 = K[4] * a0 + K[5] * a1 + K[2] * a2;
+        const T Om[9] = {T(0), -w2, w1, w2, T(0), -w0, -w1, w0, T(0)};
+        T RO[9];
+        mul3(R, Om, RO);
+        const T cl = lam * k.J * (T(1) + T(2) * k.delta), cq = lam * k.J * k.delta;
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const T t2 = t1[r * 3 + 0] * A[0 * 3 + c] + t1[r * 3 + 1] * A[1 * 3 + c] + t1[r * 3 + 2] * A[2 * 3 + c];
+                out[r * 3 + c] = T(2) * mu * (W[r * 3 + c] - RO[r * 3 + c]) + cl * fw * A[r * 3 + c] - cq * t2;
             }
     } else if (MAT == MAT_SV) {
         // dP = W S + F dS, dS = lambda tr(dE) I + 2 mu dE, dE = (W^T F + F^T W) / 2
@@ -437,6 +553,16 @@ __device__ __forceinline__ void hess_qp(const Kin<T> &k, const T *jit, T lam, T 
                 M[m * 3 + kk] = k.FmT[m * 3 + 0] * jit[kk * 3 + 0] + k.FmT[m * 3 + 1] * jit[kk * 3 + 1] + k.FmT[m * 3 + 2] * jit[kk * 3 + 2];
         cl = da * lam;
         cc = da * (mu - lam * k.lnJ);
+    } else if (MAT == MAT_FC) {
+        // k_a = F^{-T} G_a: lambda J (2J - 1) k_a k_b^T - lambda J (J - 1) k_b k_a^T; the
+        // 2 mu metric term and the rotation term are added by the caller
+#pragma unroll
+        for (int m = 0; m < 3; ++m)
+#pragma unroll
+            for (int kk = 0; kk < 3; ++kk)
+                M[m * 3 + kk] = k.FmT[m * 3 + 0] * jit[kk * 3 + 0] + k.FmT[m * 3 + 1] * jit[kk * 3 + 1] + k.FmT[m * 3 + 2] * jit[kk * 3 + 2];
+        cl = da * lam * k.J * (T(1) + T(2) * k.delta);
+        cc = -da * lam * k.J * k.delta;
     } else if (MAT == MAT_SV) {
         // k_a = F G_a; the two further SaintVenant terms are added by the caller
         const T F[9] = {T(1) + k.H[0], k.H[1], k.H[2], k.H[3], T(1) + k.H[4], k.H[5], k.H[6], k.H[7], T(1) + k.H[8]};

@@ -527,7 +527,8 @@ void fused_energy_gradient_hessian(Handle &h, const double *x, double *energy, d
             plain_energy(h, x, energy, nullptr);
         return;
     }
-    if (h.material == MAT_SV) fail(PFGPU_ERR_UNSUPPORTED, "the fused path covers NeoHookean and linear elasticity");
+    if (h.material == MAT_SV || h.material == MAT_FC)
+        fail(PFGPU_ERR_UNSUPPORTED, "the fused path covers NeoHookean and linear elasticity");
     ensure_fused_tables(h);
     double *gv = g;
     if (!gv) {
@@ -535,7 +536,7 @@ void fused_energy_gradient_hessian(Handle &h, const double *x, double *energy, d
         gv = h.loc.as<double>();
     }
     dispatch(h, [&]<typename T, int O, int Q, int M>() {
-        if constexpr (M != MAT_SV) launch_egh<T, O, Q, M>(h, x, values, gv, energy);
+        if constexpr (M == MAT_NH || M == MAT_LE) launch_egh<T, O, Q, M>(h, x, values, gv, energy);
     });
 }
 

@@ -170,19 +170,29 @@ __global__ __launch_bounds__(HB) void k_hessian(DevMesh<T> m, const double *__re
     // weights; the k vectors are formed per block row and column to keep the
     // register footprint small.  SaintVenant adds the S-weighted metric and
     // mu F F^T per point.
-    constexpr int NX = MAT == MAT_SV ? NQ : 1;
-    T Mq[NQ][9], cl[NQ], cc[NQ], sg6[NX][6], ff6[NX][6];
+    // FixedCorotational adds the rotation R and K^{-1} per point.
+    constexpr int NX = MAT == MAT_SV || MAT == MAT_FC ? NQ : 1;
+    T Mq[NQ][9], cl[NQ], cc[NQ], sg6[NX][6], ff6[NX][9];
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
         Kin<T> k;
         if (MAT != MAT_LE) field_grad<T, ORDER, NQ>(q, el.U, el.jit, k.H);
-        if (MAT == MAT_NH) kinematics(k);
+        if (MAT == MAT_NH || MAT == MAT_FC) kinematics(k);
         hess_qp<T, MAT>(k, el.jit, el.lam, el.mu, w, Mq[q], cl[q], cc[q]);
         if constexpr (MAT == MAT_SV) sv_extra(k.H, el.jit, el.lam, el.mu, w, sg6[q], ff6[q]);
+        if constexpr (MAT == MAT_FC) {
+            // sg6: K^{-1}; ff6: the rows of R
+            FcKin<T> f;
+            fc_kinematics(k.H, f);
+#pragma unroll
+            for (int i = 0; i < 6; ++i) sg6[q][i] = f.Ki6[i];
+#pragma unroll
+            for (int i = 0; i < 9; ++i) ff6[q][i] = f.Q[i] + (i % 4 == 0 ? T(1) : T(0));
+        }
     }
     T g6[6];
     metric(el.jit, g6);
-    const T cm = w * el.mu;
+    const T cm = (MAT == MAT_FC ? T(2) : T(1)) * w * el.mu;
     T *st = stage[wv];
     T *out = hloc + tile * NBLK * 576;
     const RefTab<T, ORDER, NQ> &tb = ref_tab<T, ORDER, NQ>();
@@ -211,7 +221,41 @@ __global__ __launch_bounds__(HB) void k_hessian(DevMesh<T> m, const double *__re
             const int s = a * NB - a * (a - 1) / 2 + (b - a);
             const T *c = tb.rgg + (a * NB + b) * 6;
             T gd = T(0), xf[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
-            if constexpr (MAT == MAT_SV) {
+            T xr[9] = {T(0), T(0), T(0), T(0), T(0), T(0), T(0), T(0), T(0)};
+            if constexpr (MAT == MAT_FC) {
+                // -2 mu da sum_q (G_a x r_m)^T K^{-1} (G_b x r_n), r_m the rows of R
+                gd = cm * (g6[0] * c[0] + g6[1] * c[1] + g6[2] * c[2] + g6[3] * c[3] + g6[4] * c[4] + g6[5] * c[5]);
+#pragma unroll
+                for (int q = 0; q < NQ; ++q) {
+                    const T *ra = tb.rg + (q * NB + a) * 3, *rb = tb.rg + (q * NB + b) * 3;
+                    T Ga[3], Gb[3];
+#pragma unroll
+                    for (int i = 0; i < 3; ++i) {
+                        Ga[i] = ra[0] * el.jit[0 * 3 + i] + ra[1] * el.jit[1 * 3 + i] + ra[2] * el.jit[2 * 3 + i];
+                        Gb[i] = rb[0] * el.jit[0 * 3 + i] + rb[1] * el.jit[1 * 3 + i] + rb[2] * el.jit[2 * 3 + i];
+                    }
+                    const T *R = ff6[q], *Ki = sg6[q];
+                    T ca[3][3], kb[3][3];
+#pragma unroll
+                    for (int mm = 0; mm < 3; ++mm) {
+                        const T *r = R + mm * 3;
+                        ca[mm][0] = Ga[1] * r[2] - Ga[2] * r[1];
+                        ca[mm][1] = Ga[2] * r[0] - Ga[0] * r[2];
+                        ca[mm][2] = Ga[0] * r[1] - Ga[1] * r[0];
+                        const T cb0 = Gb[1] * r[2] - Gb[2] * r[1], cb1 = Gb[2] * r[0] - Gb[0] * r[2],
+                                cb2 = Gb[0] * r[1] - Gb[1] * r[0];
+                        kb[mm][0] = Ki[0] * cb0 + Ki[3] * cb1 + Ki[4] * cb2;
+                        kb[mm][1] = Ki[3] * cb0 + Ki[1] * cb1 + Ki[5] * cb2;
+                        kb[mm][2] = Ki[4] * cb0 + Ki[5] * cb1 + Ki[2] * cb2;
+                    }
+                    const T cr = -T(2) * el.mu * w;
+#pragma unroll
+                    for (int mm = 0; mm < 3; ++mm)
+#pragma unroll
+                        for (int nn = 0; nn < 3; ++nn)
+                            xr[mm * 3 + nn] += cr * (ca[mm][0] * kb[nn][0] + ca[mm][1] * kb[nn][1] + ca[mm][2] * kb[nn][2]);
+                }
+            } else if constexpr (MAT == MAT_SV) {
                 // sum_q [ (G_a^T S G_b) I + mu (G_a . G_b) F F^T ] da
 #pragma unroll
                 for (int q = 0; q < NQ; ++q) {
@@ -231,6 +275,7 @@ __global__ __launch_bounds__(HB) void k_hessian(DevMesh<T> m, const double *__re
                 for (int nn = 0; nn < 3; ++nn) {
                     T v = mm == nn ? gd : T(0);
                     if constexpr (MAT == MAT_SV) v += xf[sym[mm * 3 + nn]];
+                    if constexpr (MAT == MAT_FC) v += xr[mm * 3 + nn];
 #pragma unroll
                     for (int q = 0; q < NQ; ++q) v += cl[q] * Ka[q][mm] * Kb[q][nn] + cc[q] * Kb[q][mm] * Ka[q][nn];
                     st[lane * 9 + mm * 3 + nn] = v;
@@ -267,7 +312,7 @@ __global__ __launch_bounds__(BLOCK) void k_hvp(DevMesh<T> m, const double *__res
     for (int q = 0; q < NQ; ++q) {
         Kin<T> k;
         field_grad<T, ORDER, NQ>(q, el.U, el.jit, k.H);
-        if (MAT == MAT_NH) kinematics(k);
+        if (MAT == MAT_NH || MAT == MAT_FC) kinematics(k);
         T Gv[9], A[9];
         field_grad<T, ORDER, NQ>(q, V, el.jit, Gv);
         dstress<T, MAT>(k, el.lam, el.mu, Gv, A);
@@ -309,6 +354,13 @@ __global__ __launch_bounds__(BLOCK) void k_material_jvp(DevMesh<T> m, const doub
             // dP/dmu = F - F^{-T}, dP/dlambda = ln J F^{-T} (NeoHookeanElasticity.cpp:747-762)
 #pragma unroll
             for (int i = 0; i < 9; ++i) A[i] = sm * (k.H[i] + k.D[i]) + sl * k.lnJ * k.FmT[i];
+        } else if (MAT == MAT_FC) {
+            // dP/dmu = 2 (F - R), dP/dlambda = (J - 1) cof F (FixedCorotational.cpp:493-546)
+            kinematics(k);
+            FcKin<T> f;
+            fc_kinematics(k.H, f);
+#pragma unroll
+            for (int i = 0; i < 9; ++i) A[i] = sm * T(2) * (k.H[i] - f.Q[i]) + sl * k.delta * k.J * k.FmT[i];
         } else if (MAT == MAT_SV) {
             // dP/dmu = 2 F E, dP/dlambda = tr(E) F
             SvKin<T> s;
@@ -364,6 +416,16 @@ __global__ __launch_bounds__(BLOCK) void k_material_vjp(DevMesh<T> m, const doub
                 il += k.FmT[i] * Gp[i];
             }
             il *= k.lnJ;
+        } else if (MAT == MAT_FC) {
+            kinematics(k);
+            FcKin<T> f;
+            fc_kinematics(k.H, f);
+#pragma unroll
+            for (int i = 0; i < 9; ++i) {
+                im += T(2) * (k.H[i] - f.Q[i]) * Gp[i];
+                il += k.FmT[i] * Gp[i];
+            }
+            il *= k.delta * k.J;
         } else if (MAT == MAT_SV) {
             SvKin<T> s;
             sv_kinematics(k.H, s);
@@ -414,7 +476,7 @@ __global__ __launch_bounds__(BLOCK) void k_shape_vjp(DevMesh<T> m, const double 
     for (int q = 0; q < NQ; ++q) {
         Kin<T> k;
         field_grad<T, ORDER, NQ>(q, el.U, el.jit, k.H);
-        if (MAT == MAT_NH) kinematics(k);
+        if (MAT == MAT_NH || MAT == MAT_FC) kinematics(k);
         T Gp[9], P[9], B[9];
         field_grad<T, ORDER, NQ>(q, Pn, el.jit, Gp);
         pk1<T, MAT>(k, el.lam, el.mu, P);

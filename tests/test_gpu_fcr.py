@@ -1,0 +1,172 @@
+"""GPU parity of the FixedCorotational material against its FP64 oracle.
+
+The oracle (oracle/fcr.py) restates the reference's SVD formulas; the HIP
+path computes the rotation by a polar iteration and the tangent in a
+structured form, so agreement here pins two independent derivations.  Bars
+as for the other materials: FP64 within 1e-10 relative, mixed precision
+within the FP32 tolerances.  FixedCorotational runs on the plain path only.
+"""
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+from oracle.fcr import FcrOracle
+from polyfem_amd import NewtonSolver, NLAssembler, NLProblem, PfgpuError
+from polyfem_amd.mesh import BASE_SEED, face_nodes, kuhn_box, random_lame, smooth_displacement
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-10
+MIXED_TOL = {"energy": 1e-5, "gradient": 2e-5, "hessian": 2e-5, "hvp": 2e-5, "vjp": 5e-5}
+NAME = "FixedCorotational"
+
+
+def rel(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def setup_case(order, seed, precision="fp64", grid=(3, 2, 3), amp=0.05):
+    m = kuhn_box(*grid, order=order)
+    lam, mu = random_lame(m.n_elements, seed=BASE_SEED + 120 + seed, E_mean=1e6, E_sigma=0.5)
+    x = smooth_displacement(m.nodes, amp, 1e-3, BASE_SEED + 130 + seed)
+    a = NLAssembler(NAME, m, lame=(lam, mu), precision=precision)
+    return m, x, a, FcrOracle(m, lam, mu), lam, mu
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_fp64_parity(order):
+    m, x, a, o, lam, mu = setup_case(order, 0)
+    print(f"FixedCorotational fp64 order={order} E={m.n_elements}", flush=True)
+    assert rel(a.assemble_energy(x), o.energy(x)) < FP64_TOL
+    assert rel(a.assemble_energy_per_element(x), o.energy_per_element(x)) < FP64_TOL
+    assert rel(a.assemble_gradient(x), o.gradient(x)) < FP64_TOL
+    outer, inner, ref = o.hessian(x)
+    go, gi = a.hessian_pattern()
+    assert np.array_equal(go, outer) and np.array_equal(gi, inner)
+    assert rel(a.hessian_values(x), ref) < FP64_TOL
+    e, g, vals = a.assemble_energy_gradient_hessian(x)
+    assert rel(e, o.energy(x)) < FP64_TOL and rel(g, o.gradient(x)) < FP64_TOL and rel(vals, ref) < FP64_TOL
+    rng = np.random.default_rng(BASE_SEED + 140)
+    v = rng.normal(size=m.n_dofs)
+    assert rel(a.hessian_vector_product(x, v), o.hessian_scipy(x) @ v) < FP64_TOL
+    p = rng.normal(size=m.n_dofs)
+    term = o.material_vjp(x, p)
+    assert rel(a.force_material_derivative(x, p), term) < FP64_TOL
+    assert rel(a.force_shape_derivative(x, p), o.shape_vjp(x, p)) < FP64_TOL
+    dl, dm = rng.normal(size=m.n_elements), rng.normal(size=m.n_elements)
+    lhs = -p @ a.material_jvp(x, dl, dm)
+    rhs = term[:m.n_elements] @ dl + term[m.n_elements:] @ dm
+    assert abs(lhs - rhs) <= 1e-10 * abs(rhs)
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_mixed_precision_parity(order):
+    m, x, a, o, lam, mu = setup_case(order, 1, precision="mixed")
+    print(f"FixedCorotational mixed order={order}", flush=True)
+    assert rel(a.assemble_energy(x), o.energy(x)) < MIXED_TOL["energy"]
+    assert rel(a.assemble_gradient(x), o.gradient(x)) < MIXED_TOL["gradient"]
+    _, _, ref = o.hessian(x)
+    vals = a.hessian_values(x)
+    assert vals.dtype == np.float32
+    assert rel(vals, ref) < MIXED_TOL["hessian"]
+    v = np.random.default_rng(BASE_SEED + 141).normal(size=m.n_dofs)
+    assert rel(a.hessian_vector_product(x, v), o.hessian_scipy(x) @ v) < MIXED_TOL["hvp"]
+    p = np.random.default_rng(BASE_SEED + 142).normal(size=m.n_dofs)
+    assert rel(a.force_material_derivative(x, p), o.material_vjp(x, p)) < MIXED_TOL["vjp"]
+
+
+def test_large_rotation():
+    """A rigid rotation by 60 degrees on top of a small strain: the rotation
+    is removed exactly, so energy and gradient stay those of the small strain
+    (the corotational property), on the device as in the oracle."""
+    m = kuhn_box(2, 2, 2, order=2)
+    lam, mu = random_lame(m.n_elements, seed=BASE_SEED + 143)
+    x = smooth_displacement(m.nodes, 0.02, 1e-3, BASE_SEED + 144)
+    c, s = np.cos(np.pi / 3), np.sin(np.pi / 3)
+    Rz = np.array([[c, -s, 0], [s, c, 0], [0, 0, 1]])
+    y = ((m.nodes + x.reshape(-1, 3)) @ Rz.T - m.nodes).ravel()
+    a = NLAssembler(NAME, m, lame=(lam, mu), precision="fp64")
+    o = FcrOracle(m, lam, mu)
+    print("FixedCorotational large rotation", flush=True)
+    assert rel(a.assemble_energy(y), o.energy(y)) < FP64_TOL
+    assert rel(a.assemble_energy(y), a.assemble_energy(x)) < 1e-12
+    assert rel(a.assemble_gradient(y), o.gradient(y)) < FP64_TOL
+    # the rotated gradient is the small-strain gradient rotated
+    assert rel(a.assemble_gradient(y).reshape(-1, 3), a.assemble_gradient(x).reshape(-1, 3) @ Rz.T) < 1e-10
+    assert rel(a.hessian_values(y), o.hessian(y)[2]) < FP64_TOL
+
+
+def test_project_to_psd():
+    """Element Hessians projected onto the PSD cone against LAPACK eigh of the
+    oracle's element Hessians, on a compressed state."""
+    m = kuhn_box(2, 2, 2, order=1)
+    lam, mu = random_lame(m.n_elements, seed=BASE_SEED + 145)
+    x = smooth_displacement(m.nodes, 0.05, 1e-3, BASE_SEED + 146)
+    x[2::3] -= 0.45 * m.nodes[:, 2]
+    o = FcrOracle(m, lam, mu)
+    n = o.n_local
+    rows, cols, vals = [], [], []
+    for e in range(m.n_elements):
+        H = o.element_hessian(e, x)
+        d, V = np.linalg.eigh(H)
+        if d[0] < 0.0:
+            H = (V * np.where(d > 0.0, d, 0.0)) @ V.T
+        gi = (3 * m.elem_nodes[e][:, None] + np.arange(3)).reshape(-1)
+        rows.append(np.repeat(gi, 3 * n))
+        cols.append(np.tile(gi, 3 * n))
+        vals.append(H.reshape(-1))
+    _, _, ref = po.triplets_to_csc(m.n_dofs, np.concatenate(rows), np.concatenate(cols), np.concatenate(vals))
+    _, _, plain = o.hessian(x)
+    assert rel(plain, ref) > 1e-6
+    print("FixedCorotational PSD projection", flush=True)
+    a = NLAssembler(NAME, m, lame=(lam, mu), precision="fp64")
+    assert rel(a.hessian_values(x, project_to_psd=True), ref) < FP64_TOL
+
+
+def test_inverted_element_reports_invalid_state():
+    """det F <= 0: energy, gradient and Hessian values are reported as
+    INVALID_STATE (the polar factor of an inverted F is a reflection)."""
+    m = kuhn_box(2, 2, 2, order=1)
+    lam, mu = random_lame(m.n_elements, seed=BASE_SEED + 147)
+    a = NLAssembler(NAME, m, lame=(lam, mu), precision="fp64")
+    y = np.zeros(m.n_dofs)
+    n0 = m.elem_nodes[0]
+    y.reshape(-1, 3)[n0[3]] = 2.0 * (m.nodes[n0[0]] - m.nodes[n0[3]])
+    print("FixedCorotational inverted element", flush=True)
+    for call in (a.assemble_energy, a.assemble_gradient, a.hessian_values):
+        with pytest.raises(PfgpuError) as ei:
+            call(y)
+        assert ei.value.status == 3
+
+
+def test_plain_path_only():
+    from polyfem_amd import _ffi
+    m = kuhn_box(2, 2, 2, order=1)
+    a = NLAssembler(NAME, m, young=1e6, poisson=0.35, precision="mixed")
+    with pytest.raises(PfgpuError) as err:
+        a.set_path("fused")
+    assert err.value.status == _ffi.ERR_UNSUPPORTED
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_newton_converges(order):
+    """Newton on a clamped, pulled box: quadratic convergence, and the final
+    gradient is small in the oracle as well."""
+    m = kuhn_box(2, 2, 3, order=order)
+    bot, top = face_nodes(m, 2, 0.0), face_nodes(m, 2, 1.0)
+    fixed = np.concatenate([3 * bot[:, None] + np.arange(3), 3 * top[:, None] + np.arange(3)]).ravel()
+    fv = np.concatenate([np.zeros(3 * bot.size), np.tile([0.0, 0.0, 0.1], top.size)])
+    prob = NLProblem(m.n_dofs, fixed, fv)
+    lam, mu = random_lame(m.n_elements, seed=BASE_SEED + 148)
+    a = NLAssembler(NAME, m, lame=(lam, mu), precision="fp64")
+    print(f"FixedCorotational Newton order={order}", flush=True)
+    x0 = np.zeros(m.n_dofs)
+    x0[fixed] = fv
+    g0 = np.linalg.norm(a.assemble_gradient(x0)[prob.free])
+    x, rep = NewtonSolver(max_iterations=30, grad_norm=1e-9 * g0, cg_max_iterations=5000,
+                          cg_tolerance=1e-12).minimize(a, np.zeros(m.n_dofs), prob)
+    assert rep.converged and rep.iterations <= 10, rep
+    go = FcrOracle(m, lam, mu).gradient(x)
+    assert np.linalg.norm(go[prob.free]) <= 1e-8 * g0

@@ -496,3 +496,81 @@ def test_psd_projection_oracle(order):
     assert np.linalg.eigvalsh(H).min() < -1e-6 * np.abs(H).max()
     assert np.linalg.eigvalsh(P).min() > -1e-9 * np.abs(P).max()
     assert np.abs(P - P.T).max() <= 1e-12 * np.abs(P).max()
+
+
+# ------------------------------------------------------ FixedCorotational
+
+def test_fcr_stress_and_stiffness_are_derivatives():
+    """P = dpsi/dF and A = dP/dF (central FD), A symmetric."""
+    from oracle import fcr
+    rng = np.random.default_rng(BASE_SEED + 16)
+    h = 1e-6
+    for _ in range(4):
+        F = np.eye(3) + 0.2 * rng.normal(size=(3, 3))
+        P = fcr.stress(F, LAM, MU)
+        A = fcr.stiffness(F, LAM, MU)
+        for i in range(3):
+            for j in range(3):
+                d = np.zeros((3, 3))
+                d[i, j] = h
+                fd = (fcr.energy_density(F + d, LAM, MU) - fcr.energy_density(F - d, LAM, MU)) / (2 * h)
+                assert abs(fd - P[i, j]) <= 1e-7 * np.abs(P).max()
+                dP = (fcr.stress(F + d, LAM, MU) - fcr.stress(F - d, LAM, MU)) / (2 * h)
+                assert np.abs(dP.reshape(-1, order="F") - A[:, i + 3 * j]).max() <= 1e-7 * np.abs(A).max()
+        assert np.abs(A - A.T).max() <= 1e-12 * np.abs(A).max()
+
+
+def test_fcr_closed_forms():
+    """Rotations cost nothing; a uniaxial stretch diag(s, 1, 1) costs
+    mu (s-1)^2 + lambda/2 (s-1)^2; at rest the element Hessian is the linear
+    elasticity stiffness."""
+    from oracle import fcr
+    th = 0.8
+    R = np.array([[np.cos(th), 0, np.sin(th)], [0, 1, 0], [-np.sin(th), 0, np.cos(th)]])
+    assert abs(fcr.energy_density(R, LAM, MU)) < 1e-9 * MU
+    assert np.abs(fcr.stress(R, LAM, MU)).max() < 1e-9 * MU
+    s = 1.3
+    assert abs(fcr.energy_density(np.diag([s, 1, 1]), LAM, MU) - (MU + LAM / 2) * (s - 1) ** 2) < 1e-9 * MU
+    m = kuhn_box(1, 1, 1, order=2)
+    o = fcr.FcrOracle(m, LAM, MU)
+    ol = po.Oracle(m, LAM, MU, material=po.LINEAR_ELASTICITY)
+    x0 = np.zeros(m.n_dofs)
+    assert abs(o.hessian_scipy(x0) - ol.hessian_scipy(x0)).max() < 1e-8 * LAM
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_fcr_assembly_derivatives_vs_fd(order):
+    """Global gradient and Hessian against central FD of the global energy
+    and gradient; material and shape terms against FD of the gradient."""
+    from oracle import fcr
+    m = kuhn_box(2, 1, 1, order=order)
+    rng = np.random.default_rng(BASE_SEED + 17)
+    lam = LAM * np.exp(rng.normal(0, 0.3, m.n_elements))
+    mu = MU * np.exp(rng.normal(0, 0.3, m.n_elements))
+    o = fcr.FcrOracle(m, lam, mu)
+    x = smooth_displacement(m.nodes, 0.05, 1e-3, BASE_SEED + 18)
+    d = rng.normal(size=m.n_dofs)
+    h = 1e-6
+    fd = (o.energy(x + h * d) - o.energy(x - h * d)) / (2 * h)
+    g = o.gradient(x)
+    assert abs(fd - g @ d) <= 1e-6 * abs(fd)
+    fdg = (o.gradient(x + h * d) - o.gradient(x - h * d)) / (2 * h)
+    assert np.abs(o.hessian_scipy(x) @ d - fdg).max() <= 1e-6 * np.abs(fdg).max()
+    p = rng.normal(size=m.n_dofs)
+    term = o.material_vjp(x, p)
+    dl, dm = rng.normal(size=m.n_elements), rng.normal(size=m.n_elements)
+    gp = fcr.FcrOracle(m, lam + h * dl * LAM, mu + h * dm * MU).gradient(x)
+    gm = fcr.FcrOracle(m, lam - h * dl * LAM, mu - h * dm * MU).gradient(x)
+    fdm = -p @ (gp - gm) / (2 * h)
+    assert abs(term[:m.n_elements] @ (dl * LAM) + term[m.n_elements:] @ (dm * MU) - fdm) <= 1e-6 * abs(fdm)
+    shape = o.shape_vjp(x, p)
+    dX = rng.normal(size=m.vertices.shape) * 1e-2
+
+    def g_at(sg):
+        m2 = kuhn_box(2, 1, 1, order=order)
+        m2.vertices = m.vertices + sg * dX
+        return fcr.FcrOracle(m2, lam, mu).gradient(x)
+
+    hs = 1e-5
+    fds = p @ (g_at(hs) - g_at(-hs)) / (2 * hs)
+    assert abs(shape @ dX.reshape(-1) - fds) <= 1e-5 * max(abs(fds), 1e-12)
