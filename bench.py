@@ -498,7 +498,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--path", default="auto", choices=["auto", "fused", "plain"],
+    ap.add_argument("--path", default="auto", choices=["auto", "fused", "plain", "factored"],
                     help="auto: the launch-shape autotuner picks path and workgroup sizes (untimed setup)")
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
     ap.add_argument("--grid", type=int, default=40)

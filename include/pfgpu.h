@@ -67,9 +67,13 @@ typedef enum { PFGPU_FP64 = 0, PFGPU_MIXED = 1 } pfgpu_precision;
 /* Assembly algorithm.  PLAIN (the default): element-local results written
  * to HBM, then gathered into the global vector / CSR in a fixed order.
  * FUSED: patch-owned rows accumulated on chip (LDS) in a fixed order, one
- * persistent launch.  Both are deterministic and give the same results to
- * rounding; PLAIN is the faster one on MI355X at the benchmarked sizes. */
-typedef enum { PFGPU_PATH_FUSED = 0, PFGPU_PATH_PLAIN = 1 } pfgpu_path;
+ * persistent launch.  FACTORED: PLAIN, except that the Hessian's element
+ * blocks are not stored: per-element factors (per point F^{-T} J^{-T} and two
+ * curvature weights, and the reference metric) are written instead and the
+ * CSR gather recomputes each block from them.  All three are deterministic
+ * and give the same results to rounding.  FUSED and FACTORED cover
+ * NeoHookean and linear elasticity. */
+typedef enum { PFGPU_PATH_FUSED = 0, PFGPU_PATH_PLAIN = 1, PFGPU_PATH_FACTORED = 2 } pfgpu_path;
 
 typedef struct {
     int32_t order;            /* 1 (P1) or 2 (P2) */
@@ -117,9 +121,10 @@ pfgpu_status pfgpu_set_path(pfgpu_handle *h, int32_t path);
 /* Launch-shape autotuner (no reference counterpart): times every candidate
  * configuration of the energy + gradient + Hessian assembly on this handle's
  * mesh at state x (device pointer; 3 timed runs each, after one warm-up) and
- * keeps the fastest.  Candidates: the fused path, and the plain path with
+ * keeps the fastest.  Candidates: the fused path, the plain path with
  * block-gather workgroups of 128/256/512 threads and Hessian-kernel
- * workgroups of 64/128/256 threads.  All candidates give identical results
+ * workgroups of 64/128/256 threads, and the factored path with gather
+ * workgroups of 128/256/512 threads.  All candidates give identical results
  * up to the summation order of the path. */
 typedef struct {
     int32_t path, gather_threads, hessian_threads; /* chosen configuration */
@@ -127,6 +132,7 @@ typedef struct {
     double ms;                                     /* best time, E+G+H */
     double ms_plain_default;                       /* plain, 256/256 */
     double ms_fused;                               /* fused (negative: not available) */
+    double ms_factored;                            /* best factored (negative: not available) */
 } pfgpu_tune_report;
 pfgpu_status pfgpu_autotune(pfgpu_handle *h, const double *x, pfgpu_tune_report *report);
 /* sets a configuration directly (gather_threads in {128,256,512},

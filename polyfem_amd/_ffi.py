@@ -16,7 +16,7 @@ STATUS_NAMES = {0: "OK", 1: "INVALID_ARGUMENT", 2: "HIP", 3: "INVALID_STATE", 4:
 OK, ERR_INVALID_ARGUMENT, ERR_HIP, ERR_INVALID_STATE, ERR_UNSUPPORTED, ERR_OUT_OF_MEMORY = range(6)
 NEOHOOKEAN, LINEAR_ELASTICITY, SAINT_VENANT, FIXED_COROTATIONAL = 0, 1, 2, 3
 FP64, MIXED = 0, 1
-PATH_FUSED, PATH_PLAIN = 0, 1
+PATH_FUSED, PATH_PLAIN, PATH_FACTORED = 0, 1, 2
 
 
 class MeshDesc(C.Structure):
@@ -61,7 +61,7 @@ class TransientStats(C.Structure):
 class TuneReport(C.Structure):
     _fields_ = [("path", C.c_int32), ("gather_threads", C.c_int32), ("hessian_threads", C.c_int32),
                 ("candidates", C.c_int32), ("ms", C.c_double), ("ms_plain_default", C.c_double),
-                ("ms_fused", C.c_double)]
+                ("ms_fused", C.c_double), ("ms_factored", C.c_double)]
 
 
 class PfgpuError(RuntimeError):

@@ -25,7 +25,7 @@ from ._ffi import check, lib
 _NAMES = {"NeoHookean": _ffi.NEOHOOKEAN, "LinearElasticity": _ffi.LINEAR_ELASTICITY, "SaintVenant": _ffi.SAINT_VENANT,
           "FixedCorotational": _ffi.FIXED_COROTATIONAL}
 _PRECISIONS = {"fp64": _ffi.FP64, "mixed": _ffi.MIXED}
-_PATHS = {"fused": _ffi.PATH_FUSED, "plain": _ffi.PATH_PLAIN}
+_PATHS = {"fused": _ffi.PATH_FUSED, "plain": _ffi.PATH_PLAIN, "factored": _ffi.PATH_FACTORED}
 
 
 def _ptr(a: np.ndarray):
@@ -114,7 +114,8 @@ class NLAssembler:
         self.path = {v: k for k, v in _PATHS.items()}[r.path]
         return {"path": self.path, "gather_threads": r.gather_threads, "hessian_threads": r.hessian_threads,
                 "candidates": r.candidates, "ms": r.ms, "ms_plain_default": r.ms_plain_default,
-                "ms_fused": r.ms_fused if r.ms_fused >= 0 else None}
+                "ms_fused": r.ms_fused if r.ms_fused >= 0 else None,
+                "ms_factored": r.ms_factored if r.ms_factored >= 0 else None}
 
     def set_tuning(self, path: str, gather_threads: int = 256, hessian_threads: int = 256):
         check(lib().pfgpu_set_tuning(self._h, _PATHS[path], gather_threads, hessian_threads))

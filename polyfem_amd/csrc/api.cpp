@@ -140,6 +140,10 @@ void egh_dev(pfgpu_handle &h, const double *x, double *energy, double *g, void *
         fused_energy_gradient_hessian(h, x, energy, g, values);
         return;
     }
+    if (h.path == PFGPU_PATH_FACTORED && g && values) {
+        factored_energy_gradient_hessian(h, x, energy, g, values);
+        return;
+    }
     if (g)
         plain_gradient(h, x, g, energy);
     else if (energy)
@@ -172,7 +176,8 @@ pfgpu_handle::~pfgpu_handle()
 int64_t pfgpu_handle::device_bytes() const
 {
     const DevBuf *bufs[] = {&conn, &vconn, &geom, &lame, &node_ptr, &node_inc, &vert_ptr, &vert_inc,
-                            &brow_ptr, &block_row, &contrib_ptr, &contrib, &row_chunk, &blk_chunk, &patch_desc, &patch_rows,
+                            &brow_ptr, &blk_out, &blk_plan.chunk, &blk_plan.slot, &blk_plan.meta, &fac_plan.chunk,
+                            &fac_plan.slot, &fac_plan.meta, &row_chunk, &patch_desc, &patch_rows, &fac_rec,
                             &patch_row_acc, &patch_elems, &patch_cs, &patch_cg, &patch_hdr, &patch_pos, &patch_vnodes,
                             &d_outer, &d_inner, &loc, &hloc, &partial, &red_out, &sum_part,
                             &h_x, &h_v, &h_out, &h_out2, &h_vals, &sv, &h_mask, &mass, &pat_bcol, &tr_fixed, &tr_f, &tr_seg, &load};
@@ -290,12 +295,12 @@ pfgpu_status pfgpu_set_tuning(pfgpu_handle *h, int32_t path, int32_t gather_thre
 {
     return guard([&] {
         check_handle(h);
-        require(path == PFGPU_PATH_FUSED || path == PFGPU_PATH_PLAIN, "unknown path");
+        require(path == PFGPU_PATH_FUSED || path == PFGPU_PATH_PLAIN || path == PFGPU_PATH_FACTORED, "unknown path");
         require(gather_threads == 128 || gather_threads == 256 || gather_threads == 512, "gather_threads must be 128, 256 or 512");
         require(hessian_threads == 64 || hessian_threads == 128 || hessian_threads == 256,
                 "hessian_threads must be 64, 128 or 256");
-        if (path == PFGPU_PATH_FUSED && (h->material == PFGPU_SAINT_VENANT || h->material == PFGPU_FIXED_COROTATIONAL))
-            fail(PFGPU_ERR_UNSUPPORTED, "the fused path covers NeoHookean and linear elasticity");
+        if (path != PFGPU_PATH_PLAIN && (h->material == PFGPU_SAINT_VENANT || h->material == PFGPU_FIXED_COROTATIONAL))
+            fail(PFGPU_ERR_UNSUPPORTED, "the fused and factored paths cover NeoHookean and linear elasticity");
         h->path = path;
         h->tune_gather_threads = gather_threads;
         h->tune_hessian_threads = hessian_threads;
@@ -331,6 +336,7 @@ pfgpu_status pfgpu_autotune(pfgpu_handle *h, const double *x, pfgpu_tune_report 
         pfgpu_tune_report r{};
         r.ms = 1e300;
         r.ms_fused = -1;
+        r.ms_factored = -1;
         const int G[3] = {128, 256, 512}, HB[3] = {64, 128, 256};
         try {
             for (int gi = 0; gi < 3; ++gi)
@@ -346,6 +352,26 @@ pfgpu_status pfgpu_autotune(pfgpu_handle *h, const double *x, pfgpu_tune_report 
                         r.path = PFGPU_PATH_PLAIN;
                         r.gather_threads = G[gi];
                         r.hessian_threads = HB[hi];
+                    }
+                }
+            // the factored path keeps the plain winner's Hessian-kernel size
+            // (used by its PSD-projected assemblies)
+            h->tune_hessian_threads = r.hessian_threads;
+            if (h->material == PFGPU_NEOHOOKEAN || h->material == PFGPU_LINEAR_ELASTICITY)
+                for (int gi = 0; gi < 3; ++gi) {
+                    h->path = PFGPU_PATH_FACTORED;
+                    h->tune_gather_threads = G[gi];
+                    try {
+                        const double ms = time_config();
+                        r.candidates++;
+                        if (r.ms_factored < 0 || ms < r.ms_factored) r.ms_factored = ms;
+                        if (ms < r.ms) {
+                            r.ms = ms;
+                            r.path = PFGPU_PATH_FACTORED;
+                            r.gather_threads = G[gi];
+                        }
+                    } catch (const Error &e) {
+                        if (e.status != PFGPU_ERR_UNSUPPORTED) throw;
                     }
                 }
             h->path = PFGPU_PATH_FUSED;
@@ -377,9 +403,9 @@ pfgpu_status pfgpu_set_path(pfgpu_handle *h, int32_t path)
 {
     return guard([&] {
         check_handle(h);
-        require(path == PFGPU_PATH_FUSED || path == PFGPU_PATH_PLAIN, "unknown path");
-        if (path == PFGPU_PATH_FUSED && (h->material == PFGPU_SAINT_VENANT || h->material == PFGPU_FIXED_COROTATIONAL))
-            fail(PFGPU_ERR_UNSUPPORTED, "the fused path covers NeoHookean and linear elasticity");
+        require(path == PFGPU_PATH_FUSED || path == PFGPU_PATH_PLAIN || path == PFGPU_PATH_FACTORED, "unknown path");
+        if (path != PFGPU_PATH_PLAIN && (h->material == PFGPU_SAINT_VENANT || h->material == PFGPU_FIXED_COROTATIONAL))
+            fail(PFGPU_ERR_UNSUPPORTED, "the fused and factored paths cover NeoHookean and linear elasticity");
         h->path = path;
     });
 }

@@ -75,6 +75,14 @@ class DevBuf {
     size_t n_ = 0;
 };
 
+// Chunked gather of the Hessian's pattern blocks for workgroups of `threads`
+// threads: chunk starts (n_chunks + 1 blocks), threads contribution slots per
+// chunk, per-block metadata (the kernels document the layout).
+struct GatherPlan {
+    DevBuf chunk, slot, meta;
+    int n_chunks = 0, threads = 0;
+};
+
 // v into dst: in place when the size matches (device pointers held elsewhere,
 // e.g. by captured graphs, stay valid), else into a new allocation
 template <typename T> void upload_into(DevBuf &dst, const std::vector<T> &v, hipStream_t s);

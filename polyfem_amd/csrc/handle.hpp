@@ -19,13 +19,18 @@ struct pfgpu_handle {
     pfgpu::DevBuf conn, vconn, geom, lame;
     // node / vertex incidence (gathers of element-local vectors)
     pfgpu::DevBuf node_ptr, node_inc, vert_ptr, vert_inc;
-    // plain-path Hessian maps
+    // plain-path Hessian maps: the contributions of every pattern block
+    // (host copies, for the gather plans), every block's first CSR value,
+    // and one gather plan per Hessian route (built for one workgroup size)
     bool plain_hess_ready = false;
-    pfgpu::DevBuf brow_ptr, block_row, contrib_ptr, contrib;
-    // chunks of the segmented gathers (node rows; pattern blocks)
-    pfgpu::DevBuf row_chunk, blk_chunk;
-    int n_row_chunks = 0, n_blk_chunks = 0, blk_chunk_threads = 0;
-    std::vector<int32_t> h_contrib_ptr; // host copy: re-chunking for another gather size
+    pfgpu::DevBuf brow_ptr, blk_out;
+    std::vector<int32_t> h_contrib_ptr, h_contrib_code;
+    pfgpu::GatherPlan blk_plan, fac_plan;
+    // chunks of the segmented node-row gather
+    pfgpu::DevBuf row_chunk;
+    int n_row_chunks = 0;
+    // factored Hessian (PFGPU_PATH_FACTORED): per-element factor records
+    pfgpu::DevBuf fac_rec;
     // launch shapes of the plain path (set by pfgpu_autotune / pfgpu_set_tuning)
     int tune_gather_threads = 256, tune_hessian_threads = 256;
     // fused-path patch tables (built on first use)
@@ -104,6 +109,7 @@ void element_material_vjp(Handle &h, const double *x, const double *p, double *o
 void energy_lame_gradient(Handle &h, const double *x, double *out);
 void plain_shape_vjp(Handle &h, const double *x, const double *p, double *out);
 void ensure_plain_hessian_maps(Handle &h);
+void factored_energy_gradient_hessian(Handle &h, const double *x, double *energy, double *g, void *values);
 
 // ---- fused path (fused.hip) -------------------------------------------
 void ensure_fused_tables(Handle &h);

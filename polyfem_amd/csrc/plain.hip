@@ -50,6 +50,27 @@ template <typename T, int ORDER, int NQ> __device__ __forceinline__ const RefTab
 
 // ------------------------------------------------------------ element kernels
 
+// Hessian factor record of the factored route (see k_hfactors): values per element
+__host__ __device__ constexpr int fac_size(int nq) { return (11 * nq + 6 + 3) & ~3; }
+
+// the point-independent tail of a factor record, and the record's store
+template <typename T, int NQ>
+__device__ __forceinline__ void finish_factors(const T *jit, T mu, T w, T (&r)[fac_size(NQ)], T *__restrict__ rec, int e)
+{
+    constexpr int R = fac_size(NQ);
+    T g6[6];
+    metric(jit, g6);
+    const T cm = w * mu;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) r[11 * NQ + i] = cm * g6[i];
+#pragma unroll
+    for (int i = 11 * NQ + 6; i < R; ++i) r[i] = T(0);
+    // records are R values (a multiple of 16 bytes) from a 256-byte aligned base
+    T *out = static_cast<T *>(__builtin_assume_aligned(rec, 16)) + int64_t(e) * R;
+#pragma unroll
+    for (int i = 0; i < R; ++i) out[i] = r[i];
+}
+
 template <typename T, int ORDER, int NQ, int MAT>
 __global__ __launch_bounds__(BLOCK) void k_energy(DevMesh<T> m, const double *__restrict__ x,
                                                   double *__restrict__ per_elem, double *__restrict__ partial)
@@ -102,10 +123,12 @@ __global__ __launch_bounds__(BLOCK) void k_energy_lame_grad(DevMesh<T> m, const 
     out[m.n_elements + e] = double(am);
 }
 
-// local gradient (+ energy partials); gloc[(e*NB + a)*3 + d]
-template <typename T, int ORDER, int NQ, int MAT>
+// local gradient (+ energy partials); gloc[(e*NB + a)*3 + d].  FAC: also the
+// Hessian factor record of the element (k_hfactors' record), from the same
+// kinematics
+template <typename T, int ORDER, int NQ, int MAT, bool FAC = false>
 __global__ __launch_bounds__(BLOCK) void k_gradient(DevMesh<T> m, const double *__restrict__ x, T *__restrict__ gloc,
-                                                    double *__restrict__ partial)
+                                                    double *__restrict__ partial, T *__restrict__ rec = nullptr)
 {
     constexpr int NB = NLocal<ORDER>::value;
     const int e = blockIdx.x * BLOCK + threadIdx.x;
@@ -121,6 +144,7 @@ __global__ __launch_bounds__(BLOCK) void k_gradient(DevMesh<T> m, const double *
 #pragma unroll
             for (int d = 0; d < 3; ++d) g[a][d] = T(0);
         T acc = T(0);
+        T r[FAC ? fac_size(NQ) : 1];
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
             Kin<T> k;
@@ -130,7 +154,9 @@ __global__ __launch_bounds__(BLOCK) void k_gradient(DevMesh<T> m, const double *
             T P[9];
             pk1<T, MAT>(k, el.lam, el.mu, P);
             contract_add<T, ORDER, NQ>(q, P, el.jit, w, g);
+            if constexpr (FAC) hess_qp<T, MAT>(k, el.jit, el.lam, el.mu, w, r + 9 * q, r[9 * NQ + q], r[10 * NQ + q]);
         }
+        if constexpr (FAC) finish_factors<T, NQ>(el.jit, el.mu, w, r, rec, e);
         val = double(acc);
         T *out = gloc + int64_t(e) * NB * 3;
 #pragma unroll
@@ -606,52 +632,178 @@ __global__ __launch_bounds__(BLOCK) void k_gather_rows_seg(int n_chunks, const i
     }
 }
 
-// src[k]: index of contribution k's block in hloc (bit 31: stored transposed)
+// Gather plan of a workgroup size GS (built on the host, see build_plan):
+// chunk c covers the pattern blocks chunk[c] .. chunk[c+1]-1 and holds at
+// most GS contributions, stored in the slots c GS .. (c+1) GS - 1 (-1 when
+// unused); out[b] = 9 r0 + 3 (b - r0) is block b's first value (r0 the first
+// block of its row) and meta[b] = q0 | cnt << 10 | deg << 20 gives its
+// contributions' first slot in the chunk, their count and the blocks of its
+// row.  Every load the workgroup needs is at most two levels deep (the
+// slot's contribution, then its data; the chunk's first block, then the
+// block's out and meta), and both chains are issued before the barrier.
+
+// block b: the sum of its staged contributions, into the CSR values
+template <typename T>
+__device__ __forceinline__ void sum_block(const T *sv, int32_t off, int32_t mt, T *__restrict__ values)
+{
+    const int q0 = mt & 1023, cnt = (mt >> 10) & 1023, deg = int(uint32_t(mt) >> 20);
+    T acc[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) acc[i] = T(0);
+    for (int k = q0; k < q0 + cnt; ++k)
+#pragma unroll
+        for (int i = 0; i < 9; ++i) acc[i] += sv[k * 9 + i];
+    T *out = values + off;
+#pragma unroll
+    for (int mm = 0; mm < 3; ++mm)
+#pragma unroll
+        for (int nn = 0; nn < 3; ++nn) out[mm * 3 * deg + nn] = acc[mm * 3 + nn];
+}
+
+// slot word: index of the contribution's block in hloc, bit 31 when the
+// block is stored transposed (the index is below 2^31 - 1, so -1 is free)
 template <typename T, int GS>
 __global__ __launch_bounds__(GS) void k_gather_blocks_seg(int n_chunks, const int32_t *__restrict__ chunk,
-                                                             const int32_t *__restrict__ brow_ptr,
-                                                             const int32_t *__restrict__ block_row,
-                                                             const int32_t *__restrict__ cptr, const int32_t *__restrict__ src,
+                                                             const int32_t *__restrict__ slot,
+                                                             const int32_t *__restrict__ bout,
+                                                             const int32_t *__restrict__ meta,
                                                              const T *__restrict__ hloc, T *__restrict__ values)
 {
     __shared__ T sv[GS * 9];
     const int c = xcd_chunk(n_chunks);
     if (c >= n_chunks) return;
-    const int b0 = chunk[c], b1 = chunk[c + 1];
-    const int k0 = cptr[b0], k1 = cptr[b1];
     const int t = threadIdx.x;
-    if (k0 + t < k1) {
-        const int32_t c = src[k0 + t];
-        const T *p = hloc + int64_t(c & 0x7fffffff) * 9;
-        T h[9];
+    const int32_t cs = slot[int64_t(c) * GS + t];
+    const int b0 = chunk[c], b1 = chunk[c + 1];
+    T h[9];
+    if (cs != -1) {
+        const T *p = hloc + int64_t(cs & 0x7fffffff) * 9;
 #pragma unroll
         for (int i = 0; i < 9; ++i) h[i] = p[i];
-        if (c >= 0) {
-#pragma unroll
-            for (int i = 0; i < 9; ++i) sv[t * 9 + i] = h[i];
-        } else {
-#pragma unroll
-            for (int mm = 0; mm < 3; ++mm)
-#pragma unroll
-                for (int nn = 0; nn < 3; ++nn) sv[t * 9 + mm * 3 + nn] = h[nn * 3 + mm];
-        }
     }
-    __syncthreads();
-    for (int b = b0 + t; b < b1; b += GS) {
-        T acc[9];
+    const int bq = b0 + t;
+    int32_t off = 0, mt = 0;
+    if (bq < b1) {
+        off = bout[bq];
+        mt = meta[bq];
+    }
+    if (cs >= 0) {
 #pragma unroll
-        for (int i = 0; i < 9; ++i) acc[i] = T(0);
-        for (int k = cptr[b] - k0; k < cptr[b + 1] - k0; ++k)
-#pragma unroll
-            for (int i = 0; i < 9; ++i) acc[i] += sv[k * 9 + i];
-        const int row = block_row[b];
-        const int r0 = brow_ptr[row], deg = brow_ptr[row + 1] - r0;
-        T *out = values + 9 * int64_t(r0) + 3 * (b - r0);
+        for (int i = 0; i < 9; ++i) sv[t * 9 + i] = h[i];
+    } else if (cs != -1) {
 #pragma unroll
         for (int mm = 0; mm < 3; ++mm)
 #pragma unroll
-            for (int nn = 0; nn < 3; ++nn) out[int64_t(mm) * 3 * deg + nn] = acc[mm * 3 + nn];
+            for (int nn = 0; nn < 3; ++nn) sv[t * 9 + mm * 3 + nn] = h[nn * 3 + mm];
     }
+    __syncthreads();
+    if (bq < b1) sum_block(sv, off, mt, values);
+    // blocks without contributions (none in a pattern built from elements)
+    for (int bb = bq + GS; bb < b1; bb += GS) sum_block(sv, bout[bb], meta[bb], values);
+}
+
+// ------------------------------------------------------------ factored Hessian
+//
+// With k_a = M_q RG(q, a), M_q = F^{-T} J^{-T} (J^{-T} for linear
+// elasticity), the NeoHookean / linear-elasticity block is
+//   H_e(a, b) = (cm sum_q G_a . G_b) I + sum_q [cl_q k_a k_b^T + cc_q k_b k_a^T],
+// the form k_hessian evaluates.  k_hfactors writes per element the record
+//   [M_q (9 per point) | cl_q | cc_q | cm g6 (the reference metric, 6)],
+// padded to a multiple of 4 values, and k_gather_factored forms each
+// contribution's block from its element's record where k_gather_blocks_seg
+// reads a stored block: 11 NQ + 6 values per element travel through HBM
+// instead of the 9 n(n+1)/2 of the element blocks, and the lanes of a
+// wavefront that share an element read its record through the same cache
+// lines.  (Records holding every node's k vectors instead take less
+// arithmetic in the gather but cost more in their strided writes.)
+template <typename T, int ORDER, int NQ, int MAT>
+__global__ __launch_bounds__(BLOCK) void k_hfactors(DevMesh<T> m, const double *__restrict__ x, T *__restrict__ rec)
+{
+    constexpr int R = fac_size(NQ);
+    const int e = blockIdx.x * BLOCK + threadIdx.x;
+    if (e >= m.n_elements) return;
+    Elem<T, ORDER, NQ> el;
+    load_geometry(m, e, el);
+    if (MAT != MAT_LE) gather(m, e, x, el.U);
+    const T w = el.det * T(quad_weight(NQ));
+    T r[R];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        Kin<T> k;
+        if (MAT != MAT_LE) field_grad<T, ORDER, NQ>(q, el.U, el.jit, k.H);
+        if (MAT == MAT_NH) kinematics(k);
+        hess_qp<T, MAT>(k, el.jit, el.lam, el.mu, w, r + 9 * q, r[9 * NQ + q], r[10 * NQ + q]);
+    }
+    finish_factors<T, NQ>(el.jit, el.mu, w, r, rec, e);
+}
+
+// gather plan as for k_gather_blocks_seg; slot word e NB^2 + a NB + b
+template <typename T, int ORDER, int NQ, int GS>
+__global__ __launch_bounds__(GS) void k_gather_factored(int n_chunks, const int32_t *__restrict__ chunk,
+                                                        const int32_t *__restrict__ slot,
+                                                        const int32_t *__restrict__ bout,
+                                                        const int32_t *__restrict__ meta, const T *__restrict__ rec,
+                                                        T *__restrict__ values)
+{
+    constexpr int NB = NLocal<ORDER>::value, R = fac_size(NQ);
+    __shared__ T sv[GS * 9];
+    __shared__ T srg[NQ * NB * 3], srgg[NB * NB * 6];
+    const int c = xcd_chunk(n_chunks);
+    if (c >= n_chunks) return;
+    const int t = threadIdx.x;
+    const int32_t cd = slot[int64_t(c) * GS + t];
+    const int b0 = chunk[c], b1 = chunk[c + 1];
+    // this thread's contribution: its element's record
+    int a = 0, b = 0;
+    T r[R];
+    if (cd >= 0) {
+        const int e = cd / (NB * NB), ab = cd - e * (NB * NB);
+        a = ab / NB;
+        b = ab - a * NB;
+        const T *rp = static_cast<const T *>(__builtin_assume_aligned(rec, 16)) + int64_t(e) * R;
+#pragma unroll
+        for (int i = 0; i < R; ++i) r[i] = rp[i];
+    }
+    // this thread's output block
+    const int bq = b0 + t;
+    int32_t off = 0, mt = 0;
+    if (bq < b1) {
+        off = bout[bq];
+        mt = meta[bq];
+    }
+    const RefTab<T, ORDER, NQ> &tb = ref_tab<T, ORDER, NQ>();
+    for (int i = t; i < NQ * NB * 3; i += GS) srg[i] = tb.rg[i];
+    for (int i = t; i < NB * NB * 6; i += GS) srgg[i] = tb.rgg[i];
+    __syncthreads();
+    if (cd >= 0) {
+        const T *cr = srgg + (a * NB + b) * 6;
+        const T gd = r[11 * NQ] * cr[0] + r[11 * NQ + 1] * cr[1] + r[11 * NQ + 2] * cr[2] + r[11 * NQ + 3] * cr[3] +
+                     r[11 * NQ + 4] * cr[4] + r[11 * NQ + 5] * cr[5];
+        T v[9];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) v[i] = i % 4 == 0 ? gd : T(0);
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            const T *M = r + 9 * q, *ra = srg + (q * NB + a) * 3, *rb = srg + (q * NB + b) * 3;
+            T ka[3], kb[3];
+#pragma unroll
+            for (int mm = 0; mm < 3; ++mm) {
+                ka[mm] = M[mm * 3] * ra[0] + M[mm * 3 + 1] * ra[1] + M[mm * 3 + 2] * ra[2];
+                kb[mm] = M[mm * 3] * rb[0] + M[mm * 3 + 1] * rb[1] + M[mm * 3 + 2] * rb[2];
+            }
+            const T cl = r[9 * NQ + q], cc = r[10 * NQ + q];
+#pragma unroll
+            for (int mm = 0; mm < 3; ++mm)
+#pragma unroll
+                for (int nn = 0; nn < 3; ++nn) v[mm * 3 + nn] += cl * ka[mm] * kb[nn] + cc * kb[mm] * ka[nn];
+        }
+#pragma unroll
+        for (int i = 0; i < 9; ++i) sv[t * 9 + i] = v[i];
+    }
+    __syncthreads();
+    if (bq < b1) sum_block(sv, off, mt, values);
+    // blocks without contributions (none in a pattern built from elements)
+    for (int bb = bq + GS; bb < b1; bb += GS) sum_block(sv, bout[bb], meta[bb], values);
 }
 
 } // namespace
@@ -728,51 +880,144 @@ void gather_rows(Handle &h, const void *loc, double *out)
     PF_LAUNCH_CHECK();
 }
 
+// Contributions of every pattern block in element order (shared by the
+// block and factored gathers): offsets, the element-local codes
+// e NB^2 + a NB + b, and every block's first CSR value.
 void ensure_plain_hessian_maps(Handle &h)
 {
     if (h.plain_hess_ready) return;
+    if (9 * h.pat.n_blocks >= (int64_t(1) << 31))
+        fail(PFGPU_ERR_UNSUPPORTED, "too many Hessian values for 32-bit value offsets");
     std::vector<int64_t> ptr64;
-    std::vector<int32_t> contrib, block_row;
-    build_block_contributions(h.pat, h.n_local, h.h_elem_nodes.data(), ptr64, contrib, block_row);
-    std::vector<int32_t> ptr(ptr64.begin(), ptr64.end());
-    // contribution (e, a, b) -> index of block s(min(a,b), max(a,b)) of element
-    // e in the tiled hloc layout of k_hessian; bit 31 when a > b (transposed)
-    {
-        const int NB = h.n_local, NBLK = h.n_blocks_sym();
-        if (int64_t((h.n_elements + 63) / 64) * NBLK * 64 >= (int64_t(1) << 31))
-            fail(PFGPU_ERR_UNSUPPORTED, "too many local Hessian blocks for 31-bit indices");
-        for (size_t k = 0; k < contrib.size(); ++k) {
-            const int32_t c = contrib[k];
-            const int e = c / (NB * NB), a = (c / NB) % NB, b = c % NB;
-            const int lo = std::min(a, b), hi = std::max(a, b);
-            const int s = lo * NB - lo * (lo - 1) / 2 + (hi - lo);
-            const int32_t idx = int32_t((int64_t(e / 64) * NBLK + s) * 64 + e % 64);
-            contrib[k] = a > b ? int32_t(uint32_t(idx) | 0x80000000u) : idx;
-        }
+    std::vector<int32_t> block_row;
+    build_block_contributions(h.pat, h.n_local, h.h_elem_nodes.data(), ptr64, h.h_contrib_code, block_row);
+    h.h_contrib_ptr.assign(ptr64.begin(), ptr64.end());
+    std::vector<int32_t> bout(size_t(h.pat.n_blocks));
+    for (int i = 0; i < h.n_nodes; ++i) {
+        const int32_t r0 = h.pat.brow_ptr[i];
+        for (int32_t b = r0; b < h.pat.brow_ptr[i + 1]; ++b) bout[size_t(b)] = 9 * r0 + 3 * (b - r0);
     }
-    h.contrib_ptr = upload(ptr, h.stream);
-    h.contrib = upload(contrib, h.stream);
-    h.h_contrib_ptr = std::move(ptr);
-    h.blk_chunk_threads = 0;
-    h.block_row = upload(block_row, h.stream);
+    h.blk_out = upload(bout, h.stream);
     if (!h.brow_ptr.get()) h.brow_ptr = upload(h.pat.brow_ptr, h.stream);
+    h.blk_plan = GatherPlan{};
+    h.fac_plan = GatherPlan{};
     h.plain_hess_ready = true;
 }
 
-// chunks of the block gather for its workgroup size (rebuilt when the tuning changes)
-void ensure_block_chunks(Handle &h)
+// the gather plan of workgroup size gs for slot words word(k) of the
+// contributions k (layout described at k_gather_blocks_seg)
+template <typename W> void build_plan(Handle &h, GatherPlan &p, int gs, W &&word)
 {
-    if (h.blk_chunk_threads == h.tune_gather_threads) return;
-    const std::vector<int32_t> c = segment_chunks(h.h_contrib_ptr.data(), h.pat.n_blocks, h.tune_gather_threads);
-    h.blk_chunk = upload(c, h.stream);
-    h.n_blk_chunks = int(c.size()) - 1;
-    h.blk_chunk_threads = h.tune_gather_threads;
+    if (p.threads == gs) return;
+    const std::vector<int32_t> &ptr = h.h_contrib_ptr;
+    const std::vector<int32_t> c = segment_chunks(ptr.data(), h.pat.n_blocks, gs);
+    const int nc = int(c.size()) - 1;
+    std::vector<int32_t> slot(size_t(nc) * gs, -1), meta(size_t(h.pat.n_blocks));
+    for (int i = 0; i < h.n_nodes; ++i) {
+        const int32_t deg = h.pat.brow_ptr[i + 1] - h.pat.brow_ptr[i];
+        if (deg > 4095) fail(PFGPU_ERR_UNSUPPORTED, "a node has more than 4095 neighbours");
+        for (int32_t b = h.pat.brow_ptr[i]; b < h.pat.brow_ptr[i + 1]; ++b) meta[size_t(b)] = deg << 20;
+    }
+    for (int ci = 0; ci < nc; ++ci) {
+        const int32_t k0 = ptr[size_t(c[ci])];
+        for (int32_t b = c[ci]; b < c[ci + 1]; ++b) {
+            meta[size_t(b)] |= (ptr[size_t(b)] - k0) | ((ptr[size_t(b) + 1] - ptr[size_t(b)]) << 10);
+            for (int32_t k = ptr[size_t(b)]; k < ptr[size_t(b) + 1]; ++k) slot[size_t(ci) * gs + size_t(k - k0)] = word(k);
+        }
+    }
+    p.chunk = upload(c, h.stream);
+    p.slot = upload(slot, h.stream);
+    p.meta = upload(meta, h.stream);
+    p.n_chunks = nc;
+    p.threads = gs;
+}
+
+void factored_hessian(Handle &h, const double *x, void *values)
+{
+    if (h.material != MAT_NH && h.material != MAT_LE)
+        fail(PFGPU_ERR_UNSUPPORTED, "the factored Hessian covers NeoHookean and linear elasticity");
+    ensure_plain_hessian_maps(h);
+    build_plan(h, h.fac_plan, h.tune_gather_threads, [&](int32_t k) { return h.h_contrib_code[size_t(k)]; });
+    const GatherPlan &p = h.fac_plan;
+    dispatch(h, [&]<typename T, int O, int Q, int M>() {
+        if constexpr (M == MAT_NH || M == MAT_LE) {
+            h.fac_rec.reserve(sizeof(T) * size_t(h.n_elements) * fac_size(Q));
+            k_hfactors<T, O, Q, M><<<grid_for(h.n_elements), BLOCK, 0, h.stream>>>(h.mesh<T>(), x, h.fac_rec.as<T>());
+            PF_LAUNCH_CHECK();
+            auto gath = [&]<int GS>() {
+                k_gather_factored<T, O, Q, GS><<<xcd_grid(p.n_chunks), GS, 0, h.stream>>>(
+                    p.n_chunks, p.chunk.as<int32_t>(), p.slot.as<int32_t>(), h.blk_out.as<int32_t>(), p.meta.as<int32_t>(),
+                    h.fac_rec.as<T>(), static_cast<T *>(values));
+            };
+            if (p.threads == 128)
+                gath.template operator()<128>();
+            else if (p.threads == 512)
+                gath.template operator()<512>();
+            else
+                gath.template operator()<256>();
+        }
+    });
+    PF_LAUNCH_CHECK();
+}
+
+// E + G + H on the factored route: one element pass writes the local
+// gradient, the energy partials and the Hessian factor records
+void factored_energy_gradient_hessian(Handle &h, const double *x, double *energy, double *g, void *values)
+{
+    if (h.material != MAT_NH && h.material != MAT_LE)
+        fail(PFGPU_ERR_UNSUPPORTED, "the factored Hessian covers NeoHookean and linear elasticity");
+    ensure_plain_hessian_maps(h);
+    build_plan(h, h.fac_plan, h.tune_gather_threads, [&](int32_t k) { return h.h_contrib_code[size_t(k)]; });
+    const GatherPlan &p = h.fac_plan;
+    const unsigned grid = grid_for(h.n_elements);
+    h.loc.reserve(h.value_size() * size_t(h.n_elements) * h.n_local * 3);
+    if (energy) h.partial.reserve(sizeof(double) * grid);
+    dispatch(h, [&]<typename T, int O, int Q, int M>() {
+        if constexpr (M == MAT_NH || M == MAT_LE) {
+            h.fac_rec.reserve(sizeof(T) * size_t(h.n_elements) * fac_size(Q));
+            k_gradient<T, O, Q, M, true><<<grid, BLOCK, 0, h.stream>>>(
+                h.mesh<T>(), x, h.loc.as<T>(), energy ? h.partial.as<double>() : nullptr, h.fac_rec.as<T>());
+            PF_LAUNCH_CHECK();
+            gather_rows(h, h.loc.get(), g);
+            if (energy) sum_doubles(h, h.partial.as<double>(), int(grid), energy);
+            auto gath = [&]<int GS>() {
+                k_gather_factored<T, O, Q, GS><<<xcd_grid(p.n_chunks), GS, 0, h.stream>>>(
+                    p.n_chunks, p.chunk.as<int32_t>(), p.slot.as<int32_t>(), h.blk_out.as<int32_t>(), p.meta.as<int32_t>(),
+                    h.fac_rec.as<T>(), static_cast<T *>(values));
+            };
+            if (p.threads == 128)
+                gath.template operator()<128>();
+            else if (p.threads == 512)
+                gath.template operator()<512>();
+            else
+                gath.template operator()<256>();
+        }
+    });
+    PF_LAUNCH_CHECK();
 }
 
 void plain_hessian(Handle &h, const double *x, void *values, bool project_to_psd)
 {
+    // the element-block route carries the PSD projection
+    if (h.path == PFGPU_PATH_FACTORED && !project_to_psd) {
+        factored_hessian(h, x, values);
+        return;
+    }
     ensure_plain_hessian_maps(h);
-    ensure_block_chunks(h);
+    // contribution (e, a, b) -> index of block s(min(a,b), max(a,b)) of element
+    // e in the tiled hloc layout of k_hessian; bit 31 when a > b (transposed)
+    const int NB = h.n_local, NBLK = h.n_blocks_sym();
+    if (int64_t((h.n_elements + 63) / 64) * NBLK * 64 >= (int64_t(1) << 31) - 1)
+        fail(PFGPU_ERR_UNSUPPORTED, "too many local Hessian blocks for 31-bit indices");
+    build_plan(h, h.blk_plan, h.tune_gather_threads, [&](int32_t k) {
+        const int32_t c = h.h_contrib_code[size_t(k)];
+        const int e = c / (NB * NB), a = (c / NB) % NB, b = c % NB;
+        const int lo = std::min(a, b), hi = std::max(a, b);
+        const int s = lo * NB - lo * (lo - 1) / 2 + (hi - lo);
+        const int32_t idx = int32_t((int64_t(e / 64) * NBLK + s) * 64 + e % 64);
+        return a > b ? int32_t(uint32_t(idx) | 0x80000000u) : idx;
+    });
+    const GatherPlan &p = h.blk_plan;
     h.hloc.reserve(h.value_size() * size_t((h.n_elements + 63) / 64) * 64 * h.n_blocks_sym() * 9);
     dispatch(h, [&]<typename T, int O, int Q, int M>() {
         auto hess = [&]<int HB>() {
@@ -787,13 +1032,13 @@ void plain_hessian(Handle &h, const double *x, void *values, bool project_to_psd
         PF_LAUNCH_CHECK();
         if (project_to_psd) psd_project_blocks(h, h.hloc.get());
         auto gath = [&]<int GS>() {
-            k_gather_blocks_seg<T, GS><<<xcd_grid(h.n_blk_chunks), GS, 0, h.stream>>>(
-                h.n_blk_chunks, h.blk_chunk.as<int32_t>(), h.brow_ptr.as<int32_t>(), h.block_row.as<int32_t>(),
-                h.contrib_ptr.as<int32_t>(), h.contrib.as<int32_t>(), h.hloc.as<T>(), static_cast<T *>(values));
+            k_gather_blocks_seg<T, GS><<<xcd_grid(p.n_chunks), GS, 0, h.stream>>>(
+                p.n_chunks, p.chunk.as<int32_t>(), p.slot.as<int32_t>(), h.blk_out.as<int32_t>(), p.meta.as<int32_t>(),
+                h.hloc.as<T>(), static_cast<T *>(values));
         };
-        if (h.tune_gather_threads == 128)
+        if (p.threads == 128)
             gath.template operator()<128>();
-        else if (h.tune_gather_threads == 512)
+        else if (p.threads == 512)
             gath.template operator()<512>();
         else
             gath.template operator()<256>();

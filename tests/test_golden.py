@@ -59,7 +59,7 @@ def test_oracle_reproduces_golden(path):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("path_kind", ["fused", "plain"])
+@pytest.mark.parametrize("path_kind", ["fused", "plain", "factored"])
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 @pytest.mark.parametrize("path", FIXTURES, ids=os.path.basename)
 def test_library_reproduces_golden(path, precision, path_kind):

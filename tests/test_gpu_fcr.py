@@ -145,9 +145,10 @@ def test_plain_path_only():
     from polyfem_amd import _ffi
     m = kuhn_box(2, 2, 2, order=1)
     a = NLAssembler(NAME, m, young=1e6, poisson=0.35, precision="mixed")
-    with pytest.raises(PfgpuError) as err:
-        a.set_path("fused")
-    assert err.value.status == _ffi.ERR_UNSUPPORTED
+    for path in ("fused", "factored"):
+        with pytest.raises(PfgpuError) as err:
+            a.set_path(path)
+        assert err.value.status == _ffi.ERR_UNSUPPORTED
 
 
 @pytest.mark.parametrize("order", [1, 2])

@@ -18,7 +18,7 @@ FP64_TOL = 1e-10
 # signs and the conditioning of ln J leave ~1e-5 relative on the arrays.
 MIXED_TOL = {"energy": 1e-5, "gradient": 2e-5, "hessian": 2e-5, "hvp": 2e-5, "vjp": 5e-5}
 
-PATHS = ["fused", "plain"]
+PATHS = ["fused", "plain", "factored"]
 CASES = [(1, "NeoHookean"), (2, "NeoHookean"), (1, "LinearElasticity"), (2, "LinearElasticity")]
 # SaintVenant runs on the plain path only
 SV_CASES = [(1, "SaintVenant"), (2, "SaintVenant")]
@@ -142,13 +142,14 @@ def test_update_lame_params():
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 @pytest.mark.parametrize("order", [1, 2])
-def test_fused_many_patches(order, precision):
-    """Meshes large enough for many patches and several staging chunks; fused
-    and plain paths against the oracle and against each other."""
+@pytest.mark.parametrize("path", ["fused", "factored"])
+def test_many_patches_and_chunks(order, precision, path):
+    """Meshes large enough for many fused patches and staging chunks and many
+    factored-gather chunks, against the oracle."""
     grid = (8, 8, 8) if order == 2 else (16, 14, 12)
-    m, x, a, o, lam, mu = setup_case(order, "NeoHookean", grid=grid, precision=precision, path="fused", seed=5)
+    m, x, a, o, lam, mu = setup_case(order, "NeoHookean", grid=grid, precision=precision, path=path, seed=5)
     tol = FP64_TOL if precision == "fp64" else MIXED_TOL["hessian"]
-    print(f"case many patches order={order} precision={precision} E={m.n_elements}", flush=True)
+    print(f"case many patches order={order} precision={precision} path={path} E={m.n_elements}", flush=True)
     e, g, vals = a.assemble_energy_gradient_hessian(x)
     outer, inner, ref = o.hessian(x)
     assert rel(vals, ref) < tol
@@ -186,10 +187,20 @@ def test_autotune_and_tunings_agree():
             if ref is None:
                 ref = (e, g, vals)
             assert e == ref[0] and np.array_equal(g, ref[1]) and np.array_equal(vals, ref[2]), (gt, ht)
+    # the factored gather's chunking depends on its size, its per-block
+    # summation order does not
+    fref = None
+    for gt in (128, 256, 512):
+        a.set_tuning("factored", gt, 256)
+        e, g, vals = a.assemble_energy_gradient_hessian(x)
+        if fref is None:
+            fref = vals
+        assert e == ref[0] and np.array_equal(g, ref[1]) and np.array_equal(vals, fref), gt
+    assert rel(fref, ref[2]) < MIXED_TOL["hessian"]
     xd = torch.tensor(x, device="cuda:0")
     rep = a.autotune_dev(xd.data_ptr())
     print("autotune:", rep, flush=True)
-    assert rep["candidates"] >= 9 and rep["ms"] > 0
+    assert rep["candidates"] >= 12 and rep["ms"] > 0 and rep["ms_factored"] > 0
     e, g, vals = a.assemble_energy_gradient_hessian(x)
     _, _, hv = o.hessian(x)
     assert rel(vals, hv) < MIXED_TOL["hessian"] and rel(g, o.gradient(x)) < MIXED_TOL["gradient"]
@@ -202,12 +213,13 @@ def test_saint_venant_is_plain_path_only():
     from polyfem_amd import _ffi
     m = kuhn_box(2, 2, 2, order=2)
     a = NLAssembler("SaintVenant", m, young=1e6, poisson=0.35, precision="mixed")
-    with pytest.raises(PfgpuError) as err:
-        a.set_path("fused")
-    assert err.value.status == _ffi.ERR_UNSUPPORTED
+    for path in ("fused", "factored"):
+        with pytest.raises(PfgpuError) as err:
+            a.set_path(path)
+        assert err.value.status == _ffi.ERR_UNSUPPORTED
     x = torch.zeros(m.n_dofs, dtype=torch.float64, device="cuda")
     rep = a.autotune_dev(x.data_ptr())
-    assert rep["path"] == "plain" and rep["ms_fused"] is None
+    assert rep["path"] == "plain" and rep["ms_fused"] is None and rep["ms_factored"] is None
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])

@@ -54,7 +54,7 @@ def test_newton_step_is_exact_newton_step(order, name):
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
-@pytest.mark.parametrize("path", ["fused", "plain"])
+@pytest.mark.parametrize("path", ["fused", "plain", "factored"])
 def test_newton_converges(precision, path):
     m, prob = clamped_box(2, (2, 2, 3))
     lam, mu = random_lame(m.n_elements, seed=BASE_SEED + 5)
