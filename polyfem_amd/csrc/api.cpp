@@ -395,6 +395,11 @@ pfgpu_status pfgpu_autotune(pfgpu_handle *h, const double *x, pfgpu_tune_report 
         h->path = r.path;
         h->tune_gather_threads = r.gather_threads ? r.gather_threads : 256;
         h->tune_hessian_threads = r.hessian_threads ? r.hessian_threads : 256;
+        // one call in the chosen configuration builds its tables (gather
+        // plans are built for one workgroup size), so that the caller's
+        // first assembly does not
+        egh_dev(*h, x, de, dg, dv);
+        PF_HIP(hipStreamSynchronize(h->stream));
         *report = r;
     });
 }

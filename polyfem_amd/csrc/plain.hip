@@ -50,6 +50,9 @@ template <typename T, int ORDER, int NQ> __device__ __forceinline__ const RefTab
 
 // ------------------------------------------------------------ element kernels
 
+__device__ __forceinline__ float fma_t(float a, float b, float c) { return fmaf(a, b, c); }
+__device__ __forceinline__ double fma_t(double a, double b, double c) { return fma(a, b, c); }
+
 // Hessian factor record of the factored route (see k_hfactors): values per element
 __host__ __device__ constexpr int fac_size(int nq) { return (11 * nq + 6 + 3) & ~3; }
 
@@ -782,20 +785,22 @@ __global__ __launch_bounds__(GS) void k_gather_factored(int n_chunks, const int3
         T v[9];
 #pragma unroll
         for (int i = 0; i < 9; ++i) v[i] = i % 4 == 0 ? gd : T(0);
+        // written as fused multiply-adds: 42 per point
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
             const T *M = r + 9 * q, *ra = srg + (q * NB + a) * 3, *rb = srg + (q * NB + b) * 3;
             T ka[3], kb[3];
 #pragma unroll
             for (int mm = 0; mm < 3; ++mm) {
-                ka[mm] = M[mm * 3] * ra[0] + M[mm * 3 + 1] * ra[1] + M[mm * 3 + 2] * ra[2];
-                kb[mm] = M[mm * 3] * rb[0] + M[mm * 3 + 1] * rb[1] + M[mm * 3 + 2] * rb[2];
+                ka[mm] = fma_t(M[mm * 3 + 2], ra[2], fma_t(M[mm * 3 + 1], ra[1], M[mm * 3] * ra[0]));
+                kb[mm] = fma_t(M[mm * 3 + 2], rb[2], fma_t(M[mm * 3 + 1], rb[1], M[mm * 3] * rb[0]));
             }
             const T cl = r[9 * NQ + q], cc = r[10 * NQ + q];
+            const T la[3] = {cl * ka[0], cl * ka[1], cl * ka[2]}, lb[3] = {cc * kb[0], cc * kb[1], cc * kb[2]};
 #pragma unroll
             for (int mm = 0; mm < 3; ++mm)
 #pragma unroll
-                for (int nn = 0; nn < 3; ++nn) v[mm * 3 + nn] += cl * ka[mm] * kb[nn] + cc * kb[mm] * ka[nn];
+                for (int nn = 0; nn < 3; ++nn) v[mm * 3 + nn] = fma_t(lb[mm], ka[nn], fma_t(la[mm], kb[nn], v[mm * 3 + nn]));
         }
 #pragma unroll
         for (int i = 0; i < 9; ++i) sv[t * 9 + i] = v[i];

@@ -5,11 +5,14 @@
 // a matrix whose eigenvalues are all >= 0 is left as it is.
 //
 // One wavefront per element.  The symmetric 3n x 3n matrix (n = 4 or 10
-// nodes) is read from the element-block tiles k_hessian writes, diagonalised
-// in LDS by the cyclic Jacobi method with the round-robin ordering (n*3/2
-// disjoint rotations per round, all applied at once, n*3-1 rounds per sweep)
-// until the off-diagonal part is at rounding level, and written back only
-// when some eigenvalue is negative.
+// nodes) is read from the element-block tiles k_hessian writes.  A Cholesky
+// factorisation of A + delta I (delta = psd_shift x the largest diagonal
+// entry) first tests whether every eigenvalue exceeds -delta; such a matrix
+// is left as it is (the exact projection would move it by at most delta).
+// Otherwise A is diagonalised in LDS by the cyclic Jacobi method with the
+// round-robin ordering (n*3/2 disjoint rotations per round, all applied at
+// once, n*3-1 rounds per sweep) until the off-diagonal part is at rounding
+// level, and written back when some eigenvalue is negative.
 #include "dispatch.hpp"
 
 namespace pfgpu {
@@ -69,6 +72,11 @@ template <int NB> __device__ __forceinline__ const BlockPairs<NB> &block_pairs()
 
 constexpr int PSD_WAVE = 64;
 
+// relative shift of the positive-definiteness test: far below the parity
+// tolerances (1e-10 in FP64, 1e-4 for FP32 values) and far above the
+// rounding of a Cholesky factorisation of order 30
+template <typename T> constexpr T psd_shift() { return sizeof(T) == 8 ? T(1e-12) : T(2e-6); }
+
 template <typename T> __device__ __forceinline__ T wave_sum(T v)
 {
 #pragma unroll
@@ -85,6 +93,7 @@ __global__ __launch_bounds__(PSD_WAVE) void k_psd_project(int n_elements, T *__r
     // (the rounding floor of the off-diagonal part is about (N eps)^2 of the diagonal)
     constexpr double TOL2 = sizeof(T) == 8 ? 1e-26 : 1e-10;
     __shared__ T A[N][N + 1], V[N][N + 1], CS[NP][2];
+    __shared__ int PQ[NP][2];
     const int e = blockIdx.x, lane = threadIdx.x;
     if (e >= n_elements) return; // uniform over the workgroup
     // tile layout [tile][s][element in tile][9]: 64 elements per tile
@@ -96,6 +105,35 @@ __global__ __launch_bounds__(PSD_WAVE) void k_psd_project(int n_elements, T *__r
         A[3 * a + j / 3][3 * b + j % 3] = v;
         A[3 * b + j % 3][3 * a + j / 3] = v;
     }
+    __syncthreads();
+    // positive-definiteness test: Cholesky of A + delta I in V (lower triangle)
+    T dmax = T(0);
+    for (int k = lane; k < N; k += PSD_WAVE) dmax = std::fmax(dmax, std::fabs(A[k][k]));
+#pragma unroll
+    for (int o = PSD_WAVE / 2; o > 0; o >>= 1) dmax = std::fmax(dmax, __shfl_xor(dmax, o, PSD_WAVE));
+    const T delta = psd_shift<T>() * dmax;
+    for (int i = lane; i < N * N; i += PSD_WAVE) V[i / N][i % N] = A[i / N][i % N] + (i / N == i % N ? delta : T(0));
+    __syncthreads();
+    bool pd = true;
+    for (int k = 0; k < N; ++k) {
+        const T d = V[k][k]; // the same value in every lane: a uniform decision
+        if (!(d > T(0))) {
+            pd = false;
+            break;
+        }
+        const T inv = T(1) / std::sqrt(d);
+        __syncthreads();
+        for (int i = k + 1 + lane; i < N; i += PSD_WAVE) V[i][k] *= inv;
+        __syncthreads();
+        // trailing lower triangle: V[i][j] -= V[i][k] V[j][k], k < j <= i
+        constexpr int NN = N * N;
+        for (int idx = lane; idx < NN; idx += PSD_WAVE) {
+            const int i = idx / N, j = idx % N;
+            if (j > k && j <= i) V[i][j] -= V[i][k] * V[j][k];
+        }
+        __syncthreads();
+    }
+    if (pd) return; // every eigenvalue above -delta: A stays
     for (int i = lane; i < N * N; i += PSD_WAVE) V[i / N][i % N] = i / N == i % N ? T(1) : T(0);
     __syncthreads();
     const JacobiRounds<N> &jr = rounds<N>();
@@ -115,6 +153,8 @@ __global__ __launch_bounds__(PSD_WAVE) void k_psd_project(int n_elements, T *__r
         for (int r = 0; r < N - 1; ++r) {
             if (lane < NP) {
                 const int p = jr.p[r * NP + lane], q = jr.q[r * NP + lane];
+                PQ[lane][0] = p;
+                PQ[lane][1] = q;
                 const T apq = A[p][q];
                 T c = T(1), s = T(0);
                 if (apq != T(0)) {
@@ -130,7 +170,7 @@ __global__ __launch_bounds__(PSD_WAVE) void k_psd_project(int n_elements, T *__r
             __syncthreads();
             // rows p, q of every pair (the pairs are disjoint)
             for (int i = lane; i < NP * N; i += PSD_WAVE) {
-                const int k = i / N, j = i % N, p = jr.p[r * NP + k], q = jr.q[r * NP + k];
+                const int k = i / N, j = i % N, p = PQ[k][0], q = PQ[k][1];
                 const T c = CS[k][0], s = CS[k][1], ap = A[p][j], aq = A[q][j];
                 A[p][j] = c * ap - s * aq;
                 A[q][j] = s * ap + c * aq;
@@ -139,7 +179,7 @@ __global__ __launch_bounds__(PSD_WAVE) void k_psd_project(int n_elements, T *__r
             // columns p, q, and the eigenvector matrix; the entry the rotation
             // annihilates is set to zero rather than left at rounding level
             for (int i = lane; i < NP * N; i += PSD_WAVE) {
-                const int k = i / N, j = i % N, p = jr.p[r * NP + k], q = jr.q[r * NP + k];
+                const int k = i / N, j = i % N, p = PQ[k][0], q = PQ[k][1];
                 const T c = CS[k][0], s = CS[k][1];
                 const T ap = A[j][p], aq = A[j][q];
                 A[j][p] = j == q ? T(0) : c * ap - s * aq;
