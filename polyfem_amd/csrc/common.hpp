@@ -79,7 +79,7 @@ class DevBuf {
 // threads: chunk starts (n_chunks + 1 blocks), threads contribution slots per
 // chunk, per-block metadata (the kernels document the layout).
 struct GatherPlan {
-    DevBuf chunk, slot, meta;
+    DevBuf chunk, slot, meta, dst;
     int n_chunks = 0, threads = 0;
 };
 

@@ -31,6 +31,7 @@ struct pfgpu_handle {
     int n_row_chunks = 0;
     // factored Hessian (PFGPU_PATH_FACTORED): per-element factor records
     pfgpu::DevBuf fac_rec;
+    int n_cu = 0; // compute units of the device (queried on first use)
     // launch shapes of the plain path (set by pfgpu_autotune / pfgpu_set_tuning)
     int tune_gather_threads = 256, tune_hessian_threads = 256;
     // fused-path patch tables (built on first use)

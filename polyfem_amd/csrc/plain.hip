@@ -740,10 +740,23 @@ __global__ __launch_bounds__(BLOCK) void k_hfactors(DevMesh<T> m, const double *
     finish_factors<T, NQ>(el.jit, el.mu, w, r, rec, e);
 }
 
-// gather plan as for k_gather_blocks_seg; slot word e NB^2 + a NB + b
+// gather plan as for k_gather_blocks_seg, except that a chunk's slots hold
+// its contributions ordered by element (the lanes of a wavefront then share
+// records, and so cache lines, in runs) and dst[slot] is the contribution's
+// position in block order, where its block is staged; slot word
+// e NB^2 + a NB + b.
+//
+// Persistent: the workgroups of XCD x (blockIdx mod 8) walk that XCD's
+// contiguous eighth of the chunks with stride gridDim / 8, and the slot
+// words and block metadata of a workgroup's next chunk are loaded while it
+// processes the current one, so only the record loads wait in each chunk
+// (requesting the next chunk's records before the current chunk's sums was
+// measured slower).  The grid is the device's resident workgroups; gridDim
+// must be a multiple of 8.
 template <typename T, int ORDER, int NQ, int GS>
 __global__ __launch_bounds__(GS) void k_gather_factored(int n_chunks, const int32_t *__restrict__ chunk,
                                                         const int32_t *__restrict__ slot,
+                                                        const uint16_t *__restrict__ dst,
                                                         const int32_t *__restrict__ bout,
                                                         const int32_t *__restrict__ meta, const T *__restrict__ rec,
                                                         T *__restrict__ values)
@@ -751,64 +764,89 @@ __global__ __launch_bounds__(GS) void k_gather_factored(int n_chunks, const int3
     constexpr int NB = NLocal<ORDER>::value, R = fac_size(NQ);
     __shared__ T sv[GS * 9];
     __shared__ T srg[NQ * NB * 3], srgg[NB * NB * 6];
-    const int c = xcd_chunk(n_chunks);
-    if (c >= n_chunks) return;
+    const int per = (n_chunks + N_XCD - 1) / N_XCD, xcd = int(blockIdx.x % N_XCD);
+    const int stride = int(gridDim.x / N_XCD), c_end = min(n_chunks, (xcd + 1) * per);
+    int c = xcd * per + int(blockIdx.x / N_XCD);
+    if (c >= c_end) return; // uniform over the workgroup
     const int t = threadIdx.x;
-    const int32_t cd = slot[int64_t(c) * GS + t];
-    const int b0 = chunk[c], b1 = chunk[c + 1];
-    // this thread's contribution: its element's record
-    int a = 0, b = 0;
-    T r[R];
-    if (cd >= 0) {
-        const int e = cd / (NB * NB), ab = cd - e * (NB * NB);
-        a = ab / NB;
-        b = ab - a * NB;
-        const T *rp = static_cast<const T *>(__builtin_assume_aligned(rec, 16)) + int64_t(e) * R;
-#pragma unroll
-        for (int i = 0; i < R; ++i) r[i] = rp[i];
-    }
-    // this thread's output block
-    const int bq = b0 + t;
-    int32_t off = 0, mt = 0;
-    if (bq < b1) {
-        off = bout[bq];
-        mt = meta[bq];
-    }
     const RefTab<T, ORDER, NQ> &tb = ref_tab<T, ORDER, NQ>();
     for (int i = t; i < NQ * NB * 3; i += GS) srg[i] = tb.rg[i];
     for (int i = t; i < NB * NB * 6; i += GS) srgg[i] = tb.rgg[i];
-    __syncthreads();
-    if (cd >= 0) {
-        const T *cr = srgg + (a * NB + b) * 6;
-        const T gd = r[11 * NQ] * cr[0] + r[11 * NQ + 1] * cr[1] + r[11 * NQ + 2] * cr[2] + r[11 * NQ + 3] * cr[3] +
-                     r[11 * NQ + 4] * cr[4] + r[11 * NQ + 5] * cr[5];
-        T v[9];
-#pragma unroll
-        for (int i = 0; i < 9; ++i) v[i] = i % 4 == 0 ? gd : T(0);
-        // written as fused multiply-adds: 42 per point
-#pragma unroll
-        for (int q = 0; q < NQ; ++q) {
-            const T *M = r + 9 * q, *ra = srg + (q * NB + a) * 3, *rb = srg + (q * NB + b) * 3;
-            T ka[3], kb[3];
-#pragma unroll
-            for (int mm = 0; mm < 3; ++mm) {
-                ka[mm] = fma_t(M[mm * 3 + 2], ra[2], fma_t(M[mm * 3 + 1], ra[1], M[mm * 3] * ra[0]));
-                kb[mm] = fma_t(M[mm * 3 + 2], rb[2], fma_t(M[mm * 3 + 1], rb[1], M[mm * 3] * rb[0]));
-            }
-            const T cl = r[9 * NQ + q], cc = r[10 * NQ + q];
-            const T la[3] = {cl * ka[0], cl * ka[1], cl * ka[2]}, lb[3] = {cc * kb[0], cc * kb[1], cc * kb[2]};
-#pragma unroll
-            for (int mm = 0; mm < 3; ++mm)
-#pragma unroll
-                for (int nn = 0; nn < 3; ++nn) v[mm * 3 + nn] = fma_t(lb[mm], ka[nn], fma_t(la[mm], kb[nn], v[mm * 3 + nn]));
+    // indices of chunk cc: this thread's slot word and stage position, the
+    // chunk's blocks, this thread's output block's offset and metadata
+    auto fetch = [&](int cc, int32_t &cd, int &ds, int &b0, int &b1, int32_t &off, int32_t &mt) {
+        cd = slot[int64_t(cc) * GS + t];
+        ds = dst[int64_t(cc) * GS + t];
+        b0 = chunk[cc];
+        b1 = chunk[cc + 1];
+        off = 0;
+        mt = 0;
+        if (b0 + t < b1) {
+            off = bout[b0 + t];
+            mt = meta[b0 + t];
         }
+    };
+    int32_t cd, off, mt;
+    int ds, b0, b1;
+    fetch(c, cd, ds, b0, b1, off, mt);
+    while (c < c_end) { // c, c_end and the chunk bounds are uniform over the workgroup
+        const int cn = c + stride;
+        int32_t cdn = -1, offn = 0, mtn = 0;
+        int dsn = 0, b0n = 0, b1n = 0;
+        if (cn < c_end) fetch(cn, cdn, dsn, b0n, b1n, offn, mtn);
+        // this thread's contribution: its element's record
+        int a = 0, b = 0;
+        T r[R];
+        if (cd >= 0) {
+            const int e = cd / (NB * NB), ab = cd - e * (NB * NB);
+            a = ab / NB;
+            b = ab - a * NB;
+            const T *rp = static_cast<const T *>(__builtin_assume_aligned(rec, 16)) + int64_t(e) * R;
 #pragma unroll
-        for (int i = 0; i < 9; ++i) sv[t * 9 + i] = v[i];
+            for (int i = 0; i < R; ++i) r[i] = rp[i];
+        }
+        // the tables are staged, and the previous chunk's sums have read sv
+        __syncthreads();
+        if (cd >= 0) {
+            const T *cr = srgg + (a * NB + b) * 6;
+            const T gd = r[11 * NQ] * cr[0] + r[11 * NQ + 1] * cr[1] + r[11 * NQ + 2] * cr[2] + r[11 * NQ + 3] * cr[3] +
+                         r[11 * NQ + 4] * cr[4] + r[11 * NQ + 5] * cr[5];
+            T v[9];
+#pragma unroll
+            for (int i = 0; i < 9; ++i) v[i] = i % 4 == 0 ? gd : T(0);
+            // written as fused multiply-adds: 42 per point
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) {
+                const T *M = r + 9 * q, *ra = srg + (q * NB + a) * 3, *rb = srg + (q * NB + b) * 3;
+                T ka[3], kb[3];
+#pragma unroll
+                for (int mm = 0; mm < 3; ++mm) {
+                    ka[mm] = fma_t(M[mm * 3 + 2], ra[2], fma_t(M[mm * 3 + 1], ra[1], M[mm * 3] * ra[0]));
+                    kb[mm] = fma_t(M[mm * 3 + 2], rb[2], fma_t(M[mm * 3 + 1], rb[1], M[mm * 3] * rb[0]));
+                }
+                const T cl = r[9 * NQ + q], cc = r[10 * NQ + q];
+                const T la[3] = {cl * ka[0], cl * ka[1], cl * ka[2]}, lb[3] = {cc * kb[0], cc * kb[1], cc * kb[2]};
+#pragma unroll
+                for (int mm = 0; mm < 3; ++mm)
+#pragma unroll
+                    for (int nn = 0; nn < 3; ++nn)
+                        v[mm * 3 + nn] = fma_t(lb[mm], ka[nn], fma_t(la[mm], kb[nn], v[mm * 3 + nn]));
+            }
+#pragma unroll
+            for (int i = 0; i < 9; ++i) sv[ds * 9 + i] = v[i];
+        }
+        __syncthreads();
+        if (b0 + t < b1) sum_block(sv, off, mt, values);
+        // blocks without contributions (none in a pattern built from elements)
+        for (int bb = b0 + t + GS; bb < b1; bb += GS) sum_block(sv, bout[bb], meta[bb], values);
+        c = cn;
+        cd = cdn;
+        ds = dsn;
+        b0 = b0n;
+        b1 = b1n;
+        off = offn;
+        mt = mtn;
     }
-    __syncthreads();
-    if (bq < b1) sum_block(sv, off, mt, values);
-    // blocks without contributions (none in a pattern built from elements)
-    for (int bb = bq + GS; bb < b1; bb += GS) sum_block(sv, bout[bb], meta[bb], values);
 }
 
 } // namespace
@@ -910,8 +948,10 @@ void ensure_plain_hessian_maps(Handle &h)
 }
 
 // the gather plan of workgroup size gs for slot words word(k) of the
-// contributions k (layout described at k_gather_blocks_seg)
-template <typename W> void build_plan(Handle &h, GatherPlan &p, int gs, W &&word)
+// contributions k (layout described at k_gather_blocks_seg); by_element:
+// each chunk's slots reordered by key(word), with their block-order
+// positions in dst (the factored gather's layout)
+template <typename W, typename K> void build_plan(Handle &h, GatherPlan &p, int gs, W &&word, bool by_element, K &&key)
 {
     if (p.threads == gs) return;
     const std::vector<int32_t> &ptr = h.h_contrib_ptr;
@@ -930,6 +970,25 @@ template <typename W> void build_plan(Handle &h, GatherPlan &p, int gs, W &&word
             for (int32_t k = ptr[size_t(b)]; k < ptr[size_t(b) + 1]; ++k) slot[size_t(ci) * gs + size_t(k - k0)] = word(k);
         }
     }
+    if (by_element) {
+        // each chunk's slots ordered by element (stable), with the block-order
+        // position of every contribution; the element of word w is key(w)
+        std::vector<uint16_t> dst(size_t(nc) * gs, 0);
+        std::vector<int32_t> ord;
+        for (int ci = 0; ci < nc; ++ci) {
+            const int32_t k0 = ptr[size_t(c[ci])], n = ptr[size_t(c[ci + 1])] - k0;
+            int32_t *sl = slot.data() + size_t(ci) * gs;
+            ord.resize(size_t(n));
+            for (int32_t i = 0; i < n; ++i) ord[size_t(i)] = i;
+            std::stable_sort(ord.begin(), ord.end(), [&](int32_t u, int32_t v) { return key(sl[u]) < key(sl[v]); });
+            std::vector<int32_t> w(sl, sl + n);
+            for (int32_t i = 0; i < n; ++i) {
+                sl[i] = w[size_t(ord[size_t(i)])];
+                dst[size_t(ci) * gs + size_t(i)] = uint16_t(ord[size_t(i)]);
+            }
+        }
+        p.dst = upload(dst, h.stream);
+    }
     p.chunk = upload(c, h.stream);
     p.slot = upload(slot, h.stream);
     p.meta = upload(meta, h.stream);
@@ -937,12 +996,27 @@ template <typename W> void build_plan(Handle &h, GatherPlan &p, int gs, W &&word
     p.threads = gs;
 }
 
+// persistent grid of a kernel: the workgroups of gs threads resident at
+// once on the device (from the kernel's occupancy), a multiple of the 8
+// XCDs, and no more than the chunks
+template <typename K> unsigned persistent_grid(Handle &h, K kernel, int n_chunks, int gs)
+{
+    if (h.n_cu == 0) PF_HIP(hipDeviceGetAttribute(&h.n_cu, hipDeviceAttributeMultiprocessorCount, h.device));
+    int per_cu = 0;
+    PF_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, gs, 0));
+    const int g = std::max(1, h.n_cu) * std::max(1, per_cu);
+    return std::min(xcd_grid(n_chunks), unsigned((g + N_XCD - 1) / N_XCD * N_XCD));
+}
+
 void factored_hessian(Handle &h, const double *x, void *values)
 {
     if (h.material != MAT_NH && h.material != MAT_LE)
         fail(PFGPU_ERR_UNSUPPORTED, "the factored Hessian covers NeoHookean and linear elasticity");
     ensure_plain_hessian_maps(h);
-    build_plan(h, h.fac_plan, h.tune_gather_threads, [&](int32_t k) { return h.h_contrib_code[size_t(k)]; });
+    const int nb2 = h.n_local * h.n_local;
+    build_plan(
+        h, h.fac_plan, h.tune_gather_threads, [&](int32_t k) { return h.h_contrib_code[size_t(k)]; }, true,
+        [&](int32_t w) { return w / nb2; });
     const GatherPlan &p = h.fac_plan;
     dispatch(h, [&]<typename T, int O, int Q, int M>() {
         if constexpr (M == MAT_NH || M == MAT_LE) {
@@ -950,9 +1024,10 @@ void factored_hessian(Handle &h, const double *x, void *values)
             k_hfactors<T, O, Q, M><<<grid_for(h.n_elements), BLOCK, 0, h.stream>>>(h.mesh<T>(), x, h.fac_rec.as<T>());
             PF_LAUNCH_CHECK();
             auto gath = [&]<int GS>() {
-                k_gather_factored<T, O, Q, GS><<<xcd_grid(p.n_chunks), GS, 0, h.stream>>>(
-                    p.n_chunks, p.chunk.as<int32_t>(), p.slot.as<int32_t>(), h.blk_out.as<int32_t>(), p.meta.as<int32_t>(),
-                    h.fac_rec.as<T>(), static_cast<T *>(values));
+                k_gather_factored<T, O, Q, GS><<<persistent_grid(h, k_gather_factored<T, O, Q, GS>, p.n_chunks, GS), GS,
+                                                   0, h.stream>>>(
+                    p.n_chunks, p.chunk.as<int32_t>(), p.slot.as<int32_t>(), p.dst.as<uint16_t>(), h.blk_out.as<int32_t>(),
+                    p.meta.as<int32_t>(), h.fac_rec.as<T>(), static_cast<T *>(values));
             };
             if (p.threads == 128)
                 gath.template operator()<128>();
@@ -972,7 +1047,10 @@ void factored_energy_gradient_hessian(Handle &h, const double *x, double *energy
     if (h.material != MAT_NH && h.material != MAT_LE)
         fail(PFGPU_ERR_UNSUPPORTED, "the factored Hessian covers NeoHookean and linear elasticity");
     ensure_plain_hessian_maps(h);
-    build_plan(h, h.fac_plan, h.tune_gather_threads, [&](int32_t k) { return h.h_contrib_code[size_t(k)]; });
+    const int nb2 = h.n_local * h.n_local;
+    build_plan(
+        h, h.fac_plan, h.tune_gather_threads, [&](int32_t k) { return h.h_contrib_code[size_t(k)]; }, true,
+        [&](int32_t w) { return w / nb2; });
     const GatherPlan &p = h.fac_plan;
     const unsigned grid = grid_for(h.n_elements);
     h.loc.reserve(h.value_size() * size_t(h.n_elements) * h.n_local * 3);
@@ -986,9 +1064,10 @@ void factored_energy_gradient_hessian(Handle &h, const double *x, double *energy
             gather_rows(h, h.loc.get(), g);
             if (energy) sum_doubles(h, h.partial.as<double>(), int(grid), energy);
             auto gath = [&]<int GS>() {
-                k_gather_factored<T, O, Q, GS><<<xcd_grid(p.n_chunks), GS, 0, h.stream>>>(
-                    p.n_chunks, p.chunk.as<int32_t>(), p.slot.as<int32_t>(), h.blk_out.as<int32_t>(), p.meta.as<int32_t>(),
-                    h.fac_rec.as<T>(), static_cast<T *>(values));
+                k_gather_factored<T, O, Q, GS><<<persistent_grid(h, k_gather_factored<T, O, Q, GS>, p.n_chunks, GS), GS,
+                                                   0, h.stream>>>(
+                    p.n_chunks, p.chunk.as<int32_t>(), p.slot.as<int32_t>(), p.dst.as<uint16_t>(), h.blk_out.as<int32_t>(),
+                    p.meta.as<int32_t>(), h.fac_rec.as<T>(), static_cast<T *>(values));
             };
             if (p.threads == 128)
                 gath.template operator()<128>();
@@ -1021,7 +1100,7 @@ void plain_hessian(Handle &h, const double *x, void *values, bool project_to_psd
         const int s = lo * NB - lo * (lo - 1) / 2 + (hi - lo);
         const int32_t idx = int32_t((int64_t(e / 64) * NBLK + s) * 64 + e % 64);
         return a > b ? int32_t(uint32_t(idx) | 0x80000000u) : idx;
-    });
+    }, false, [](int32_t w) { return w; });
     const GatherPlan &p = h.blk_plan;
     h.hloc.reserve(h.value_size() * size_t((h.n_elements + 63) / 64) * 64 * h.n_blocks_sym() * 9);
     dispatch(h, [&]<typename T, int O, int Q, int M>() {
