@@ -291,6 +291,13 @@ pfgpu_status pfgpu_set_load(pfgpu_handle *h, const double *f);
  * reference's Mass assembler (assembler/Mass.cpp:4-20) integrated exactly;
  * rho: per-element density (n_elements) or NULL for rho_uniform */
 pfgpu_status pfgpu_set_density(pfgpu_handle *h, const double *rho, double rho_uniform);
+/* row-sum lumping of the mass matrix (lumped = 1) or the consistent one
+ * (0, the default): the reference's solver.advanced.lump_mass_matrix
+ * (State.cpp:1542-1545, utils::lump_matrix, MatrixUtils.cpp:84-101), every
+ * row's sum on its diagonal block.  Applies to the current and later
+ * pfgpu_set_density calls.  (At P2 the vertex rows sum to negative masses,
+ * as in the reference.) */
+pfgpu_status pfgpu_set_mass_lumping(pfgpu_handle *h, int32_t lumped);
 /* y = M v */
 pfgpu_status pfgpu_mass_matvec(pfgpu_handle *h, const double *v, double *y);
 pfgpu_status pfgpu_mass_matvec_dev(pfgpu_handle *h, const double *v, double *y);

@@ -115,6 +115,7 @@ _SIGNATURES = {
     "pfgpu_newton_solve_batch": [vp, vp, vp, C.c_int32, vp, vp, C.POINTER(NewtonParams), vp],
     "pfgpu_energy_lame_gradient": [vp, vp, vp],
     "pfgpu_set_density": [vp, vp, C.c_double],
+    "pfgpu_set_mass_lumping": [vp, C.c_int32],
     "pfgpu_mass_matvec": [vp, vp, vp],
     "pfgpu_mass_matvec_dev": [vp, vp, vp],
     "pfgpu_transient_forward": [vp, vp, vp, vp, vp, C.POINTER(TransientParams), C.POINTER(NewtonParams), vp, vp],

@@ -885,6 +885,21 @@ pfgpu_status pfgpu_set_load(pfgpu_handle *h, const double *f)
     });
 }
 
+pfgpu_status pfgpu_set_mass_lumping(pfgpu_handle *h, int32_t lumped)
+{
+    return guard([&] {
+        check_handle(h);
+        require(lumped == 0 || lumped == 1, "lumped must be 0 or 1");
+        if (h->lump_mass == (lumped == 1)) return;
+        h->lump_mass = lumped == 1;
+        if (h->has_mass) {
+            build_mass(*h, h->h_rho.empty() ? nullptr : h->h_rho.data(), h->rho_uniform);
+            solver_release(*h);
+            PF_HIP(hipStreamSynchronize(h->stream));
+        }
+    });
+}
+
 pfgpu_status pfgpu_set_density(pfgpu_handle *h, const double *rho, double rho_uniform)
 {
     return guard([&] {

@@ -51,7 +51,7 @@ struct pfgpu_handle {
     pfgpu::DevBuf h_x, h_v, h_out, h_out2, h_vals; // staging for host-pointer calls
     // consistent mass matrix: one value per 3x3 block (block = m I_3)
     pfgpu::DevBuf mass, pat_bcol;
-    bool has_mass = false;
+    bool has_mass = false, lump_mass = false;
     std::vector<double> h_rho; // per-element density of the mass matrix (empty: rho_uniform)
     double rho_uniform = 0;
     // objective of the solves: w E(x) + 1/2 (x - xt)^T M (x - xt) - w f^T x

@@ -1,4 +1,4 @@
-// Consistent mass matrix of P1/P2 tets (Mass::assemble, assembler/Mass.cpp:4-20):
+// Consistent (or row-sum lumped) mass matrix of P1/P2 tets (Mass::assemble, assembler/Mass.cpp:4-20):
 // M_ab = rho int phi_a phi_b, block (a, b) = M_ab I_3.  The reference
 // integrates with the quadrature of order 2p (AssemblerUtils.cpp:204-211),
 // which is exact for these degree-2p integrands; here the element matrix is
@@ -109,6 +109,20 @@ void build_mass(Handle &h, const double *rho, double rho_uniform)
                 const int64_t pos = std::lower_bound(c0, c1, en[c]) - h.pat.bcol.data();
                 vals[size_t(pos)] += scale * R[size_t(a) * NB + c];
             }
+        }
+    }
+    if (h.lump_mass) {
+        // utils::lump_matrix (MatrixUtils.cpp:84-101): every row's sum on its
+        // diagonal, nothing off it (State.cpp:1542-1545)
+        for (int i = 0; i < h.n_nodes; ++i) {
+            const int32_t r0 = h.pat.brow_ptr[i], r1 = h.pat.brow_ptr[i + 1];
+            double sum = 0;
+            for (int32_t b = r0; b < r1; ++b) {
+                sum += vals[size_t(b)];
+                vals[size_t(b)] = 0.0;
+            }
+            const int64_t d = std::lower_bound(h.pat.bcol.data() + r0, h.pat.bcol.data() + r1, i) - h.pat.bcol.data();
+            vals[size_t(d)] = sum;
         }
     }
     upload_into(h.mass, vals, h.stream);

@@ -49,6 +49,32 @@ def numpy_mass(m, rho):
 
 
 @pytest.mark.parametrize("order", [1, 2])
+def test_lumped_mass_is_row_sums(order):
+    """lump_mass: diag(row sums of the consistent mass), utils::lump_matrix
+    (MatrixUtils.cpp:84-101), against the numerically integrated matrix; the
+    total mass is kept; switching back restores the consistent matrix."""
+    m = kuhn_box(3, 2, 2, order=order)
+    rho = np.random.default_rng(BASE_SEED + 22).uniform(500, 1500, size=m.n_elements)
+    a = NLAssembler("NeoHookean", m, young=1e6, poisson=0.3)
+    print(f"lumped mass order={order}", flush=True)
+    sim = TransientSimulation(a, rho=rho, lump_mass=True)
+    v = np.random.default_rng(BASE_SEED + 23).normal(size=m.n_dofs)
+    Ms = numpy_mass(m, rho)
+    d = np.asarray(Ms.sum(axis=1)).ravel()
+    ref = (d[:, None] * v.reshape(-1, 3)).ravel()
+    y = sim.mass_matvec(v)
+    assert np.max(np.abs(y - ref)) <= 1e-12 * np.max(np.abs(ref))
+    ones = np.ones(m.n_dofs)
+    X = m.vertices[m.elem_vertices]
+    vol = np.abs(np.linalg.det(X[:, 1:] - X[:, :1])) / 6
+    assert abs(sim.mass_matvec(ones).reshape(-1, 3)[:, 0].sum() - rho @ vol) <= 1e-12 * (rho @ vol)
+    check_lib = __import__("polyfem_amd._ffi", fromlist=["lib"])
+    check_lib.check(check_lib.lib().pfgpu_set_mass_lumping(a._h, 0))
+    full = (Ms @ v.reshape(-1, 3)).ravel()
+    assert np.max(np.abs(sim.mass_matvec(v) - full)) <= 1e-12 * np.max(np.abs(full))
+
+
+@pytest.mark.parametrize("order", [1, 2])
 def test_mass_matches_numerical_integration(order):
     m = kuhn_box(3, 2, 2, order=order)
     rho = np.random.default_rng(BASE_SEED + 20).uniform(500, 1500, size=m.n_elements)
@@ -66,7 +92,7 @@ def test_mass_matches_numerical_integration(order):
     assert abs(ones @ sim.mass_matvec(ones) - rho @ vol) <= 1e-12 * (rho @ vol)
 
 
-def setup(every, lam=None, mu=None, material="NeoHookean"):
+def setup(every, lam=None, mu=None, material="NeoHookean", lump=False):
     m = kuhn_box(2, 2, 2, order=1)
     if lam is None:
         lam, mu = random_lame(m.n_elements, seed=BASE_SEED + 22)
@@ -74,7 +100,8 @@ def setup(every, lam=None, mu=None, material="NeoHookean"):
     prob = NLProblem(m.n_dofs, (3 * bot[:, None] + np.arange(3)).ravel())
     a = NLAssembler(material, m, lame=(lam, mu), precision="fp64")
     newton = NewtonSolver(max_iterations=50, grad_norm=1e-13, cg_max_iterations=5000, cg_tolerance=1e-14)
-    sim = TransientSimulation(a, rho=1000.0, dt=2e-3, time_steps=6, checkpoint_every=every, newton=newton)
+    sim = TransientSimulation(a, rho=1000.0, dt=2e-3, time_steps=6, checkpoint_every=every, newton=newton,
+                              lump_mass=lump)
     # seeded initial velocity (the sine field of the box configurations
     # vanishes at every node of a 2x2x2 grid)
     X = m.nodes
@@ -107,8 +134,10 @@ def test_checkpointed_adjoint_equals_store_all():
             assert np.array_equal(u, w)
 
 
-def test_transient_adjoint_matches_finite_differences():
-    m, prob, a, sim, x0, v0, lam, mu = setup(2)
+@pytest.mark.parametrize("lump", [False, True])
+def test_transient_adjoint_matches_finite_differences(lump):
+    m, prob, a, sim, x0, v0, lam, mu = setup(2, lump=lump)
+    print(f"transient adjoint FD lump_mass={lump}", flush=True)
     xT, vT = sim.forward(x0, v0, prob)
     J0, gx, gv = objective(a, sim, xT, vT)
     dth, dx0, dv0 = sim.adjoint(gx, gv)
@@ -117,7 +146,7 @@ def test_transient_adjoint_matches_finite_differences():
     rng = np.random.default_rng(BASE_SEED + 24)
 
     def J_of(lm, mm, vv):
-        _, _, a2, s2, _, _, _, _ = setup(2, lm, mm)
+        _, _, a2, s2, _, _, _, _ = setup(2, lm, mm, lump=lump)
         xx, v2 = s2.forward(x0, vv, prob)
         return objective(a2, s2, xx, v2)[0]
 
