@@ -500,6 +500,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--path", default="auto", choices=["auto", "fused", "plain", "factored"],
                     help="auto: the launch-shape autotuner picks path and workgroup sizes (untimed setup)")
+    ap.add_argument("--gather-threads", type=int, default=0, choices=[0, 128, 256, 512],
+                    help="with --path plain/factored: the gather workgroup size (0: the library default, 256)")
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
     ap.add_argument("--grid", type=int, default=40)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -571,6 +573,8 @@ def main():
     torch.cuda.synchronize()
 
     tuning = a.autotune_dev(x.data_ptr()) if args.path == "auto" else None
+    if args.path in ("plain", "factored") and args.gather_threads:
+        a.set_tuning(args.path, args.gather_threads, 256)
 
     def step():
         a.energy_gradient_hessian_dev(x.data_ptr(), e.data_ptr(), g.data_ptr(), vals.data_ptr())
