@@ -500,7 +500,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--path", default="auto", choices=["auto", "fused", "plain", "factored"],
                     help="auto: the launch-shape autotuner picks path and workgroup sizes (untimed setup)")
-    ap.add_argument("--gather-threads", type=int, default=0, choices=[0, 128, 256, 512],
+    ap.add_argument("--gather-threads", type=int, default=0, choices=[0, 64, 128, 256, 512],
                     help="with --path plain/factored: the gather workgroup size (0: the library default, 256)")
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
     ap.add_argument("--grid", type=int, default=40)

@@ -124,7 +124,7 @@ pfgpu_status pfgpu_set_path(pfgpu_handle *h, int32_t path);
  * keeps the fastest.  Candidates: the fused path, the plain path with
  * block-gather workgroups of 128/256/512 threads and Hessian-kernel
  * workgroups of 64/128/256 threads, and the factored path with gather
- * workgroups of 128/256/512 threads.  All candidates give identical results
+ * workgroups of 64/128/256/512 threads.  All candidates give identical results
  * up to the summation order of the path. */
 typedef struct {
     int32_t path, gather_threads, hessian_threads; /* chosen configuration */
@@ -135,7 +135,7 @@ typedef struct {
     double ms_factored;                            /* best factored (negative: not available) */
 } pfgpu_tune_report;
 pfgpu_status pfgpu_autotune(pfgpu_handle *h, const double *x, pfgpu_tune_report *report);
-/* sets a configuration directly (gather_threads in {128,256,512},
+/* sets a configuration directly (gather_threads in {64,128,256,512},
  * hessian_threads in {64,128,256}) */
 pfgpu_status pfgpu_set_tuning(pfgpu_handle *h, int32_t path, int32_t gather_threads, int32_t hessian_threads);
 /* replaces ElasticityAssembler::update_lame_params (Assembler.hpp:147-156) */

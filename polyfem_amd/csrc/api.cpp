@@ -296,7 +296,8 @@ pfgpu_status pfgpu_set_tuning(pfgpu_handle *h, int32_t path, int32_t gather_thre
     return guard([&] {
         check_handle(h);
         require(path == PFGPU_PATH_FUSED || path == PFGPU_PATH_PLAIN || path == PFGPU_PATH_FACTORED, "unknown path");
-        require(gather_threads == 128 || gather_threads == 256 || gather_threads == 512, "gather_threads must be 128, 256 or 512");
+        require(gather_threads == 64 || gather_threads == 128 || gather_threads == 256 || gather_threads == 512,
+                "gather_threads must be 64, 128, 256 or 512");
         require(hessian_threads == 64 || hessian_threads == 128 || hessian_threads == 256,
                 "hessian_threads must be 64, 128 or 256");
         if (path != PFGPU_PATH_PLAIN && (h->material == PFGPU_SAINT_VENANT || h->material == PFGPU_FIXED_COROTATIONAL))
@@ -357,10 +358,11 @@ pfgpu_status pfgpu_autotune(pfgpu_handle *h, const double *x, pfgpu_tune_report 
             // the factored path keeps the plain winner's Hessian-kernel size
             // (used by its PSD-projected assemblies)
             h->tune_hessian_threads = r.hessian_threads;
+            const int GF[4] = {64, 128, 256, 512};
             if (h->material == PFGPU_NEOHOOKEAN || h->material == PFGPU_LINEAR_ELASTICITY)
-                for (int gi = 0; gi < 3; ++gi) {
+                for (int gi = 0; gi < 4; ++gi) {
                     h->path = PFGPU_PATH_FACTORED;
-                    h->tune_gather_threads = G[gi];
+                    h->tune_gather_threads = GF[gi];
                     try {
                         const double ms = time_config();
                         r.candidates++;
@@ -368,7 +370,7 @@ pfgpu_status pfgpu_autotune(pfgpu_handle *h, const double *x, pfgpu_tune_report 
                         if (ms < r.ms) {
                             r.ms = ms;
                             r.path = PFGPU_PATH_FACTORED;
-                            r.gather_threads = G[gi];
+                            r.gather_threads = GF[gi];
                         }
                     } catch (const Error &e) {
                         if (e.status != PFGPU_ERR_UNSUPPORTED) throw;

@@ -1029,7 +1029,9 @@ void factored_hessian(Handle &h, const double *x, void *values)
                     p.n_chunks, p.chunk.as<int32_t>(), p.slot.as<int32_t>(), p.dst.as<uint16_t>(), h.blk_out.as<int32_t>(),
                     p.meta.as<int32_t>(), h.fac_rec.as<T>(), static_cast<T *>(values));
             };
-            if (p.threads == 128)
+            if (p.threads == 64)
+                gath.template operator()<64>();
+            else if (p.threads == 128)
                 gath.template operator()<128>();
             else if (p.threads == 512)
                 gath.template operator()<512>();
@@ -1069,7 +1071,9 @@ void factored_energy_gradient_hessian(Handle &h, const double *x, double *energy
                     p.n_chunks, p.chunk.as<int32_t>(), p.slot.as<int32_t>(), p.dst.as<uint16_t>(), h.blk_out.as<int32_t>(),
                     p.meta.as<int32_t>(), h.fac_rec.as<T>(), static_cast<T *>(values));
             };
-            if (p.threads == 128)
+            if (p.threads == 64)
+                gath.template operator()<64>();
+            else if (p.threads == 128)
                 gath.template operator()<128>();
             else if (p.threads == 512)
                 gath.template operator()<512>();
@@ -1120,7 +1124,9 @@ void plain_hessian(Handle &h, const double *x, void *values, bool project_to_psd
                 p.n_chunks, p.chunk.as<int32_t>(), p.slot.as<int32_t>(), h.blk_out.as<int32_t>(), p.meta.as<int32_t>(),
                 h.hloc.as<T>(), static_cast<T *>(values));
         };
-        if (p.threads == 128)
+        if (p.threads == 64)
+            gath.template operator()<64>();
+        else if (p.threads == 128)
             gath.template operator()<128>();
         else if (p.threads == 512)
             gath.template operator()<512>();

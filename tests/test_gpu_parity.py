@@ -190,7 +190,7 @@ def test_autotune_and_tunings_agree():
     # the factored gather's chunking depends on its size, its per-block
     # summation order does not
     fref = None
-    for gt in (128, 256, 512):
+    for gt in (64, 128, 256, 512):
         a.set_tuning("factored", gt, 256)
         e, g, vals = a.assemble_energy_gradient_hessian(x)
         if fref is None:
@@ -200,7 +200,7 @@ def test_autotune_and_tunings_agree():
     xd = torch.tensor(x, device="cuda:0")
     rep = a.autotune_dev(xd.data_ptr())
     print("autotune:", rep, flush=True)
-    assert rep["candidates"] >= 12 and rep["ms"] > 0 and rep["ms_factored"] > 0
+    assert rep["candidates"] >= 13 and rep["ms"] > 0 and rep["ms_factored"] > 0
     e, g, vals = a.assemble_energy_gradient_hessian(x)
     _, _, hv = o.hessian(x)
     assert rel(vals, hv) < MIXED_TOL["hessian"] and rel(g, o.gradient(x)) < MIXED_TOL["gradient"]
