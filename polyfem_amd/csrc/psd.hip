@@ -137,6 +137,16 @@ __global__ __launch_bounds__(PSD_WAVE) void k_psd_project(int n_elements, T *__r
     for (int i = lane; i < N * N; i += PSD_WAVE) V[i / N][i % N] = i / N == i % N ? T(1) : T(0);
     __syncthreads();
     const JacobiRounds<N> &jr = rounds<N>();
+    // this lane's (pair, column) items of the row and column passes: item
+    // i = lane + 64 it is pair i / N, column i % N (the same in every round)
+    constexpr int ITEMS = NP * N, NIT = (ITEMS + PSD_WAVE - 1) / PSD_WAVE;
+    int ik[NIT], ij[NIT];
+#pragma unroll
+    for (int it = 0; it < NIT; ++it) {
+        const int i = lane + PSD_WAVE * it;
+        ik[it] = i < ITEMS ? i / N : -1;
+        ij[it] = i < ITEMS ? i % N : 0;
+    }
     for (int sweep = 0; sweep < MAX_SWEEPS; ++sweep) {
         T off = T(0), dia = T(0);
         for (int i = lane; i < N * N; i += PSD_WAVE) {
@@ -168,25 +178,34 @@ __global__ __launch_bounds__(PSD_WAVE) void k_psd_project(int n_elements, T *__r
                 CS[lane][1] = s;
             }
             __syncthreads();
-            // rows p, q of every pair (the pairs are disjoint)
-            for (int i = lane; i < NP * N; i += PSD_WAVE) {
-                const int k = i / N, j = i % N, p = PQ[k][0], q = PQ[k][1];
-                const T c = CS[k][0], s = CS[k][1], ap = A[p][j], aq = A[q][j];
-                A[p][j] = c * ap - s * aq;
-                A[q][j] = s * ap + c * aq;
+            // rows p, q of every pair (the pairs are disjoint), and the
+            // eigenvector matrix's columns p, q
+#pragma unroll
+            for (int it = 0; it < NIT; ++it) {
+                const int k = ik[it], j = ij[it];
+                if (k >= 0) {
+                    const int p = PQ[k][0], q = PQ[k][1];
+                    const T c = CS[k][0], s = CS[k][1], ap = A[p][j], aq = A[q][j];
+                    A[p][j] = c * ap - s * aq;
+                    A[q][j] = s * ap + c * aq;
+                    const T vp = V[j][p], vq = V[j][q];
+                    V[j][p] = c * vp - s * vq;
+                    V[j][q] = s * vp + c * vq;
+                }
             }
             __syncthreads();
-            // columns p, q, and the eigenvector matrix; the entry the rotation
-            // annihilates is set to zero rather than left at rounding level
-            for (int i = lane; i < NP * N; i += PSD_WAVE) {
-                const int k = i / N, j = i % N, p = PQ[k][0], q = PQ[k][1];
-                const T c = CS[k][0], s = CS[k][1];
-                const T ap = A[j][p], aq = A[j][q];
-                A[j][p] = j == q ? T(0) : c * ap - s * aq;
-                A[j][q] = j == p ? T(0) : s * ap + c * aq;
-                const T vp = V[j][p], vq = V[j][q];
-                V[j][p] = c * vp - s * vq;
-                V[j][q] = s * vp + c * vq;
+            // columns p, q; the entry the rotation annihilates is set to zero
+            // rather than left at rounding level
+#pragma unroll
+            for (int it = 0; it < NIT; ++it) {
+                const int k = ik[it], j = ij[it];
+                if (k >= 0) {
+                    const int p = PQ[k][0], q = PQ[k][1];
+                    const T c = CS[k][0], s = CS[k][1];
+                    const T ap = A[j][p], aq = A[j][q];
+                    A[j][p] = j == q ? T(0) : c * ap - s * aq;
+                    A[j][q] = j == p ? T(0) : s * ap + c * aq;
+                }
             }
             __syncthreads();
         }
