@@ -64,15 +64,16 @@ typedef enum {
  * values are float. */
 typedef enum { PFGPU_FP64 = 0, PFGPU_MIXED = 1 } pfgpu_precision;
 
-/* Assembly algorithm.  PLAIN (the default): element-local results written
- * to HBM, then gathered into the global vector / CSR in a fixed order.
- * FUSED: patch-owned rows accumulated on chip (LDS) in a fixed order, one
- * persistent launch.  FACTORED: PLAIN, except that the Hessian's element
- * blocks are not stored: per-element factors (per point F^{-T} J^{-T} and two
+/* Assembly algorithm.  PLAIN: element-local results written to HBM, then
+ * gathered into the global vector / CSR in a fixed order (the default for
+ * SaintVenant and FixedCorotational).  FACTORED (the default for NeoHookean
+ * and linear elasticity): PLAIN, except that the Hessian's element blocks
+ * are not stored: per-element factors (per point F^{-T} J^{-T} and two
  * curvature weights, and the reference metric) are written instead and the
- * CSR gather recomputes each block from them.  All three are deterministic
- * and give the same results to rounding.  FUSED and FACTORED cover
- * NeoHookean and linear elasticity. */
+ * CSR gather recomputes each block from them.  FUSED: patch-owned rows
+ * accumulated on chip (LDS) in a fixed order, one persistent launch.  All
+ * three are deterministic and give the same results to rounding.  FUSED and
+ * FACTORED cover NeoHookean and linear elasticity. */
 typedef enum { PFGPU_PATH_FUSED = 0, PFGPU_PATH_PLAIN = 1, PFGPU_PATH_FACTORED = 2 } pfgpu_path;
 
 typedef struct {

@@ -43,7 +43,7 @@ class NLAssembler:
     """Elasticity assembler on one tet mesh (P1 or P2, P1 geometry)."""
 
     def __init__(self, name: str, mesh, lame=None, young: float = None, poisson: float = None,
-                 precision: str = "fp64", path: str = "plain", quadrature_order: int = 0):
+                 precision: str = "fp64", path: str = None, quadrature_order: int = 0):
         if name not in _NAMES:
             raise ValueError(f"unknown material {name!r}; expected one of {sorted(_NAMES)}")
         self._name = name
@@ -66,9 +66,12 @@ class NLAssembler:
         check(lib().pfgpu_create(C.byref(md), C.byref(mat), _PRECISIONS[precision], C.byref(h)))
         self._h = h
         self.precision = precision
-        self.set_path(path)
         info = _ffi.Info()
         check(lib().pfgpu_info(self._h, C.byref(info)))
+        # None keeps the library's default path for the material
+        self.path = {v: k for k, v in _PATHS.items()}[info.path]
+        if path is not None:
+            self.set_path(path)
         self.n_dofs = info.n_dofs
         self.n_elements = info.n_elements
         self.n_vertices = info.n_vertices

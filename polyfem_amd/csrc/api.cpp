@@ -217,7 +217,8 @@ pfgpu_status pfgpu_create(const pfgpu_mesh_desc *mesh, const pfgpu_material_desc
         h->n_quad = qo == 1 ? 1 : 4;
         h->material = material->kind;
         h->precision = precision;
-        h->path = PFGPU_PATH_PLAIN;
+        h->path = material->kind == PFGPU_NEOHOOKEAN || material->kind == PFGPU_LINEAR_ELASTICITY ? PFGPU_PATH_FACTORED
+                                                                                                 : PFGPU_PATH_PLAIN;
         h->n_elements = m.n_elements;
         h->n_nodes = m.n_nodes;
         h->n_vertices = m.n_vertices;

@@ -3,6 +3,9 @@
 // entry in element order.  Deterministic, no atomics.  This is the
 // straightforward GPU restatement of NLAssembler's element loop
 // (Assembler.cpp:495-771) and the baseline the fused path is measured against.
+#include <algorithm>
+#include <thread>
+
 #include "dispatch.hpp"
 
 namespace pfgpu {
@@ -963,32 +966,38 @@ template <typename W, typename K> void build_plan(Handle &h, GatherPlan &p, int 
         if (deg > 4095) fail(PFGPU_ERR_UNSUPPORTED, "a node has more than 4095 neighbours");
         for (int32_t b = h.pat.brow_ptr[i]; b < h.pat.brow_ptr[i + 1]; ++b) meta[size_t(b)] = deg << 20;
     }
-    for (int ci = 0; ci < nc; ++ci) {
-        const int32_t k0 = ptr[size_t(c[ci])];
-        for (int32_t b = c[ci]; b < c[ci + 1]; ++b) {
-            meta[size_t(b)] |= (ptr[size_t(b)] - k0) | ((ptr[size_t(b) + 1] - ptr[size_t(b)]) << 10);
-            for (int32_t k = ptr[size_t(b)]; k < ptr[size_t(b) + 1]; ++k) slot[size_t(ci) * gs + size_t(k - k0)] = word(k);
-        }
-    }
-    if (by_element) {
-        // each chunk's slots ordered by element (stable), with the block-order
-        // position of every contribution; the element of word w is key(w)
-        std::vector<uint16_t> dst(size_t(nc) * gs, 0);
-        std::vector<int32_t> ord;
-        for (int ci = 0; ci < nc; ++ci) {
+    std::vector<uint16_t> dst(by_element ? size_t(nc) * gs : 0, 0);
+    // chunks are independent: fill them on the host's threads
+    auto fill = [&](int c0, int c1) {
+        std::vector<int64_t> kv;
+        std::vector<int32_t> w;
+        for (int ci = c0; ci < c1; ++ci) {
             const int32_t k0 = ptr[size_t(c[ci])], n = ptr[size_t(c[ci + 1])] - k0;
             int32_t *sl = slot.data() + size_t(ci) * gs;
-            ord.resize(size_t(n));
-            for (int32_t i = 0; i < n; ++i) ord[size_t(i)] = i;
-            std::stable_sort(ord.begin(), ord.end(), [&](int32_t u, int32_t v) { return key(sl[u]) < key(sl[v]); });
-            std::vector<int32_t> w(sl, sl + n);
+            for (int32_t b = c[ci]; b < c[ci + 1]; ++b) {
+                meta[size_t(b)] |= (ptr[size_t(b)] - k0) | ((ptr[size_t(b) + 1] - ptr[size_t(b)]) << 10);
+                for (int32_t k = ptr[size_t(b)]; k < ptr[size_t(b) + 1]; ++k) sl[k - k0] = word(k);
+            }
+            if (!by_element) continue;
+            // the chunk's slots ordered by element, ties in block order, with
+            // the block-order position of every contribution; the element of
+            // word w is key(w)
+            kv.resize(size_t(n));
+            w.assign(sl, sl + n);
+            for (int32_t i = 0; i < n; ++i) kv[size_t(i)] = (int64_t(key(w[size_t(i)])) << 16) | i;
+            std::sort(kv.begin(), kv.end());
             for (int32_t i = 0; i < n; ++i) {
-                sl[i] = w[size_t(ord[size_t(i)])];
-                dst[size_t(ci) * gs + size_t(i)] = uint16_t(ord[size_t(i)]);
+                const int32_t o = int32_t(kv[size_t(i)] & 0xffff);
+                sl[i] = w[size_t(o)];
+                dst[size_t(ci) * gs + size_t(i)] = uint16_t(o);
             }
         }
-        p.dst = upload(dst, h.stream);
-    }
+    };
+    const int nt = int(std::max(1u, std::min(16u, std::thread::hardware_concurrency())));
+    std::vector<std::thread> pool;
+    for (int t = 0; t < nt; ++t) pool.emplace_back(fill, int(int64_t(nc) * t / nt), int(int64_t(nc) * (t + 1) / nt));
+    for (std::thread &th : pool) th.join();
+    if (by_element) p.dst = upload(dst, h.stream);
     p.chunk = upload(c, h.stream);
     p.slot = upload(slot, h.stream);
     p.meta = upload(meta, h.stream);

@@ -244,3 +244,13 @@ def test_project_to_psd(order, name, precision):
     assert rel(a.hessian_values(x, project_to_psd=True), ref) < tol
     # the plain values are unaffected by an earlier projected call
     assert rel(a.hessian_values(x), plain) < (FP64_TOL if precision == "fp64" else MIXED_TOL["hessian"])
+
+
+def test_default_paths():
+    """Without a path argument the library takes the factored path for
+    NeoHookean and linear elasticity and the plain path for the others."""
+    m = kuhn_box(2, 2, 2, order=1)
+    for name, path in (("NeoHookean", "factored"), ("LinearElasticity", "factored"), ("SaintVenant", "plain"),
+                       ("FixedCorotational", "plain")):
+        a = NLAssembler(name, m, young=1e6, poisson=0.3)
+        assert a.path == path, (name, a.path)
