@@ -254,3 +254,38 @@ def test_default_paths():
                        ("FixedCorotational", "plain")):
         a = NLAssembler(name, m, young=1e6, poisson=0.3)
         assert a.path == path, (name, a.path)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("order", [1, 2])
+def test_irregular_mesh_all_paths(order, precision):
+    """Shuffled element order, shuffled vertex ids and perturbed vertices (no
+    lattice structure left in the numbering or the adjacency): every path
+    against the oracle, and the paths against each other."""
+    from polyfem_amd.mesh import tet_mesh
+    base = kuhn_box(4, 3, 3, order=1)
+    rng = np.random.default_rng(BASE_SEED + 150)
+    perm_v = rng.permutation(base.vertices.shape[0])
+    inv = np.empty_like(perm_v)
+    inv[perm_v] = np.arange(perm_v.size)
+    V = base.vertices[perm_v] + rng.normal(size=base.vertices.shape) * 0.03
+    T = inv[base.elem_vertices][rng.permutation(base.n_elements)]
+    m = tet_mesh(V, T, order=order)
+    lam, mu = random_lame(m.n_elements, seed=BASE_SEED + 151, E_mean=1e6, E_sigma=0.5)
+    x = smooth_displacement(m.nodes, 0.05, 1e-3, BASE_SEED + 152)
+    o = po.Oracle(m, lam, mu)
+    outer, inner, ref = o.hessian(x)
+    g_ref, e_ref = o.gradient(x), o.energy(x)
+    tol = FP64_TOL if precision == "fp64" else MIXED_TOL["hessian"]
+    print(f"irregular mesh order={order} {precision} E={m.n_elements}", flush=True)
+    vals = {}
+    for path in PATHS:
+        a = NLAssembler("NeoHookean", m, lame=(lam, mu), precision=precision, path=path)
+        e, g, v = a.assemble_energy_gradient_hessian(x)
+        go, gi = a.hessian_pattern()
+        assert np.array_equal(go, outer) and np.array_equal(gi, inner)
+        assert rel(v, ref) < tol, path
+        assert rel(g, g_ref) < (FP64_TOL if precision == "fp64" else MIXED_TOL["gradient"]), path
+        assert rel(e, e_ref) < (FP64_TOL if precision == "fp64" else MIXED_TOL["energy"]), path
+        vals[path] = v
+    assert rel(vals["factored"], vals["plain"]) < tol
