@@ -81,6 +81,7 @@ class DevBuf {
 struct GatherPlan {
     DevBuf chunk, slot, meta, dst;
     int n_chunks = 0, threads = 0;
+    unsigned grid = 0; // launch grid of a persistent gather over this plan (0: not yet known)
 };
 
 // v into dst: in place when the size matches (device pointers held elsewhere,

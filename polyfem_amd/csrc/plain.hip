@@ -1003,6 +1003,7 @@ template <typename W, typename K> void build_plan(Handle &h, GatherPlan &p, int 
     p.meta = upload(meta, h.stream);
     p.n_chunks = nc;
     p.threads = gs;
+    p.grid = 0;
 }
 
 // persistent grid of a kernel: the workgroups of gs threads resident at
@@ -1026,15 +1027,15 @@ void factored_hessian(Handle &h, const double *x, void *values)
     build_plan(
         h, h.fac_plan, h.tune_gather_threads, [&](int32_t k) { return h.h_contrib_code[size_t(k)]; }, true,
         [&](int32_t w) { return w / nb2; });
-    const GatherPlan &p = h.fac_plan;
+    GatherPlan &p = h.fac_plan;
     dispatch(h, [&]<typename T, int O, int Q, int M>() {
         if constexpr (M == MAT_NH || M == MAT_LE) {
             h.fac_rec.reserve(sizeof(T) * size_t(h.n_elements) * fac_size(Q));
             k_hfactors<T, O, Q, M><<<grid_for(h.n_elements), BLOCK, 0, h.stream>>>(h.mesh<T>(), x, h.fac_rec.as<T>());
             PF_LAUNCH_CHECK();
             auto gath = [&]<int GS>() {
-                k_gather_factored<T, O, Q, GS><<<persistent_grid(h, k_gather_factored<T, O, Q, GS>, p.n_chunks, GS), GS,
-                                                   0, h.stream>>>(
+                if (p.grid == 0) p.grid = persistent_grid(h, k_gather_factored<T, O, Q, GS>, p.n_chunks, GS);
+                k_gather_factored<T, O, Q, GS><<<p.grid, GS, 0, h.stream>>>(
                     p.n_chunks, p.chunk.as<int32_t>(), p.slot.as<int32_t>(), p.dst.as<uint16_t>(), h.blk_out.as<int32_t>(),
                     p.meta.as<int32_t>(), h.fac_rec.as<T>(), static_cast<T *>(values));
             };
@@ -1062,7 +1063,7 @@ void factored_energy_gradient_hessian(Handle &h, const double *x, double *energy
     build_plan(
         h, h.fac_plan, h.tune_gather_threads, [&](int32_t k) { return h.h_contrib_code[size_t(k)]; }, true,
         [&](int32_t w) { return w / nb2; });
-    const GatherPlan &p = h.fac_plan;
+    GatherPlan &p = h.fac_plan;
     const unsigned grid = grid_for(h.n_elements);
     h.loc.reserve(h.value_size() * size_t(h.n_elements) * h.n_local * 3);
     if (energy) h.partial.reserve(sizeof(double) * grid);
@@ -1075,8 +1076,8 @@ void factored_energy_gradient_hessian(Handle &h, const double *x, double *energy
             gather_rows(h, h.loc.get(), g);
             if (energy) sum_doubles(h, h.partial.as<double>(), int(grid), energy);
             auto gath = [&]<int GS>() {
-                k_gather_factored<T, O, Q, GS><<<persistent_grid(h, k_gather_factored<T, O, Q, GS>, p.n_chunks, GS), GS,
-                                                   0, h.stream>>>(
+                if (p.grid == 0) p.grid = persistent_grid(h, k_gather_factored<T, O, Q, GS>, p.n_chunks, GS);
+                k_gather_factored<T, O, Q, GS><<<p.grid, GS, 0, h.stream>>>(
                     p.n_chunks, p.chunk.as<int32_t>(), p.slot.as<int32_t>(), p.dst.as<uint16_t>(), h.blk_out.as<int32_t>(),
                     p.meta.as<int32_t>(), h.fac_rec.as<T>(), static_cast<T *>(values));
             };
