@@ -60,8 +60,7 @@ __device__ __forceinline__ double fma_t(double a, double b, double c) { return f
 __host__ __device__ constexpr int fac_size(int nq) { return (11 * nq + 6 + 3) & ~3; }
 
 // the point-independent tail of a factor record, and the record's store
-template <typename T, int NQ>
-__device__ __forceinline__ void finish_factors(const T *jit, T mu, T w, T (&r)[fac_size(NQ)], T *__restrict__ rec, int e)
+template <typename T, int NQ> __device__ __forceinline__ void factors_tail(const T *jit, T mu, T w, T (&r)[fac_size(NQ)])
 {
     constexpr int R = fac_size(NQ);
     T g6[6];
@@ -71,6 +70,13 @@ __device__ __forceinline__ void finish_factors(const T *jit, T mu, T w, T (&r)[f
     for (int i = 0; i < 6; ++i) r[11 * NQ + i] = cm * g6[i];
 #pragma unroll
     for (int i = 11 * NQ + 6; i < R; ++i) r[i] = T(0);
+}
+
+template <typename T, int NQ>
+__device__ __forceinline__ void finish_factors(const T *jit, T mu, T w, T (&r)[fac_size(NQ)], T *__restrict__ rec, int e)
+{
+    constexpr int R = fac_size(NQ);
+    factors_tail<T, NQ>(jit, mu, w, r);
     // records are R values (a multiple of 16 bytes) from a 256-byte aligned base
     T *out = static_cast<T *>(__builtin_assume_aligned(rec, 16)) + int64_t(e) * R;
 #pragma unroll
@@ -173,6 +179,68 @@ __global__ __launch_bounds__(BLOCK) void k_gradient(DevMesh<T> m, const double *
     if (partial) {
         const double s = block_sum(val);
         if (threadIdx.x == 0) partial[blockIdx.x] = s;
+    }
+}
+
+// k_gradient<..., true> with one wavefront per workgroup: the 64 elements'
+// local gradients and factor records are staged in LDS and stored as
+// contiguous runs (the per-thread form writes 120- and 208-byte records at a
+// per-lane stride); energy partials per 64 elements
+template <typename T, int ORDER, int NQ, int MAT>
+__global__ __launch_bounds__(64) void k_grad_factors(DevMesh<T> m, const double *__restrict__ x, T *__restrict__ gloc,
+                                                     double *__restrict__ partial, T *__restrict__ rec)
+{
+    constexpr int NB = NLocal<ORDER>::value, R = fac_size(NQ), G = 3 * NB;
+    __shared__ T srec[64 * (R + 1)];
+    __shared__ T sgr[64 * (G + 1)];
+    const int lane = threadIdx.x, e0 = blockIdx.x * 64, e = e0 + lane;
+    double val = 0;
+    if (e < m.n_elements) {
+        Elem<T, ORDER, NQ> el;
+        load_geometry(m, e, el);
+        gather(m, e, x, el.U);
+        const T w = el.det * T(quad_weight(NQ));
+        T g[NB][3];
+#pragma unroll
+        for (int a = 0; a < NB; ++a)
+#pragma unroll
+            for (int d = 0; d < 3; ++d) g[a][d] = T(0);
+        T acc = T(0);
+        T r[R];
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            Kin<T> k;
+            field_grad<T, ORDER, NQ>(q, el.U, el.jit, k.H);
+            kinematics(k);
+            acc += energy_density<T, MAT>(k, el.lam, el.mu) * w;
+            T P[9];
+            pk1<T, MAT>(k, el.lam, el.mu, P);
+            contract_add<T, ORDER, NQ>(q, P, el.jit, w, g);
+            hess_qp<T, MAT>(k, el.jit, el.lam, el.mu, w, r + 9 * q, r[9 * NQ + q], r[10 * NQ + q]);
+        }
+        factors_tail<T, NQ>(el.jit, el.mu, w, r);
+        val = double(acc);
+#pragma unroll
+        for (int i = 0; i < R; ++i) srec[lane * (R + 1) + i] = r[i];
+#pragma unroll
+        for (int a = 0; a < NB; ++a)
+#pragma unroll
+            for (int d = 0; d < 3; ++d) sgr[lane * (G + 1) + a * 3 + d] = g[a][d];
+    }
+    __syncthreads();
+    const int ne = min(64, m.n_elements - e0);
+    for (int i = lane; i < ne * R; i += 64) {
+        const int le = i / R;
+        rec[int64_t(e0) * R + i] = srec[le * (R + 1) + (i - le * R)];
+    }
+    for (int i = lane; i < ne * G; i += 64) {
+        const int le = i / G;
+        gloc[int64_t(e0) * G + i] = sgr[le * (G + 1) + (i - le * G)];
+    }
+    if (partial) {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) val += __shfl_xor(val, o, 64);
+        if (lane == 0) partial[blockIdx.x] = val;
     }
 }
 
@@ -1064,13 +1132,13 @@ void factored_energy_gradient_hessian(Handle &h, const double *x, double *energy
         h, h.fac_plan, h.tune_gather_threads, [&](int32_t k) { return h.h_contrib_code[size_t(k)]; }, true,
         [&](int32_t w) { return w / nb2; });
     GatherPlan &p = h.fac_plan;
-    const unsigned grid = grid_for(h.n_elements);
+    const unsigned grid = grid_for(h.n_elements, 64);
     h.loc.reserve(h.value_size() * size_t(h.n_elements) * h.n_local * 3);
     if (energy) h.partial.reserve(sizeof(double) * grid);
     dispatch(h, [&]<typename T, int O, int Q, int M>() {
         if constexpr (M == MAT_NH || M == MAT_LE) {
             h.fac_rec.reserve(sizeof(T) * size_t(h.n_elements) * fac_size(Q));
-            k_gradient<T, O, Q, M, true><<<grid, BLOCK, 0, h.stream>>>(
+            k_grad_factors<T, O, Q, M><<<grid, 64, 0, h.stream>>>(
                 h.mesh<T>(), x, h.loc.as<T>(), energy ? h.partial.as<double>() : nullptr, h.fac_rec.as<T>());
             PF_LAUNCH_CHECK();
             gather_rows(h, h.loc.get(), g);
