@@ -389,40 +389,49 @@ __global__ __launch_bounds__(HB) void k_hessian(DevMesh<T> m, const double *__re
     }
 }
 
-// local H(x) v; yloc[(e*NB + a)*3 + d]
+// local H(x) v; yloc[(e*NB + a)*3 + d].  One wavefront per workgroup: the
+// 64 elements' results are staged in LDS and stored as one contiguous run
+// (elements of finished problems compute and store nothing)
 template <typename T, int ORDER, int NQ, int MAT>
-__global__ __launch_bounds__(BLOCK) void k_hvp(DevMesh<T> m, const double *__restrict__ x, const double *__restrict__ v,
-                                               T *__restrict__ yloc, SegSkip skip)
+__global__ __launch_bounds__(64) void k_hvp(DevMesh<T> m, const double *__restrict__ x, const double *__restrict__ v,
+                                            T *__restrict__ yloc, SegSkip skip)
 {
-    constexpr int NB = NLocal<ORDER>::value;
-    const int e = blockIdx.x * BLOCK + threadIdx.x;
-    if (e >= m.n_elements || skip.element(e)) return;
-    Elem<T, ORDER, NQ> el;
-    load_geometry(m, e, el);
-    gather(m, e, x, el.U);
-    T V[NB][3];
-    gather(m, e, v, V);
-    const T w = el.det * T(quad_weight(NQ));
-    T y[NB][3];
+    constexpr int NB = NLocal<ORDER>::value, G = 3 * NB;
+    __shared__ T sy[64 * (G + 1)];
+    const int lane = threadIdx.x, e0 = blockIdx.x * 64, e = e0 + lane;
+    if (e < m.n_elements && !skip.element(e)) {
+        Elem<T, ORDER, NQ> el;
+        load_geometry(m, e, el);
+        gather(m, e, x, el.U);
+        T V[NB][3];
+        gather(m, e, v, V);
+        const T w = el.det * T(quad_weight(NQ));
+        T y[NB][3];
 #pragma unroll
-    for (int a = 0; a < NB; ++a)
+        for (int a = 0; a < NB; ++a)
 #pragma unroll
-        for (int d = 0; d < 3; ++d) y[a][d] = T(0);
+            for (int d = 0; d < 3; ++d) y[a][d] = T(0);
 #pragma unroll
-    for (int q = 0; q < NQ; ++q) {
-        Kin<T> k;
-        field_grad<T, ORDER, NQ>(q, el.U, el.jit, k.H);
-        if (MAT == MAT_NH || MAT == MAT_FC) kinematics(k);
-        T Gv[9], A[9];
-        field_grad<T, ORDER, NQ>(q, V, el.jit, Gv);
-        dstress<T, MAT>(k, el.lam, el.mu, Gv, A);
-        contract_add<T, ORDER, NQ>(q, A, el.jit, w, y);
+        for (int q = 0; q < NQ; ++q) {
+            Kin<T> k;
+            field_grad<T, ORDER, NQ>(q, el.U, el.jit, k.H);
+            if (MAT == MAT_NH || MAT == MAT_FC) kinematics(k);
+            T Gv[9], A[9];
+            field_grad<T, ORDER, NQ>(q, V, el.jit, Gv);
+            dstress<T, MAT>(k, el.lam, el.mu, Gv, A);
+            contract_add<T, ORDER, NQ>(q, A, el.jit, w, y);
+        }
+#pragma unroll
+        for (int a = 0; a < NB; ++a)
+#pragma unroll
+            for (int d = 0; d < 3; ++d) sy[lane * (G + 1) + a * 3 + d] = y[a][d];
     }
-    T *out = yloc + int64_t(e) * NB * 3;
-#pragma unroll
-    for (int a = 0; a < NB; ++a)
-#pragma unroll
-        for (int d = 0; d < 3; ++d) out[a * 3 + d] = y[a][d];
+    __syncthreads();
+    const int ne = min(64, m.n_elements - e0);
+    for (int i = lane; i < ne * G; i += 64) {
+        const int le = i / G;
+        if (!skip.element(e0 + le)) yloc[int64_t(e0) * G + i] = sy[le * (G + 1) + (i - le * G)];
+    }
 }
 
 // (d grad/d lambda) dl + (d grad/d mu) dm, element-local
@@ -1216,10 +1225,10 @@ void plain_hessian(Handle &h, const double *x, void *values, bool project_to_psd
 
 void plain_hvp(Handle &h, const double *x, const double *v, double *y)
 {
-    const unsigned grid = grid_for(h.n_elements);
+    const unsigned grid = grid_for(h.n_elements, 64);
     h.loc.reserve(h.value_size() * size_t(h.n_elements) * h.n_local * 3);
     dispatch(h, [&]<typename T, int O, int Q, int M>() {
-        k_hvp<T, O, Q, M><<<grid, BLOCK, 0, h.stream>>>(h.mesh<T>(), x, v, h.loc.as<T>(), h.skip);
+        k_hvp<T, O, Q, M><<<grid, 64, 0, h.stream>>>(h.mesh<T>(), x, v, h.loc.as<T>(), h.skip);
         PF_LAUNCH_CHECK();
         gather_rows(h, h.loc.get(), y);
     });
