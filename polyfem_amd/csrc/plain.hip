@@ -135,12 +135,10 @@ __global__ __launch_bounds__(BLOCK) void k_energy_lame_grad(DevMesh<T> m, const 
     out[m.n_elements + e] = double(am);
 }
 
-// local gradient (+ energy partials); gloc[(e*NB + a)*3 + d].  FAC: also the
-// Hessian factor record of the element (k_hfactors' record), from the same
-// kinematics
-template <typename T, int ORDER, int NQ, int MAT, bool FAC = false>
+// local gradient (+ energy partials); gloc[(e*NB + a)*3 + d]
+template <typename T, int ORDER, int NQ, int MAT>
 __global__ __launch_bounds__(BLOCK) void k_gradient(DevMesh<T> m, const double *__restrict__ x, T *__restrict__ gloc,
-                                                    double *__restrict__ partial, T *__restrict__ rec = nullptr)
+                                                    double *__restrict__ partial)
 {
     constexpr int NB = NLocal<ORDER>::value;
     const int e = blockIdx.x * BLOCK + threadIdx.x;
@@ -156,7 +154,6 @@ __global__ __launch_bounds__(BLOCK) void k_gradient(DevMesh<T> m, const double *
 #pragma unroll
             for (int d = 0; d < 3; ++d) g[a][d] = T(0);
         T acc = T(0);
-        T r[FAC ? fac_size(NQ) : 1];
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
             Kin<T> k;
@@ -166,9 +163,7 @@ __global__ __launch_bounds__(BLOCK) void k_gradient(DevMesh<T> m, const double *
             T P[9];
             pk1<T, MAT>(k, el.lam, el.mu, P);
             contract_add<T, ORDER, NQ>(q, P, el.jit, w, g);
-            if constexpr (FAC) hess_qp<T, MAT>(k, el.jit, el.lam, el.mu, w, r + 9 * q, r[9 * NQ + q], r[10 * NQ + q]);
         }
-        if constexpr (FAC) finish_factors<T, NQ>(el.jit, el.mu, w, r, rec, e);
         val = double(acc);
         T *out = gloc + int64_t(e) * NB * 3;
 #pragma unroll
@@ -182,10 +177,11 @@ __global__ __launch_bounds__(BLOCK) void k_gradient(DevMesh<T> m, const double *
     }
 }
 
-// k_gradient<..., true> with one wavefront per workgroup: the 64 elements'
-// local gradients and factor records are staged in LDS and stored as
-// contiguous runs (the per-thread form writes 120- and 208-byte records at a
-// per-lane stride); energy partials per 64 elements
+// k_gradient plus the element's Hessian factor record (k_hfactors' record)
+// from the same kinematics, with one wavefront per workgroup: the 64
+// elements' local gradients and records are staged in LDS and stored as
+// contiguous runs (a thread's own 120- and 208-byte results would be stored
+// at a per-lane stride); energy partials per 64 elements
 template <typename T, int ORDER, int NQ, int MAT>
 __global__ __launch_bounds__(64) void k_grad_factors(DevMesh<T> m, const double *__restrict__ x, T *__restrict__ gloc,
                                                      double *__restrict__ partial, T *__restrict__ rec)
