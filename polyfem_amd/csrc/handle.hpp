@@ -26,8 +26,8 @@ struct pfgpu_handle {
     pfgpu::DevBuf brow_ptr, blk_out;
     std::vector<int32_t> h_contrib_ptr, h_contrib_code;
     pfgpu::GatherPlan blk_plan, fac_plan;
-    // chunks of the segmented node-row gather
-    pfgpu::DevBuf row_chunk;
+    // plan of the segmented node-row gather (chunks, padded slots, row metadata)
+    pfgpu::DevBuf row_chunk, row_slot, row_meta;
     int n_row_chunks = 0;
     // factored Hessian (PFGPU_PATH_FACTORED): per-element factor records
     pfgpu::DevBuf fac_rec;

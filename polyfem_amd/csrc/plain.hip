@@ -666,18 +666,24 @@ __device__ __forceinline__ int xcd_chunk(int n_chunks)
 }
 inline unsigned xcd_grid(int n_chunks) { return unsigned((n_chunks + N_XCD - 1) / N_XCD * N_XCD); }
 
+// Node-row gather of element-local 3-vectors over a row plan: chunk c covers
+// the rows chunk[c] .. chunk[c+1]-1, whose at most SEG contributions (local
+// vector indices) sit in the slots c SEG .. (c+1) SEG - 1 (-1 when unused);
+// rmeta[i] = q0 | count << 10 gives row i's contributions within its chunk.
+// Both dependent load chains (slot, then the local vector; chunk start,
+// then the row metadata) are two deep and issued before the barrier.
 template <typename T>
 __global__ __launch_bounds__(BLOCK) void k_gather_rows_seg(int n_chunks, const int32_t *__restrict__ chunk,
-                                                           const int32_t *__restrict__ ptr, const int32_t *__restrict__ inc,
-                                                           const T *__restrict__ loc, double *__restrict__ out,
-                                                           SegSkip skip)
+                                                           const int32_t *__restrict__ slot,
+                                                           const int32_t *__restrict__ rmeta, const T *__restrict__ loc,
+                                                           double *__restrict__ out, SegSkip skip)
 {
     __shared__ double sv[SEG * 3];
     const int c = xcd_chunk(n_chunks);
     if (c >= n_chunks) return;
-    const int i0 = chunk[c], i1 = chunk[c + 1];
-    const int k0 = ptr[i0], k1 = ptr[i1];
     const int t = threadIdx.x;
+    const int32_t w = slot[int64_t(c) * SEG + t];
+    const int i0 = chunk[c], i1 = chunk[c + 1];
     // rows of skipped problems are written as zero; a chunk lies in at most a
     // few problems, so a fully skipped chunk reads nothing
     if (skip.node(i0) && skip.node(i1 - 1)) {
@@ -687,28 +693,40 @@ __global__ __launch_bounds__(BLOCK) void k_gather_rows_seg(int n_chunks, const i
         for (int i = i0; i < i1 && all; ++i) all = skip.node(i);
         if (all) return;
     }
-    if (k0 + t < k1) {
-        const T *l = loc + int64_t(inc[k0 + t]) * 3;
-        sv[t * 3] = double(l[0]);
-        sv[t * 3 + 1] = double(l[1]);
-        sv[t * 3 + 2] = double(l[2]);
+    T l0 = T(0), l1 = T(0), l2 = T(0);
+    if (w >= 0) {
+        const T *l = loc + int64_t(w) * 3;
+        l0 = l[0];
+        l1 = l[1];
+        l2 = l[2];
+    }
+    const int i = i0 + t;
+    const int32_t mt = i < i1 ? rmeta[i] : 0;
+    if (w >= 0) {
+        sv[t * 3] = double(l0);
+        sv[t * 3 + 1] = double(l1);
+        sv[t * 3 + 2] = double(l2);
     }
     __syncthreads();
-    for (int i = i0 + t; i < i1; i += BLOCK) {
-        double s0 = 0, s1 = 0, s2 = 0;
-        if (skip.node(i)) {
-            out[3 * int64_t(i)] = out[3 * int64_t(i) + 1] = out[3 * int64_t(i) + 2] = 0.0;
-            continue;
+    auto row = [&](int ii, int32_t m) {
+        if (skip.node(ii)) {
+            out[3 * int64_t(ii)] = out[3 * int64_t(ii) + 1] = out[3 * int64_t(ii) + 2] = 0.0;
+            return;
         }
-        for (int k = ptr[i] - k0; k < ptr[i + 1] - k0; ++k) {
+        const int q0 = m & 1023, q1 = q0 + (m >> 10);
+        double s0 = 0, s1 = 0, s2 = 0;
+        for (int k = q0; k < q1; ++k) {
             s0 += sv[k * 3];
             s1 += sv[k * 3 + 1];
             s2 += sv[k * 3 + 2];
         }
-        out[3 * int64_t(i)] = s0;
-        out[3 * int64_t(i) + 1] = s1;
-        out[3 * int64_t(i) + 2] = s2;
-    }
+        out[3 * int64_t(ii)] = s0;
+        out[3 * int64_t(ii) + 1] = s1;
+        out[3 * int64_t(ii) + 2] = s2;
+    };
+    if (i < i1) row(i, mt);
+    // rows without contributions (nodes no element uses)
+    for (int ii = i + BLOCK; ii < i1; ii += BLOCK) row(ii, rmeta[ii]);
 }
 
 // Gather plan of a workgroup size GS (built on the host, see build_plan):
@@ -975,14 +993,25 @@ template <typename P> std::vector<int32_t> segment_chunks(const P *ptr, int64_t 
     return c;
 }
 
+// the row plan of k_gather_rows_seg, from the node incidence
 void ensure_row_chunks(Handle &h)
 {
     if (h.row_chunk.get()) return;
-    std::vector<int32_t> ptr(size_t(h.n_nodes) + 1);
-    PF_HIP(hipMemcpy(ptr.data(), h.node_ptr.get(), sizeof(int32_t) * ptr.size(), hipMemcpyDeviceToHost));
+    const std::vector<int32_t> &ptr = h.pat.node_ptr, &inc = h.pat.node_inc;
     const std::vector<int32_t> c = segment_chunks(ptr.data(), h.n_nodes);
+    const int nc = int(c.size()) - 1;
+    std::vector<int32_t> slot(size_t(nc) * SEG, -1), rmeta(size_t(h.n_nodes));
+    for (int ci = 0; ci < nc; ++ci) {
+        const int32_t k0 = ptr[size_t(c[ci])];
+        for (int32_t i = c[ci]; i < c[ci + 1]; ++i) {
+            rmeta[size_t(i)] = (ptr[size_t(i)] - k0) | ((ptr[size_t(i) + 1] - ptr[size_t(i)]) << 10);
+            for (int32_t k = ptr[size_t(i)]; k < ptr[size_t(i) + 1]; ++k) slot[size_t(ci) * SEG + size_t(k - k0)] = inc[size_t(k)];
+        }
+    }
+    h.row_slot = upload(slot, h.stream);
+    h.row_meta = upload(rmeta, h.stream);
     h.row_chunk = upload(c, h.stream);
-    h.n_row_chunks = int(c.size()) - 1;
+    h.n_row_chunks = nc;
 }
 
 void gather_rows(Handle &h, const void *loc, double *out)
@@ -990,12 +1019,12 @@ void gather_rows(Handle &h, const void *loc, double *out)
     ensure_row_chunks(h);
     if (h.precision == PFGPU_MIXED)
         k_gather_rows_seg<float><<<xcd_grid(h.n_row_chunks), BLOCK, 0, h.stream>>>(
-            h.n_row_chunks, h.row_chunk.as<int32_t>(), h.node_ptr.as<int32_t>(), h.node_inc.as<int32_t>(), static_cast<const float *>(loc), out,
-            h.skip);
+            h.n_row_chunks, h.row_chunk.as<int32_t>(), h.row_slot.as<int32_t>(), h.row_meta.as<int32_t>(),
+            static_cast<const float *>(loc), out, h.skip);
     else
         k_gather_rows_seg<double><<<xcd_grid(h.n_row_chunks), BLOCK, 0, h.stream>>>(
-            h.n_row_chunks, h.row_chunk.as<int32_t>(), h.node_ptr.as<int32_t>(), h.node_inc.as<int32_t>(), static_cast<const double *>(loc), out,
-            h.skip);
+            h.n_row_chunks, h.row_chunk.as<int32_t>(), h.row_slot.as<int32_t>(), h.row_meta.as<int32_t>(),
+            static_cast<const double *>(loc), out, h.skip);
     PF_LAUNCH_CHECK();
 }
 
