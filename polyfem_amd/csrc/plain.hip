@@ -1120,7 +1120,8 @@ template <typename K> unsigned persistent_grid(Handle &h, K kernel, int n_chunks
     return std::min(xcd_grid(n_chunks), unsigned((g + N_XCD - 1) / N_XCD * N_XCD));
 }
 
-void factored_hessian(Handle &h, const double *x, void *values)
+// the factored route's plan (element-ordered slots) for the tuned gather size
+GatherPlan &factored_plan(Handle &h)
 {
     if (h.material != MAT_NH && h.material != MAT_LE)
         fail(PFGPU_ERR_UNSUPPORTED, "the factored Hessian covers NeoHookean and linear elasticity");
@@ -1129,43 +1130,48 @@ void factored_hessian(Handle &h, const double *x, void *values)
     build_plan(
         h, h.fac_plan, h.tune_gather_threads, [&](int32_t k) { return h.h_contrib_code[size_t(k)]; }, true,
         [&](int32_t w) { return w / nb2; });
-    GatherPlan &p = h.fac_plan;
+    return h.fac_plan;
+}
+
+// the CSR values from the factor records in h.fac_rec
+template <typename T, int O, int Q> void launch_factored_gather(Handle &h, GatherPlan &p, void *values)
+{
+    auto gath = [&]<int GS>() {
+        if (p.grid == 0) p.grid = persistent_grid(h, k_gather_factored<T, O, Q, GS>, p.n_chunks, GS);
+        k_gather_factored<T, O, Q, GS><<<p.grid, GS, 0, h.stream>>>(p.n_chunks, p.chunk.as<int32_t>(), p.slot.as<int32_t>(),
+                                                                    p.dst.as<uint16_t>(), h.blk_out.as<int32_t>(),
+                                                                    p.meta.as<int32_t>(), h.fac_rec.as<T>(),
+                                                                    static_cast<T *>(values));
+    };
+    if (p.threads == 64)
+        gath.template operator()<64>();
+    else if (p.threads == 128)
+        gath.template operator()<128>();
+    else if (p.threads == 512)
+        gath.template operator()<512>();
+    else
+        gath.template operator()<256>();
+    PF_LAUNCH_CHECK();
+}
+
+void factored_hessian(Handle &h, const double *x, void *values)
+{
+    GatherPlan &p = factored_plan(h);
     dispatch(h, [&]<typename T, int O, int Q, int M>() {
         if constexpr (M == MAT_NH || M == MAT_LE) {
             h.fac_rec.reserve(sizeof(T) * size_t(h.n_elements) * fac_size(Q));
             k_hfactors<T, O, Q, M><<<grid_for(h.n_elements), BLOCK, 0, h.stream>>>(h.mesh<T>(), x, h.fac_rec.as<T>());
             PF_LAUNCH_CHECK();
-            auto gath = [&]<int GS>() {
-                if (p.grid == 0) p.grid = persistent_grid(h, k_gather_factored<T, O, Q, GS>, p.n_chunks, GS);
-                k_gather_factored<T, O, Q, GS><<<p.grid, GS, 0, h.stream>>>(
-                    p.n_chunks, p.chunk.as<int32_t>(), p.slot.as<int32_t>(), p.dst.as<uint16_t>(), h.blk_out.as<int32_t>(),
-                    p.meta.as<int32_t>(), h.fac_rec.as<T>(), static_cast<T *>(values));
-            };
-            if (p.threads == 64)
-                gath.template operator()<64>();
-            else if (p.threads == 128)
-                gath.template operator()<128>();
-            else if (p.threads == 512)
-                gath.template operator()<512>();
-            else
-                gath.template operator()<256>();
+            launch_factored_gather<T, O, Q>(h, p, values);
         }
     });
-    PF_LAUNCH_CHECK();
 }
 
 // E + G + H on the factored route: one element pass writes the local
 // gradient, the energy partials and the Hessian factor records
 void factored_energy_gradient_hessian(Handle &h, const double *x, double *energy, double *g, void *values)
 {
-    if (h.material != MAT_NH && h.material != MAT_LE)
-        fail(PFGPU_ERR_UNSUPPORTED, "the factored Hessian covers NeoHookean and linear elasticity");
-    ensure_plain_hessian_maps(h);
-    const int nb2 = h.n_local * h.n_local;
-    build_plan(
-        h, h.fac_plan, h.tune_gather_threads, [&](int32_t k) { return h.h_contrib_code[size_t(k)]; }, true,
-        [&](int32_t w) { return w / nb2; });
-    GatherPlan &p = h.fac_plan;
+    GatherPlan &p = factored_plan(h);
     const unsigned grid = grid_for(h.n_elements, 64);
     h.loc.reserve(h.value_size() * size_t(h.n_elements) * h.n_local * 3);
     if (energy) h.partial.reserve(sizeof(double) * grid);
@@ -1177,23 +1183,9 @@ void factored_energy_gradient_hessian(Handle &h, const double *x, double *energy
             PF_LAUNCH_CHECK();
             gather_rows(h, h.loc.get(), g);
             if (energy) sum_doubles(h, h.partial.as<double>(), int(grid), energy);
-            auto gath = [&]<int GS>() {
-                if (p.grid == 0) p.grid = persistent_grid(h, k_gather_factored<T, O, Q, GS>, p.n_chunks, GS);
-                k_gather_factored<T, O, Q, GS><<<p.grid, GS, 0, h.stream>>>(
-                    p.n_chunks, p.chunk.as<int32_t>(), p.slot.as<int32_t>(), p.dst.as<uint16_t>(), h.blk_out.as<int32_t>(),
-                    p.meta.as<int32_t>(), h.fac_rec.as<T>(), static_cast<T *>(values));
-            };
-            if (p.threads == 64)
-                gath.template operator()<64>();
-            else if (p.threads == 128)
-                gath.template operator()<128>();
-            else if (p.threads == 512)
-                gath.template operator()<512>();
-            else
-                gath.template operator()<256>();
+            launch_factored_gather<T, O, Q>(h, p, values);
         }
     });
-    PF_LAUNCH_CHECK();
 }
 
 void plain_hessian(Handle &h, const double *x, void *values, bool project_to_psd)
