@@ -160,16 +160,24 @@ __global__ __launch_bounds__(BLOCK) void k_pcg_update(int n_nodes, double *__res
     double rz = 0, rr = 0;
     for (int i = blockIdx.x * BLOCK + threadIdx.x; i < n_nodes; i += DOT_BLOCKS * BLOCK) {
         const int64_t o = 3 * int64_t(i);
-        double ri[3], zi[3];
+        // every load of the node first, then the masked update
+        double ri[3], zi[3], pi[3], qi[3], yi[3];
+        bool upd[3];
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
             ri[d] = r[o + d];
-            if (a != 0.0 && !(fixed && fixed[o + d])) {
-                y[o + d] = fma(a, p[o + d], y[o + d]);
-                ri[d] = fma(-a, q[o + d], ri[d]);
+            pi[d] = p[o + d];
+            qi[d] = q[o + d];
+            yi[d] = y[o + d];
+            upd[d] = a != 0.0 && !(fixed && fixed[o + d]);
+        }
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+            if (upd[d]) {
+                y[o + d] = fma(a, pi[d], yi[d]);
+                ri[d] = fma(-a, qi[d], ri[d]);
                 r[o + d] = ri[d];
             }
-        }
         precondition(pinv, i, ri, zi);
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
