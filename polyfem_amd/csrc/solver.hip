@@ -1211,16 +1211,24 @@ __global__ __launch_bounds__(BLOCK) void k_chunk_pcg_update(double *__restrict__
     double rz = 0, rr = 0;
     for (int i = cb[c] / 3 + threadIdx.x; i < cb[c + 1] / 3; i += BLOCK) {
         const int64_t o = 3 * int64_t(i);
-        double ri[3], zi[3];
+        // every load of the node first, then the masked update
+        double ri[3], zi[3], pi[3], qi[3], yi[3];
+        bool upd[3];
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
             ri[d] = r[o + d];
-            if (a != 0.0 && !mask[o + d]) {
-                y[o + d] = fma(a, p[o + d], y[o + d]);
-                ri[d] = fma(-a, q[o + d], ri[d]);
+            pi[d] = p[o + d];
+            qi[d] = q[o + d];
+            yi[d] = y[o + d];
+            upd[d] = a != 0.0 && !mask[o + d];
+        }
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+            if (upd[d]) {
+                y[o + d] = fma(a, pi[d], yi[d]);
+                ri[d] = fma(-a, qi[d], ri[d]);
                 r[o + d] = ri[d];
             }
-        }
         precondition(pinv, i, ri, zi);
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
