@@ -77,7 +77,7 @@ struct pfgpu_handle {
     int solver_hessian = PFGPU_SOLVER_HESSIAN_AUTO, solver_precond = PFGPU_PRECOND_BLOCK_JACOBI;
     bool op_valid = false, op_assembled = false, op_precond = false;
     pfgpu::SegSkip skip; // set only inside a batched CG
-    pfgpu::DevBuf solver_vals, pinv, diag_pos;
+    pfgpu::DevBuf solver_vals, pinv, diag_pos, node_diag;
     hipStream_t own_stream = nullptr; // used when the caller's stream is the legacy default
     ~pfgpu_handle();
 
@@ -111,6 +111,7 @@ void energy_lame_gradient(Handle &h, const double *x, double *out);
 void plain_shape_vjp(Handle &h, const double *x, const double *p, double *out);
 void ensure_plain_hessian_maps(Handle &h);
 void factored_energy_gradient_hessian(Handle &h, const double *x, double *energy, double *g, void *values);
+void factored_node_diagonal(Handle &h, const double *x, void *diag);
 
 // ---- fused path (fused.hip) -------------------------------------------
 void ensure_fused_tables(Handle &h);

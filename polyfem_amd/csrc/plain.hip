@@ -943,6 +943,78 @@ __global__ __launch_bounds__(GS) void k_gather_factored(int n_chunks, const int3
     }
 }
 
+// Diagonal blocks of the Hessian per node from the factor records, over the
+// row plan of k_gather_rows_seg (slot word e NB + a): H_e(a, a) =
+// (cm sum_q G_a . G_a) I + sum_q (cl_q + cc_q) k_a k_a^T, summed over the
+// node's (element, local node) pairs in incidence order into diag[9 i ..]
+// (row-major 3x3).  The block-Jacobi preconditioner of a matrix-free
+// operator needs nothing else of the Hessian.
+template <typename T, int ORDER, int NQ>
+__global__ __launch_bounds__(BLOCK) void k_node_diag(int n_chunks, const int32_t *__restrict__ chunk,
+                                                     const int32_t *__restrict__ slot, const int32_t *__restrict__ rmeta,
+                                                     const T *__restrict__ rec, T *__restrict__ diag)
+{
+    constexpr int NB = NLocal<ORDER>::value, R = fac_size(NQ);
+    __shared__ T sv[SEG * 6];
+    __shared__ T srg[NQ * NB * 3], srgg[NB * NB * 6];
+    const int c = xcd_chunk(n_chunks);
+    if (c >= n_chunks) return;
+    const int t = threadIdx.x;
+    const int32_t w = slot[int64_t(c) * SEG + t];
+    const int i0 = chunk[c], i1 = chunk[c + 1];
+    int a = 0;
+    T r[R];
+    if (w >= 0) {
+        const int e = w / NB;
+        a = w - e * NB;
+        const T *rp = static_cast<const T *>(__builtin_assume_aligned(rec, 16)) + int64_t(e) * R;
+#pragma unroll
+        for (int k = 0; k < R; ++k) r[k] = rp[k];
+    }
+    const int i = i0 + t;
+    const int32_t mt = i < i1 ? rmeta[i] : 0;
+    const RefTab<T, ORDER, NQ> &tb = ref_tab<T, ORDER, NQ>();
+    for (int k = t; k < NQ * NB * 3; k += BLOCK) srg[k] = tb.rg[k];
+    for (int k = t; k < NB * NB * 6; k += BLOCK) srgg[k] = tb.rgg[k];
+    __syncthreads();
+    if (w >= 0) {
+        const T *cr = srgg + (a * NB + a) * 6;
+        const T gd = r[11 * NQ] * cr[0] + r[11 * NQ + 1] * cr[1] + r[11 * NQ + 2] * cr[2] + r[11 * NQ + 3] * cr[3] +
+                     r[11 * NQ + 4] * cr[4] + r[11 * NQ + 5] * cr[5];
+        T v6[6] = {gd, gd, gd, T(0), T(0), T(0)}; // 00 11 22 01 02 12
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            const T *M = r + 9 * q, *ra = srg + (q * NB + a) * 3;
+            T ka[3];
+#pragma unroll
+            for (int mm = 0; mm < 3; ++mm) ka[mm] = fma_t(M[mm * 3 + 2], ra[2], fma_t(M[mm * 3 + 1], ra[1], M[mm * 3] * ra[0]));
+            const T s = r[9 * NQ + q] + r[10 * NQ + q];
+            v6[0] = fma_t(s * ka[0], ka[0], v6[0]);
+            v6[1] = fma_t(s * ka[1], ka[1], v6[1]);
+            v6[2] = fma_t(s * ka[2], ka[2], v6[2]);
+            v6[3] = fma_t(s * ka[0], ka[1], v6[3]);
+            v6[4] = fma_t(s * ka[0], ka[2], v6[4]);
+            v6[5] = fma_t(s * ka[1], ka[2], v6[5]);
+        }
+#pragma unroll
+        for (int k = 0; k < 6; ++k) sv[t * 6 + k] = v6[k];
+    }
+    __syncthreads();
+    auto row = [&](int ii, int32_t m) {
+        const int q0 = m & 1023, q1 = q0 + (m >> 10);
+        T acc[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
+        for (int k = q0; k < q1; ++k)
+#pragma unroll
+            for (int j = 0; j < 6; ++j) acc[j] += sv[k * 6 + j];
+        T *o = diag + 9 * int64_t(ii);
+        o[0] = acc[0], o[1] = acc[3], o[2] = acc[4];
+        o[3] = acc[3], o[4] = acc[1], o[5] = acc[5];
+        o[6] = acc[4], o[7] = acc[5], o[8] = acc[2];
+    };
+    if (i < i1) row(i, mt);
+    for (int ii = i + BLOCK; ii < i1; ii += BLOCK) row(ii, rmeta[ii]);
+}
+
 } // namespace
 
 // ------------------------------------------------------------------ launchers
@@ -1184,6 +1256,26 @@ void factored_energy_gradient_hessian(Handle &h, const double *x, double *energy
             gather_rows(h, h.loc.get(), g);
             if (energy) sum_doubles(h, h.partial.as<double>(), int(grid), energy);
             launch_factored_gather<T, O, Q>(h, p, values);
+        }
+    });
+}
+
+// the Hessian's diagonal block of every node (n_nodes x 9, value type of the
+// handle) from the factor records; NeoHookean and linear elasticity
+void factored_node_diagonal(Handle &h, const double *x, void *diag)
+{
+    if (h.material != MAT_NH && h.material != MAT_LE)
+        fail(PFGPU_ERR_UNSUPPORTED, "the factored Hessian covers NeoHookean and linear elasticity");
+    ensure_row_chunks(h);
+    dispatch(h, [&]<typename T, int O, int Q, int M>() {
+        if constexpr (M == MAT_NH || M == MAT_LE) {
+            h.fac_rec.reserve(sizeof(T) * size_t(h.n_elements) * fac_size(Q));
+            k_hfactors<T, O, Q, M><<<grid_for(h.n_elements), BLOCK, 0, h.stream>>>(h.mesh<T>(), x, h.fac_rec.as<T>());
+            PF_LAUNCH_CHECK();
+            k_node_diag<T, O, Q><<<xcd_grid(h.n_row_chunks), BLOCK, 0, h.stream>>>(
+                h.n_row_chunks, h.row_chunk.as<int32_t>(), h.row_slot.as<int32_t>(), h.row_meta.as<int32_t>(),
+                h.fac_rec.as<T>(), static_cast<T *>(diag));
+            PF_LAUNCH_CHECK();
         }
     });
 }

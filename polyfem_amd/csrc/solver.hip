@@ -265,21 +265,25 @@ __global__ __launch_bounds__(BLOCK) void k_bsr_spmv(int n_nodes, const int32_t *
 // inverse of every node's 3x3 diagonal block of w H + M, with the rows and
 // columns of fixed dofs replaced by the identity; a block that is not
 // positive definite (an indefinite tangent) falls back to 1 / |diagonal|
+// The diagonal blocks come from the assembled values (vals, pattern layout)
+// or, when ndiag is given, from a node-diagonal array (9 values per node).
 template <typename T>
 __global__ __launch_bounds__(BLOCK) void k_block_jacobi(int n_nodes, const int32_t *__restrict__ brow_ptr,
                                                         const int32_t *__restrict__ diag_pos, const T *__restrict__ vals,
-                                                        const double *__restrict__ mass, double w,
-                                                        const uint8_t *__restrict__ fixed, float *__restrict__ pinv)
+                                                        const T *__restrict__ ndiag, const double *__restrict__ mass,
+                                                        double w, const uint8_t *__restrict__ fixed,
+                                                        float *__restrict__ pinv)
 {
     const int i = blockIdx.x * BLOCK + threadIdx.x;
     if (i >= n_nodes) return;
     const int b0 = brow_ptr[i], deg = brow_ptr[i + 1] - b0, dp = diag_pos[i];
-    const T *row = vals + 9 * int64_t(b0) + 3 * dp;
+    const T *row = ndiag ? ndiag + 9 * int64_t(i) : vals + 9 * int64_t(b0) + 3 * dp;
+    const int rs = ndiag ? 3 : 3 * deg;
     double B[3][3];
 #pragma unroll
     for (int m = 0; m < 3; ++m)
 #pragma unroll
-        for (int n = 0; n < 3; ++n) B[m][n] = w * double(row[m * 3 * deg + n]) + (mass && m == n ? mass[b0 + dp] : 0.0);
+        for (int n = 0; n < 3; ++n) B[m][n] = w * double(row[m * rs + n]) + (mass && m == n ? mass[b0 + dp] : 0.0);
 #pragma unroll
     for (int d = 0; d < 3; ++d)
         if (fixed && fixed[3 * int64_t(i) + d])
@@ -509,7 +513,16 @@ void prepare_operator(Handle &h, const double *x, const uint8_t *fixed)
 {
     h.op_assembled = assembled_operator(h);
     h.op_precond = h.solver_precond == PFGPU_PRECOND_BLOCK_JACOBI;
-    if (h.op_assembled || h.op_precond) {
+    // a matrix-free operator on the factored path needs only the diagonal
+    // blocks for its preconditioner: those come from the factor records
+    const bool diag_only = !h.op_assembled && h.op_precond && h.path == PFGPU_PATH_FACTORED &&
+                           h.solver_hessian != PFGPU_SOLVER_HESSIAN_ASSEMBLED_PSD &&
+                           (h.material == PFGPU_NEOHOOKEAN || h.material == PFGPU_LINEAR_ELASTICITY);
+    if (diag_only) {
+        if (!h.brow_ptr.get()) h.brow_ptr = upload(h.pat.brow_ptr, h.stream);
+        h.node_diag.reserve(h.value_size() * 9 * size_t(h.n_nodes));
+        factored_node_diagonal(h, x, h.node_diag.get());
+    } else if (h.op_assembled || h.op_precond) {
         if (!h.brow_ptr.get()) h.brow_ptr = upload(h.pat.brow_ptr, h.stream);
         if (!h.pat_bcol.get()) h.pat_bcol = upload(h.pat.bcol, h.stream);
         h.solver_vals.reserve(h.value_size() * size_t(h.pat.nnz));
@@ -548,7 +561,8 @@ void prepare_operator(Handle &h, const double *x, const uint8_t *fixed)
         auto jac = [&]<typename T>() {
             k_block_jacobi<T><<<grid_for(h.n_nodes), BLOCK, 0, h.stream>>>(
                 h.n_nodes, h.brow_ptr.as<int32_t>(), h.diag_pos.as<int32_t>(), h.solver_vals.as<T>(),
-                fold ? nullptr : mass, fold ? 1.0 : h.obj_w, fixed, h.pinv.as<float>());
+                diag_only ? h.node_diag.as<T>() : nullptr, fold ? nullptr : mass, fold ? 1.0 : h.obj_w, fixed,
+                h.pinv.as<float>());
         };
         if (h.precision == PFGPU_MIXED)
             jac.template operator()<float>();
