@@ -72,17 +72,6 @@ template <typename T, int NQ> __device__ __forceinline__ void factors_tail(const
     for (int i = 11 * NQ + 6; i < R; ++i) r[i] = T(0);
 }
 
-template <typename T, int NQ>
-__device__ __forceinline__ void finish_factors(const T *jit, T mu, T w, T (&r)[fac_size(NQ)], T *__restrict__ rec, int e)
-{
-    constexpr int R = fac_size(NQ);
-    factors_tail<T, NQ>(jit, mu, w, r);
-    // records are R values (a multiple of 16 bytes) from a 256-byte aligned base
-    T *out = static_cast<T *>(__builtin_assume_aligned(rec, 16)) + int64_t(e) * R;
-#pragma unroll
-    for (int i = 0; i < R; ++i) out[i] = r[i];
-}
-
 template <typename T, int ORDER, int NQ, int MAT>
 __global__ __launch_bounds__(BLOCK) void k_energy(DevMesh<T> m, const double *__restrict__ x,
                                                   double *__restrict__ per_elem, double *__restrict__ partial)
@@ -814,24 +803,36 @@ __global__ __launch_bounds__(GS) void k_gather_blocks_seg(int n_chunks, const in
 // lines.  (Records holding every node's k vectors instead take less
 // arithmetic in the gather but cost more in their strided writes.)
 template <typename T, int ORDER, int NQ, int MAT>
-__global__ __launch_bounds__(BLOCK) void k_hfactors(DevMesh<T> m, const double *__restrict__ x, T *__restrict__ rec)
+__global__ __launch_bounds__(64) void k_hfactors(DevMesh<T> m, const double *__restrict__ x, T *__restrict__ rec)
 {
+    // one wavefront per workgroup: the 64 records are staged in LDS and
+    // stored as one contiguous run
     constexpr int R = fac_size(NQ);
-    const int e = blockIdx.x * BLOCK + threadIdx.x;
-    if (e >= m.n_elements) return;
-    Elem<T, ORDER, NQ> el;
-    load_geometry(m, e, el);
-    if (MAT != MAT_LE) gather(m, e, x, el.U);
-    const T w = el.det * T(quad_weight(NQ));
-    T r[R];
+    __shared__ T srec[64 * (R + 1)];
+    const int lane = threadIdx.x, e0 = blockIdx.x * 64, e = e0 + lane;
+    if (e < m.n_elements) {
+        Elem<T, ORDER, NQ> el;
+        load_geometry(m, e, el);
+        if (MAT != MAT_LE) gather(m, e, x, el.U);
+        const T w = el.det * T(quad_weight(NQ));
+        T r[R];
 #pragma unroll
-    for (int q = 0; q < NQ; ++q) {
-        Kin<T> k;
-        if (MAT != MAT_LE) field_grad<T, ORDER, NQ>(q, el.U, el.jit, k.H);
-        if (MAT == MAT_NH) kinematics(k);
-        hess_qp<T, MAT>(k, el.jit, el.lam, el.mu, w, r + 9 * q, r[9 * NQ + q], r[10 * NQ + q]);
+        for (int q = 0; q < NQ; ++q) {
+            Kin<T> k;
+            if (MAT != MAT_LE) field_grad<T, ORDER, NQ>(q, el.U, el.jit, k.H);
+            if (MAT == MAT_NH) kinematics(k);
+            hess_qp<T, MAT>(k, el.jit, el.lam, el.mu, w, r + 9 * q, r[9 * NQ + q], r[10 * NQ + q]);
+        }
+        factors_tail<T, NQ>(el.jit, el.mu, w, r);
+#pragma unroll
+        for (int i = 0; i < R; ++i) srec[lane * (R + 1) + i] = r[i];
     }
-    finish_factors<T, NQ>(el.jit, el.mu, w, r, rec, e);
+    __syncthreads();
+    const int ne = min(64, m.n_elements - e0);
+    for (int i = lane; i < ne * R; i += 64) {
+        const int le = i / R;
+        rec[int64_t(e0) * R + i] = srec[le * (R + 1) + (i - le * R)];
+    }
 }
 
 // gather plan as for k_gather_blocks_seg, except that a chunk's slots hold
@@ -1232,7 +1233,7 @@ void factored_hessian(Handle &h, const double *x, void *values)
     dispatch(h, [&]<typename T, int O, int Q, int M>() {
         if constexpr (M == MAT_NH || M == MAT_LE) {
             h.fac_rec.reserve(sizeof(T) * size_t(h.n_elements) * fac_size(Q));
-            k_hfactors<T, O, Q, M><<<grid_for(h.n_elements), BLOCK, 0, h.stream>>>(h.mesh<T>(), x, h.fac_rec.as<T>());
+            k_hfactors<T, O, Q, M><<<grid_for(h.n_elements, 64), 64, 0, h.stream>>>(h.mesh<T>(), x, h.fac_rec.as<T>());
             PF_LAUNCH_CHECK();
             launch_factored_gather<T, O, Q>(h, p, values);
         }
@@ -1270,7 +1271,7 @@ void factored_node_diagonal(Handle &h, const double *x, void *diag)
     dispatch(h, [&]<typename T, int O, int Q, int M>() {
         if constexpr (M == MAT_NH || M == MAT_LE) {
             h.fac_rec.reserve(sizeof(T) * size_t(h.n_elements) * fac_size(Q));
-            k_hfactors<T, O, Q, M><<<grid_for(h.n_elements), BLOCK, 0, h.stream>>>(h.mesh<T>(), x, h.fac_rec.as<T>());
+            k_hfactors<T, O, Q, M><<<grid_for(h.n_elements, 64), 64, 0, h.stream>>>(h.mesh<T>(), x, h.fac_rec.as<T>());
             PF_LAUNCH_CHECK();
             k_node_diag<T, O, Q><<<xcd_grid(h.n_row_chunks), BLOCK, 0, h.stream>>>(
                 h.n_row_chunks, h.row_chunk.as<int32_t>(), h.row_slot.as<int32_t>(), h.row_meta.as<int32_t>(),
