@@ -71,7 +71,7 @@ void sum_doubles(Handle &h, const double *in, int n, double *out)
     // contiguous ranges, then one block over their sums.  The order depends
     // on n only.
     constexpr int SUM_BLOCKS = 128;
-    if (n <= 8 * BLOCK) {
+    if (n <= 32 * BLOCK) {
         k_sum_doubles<<<1, BLOCK, 0, h.stream>>>(in, n, out);
     } else {
         if (h.sum_part.bytes() < sizeof(double) * SUM_BLOCKS) h.sum_part.reserve(sizeof(double) * SUM_BLOCKS);
