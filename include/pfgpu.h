@@ -52,11 +52,17 @@ typedef enum {
 /* FixedCorotational: psi = mu |F - R|^2 + lambda/2 (J - 1)^2 with R the
  * rotation of F (FixedCorotational.cpp:589-706); elements with det F <= 0
  * are reported like inverted NeoHookean elements; the plain path only */
+/* MooneyRivlin: psi = c1 (I1~ - 3) + c2 (I2~ - 3) + k/2 ln^2 J, the invariants
+ * of C~ = F~ F~^T with F~ = J^{-1/3} F (MooneyRivlinElasticity.hpp:25-44, a
+ * GenericElastic material, GenericElastic.hpp:93-348); per-element c1, c2,
+ * k; the plain path only; no material (Lame) derivatives, as in the
+ * reference */
 typedef enum {
     PFGPU_NEOHOOKEAN = 0,
     PFGPU_LINEAR_ELASTICITY = 1,
     PFGPU_SAINT_VENANT = 2,
-    PFGPU_FIXED_COROTATIONAL = 3
+    PFGPU_FIXED_COROTATIONAL = 3,
+    PFGPU_MOONEY_RIVLIN = 4
 } pfgpu_material_kind;
 
 /* FP64: every kernel in double.  MIXED: element kernels in float; the energy
@@ -93,6 +99,10 @@ typedef struct {
     const double *mu;      /* per-element Lame mu (n_elements), or NULL */
     double young;          /* uniform E when lambda/mu are NULL */
     double poisson;        /* uniform nu when lambda/mu are NULL */
+    /* MooneyRivlin only (the Lame fields are then unused): per-element c1,
+     * c2, k (n_elements each, all three or none), else the uniform values */
+    const double *c1, *c2, *k;
+    double c1_uniform, c2_uniform, k_uniform;
 } pfgpu_material_desc;
 
 typedef struct {
@@ -141,6 +151,9 @@ pfgpu_status pfgpu_autotune(pfgpu_handle *h, const double *x, pfgpu_tune_report 
 pfgpu_status pfgpu_set_tuning(pfgpu_handle *h, int32_t path, int32_t gather_threads, int32_t hessian_threads);
 /* replaces ElasticityAssembler::update_lame_params (Assembler.hpp:147-156) */
 pfgpu_status pfgpu_set_lame(pfgpu_handle *h, const double *lambda, const double *mu);
+/* MooneyRivlin parameters per element (n_elements each), as the reference's
+ * c1 / c2 / k material parameters (MooneyRivlinElasticity.cpp:10-15) */
+pfgpu_status pfgpu_set_mooney_rivlin(pfgpu_handle *h, const double *c1, const double *c2, const double *k);
 /* new rest positions of the mesh vertices (n_vertices x 3, same topology):
  * replaces Mesh::set_point over the vertices followed by the rebuild of the
  * assembly-values cache (optimization/var2sims/ShapeVariableToSimulation.cpp:84-95,
@@ -292,12 +305,19 @@ pfgpu_status pfgpu_set_load(pfgpu_handle *h, const double *f);
  * reference's Mass assembler (assembler/Mass.cpp:4-20) integrated exactly;
  * rho: per-element density (n_elements) or NULL for rho_uniform */
 pfgpu_status pfgpu_set_density(pfgpu_handle *h, const double *rho, double rho_uniform);
-/* row-sum lumping of the mass matrix (lumped = 1) or the consistent one
- * (0, the default): the reference's solver.advanced.lump_mass_matrix
- * (State.cpp:1542-1545, utils::lump_matrix, MatrixUtils.cpp:84-101), every
- * row's sum on its diagonal block.  Applies to the current and later
- * pfgpu_set_density calls.  (At P2 the vertex rows sum to negative masses,
- * as in the reference.) */
+/* mass matrix kind:
+ *   PFGPU_MASS_CONSISTENT (0, the default) the consistent matrix;
+ *   PFGPU_MASS_ROW_SUM (1) row-sum lumping, the reference's
+ *     solver.advanced.lump_mass_matrix (State.cpp:1542-1545,
+ *     utils::lump_matrix, MatrixUtils.cpp:84-101): every row's sum on its
+ *     diagonal block (at P2 the vertex rows sum to negative masses, as in the
+ *     reference);
+ *   PFGPU_MASS_HRZ (2) the Hinton-Rock-Zienkiewicz diagonal of HRZMass
+ *     (assembler/Mass.cpp:48-86): per element, diagonal entry a is
+ *     M_e(a,a) sum_bc M_e(b,c) / sum_b M_e(b,b), nothing off the diagonal
+ *     (positive at P2; with density 1 it is the reference's pure mass).
+ * Applies to the current and later pfgpu_set_density calls. */
+enum { PFGPU_MASS_CONSISTENT = 0, PFGPU_MASS_ROW_SUM = 1, PFGPU_MASS_HRZ = 2 };
 pfgpu_status pfgpu_set_mass_lumping(pfgpu_handle *h, int32_t lumped);
 /* y = M v */
 pfgpu_status pfgpu_mass_matvec(pfgpu_handle *h, const double *v, double *y);

@@ -84,6 +84,16 @@ def stiffness(F, lam, mu):
 class FcrOracle:
     """Element loops of the FixedCorotational form on one mesh (FP64)."""
 
+    # the material's energy density, stress and 9x9 stiffness of (F, e)
+    def _psi(self, F, e):
+        return energy_density(F, self.lam[e], self.mu[e])
+
+    def _P(self, F, e):
+        return stress(F, self.lam[e], self.mu[e])
+
+    def _A(self, F, e):
+        return stiffness(F, self.lam[e], self.mu[e])
+
     def __init__(self, mesh, lam, mu, quad_order=0):
         self.mesh = mesh
         self.lam = np.broadcast_to(np.asarray(lam, dtype=np.float64), (mesh.n_elements,))
@@ -109,7 +119,7 @@ class FcrOracle:
             yield G, np.eye(3) + U.T @ G, det * self.w[q]
 
     def element_energy(self, e, x):
-        return sum(energy_density(F, self.lam[e], self.mu[e]) * da for _, F, da in self._qp(e, x))
+        return sum(self._psi(F, e) * da for _, F, da in self._qp(e, x))
 
     def energy_per_element(self, x):
         return np.array([self.element_energy(e, x) for e in range(self.n_elements)])
@@ -120,7 +130,7 @@ class FcrOracle:
     def element_gradient(self, e, x):
         g = np.zeros((self.n_local, 3))
         for G, F, da in self._qp(e, x):
-            g += G @ stress(F, self.lam[e], self.mu[e]).T * da
+            g += G @ self._P(F, e).T * da
         return g.reshape(-1)
 
     def gradient(self, x):
@@ -135,7 +145,7 @@ class FcrOracle:
         n = self.n_local
         Hl = np.zeros((3 * n, 3 * n))
         for G, F, da in self._qp(e, x):
-            A = stiffness(F, self.lam[e], self.mu[e])
+            A = self._A(F, e)
             D = np.zeros((9, 3 * n))
             for a in range(n):
                 for j in range(3):
@@ -192,8 +202,8 @@ class FcrOracle:
             for q, (G, F, da) in enumerate(self._qp(e, x)):
                 gu = F - np.eye(3)
                 gp = self._grad_field(e, G, p)
-                A = stiffness(F, self.lam[e], self.mu[e])
-                P = stress(F, self.lam[e], self.mu[e])
+                A = self._A(F, e)
+                P = self._P(F, e)
                 gv_all = po.basis_grad(1, self.pts[q]) @ jit
                 for v in range(4):
                     gv = gv_all[v]

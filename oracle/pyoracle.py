@@ -128,9 +128,14 @@ class Oracle:
         self.n_quad = lib().orc_n_quad(self.p)
 
     def __del__(self):
-        if getattr(self, "p", None):
-            lib().orc_destroy(self.p)
-            self.p = None
+        # at interpreter teardown the module globals may already be gone
+        p = getattr(self, "p", None)
+        self.p = None
+        if p:
+            try:
+                lib().orc_destroy(p)
+            except Exception:
+                pass
 
     @staticmethod
     def _x(x):
@@ -222,9 +227,13 @@ class Baseline:
         self.threads = lib().orc_baseline_threads(self.b)
 
     def __del__(self):
-        if getattr(self, "b", None):
-            lib().orc_baseline_destroy(self.b)
-            self.b = None
+        b = getattr(self, "b", None)
+        self.b = None
+        if b:
+            try:
+                lib().orc_baseline_destroy(b)
+            except Exception:
+                pass
 
     def energy(self, x):
         return lib().orc_baseline_energy(self.b, Oracle._x(x))

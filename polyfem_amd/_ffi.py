@@ -14,7 +14,7 @@ LIB_PATH = os.environ.get("PFGPU_LIB", os.path.join(HERE, "libpfgpu.so"))
 PFGPU_OK = 0
 STATUS_NAMES = {0: "OK", 1: "INVALID_ARGUMENT", 2: "HIP", 3: "INVALID_STATE", 4: "UNSUPPORTED", 5: "OUT_OF_MEMORY"}
 OK, ERR_INVALID_ARGUMENT, ERR_HIP, ERR_INVALID_STATE, ERR_UNSUPPORTED, ERR_OUT_OF_MEMORY = range(6)
-NEOHOOKEAN, LINEAR_ELASTICITY, SAINT_VENANT, FIXED_COROTATIONAL = 0, 1, 2, 3
+NEOHOOKEAN, LINEAR_ELASTICITY, SAINT_VENANT, FIXED_COROTATIONAL, MOONEY_RIVLIN = 0, 1, 2, 3, 4
 FP64, MIXED = 0, 1
 PATH_FUSED, PATH_PLAIN, PATH_FACTORED = 0, 1, 2
 
@@ -27,7 +27,9 @@ class MeshDesc(C.Structure):
 
 class MaterialDesc(C.Structure):
     _fields_ = [("kind", C.c_int32), ("lam", C.c_void_p), ("mu", C.c_void_p),
-                ("young", C.c_double), ("poisson", C.c_double)]
+                ("young", C.c_double), ("poisson", C.c_double),
+                ("c1", C.c_void_p), ("c2", C.c_void_p), ("k", C.c_void_p),
+                ("c1_uniform", C.c_double), ("c2_uniform", C.c_double), ("k_uniform", C.c_double)]
 
 
 class Info(C.Structure):
@@ -81,6 +83,7 @@ _SIGNATURES = {
     "pfgpu_set_stream": [vp, vp],
     "pfgpu_set_path": [vp, C.c_int32],
     "pfgpu_set_lame": [vp, vp, vp],
+    "pfgpu_set_mooney_rivlin": [vp, vp, vp, vp],
     "pfgpu_value": [vp, vp, vp],
     "pfgpu_value_per_element": [vp, vp, vp],
     "pfgpu_gradient": [vp, vp, vp],

@@ -23,7 +23,7 @@ from . import _ffi
 from ._ffi import check, lib
 
 _NAMES = {"NeoHookean": _ffi.NEOHOOKEAN, "LinearElasticity": _ffi.LINEAR_ELASTICITY, "SaintVenant": _ffi.SAINT_VENANT,
-          "FixedCorotational": _ffi.FIXED_COROTATIONAL}
+          "FixedCorotational": _ffi.FIXED_COROTATIONAL, "MooneyRivlin": _ffi.MOONEY_RIVLIN}
 _PRECISIONS = {"fp64": _ffi.FP64, "mixed": _ffi.MIXED}
 _PATHS = {"fused": _ffi.PATH_FUSED, "plain": _ffi.PATH_PLAIN, "factored": _ffi.PATH_FACTORED}
 
@@ -43,7 +43,9 @@ class NLAssembler:
     """Elasticity assembler on one tet mesh (P1 or P2, P1 geometry)."""
 
     def __init__(self, name: str, mesh, lame=None, young: float = None, poisson: float = None,
-                 precision: str = "fp64", path: str = None, quadrature_order: int = 0):
+                 precision: str = "fp64", path: str = None, quadrature_order: int = 0, params=None):
+        """``lame=(lambda, mu)`` (scalars or per element) or ``young`` and
+        ``poisson``; MooneyRivlin takes ``params=(c1, c2, k)`` instead."""
         if name not in _NAMES:
             raise ValueError(f"unknown material {name!r}; expected one of {sorted(_NAMES)}")
         self._name = name
@@ -54,7 +56,13 @@ class NLAssembler:
         self._ev = np.ascontiguousarray(mesh.elem_vertices, dtype=np.int32)
         md = _ffi.MeshDesc(mesh.order, quadrature_order, E, mesh.n_nodes, _ptr(self._en), self._vx.shape[0],
                            _ptr(self._vx), _ptr(self._ev))
-        if lame is not None:
+        if name == "MooneyRivlin":
+            if params is None:
+                raise ValueError("MooneyRivlin takes params=(c1, c2, k)")
+            self._mr = [_f64(np.broadcast_to(np.asarray(v, dtype=np.float64), (E,))) for v in params]
+            mat = _ffi.MaterialDesc(_NAMES[name], None, None, 0.0, 0.0, _ptr(self._mr[0]), _ptr(self._mr[1]),
+                                    _ptr(self._mr[2]), 0.0, 0.0, 0.0)
+        elif lame is not None:
             lam = _f64(np.broadcast_to(np.asarray(lame[0], dtype=np.float64), (E,)))
             mu = _f64(np.broadcast_to(np.asarray(lame[1], dtype=np.float64), (E,)))
             mat = _ffi.MaterialDesc(_NAMES[name], _ptr(lam), _ptr(mu), 0.0, 0.0)
@@ -138,6 +146,11 @@ class NLAssembler:
 
     def set_stream(self, stream_handle: int):
         check(lib().pfgpu_set_stream(self._h, C.c_void_p(stream_handle)))
+
+    def update_mooney_rivlin_params(self, c1, c2, k):
+        """Per-element MooneyRivlin c1, c2, k (scalars broadcast)."""
+        v = [_f64(np.broadcast_to(np.asarray(a, dtype=np.float64), (self.n_elements,))) for a in (c1, c2, k)]
+        check(lib().pfgpu_set_mooney_rivlin(self._h, _ptr(v[0]), _ptr(v[1]), _ptr(v[2])))
 
     def update_lame_params(self, lambdas, mus):
         lam = _f64(lambdas, self.n_elements)

@@ -5,6 +5,7 @@
 #include <memory>
 #include <new>
 
+#include "dispatch.hpp"
 #include "handle.hpp"
 
 using namespace pfgpu;
@@ -82,19 +83,26 @@ void geometry_host(const pfgpu_mesh_desc &m, std::vector<double> &g /* E x 10 */
     }
 }
 
-void upload_lame(pfgpu_handle &h, const double *lambda, const double *mu)
+// per-element material parameters, k rows (lambda, mu; MooneyRivlin c1, c2, k)
+void upload_params(pfgpu_handle &h, const double *const *rows, int k)
 {
-    std::vector<double> lm(size_t(h.n_elements) * 2);
-    for (int e = 0; e < h.n_elements; ++e) {
-        require(std::isfinite(lambda[e]) && std::isfinite(mu[e]), "non-finite Lame parameter");
-        lm[size_t(e) * 2] = lambda[e];
-        lm[size_t(e) * 2 + 1] = mu[e];
-    }
+    std::vector<double> lm(size_t(h.n_elements) * k);
+    for (int e = 0; e < h.n_elements; ++e)
+        for (int r = 0; r < k; ++r) {
+            require(std::isfinite(rows[r][e]), "non-finite material parameter");
+            lm[size_t(e) * k + r] = rows[r][e];
+        }
     // in place: graphs captured over the handle's kernels keep reading the current values
     if (h.precision == PFGPU_MIXED)
-        upload_into(h.lame, soa<double, float>(lm.data(), h.n_elements, 2), h.stream);
+        upload_into(h.lame, soa<double, float>(lm.data(), h.n_elements, k), h.stream);
     else
-        upload_into(h.lame, soa<double, double>(lm.data(), h.n_elements, 2), h.stream);
+        upload_into(h.lame, soa<double, double>(lm.data(), h.n_elements, k), h.stream);
+}
+
+void upload_lame(pfgpu_handle &h, const double *lambda, const double *mu)
+{
+    const double *rows[2] = {lambda, mu};
+    upload_params(h, rows, 2);
 }
 
 void upload_geometry(pfgpu_handle &h, const pfgpu_mesh_desc &m)
@@ -177,7 +185,7 @@ int64_t pfgpu_handle::device_bytes() const
 {
     const DevBuf *bufs[] = {&conn, &vconn, &geom, &lame, &node_ptr, &node_inc, &vert_ptr, &vert_inc,
                             &brow_ptr, &blk_out, &blk_plan.chunk, &blk_plan.slot, &blk_plan.meta, &fac_plan.chunk,
-                            &fac_plan.slot, &fac_plan.meta, &fac_plan.dst, &row_chunk, &row_slot, &row_meta, &node_diag, &patch_desc, &patch_rows, &fac_rec,
+                            &fac_plan.slot, &fac_plan.meta, &fac_plan.dst, &unit_plan.chunk_row, &unit_plan.udst, &unit_plan.bq0, &row_chunk, &row_slot, &row_meta, &node_diag, &patch_desc, &patch_rows, &fac_rec,
                             &patch_row_acc, &patch_elems, &patch_cs, &patch_cg, &patch_hdr, &patch_pos, &patch_vnodes,
                             &d_outer, &d_inner, &loc, &hloc, &partial, &red_out, &sum_part,
                             &h_x, &h_v, &h_out, &h_out2, &h_vals, &sv, &h_mask, &mass, &pat_bcol, &tr_fixed, &tr_f, &tr_seg, &load};
@@ -204,7 +212,8 @@ pfgpu_status pfgpu_create(const pfgpu_mesh_desc *mesh, const pfgpu_material_desc
         require(m.n_elements > 0 && m.n_nodes > 0 && m.n_vertices > 0, "empty mesh");
         require(m.elem_nodes && m.vertices && m.elem_vertices, "null mesh array");
         require(material->kind == PFGPU_NEOHOOKEAN || material->kind == PFGPU_LINEAR_ELASTICITY ||
-                    material->kind == PFGPU_SAINT_VENANT || material->kind == PFGPU_FIXED_COROTATIONAL,
+                    material->kind == PFGPU_SAINT_VENANT || material->kind == PFGPU_FIXED_COROTATIONAL ||
+                    material->kind == PFGPU_MOONEY_RIVLIN,
                 "unknown material");
         require(precision == PFGPU_FP64 || precision == PFGPU_MIXED, "unknown precision");
         require((material->lambda == nullptr) == (material->mu == nullptr), "lambda and mu must both be given or both be NULL");
@@ -232,7 +241,20 @@ pfgpu_status pfgpu_create(const pfgpu_mesh_desc *mesh, const pfgpu_material_desc
         h->vconn = upload_soa<int32_t, int32_t>(m.elem_vertices, m.n_elements, 4, h->stream);
         upload_geometry(*h, m);
 
-        if (material->lambda) {
+        if (material->kind == PFGPU_MOONEY_RIVLIN) {
+            const bool given = material->c1 != nullptr;
+            require(given == (material->c2 != nullptr) && given == (material->k != nullptr),
+                    "c1, c2 and k must all be given or all be NULL");
+            std::vector<double> u1, u2, u3;
+            const double *rows[3] = {material->c1, material->c2, material->k};
+            if (!given) {
+                u1.assign(size_t(m.n_elements), material->c1_uniform);
+                u2.assign(size_t(m.n_elements), material->c2_uniform);
+                u3.assign(size_t(m.n_elements), material->k_uniform);
+                rows[0] = u1.data(), rows[1] = u2.data(), rows[2] = u3.data();
+            }
+            upload_params(*h, rows, 3);
+        } else if (material->lambda) {
             upload_lame(*h, material->lambda, material->mu);
         } else {
             const double E = material->young, nu = material->poisson;
@@ -301,7 +323,8 @@ pfgpu_status pfgpu_set_tuning(pfgpu_handle *h, int32_t path, int32_t gather_thre
                 "gather_threads must be 64, 128, 256 or 512");
         require(hessian_threads == 64 || hessian_threads == 128 || hessian_threads == 256,
                 "hessian_threads must be 64, 128 or 256");
-        if (path != PFGPU_PATH_PLAIN && (h->material == PFGPU_SAINT_VENANT || h->material == PFGPU_FIXED_COROTATIONAL))
+        if (path != PFGPU_PATH_PLAIN && (h->material == PFGPU_SAINT_VENANT || h->material == PFGPU_FIXED_COROTATIONAL ||
+                                         h->material == PFGPU_MOONEY_RIVLIN))
             fail(PFGPU_ERR_UNSUPPORTED, "the fused and factored paths cover NeoHookean and linear elasticity");
         h->path = path;
         h->tune_gather_threads = gather_threads;
@@ -412,9 +435,23 @@ pfgpu_status pfgpu_set_path(pfgpu_handle *h, int32_t path)
     return guard([&] {
         check_handle(h);
         require(path == PFGPU_PATH_FUSED || path == PFGPU_PATH_PLAIN || path == PFGPU_PATH_FACTORED, "unknown path");
-        if (path != PFGPU_PATH_PLAIN && (h->material == PFGPU_SAINT_VENANT || h->material == PFGPU_FIXED_COROTATIONAL))
+        if (path != PFGPU_PATH_PLAIN && (h->material == PFGPU_SAINT_VENANT || h->material == PFGPU_FIXED_COROTATIONAL ||
+                                         h->material == PFGPU_MOONEY_RIVLIN))
             fail(PFGPU_ERR_UNSUPPORTED, "the fused and factored paths cover NeoHookean and linear elasticity");
         h->path = path;
+    });
+}
+
+pfgpu_status pfgpu_set_mooney_rivlin(pfgpu_handle *h, const double *c1, const double *c2, const double *k)
+{
+    return guard([&] {
+        check_handle(h);
+        require(c1 && c2 && k, "null c1/c2/k");
+        require(h->material == PFGPU_MOONEY_RIVLIN, "the handle's material is not MooneyRivlin");
+        PF_HIP(hipStreamSynchronize(h->stream));
+        const double *rows[3] = {c1, c2, k};
+        upload_params(*h, rows, 3);
+        solver_release(*h);
     });
 }
 
@@ -423,6 +460,7 @@ pfgpu_status pfgpu_set_lame(pfgpu_handle *h, const double *lambda, const double 
     return guard([&] {
         check_handle(h);
         require(lambda && mu, "null lambda/mu");
+        require(h->material != PFGPU_MOONEY_RIVLIN, "MooneyRivlin takes c1, c2, k (pfgpu_set_mooney_rivlin)");
         PF_HIP(hipStreamSynchronize(h->stream));
         upload_lame(*h, lambda, mu);
         solver_release(*h);
@@ -892,9 +930,10 @@ pfgpu_status pfgpu_set_mass_lumping(pfgpu_handle *h, int32_t lumped)
 {
     return guard([&] {
         check_handle(h);
-        require(lumped == 0 || lumped == 1, "lumped must be 0 or 1");
-        if (h->lump_mass == (lumped == 1)) return;
-        h->lump_mass = lumped == 1;
+        require(lumped == PFGPU_MASS_CONSISTENT || lumped == PFGPU_MASS_ROW_SUM || lumped == PFGPU_MASS_HRZ,
+                "lumped must be 0 (consistent), 1 (row sums) or 2 (HRZ)");
+        if (h->lump_mass == lumped) return;
+        h->lump_mass = lumped;
         if (h->has_mass) {
             build_mass(*h, h->h_rho.empty() ? nullptr : h->h_rho.data(), h->rho_uniform);
             solver_release(*h);

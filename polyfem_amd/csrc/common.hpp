@@ -84,6 +84,17 @@ struct GatherPlan {
     unsigned grid = 0; // launch grid of a persistent gather over this plan (0: not yet known)
 };
 
+// Row-unit plan of the factored gather for workgroups of `threads` threads:
+// chunks of whole node rows (first row of each chunk, n_chunks + 1 entries),
+// per unit (node incidence, in node_inc order) the stage positions of its
+// n_local blocks, per pattern block the first stage position of its
+// contributions within its chunk.
+struct UnitPlan {
+    DevBuf chunk_row, udst, bq0;
+    int n_chunks = 0, threads = 0;
+    unsigned grid = 0;
+};
+
 // v into dst: in place when the size matches (device pointers held elsewhere,
 // e.g. by captured graphs, stay valid), else into a new allocation
 template <typename T> void upload_into(DevBuf &dst, const std::vector<T> &v, hipStream_t s);
@@ -129,7 +140,8 @@ template <typename T> struct DevMesh {
     const int32_t *conn;  // [n_local][E]   dof-node ids
     const int32_t *vconn; // [4][E]         geometric vertex ids
     const T *geom;        // [10][E]        J^{-T} row-major (0..8), det J (9)
-    const T *lame;        // [2][E]         lambda, mu
+    const T *lame;        // [n_params][E]  lambda, mu (MooneyRivlin: c1, c2, k)
+    int n_params;         // 2, or 3 for MooneyRivlin
 };
 
 // Sparsity pattern and the maps that scatter element-local results into it.

@@ -20,6 +20,8 @@ template <typename F> void dispatch(const Handle &h, F &&f)
             f.template operator()<T, O, Q, MAT_SV>();
         else if (h.material == MAT_FC)
             f.template operator()<T, O, Q, MAT_FC>();
+        else if (h.material == MAT_MR)
+            f.template operator()<T, O, Q, MAT_MR>();
         else
             f.template operator()<T, O, Q, MAT_LE>();
     };

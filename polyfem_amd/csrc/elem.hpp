@@ -23,7 +23,8 @@ enum {
     MAT_NH = PFGPU_NEOHOOKEAN,
     MAT_LE = PFGPU_LINEAR_ELASTICITY,
     MAT_SV = PFGPU_SAINT_VENANT,
-    MAT_FC = PFGPU_FIXED_COROTATIONAL
+    MAT_FC = PFGPU_FIXED_COROTATIONAL,
+    MAT_MR = PFGPU_MOONEY_RIVLIN
 };
 
 template <int ORDER> struct NLocal { static constexpr int value = ORDER == 1 ? 4 : 10; };
@@ -107,6 +108,8 @@ template <typename T> __device__ __forceinline__ T det3(const T *m)
 
 __device__ __forceinline__ float pf_log1p(float v) { return log1pf(v); }
 __device__ __forceinline__ double pf_log1p(double v) { return log1p(v); }
+__device__ __forceinline__ float pf_expm1(float v) { return expm1f(v); }
+__device__ __forceinline__ double pf_expm1(double v) { return expm1(v); }
 
 // delta - log1p(delta) without cancellation: series below a threshold
 __device__ __forceinline__ float xmlog1p(float d)
@@ -179,7 +182,8 @@ template <typename T, int ORDER, int NQ> struct Elem {
     static constexpr int NB = NLocal<ORDER>::value;
     T U[NB][3];
     T jit[9];
-    T det, lam, mu;
+    // material parameters: (lambda, mu), or (c1, c2, k) for MooneyRivlin
+    T det, lam, mu, p3;
 };
 
 template <typename T, int ORDER, int NQ>
@@ -191,18 +195,29 @@ __device__ __forceinline__ void load_geometry(const DevMesh<T> &m, int e, Elem<T
     el.det = m.geom[9 * E + e];
     el.lam = m.lame[e];
     el.mu = m.lame[E + e];
+    el.p3 = m.n_params > 2 ? m.lame[2 * int64_t(E) + e] : T(0);
 }
 
-// gather of the element's nodal values from an interleaved dof vector
+// gather of the element's nodal values from an interleaved dof vector,
+// relative to local node 0: U[a] = x_a - x_0, formed in FP64 before the
+// conversion to T.  The field gradient is unchanged (the reference
+// gradients of the basis sum to zero), but in FP32 the rounding of the
+// stored values is relative to the small differences across the element
+// instead of to the displacements themselves, which the gradient sum would
+// otherwise amplify by |u| / (h |grad u|).
 template <typename T, int NB>
 __device__ __forceinline__ void gather(const DevMesh<T> &m, int e, const double *__restrict__ x, T (&U)[NB][3])
 {
     const int E = m.n_elements;
+    const int64_t n0 = 3 * int64_t(m.conn[e]);
+    const double x0 = x[n0], x1 = x[n0 + 1], x2 = x[n0 + 2];
+    U[0][0] = U[0][1] = U[0][2] = T(0);
 #pragma unroll
-    for (int a = 0; a < NB; ++a) {
+    for (int a = 1; a < NB; ++a) {
         const int64_t n3 = 3 * int64_t(m.conn[a * E + e]);
-#pragma unroll
-        for (int d = 0; d < 3; ++d) U[a][d] = T(x[n3 + d]);
+        U[a][0] = T(x[n3] - x0);
+        U[a][1] = T(x[n3 + 1] - x1);
+        U[a][2] = T(x[n3 + 2] - x2);
     }
 }
 
@@ -344,11 +359,115 @@ template <typename T> __device__ __forceinline__ void mul3(const T *A, const T *
         for (int c = 0; c < 3; ++c) out[r * 3 + c] = A[r * 3 + 0] * B[0 * 3 + c] + A[r * 3 + 1] * B[1 * 3 + c] + A[r * 3 + 2] * B[2 * 3 + c];
 }
 
-// energy density (NeoHookeanElasticity.cpp:384; LinearElasticity.cpp:129)
-template <typename T, int MAT> __device__ __forceinline__ T energy_density(const Kin<T> &k, T lam, T mu)
+// MooneyRivlin (MooneyRivlinElasticity.hpp:25-44, the GenericElastic energy
+// c1 (I1~ - 3) + c2 (I2~ - 3) + k/2 ln^2 J of F~ = J^{-1/3} F) in forms that
+// do not cancel for small H (kinematics() first): with B = F F^T = I + S,
+// S = H + H^T + H H^T,
+//   I1 - 3 = tr S,  I2 - 3 = 2 tr S + (tr^2 S - tr S^2) / 2,
+//   J^{-2/3} - 1 = expm1(-2/3 ln J),  J^{-4/3} - 1 = expm1(-4/3 ln J),
+// and the stress P = c1 J^{-2/3} R1 + c2 J^{-4/3} Q2 + k ln J F^{-T} with
+//   R1 = 2 F - 2/3 I1 F^{-T}           = 2 (H + D) - 2/3 (I1 - 3) F^{-T},
+//   Q2 = 2 I1 F - 2 B F - 4/3 I2 F^{-T} = 4 (H + D) + 2 ((I1 - 3) I - S) F - 4/3 (I2 - 3) F^{-T},
+// D = I - F^{-T}.
+template <typename T> struct MrKin {
+    T S[9];
+    T i1, i2, ea, eb;
+};
+template <typename T> __device__ __forceinline__ void mr_kinematics(const Kin<T> &k, MrKin<T> &r)
 {
     const T *H = k.H;
-    if (MAT == MAT_NH) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            r.S[a * 3 + c] = H[a * 3 + c] + H[c * 3 + a] + H[a * 3 + 0] * H[c * 3 + 0] + H[a * 3 + 1] * H[c * 3 + 1] +
+                             H[a * 3 + 2] * H[c * 3 + 2];
+    r.i1 = r.S[0] + r.S[4] + r.S[8];
+    T ss = T(0);
+#pragma unroll
+    for (int i = 0; i < 9; ++i) ss += r.S[i] * r.S[i];
+    r.i2 = T(2) * r.i1 + T(0.5) * (r.i1 * r.i1 - ss);
+    r.ea = pf_expm1(T(-2.0 / 3.0) * k.lnJ);
+    r.eb = pf_expm1(T(-4.0 / 3.0) * k.lnJ);
+}
+template <typename T> __device__ __forceinline__ void mr_brackets(const Kin<T> &k, const MrKin<T> &r, T *R1, T *Q2)
+{
+    const T *H = k.H, *A = k.FmT;
+    // ((I1 - 3) I - S) F
+    T M[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) M[i] = (i % 4 == 0 ? r.i1 : T(0)) - r.S[i];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const T mf = M[a * 3 + c] + M[a * 3 + 0] * H[0 * 3 + c] + M[a * 3 + 1] * H[1 * 3 + c] + M[a * 3 + 2] * H[2 * 3 + c];
+            const T hd = H[a * 3 + c] + k.D[a * 3 + c];
+            R1[a * 3 + c] = T(2) * hd - T(2.0 / 3.0) * r.i1 * A[a * 3 + c];
+            Q2[a * 3 + c] = T(4) * hd + T(2) * mf - T(4.0 / 3.0) * r.i2 * A[a * 3 + c];
+        }
+}
+// dP[W] = c1 J^{-2/3} (dR1 - 2/3 (A:W) R1) + c2 J^{-4/3} (dQ2 - 4/3 (A:W) Q2)
+//         + k ((A:W) A - ln J A W^T A),  A = F^{-T}, with
+//   dR1 = 2 W - 4/3 (F:W) A + 2/3 I1 A W^T A,
+//   dQ2 = 4 (F:W) F + 2 I1 W - 2 W F^T F - 2 F W^T F - 2 B W
+//         - 4/3 (2 I1 (F:W) - 2 (B F):W) A + 4/3 I2 A W^T A
+template <typename T>
+__device__ __forceinline__ void mr_dstress(const Kin<T> &k, T c1, T c2, T kk, const T *W, T *out)
+{
+    MrKin<T> r;
+    mr_kinematics(k, r);
+    T R1[9], Q2[9];
+    mr_brackets(k, r, R1, Q2);
+    const T *H = k.H, *A = k.FmT;
+    const T F[9] = {T(1) + H[0], H[1], H[2], H[3], T(1) + H[4], H[5], H[6], H[7], T(1) + H[8]};
+    const T Bm[9] = {T(1) + r.S[0], r.S[1], r.S[2], r.S[3], T(1) + r.S[4], r.S[5], r.S[6], r.S[7], T(1) + r.S[8]};
+    const T I1 = T(3) + r.i1, I2 = T(3) + r.i2;
+    T BF[9], C[9], AWt[9], AWA[9], WC[9], FWt[9], FWF[9], BW[9];
+    mul3(Bm, F, BF);
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            C[a * 3 + c] = F[0 * 3 + a] * F[0 * 3 + c] + F[1 * 3 + a] * F[1 * 3 + c] + F[2 * 3 + a] * F[2 * 3 + c];
+            AWt[a * 3 + c] = A[a * 3 + 0] * W[c * 3 + 0] + A[a * 3 + 1] * W[c * 3 + 1] + A[a * 3 + 2] * W[c * 3 + 2];
+            FWt[a * 3 + c] = F[a * 3 + 0] * W[c * 3 + 0] + F[a * 3 + 1] * W[c * 3 + 1] + F[a * 3 + 2] * W[c * 3 + 2];
+        }
+    mul3(AWt, A, AWA);
+    mul3(W, C, WC);
+    mul3(FWt, F, FWF);
+    mul3(Bm, W, BW);
+    T aw = T(0), fw = T(0), gw = T(0);
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        aw += A[i] * W[i];
+        fw += F[i] * W[i];
+        gw += BF[i] * W[i];
+    }
+    const T a = c1 * (T(1) + r.ea), b = c2 * (T(1) + r.eb);
+    const T g = T(4.0 / 3.0) * (T(2) * I1 * fw - T(2) * gw);
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        const T dR1 = T(2) * W[i] - T(4.0 / 3.0) * fw * A[i] + T(2.0 / 3.0) * I1 * AWA[i];
+        const T dQ2 = T(4) * fw * F[i] + T(2) * I1 * W[i] - T(2) * WC[i] - T(2) * FWF[i] - T(2) * BW[i] - g * A[i] +
+                      T(4.0 / 3.0) * I2 * AWA[i];
+        out[i] = a * (dR1 - T(2.0 / 3.0) * aw * R1[i]) + b * (dQ2 - T(4.0 / 3.0) * aw * Q2[i]) +
+                 kk * (aw * A[i] - k.lnJ * AWA[i]);
+    }
+}
+
+
+// energy density (NeoHookeanElasticity.cpp:384; LinearElasticity.cpp:129)
+template <typename T, int MAT> __device__ __forceinline__ T energy_density(const Kin<T> &k, T lam, T mu, T p3)
+{
+    const T *H = k.H;
+    if (MAT == MAT_MR) {
+        // c1 (J^{-2/3} I1 - 3) + c2 (J^{-4/3} I2 - 3) + k/2 ln^2 J with (c1, c2, k) = (lam, mu, p3)
+        MrKin<T> r;
+        mr_kinematics(k, r);
+        return lam * (T(3) * r.ea + r.i1 + r.ea * r.i1) + mu * (T(3) * r.eb + r.i2 + r.eb * r.i2) +
+               T(0.5) * p3 * k.lnJ * k.lnJ;
+    } else if (MAT == MAT_NH) {
         // mu/2 (|F|^2 - 3 - 2 ln J) + lambda/2 ln^2 J with |F|^2 - 3 = 2 tr H + |H|^2
         // and tr H - ln J = -(s2 + det H) + (delta - ln(1 + delta))
         T hh = T(0);
@@ -388,10 +507,18 @@ template <typename T, int MAT> __device__ __forceinline__ T energy_density(const
 }
 
 // first Piola-Kirchhoff stress (NeoHookeanElasticity.cpp:532: mu F - mu/J cof F + lambda ln J / J cof F)
-template <typename T, int MAT> __device__ __forceinline__ void pk1(const Kin<T> &k, T lam, T mu, T *P)
+template <typename T, int MAT> __device__ __forceinline__ void pk1(const Kin<T> &k, T lam, T mu, T p3, T *P)
 {
     const T *H = k.H;
-    if (MAT == MAT_NH) {
+    if (MAT == MAT_MR) {
+        MrKin<T> r;
+        mr_kinematics(k, r);
+        T R1[9], Q2[9];
+        mr_brackets(k, r, R1, Q2);
+        const T a = lam * (T(1) + r.ea), b = mu * (T(1) + r.eb), l = p3 * k.lnJ;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) P[i] = a * R1[i] + b * Q2[i] + l * k.FmT[i];
+    } else if (MAT == MAT_NH) {
         const T ll = lam * k.lnJ;
 #pragma unroll
         for (int i = 0; i < 9; ++i) P[i] = mu * (H[i] + k.D[i]) + ll * k.FmT[i];
@@ -421,9 +548,11 @@ template <typename T, int MAT> __device__ __forceinline__ void pk1(const Kin<T> 
 // linearised stress C : W (NeoHookeanElasticity.cpp:660-680 'result';
 // LinearElasticity.cpp:253-270)
 template <typename T, int MAT>
-__device__ __forceinline__ void dstress(const Kin<T> &k, T lam, T mu, const T *W, T *out)
+__device__ __forceinline__ void dstress(const Kin<T> &k, T lam, T mu, T p3, const T *W, T *out)
 {
-    if (MAT == MAT_NH) {
+    if (MAT == MAT_MR) {
+        mr_dstress(k, lam, mu, p3, W, out);
+    } else if (MAT == MAT_NH) {
         const T *A = k.FmT;
         T fw = T(0);
 #pragma unroll

@@ -108,17 +108,18 @@ __device__ __forceinline__ void setup_point(const DevMesh<T> &m, int e, const in
     constexpr int NB = NLocal<ORDER>::value;
     Elem<T, ORDER, NQ> el;
     load_geometry(m, e, el);
+    // relative to local node 0, in FP64 (as gather() in elem.hpp)
 #pragma unroll
     for (int a = 0; a < NB; ++a)
 #pragma unroll
-        for (int d = 0; d < 3; ++d) el.U[a][d] = T(x[3 * int64_t(nodes[a]) + d]);
+        for (int d = 0; d < 3; ++d) el.U[a][d] = T(x[3 * int64_t(nodes[a]) + d] - x[3 * int64_t(nodes[0]) + d]);
     const T w = el.det * T(quad_weight(NQ));
     Kin<T> k;
     field_grad<T, ORDER, NQ>(Q, el.U, el.jit, k.H);
     if (MAT == MAT_NH) kinematics(k);
-    if (own) eacc += double(energy_density<T, MAT>(k, el.lam, el.mu) * w);
+    if (own) eacc += double(energy_density<T, MAT>(k, el.lam, el.mu, el.p3) * w);
     T P[9];
-    pk1<T, MAT>(k, el.lam, el.mu, P);
+    pk1<T, MAT>(k, el.lam, el.mu, el.p3, P);
     // gradient row of node a at this point: sum_k RG(Q,a,k) Qm[d][k], Qm = w P jit^T
     {
         T Qm[9];
@@ -527,7 +528,7 @@ void fused_energy_gradient_hessian(Handle &h, const double *x, double *energy, d
             plain_energy(h, x, energy, nullptr);
         return;
     }
-    if (h.material == MAT_SV || h.material == MAT_FC)
+    if (h.material != MAT_NH && h.material != MAT_LE)
         fail(PFGPU_ERR_UNSUPPORTED, "the fused path covers NeoHookean and linear elasticity");
     ensure_fused_tables(h);
     double *gv = g;

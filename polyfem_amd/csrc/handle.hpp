@@ -26,6 +26,7 @@ struct pfgpu_handle {
     pfgpu::DevBuf brow_ptr, blk_out;
     std::vector<int32_t> h_contrib_ptr, h_contrib_code;
     pfgpu::GatherPlan blk_plan, fac_plan;
+    pfgpu::UnitPlan unit_plan; // the factored route's row-unit gather
     // plan of the segmented node-row gather (chunks, padded slots, row metadata)
     pfgpu::DevBuf row_chunk, row_slot, row_meta;
     int n_row_chunks = 0;
@@ -51,7 +52,8 @@ struct pfgpu_handle {
     pfgpu::DevBuf h_x, h_v, h_out, h_out2, h_vals; // staging for host-pointer calls
     // consistent mass matrix: one value per 3x3 block (block = m I_3)
     pfgpu::DevBuf mass, pat_bcol;
-    bool has_mass = false, lump_mass = false;
+    bool has_mass = false;
+    int lump_mass = 0; // 0 consistent, 1 row sums, 2 HRZ (PFGPU_MASS_*)
     std::vector<double> h_rho; // per-element density of the mass matrix (empty: rho_uniform)
     double rho_uniform = 0;
     // objective of the solves: w E(x) + 1/2 (x - xt)^T M (x - xt) - w f^T x
@@ -87,7 +89,7 @@ struct pfgpu_handle {
     template <typename T> pfgpu::DevMesh<T> mesh() const
     {
         return pfgpu::DevMesh<T>{n_elements, n_nodes, n_vertices, conn.as<int32_t>(), vconn.as<int32_t>(),
-                                 geom.as<T>(), lame.as<T>()};
+                                 geom.as<T>(), lame.as<T>(), material == PFGPU_MOONEY_RIVLIN ? 3 : 2};
     }
 };
 
@@ -97,9 +99,16 @@ using Handle = pfgpu_handle;
 
 // Deterministic FP64 sum of n doubles into out[0] (fixed two-level tree).
 void sum_doubles(Handle &h, const double *in, int n, double *out);
+// the same with the caller's first-level scratch (SUM_BLOCKS doubles, used
+// when n > 32 BLOCK); allocates nothing
+constexpr int SUM_BLOCKS = 128;
+void sum_doubles_into(Handle &h, const double *in, int n, double *out, double *part);
 
 // ---- plain path (plain.hip) -------------------------------------------
 void plain_energy(Handle &h, const double *x, double *energy, double *per_element);
+// the energy with the caller's scratch: partial (ceil(E / BLOCK) doubles) and
+// part (SUM_BLOCKS doubles); allocates nothing
+void energy_into(Handle &h, const double *x, double *energy, double *partial, double *part);
 void plain_gradient(Handle &h, const double *x, double *g, double *energy);
 void plain_hessian(Handle &h, const double *x, void *values, bool project_to_psd = false);
 // every element's local Hessian in the block tiles projected onto the PSD cone (psd.hip)

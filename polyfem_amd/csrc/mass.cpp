@@ -1,4 +1,4 @@
-// Consistent (or row-sum lumped) mass matrix of P1/P2 tets (Mass::assemble, assembler/Mass.cpp:4-20):
+// Consistent (row-sum or HRZ lumped) mass matrix of P1/P2 tets (Mass::assemble, assembler/Mass.cpp:4-20):
 // M_ab = rho int phi_a phi_b, block (a, b) = M_ab I_3.  The reference
 // integrates with the quadrature of order 2p (AssemblerUtils.cpp:204-211),
 // which is exact for these degree-2p integrands; here the element matrix is
@@ -88,7 +88,20 @@ std::vector<double> reference_mass(int order)
 void build_mass(Handle &h, const double *rho, double rho_uniform)
 {
     const int NB = h.n_local;
-    const std::vector<double> R = reference_mass(h.order);
+    std::vector<double> R = reference_mass(h.order);
+    if (h.lump_mass == PFGPU_MASS_HRZ) {
+        // HRZMass::assemble (Mass.cpp:48-86): the element's diagonal scaled
+        // so that it keeps the element's total mass; the same factor for
+        // every element, as the reference element matrix only scales
+        double all = 0, diag = 0;
+        for (int a = 0; a < NB; ++a)
+            for (int c = 0; c < NB; ++c) {
+                all += R[size_t(a) * NB + c];
+                if (a == c) diag += R[size_t(a) * NB + c];
+            }
+        for (int a = 0; a < NB; ++a)
+            for (int c = 0; c < NB; ++c) R[size_t(a) * NB + c] = a == c ? R[size_t(a) * NB + c] * all / diag : 0.0;
+    }
     std::vector<double> vals(size_t(h.pat.n_blocks), 0.0);
     for (int e = 0; e < h.n_elements; ++e) {
         // |T| from the P1 geometry
@@ -111,7 +124,7 @@ void build_mass(Handle &h, const double *rho, double rho_uniform)
             }
         }
     }
-    if (h.lump_mass) {
+    if (h.lump_mass == PFGPU_MASS_ROW_SUM) {
         // utils::lump_matrix (MatrixUtils.cpp:84-101): every row's sum on its
         // diagonal, nothing off it (State.cpp:1542-1545)
         for (int i = 0; i < h.n_nodes; ++i) {

@@ -7,6 +7,7 @@
 #include <thread>
 
 #include "dispatch.hpp"
+#include "plans.hpp"
 
 namespace pfgpu {
 
@@ -89,7 +90,7 @@ __global__ __launch_bounds__(BLOCK) void k_energy(DevMesh<T> m, const double *__
             Kin<T> k;
             field_grad<T, ORDER, NQ>(q, el.U, el.jit, k.H);
             kinematics(k);
-            acc += energy_density<T, MAT>(k, el.lam, el.mu) * w;
+            acc += energy_density<T, MAT>(k, el.lam, el.mu, el.p3) * w;
         }
         val = double(acc);
         if (per_elem) per_elem[e] = val;
@@ -117,8 +118,8 @@ __global__ __launch_bounds__(BLOCK) void k_energy_lame_grad(DevMesh<T> m, const 
         Kin<T> k;
         field_grad<T, ORDER, NQ>(q, el.U, el.jit, k.H);
         kinematics(k);
-        al += energy_density<T, MAT>(k, T(1), T(0)) * w;
-        am += energy_density<T, MAT>(k, T(0), T(1)) * w;
+        al += energy_density<T, MAT>(k, T(1), T(0), T(0)) * w;
+        am += energy_density<T, MAT>(k, T(0), T(1), T(0)) * w;
     }
     out[e] = double(al);
     out[m.n_elements + e] = double(am);
@@ -148,9 +149,9 @@ __global__ __launch_bounds__(BLOCK) void k_gradient(DevMesh<T> m, const double *
             Kin<T> k;
             field_grad<T, ORDER, NQ>(q, el.U, el.jit, k.H);
             kinematics(k);
-            acc += energy_density<T, MAT>(k, el.lam, el.mu) * w;
+            acc += energy_density<T, MAT>(k, el.lam, el.mu, el.p3) * w;
             T P[9];
-            pk1<T, MAT>(k, el.lam, el.mu, P);
+            pk1<T, MAT>(k, el.lam, el.mu, el.p3, P);
             contract_add<T, ORDER, NQ>(q, P, el.jit, w, g);
         }
         val = double(acc);
@@ -197,9 +198,9 @@ __global__ __launch_bounds__(64) void k_grad_factors(DevMesh<T> m, const double 
             Kin<T> k;
             field_grad<T, ORDER, NQ>(q, el.U, el.jit, k.H);
             kinematics(k);
-            acc += energy_density<T, MAT>(k, el.lam, el.mu) * w;
+            acc += energy_density<T, MAT>(k, el.lam, el.mu, el.p3) * w;
             T P[9];
-            pk1<T, MAT>(k, el.lam, el.mu, P);
+            pk1<T, MAT>(k, el.lam, el.mu, el.p3, P);
             contract_add<T, ORDER, NQ>(q, P, el.jit, w, g);
             hess_qp<T, MAT>(k, el.jit, el.lam, el.mu, w, r + 9 * q, r[9 * NQ + q], r[10 * NQ + q]);
         }
@@ -257,9 +258,18 @@ __global__ __launch_bounds__(HB) void k_hessian(DevMesh<T> m, const double *__re
     // mu F F^T per point.
     // FixedCorotational adds the rotation R and K^{-1} per point.
     constexpr int NX = MAT == MAT_SV || MAT == MAT_FC ? NQ : 1;
-    T Mq[NQ][9], cl[NQ], cc[NQ], sg6[NX][6], ff6[NX][9];
+    // MooneyRivlin forms its blocks from the directional stress derivative
+    // (below) and keeps nothing per point
+    constexpr int NM = MAT == MAT_MR ? 1 : NQ;
+    T Mq[NM][9], cl[NM], cc[NM], sg6[NX][6], ff6[NX][9];
 #pragma unroll
-    for (int q = 0; q < NQ; ++q) {
+    for (int q = 0; q < NM; ++q) {
+        if constexpr (MAT == MAT_MR) {
+#pragma unroll
+            for (int i = 0; i < 9; ++i) Mq[q][i] = T(0);
+            cl[q] = cc[q] = T(0);
+            continue;
+        }
         Kin<T> k;
         if (MAT != MAT_LE) field_grad<T, ORDER, NQ>(q, el.U, el.jit, k.H);
         if (MAT == MAT_NH || MAT == MAT_FC) kinematics(k);
@@ -284,9 +294,9 @@ __global__ __launch_bounds__(HB) void k_hessian(DevMesh<T> m, const double *__re
     // a and b are uniform over the wavefront: the table reads are scalar
 #pragma unroll 1
     for (int a = 0; a < NB; ++a) {
-        T Ka[NQ][3];
+        T Ka[NM][3];
 #pragma unroll
-        for (int q = 0; q < NQ; ++q)
+        for (int q = 0; q < NM; ++q)
 #pragma unroll
             for (int mm = 0; mm < 3; ++mm) {
                 const T *r = tb.rg + (q * NB + a) * 3;
@@ -295,9 +305,9 @@ __global__ __launch_bounds__(HB) void k_hessian(DevMesh<T> m, const double *__re
         // not unrolled: the k_b of all columns at once would not fit in registers
 #pragma unroll 1
         for (int b = a; b < NB; ++b) {
-            T Kb[NQ][3];
+            T Kb[NM][3];
 #pragma unroll
-            for (int q = 0; q < NQ; ++q)
+            for (int q = 0; q < NM; ++q)
 #pragma unroll
                 for (int mm = 0; mm < 3; ++mm) {
                     const T *r = tb.rg + (q * NB + b) * 3;
@@ -340,6 +350,32 @@ __global__ __launch_bounds__(HB) void k_hessian(DevMesh<T> m, const double *__re
                         for (int nn = 0; nn < 3; ++nn)
                             xr[mm * 3 + nn] += cr * (ca[mm][0] * kb[nn][0] + ca[mm][1] * kb[nn][1] + ca[mm][2] * kb[nn][2]);
                 }
+            } else if constexpr (MAT == MAT_MR) {
+                // column n of the block: sum_q da dP[e_n G_b^T] G_a (the
+                // kinematics of each point are formed again per block)
+#pragma unroll
+                for (int q = 0; q < NQ; ++q) {
+                    Kin<T> k;
+                    field_grad<T, ORDER, NQ>(q, el.U, el.jit, k.H);
+                    kinematics(k);
+                    const T *ra = tb.rg + (q * NB + a) * 3, *rb = tb.rg + (q * NB + b) * 3;
+                    T Ga[3], Gb[3];
+#pragma unroll
+                    for (int i = 0; i < 3; ++i) {
+                        Ga[i] = ra[0] * el.jit[0 * 3 + i] + ra[1] * el.jit[1 * 3 + i] + ra[2] * el.jit[2 * 3 + i];
+                        Gb[i] = rb[0] * el.jit[0 * 3 + i] + rb[1] * el.jit[1 * 3 + i] + rb[2] * el.jit[2 * 3 + i];
+                    }
+#pragma unroll
+                    for (int nn = 0; nn < 3; ++nn) {
+                        T W[9], dP[9];
+#pragma unroll
+                        for (int i = 0; i < 9; ++i) W[i] = i / 3 == nn ? Gb[i % 3] : T(0);
+                        mr_dstress(k, el.lam, el.mu, el.p3, W, dP);
+#pragma unroll
+                        for (int mm = 0; mm < 3; ++mm)
+                            xr[mm * 3 + nn] += w * (dP[mm * 3] * Ga[0] + dP[mm * 3 + 1] * Ga[1] + dP[mm * 3 + 2] * Ga[2]);
+                    }
+                }
             } else if constexpr (MAT == MAT_SV) {
                 // sum_q [ (G_a^T S G_b) I + mu (G_a . G_b) F F^T ] da
 #pragma unroll
@@ -360,9 +396,9 @@ __global__ __launch_bounds__(HB) void k_hessian(DevMesh<T> m, const double *__re
                 for (int nn = 0; nn < 3; ++nn) {
                     T v = mm == nn ? gd : T(0);
                     if constexpr (MAT == MAT_SV) v += xf[sym[mm * 3 + nn]];
-                    if constexpr (MAT == MAT_FC) v += xr[mm * 3 + nn];
+                    if constexpr (MAT == MAT_FC || MAT == MAT_MR) v += xr[mm * 3 + nn];
 #pragma unroll
-                    for (int q = 0; q < NQ; ++q) v += cl[q] * Ka[q][mm] * Kb[q][nn] + cc[q] * Kb[q][mm] * Ka[q][nn];
+                    for (int q = 0; q < NM; ++q) v += cl[q] * Ka[q][mm] * Kb[q][nn] + cc[q] * Kb[q][mm] * Ka[q][nn];
                     st[lane * 9 + mm * 3 + nn] = v;
                 }
             // the wavefront's 64 blocks s are 576 consecutive values
@@ -400,10 +436,10 @@ __global__ __launch_bounds__(64) void k_hvp(DevMesh<T> m, const double *__restri
         for (int q = 0; q < NQ; ++q) {
             Kin<T> k;
             field_grad<T, ORDER, NQ>(q, el.U, el.jit, k.H);
-            if (MAT == MAT_NH || MAT == MAT_FC) kinematics(k);
+            if (MAT == MAT_NH || MAT == MAT_FC || MAT == MAT_MR) kinematics(k);
             T Gv[9], A[9];
             field_grad<T, ORDER, NQ>(q, V, el.jit, Gv);
-            dstress<T, MAT>(k, el.lam, el.mu, Gv, A);
+            dstress<T, MAT>(k, el.lam, el.mu, el.p3, Gv, A);
             contract_add<T, ORDER, NQ>(q, A, el.jit, w, y);
         }
 #pragma unroll
@@ -570,11 +606,11 @@ __global__ __launch_bounds__(BLOCK) void k_shape_vjp(DevMesh<T> m, const double 
     for (int q = 0; q < NQ; ++q) {
         Kin<T> k;
         field_grad<T, ORDER, NQ>(q, el.U, el.jit, k.H);
-        if (MAT == MAT_NH || MAT == MAT_FC) kinematics(k);
+        if (MAT == MAT_NH || MAT == MAT_FC || MAT == MAT_MR) kinematics(k);
         T Gp[9], P[9], B[9];
         field_grad<T, ORDER, NQ>(q, Pn, el.jit, Gp);
-        pk1<T, MAT>(k, el.lam, el.mu, P);
-        dstress<T, MAT>(k, el.lam, el.mu, Gp, B);
+        pk1<T, MAT>(k, el.lam, el.mu, el.p3, P);
+        dstress<T, MAT>(k, el.lam, el.mu, el.p3, Gp, B);
         T sp = T(0);
 #pragma unroll
         for (int i = 0; i < 9; ++i) sp += P[i] * Gp[i];
@@ -944,6 +980,186 @@ __global__ __launch_bounds__(GS) void k_gather_factored(int n_chunks, const int3
     }
 }
 
+// Row-unit gather of the factored route.  A unit is one node incidence
+// (row node i = local node a of element e); the units of a node row are its
+// incident elements in element order (node_inc).  A chunk is a run of whole
+// rows holding at most GS units; thread t of the chunk's workgroup takes unit
+// t: it reads the element's factor record once, forms the n_local blocks
+// H_e(a, b) of its row, b = 0 .. n_local - 1 (b is unrolled, so the
+// reference gradients RG(q, b) are compile-time constants and their zeros
+// cost nothing), and stages each block at its position udst (the blocks of
+// the chunk in pattern order, the contributions of a block in unit order).
+// One thread per pattern block then sums its contributions into the CSR
+// values (3 runs of 3 values per block, consecutive blocks of a row
+// contiguous), and one thread per row sums the local gradient rows of its
+// units in FP64 when gloc is given.  Summation orders are fixed by the
+// tables: results are bitwise reproducible.  Per unit the record is read
+// once (the per-contribution form reads it n_local times) and the k vectors
+// of the row node are formed once.
+//
+//   H_e(a, b) = (sum_q RG(q,a)^T G RG(q,b)) I + sum_q [cl_q k_a k_b^T + cc_q k_b k_a^T],
+//   k_a = M_q RG(q, a),  G = w mu J^{-1} J^{-T} (the record's metric tail)
+template <typename T, int ORDER, int NQ, int GS>
+__global__ __launch_bounds__(GS) void k_gather_units(int n_chunks, const int32_t *__restrict__ chunk_row,
+                                                     const int32_t *__restrict__ node_ptr,
+                                                     const int32_t *__restrict__ node_inc,
+                                                     const int32_t *__restrict__ brow_ptr,
+                                                     const uint16_t *__restrict__ udst, const uint16_t *__restrict__ bq0,
+                                                     const T *__restrict__ rec, const T *__restrict__ gloc,
+                                                     T *__restrict__ values, double *__restrict__ grad)
+{
+    constexpr int NB = NLocal<ORDER>::value, R = fac_size(NQ), DW = NB / 2;
+    static_assert(NB % 2 == 0, "stage positions are read as 32-bit pairs");
+    __shared__ T srg[NQ * NB * 3];
+    __shared__ int32_t srb[GS + 1], sru[GS + 1];
+    extern __shared__ __align__(16) unsigned char smem[];
+    T *sv = reinterpret_cast<T *>(smem); // GS NB staged blocks of 9 values
+    T *sg = sv + GS * NB * 9;            // GS local gradient rows
+    const int c = xcd_chunk(n_chunks);
+    if (c >= n_chunks) return;
+    const int t = threadIdx.x;
+    const int i0 = chunk_row[c], i1 = chunk_row[c + 1], nr = i1 - i0;
+    const int u0 = node_ptr[i0], nu = node_ptr[i1] - u0;
+    const int B0 = brow_ptr[i0], nb = brow_ptr[i1] - B0;
+    // this thread's unit: the element's record, the stage positions of its
+    // blocks and its local gradient row
+    const bool has = t < nu;
+    int a = 0;
+    T r[R];
+    uint32_t dw[DW];
+    T l0 = T(0), l1 = T(0), l2 = T(0);
+    if (has) {
+        const int code = node_inc[u0 + t];
+        const int e = code / NB;
+        a = code - e * NB;
+        const T *rp = static_cast<const T *>(__builtin_assume_aligned(rec, 16)) + int64_t(e) * R;
+#pragma unroll
+        for (int i = 0; i < R; ++i) r[i] = rp[i];
+        const uint32_t *dp = reinterpret_cast<const uint32_t *>(udst + int64_t(u0 + t) * NB);
+#pragma unroll
+        for (int j = 0; j < DW; ++j) dw[j] = dp[j];
+        if (gloc) {
+            const T *gp = gloc + int64_t(code) * 3;
+            l0 = gp[0];
+            l1 = gp[1];
+            l2 = gp[2];
+        }
+    }
+    for (int k = t; k <= nr; k += GS) {
+        srb[k] = brow_ptr[i0 + k] - B0;
+        sru[k] = node_ptr[i0 + k] - u0;
+    }
+    const RefTab<T, ORDER, NQ> &tb = ref_tab<T, ORDER, NQ>();
+    for (int k = t; k < NQ * NB * 3; k += GS) srg[k] = tb.rg[k];
+    __syncthreads();
+    if (has) {
+        if (gloc) {
+            sg[t * 3] = l0;
+            sg[t * 3 + 1] = l1;
+            sg[t * 3 + 2] = l2;
+        }
+        // per point: k_a, cl k_a and G RG(q, a)
+        const T *G6 = r + 11 * NQ; // 00 11 22 01 02 12
+        T ka[NQ][3], la[NQ][3], ha[NQ][3];
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            const T *M = r + 9 * q, *ra = srg + (q * NB + a) * 3;
+            const T ra0 = ra[0], ra1 = ra[1], ra2 = ra[2];
+#pragma unroll
+            for (int mm = 0; mm < 3; ++mm) {
+                ka[q][mm] = fma_t(M[mm * 3 + 2], ra2, fma_t(M[mm * 3 + 1], ra1, M[mm * 3] * ra0));
+                la[q][mm] = r[9 * NQ + q] * ka[q][mm];
+            }
+            ha[q][0] = fma_t(G6[4], ra2, fma_t(G6[3], ra1, G6[0] * ra0));
+            ha[q][1] = fma_t(G6[5], ra2, fma_t(G6[1], ra1, G6[3] * ra0));
+            ha[q][2] = fma_t(G6[2], ra2, fma_t(G6[5], ra1, G6[4] * ra0));
+        }
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            T gd = T(0);
+#pragma unroll
+            for (int q = 0; q < NQ; ++q)
+#pragma unroll
+                for (int l = 0; l < 3; ++l) {
+                    const double rb = RG<ORDER, NQ>(q, b, l);
+                    if (rb != 0.0) gd = fma_t(ha[q][l], T(rb), gd);
+                }
+            T v[9];
+#pragma unroll
+            for (int i = 0; i < 9; ++i) v[i] = i % 4 == 0 ? gd : T(0);
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) {
+                const T *M = r + 9 * q;
+                T kb[3];
+#pragma unroll
+                for (int mm = 0; mm < 3; ++mm) {
+                    T s = T(0);
+                    bool first = true;
+#pragma unroll
+                    for (int l = 0; l < 3; ++l) {
+                        const double rb = RG<ORDER, NQ>(q, b, l);
+                        if (rb != 0.0) {
+                            s = first ? M[mm * 3 + l] * T(rb) : fma_t(M[mm * 3 + l], T(rb), s);
+                            first = false;
+                        }
+                    }
+                    kb[mm] = s;
+                }
+                const T cc = r[10 * NQ + q];
+                const T lb[3] = {cc * kb[0], cc * kb[1], cc * kb[2]};
+#pragma unroll
+                for (int mm = 0; mm < 3; ++mm)
+#pragma unroll
+                    for (int nn = 0; nn < 3; ++nn)
+                        v[mm * 3 + nn] = fma_t(lb[mm], ka[q][nn], fma_t(la[q][mm], kb[nn], v[mm * 3 + nn]));
+            }
+            const int d = int((dw[b >> 1] >> (16 * (b & 1))) & 0xffffu);
+            T *o = sv + d * 9;
+#pragma unroll
+            for (int i = 0; i < 9; ++i) o[i] = v[i];
+        }
+    }
+    __syncthreads();
+    // one thread per pattern block: its contributions in unit order
+    for (int bb = t; bb < nb; bb += GS) {
+        int lo = 0, hi = nr - 1; // the row: srb[lo] <= bb < srb[lo + 1]
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (srb[mid] <= bb)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        const int q0 = bq0[B0 + bb], q1 = bb + 1 < nb ? int(bq0[B0 + bb + 1]) : nu * NB;
+        T acc[9];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) acc[i] = T(0);
+        for (int k = q0; k < q1; ++k)
+#pragma unroll
+            for (int i = 0; i < 9; ++i) acc[i] += sv[k * 9 + i];
+        const int rb0 = srb[lo], deg = srb[lo + 1] - rb0;
+        T *out = values + 9 * int64_t(B0 + rb0) + 3 * (bb - rb0);
+#pragma unroll
+        for (int mm = 0; mm < 3; ++mm)
+#pragma unroll
+            for (int nn = 0; nn < 3; ++nn) out[mm * 3 * deg + nn] = acc[mm * 3 + nn];
+    }
+    // one thread per row: the gradient, in FP64 (rows without units get 0)
+    if (gloc)
+        for (int k = t; k < nr; k += GS) {
+            double s0 = 0, s1 = 0, s2 = 0;
+            for (int u = sru[k]; u < sru[k + 1]; ++u) {
+                s0 += double(sg[u * 3]);
+                s1 += double(sg[u * 3 + 1]);
+                s2 += double(sg[u * 3 + 2]);
+            }
+            double *g = grad + 3 * int64_t(i0 + k);
+            g[0] = s0;
+            g[1] = s1;
+            g[2] = s2;
+        }
+}
+
 // Diagonal blocks of the Hessian per node from the factor records, over the
 // row plan of k_gather_rows_seg (slot word e NB + a): H_e(a, a) =
 // (cm sum_q G_a . G_a) I + sum_q (cl_q + cc_q) k_a k_a^T, summed over the
@@ -1022,6 +1238,16 @@ __global__ __launch_bounds__(BLOCK) void k_node_diag(int n_chunks, const int32_t
 
 // node-row gather of element-local 3-vectors (defined below)
 void gather_rows(Handle &h, const void *loc, double *out);
+
+void energy_into(Handle &h, const double *x, double *energy, double *partial, double *part)
+{
+    const unsigned g = grid_for(h.n_elements);
+    dispatch(h, [&]<typename T, int O, int Q, int M>() {
+        k_energy<T, O, Q, M><<<g, BLOCK, 0, h.stream>>>(h.mesh<T>(), x, nullptr, partial);
+    });
+    PF_LAUNCH_CHECK();
+    sum_doubles_into(h, partial, int(g), energy, part);
+}
 
 void plain_energy(Handle &h, const double *x, double *energy, double *per_element)
 {
@@ -1227,15 +1453,79 @@ template <typename T, int O, int Q> void launch_factored_gather(Handle &h, Gathe
     PF_LAUNCH_CHECK();
 }
 
+// the row-unit plan of workgroup size gs (layout at k_gather_units, built by
+// make_unit_plan); false when a node has more incident elements than one
+// workgroup holds (the per-contribution gather then serves the handle)
+bool build_unit_plan(Handle &h, UnitPlan &p, int gs)
+{
+    if (p.threads == gs) return true;
+    UnitPlanHost u;
+    if (!make_unit_plan(h.pat, h.n_local, h.h_elem_nodes.data(), gs, u)) return false;
+    p.chunk_row = upload(u.chunk_row, h.stream);
+    p.udst = upload(u.udst, h.stream);
+    p.bq0 = upload(u.bq0, h.stream);
+    p.n_chunks = int(u.chunk_row.size()) - 1;
+    p.threads = gs;
+    p.grid = 0;
+    return true;
+}
+
+// the CSR values (and, with gloc, the gradient) from the factor records in
+// h.fac_rec over the row-unit plan
+template <typename T, int O, int Q> void launch_unit_gather(Handle &h, UnitPlan &p, const T *gloc, double *g, void *values)
+{
+    constexpr int NB = NLocal<O>::value;
+    auto gath = [&]<int GS>() {
+        auto kern = k_gather_units<T, O, Q, GS>;
+        const int lds = int(sizeof(T)) * GS * (NB * 9 + 3);
+        if (p.grid == 0) {
+            PF_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(kern), hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+            p.grid = xcd_grid(p.n_chunks);
+        }
+        kern<<<p.grid, GS, lds, h.stream>>>(p.n_chunks, p.chunk_row.as<int32_t>(), h.node_ptr.as<int32_t>(),
+                                            h.node_inc.as<int32_t>(), h.brow_ptr.as<int32_t>(), p.udst.as<uint16_t>(),
+                                            p.bq0.as<uint16_t>(), h.fac_rec.as<T>(), gloc, static_cast<T *>(values), g);
+    };
+    if (p.threads == 64)
+        gath.template operator()<64>();
+    else if (p.threads == 256)
+        gath.template operator()<256>();
+    else
+        gath.template operator()<128>();
+    PF_LAUNCH_CHECK();
+}
+
+// the factored route's gather for the tuned workgroup size: the row-unit
+// plan, or the per-contribution plan when a node has too many incident
+// elements for it (or the tuned size is 512)
+bool unit_plan_ready(Handle &h)
+{
+    if (h.material != MAT_NH && h.material != MAT_LE)
+        fail(PFGPU_ERR_UNSUPPORTED, "the factored Hessian covers NeoHookean and linear elasticity");
+    ensure_plain_hessian_maps(h);
+    int gs = h.tune_gather_threads;
+    if (gs > 256) return false;
+    // the staged blocks must fit the 160 KB of LDS (FP64 P2 at 256 threads
+    // does not): the largest fitting size at or below the tuned one
+    auto lds = [&](int g) { return int64_t(h.value_size()) * g * (h.n_local * 9 + 3) + 8 * (g + 1) + 4096; };
+    while (gs > 64 && lds(gs) > 160 * 1024) gs /= 2;
+    if (lds(gs) > 160 * 1024) return false;
+    return build_unit_plan(h, h.unit_plan, gs);
+}
+
 void factored_hessian(Handle &h, const double *x, void *values)
 {
-    GatherPlan &p = factored_plan(h);
+    const bool units = unit_plan_ready(h);
+    GatherPlan *p = units ? nullptr : &factored_plan(h);
     dispatch(h, [&]<typename T, int O, int Q, int M>() {
         if constexpr (M == MAT_NH || M == MAT_LE) {
             h.fac_rec.reserve(sizeof(T) * size_t(h.n_elements) * fac_size(Q));
             k_hfactors<T, O, Q, M><<<grid_for(h.n_elements, 64), 64, 0, h.stream>>>(h.mesh<T>(), x, h.fac_rec.as<T>());
             PF_LAUNCH_CHECK();
-            launch_factored_gather<T, O, Q>(h, p, values);
+            if (units)
+                launch_unit_gather<T, O, Q>(h, h.unit_plan, nullptr, nullptr, values);
+            else
+                launch_factored_gather<T, O, Q>(h, *p, values);
         }
     });
 }
@@ -1244,7 +1534,8 @@ void factored_hessian(Handle &h, const double *x, void *values)
 // gradient, the energy partials and the Hessian factor records
 void factored_energy_gradient_hessian(Handle &h, const double *x, double *energy, double *g, void *values)
 {
-    GatherPlan &p = factored_plan(h);
+    const bool units = unit_plan_ready(h);
+    GatherPlan *p = units ? nullptr : &factored_plan(h);
     const unsigned grid = grid_for(h.n_elements, 64);
     h.loc.reserve(h.value_size() * size_t(h.n_elements) * h.n_local * 3);
     if (energy) h.partial.reserve(sizeof(double) * grid);
@@ -1254,9 +1545,14 @@ void factored_energy_gradient_hessian(Handle &h, const double *x, double *energy
             k_grad_factors<T, O, Q, M><<<grid, 64, 0, h.stream>>>(
                 h.mesh<T>(), x, h.loc.as<T>(), energy ? h.partial.as<double>() : nullptr, h.fac_rec.as<T>());
             PF_LAUNCH_CHECK();
-            gather_rows(h, h.loc.get(), g);
+            if (units) {
+                // the gradient rows are summed by the same pass
+                launch_unit_gather<T, O, Q>(h, h.unit_plan, h.loc.as<T>(), g, values);
+            } else {
+                gather_rows(h, h.loc.get(), g);
+                launch_factored_gather<T, O, Q>(h, *p, values);
+            }
             if (energy) sum_doubles(h, h.partial.as<double>(), int(grid), energy);
-            launch_factored_gather<T, O, Q>(h, p, values);
         }
     });
 }
@@ -1347,6 +1643,8 @@ void plain_hvp(Handle &h, const double *x, const double *v, double *y)
 
 void plain_material_jvp(Handle &h, const double *x, const double *dl, const double *dm, double *y)
 {
+    if (h.material == MAT_MR)
+        fail(PFGPU_ERR_UNSUPPORTED, "MooneyRivlin has no Lame-parameter derivatives (as in the reference)");
     const unsigned grid = grid_for(h.n_elements);
     h.loc.reserve(h.value_size() * size_t(h.n_elements) * h.n_local * 3);
     dispatch(h, [&]<typename T, int O, int Q, int M>() {
@@ -1359,6 +1657,8 @@ void plain_material_jvp(Handle &h, const double *x, const double *dl, const doub
 
 void element_material_vjp(Handle &h, const double *x, const double *p, double *out)
 {
+    if (h.material == MAT_MR)
+        fail(PFGPU_ERR_UNSUPPORTED, "MooneyRivlin has no Lame-parameter derivatives (as in the reference)");
     dispatch(h, [&]<typename T, int O, int Q, int M>() {
         k_material_vjp<T, O, Q, M><<<grid_for(h.n_elements), BLOCK, 0, h.stream>>>(h.mesh<T>(), x, p, out);
     });
@@ -1367,6 +1667,8 @@ void element_material_vjp(Handle &h, const double *x, const double *p, double *o
 
 void energy_lame_gradient(Handle &h, const double *x, double *out)
 {
+    if (h.material == MAT_MR)
+        fail(PFGPU_ERR_UNSUPPORTED, "MooneyRivlin has no Lame-parameter derivatives (as in the reference)");
     dispatch(h, [&]<typename T, int O, int Q, int M>() {
         k_energy_lame_grad<T, O, Q, M><<<grid_for(h.n_elements), BLOCK, 0, h.stream>>>(h.mesh<T>(), x, out);
     });

@@ -65,21 +65,25 @@ __global__ __launch_bounds__(BLOCK) void k_expand_inner(int n_nodes, const int32
 
 } // namespace
 
-void sum_doubles(Handle &h, const double *in, int n, double *out)
+void sum_doubles_into(Handle &h, const double *in, int n, double *out, double *part)
 {
     // one block for short inputs; above that a fixed grid of SUM_BLOCKS
     // contiguous ranges, then one block over their sums.  The order depends
     // on n only.
-    constexpr int SUM_BLOCKS = 128;
     if (n <= 32 * BLOCK) {
         k_sum_doubles<<<1, BLOCK, 0, h.stream>>>(in, n, out);
     } else {
-        if (h.sum_part.bytes() < sizeof(double) * SUM_BLOCKS) h.sum_part.reserve(sizeof(double) * SUM_BLOCKS);
         const int seg = (n + SUM_BLOCKS - 1) / SUM_BLOCKS;
-        k_sum_doubles_blocks<<<SUM_BLOCKS, BLOCK, 0, h.stream>>>(in, n, seg, h.sum_part.as<double>());
-        k_sum_doubles<<<1, BLOCK, 0, h.stream>>>(h.sum_part.as<double>(), SUM_BLOCKS, out);
+        k_sum_doubles_blocks<<<SUM_BLOCKS, BLOCK, 0, h.stream>>>(in, n, seg, part);
+        k_sum_doubles<<<1, BLOCK, 0, h.stream>>>(part, SUM_BLOCKS, out);
     }
     PF_LAUNCH_CHECK();
+}
+
+void sum_doubles(Handle &h, const double *in, int n, double *out)
+{
+    if (n > 32 * BLOCK && h.sum_part.bytes() < sizeof(double) * SUM_BLOCKS) h.sum_part.reserve(sizeof(double) * SUM_BLOCKS);
+    sum_doubles_into(h, in, n, out, h.sum_part.as<double>());
 }
 
 bool all_finite(Handle &h, const void *dev, int64_t n, bool is_float)

@@ -1,4 +1,4 @@
-"""Gmsh ``.msh`` input for tetrahedral meshes (ASCII formats 2.2 and 4.1).
+"""Gmsh ``.msh`` input for tetrahedral meshes (formats 2.2 and 4.1, ASCII or binary).
 
 Follows the reference's reader (``io/MshReader.cpp:39-230``, which parses with
 mshio and keeps the volume elements):
@@ -11,7 +11,8 @@ mshio and keeps the volume elements):
 * the body id of an element is the first physical tag of its volume entity
   (format 4.1) or its first tag (format 2.2), 0 when there is none.
 
-Binary files and 2-D meshes are rejected with an error.
+Binary files (either byte order, detected from the format's endianness
+word) follow the same rules; 2-D meshes are rejected with an error.
 """
 from __future__ import annotations
 
@@ -144,27 +145,171 @@ def _read_41(L: _Lines):
     return V, tets, body
 
 
+class _Bytes:
+    """Cursor over a binary .msh file: section headers and counts are text
+    lines, the payload is native binary data in the file's byte order."""
+
+    def __init__(self, raw: bytes, order: str, size_t: int):
+        self.raw = raw
+        self.i = 0
+        self.o = order
+        self.st = {4: "u4", 8: "u8"}[size_t]
+
+    def section(self, name: str) -> bool:
+        key = b"$" + name.encode() + b"\n"
+        j = self.raw.find(key, self.i)
+        if j < 0:
+            return False
+        self.i = j + len(key)
+        return True
+
+    def line(self) -> str:
+        j = self.raw.find(b"\n", self.i)
+        if j < 0:
+            raise ValueError("unexpected end of .msh file")
+        ln = self.raw[self.i:j].decode("ascii").strip()
+        self.i = j + 1
+        return ln if ln else self.line()
+
+    def read(self, kind: str, n: int = 1) -> np.ndarray:
+        dt = np.dtype(self.o + (self.st if kind == "size_t" else kind))
+        if self.i + dt.itemsize * n > len(self.raw):
+            raise ValueError("unexpected end of .msh file")
+        a = np.frombuffer(self.raw, dtype=dt, count=n, offset=self.i)
+        self.i += dt.itemsize * n
+        return a.astype(np.float64 if kind == "f8" else np.int64)
+
+    def one(self, kind: str) -> int:
+        return int(self.read(kind, 1)[0])
+
+
+def _read_22_binary(B: _Bytes):
+    if not B.section("Nodes"):
+        raise ValueError("no $Nodes section")
+    n = int(B.line())
+    rec = np.dtype([("tag", B.o + "i4"), ("xyz", B.o + "f8", 3)])
+    if B.i + rec.itemsize * n > len(B.raw):
+        raise ValueError("unexpected end of .msh file")
+    a = np.frombuffer(B.raw, dtype=rec, count=n, offset=B.i)
+    B.i += rec.itemsize * n
+    tags = a["tag"].astype(np.int64)
+    lut = _node_index(tags, n)
+    V = np.empty((n, 3))
+    V[lut[tags]] = a["xyz"]
+    if not B.section("Elements"):
+        raise ValueError("no $Elements section")
+    ne = int(B.line())
+    tets, body = [], []
+    done = 0
+    while done < ne:
+        etype, nfollow, ntags = (int(v) for v in B.read("i4", 3))
+        if etype in TET_TYPES:
+            nn = TET_TYPES[etype]
+        elif etype in OTHER_TYPES:
+            nn = OTHER_TYPES[etype]
+        else:
+            raise ValueError(f"unknown gmsh element type {etype}")
+        blk = B.read("i4", nfollow * (1 + ntags + nn)).reshape(nfollow, 1 + ntags + nn)
+        if etype in TET_TYPES:
+            tets.extend(lut[blk[:, 1 + ntags:5 + ntags]])
+            body.extend(blk[:, 1].tolist() if ntags > 0 else [0] * nfollow)
+        done += nfollow
+    return V, tets, body
+
+
+def _read_41_binary(B: _Bytes):
+    volume_phys = {}
+    if B.section("Entities"):
+        npt, ncu, nsu, nvo = (int(v) for v in B.read("size_t", 4))
+        for _ in range(npt):
+            B.one("i4")
+            B.read("f8", 3)
+            B.read("i4", B.one("size_t"))
+        for kind in range(3):
+            for _ in range((ncu, nsu, nvo)[kind]):
+                tag = B.one("i4")
+                B.read("f8", 6)
+                phys = B.read("i4", B.one("size_t"))
+                B.read("i4", B.one("size_t"))
+                if kind == 2:
+                    volume_phys[tag] = int(phys[0]) if phys.size else 0
+    if not B.section("Nodes"):
+        raise ValueError("no $Nodes section")
+    nblocks, n, _, _ = (int(v) for v in B.read("size_t", 4))
+    tags = np.empty(n, dtype=np.int64)
+    xyz = np.empty((n, 3))
+    k = 0
+    for _ in range(nblocks):
+        _, _, parametric = B.read("i4", 3)
+        nb = B.one("size_t")
+        if parametric:
+            raise ValueError("parametric node coordinates are not supported")
+        tags[k:k + nb] = B.read("size_t", nb)
+        xyz[k:k + nb] = B.read("f8", 3 * nb).reshape(nb, 3)
+        k += nb
+    lut = _node_index(tags, n)
+    V = np.empty((n, 3))
+    V[lut[tags]] = xyz
+    if not B.section("Elements"):
+        raise ValueError("no $Elements section")
+    nblocks = int(B.read("size_t", 4)[0])
+    tets, body = [], []
+    for _ in range(nblocks):
+        dim, etag, etype = (int(v) for v in B.read("i4", 3))
+        nb = B.one("size_t")
+        if etype in TET_TYPES:
+            nn = TET_TYPES[etype]
+        elif etype in OTHER_TYPES:
+            nn = OTHER_TYPES[etype]
+        else:
+            raise ValueError(f"unknown gmsh element type {etype}")
+        blk = B.read("size_t", nb * (1 + nn)).reshape(nb, 1 + nn)
+        if etype in TET_TYPES and dim == 3:
+            tets.extend(lut[blk[:, 1:5]])
+            body.extend([volume_phys.get(etag, 0)] * nb)
+    return V, tets, body
+
+
 def read_msh(path: str):
     """(vertices (N, 3), tets (E, 4) vertex indices, body_ids (E,)) of a tet mesh."""
     with open(path, "rb") as f:
         raw = f.read()
-    try:
-        text = raw.decode("ascii")
-    except UnicodeDecodeError:
-        raise ValueError(f"{path}: binary .msh files are not supported") from None
-    L = _Lines(text)
-    if not L.section("MeshFormat"):
+    head = raw.find(b"$MeshFormat\n")
+    if head < 0:
         raise ValueError(f"{path}: no $MeshFormat section")
-    fmt = L.next().split()
-    version, file_type = fmt[0], int(fmt[1])
-    if file_type != 0:
-        raise ValueError(f"{path}: binary .msh files are not supported")
-    if version.startswith("2"):
-        V, tets, body = _read_22(L)
-    elif version.startswith("4"):
-        V, tets, body = _read_41(L)
+    j = raw.find(b"\n", head + 12)
+    fmt = raw[head + 12:j].decode("ascii", errors="replace").split()
+    if len(fmt) < 3:
+        raise ValueError(f"{path}: malformed $MeshFormat")
+    version, file_type, data_size = fmt[0], int(fmt[1]), int(fmt[2])
+    if file_type == 1:
+        # the endianness word: the integer 1 in the writer's byte order
+        word = raw[j + 1:j + 5]
+        if len(word) < 4:
+            raise ValueError(f"{path}: truncated binary header")
+        order = "<" if int.from_bytes(word, "little") == 1 else ">"
+        B = _Bytes(raw, order, data_size)
+        B.i = j + 5
+        if version.startswith("2"):
+            V, tets, body = _read_22_binary(B)
+        elif version.startswith("4"):
+            V, tets, body = _read_41_binary(B)
+        else:
+            raise ValueError(f"{path}: unsupported .msh version {version}")
     else:
-        raise ValueError(f"{path}: unsupported .msh version {version}")
+        try:
+            text = raw.decode("ascii")
+        except UnicodeDecodeError:
+            raise ValueError(f"{path}: non-ASCII bytes in an ASCII .msh file") from None
+        L = _Lines(text)
+        L.section("MeshFormat")
+        L.next()
+        if version.startswith("2"):
+            V, tets, body = _read_22(L)
+        elif version.startswith("4"):
+            V, tets, body = _read_41(L)
+        else:
+            raise ValueError(f"{path}: unsupported .msh version {version}")
     if not tets:
         raise ValueError(f"{path}: no tetrahedra (only 3-D tet meshes are supported)")
     tets = np.array(tets, dtype=np.int64)

@@ -130,15 +130,19 @@ class TransientSimulation:
     solver/forms/InertiaForm.cpp:17-35 with the elastic form weighted by dt^2)
     and its backward recurrence (optimization/StateDiff.cpp:320-413,
     AdjointTools.cpp:852-887 and 992-1010).  The mass matrix is the
-    consistent one (assembler/Mass.cpp:4-20) with density ``rho``, or its
-    row-sum lumping with ``lump_mass`` (the reference's
-    solver.advanced.lump_mass_matrix, State.cpp:1542-1545)."""
+    consistent one (assembler/Mass.cpp:4-20) with density ``rho``, its
+    row-sum lumping with ``lump_mass=True`` or ``"row_sum"`` (the reference's
+    solver.advanced.lump_mass_matrix, State.cpp:1542-1545), or the HRZ
+    diagonal with ``lump_mass="hrz"`` (HRZMass, Mass.cpp:48-86)."""
 
     def __init__(self, assembler, rho=1000.0, dt=1e-3, time_steps=10, checkpoint_every=1,
                  newton: NewtonSolver = None, lump_mass: bool = False):
         self.a = assembler
         E = assembler.mesh.n_elements
-        check(lib().pfgpu_set_mass_lumping(assembler._h, 1 if lump_mass else 0))
+        kinds = {False: 0, True: 1, "consistent": 0, "row_sum": 1, "hrz": 2}
+        if lump_mass not in kinds:
+            raise ValueError("lump_mass must be False, True, 'consistent', 'row_sum' or 'hrz'")
+        check(lib().pfgpu_set_mass_lumping(assembler._h, kinds[lump_mass]))
         if np.ndim(rho) == 0:
             check(lib().pfgpu_set_density(assembler._h, None, float(rho)))
         else:

@@ -1,0 +1,137 @@
+// CPU check of the host-built index tables (no GPU), meant to run under
+// AddressSanitizer / UndefinedBehaviorSanitizer.
+//
+// Builds the pattern (pattern.cpp), the row-unit plans of the factored
+// gather for 64, 128 and 256 threads (plans.cpp) and the fused kernel's
+// patch tables (fused_tables.cpp) exactly as the library does, then checks
+// every index the unit-gather kernel derives from its plan:
+//  1. chunks cover the rows in order, with at most GS units and GS rows;
+//  2. every unit's n_local stage positions lie inside the chunk's stage
+//     (units x n_local <= GS x n_local slots) and inside the range of the
+//     pattern block (row, node_e(b)) that bq0 gives; the positions of a
+//     chunk are a bijection onto [0, units x n_local);
+//  3. within a block, contributions are staged in unit (element) order, and
+//     each block's count equals a brute-force count of the row's elements
+//     that contain the column node.
+// Input: a binary mesh file (order, E, N, Nv, elem_nodes, elem_vertices,
+// vertices) written by tests/test_plans.py.
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <vector>
+
+#include "../../polyfem_amd/csrc/fused.hpp"
+#include "../../polyfem_amd/csrc/plans.hpp"
+
+using namespace pfgpu;
+
+static long n_fail = 0;
+#define CHECK(c, ...)                                                                                          \
+    do {                                                                                                       \
+        if (!(c)) {                                                                                            \
+            if (n_fail < 20) {                                                                                 \
+                std::fprintf(stderr, "CHECK failed %s:%d: ", __FILE__, __LINE__);                              \
+                std::fprintf(stderr, __VA_ARGS__);                                                             \
+                std::fprintf(stderr, "\n");                                                                    \
+            }                                                                                                  \
+            n_fail++;                                                                                          \
+        }                                                                                                      \
+    } while (0)
+
+int main(int argc, char **argv)
+{
+    if (argc < 2) {
+        std::fprintf(stderr, "usage: plan_check mesh.bin\n");
+        return 2;
+    }
+    FILE *f = std::fopen(argv[1], "rb");
+    if (!f) return 2;
+    int32_t hdr[4];
+    if (std::fread(hdr, 4, 4, f) != 4) return 2;
+    const int order = hdr[0], E = hdr[1], N = hdr[2], Nv = hdr[3], NB = order == 1 ? 4 : 10;
+    std::vector<int32_t> en(size_t(E) * NB), ev(size_t(E) * 4);
+    std::vector<double> vx(size_t(Nv) * 3);
+    if (std::fread(en.data(), 4, en.size(), f) != en.size() || std::fread(ev.data(), 4, ev.size(), f) != ev.size() ||
+        std::fread(vx.data(), 8, vx.size(), f) != vx.size())
+        return 2;
+    std::fclose(f);
+
+    const Pattern pat = build_pattern(E, NB, en.data(), N, Nv, ev.data());
+    const auto &np = pat.node_ptr, &ni = pat.node_inc, &bp = pat.brow_ptr, &bc = pat.bcol;
+    for (int gs : {64, 128, 256}) {
+        UnitPlanHost u;
+        if (!make_unit_plan(pat, NB, en.data(), gs, u)) {
+            std::printf("gs %d: a row has too many units (the per-contribution gather serves)\n", gs);
+            continue;
+        }
+        const int nc = int(u.chunk_row.size()) - 1;
+        CHECK(u.chunk_row.front() == 0 && u.chunk_row.back() == N, "chunk rows do not cover the nodes");
+        CHECK(u.udst.size() == size_t(np[N]) * NB && u.bq0.size() == size_t(pat.n_blocks), "plan sizes");
+        std::vector<uint8_t> seen;
+        for (int c = 0; c < nc; ++c) {
+            const int i0 = u.chunk_row[c], i1 = u.chunk_row[c + 1];
+            CHECK(i1 > i0, "empty chunk %d", c);
+            const int u0 = np[i0], nu = np[i1] - u0, nr = i1 - i0;
+            CHECK(nu <= gs && nr <= gs, "chunk %d: %d units, %d rows > %d", c, nu, nr, gs);
+            const int B0 = bp[i0], nb = bp[i1] - B0, total = nu * NB;
+            CHECK(nb <= total, "chunk %d: %d blocks > %d contributions", c, nb, total);
+            if (nb > 0) CHECK(u.bq0[B0] == 0, "chunk %d: first block does not start at 0", c);
+            for (int k = 0; k < nb; ++k) {
+                const int q0 = u.bq0[B0 + k], q1 = k + 1 < nb ? u.bq0[B0 + k + 1] : total;
+                CHECK(q0 <= q1 && q1 <= total, "chunk %d block %d: range [%d, %d) of %d", c, k, q0, q1, total);
+            }
+            seen.assign(size_t(total), 0);
+            for (int i = i0; i < i1; ++i) {
+                const int r0 = bp[i], deg = bp[i + 1] - r0;
+                for (int t = 1; t < deg; ++t) CHECK(bc[r0 + t - 1] < bc[r0 + t], "row %d: columns not sorted", i);
+                std::vector<int> last(size_t(deg), -1), count(size_t(deg), 0);
+                for (int uu = np[i]; uu < np[i + 1]; ++uu) {
+                    const int e = ni[uu] / NB, a = ni[uu] % NB;
+                    CHECK(e >= 0 && e < E && en[size_t(e) * NB + a] == i, "unit %d is not an incidence of row %d", uu, i);
+                    if (uu > np[i]) CHECK(ni[uu - 1] / NB < e, "row %d: units not in element order", i);
+                    for (int b = 0; b < NB; ++b) {
+                        const int j = en[size_t(e) * NB + b];
+                        const int32_t *cb = bc.data() + r0;
+                        const int p = int(std::lower_bound(cb, cb + deg, j) - cb);
+                        const int k = p < deg && cb[p] == j ? p : -1;
+                        CHECK(k >= 0, "row %d lacks column %d", i, j);
+                        if (k < 0) continue;
+                        const int d = u.udst[size_t(uu) * NB + b];
+                        const int q0 = u.bq0[r0 + k], q1 = (r0 + k + 1 - B0) < nb ? u.bq0[r0 + k + 1] : total;
+                        CHECK(d >= 0 && d < total, "unit %d block %d: stage %d outside [0, %d)", uu, b, d, total);
+                        CHECK(d >= q0 && d < q1, "unit %d block %d: stage %d outside its block [%d, %d)", uu, b, d, q0, q1);
+                        CHECK(d > last[size_t(k)], "block (%d, %d): not in unit order", i, j);
+                        last[size_t(k)] = d;
+                        count[size_t(k)]++;
+                        if (d >= 0 && d < total) {
+                            CHECK(!seen[size_t(d)], "chunk %d: stage %d used twice", c, d);
+                            seen[size_t(d)] = 1;
+                        }
+                    }
+                }
+                for (int t = 0; t < deg; ++t) {
+                    const int q0 = u.bq0[r0 + t], q1 = (r0 + t + 1 - B0) < nb ? u.bq0[r0 + t + 1] : total;
+                    CHECK(q1 - q0 == count[size_t(t)], "block (%d, %d): count %d, staged %d", i, bc[r0 + t], q1 - q0,
+                          count[size_t(t)]);
+                }
+            }
+            for (int d = 0; d < total; ++d) CHECK(seen[size_t(d)], "chunk %d: stage %d never written", c, d);
+        }
+        std::printf("gs %d: %d chunks checked\n", gs, nc);
+    }
+    // the fused kernel's tables, built for the FP32 P1/P2 launch shape (the
+    // sanitizers watch the builder; fused_check verifies their content)
+    {
+        FusedShape s{};
+        s.groups = order == 1 ? 70 : 28;
+        s.chunk = order == 1 ? 128 : 32;
+        s.item_cap = s.chunk * NB;
+        s.max_rows = 256;
+        s.acc_capacity = 30000;
+        const FusedTables t = build_fused_tables(pat, NB, en.data(), E, vx.data(), ev.data(), s);
+        CHECK(t.n_patches > 0, "no patches");
+        std::printf("fused tables: %d patches\n", t.n_patches);
+    }
+    std::printf("%s (%ld failures)\n", n_fail ? "FAIL" : "OK", n_fail);
+    return n_fail ? 1 : 0;
+}

@@ -84,6 +84,7 @@ def test_mixed_precision_parity(order, name, path):
     assert rel(a.hessian_vector_product(x, v), o.hessian_scipy(x) @ v) < MIXED_TOL["hvp"]
     p = np.random.default_rng(BASE_SEED + 302).normal(size=m.n_dofs)
     assert rel(a.force_material_derivative(x, p), o.material_vjp(x, p)) < MIXED_TOL["vjp"]
+    assert rel(a.force_shape_derivative(x, p), o.shape_vjp(x, p)) < MIXED_TOL["vjp"]
 
 
 @pytest.mark.parametrize("path", PATHS)
@@ -187,15 +188,21 @@ def test_autotune_and_tunings_agree():
             if ref is None:
                 ref = (e, g, vals)
             assert e == ref[0] and np.array_equal(g, ref[1]) and np.array_equal(vals, ref[2]), (gt, ht)
-    # the factored gather's chunking depends on its size, its per-block
-    # summation order does not
+    # the factored route's row-unit gather (64 to 256 threads): its chunking
+    # depends on the size, its per-block summation order does not; at 512
+    # threads the per-contribution gather forms each block with another
+    # operation order (same values to rounding)
     fref = None
     for gt in (64, 128, 256, 512):
         a.set_tuning("factored", gt, 256)
         e, g, vals = a.assemble_energy_gradient_hessian(x)
         if fref is None:
             fref = vals
-        assert e == ref[0] and np.array_equal(g, ref[1]) and np.array_equal(vals, fref), gt
+        assert e == ref[0] and np.array_equal(g, ref[1]), gt
+        if gt <= 256:
+            assert np.array_equal(vals, fref), gt
+        else:
+            assert rel(vals, fref) < 1e-6, gt
     assert rel(fref, ref[2]) < MIXED_TOL["hessian"]
     xd = torch.tensor(x, device="cuda:0")
     rep = a.autotune_dev(xd.data_ptr())

@@ -75,6 +75,36 @@ def test_lumped_mass_is_row_sums(order):
 
 
 @pytest.mark.parametrize("order", [1, 2])
+def test_hrz_lumped_mass(order):
+    """HRZ lumping (HRZMass, Mass.cpp:48-86): per element the diagonal of the
+    numerically integrated element mass, scaled to keep the element's total;
+    positive at P2 (where row sums are negative at the vertices); the total
+    mass is kept."""
+    m = kuhn_box(3, 2, 2, order=order)
+    rho = np.random.default_rng(BASE_SEED + 24).uniform(500, 1500, size=m.n_elements)
+    a = NLAssembler("NeoHookean", m, young=1e6, poisson=0.3)
+    print(f"HRZ mass order={order}", flush=True)
+    sim = TransientSimulation(a, rho=rho, lump_mass="hrz")
+    pts, wts = tet_rule()
+    phi = np.array([po.basis_val(m.order, p) for p in pts])
+    R = (phi * wts[:, None]).T @ phi
+    X = m.vertices[m.elem_vertices]
+    det = np.linalg.det(X[:, 1:] - X[:, :1])
+    d = np.zeros(m.n_nodes)
+    for e in range(m.n_elements):
+        Me = rho[e] * det[e] * R
+        np.add.at(d, m.elem_nodes[e], np.diag(Me) * Me.sum() / np.trace(Me))
+    v = np.random.default_rng(BASE_SEED + 25).normal(size=m.n_dofs)
+    ref = (d[:, None] * v.reshape(-1, 3)).ravel()
+    y = sim.mass_matvec(v)
+    assert np.max(np.abs(y - ref)) <= 1e-12 * np.max(np.abs(ref))
+    diag = sim.mass_matvec(np.ones(m.n_dofs)).reshape(-1, 3)[:, 0]
+    assert np.all(diag > 0)
+    vol = np.abs(det) / 6
+    assert abs(diag.sum() - rho @ vol) <= 1e-12 * (rho @ vol)
+
+
+@pytest.mark.parametrize("order", [1, 2])
 def test_mass_matches_numerical_integration(order):
     m = kuhn_box(3, 2, 2, order=order)
     rho = np.random.default_rng(BASE_SEED + 20).uniform(500, 1500, size=m.n_elements)

@@ -2,8 +2,10 @@
 
 Hand-written files pin the format details (node-tag mapping, skipped
 element types, body ids from physical tags, tet types with more than four
-nodes); a mesh written out and read back must give the same P1/P2 mesh as
-the generator, node numbering included.
+nodes); binary files are written per the gmsh format description in both
+byte orders; a mesh written out and read back must give the same P1/P2 mesh
+as the generator, node numbering included; a two-tet P2 mesh pins the node
+numbering against ids derived by hand from the reference's rule.
 """
 import numpy as np
 import pytest
@@ -85,11 +87,118 @@ def test_msh41_tags_entities_and_tet10(tmp_path):
     assert np.array_equal(body, [7, 7])
 
 
-def test_binary_is_rejected(tmp_path):
-    p = tmp_path / "bin.msh"
-    p.write_bytes(b"$MeshFormat\n4.1 1 8\n\x01\x00\x00\x00\n$EndMeshFormat\n")
-    with pytest.raises(ValueError, match="binary"):
+def write_msh41_binary(path, V, T, order="<", phys=7, with_surface=True):
+    """Binary MSH 4.1 per the gmsh format description: size_t = 8 bytes, an
+    $Entities section with one volume (physical tag phys), one node block,
+    a triangle block (skipped by the reader) and a tet block."""
+    import struct
+    st, i4, f8 = order + "Q", order + "i", order + "d"
+    b = bytearray(b"$MeshFormat\n4.1 1 8\n")
+    b += struct.pack(i4, 1) + b"\n$EndMeshFormat\n$Entities\n"
+    b += struct.pack(order + "4Q", 1, 0, 1, 1)
+    b += struct.pack(i4, 1) + struct.pack(order + "3d", 0, 0, 0) + struct.pack(st, 0)
+    b += struct.pack(i4, 1) + struct.pack(order + "6d", 0, 0, 0, 1, 1, 1) + struct.pack(st, 0) + struct.pack(st, 0)
+    b += struct.pack(i4, 1) + struct.pack(order + "6d", 0, 0, 0, 1, 1, 1) + struct.pack(st, 1) + struct.pack(i4, phys)
+    b += struct.pack(st, 1) + struct.pack(i4, 1)
+    b += b"\n$EndEntities\n$Nodes\n"
+    n = len(V)
+    b += struct.pack(order + "4Q", 1, n, 1, n) + struct.pack(order + "3i", 3, 1, 0) + struct.pack(st, n)
+    b += struct.pack(order + f"{n}Q", *range(1, n + 1))
+    b += struct.pack(order + f"{3 * n}d", *np.asarray(V, dtype=np.float64).ravel())
+    b += b"\n$EndNodes\n$Elements\n"
+    nt = len(T)
+    blocks = 2 if with_surface else 1
+    b += struct.pack(order + "4Q", blocks, nt + (1 if with_surface else 0), 1, nt + 1)
+    if with_surface:
+        b += struct.pack(order + "3i", 2, 1, 2) + struct.pack(st, 1) + struct.pack(order + "4Q", nt + 1, 1, 2, 3)
+    b += struct.pack(order + "3i", 3, 1, 4) + struct.pack(st, nt)
+    for e, t in enumerate(T):
+        b += struct.pack(order + "5Q", e + 1, *(int(v) + 1 for v in t))
+    b += b"\n$EndElements\n"
+    with open(path, "wb") as f:
+        f.write(bytes(b))
+
+
+def write_msh22_binary(path, V, T, order="<"):
+    """Binary MSH 2.2: ASCII counts, int tags and doubles, element blocks of
+    (type, count, ntags) headers; two tags (physical 5, elementary 1)."""
+    import struct
+    i4 = order + "i"
+    b = bytearray(b"$MeshFormat\n2.2 1 8\n") + struct.pack(i4, 1) + b"\n$EndMeshFormat\n"
+    b += f"$Nodes\n{len(V)}\n".encode()
+    for k, (x, y, z) in enumerate(np.asarray(V, dtype=np.float64)):
+        b += struct.pack(order + "i3d", k + 1, x, y, z)
+    b += b"\n$EndNodes\n"
+    b += f"$Elements\n{len(T) + 1}\n".encode()
+    b += struct.pack(order + "3i", 2, 1, 2) + struct.pack(order + "6i", 1, 5, 1, 1, 2, 3)
+    b += struct.pack(order + "3i", 4, len(T), 2)
+    for e, t in enumerate(T):
+        b += struct.pack(order + "7i", e + 2, 5, 1, *(int(v) + 1 for v in t))
+    b += b"\n$EndElements\n"
+    with open(path, "wb") as f:
+        f.write(bytes(b))
+
+
+@pytest.mark.parametrize("order", ["<", ">"])
+def test_binary_msh41(tmp_path, order):
+    m = kuhn_box(3, 2, 2, order=2)
+    p = tmp_path / "bin41.msh"
+    write_msh41_binary(p, m.vertices, m.elem_vertices, order=order)
+    V, T, body = read_msh(str(p))
+    assert np.array_equal(V, m.vertices) and np.array_equal(T, m.elem_vertices)
+    assert np.all(body == 7)
+    r = load_msh(str(p), order=2)
+    assert np.array_equal(r.elem_nodes, m.elem_nodes) and np.array_equal(r.nodes, m.nodes)
+
+
+@pytest.mark.parametrize("order", ["<", ">"])
+def test_binary_msh22(tmp_path, order):
+    m = kuhn_box(2, 3, 2, order=1)
+    p = tmp_path / "bin22.msh"
+    write_msh22_binary(p, m.vertices, m.elem_vertices, order=order)
+    V, T, body = read_msh(str(p))
+    assert np.array_equal(V, m.vertices) and np.array_equal(T, m.elem_vertices)
+    assert np.all(body == 5)
+
+
+def test_truncated_binary_is_an_error(tmp_path):
+    m = kuhn_box(1, 1, 1, order=1)
+    p = tmp_path / "cut.msh"
+    write_msh41_binary(p, m.vertices, m.elem_vertices)
+    raw = p.read_bytes()
+    cut = raw[:raw.find(b"$Elements") + 40]
+    p.write_bytes(cut)
+    with pytest.raises(ValueError):
         read_msh(str(p))
+
+
+def test_p2_numbering_known_answer():
+    """Node ids of a two-tet P2 mesh derived by hand from the reference's
+    rule: cells in order; a cell's vertices, then its edges (v0v1, v1v2,
+    v2v0, v0v3, v1v3, v2v3) get the next id at their first encounter
+    (LagrangeBasis3d.cpp:238-330 with MeshNodes::node_id_from_primitive,
+    MeshNodes.cpp:159-180; the cell loop LagrangeBasis3d.cpp:853-888)."""
+    V = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1], [1, 1, 1.0]])
+    T = np.array([[0, 1, 2, 3], [1, 2, 3, 4]])
+    # both tets positively oriented, so no vertex swap applies
+    for t in T:
+        X = V[t]
+        assert np.linalg.det(X[1:] - X[0]) > 0
+    m = tet_mesh(V, T, order=2)
+    # cell 0: vertices 0..3 -> 0..3; edges {0,1}->4 {1,2}->5 {2,0}->6 {0,3}->7 {1,3}->8 {2,3}->9
+    # cell 1 (1,2,3,4): vertex 4 -> 10; edges {1,2}=5 {2,3}=9 {3,1}=8, then new {1,4}->11 {2,4}->12 {3,4}->13
+    assert np.array_equal(m.elem_nodes, [[0, 1, 2, 3, 4, 5, 6, 7, 8, 9], [1, 2, 3, 10, 5, 9, 8, 11, 12, 13]])
+    assert m.n_nodes == 14
+    mid = lambda i, j: (V[i] + V[j]) / 2
+    expect = np.array([V[0], V[1], V[2], V[3], mid(0, 1), mid(1, 2), mid(2, 0), mid(0, 3), mid(1, 3), mid(2, 3),
+                       V[4], mid(1, 4), mid(2, 4), mid(3, 4)])
+    assert np.array_equal(m.nodes, expect)
+    # P1: the vertices in first-encounter order
+    m1 = tet_mesh(V, T, order=1)
+    assert np.array_equal(m1.elem_nodes, [[0, 1, 2, 3], [1, 2, 3, 4]])
+    # first encounter differs from the vertex ids when the cells list them out of order
+    m2 = tet_mesh(V, [[1, 2, 3, 4], [0, 1, 2, 3]], order=1)
+    assert np.array_equal(m2.elem_nodes, [[0, 1, 2, 3], [4, 0, 1, 2]])
 
 
 def write_msh41(path, V, T):

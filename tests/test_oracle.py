@@ -574,3 +574,96 @@ def test_fcr_assembly_derivatives_vs_fd(order):
     hs = 1e-5
     fds = p @ (g_at(hs) - g_at(-hs)) / (2 * hs)
     assert abs(shape @ dX.reshape(-1) - fds) <= 1e-5 * max(abs(fds), 1e-12)
+
+
+# ------------------------------------------------------ MooneyRivlin
+
+MR_C1, MR_C2, MR_K = 2.0e5, 1.0e5, 2.0e6
+
+
+def test_mooney_stress_is_complex_step_of_the_energy():
+    """The closed-form P against d psi / d F by complex step of the literal
+    energy (F~ = J^{-1/3} F and the invariants of C~), exact to rounding;
+    the stiffness is symmetric and agrees with central FD of P."""
+    from oracle import mooney
+    rng = np.random.default_rng(BASE_SEED + 30)
+    for scale in (0.05, 0.3):
+        F = np.eye(3) + scale * rng.normal(size=(3, 3))
+        if np.linalg.det(F) <= 0.2:
+            F = np.eye(3) + 0.1 * rng.normal(size=(3, 3))
+        P = mooney.stress(F, MR_C1, MR_C2, MR_K)
+        cs = np.zeros((3, 3))
+        for i in range(3):
+            for j in range(3):
+                Fc = F.astype(np.complex128)
+                Fc[i, j] += 1e-30j
+                cs[i, j] = mooney.energy_density(Fc, MR_C1, MR_C2, MR_K).imag / 1e-30
+        assert np.abs(cs - P).max() <= 1e-12 * np.abs(P).max()
+        A = mooney.stiffness(F, MR_C1, MR_C2, MR_K)
+        assert np.abs(A - A.T).max() <= 1e-10 * np.abs(A).max()
+        h = 1e-6
+        for i in range(3):
+            for j in range(3):
+                d = np.zeros((3, 3))
+                d[i, j] = h
+                dP = (mooney.stress(F + d, MR_C1, MR_C2, MR_K) - mooney.stress(F - d, MR_C1, MR_C2, MR_K)) / (2 * h)
+                assert np.abs(dP.reshape(-1, order="F") - A[:, i + 3 * j]).max() <= 1e-7 * np.abs(A).max()
+
+
+def test_mooney_closed_forms():
+    """Zero energy and stress at rest and under a rotation; a pure volume
+    change s I costs k/2 ln^2 s^3 (the isochoric invariants stay 3); at rest
+    the element Hessian is the linear-elasticity stiffness with
+    mu = 2 (c1 + c2), lambda = k - 4/3 (c1 + c2)."""
+    from oracle import mooney
+    th = 0.7
+    R = np.array([[np.cos(th), -np.sin(th), 0], [np.sin(th), np.cos(th), 0], [0, 0, 1]])
+    for F in (np.eye(3), R):
+        assert abs(mooney.energy_density(F, MR_C1, MR_C2, MR_K)) < 1e-9 * MR_C1
+        assert np.abs(mooney.stress(F, MR_C1, MR_C2, MR_K)).max() < 1e-9 * MR_C1
+    s = 1.1
+    assert abs(mooney.energy_density(s * np.eye(3), MR_C1, MR_C2, MR_K) - MR_K / 2 * np.log(s ** 3) ** 2) < 1e-9 * MR_K
+    F = np.eye(3) + 0.2 * np.random.default_rng(BASE_SEED + 31).normal(size=(3, 3))
+    assert abs(mooney.energy_density(R @ F, MR_C1, MR_C2, MR_K) - mooney.energy_density(F, MR_C1, MR_C2, MR_K)) \
+        < 1e-9 * abs(mooney.energy_density(F, MR_C1, MR_C2, MR_K))
+    mu = 2 * (MR_C1 + MR_C2)
+    lam = MR_K - 2.0 / 3.0 * mu
+    for order in (1, 2):
+        m = kuhn_box(1, 1, 1, order=order)
+        o = mooney.MooneyOracle(m, MR_C1, MR_C2, MR_K)
+        ol = po.Oracle(m, lam, mu, material=po.LINEAR_ELASTICITY)
+        x0 = np.zeros(m.n_dofs)
+        assert abs(o.hessian_scipy(x0) - ol.hessian_scipy(x0)).max() < 1e-9 * MR_K
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_mooney_assembly_derivatives_vs_fd(order):
+    """Global gradient and Hessian against central FD of the global energy
+    and gradient; the shape term against FD of the gradient."""
+    from oracle import mooney
+    m = kuhn_box(2, 1, 1, order=order)
+    rng = np.random.default_rng(BASE_SEED + 32)
+    c1 = MR_C1 * np.exp(rng.normal(0, 0.3, m.n_elements))
+    c2 = MR_C2 * np.exp(rng.normal(0, 0.3, m.n_elements))
+    kk = MR_K * np.exp(rng.normal(0, 0.3, m.n_elements))
+    o = mooney.MooneyOracle(m, c1, c2, kk)
+    x = smooth_displacement(m.nodes, 0.05, 1e-3, BASE_SEED + 33)
+    d = rng.normal(size=m.n_dofs)
+    h = 1e-6
+    fd = (o.energy(x + h * d) - o.energy(x - h * d)) / (2 * h)
+    g = o.gradient(x)
+    assert abs(fd - g @ d) <= 1e-6 * abs(fd)
+    fdg = (o.gradient(x + h * d) - o.gradient(x - h * d)) / (2 * h)
+    assert np.abs(o.hessian_scipy(x) @ d - fdg).max() <= 1e-6 * np.abs(fdg).max()
+    p = rng.normal(size=m.n_dofs)
+    shape = o.shape_vjp(x, p)
+    dX = rng.normal(size=m.vertices.shape) * 1e-2
+
+    def g_at(sg):
+        m2 = kuhn_box(2, 1, 1, order=order)
+        m2.vertices = m.vertices + sg * dX
+        return mooney.MooneyOracle(m2, c1, c2, kk).gradient(x)
+
+    hs = 1e-5
+    fds = p @ (g_at(hs) - g_at(-hs)) / (2 * hs)
+    assert abs(shape @ dX.reshape(-1) - fds) <= 1e-5 * max(abs(fds), 1e-12)
