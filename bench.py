@@ -162,18 +162,33 @@ def extras(args, m, lam, mu, x_host, a, x, g, vals, y, eqp):
                       "mixed_ms": t_mixed, "fp64_plain_ms": t_fp64,
                       "speedup_vs_fp64_plain": t_fp64 / t_mixed, "cg_iterations": it_m}
     del a64
-    # Newton-CG from the seeded displacement with the Dirichlet values applied
+    # Newton-CG from the seeded displacement with the Dirichlet values applied,
+    # in mixed precision and, for the trajectory comparison, in FP64
     if args.newton_iterations > 0:
         solver = NewtonSolver(max_iterations=args.newton_iterations, grad_norm=0.0, cg_max_iterations=200,
                               cg_tolerance=1e-4)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        _, rep = solver.minimize(a, x_bc, prob)
-        t = time.perf_counter() - t0
-        res["newton_cg"] = {"iterations": rep.iterations, "cg_iterations": rep.cg_iterations, "seconds": t,
-                            "ms_per_newton_iteration": t * 1e3 / max(rep.iterations, 1),
-                            "final_grad_norm": rep.grad_norm, "energy": rep.energy,
-                            "line_search_halvings": rep.line_search_halvings}
+
+        def newton(asm):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            xs, rep = solver.minimize(asm, x_bc, prob)
+            t = time.perf_counter() - t0
+            return xs, {"iterations": rep.iterations, "cg_iterations": rep.cg_iterations, "seconds": t,
+                        "ms_per_newton_iteration": t * 1e3 / max(rep.iterations, 1),
+                        "initial_grad_norm": rep.initial_grad_norm, "final_grad_norm": rep.grad_norm,
+                        "grad_norm_ratio": rep.grad_norm / rep.initial_grad_norm, "energy": rep.energy,
+                        "line_search_halvings": rep.line_search_halvings, "gradient_steps": rep.gradient_steps,
+                        "cg_stops": {"tolerance": rep.cg_stops_tolerance, "curvature": rep.cg_stops_curvature,
+                                     "cap": rep.cg_stops_cap}}
+
+        xm, res["newton_cg"] = newton(a)
+        a64n = NLAssembler("NeoHookean", m, lame=(lam, mu), precision="fp64")
+        xd64, r64 = newton(a64n)
+        del a64n
+        u0 = x_bc[prob.free]
+        res["newton_cg"]["fp64"] = r64
+        res["newton_cg"]["mixed_vs_fp64_step_rel"] = float(
+            np.abs((xm - xd64)[prob.free]).max() / max(np.abs((xd64[prob.free] - u0)).max(), 1e-300))
     return res
 
 

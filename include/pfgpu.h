@@ -247,7 +247,14 @@ typedef struct {
     /* steps that fell back to steepest descent (Cauchy step) after the
      * Newton direction was rejected by the line search */
     int32_t gradient_steps;
+    /* how each Newton iteration's CG ended: relative residual at the
+     * tolerance, non-positive curvature p^T H p <= 0 (the direction is the
+     * iterate reached so far, the iteration where it happens), or the
+     * iteration cap */
+    int32_t cg_stops_tolerance, cg_stops_curvature, cg_stops_cap;
+    double initial_grad_norm; /* reduced gradient norm at the initial guess */
 } pfgpu_newton_report;
+enum { PFGPU_CG_TOLERANCE = 0, PFGPU_CG_CURVATURE = 1, PFGPU_CG_ITERATION_CAP = 2 };
 
 /* x: n_dofs values, the initial guess with the Dirichlet values already on
  * the fixed dofs, overwritten by the result; fixed: n_dofs bytes, 1 = fixed */
@@ -268,6 +275,24 @@ pfgpu_status pfgpu_newton_solve_dev(pfgpu_handle *h, double *x, const uint8_t *f
 pfgpu_status pfgpu_newton_solve_batch(pfgpu_handle *h, double *x, const uint8_t *fixed, int32_t n_problems,
                                       const int32_t *elem_offsets, const int32_t *node_offsets,
                                       const pfgpu_newton_params *params, pfgpu_newton_report *reports);
+/* A batch of independent problems of uneven size and mixed order and
+ * material (P1/P2 x NeoHookean/linear elasticity, default quadrature), the
+ * reference's independent states solved side by side (the parallel loop of
+ * AdjointNLProblem.cpp:439): one handle whose element kernels (energy,
+ * gradient, Hv, the preconditioner's diagonal blocks) and node-row gathers run
+ * as one launch each over all problems, whatever their order and material;
+ * the problems are grouped by (order, material) internally, each group's
+ * meshes concatenated.  meshes / materials: n_problems descriptors. */
+pfgpu_status pfgpu_batch_create(int32_t n_problems, const pfgpu_mesh_desc *meshes,
+                                const pfgpu_material_desc *materials, int32_t precision, pfgpu_handle **out);
+/* Newton-CG on every problem of a pfgpu_batch_create handle at once, with
+ * the per-problem line search, CG stopping and convergence of
+ * pfgpu_newton_solve_batch.  x[i]: problem i's n_dofs initial guess (host;
+ * Dirichlet values already on its fixed dofs), overwritten by the result;
+ * fixed[i]: its mask (1 = fixed) or NULL; reports: n_problems entries, in
+ * the caller's order. */
+pfgpu_status pfgpu_batch_newton(pfgpu_handle *h, double *const *x, const uint8_t *const *fixed,
+                                const pfgpu_newton_params *params, pfgpu_newton_report *reports);
 /* CG solve of H(x)_ff y_f = b_f with y = 0 on fixed dofs, matrix-free; the
  * static adjoint solve (solve_static_adjoint, StateDiff.cpp:230-318; the
  * Hessian is symmetric, so H^T = H).  Stops at ||r|| <= tolerance ||b_f||. */

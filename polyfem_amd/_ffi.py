@@ -47,7 +47,8 @@ class NewtonParams(C.Structure):
 class NewtonReport(C.Structure):
     _fields_ = [("iterations", C.c_int32), ("cg_iterations", C.c_int32), ("line_search_halvings", C.c_int32),
                 ("converged", C.c_int32), ("energy", C.c_double), ("grad_norm", C.c_double),
-                ("last_alpha", C.c_double), ("gradient_steps", C.c_int32)]
+                ("last_alpha", C.c_double), ("gradient_steps", C.c_int32), ("cg_stops_tolerance", C.c_int32),
+                ("cg_stops_curvature", C.c_int32), ("cg_stops_cap", C.c_int32), ("initial_grad_norm", C.c_double)]
 
 
 class TransientParams(C.Structure):
@@ -116,6 +117,8 @@ _SIGNATURES = {
     "pfgpu_autotune": [vp, vp, C.POINTER(TuneReport)],
     "pfgpu_set_tuning": [vp, C.c_int32, C.c_int32, C.c_int32],
     "pfgpu_newton_solve_batch": [vp, vp, vp, C.c_int32, vp, vp, C.POINTER(NewtonParams), vp],
+    "pfgpu_batch_create": [C.c_int32, vp, vp, C.c_int32, C.POINTER(vp)],
+    "pfgpu_batch_newton": [vp, vp, vp, C.POINTER(NewtonParams), vp],
     "pfgpu_energy_lame_gradient": [vp, vp, vp],
     "pfgpu_set_density": [vp, vp, C.c_double],
     "pfgpu_set_mass_lumping": [vp, C.c_int32],

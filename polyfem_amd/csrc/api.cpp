@@ -192,6 +192,7 @@ int64_t pfgpu_handle::device_bytes() const
     int64_t s = 0;
     for (const DevBuf *b : bufs) s += int64_t(b->bytes());
     for (const DevBuf &b : ckpt) s += int64_t(b.bytes());
+    for (const auto &g : groups) s += g->device_bytes();
     return s;
 }
 
@@ -199,81 +200,104 @@ extern "C" {
 
 const char *pfgpu_last_error(void) { return g_last_error.c_str(); }
 
+} // extern "C"
+
+namespace {
+
+// the handle of one mesh and material (pfgpu_create)
+std::unique_ptr<pfgpu_handle> create_handle(const pfgpu_mesh_desc *mesh, const pfgpu_material_desc *material,
+                                            int32_t precision)
+{
+    require(mesh != nullptr && material != nullptr, "null mesh or material descriptor");
+    const pfgpu_mesh_desc &m = *mesh;
+    require(m.order == 1 || m.order == 2, "order must be 1 or 2");
+    require(m.quadrature_order <= 2, "quadrature orders above 2 are not provided");
+    require(m.n_elements > 0 && m.n_nodes > 0 && m.n_vertices > 0, "empty mesh");
+    require(m.elem_nodes && m.vertices && m.elem_vertices, "null mesh array");
+    require(material->kind == PFGPU_NEOHOOKEAN || material->kind == PFGPU_LINEAR_ELASTICITY ||
+                material->kind == PFGPU_SAINT_VENANT || material->kind == PFGPU_FIXED_COROTATIONAL ||
+                material->kind == PFGPU_MOONEY_RIVLIN,
+            "unknown material");
+    require(precision == PFGPU_FP64 || precision == PFGPU_MIXED, "unknown precision");
+    require((material->lambda == nullptr) == (material->mu == nullptr), "lambda and mu must both be given or both be NULL");
+
+    auto h = std::make_unique<pfgpu_handle>();
+    PF_HIP(hipGetDevice(&h->device));
+    h->order = m.order;
+    h->n_local = m.order == 1 ? 4 : 10;
+    const int qo = m.quadrature_order <= 0 ? (m.order == 1 ? 1 : 2) : m.quadrature_order;
+    h->n_quad = qo == 1 ? 1 : 4;
+    h->material = material->kind;
+    h->precision = precision;
+    h->path = material->kind == PFGPU_NEOHOOKEAN || material->kind == PFGPU_LINEAR_ELASTICITY ? PFGPU_PATH_FACTORED
+                                                                                             : PFGPU_PATH_PLAIN;
+    h->n_elements = m.n_elements;
+    h->n_nodes = m.n_nodes;
+    h->n_vertices = m.n_vertices;
+    h->n_dofs = 3 * m.n_nodes;
+
+    h->h_elem_nodes.assign(m.elem_nodes, m.elem_nodes + size_t(m.n_elements) * h->n_local);
+    h->h_vertices.assign(m.vertices, m.vertices + size_t(m.n_vertices) * 3);
+    h->h_elem_vertices.assign(m.elem_vertices, m.elem_vertices + size_t(m.n_elements) * 4);
+    h->pat = build_pattern(m.n_elements, h->n_local, m.elem_nodes, m.n_nodes, m.n_vertices, m.elem_vertices);
+
+    h->vconn = upload_soa<int32_t, int32_t>(m.elem_vertices, m.n_elements, 4, h->stream);
+    upload_geometry(*h, m);
+
+    if (material->kind == PFGPU_MOONEY_RIVLIN) {
+        const bool given = material->c1 != nullptr;
+        require(given == (material->c2 != nullptr) && given == (material->k != nullptr),
+                "c1, c2 and k must all be given or all be NULL");
+        std::vector<double> u1, u2, u3;
+        const double *rows[3] = {material->c1, material->c2, material->k};
+        if (!given) {
+            u1.assign(size_t(m.n_elements), material->c1_uniform);
+            u2.assign(size_t(m.n_elements), material->c2_uniform);
+            u3.assign(size_t(m.n_elements), material->k_uniform);
+            rows[0] = u1.data(), rows[1] = u2.data(), rows[2] = u3.data();
+        }
+        upload_params(*h, rows, 3);
+    } else if (material->lambda) {
+        upload_lame(*h, material->lambda, material->mu);
+    } else {
+        const double E = material->young, nu = material->poisson;
+        require(std::isfinite(E) && E > 0 && nu > -1.0 && nu < 0.5, "invalid Young's modulus / Poisson ratio");
+        // LameParameters (MatParams.cpp:11-22), 3D
+        const double lam = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu)), mu = E / (2.0 * (1.0 + nu));
+        std::vector<double> la(m.n_elements, lam), mv(m.n_elements, mu);
+        upload_lame(*h, la.data(), mv.data());
+    }
+    h->conn = upload_soa<int32_t, int32_t>(m.elem_nodes, m.n_elements, h->n_local, h->stream);
+    h->node_ptr = upload(h->pat.node_ptr, h->stream);
+    h->node_inc = upload(h->pat.node_inc, h->stream);
+    h->vert_ptr = upload(h->pat.vert_ptr, h->stream);
+    h->vert_inc = upload(h->pat.vert_inc, h->stream);
+    // fixed-size scratch of the reductions, and the element-level scratch
+    // that several entry points share, sized here for the largest use
+    // (energy partials per 64 elements, 3 n_local local values per element,
+    // the factor records) so that a graph captured over one *_dev call stays
+    // valid after any other call (a path's gather plans are built by its
+    // first call)
+    h->sum_part.reserve(sizeof(double) * 128);
+    h->red_out.reserve(sizeof(int) * 4);
+    h->partial.reserve(sizeof(double) * size_t((int64_t(m.n_elements) + 63) / 64 + 1));
+    h->loc.reserve(h->value_size() * size_t(m.n_elements) * size_t(h->n_local) * 3);
+    if (h->material == PFGPU_NEOHOOKEAN || h->material == PFGPU_LINEAR_ELASTICITY)
+        h->fac_rec.reserve(h->value_size() * size_t(m.n_elements) * size_t((11 * h->n_quad + 6 + 3) & ~3));
+    return h;
+}
+
+} // namespace
+
+extern "C" {
+
 pfgpu_status pfgpu_create(const pfgpu_mesh_desc *mesh, const pfgpu_material_desc *material, int32_t precision,
                           pfgpu_handle **out)
 {
     return guard([&] {
         require(out != nullptr, "null output handle pointer");
         *out = nullptr;
-        require(mesh != nullptr && material != nullptr, "null mesh or material descriptor");
-        const pfgpu_mesh_desc &m = *mesh;
-        require(m.order == 1 || m.order == 2, "order must be 1 or 2");
-        require(m.quadrature_order <= 2, "quadrature orders above 2 are not provided");
-        require(m.n_elements > 0 && m.n_nodes > 0 && m.n_vertices > 0, "empty mesh");
-        require(m.elem_nodes && m.vertices && m.elem_vertices, "null mesh array");
-        require(material->kind == PFGPU_NEOHOOKEAN || material->kind == PFGPU_LINEAR_ELASTICITY ||
-                    material->kind == PFGPU_SAINT_VENANT || material->kind == PFGPU_FIXED_COROTATIONAL ||
-                    material->kind == PFGPU_MOONEY_RIVLIN,
-                "unknown material");
-        require(precision == PFGPU_FP64 || precision == PFGPU_MIXED, "unknown precision");
-        require((material->lambda == nullptr) == (material->mu == nullptr), "lambda and mu must both be given or both be NULL");
-
-        auto h = std::make_unique<pfgpu_handle>();
-        PF_HIP(hipGetDevice(&h->device));
-        h->order = m.order;
-        h->n_local = m.order == 1 ? 4 : 10;
-        const int qo = m.quadrature_order <= 0 ? (m.order == 1 ? 1 : 2) : m.quadrature_order;
-        h->n_quad = qo == 1 ? 1 : 4;
-        h->material = material->kind;
-        h->precision = precision;
-        h->path = material->kind == PFGPU_NEOHOOKEAN || material->kind == PFGPU_LINEAR_ELASTICITY ? PFGPU_PATH_FACTORED
-                                                                                                 : PFGPU_PATH_PLAIN;
-        h->n_elements = m.n_elements;
-        h->n_nodes = m.n_nodes;
-        h->n_vertices = m.n_vertices;
-        h->n_dofs = 3 * m.n_nodes;
-
-        h->h_elem_nodes.assign(m.elem_nodes, m.elem_nodes + size_t(m.n_elements) * h->n_local);
-        h->h_vertices.assign(m.vertices, m.vertices + size_t(m.n_vertices) * 3);
-        h->h_elem_vertices.assign(m.elem_vertices, m.elem_vertices + size_t(m.n_elements) * 4);
-        h->pat = build_pattern(m.n_elements, h->n_local, m.elem_nodes, m.n_nodes, m.n_vertices, m.elem_vertices);
-
-        h->vconn = upload_soa<int32_t, int32_t>(m.elem_vertices, m.n_elements, 4, h->stream);
-        upload_geometry(*h, m);
-
-        if (material->kind == PFGPU_MOONEY_RIVLIN) {
-            const bool given = material->c1 != nullptr;
-            require(given == (material->c2 != nullptr) && given == (material->k != nullptr),
-                    "c1, c2 and k must all be given or all be NULL");
-            std::vector<double> u1, u2, u3;
-            const double *rows[3] = {material->c1, material->c2, material->k};
-            if (!given) {
-                u1.assign(size_t(m.n_elements), material->c1_uniform);
-                u2.assign(size_t(m.n_elements), material->c2_uniform);
-                u3.assign(size_t(m.n_elements), material->k_uniform);
-                rows[0] = u1.data(), rows[1] = u2.data(), rows[2] = u3.data();
-            }
-            upload_params(*h, rows, 3);
-        } else if (material->lambda) {
-            upload_lame(*h, material->lambda, material->mu);
-        } else {
-            const double E = material->young, nu = material->poisson;
-            require(std::isfinite(E) && E > 0 && nu > -1.0 && nu < 0.5, "invalid Young's modulus / Poisson ratio");
-            // LameParameters (MatParams.cpp:11-22), 3D
-            const double lam = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu)), mu = E / (2.0 * (1.0 + nu));
-            std::vector<double> la(m.n_elements, lam), mv(m.n_elements, mu);
-            upload_lame(*h, la.data(), mv.data());
-        }
-        h->conn = upload_soa<int32_t, int32_t>(m.elem_nodes, m.n_elements, h->n_local, h->stream);
-        h->node_ptr = upload(h->pat.node_ptr, h->stream);
-        h->node_inc = upload(h->pat.node_inc, h->stream);
-        h->vert_ptr = upload(h->pat.vert_ptr, h->stream);
-        h->vert_inc = upload(h->pat.vert_inc, h->stream);
-        // fixed-size scratch of the reductions, allocated here so that *_dev
-        // calls enqueue no allocation (they may be captured in a graph)
-        h->sum_part.reserve(sizeof(double) * 128);
-        h->red_out.reserve(sizeof(int) * 4);
-        *out = h.release();
+        *out = create_handle(mesh, material, precision).release();
     });
 }
 
@@ -863,6 +887,150 @@ pfgpu_status pfgpu_newton_solve_batch(pfgpu_handle *h, double *x, const uint8_t 
         const uint8_t *dm = stage_mask(*h, fixed);
         newton_solve_batch(*h, dx, dm, n_problems, elem_offsets, node_offsets, *params, reports);
         fetch(*h, x, dx, sizeof(double) * size_t(h->n_dofs));
+    });
+}
+
+pfgpu_status pfgpu_batch_create(int32_t n_problems, const pfgpu_mesh_desc *meshes, const pfgpu_material_desc *materials,
+                                int32_t precision, pfgpu_handle **out)
+{
+    return guard([&] {
+        require(out != nullptr, "null output handle pointer");
+        *out = nullptr;
+        require(n_problems >= 1 && meshes && materials, "empty batch or null descriptors");
+        require(precision == PFGPU_FP64 || precision == PFGPU_MIXED, "unknown precision");
+        // group of each problem: (order, material) with the default quadrature
+        std::vector<int> var(static_cast<size_t>(n_problems) + 0);
+        for (int i = 0; i < n_problems; ++i) {
+            const pfgpu_mesh_desc &m = meshes[i];
+            const pfgpu_material_desc &mt = materials[i];
+            require(m.order == 1 || m.order == 2, "order must be 1 or 2");
+            require(m.quadrature_order <= 0 || m.quadrature_order == (m.order == 1 ? 1 : 2),
+                    "batched problems use the default quadrature");
+            require(mt.kind == PFGPU_NEOHOOKEAN || mt.kind == PFGPU_LINEAR_ELASTICITY,
+                    "batched problems are NeoHookean or linear elasticity");
+            require(m.n_elements > 0 && m.n_nodes > 0 && m.elem_nodes, "empty problem in the batch");
+            var[size_t(i)] = (m.order == 2 ? 2 : 0) + (mt.kind == PFGPU_LINEAR_ELASTICITY ? 1 : 0);
+        }
+        auto H = std::make_unique<pfgpu_handle>();
+        PF_HIP(hipGetDevice(&H->device));
+        H->precision = precision;
+        H->path = PFGPU_PATH_PLAIN;
+        H->solver_hessian = PFGPU_SOLVER_HESSIAN_MATRIX_FREE;
+        int64_t N = 0, E = 0, NV = 0;
+        H->batch_eo.push_back(0);
+        H->batch_no.push_back(0);
+        for (int v = 0; v < 4; ++v) {
+            std::vector<int> idx;
+            for (int i = 0; i < n_problems; ++i)
+                if (var[size_t(i)] == v) idx.push_back(i);
+            if (idx.empty()) continue;
+            const int order = v >= 2 ? 2 : 1, NB = order == 1 ? 4 : 10;
+            std::vector<int32_t> en, ev;
+            std::vector<double> vx, lam, mu;
+            int64_t nn = 0, nv = 0, ne = 0;
+            for (int i : idx) {
+                const pfgpu_mesh_desc &m = meshes[i];
+                const pfgpu_material_desc &mt = materials[i];
+                require(m.vertices && m.elem_vertices && m.n_vertices > 0, "null mesh array");
+                for (int64_t k = 0; k < int64_t(m.n_elements) * NB; ++k) {
+                    const int32_t nd = m.elem_nodes[k];
+                    require(nd >= 0 && nd < m.n_nodes, "elem_nodes references a node out of range");
+                    en.push_back(int32_t(nd + nn));
+                }
+                for (int64_t k = 0; k < int64_t(m.n_elements) * 4; ++k) {
+                    const int32_t vv = m.elem_vertices[k];
+                    require(vv >= 0 && vv < m.n_vertices, "elem_vertices references a vertex out of range");
+                    ev.push_back(int32_t(vv + nv));
+                }
+                vx.insert(vx.end(), m.vertices, m.vertices + size_t(m.n_vertices) * 3);
+                require((mt.lambda == nullptr) == (mt.mu == nullptr), "lambda and mu must both be given or both be NULL");
+                for (int e = 0; e < m.n_elements; ++e) {
+                    if (mt.lambda) {
+                        lam.push_back(mt.lambda[e]);
+                        mu.push_back(mt.mu[e]);
+                    } else {
+                        const double Y = mt.young, nu = mt.poisson;
+                        require(std::isfinite(Y) && Y > 0 && nu > -1.0 && nu < 0.5, "invalid Young's modulus / Poisson ratio");
+                        lam.push_back(Y * nu / ((1.0 + nu) * (1.0 - 2.0 * nu)));
+                        mu.push_back(Y / (2.0 * (1.0 + nu)));
+                    }
+                }
+                H->batch_perm.push_back(i);
+                H->batch_eo.push_back(int32_t(E + ne + m.n_elements));
+                H->batch_no.push_back(int32_t(N + nn + m.n_nodes));
+                nn += m.n_nodes;
+                nv += m.n_vertices;
+                ne += m.n_elements;
+            }
+            require(N + nn < (int64_t(1) << 31) / 3 && E + ne < (int64_t(1) << 31), "batch too large for 32-bit indices");
+            pfgpu_mesh_desc gm{order, 0, int32_t(ne), int32_t(nn), en.data(), int32_t(nv), vx.data(), ev.data()};
+            pfgpu_material_desc gmat{};
+            gmat.kind = v % 2 == 1 ? PFGPU_LINEAR_ELASTICITY : PFGPU_NEOHOOKEAN;
+            gmat.lambda = lam.data();
+            gmat.mu = mu.data();
+            std::unique_ptr<pfgpu_handle> sub = create_handle(&gm, &gmat, precision);
+            sub->path = PFGPU_PATH_PLAIN;
+            // the group's block pattern, shifted into the batch's node space
+            Pattern &P = H->pat;
+            if (P.brow_ptr.empty()) P.brow_ptr.push_back(0);
+            const int32_t b0 = P.brow_ptr.back();
+            for (int i = 0; i < sub->n_nodes; ++i) P.brow_ptr.push_back(b0 + sub->pat.brow_ptr[size_t(i) + 1]);
+            for (int32_t c : sub->pat.bcol) P.bcol.push_back(int32_t(c + N));
+            P.n_blocks += sub->pat.n_blocks;
+            P.nnz += sub->pat.nnz;
+            N += nn;
+            E += ne;
+            NV += nv;
+            H->groups.push_back(std::move(sub));
+        }
+        H->pat.n_nodes = int(N);
+        H->n_nodes = int(N);
+        H->n_dofs = int(3 * N);
+        H->n_elements = int(E);
+        H->n_vertices = int(NV);
+        H->order = H->groups.front()->order;
+        H->n_local = H->groups.front()->n_local;
+        H->n_quad = H->groups.front()->n_quad;
+        H->material = H->groups.front()->material;
+        H->sum_part.reserve(sizeof(double) * 128);
+        H->red_out.reserve(sizeof(int) * 4);
+        *out = H.release();
+    });
+}
+
+pfgpu_status pfgpu_batch_newton(pfgpu_handle *h, double *const *x, const uint8_t *const *fixed,
+                                const pfgpu_newton_params *params, pfgpu_newton_report *reports)
+{
+    return guard([&] {
+        check_handle(h);
+        require(!h->groups.empty(), "not a batch handle (pfgpu_batch_create)");
+        require(x && reports, "null argument");
+        check_newton_params(params);
+        const int n = int(h->batch_perm.size());
+        // the problems' vectors in the batch's order (groups back to back)
+        std::vector<double> xg(size_t(h->n_dofs));
+        std::vector<uint8_t> mg(size_t(h->n_dofs), 0);
+        for (int k = 0; k < n; ++k) {
+            const int i = h->batch_perm[size_t(k)];
+            const size_t o = 3 * size_t(h->batch_no[size_t(k)]), len = 3 * size_t(h->batch_no[size_t(k) + 1] - h->batch_no[size_t(k)]);
+            require(x[i] != nullptr, "null problem vector");
+            std::copy(x[i], x[i] + len, xg.begin() + long(o));
+            if (fixed && fixed[i]) std::copy(fixed[i], fixed[i] + len, mg.begin() + long(o));
+        }
+        SolverStream ss(*h);
+        h->h_out2.reserve(sizeof(double) * size_t(h->n_dofs));
+        double *dx = h->h_out2.as<double>();
+        PF_HIP(hipMemcpyAsync(dx, xg.data(), sizeof(double) * size_t(h->n_dofs), hipMemcpyHostToDevice, h->stream));
+        const uint8_t *dm = stage_mask(*h, mg.data());
+        std::vector<pfgpu_newton_report> reps(static_cast<size_t>(n) + 0);
+        newton_solve_batch(*h, dx, dm, n, h->batch_eo.data(), h->batch_no.data(), *params, reps.data());
+        fetch(*h, xg.data(), dx, sizeof(double) * size_t(h->n_dofs));
+        for (int k = 0; k < n; ++k) {
+            const int i = h->batch_perm[size_t(k)];
+            const size_t o = 3 * size_t(h->batch_no[size_t(k)]), len = 3 * size_t(h->batch_no[size_t(k) + 1] - h->batch_no[size_t(k)]);
+            std::copy(xg.begin() + long(o), xg.begin() + long(o + len), x[i]);
+            reports[i] = reps[size_t(k)];
+        }
     });
 }
 

@@ -85,10 +85,11 @@ struct GatherPlan {
 };
 
 // Row-unit plan of the factored gather for workgroups of `threads` threads:
-// chunks of whole node rows (first row of each chunk, n_chunks + 1 entries),
-// per unit (node incidence, in node_inc order) the stage positions of its
-// n_local blocks, per pattern block the first stage position of its
-// contributions within its chunk.
+// chunks of whole node rows (per chunk boundary its first row, first unit
+// and first pattern block, 4 ints, n_chunks + 1 entries), per unit (node
+// incidence, in node_inc order) the stage positions of its n_local blocks,
+// per pattern block the first stage position of its contributions within
+// its chunk.
 struct UnitPlan {
     DevBuf chunk_row, udst, bq0;
     int n_chunks = 0, threads = 0;

@@ -514,6 +514,8 @@ void ensure_fused_tables(Handle &h)
     h.fused_max_rows = t.max_rows;
     h.fused_chunks = t.n_chunks;
     h.fused_visits = t.visits;
+    // one energy partial per chunk: sized with the tables, not by a later call
+    h.partial.reserve(sizeof(double) * size_t(h.fused_chunks));
     h.fused_ready = true;
 }
 

@@ -2,6 +2,7 @@
 #pragma once
 
 #include <array>
+#include <memory>
 
 #include "common.hpp"
 
@@ -81,6 +82,12 @@ struct pfgpu_handle {
     pfgpu::SegSkip skip; // set only inside a batched CG
     pfgpu::DevBuf solver_vals, pinv, diag_pos, node_diag;
     hipStream_t own_stream = nullptr; // used when the caller's stream is the legacy default
+    // mixed batch (pfgpu_batch_create): one sub-handle per (order, material)
+    // group; this handle's node space concatenates theirs, its element kernels
+    // run over all groups in one launch (batch.hip).  Empty otherwise.
+    std::vector<std::unique_ptr<pfgpu_handle>> groups;
+    std::vector<int32_t> batch_perm;     // caller's problem index of each problem, in group order
+    std::vector<int32_t> batch_eo, batch_no; // element / node offsets of the problems, group order
     ~pfgpu_handle();
 
     size_t value_size() const { return precision == PFGPU_MIXED ? sizeof(float) : sizeof(double); }
@@ -122,6 +129,13 @@ void ensure_plain_hessian_maps(Handle &h);
 void factored_energy_gradient_hessian(Handle &h, const double *x, double *energy, double *g, void *values);
 void factored_node_diagonal(Handle &h, const double *x, void *diag);
 
+// ---- mixed batches (batch.hip) -----------------------------------------
+void mixed_energy(Handle &h, const double *x, double *energy, double *per_element);
+void mixed_gradient(Handle &h, const double *x, double *g, double *energy);
+void mixed_hvp(Handle &h, const double *x, const double *v, double *y);
+void mixed_node_diagonal(Handle &h, const double *x, void *diag);
+void ensure_row_chunks(Handle &h);
+
 // ---- fused path (fused.hip) -------------------------------------------
 void ensure_fused_tables(Handle &h);
 void fused_gradient(Handle &h, const double *x, double *g, double *energy);
@@ -144,8 +158,9 @@ void transient_forward(Handle &h, const double *x0, const double *v0, const uint
                        double dt, int steps, int every, const pfgpu_newton_params &np, double *xT, double *vT);
 void transient_adjoint(Handle &h, const double *dJdx, const double *dJdv, int cg_max, double cg_tol, double *dtheta,
                        double *dx0, double *dv0);
+// exit_reason (optional): PFGPU_CG_TOLERANCE, PFGPU_CG_CURVATURE or PFGPU_CG_ITERATION_CAP
 int cg_solve_ws(Handle &h, const double *x, const uint8_t *fixed, const double *b, int max_it, double tol, bool use_graph,
-                double *rel);
+                double *rel, int *exit_reason = nullptr);
 double *cg_solution(Handle &h);
 void solver_release(Handle &h);
 

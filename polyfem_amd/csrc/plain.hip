@@ -983,24 +983,31 @@ __global__ __launch_bounds__(GS) void k_gather_factored(int n_chunks, const int3
 // Row-unit gather of the factored route.  A unit is one node incidence
 // (row node i = local node a of element e); the units of a node row are its
 // incident elements in element order (node_inc).  A chunk is a run of whole
-// rows holding at most GS units; thread t of the chunk's workgroup takes unit
-// t: it reads the element's factor record once, forms the n_local blocks
-// H_e(a, b) of its row, b = 0 .. n_local - 1 (b is unrolled, so the
-// reference gradients RG(q, b) are compile-time constants and their zeros
-// cost nothing), and stages each block at its position udst (the blocks of
-// the chunk in pattern order, the contributions of a block in unit order).
-// One thread per pattern block then sums its contributions into the CSR
-// values (3 runs of 3 values per block, consecutive blocks of a row
+// rows holding at most GS units and GS rows; thread t of the workgroup takes
+// unit t of the chunk: it reads the element's factor record once, forms the
+// n_local blocks H_e(a, b) of its row, b = 0 .. n_local - 1 (b is unrolled,
+// so the reference gradients RG(q, b) are compile-time constants and their
+// zeros cost nothing), and stages each block at its position udst (the
+// blocks of the chunk in pattern order, the contributions of a block in unit
+// order).  One thread per pattern block then sums its contributions into the
+// CSR values (3 runs of 3 values per block, consecutive blocks of a row
 // contiguous), and one thread per row sums the local gradient rows of its
 // units in FP64 when gloc is given.  Summation orders are fixed by the
-// tables: results are bitwise reproducible.  Per unit the record is read
-// once (the per-contribution form reads it n_local times) and the k vectors
-// of the row node are formed once.
+// tables: results are bitwise reproducible and independent of GS.
 //
 //   H_e(a, b) = (sum_q RG(q,a)^T G RG(q,b)) I + sum_q [cl_q k_a k_b^T + cc_q k_b k_a^T],
 //   k_a = M_q RG(q, a),  G = w mu J^{-1} J^{-T} (the record's metric tail)
+//
+// The staged blocks (GS n_local 9 values) bound the workgroups per CU to a
+// few, so the kernel is persistent and software-pipelined: the workgroups of
+// XCD x (blockIdx mod 8) walk that XCD's contiguous eighth of the chunks with
+// stride gridDim / 8; while chunk k is formed and summed, the records, stage
+// positions, local gradients and row / block tables of chunk k + 1 are in
+// flight, and the unit codes of chunk k + 2 (on which the record addresses
+// depend).  ctab[c] = (first row, first unit, first pattern block) of chunk c,
+// ctab[n_chunks] the totals.  gridDim must be a multiple of 8.
 template <typename T, int ORDER, int NQ, int GS>
-__global__ __launch_bounds__(GS) void k_gather_units(int n_chunks, const int32_t *__restrict__ chunk_row,
+__global__ __launch_bounds__(GS) void k_gather_units(int n_chunks, const int4 *__restrict__ ctab,
                                                      const int32_t *__restrict__ node_ptr,
                                                      const int32_t *__restrict__ node_inc,
                                                      const int32_t *__restrict__ brow_ptr,
@@ -1009,155 +1016,216 @@ __global__ __launch_bounds__(GS) void k_gather_units(int n_chunks, const int32_t
                                                      T *__restrict__ values, double *__restrict__ grad)
 {
     constexpr int NB = NLocal<ORDER>::value, R = fac_size(NQ), DW = NB / 2;
+    // 32-bit words of block starts per thread: a chunk has at most GS NB blocks
+    constexpr int NWMAX = GS * NB / 2 + 2, QW = (NWMAX + GS - 1) / GS;
     static_assert(NB % 2 == 0, "stage positions are read as 32-bit pairs");
     __shared__ T srg[NQ * NB * 3];
     __shared__ int32_t srb[GS + 1], sru[GS + 1];
+    __shared__ uint32_t sq[NWMAX];
+    __shared__ uint8_t srow[GS * NB];
     extern __shared__ __align__(16) unsigned char smem[];
     T *sv = reinterpret_cast<T *>(smem); // GS NB staged blocks of 9 values
     T *sg = sv + GS * NB * 9;            // GS local gradient rows
-    const int c = xcd_chunk(n_chunks);
-    if (c >= n_chunks) return;
+    const int per = (n_chunks + N_XCD - 1) / N_XCD, xcd = int(blockIdx.x % N_XCD);
+    const int stride = int(gridDim.x / N_XCD), c_end = min(n_chunks, (xcd + 1) * per);
+    int c = xcd * per + int(blockIdx.x / N_XCD);
+    if (c >= c_end) return; // uniform over the workgroup
     const int t = threadIdx.x;
-    const int i0 = chunk_row[c], i1 = chunk_row[c + 1], nr = i1 - i0;
-    const int u0 = node_ptr[i0], nu = node_ptr[i1] - u0;
-    const int B0 = brow_ptr[i0], nb = brow_ptr[i1] - B0;
-    // this thread's unit: the element's record, the stage positions of its
-    // blocks and its local gradient row
-    const bool has = t < nu;
-    int a = 0;
-    T r[R];
-    uint32_t dw[DW];
-    T l0 = T(0), l1 = T(0), l2 = T(0);
-    if (has) {
-        const int code = node_inc[u0 + t];
-        const int e = code / NB;
-        a = code - e * NB;
-        const T *rp = static_cast<const T *>(__builtin_assume_aligned(rec, 16)) + int64_t(e) * R;
-#pragma unroll
-        for (int i = 0; i < R; ++i) r[i] = rp[i];
-        const uint32_t *dp = reinterpret_cast<const uint32_t *>(udst + int64_t(u0 + t) * NB);
-#pragma unroll
-        for (int j = 0; j < DW; ++j) dw[j] = dp[j];
-        if (gloc) {
-            const T *gp = gloc + int64_t(code) * 3;
-            l0 = gp[0];
-            l1 = gp[1];
-            l2 = gp[2];
-        }
-    }
-    for (int k = t; k <= nr; k += GS) {
-        srb[k] = brow_ptr[i0 + k] - B0;
-        sru[k] = node_ptr[i0 + k] - u0;
-    }
     const RefTab<T, ORDER, NQ> &tb = ref_tab<T, ORDER, NQ>();
     for (int k = t; k < NQ * NB * 3; k += GS) srg[k] = tb.rg[k];
     __syncthreads();
-    if (has) {
-        if (gloc) {
-            sg[t * 3] = l0;
-            sg[t * 3 + 1] = l1;
-            sg[t * 3 + 2] = l2;
-        }
-        // per point: k_a, cl k_a and G RG(q, a)
-        const T *G6 = r + 11 * NQ; // 00 11 22 01 02 12
-        T ka[NQ][3], la[NQ][3], ha[NQ][3];
+
+    struct Meta {
+        int i0, nr, u0, nu, B0, nb;
+    };
+    auto meta = [&](int cc) {
+        const int4 a = ctab[cc], b = ctab[cc + 1];
+        return Meta{a.x, b.x - a.x, a.y, b.y - a.y, a.z, b.z - a.z};
+    };
+    auto code_of = [&](const Meta &m) { return t < m.nu ? node_inc[m.u0 + t] : -1; };
+    // everything else a chunk needs, in registers
+    struct Unit {
+        T r[R];
+        uint32_t dw[DW];
+        T l0, l1, l2;
+        int32_t rb0, rb1, ru;
+        uint32_t qw[QW];
+    };
+    auto load_unit = [&](const Meta &m, int code, Unit &u) {
+        if (code >= 0) {
+            const int e = code / NB;
+            const T *rp = static_cast<const T *>(__builtin_assume_aligned(rec, 16)) + int64_t(e) * R;
 #pragma unroll
-        for (int q = 0; q < NQ; ++q) {
-            const T *M = r + 9 * q, *ra = srg + (q * NB + a) * 3;
-            const T ra0 = ra[0], ra1 = ra[1], ra2 = ra[2];
+            for (int i = 0; i < R; ++i) u.r[i] = rp[i];
+            const uint32_t *dp = reinterpret_cast<const uint32_t *>(udst + int64_t(m.u0 + t) * NB);
 #pragma unroll
-            for (int mm = 0; mm < 3; ++mm) {
-                ka[q][mm] = fma_t(M[mm * 3 + 2], ra2, fma_t(M[mm * 3 + 1], ra1, M[mm * 3] * ra0));
-                la[q][mm] = r[9 * NQ + q] * ka[q][mm];
+            for (int j = 0; j < DW; ++j) u.dw[j] = dp[j];
+            if (gloc) {
+                const T *gp = gloc + int64_t(code) * 3;
+                u.l0 = gp[0];
+                u.l1 = gp[1];
+                u.l2 = gp[2];
             }
-            ha[q][0] = fma_t(G6[4], ra2, fma_t(G6[3], ra1, G6[0] * ra0));
-            ha[q][1] = fma_t(G6[5], ra2, fma_t(G6[1], ra1, G6[3] * ra0));
-            ha[q][2] = fma_t(G6[2], ra2, fma_t(G6[5], ra1, G6[4] * ra0));
         }
+        if (t <= m.nr) {
+            u.rb0 = brow_ptr[m.i0 + t];
+            u.ru = node_ptr[m.i0 + t];
+            u.rb1 = t < m.nr ? brow_ptr[m.i0 + t + 1] : u.rb0;
+        }
+        // block starts: bq0[B0 .. B0 + nb - 1] as 32-bit words from an even index
+        const uint32_t *qp = reinterpret_cast<const uint32_t *>(bq0 + (m.B0 & ~1));
+        const int nw = (m.nb + (m.B0 & 1) + 1) / 2;
 #pragma unroll
-        for (int b = 0; b < NB; ++b) {
-            T gd = T(0);
-#pragma unroll
-            for (int q = 0; q < NQ; ++q)
-#pragma unroll
-                for (int l = 0; l < 3; ++l) {
-                    const double rb = RG<ORDER, NQ>(q, b, l);
-                    if (rb != 0.0) gd = fma_t(ha[q][l], T(rb), gd);
-                }
-            T v[9];
-#pragma unroll
-            for (int i = 0; i < 9; ++i) v[i] = i % 4 == 0 ? gd : T(0);
+        for (int j = 0; j < QW; ++j)
+            if (t + j * GS < nw) u.qw[j] = qp[t + j * GS];
+    };
+
+    Meta m0 = meta(c);
+    Unit u0;
+    int code0 = code_of(m0);
+    load_unit(m0, code0, u0);
+    int c1 = c + stride;
+    Meta m1{};
+    int code1 = -1;
+    if (c1 < c_end) {
+        m1 = meta(c1);
+        code1 = code_of(m1);
+    }
+    while (true) { // c, c1 and the chunk metadata are uniform over the workgroup
+        // in flight while chunk c is processed: chunk c1's data, chunk c2's codes
+        Unit u1;
+        if (c1 < c_end) load_unit(m1, code1, u1);
+        const int c2 = c1 + stride;
+        Meta m2{};
+        int code2 = -1;
+        if (c2 < c_end) {
+            m2 = meta(c2);
+            code2 = code_of(m2);
+        }
+        // ---- chunk c: blocks of this thread's unit into the stage
+        if (code0 >= 0) {
+            const int a = code0 - (code0 / NB) * NB;
+            const T *r = u0.r;
+            if (gloc) {
+                sg[t * 3] = u0.l0;
+                sg[t * 3 + 1] = u0.l1;
+                sg[t * 3 + 2] = u0.l2;
+            }
+            const T *G6 = r + 11 * NQ; // 00 11 22 01 02 12
+            T ka[NQ][3], la[NQ][3], ha[NQ][3];
 #pragma unroll
             for (int q = 0; q < NQ; ++q) {
-                const T *M = r + 9 * q;
-                T kb[3];
+                const T *M = r + 9 * q, *ra = srg + (q * NB + a) * 3;
+                const T ra0 = ra[0], ra1 = ra[1], ra2 = ra[2];
 #pragma unroll
                 for (int mm = 0; mm < 3; ++mm) {
-                    T s = T(0);
-                    bool first = true;
+                    ka[q][mm] = fma_t(M[mm * 3 + 2], ra2, fma_t(M[mm * 3 + 1], ra1, M[mm * 3] * ra0));
+                    la[q][mm] = r[9 * NQ + q] * ka[q][mm];
+                }
+                ha[q][0] = fma_t(G6[4], ra2, fma_t(G6[3], ra1, G6[0] * ra0));
+                ha[q][1] = fma_t(G6[5], ra2, fma_t(G6[1], ra1, G6[3] * ra0));
+                ha[q][2] = fma_t(G6[2], ra2, fma_t(G6[5], ra1, G6[4] * ra0));
+            }
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+                T gd = T(0);
+#pragma unroll
+                for (int q = 0; q < NQ; ++q)
 #pragma unroll
                     for (int l = 0; l < 3; ++l) {
                         const double rb = RG<ORDER, NQ>(q, b, l);
-                        if (rb != 0.0) {
-                            s = first ? M[mm * 3 + l] * T(rb) : fma_t(M[mm * 3 + l], T(rb), s);
-                            first = false;
-                        }
+                        if (rb != 0.0) gd = fma_t(ha[q][l], T(rb), gd);
                     }
-                    kb[mm] = s;
+                T v[9];
+#pragma unroll
+                for (int i = 0; i < 9; ++i) v[i] = i % 4 == 0 ? gd : T(0);
+#pragma unroll
+                for (int q = 0; q < NQ; ++q) {
+                    const T *M = r + 9 * q;
+                    T kb[3];
+#pragma unroll
+                    for (int mm = 0; mm < 3; ++mm) {
+                        T s = T(0);
+                        bool first = true;
+#pragma unroll
+                        for (int l = 0; l < 3; ++l) {
+                            const double rb = RG<ORDER, NQ>(q, b, l);
+                            if (rb != 0.0) {
+                                s = first ? M[mm * 3 + l] * T(rb) : fma_t(M[mm * 3 + l], T(rb), s);
+                                first = false;
+                            }
+                        }
+                        kb[mm] = s;
+                    }
+                    const T cc = r[10 * NQ + q];
+                    const T lb[3] = {cc * kb[0], cc * kb[1], cc * kb[2]};
+#pragma unroll
+                    for (int mm = 0; mm < 3; ++mm)
+#pragma unroll
+                        for (int nn = 0; nn < 3; ++nn)
+                            v[mm * 3 + nn] = fma_t(lb[mm], ka[q][nn], fma_t(la[q][mm], kb[nn], v[mm * 3 + nn]));
                 }
-                const T cc = r[10 * NQ + q];
-                const T lb[3] = {cc * kb[0], cc * kb[1], cc * kb[2]};
+                const int d = int((u0.dw[b >> 1] >> (16 * (b & 1))) & 0xffffu);
+                T *o = sv + d * 9;
 #pragma unroll
-                for (int mm = 0; mm < 3; ++mm)
-#pragma unroll
-                    for (int nn = 0; nn < 3; ++nn)
-                        v[mm * 3 + nn] = fma_t(lb[mm], ka[q][nn], fma_t(la[q][mm], kb[nn], v[mm * 3 + nn]));
+                for (int i = 0; i < 9; ++i) o[i] = v[i];
             }
-            const int d = int((dw[b >> 1] >> (16 * (b & 1))) & 0xffffu);
-            T *o = sv + d * 9;
-#pragma unroll
-            for (int i = 0; i < 9; ++i) o[i] = v[i];
         }
-    }
-    __syncthreads();
-    // one thread per pattern block: its contributions in unit order
-    for (int bb = t; bb < nb; bb += GS) {
-        int lo = 0, hi = nr - 1; // the row: srb[lo] <= bb < srb[lo + 1]
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (srb[mid] <= bb)
-                lo = mid;
-            else
-                hi = mid - 1;
+        // ---- chunk c: row and block tables
+        if (t <= m0.nr) {
+            srb[t] = u0.rb0 - m0.B0;
+            sru[t] = u0.ru - m0.u0;
         }
-        const int q0 = bq0[B0 + bb], q1 = bb + 1 < nb ? int(bq0[B0 + bb + 1]) : nu * NB;
-        T acc[9];
+        if (t < m0.nr)
+            for (int k = u0.rb0 - m0.B0; k < u0.rb1 - m0.B0; ++k) srow[k] = uint8_t(t);
+        {
+            const int nw = (m0.nb + (m0.B0 & 1) + 1) / 2;
 #pragma unroll
-        for (int i = 0; i < 9; ++i) acc[i] = T(0);
-        for (int k = q0; k < q1; ++k)
+            for (int j = 0; j < QW; ++j)
+                if (t + j * GS < nw) sq[t + j * GS] = u0.qw[j];
+        }
+        __syncthreads();
+        // one thread per pattern block: its contributions in unit order
+        const uint16_t *sq16 = reinterpret_cast<const uint16_t *>(sq) + (m0.B0 & 1);
+        for (int bb = t; bb < m0.nb; bb += GS) {
+            const int rr = srow[bb];
+            const int q0 = sq16[bb], q1 = bb + 1 < m0.nb ? int(sq16[bb + 1]) : m0.nu * NB;
+            T acc[9];
 #pragma unroll
-            for (int i = 0; i < 9; ++i) acc[i] += sv[k * 9 + i];
-        const int rb0 = srb[lo], deg = srb[lo + 1] - rb0;
-        T *out = values + 9 * int64_t(B0 + rb0) + 3 * (bb - rb0);
+            for (int i = 0; i < 9; ++i) acc[i] = T(0);
+            for (int k = q0; k < q1; ++k)
 #pragma unroll
-        for (int mm = 0; mm < 3; ++mm)
+                for (int i = 0; i < 9; ++i) acc[i] += sv[k * 9 + i];
+            const int rb0 = srb[rr], deg = srb[rr + 1] - rb0;
+            T *out = values + 9 * int64_t(m0.B0 + rb0) + 3 * (bb - rb0);
 #pragma unroll
-            for (int nn = 0; nn < 3; ++nn) out[mm * 3 * deg + nn] = acc[mm * 3 + nn];
-    }
-    // one thread per row: the gradient, in FP64 (rows without units get 0)
-    if (gloc)
-        for (int k = t; k < nr; k += GS) {
+            for (int mm = 0; mm < 3; ++mm)
+#pragma unroll
+                for (int nn = 0; nn < 3; ++nn) out[mm * 3 * deg + nn] = acc[mm * 3 + nn];
+        }
+        // one thread per row: the gradient, in FP64 (rows without units get 0)
+        if (gloc && t < m0.nr) {
             double s0 = 0, s1 = 0, s2 = 0;
-            for (int u = sru[k]; u < sru[k + 1]; ++u) {
+            for (int u = sru[t]; u < sru[t + 1]; ++u) {
                 s0 += double(sg[u * 3]);
                 s1 += double(sg[u * 3 + 1]);
                 s2 += double(sg[u * 3 + 2]);
             }
-            double *g = grad + 3 * int64_t(i0 + k);
+            double *g = grad + 3 * int64_t(m0.i0 + t);
             g[0] = s0;
             g[1] = s1;
             g[2] = s2;
         }
+        if (c1 >= c_end) break;
+        __syncthreads(); // the stage and tables are read before chunk c1 overwrites them
+        c = c1;
+        m0 = m1;
+        u0 = u1;
+        code0 = code1;
+        c1 = c2;
+        m1 = m2;
+        code1 = code2;
+    }
 }
 
 // Diagonal blocks of the Hessian per node from the factor records, over the
@@ -1251,6 +1319,10 @@ void energy_into(Handle &h, const double *x, double *energy, double *partial, do
 
 void plain_energy(Handle &h, const double *x, double *energy, double *per_element)
 {
+    if (!h.groups.empty()) {
+        mixed_energy(h, x, energy, per_element);
+        return;
+    }
     const unsigned g = grid_for(h.n_elements);
     h.partial.reserve(sizeof(double) * g);
     dispatch(h, [&]<typename T, int O, int Q, int M>() {
@@ -1262,6 +1334,10 @@ void plain_energy(Handle &h, const double *x, double *energy, double *per_elemen
 
 void plain_gradient(Handle &h, const double *x, double *g, double *energy)
 {
+    if (!h.groups.empty()) {
+        mixed_gradient(h, x, g, energy);
+        return;
+    }
     const unsigned grid = grid_for(h.n_elements);
     h.loc.reserve(h.value_size() * size_t(h.n_elements) * h.n_local * 3);
     if (energy) h.partial.reserve(sizeof(double) * grid);
@@ -1461,10 +1537,23 @@ bool build_unit_plan(Handle &h, UnitPlan &p, int gs)
     if (p.threads == gs) return true;
     UnitPlanHost u;
     if (!make_unit_plan(h.pat, h.n_local, h.h_elem_nodes.data(), gs, u)) return false;
-    p.chunk_row = upload(u.chunk_row, h.stream);
+    // per chunk boundary: first row, first unit, first pattern block
+    const int nc = int(u.chunk_row.size()) - 1;
+    std::vector<int32_t> ctab(size_t(nc + 1) * 4, 0);
+    for (int c = 0; c <= nc; ++c) {
+        const int32_t i = u.chunk_row[size_t(c)];
+        ctab[size_t(c) * 4] = i;
+        ctab[size_t(c) * 4 + 1] = h.pat.node_ptr[size_t(i)];
+        ctab[size_t(c) * 4 + 2] = h.pat.brow_ptr[size_t(i)];
+    }
+    p.chunk_row = upload(ctab, h.stream);
     p.udst = upload(u.udst, h.stream);
+    // two spare entries: the kernel reads the block starts as 32-bit words
+    // from an even index, up to one entry past a chunk's last block
+    u.bq0.push_back(0);
+    u.bq0.push_back(0);
     p.bq0 = upload(u.bq0, h.stream);
-    p.n_chunks = int(u.chunk_row.size()) - 1;
+    p.n_chunks = nc;
     p.threads = gs;
     p.grid = 0;
     return true;
@@ -1480,9 +1569,14 @@ template <typename T, int O, int Q> void launch_unit_gather(Handle &h, UnitPlan 
         const int lds = int(sizeof(T)) * GS * (NB * 9 + 3);
         if (p.grid == 0) {
             PF_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(kern), hipFuncAttributeMaxDynamicSharedMemorySize, lds));
-            p.grid = xcd_grid(p.n_chunks);
+            // persistent: the workgroups resident at once, a multiple of the 8 XCDs
+            if (h.n_cu == 0) PF_HIP(hipDeviceGetAttribute(&h.n_cu, hipDeviceAttributeMultiprocessorCount, h.device));
+            int per_cu = 0;
+            PF_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, GS, size_t(lds)));
+            const int g = std::max(1, h.n_cu) * std::max(1, per_cu);
+            p.grid = std::min(xcd_grid(p.n_chunks), unsigned((g + N_XCD - 1) / N_XCD * N_XCD));
         }
-        kern<<<p.grid, GS, lds, h.stream>>>(p.n_chunks, p.chunk_row.as<int32_t>(), h.node_ptr.as<int32_t>(),
+        kern<<<p.grid, GS, lds, h.stream>>>(p.n_chunks, p.chunk_row.as<int4>(), h.node_ptr.as<int32_t>(),
                                             h.node_inc.as<int32_t>(), h.brow_ptr.as<int32_t>(), p.udst.as<uint16_t>(),
                                             p.bq0.as<uint16_t>(), h.fac_rec.as<T>(), gloc, static_cast<T *>(values), g);
     };
@@ -1507,7 +1601,11 @@ bool unit_plan_ready(Handle &h)
     if (gs > 256) return false;
     // the staged blocks must fit the 160 KB of LDS (FP64 P2 at 256 threads
     // does not): the largest fitting size at or below the tuned one
-    auto lds = [&](int g) { return int64_t(h.value_size()) * g * (h.n_local * 9 + 3) + 8 * (g + 1) + 4096; };
+    // dynamic stage + the static tables (row tables, block starts, block rows, reference gradients)
+    auto lds = [&](int g) {
+        return int64_t(h.value_size()) * g * (h.n_local * 9 + 3) + 8 * (g + 1) + 4 * (g * h.n_local / 2 + 2) +
+               int64_t(g) * h.n_local + 1024;
+    };
     while (gs > 64 && lds(gs) > 160 * 1024) gs /= 2;
     if (lds(gs) > 160 * 1024) return false;
     return build_unit_plan(h, h.unit_plan, gs);
@@ -1631,6 +1729,10 @@ void plain_hessian(Handle &h, const double *x, void *values, bool project_to_psd
 
 void plain_hvp(Handle &h, const double *x, const double *v, double *y)
 {
+    if (!h.groups.empty()) {
+        mixed_hvp(h, x, v, y);
+        return;
+    }
     const unsigned grid = grid_for(h.n_elements, 64);
     h.loc.reserve(h.value_size() * size_t(h.n_elements) * h.n_local * 3);
     dispatch(h, [&]<typename T, int O, int Q, int M>() {

@@ -41,7 +41,10 @@ constexpr int CHECK = 8;         // CG iterations between host convergence check
 constexpr int SPMV_LANES = PF_SPMV_LANES; // lanes per block row of the block-sparse product
 
 // scalar slots of the CG workspace (r.z: preconditioned residual product)
-enum { S_RR, S_PQ, S_ALPHA, S_BETA, S_RZ, S_FLAG, S_B2, S_GD, S_E0, S_E1, S_M, S_F, S_COUNT };
+// S_FLAG: 0 while the CG runs, 1 stopped on non-positive curvature, 2 stopped
+// at the tolerance (r.r <= S_TOL); S_ITER counts the updates taken
+enum { S_RR, S_PQ, S_ALPHA, S_BETA, S_RZ, S_FLAG, S_B2, S_GD, S_E0, S_E1, S_M, S_F, S_TOL, S_ITER, S_COUNT };
+enum { CG_RUNNING = 0, CG_CURVATURE = 1, CG_TOLERANCE = 2 };
 
 __device__ double block_reduce(double v, double *sh)
 {
@@ -95,7 +98,7 @@ __global__ __launch_bounds__(BLOCK) void k_finish_dot(const double *__restrict__
         // alpha = r.z / p.Hp; a non-positive curvature stops the iteration
         const double rz = scal[S_RZ];
         const bool ok = v > 0.0 && rz > 0.0 && scal[S_FLAG] == 0.0;
-        if (!(v > 0.0) && rz > 0.0) scal[S_FLAG] = 1.0;
+        if (!(v > 0.0) && rz > 0.0 && scal[S_FLAG] == 0.0) scal[S_FLAG] = CG_CURVATURE;
         scal[S_ALPHA] = ok ? rz / v : 0.0;
     }
 }
@@ -204,7 +207,8 @@ __global__ __launch_bounds__(BLOCK) void k_finish_pcg(const double *__restrict__
     if (start) {
         scal[S_RZ] = rz;
         scal[S_RR] = rr;
-        scal[S_FLAG] = 0.0;
+        scal[S_FLAG] = CG_RUNNING;
+        scal[S_ITER] = 0.0;
         return;
     }
     const double old = scal[S_RZ];
@@ -213,6 +217,9 @@ __global__ __launch_bounds__(BLOCK) void k_finish_pcg(const double *__restrict__
     if (scal[S_FLAG] == 0.0) {
         scal[S_RZ] = rz;
         scal[S_RR] = rr;
+        if (live) scal[S_ITER] += 1.0;
+        // the iteration stops where it meets the tolerance
+        if (rr <= scal[S_TOL]) scal[S_FLAG] = CG_TOLERANCE;
     }
 }
 
@@ -469,6 +476,7 @@ bool inertia(const Handle &h) { return h.obj_xt != nullptr; }
 
 bool assembled_operator(const Handle &h)
 {
+    if (!h.groups.empty()) return false; // mixed batches: matrix-free
     return h.solver_hessian == PFGPU_SOLVER_HESSIAN_ASSEMBLED || h.solver_hessian == PFGPU_SOLVER_HESSIAN_ASSEMBLED_PSD ||
            (h.solver_hessian == PFGPU_SOLVER_HESSIAN_AUTO && h.order == 1);
 }
@@ -518,10 +526,14 @@ void prepare_operator(Handle &h, const double *x, const uint8_t *fixed)
     const bool diag_only = !h.op_assembled && h.op_precond && h.path == PFGPU_PATH_FACTORED &&
                            h.solver_hessian != PFGPU_SOLVER_HESSIAN_ASSEMBLED_PSD &&
                            (h.material == PFGPU_NEOHOOKEAN || h.material == PFGPU_LINEAR_ELASTICITY);
-    if (diag_only) {
+    const bool mixed_diag = !h.groups.empty() && h.op_precond;
+    if (diag_only || mixed_diag) {
         if (!h.brow_ptr.get()) h.brow_ptr = upload(h.pat.brow_ptr, h.stream);
         h.node_diag.reserve(h.value_size() * 9 * size_t(h.n_nodes));
-        factored_node_diagonal(h, x, h.node_diag.get());
+        if (mixed_diag)
+            mixed_node_diagonal(h, x, h.node_diag.get());
+        else
+            factored_node_diagonal(h, x, h.node_diag.get());
     } else if (h.op_assembled || h.op_precond) {
         if (!h.brow_ptr.get()) h.brow_ptr = upload(h.pat.brow_ptr, h.stream);
         if (!h.pat_bcol.get()) h.pat_bcol = upload(h.pat.bcol, h.stream);
@@ -561,7 +573,7 @@ void prepare_operator(Handle &h, const double *x, const uint8_t *fixed)
         auto jac = [&]<typename T>() {
             k_block_jacobi<T><<<grid_for(h.n_nodes), BLOCK, 0, h.stream>>>(
                 h.n_nodes, h.brow_ptr.as<int32_t>(), h.diag_pos.as<int32_t>(), h.solver_vals.as<T>(),
-                diag_only ? h.node_diag.as<T>() : nullptr, fold ? nullptr : mass, fold ? 1.0 : h.obj_w, fixed,
+                diag_only || mixed_diag ? h.node_diag.as<T>() : nullptr, fold ? nullptr : mass, fold ? 1.0 : h.obj_w, fixed,
                 h.pinv.as<float>());
         };
         if (h.precision == PFGPU_MIXED)
@@ -686,15 +698,19 @@ void solver_release(Handle &h)
 // CG on H(x)_ff y = b_f starting from y = 0; the solution is left in the
 // workspace's y.  Returns the iterations taken; *rel = ||r|| / ||b_f||.
 int cg_solve_ws(Handle &h, const double *x, const uint8_t *fixed, const double *b, int max_it, double tol,
-                bool use_graph, double *rel)
+                bool use_graph, double *rel, int *exit_reason)
 {
     const Work w = workspace(h);
     if (!h.op_valid) prepare_operator(h, x, fixed);
     cg_start(h, w, fixed, b);
     const double b2 = read_scalar(h, w, S_RR);
     if (rel) *rel = 0.0;
+    if (exit_reason) *exit_reason = CG_TOLERANCE;
     if (b2 == 0.0 || max_it <= 0) return 0;
     if (!std::isfinite(b2)) fail(PFGPU_ERR_INVALID_STATE, "non-finite right-hand side in the CG solve");
+    // the device stops the iteration where r.r falls to tol^2 b.b
+    k_set_scalar<<<1, 1, 0, h.stream>>>(w.scal, S_TOL, tol * tol * b2);
+    PF_LAUNCH_CHECK();
     // the graph replays one iteration; it is rebuilt when its arguments change
     // (scratch buffers are grow-only: a reallocation by another call rebuilds the graph)
     const std::array<const void *, 8> key{x,
@@ -728,10 +744,11 @@ int cg_solve_ws(Handle &h, const double *x, const uint8_t *fixed, const double *
         h.cg_graph_key = key;
         h.cg_graph_key[4] = h.loc.get(); // as sized by the eager pass
     }
-    int it = 0;
-    double rr = b2;
+    int it = 0, done = 0;
+    double rr = b2, flag = CG_RUNNING;
     while (it < max_it) {
         // the graph holds CHECK iterations; a shorter remainder runs eagerly
+        // (iterations after a stop change nothing)
         const int n = std::min(CHECK, max_it - it);
         if (use_graph && n == CHECK)
             PF_HIP(hipGraphLaunch(static_cast<hipGraphExec_t>(h.cg_graph), h.stream));
@@ -742,11 +759,14 @@ int cg_solve_ws(Handle &h, const double *x, const uint8_t *fixed, const double *
         PF_HIP(hipMemcpyAsync(sc, w.scal, sizeof(sc), hipMemcpyDeviceToHost, h.stream));
         PF_HIP(hipStreamSynchronize(h.stream));
         rr = sc[S_RR];
+        flag = sc[S_FLAG];
+        done = int(sc[S_ITER]);
         if (!std::isfinite(rr)) fail(PFGPU_ERR_INVALID_STATE, "non-finite residual in the CG solve (inverted element?)");
-        if (std::sqrt(rr) <= tol * std::sqrt(b2) || sc[S_FLAG] != 0.0) break;
+        if (flag != CG_RUNNING) break;
     }
     if (rel) *rel = std::sqrt(rr / b2);
-    return it;
+    if (exit_reason) *exit_reason = flag == CG_RUNNING ? PFGPU_CG_ITERATION_CAP : (flag == CG_CURVATURE ? PFGPU_CG_CURVATURE : PFGPU_CG_TOLERANCE);
+    return done;
 }
 
 double *cg_solution(Handle &h) { return workspace(h).y; }
@@ -772,7 +792,10 @@ void newton_solve(Handle &h, double *x, const uint8_t *fixed, const pfgpu_newton
         rep.energy = e0;
         rep.grad_norm = gn;
         if (!std::isfinite(gn) || !std::isfinite(e0)) fail(PFGPU_ERR_INVALID_STATE, "non-finite energy or gradient at the Newton iterate");
-        if (it == 0) gn0 = gn;
+        if (it == 0) {
+            gn0 = gn;
+            rep.initial_grad_norm = gn;
+        }
         if (gn <= prm.grad_norm || gn <= prm.relative_grad_norm * gn0) {
             rep.converged = 1;
             return;
@@ -781,8 +804,16 @@ void newton_solve(Handle &h, double *x, const uint8_t *fixed, const pfgpu_newton
         k_neg_free<<<gr, BLOCK, 0, h.stream>>>(w.d, w.g, fixed, h.n_dofs);
         PF_LAUNCH_CHECK();
         double rel = 0;
-        const int cg_it = cg_solve_ws(h, x, fixed, w.d, prm.cg_max_iterations, prm.cg_tolerance, prm.use_graph != 0, &rel);
+        int why = 0;
+        const int cg_it =
+            cg_solve_ws(h, x, fixed, w.d, prm.cg_max_iterations, prm.cg_tolerance, prm.use_graph != 0, &rel, &why);
         rep.cg_iterations += cg_it;
+        if (why == PFGPU_CG_CURVATURE)
+            rep.cg_stops_curvature++;
+        else if (why == PFGPU_CG_ITERATION_CAP)
+            rep.cg_stops_cap++;
+        else
+            rep.cg_stops_tolerance++;
         // direction in w.y; g.d for the Armijo condition
         dot(h, w, w.g, w.y, fixed, S_GD, AFTER_NONE);
         double gd = read_scalar(h, w, S_GD);
@@ -1545,7 +1576,10 @@ void newton_solve_batch(Handle &h, double *x, const uint8_t *fixed, int nseg, co
                 fail(PFGPU_ERR_INVALID_STATE, "non-finite energy or gradient at a batched Newton iterate");
             rep[sg].energy = e0[sg];
             rep[sg].grad_norm = gn[sg];
-            if (it == 0) gn0[sg] = gn[sg];
+            if (it == 0) {
+                gn0[sg] = gn[sg];
+                rep[sg].initial_grad_norm = gn[sg];
+            }
             if (!fr[sg] && (gn[sg] <= prm.grad_norm || gn[sg] <= prm.relative_grad_norm * gn0[sg])) {
                 fr[sg] = 1;
                 rep[sg].converged = 1;

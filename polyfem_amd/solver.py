@@ -66,6 +66,16 @@ class NewtonReport:
     grad_norm: float
     last_alpha: float
     gradient_steps: int
+    cg_stops_tolerance: int = 0
+    cg_stops_curvature: int = 0
+    cg_stops_cap: int = 0
+    initial_grad_norm: float = 0.0
+
+
+def _report(r) -> "NewtonReport":
+    return NewtonReport(r.iterations, r.cg_iterations, r.line_search_halvings, bool(r.converged), r.energy,
+                        r.grad_norm, r.last_alpha, r.gradient_steps, r.cg_stops_tolerance, r.cg_stops_curvature,
+                        r.cg_stops_cap, r.initial_grad_norm)
 
 
 class NewtonSolver:
@@ -90,8 +100,7 @@ class NewtonSolver:
         rep = _ffi.NewtonReport()
         check(lib().pfgpu_newton_solve(assembler._h, xx.ctypes.data_as(C.c_void_p),
                                        problem.mask.ctypes.data_as(C.c_void_p), C.byref(self.params), C.byref(rep)))
-        return xx, NewtonReport(rep.iterations, rep.cg_iterations, rep.line_search_halvings, bool(rep.converged),
-                                rep.energy, rep.grad_norm, rep.last_alpha, rep.gradient_steps)
+        return xx, _report(rep)
 
 
 def cg_solve(assembler, x, rhs, fixed_mask=None, max_iterations: int = 1000, tolerance: float = 1e-8):
@@ -212,50 +221,61 @@ def concatenate_meshes(meshes):
 
 class BatchSolver:
     """Independent static problems solved together (many meshes of uneven
-    size; one handle, one launch per kernel, per group of equal order and
-    material).  ``problems`` is a list of dicts with keys ``mesh``,
-    ``material`` ("NeoHookean" | "LinearElasticity"), ``lame`` (lambda, mu),
+    size, P1 and P2, NeoHookean and linear elasticity, in one batch handle:
+    every element kernel and node gather is one launch over all problems,
+    pfgpu_batch_create / pfgpu_batch_newton).  ``problems`` is a list of
+    dicts with keys ``mesh``, ``material`` ("NeoHookean" |
+    "LinearElasticity"), ``lame`` (lambda, mu; scalars or per element),
     ``fixed_dofs``, ``fixed_values`` and optionally ``x0``."""
 
     def __init__(self, problems, precision="mixed", newton: NewtonSolver = None):
-        from .assembler import NLAssembler
+        from .assembler import _NAMES, _PRECISIONS
         self.problems = problems
         self.newton = newton or NewtonSolver(max_iterations=20, grad_norm=1e-8)
-        groups = {}
+        n = len(problems)
+        self._keep = []
+        mds = (_ffi.MeshDesc * n)()
+        mats = (_ffi.MaterialDesc * n)()
+        self.nl = []
         for i, p in enumerate(problems):
-            groups.setdefault((p["mesh"].order, p["material"]), []).append(i)
-        self.groups = []
-        for (order, material), idx in sorted(groups.items()):
-            big, eo, no = concatenate_meshes([problems[i]["mesh"] for i in idx])
-            lam = np.concatenate([np.broadcast_to(np.asarray(problems[i]["lame"][0], dtype=np.float64),
-                                                  (problems[i]["mesh"].n_elements,)) for i in idx])
-            mu = np.concatenate([np.broadcast_to(np.asarray(problems[i]["lame"][1], dtype=np.float64),
-                                                 (problems[i]["mesh"].n_elements,)) for i in idx])
-            a = NLAssembler(material, big, lame=(lam, mu), precision=precision)
-            fixed = np.concatenate([np.asarray(problems[i]["fixed_dofs"], dtype=np.int64) + 3 * no[k]
-                                    for k, i in enumerate(idx)])
-            vals = np.concatenate([np.asarray(problems[i]["fixed_values"], dtype=np.float64) for i in idx])
-            self.groups.append(dict(idx=idx, a=a, eo=eo, no=no, problem=NLProblem(big.n_dofs, fixed, vals)))
+            m = p["mesh"]
+            E = m.n_elements
+            en = np.ascontiguousarray(m.elem_nodes, dtype=np.int32)
+            vx = np.ascontiguousarray(m.vertices, dtype=np.float64)
+            ev = np.ascontiguousarray(m.elem_vertices, dtype=np.int32)
+            lam = np.ascontiguousarray(np.broadcast_to(np.asarray(p["lame"][0], dtype=np.float64), (E,)))
+            mu = np.ascontiguousarray(np.broadcast_to(np.asarray(p["lame"][1], dtype=np.float64), (E,)))
+            self._keep += [en, vx, ev, lam, mu]
+            mds[i] = _ffi.MeshDesc(m.order, 0, E, m.n_nodes, en.ctypes.data, vx.shape[0], vx.ctypes.data,
+                                   ev.ctypes.data)
+            mats[i] = _ffi.MaterialDesc(_NAMES[p["material"]], lam.ctypes.data, mu.ctypes.data, 0.0, 0.0,
+                                        None, None, None, 0.0, 0.0, 0.0)
+            self.nl.append(NLProblem(m.n_dofs, p["fixed_dofs"], p["fixed_values"]))
+        h = C.c_void_p()
+        check(lib().pfgpu_batch_create(n, mds, mats, _PRECISIONS[precision], C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            check(lib().pfgpu_destroy(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
     def solve(self):
         """Returns (list of x per problem, list of NewtonReport per problem)."""
-        xs, reps = [None] * len(self.problems), [None] * len(self.problems)
-        for g in self.groups:
-            a, eo, no, prob = g["a"], g["eo"], g["no"], g["problem"]
-            x = np.zeros(a.mesh.n_dofs)
-            for k, i in enumerate(g["idx"]):
-                if "x0" in self.problems[i]:
-                    x[3 * no[k]:3 * no[k + 1]] = self.problems[i]["x0"]
-            x[prob.fixed] = prob.fixed_values
-            n = len(g["idx"])
-            r = (_ffi.NewtonReport * n)()
-            check(lib().pfgpu_newton_solve_batch(a._h, x.ctypes.data_as(C.c_void_p),
-                                                 prob.mask.ctypes.data_as(C.c_void_p), n,
-                                                 eo.ctypes.data_as(C.c_void_p), no.ctypes.data_as(C.c_void_p),
-                                                 C.byref(self.newton.params), r))
-            for k, i in enumerate(g["idx"]):
-                xs[i] = x[3 * no[k]:3 * no[k + 1]].copy()
-                rr = r[k]
-                reps[i] = NewtonReport(rr.iterations, rr.cg_iterations, rr.line_search_halvings, bool(rr.converged),
-                                       rr.energy, rr.grad_norm, rr.last_alpha, rr.gradient_steps)
-        return xs, reps
+        n = len(self.problems)
+        xs = []
+        for p, pr in zip(self.problems, self.nl):
+            x = np.array(p["x0"], dtype=np.float64) if "x0" in p else np.zeros(p["mesh"].n_dofs)
+            x[pr.fixed] = pr.fixed_values
+            xs.append(x)
+        xp = (C.c_void_p * n)(*[x.ctypes.data for x in xs])
+        mp = (C.c_void_p * n)(*[pr.mask.ctypes.data for pr in self.nl])
+        r = (_ffi.NewtonReport * n)()
+        check(lib().pfgpu_batch_newton(self._h, xp, mp, C.byref(self.newton.params), r))
+        return xs, [_report(rr) for rr in r]

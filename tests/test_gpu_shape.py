@@ -48,7 +48,7 @@ def test_set_vertices_matches_fresh_handle(order, precision):
     assert np.array_equal(sim.mass_matvec(b), sim_f.mass_matvec(b))
     y, it, _ = cg_solve(a, x, b, fixed, max_iterations=40, tolerance=0.0)
     yf, itf, _ = cg_solve(fresh, x, b, fixed, max_iterations=40, tolerance=0.0)
-    assert it == itf == 40
+    assert it == itf and 0 < it <= 40
     assert np.array_equal(y, yf)
 
 
