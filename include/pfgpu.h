@@ -204,6 +204,16 @@ pfgpu_status pfgpu_material_jvp(pfgpu_handle *h, const double *x, const double *
 /* ---- assembly, device pointers (asynchronous on the handle's stream) ---- */
 
 pfgpu_status pfgpu_value_dev(pfgpu_handle *h, const double *x, double *energy);
+/* The energy of pfgpu_value_dev without any allocation, for callers that
+ * cannot allocate (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  No counterpart in the reference.  Its
+ * scratch is a fixed device array of the library (65,536 + 128 doubles per
+ * GPU, held until the process ends): meshes of up to 16,777,216 elements;
+ * larger ones return PFGPU_ERR_UNSUPPORTED.  Every call on one GPU shares
+ * that array, so calls that overlap on one GPU (other streams, other
+ * handles) race.  The handle's device must be current.  Results equal
+ * pfgpu_value_dev's bit for bit. */
+pfgpu_status pfgpu_value_noalloc(pfgpu_handle *h, const double *x, double *energy);
 pfgpu_status pfgpu_gradient_dev(pfgpu_handle *h, const double *x, double *g);
 pfgpu_status pfgpu_hessian_values_dev(pfgpu_handle *h, const double *x, void *values);
 pfgpu_status pfgpu_energy_gradient_hessian_dev(pfgpu_handle *h, const double *x, double *energy,

@@ -97,6 +97,7 @@ _SIGNATURES = {
     "pfgpu_shape_vjp": [vp, vp, vp, vp],
     "pfgpu_material_jvp": [vp, vp, vp, vp, vp],
     "pfgpu_value_dev": [vp, vp, vp],
+    "pfgpu_value_noalloc": [vp, vp, vp],
     "pfgpu_gradient_dev": [vp, vp, vp],
     "pfgpu_hessian_values_dev": [vp, vp, vp],
     "pfgpu_energy_gradient_hessian_dev": [vp, vp, vp, vp, vp],

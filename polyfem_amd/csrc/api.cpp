@@ -3,6 +3,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <new>
 
 #include "dispatch.hpp"
@@ -707,6 +708,55 @@ pfgpu_status pfgpu_value_dev(pfgpu_handle *h, const double *x, double *energy)
         check_handle(h);
         require(x && energy, "null argument");
         plain_energy(*h, x, energy, nullptr);
+    });
+}
+
+} // extern "C"
+
+namespace {
+
+// Scratch of pfgpu_value_noalloc: the energy partials of up to
+// NOALLOC_MAX_PARTIALS workgroups of BLOCK elements, then SUM_BLOCKS
+// second-level sums.  Each GPU the library runs on holds its own copy until
+// the process ends; host code uses only the per-device addresses from
+// hipGetSymbolAddress, never the array's name.
+constexpr int NOALLOC_MAX_PARTIALS = 65536;
+constexpr int64_t NOALLOC_MAX_ELEMENTS = int64_t(NOALLOC_MAX_PARTIALS) * BLOCK;
+static __device__ __align__(256) double scratch[NOALLOC_MAX_PARTIALS + SUM_BLOCKS];
+
+std::mutex g_scratch_mutex;
+std::vector<double *> g_scratch_ptr; // per device, set on the first call on that device
+
+// the device address of `scratch` on device dev (the current device)
+double *scratch_address(int dev)
+{
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (size_t(dev) >= g_scratch_ptr.size()) g_scratch_ptr.resize(size_t(dev) + 1, nullptr);
+    if (!g_scratch_ptr[size_t(dev)]) {
+        void *p = nullptr;
+        PF_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch_ptr[size_t(dev)] = static_cast<double *>(p);
+    }
+    return g_scratch_ptr[size_t(dev)];
+}
+
+} // namespace
+
+extern "C" {
+
+pfgpu_status pfgpu_value_noalloc(pfgpu_handle *h, const double *x, double *energy)
+{
+    return guard([&] {
+        check_handle(h);
+        require(x && energy, "null argument");
+        if (int64_t(h->n_elements) > NOALLOC_MAX_ELEMENTS)
+            fail(PFGPU_ERR_UNSUPPORTED, "pfgpu_value_noalloc holds at most " + std::to_string(NOALLOC_MAX_ELEMENTS) +
+                                            " elements (its scratch is a fixed device array); use pfgpu_value_dev");
+        int cur = 0;
+        PF_HIP(hipGetDevice(&cur));
+        require(cur == h->device, "the handle's device must be the current device");
+        double *s = scratch_address(cur);
+        energy_into(*h, x, energy, s, s + NOALLOC_MAX_PARTIALS);
     });
 }
 
