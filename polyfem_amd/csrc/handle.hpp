@@ -79,6 +79,7 @@ struct pfgpu_handle {
     // CG operator: assembled block-sparse values or matrix-free, block-Jacobi preconditioner
     int solver_hessian = PFGPU_SOLVER_HESSIAN_AUTO, solver_precond = PFGPU_PRECOND_BLOCK_JACOBI;
     bool op_valid = false, op_assembled = false, op_precond = false;
+    bool op_records = false; // fac_rec holds the factor records at the operator's x
     pfgpu::SegSkip skip; // set only inside a batched CG
     pfgpu::DevBuf solver_vals, pinv, diag_pos, node_diag;
     hipStream_t own_stream = nullptr; // used when the caller's stream is the legacy default
@@ -128,6 +129,8 @@ void plain_shape_vjp(Handle &h, const double *x, const double *p, double *out);
 void ensure_plain_hessian_maps(Handle &h);
 void factored_energy_gradient_hessian(Handle &h, const double *x, double *energy, double *g, void *values);
 void factored_node_diagonal(Handle &h, const double *x, void *diag);
+// H v from the factor records in fac_rec (valid only while they belong to x)
+void factored_hvp(Handle &h, const double *v, double *y);
 
 // ---- mixed batches (batch.hip) -----------------------------------------
 void mixed_energy(Handle &h, const double *x, double *energy, double *per_element);

@@ -499,7 +499,9 @@ void hvp_any(Handle &h, const double *x, const double *v, double *y)
         PF_LAUNCH_CHECK();
         return;
     }
-    if (h.path == PFGPU_PATH_FUSED)
+    if (h.op_valid && h.op_records)
+        factored_hvp(h, v, y); // the records prepare_operator computed at x
+    else if (h.path == PFGPU_PATH_FUSED)
         fused_hvp(h, x, v, y);
     else
         plain_hvp(h, x, v, y);
@@ -521,6 +523,7 @@ void prepare_operator(Handle &h, const double *x, const uint8_t *fixed)
 {
     h.op_assembled = assembled_operator(h);
     h.op_precond = h.solver_precond == PFGPU_PRECOND_BLOCK_JACOBI;
+    h.op_records = false;
     // a matrix-free operator on the factored path needs only the diagonal
     // blocks for its preconditioner: those come from the factor records
     const bool diag_only = !h.op_assembled && h.op_precond && h.path == PFGPU_PATH_FACTORED &&
@@ -530,10 +533,12 @@ void prepare_operator(Handle &h, const double *x, const uint8_t *fixed)
     if (diag_only || mixed_diag) {
         if (!h.brow_ptr.get()) h.brow_ptr = upload(h.pat.brow_ptr, h.stream);
         h.node_diag.reserve(h.value_size() * 9 * size_t(h.n_nodes));
-        if (mixed_diag)
+        if (mixed_diag) {
             mixed_node_diagonal(h, x, h.node_diag.get());
-        else
+        } else {
             factored_node_diagonal(h, x, h.node_diag.get());
+            h.op_records = true; // the matrix-free products of this operator use them
+        }
     } else if (h.op_assembled || h.op_precond) {
         if (!h.brow_ptr.get()) h.brow_ptr = upload(h.pat.brow_ptr, h.stream);
         if (!h.pat_bcol.get()) h.pat_bcol = upload(h.pat.bcol, h.stream);
