@@ -186,7 +186,7 @@ int64_t pfgpu_handle::device_bytes() const
 {
     const DevBuf *bufs[] = {&conn, &vconn, &geom, &lame, &node_ptr, &node_inc, &vert_ptr, &vert_inc,
                             &brow_ptr, &blk_out, &blk_plan.chunk, &blk_plan.slot, &blk_plan.meta, &fac_plan.chunk,
-                            &fac_plan.slot, &fac_plan.meta, &fac_plan.dst, &unit_plan.chunk_row, &unit_plan.udst, &unit_plan.bq0, &row_chunk, &row_slot, &row_meta, &node_diag, &patch_desc, &patch_rows, &fac_rec,
+                            &fac_plan.slot, &fac_plan.meta, &fac_plan.dst, &unit_plan.chunk_row, &unit_plan.udst, &unit_plan.bq0, &unit_plan.lanes, &row_chunk, &row_slot, &row_meta, &node_diag, &patch_desc, &patch_rows, &fac_rec,
                             &patch_row_acc, &patch_elems, &patch_cs, &patch_cg, &patch_hdr, &patch_pos, &patch_vnodes,
                             &d_outer, &d_inner, &loc, &hloc, &partial, &red_out, &sum_part,
                             &h_x, &h_v, &h_out, &h_out2, &h_vals, &sv, &h_mask, &mass, &pat_bcol, &tr_fixed, &tr_f, &tr_seg, &load};

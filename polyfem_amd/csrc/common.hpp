@@ -91,9 +91,9 @@ struct GatherPlan {
 // per pattern block the first stage position of its contributions within
 // its chunk.
 struct UnitPlan {
-    DevBuf chunk_row, udst, bq0;
-    int n_chunks = 0, threads = 0;
-    unsigned grid = 0;
+    DevBuf chunk_row, udst, bq0, lanes;
+    int n_chunks = 0, threads = 0, n_words = 0;
+    unsigned grid = 0, grid_grad = 0;
 };
 
 // v into dst: in place when the size matches (device pointers held elsewhere,

@@ -1068,17 +1068,23 @@ __global__ __launch_bounds__(GS) void k_gather_factored(int n_chunks, const int3
 // Row-unit gather of the factored route.  A unit is one node incidence
 // (row node i = local node a of element e); the units of a node row are its
 // incident elements in element order (node_inc).  A chunk is a run of whole
-// rows holding at most GS units and GS rows; thread t of the workgroup takes
-// unit t of the chunk: it reads the element's factor record once, forms the
-// n_local blocks H_e(a, b) of its row, b = 0 .. n_local - 1 (b is unrolled,
-// so the reference gradients RG(q, b) are compile-time constants and their
-// zeros cost nothing), and stages each block at its position udst (the
-// blocks of the chunk in pattern order, the contributions of a block in unit
-// order).  One thread per pattern block then sums its contributions into the
-// CSR values (3 runs of 3 values per block, consecutive blocks of a row
-// contiguous), and one thread per row sums the local gradient rows of its
-// units in FP64 when gloc is given.  Summation orders are fixed by the
-// tables: results are bitwise reproducible and independent of GS.
+// rows holding at most GS units and GS - 1 rows; thread t of the workgroup
+// takes unit t of the chunk: it reads the element's factor record once,
+// forms the n_local blocks H_e(a, b) of its row, b = 0 .. n_local - 1 (b is
+// unrolled, so the reference gradients RG(q, b) are compile-time constants
+// and their zeros cost nothing), and stages each block at its position udst
+// (the blocks of the chunk in pattern order, the contributions of a block in
+// unit order).  The block sums are then split into pieces of at most
+// UNIT_PIECE contributions (plans.hpp); thread t sums its run of pieces,
+// lanes[c (GS + 1) + t] .. lanes[c (GS + 1) + t + 1] (balanced by the plan),
+// each piece's sum over the piece's first staged contribution.  After a
+// barrier thread t takes blocks t, t + GS, ...: it adds the block's piece sums
+// in piece order and stores the block into the CSR values (3 runs of 3 values
+// per block, consecutive blocks of a row contiguous, so the stores of
+// consecutive threads coalesce).  One thread per (row, component) sums the
+// local gradient rows of the row's units in FP64 when GRAD.  Summation orders
+// are fixed by the tables and do not depend on GS: results are bitwise
+// reproducible.
 //
 //   H_e(a, b) = (sum_q RG(q,a)^T G RG(q,b)) I + sum_q [cl_q k_a k_b^T + cc_q k_b k_a^T],
 //   k_a = M_q RG(q, a),  G = w mu J^{-1} J^{-T} (the record's metric tail)
@@ -1087,20 +1093,25 @@ __global__ __launch_bounds__(GS) void k_gather_factored(int n_chunks, const int3
 // few, so the kernel is persistent and software-pipelined: the workgroups of
 // XCD x (blockIdx mod 8) walk that XCD's contiguous eighth of the chunks with
 // stride gridDim / 8; while chunk k is formed and summed, the records, stage
-// positions, local gradients and row / block tables of chunk k + 1 are in
-// flight, and the unit codes of chunk k + 2 (on which the record addresses
-// depend).  ctab[c] = (first row, first unit, first pattern block) of chunk c,
-// ctab[n_chunks] the totals.  gridDim must be a multiple of 8.
-template <typename T, int ORDER, int NQ, int GS>
-__global__ __launch_bounds__(GS) void k_gather_units(int n_chunks, const int4 *__restrict__ ctab,
+// positions, local gradients, lane entries and row / block tables of chunk
+// k + 1 are in flight, and the unit codes of chunk k + 2 (on which the record
+// addresses depend).  Every load of the pipeline is issued unconditionally
+// (indices clamped into their arrays, results of lanes past the chunk
+// unused), so the compiler can wait for exactly the loads of the chunk at
+// hand.  ctab[c] = (first row, first unit, first pattern block) of chunk c,
+// ctab[n_chunks] the totals; n_words = 32-bit words of bq0.  gridDim must be
+// a multiple of 8.
+template <typename T, int ORDER, int NQ, int GS, bool GRAD>
+__global__ __launch_bounds__(GS) void k_gather_units(int n_chunks, int n_words, const int4 *__restrict__ ctab,
                                                      const int32_t *__restrict__ node_ptr,
                                                      const int32_t *__restrict__ node_inc,
                                                      const int32_t *__restrict__ brow_ptr,
                                                      const uint16_t *__restrict__ udst, const uint16_t *__restrict__ bq0,
-                                                     const T *__restrict__ rec, const T *__restrict__ gloc,
-                                                     T *__restrict__ values, double *__restrict__ grad)
+                                                     const uint32_t *__restrict__ lanes, const T *__restrict__ rec,
+                                                     const T *__restrict__ gloc, T *__restrict__ values,
+                                                     double *__restrict__ grad)
 {
-    constexpr int NB = NLocal<ORDER>::value, R = fac_size(NQ), DW = NB / 2;
+    constexpr int NB = NLocal<ORDER>::value, R = fac_size(NQ), DW = NB / 2, L = UNIT_PIECE;
     // 32-bit words of block starts per thread: a chunk has at most GS NB blocks
     constexpr int NWMAX = GS * NB / 2 + 2, QW = (NWMAX + GS - 1) / GS;
     static_assert(NB % 2 == 0, "stage positions are read as 32-bit pairs");
@@ -1115,7 +1126,7 @@ __global__ __launch_bounds__(GS) void k_gather_units(int n_chunks, const int4 *_
     const int stride = int(gridDim.x / N_XCD), c_end = min(n_chunks, (xcd + 1) * per);
     int c = xcd * per + int(blockIdx.x / N_XCD);
     if (c >= c_end) return; // uniform over the workgroup
-    const int t = threadIdx.x;
+    const int t = threadIdx.x, last_unit = ctab[n_chunks].y - 1;
     const RefTab<T, ORDER, NQ> &tb = ref_tab<T, ORDER, NQ>();
     for (int k = t; k < NQ * NB * 3; k += GS) srg[k] = tb.rg[k];
     __syncthreads();
@@ -1123,89 +1134,85 @@ __global__ __launch_bounds__(GS) void k_gather_units(int n_chunks, const int4 *_
     struct Meta {
         int i0, nr, u0, nu, B0, nb;
     };
+    // chunk cc's metadata; past the XCD's last chunk, that chunk's again
     auto meta = [&](int cc) {
+        cc = min(cc, c_end - 1);
         const int4 a = ctab[cc], b = ctab[cc + 1];
         return Meta{a.x, b.x - a.x, a.y, b.y - a.y, a.z, b.z - a.z};
     };
-    auto code_of = [&](const Meta &m) { return t < m.nu ? node_inc[m.u0 + t] : -1; };
+    // the unit code (e n_local + a) of thread t, some other unit's past the chunk
+    auto code_of = [&](const Meta &m) { return node_inc[min(m.u0 + t, last_unit)]; };
     // everything else a chunk needs, in registers
     struct Unit {
         T r[R];
         uint32_t dw[DW];
         T l0, l1, l2;
-        int32_t rb0, rb1, ru;
+        int32_t rb0, ru;
         uint32_t qw[QW];
+        uint32_t lane0, lane1;
     };
-    auto load_unit = [&](const Meta &m, int code, Unit &u) {
-        if (code >= 0) {
-            const int e = code / NB;
-            const T *rp = static_cast<const T *>(__builtin_assume_aligned(rec, 16)) + int64_t(e) * R;
+    auto load_unit = [&](int cc, const Meta &m, int code, Unit &u) {
+        const int e = code / NB;
+        const T *rp = static_cast<const T *>(__builtin_assume_aligned(rec, 16)) + int64_t(e) * R;
 #pragma unroll
-            for (int i = 0; i < R; ++i) u.r[i] = rp[i];
-            const uint32_t *dp = reinterpret_cast<const uint32_t *>(udst + int64_t(m.u0 + t) * NB);
+        for (int i = 0; i < R; ++i) u.r[i] = rp[i];
+        const uint32_t *dp = reinterpret_cast<const uint32_t *>(udst + int64_t(min(m.u0 + t, last_unit)) * NB);
 #pragma unroll
-            for (int j = 0; j < DW; ++j) u.dw[j] = dp[j];
-            if (gloc) {
-                const T *gp = gloc + int64_t(code) * 3;
-                u.l0 = gp[0];
-                u.l1 = gp[1];
-                u.l2 = gp[2];
-            }
+        for (int j = 0; j < DW; ++j) u.dw[j] = dp[j];
+        if constexpr (GRAD) {
+            const T *gp = gloc + int64_t(code) * 3;
+            u.l0 = gp[0];
+            u.l1 = gp[1];
+            u.l2 = gp[2];
         }
-        if (t <= m.nr) {
-            u.rb0 = brow_ptr[m.i0 + t];
-            u.ru = node_ptr[m.i0 + t];
-            u.rb1 = t < m.nr ? brow_ptr[m.i0 + t + 1] : u.rb0;
-        }
+        const int rowi = m.i0 + min(t, m.nr);
+        u.rb0 = brow_ptr[rowi];
+        u.ru = node_ptr[rowi];
         // block starts: bq0[B0 .. B0 + nb - 1] as 32-bit words from an even index
-        const uint32_t *qp = reinterpret_cast<const uint32_t *>(bq0 + (m.B0 & ~1));
-        const int nw = (m.nb + (m.B0 & 1) + 1) / 2;
+        const uint32_t *qp = reinterpret_cast<const uint32_t *>(bq0);
+        const int w0 = m.B0 >> 1;
 #pragma unroll
-        for (int j = 0; j < QW; ++j)
-            if (t + j * GS < nw) u.qw[j] = qp[t + j * GS];
+        for (int j = 0; j < QW; ++j) u.qw[j] = qp[min(w0 + t + j * GS, n_words - 1)];
+        const uint32_t *lp = lanes + int64_t(min(cc, c_end - 1)) * (GS + 1) + t;
+        u.lane0 = lp[0];
+        u.lane1 = lp[1];
     };
 
     Meta m0 = meta(c);
-    Unit u0;
     int code0 = code_of(m0);
-    load_unit(m0, code0, u0);
+    Unit u0;
+    load_unit(c, m0, code0, u0);
     int c1 = c + stride;
-    Meta m1{};
-    int code1 = -1;
-    if (c1 < c_end) {
-        m1 = meta(c1);
-        code1 = code_of(m1);
-    }
+    Meta m1 = meta(c1);
+    int code1 = code_of(m1);
     while (true) { // c, c1 and the chunk metadata are uniform over the workgroup
         // in flight while chunk c is processed: chunk c1's data, chunk c2's codes
         Unit u1;
-        if (c1 < c_end) load_unit(m1, code1, u1);
+        load_unit(c1, m1, code1, u1);
         const int c2 = c1 + stride;
-        Meta m2{};
-        int code2 = -1;
-        if (c2 < c_end) {
-            m2 = meta(c2);
-            code2 = code_of(m2);
-        }
+        const Meta m2 = meta(c2);
+        const int code2 = code_of(m2);
         // ---- chunk c: blocks of this thread's unit into the stage
-        if (code0 >= 0) {
+        if (t < m0.nu) {
             const int a = code0 - (code0 / NB) * NB;
             const T *r = u0.r;
-            if (gloc) {
+            if constexpr (GRAD) {
                 sg[t * 3] = u0.l0;
                 sg[t * 3 + 1] = u0.l1;
                 sg[t * 3 + 2] = u0.l2;
             }
             const T *G6 = r + 11 * NQ; // 00 11 22 01 02 12
-            T ka[NQ][3], la[NQ][3], ha[NQ][3];
+            // per quadrature point: cl k_a, cc k_a and G RG(q, a)
+            T la[NQ][3], lc[NQ][3], ha[NQ][3];
 #pragma unroll
             for (int q = 0; q < NQ; ++q) {
                 const T *M = r + 9 * q, *ra = srg + (q * NB + a) * 3;
                 const T ra0 = ra[0], ra1 = ra[1], ra2 = ra[2];
 #pragma unroll
                 for (int mm = 0; mm < 3; ++mm) {
-                    ka[q][mm] = fma_t(M[mm * 3 + 2], ra2, fma_t(M[mm * 3 + 1], ra1, M[mm * 3] * ra0));
-                    la[q][mm] = r[9 * NQ + q] * ka[q][mm];
+                    const T ka = fma_t(M[mm * 3 + 2], ra2, fma_t(M[mm * 3 + 1], ra1, M[mm * 3] * ra0));
+                    la[q][mm] = r[9 * NQ + q] * ka;
+                    lc[q][mm] = r[10 * NQ + q] * ka;
                 }
                 ha[q][0] = fma_t(G6[4], ra2, fma_t(G6[3], ra1, G6[0] * ra0));
                 ha[q][1] = fma_t(G6[5], ra2, fma_t(G6[1], ra1, G6[3] * ra0));
@@ -1242,13 +1249,12 @@ __global__ __launch_bounds__(GS) void k_gather_units(int n_chunks, const int4 *_
                         }
                         kb[mm] = s;
                     }
-                    const T cc = r[10 * NQ + q];
-                    const T lb[3] = {cc * kb[0], cc * kb[1], cc * kb[2]};
+                    // cl k_a k_b^T + cc k_b k_a^T
 #pragma unroll
                     for (int mm = 0; mm < 3; ++mm)
 #pragma unroll
                         for (int nn = 0; nn < 3; ++nn)
-                            v[mm * 3 + nn] = fma_t(lb[mm], ka[q][nn], fma_t(la[q][mm], kb[nn], v[mm * 3 + nn]));
+                            v[mm * 3 + nn] = fma_t(kb[mm], lc[q][nn], fma_t(la[q][mm], kb[nn], v[mm * 3 + nn]));
                 }
                 const int d = int((u0.dw[b >> 1] >> (16 * (b & 1))) & 0xffffu);
                 T *o = sv + d * 9;
@@ -1261,8 +1267,6 @@ __global__ __launch_bounds__(GS) void k_gather_units(int n_chunks, const int4 *_
             srb[t] = u0.rb0 - m0.B0;
             sru[t] = u0.ru - m0.u0;
         }
-        if (t < m0.nr)
-            for (int k = u0.rb0 - m0.B0; k < u0.rb1 - m0.B0; ++k) srow[k] = uint8_t(t);
         {
             const int nw = (m0.nb + (m0.B0 & 1) + 1) / 2;
 #pragma unroll
@@ -1270,36 +1274,99 @@ __global__ __launch_bounds__(GS) void k_gather_units(int n_chunks, const int4 *_
                 if (t + j * GS < nw) sq[t + j * GS] = u0.qw[j];
         }
         __syncthreads();
-        // one thread per pattern block: its contributions in unit order
+        // ---- chunk c: this thread's run of pieces, each piece's sum over its
+        // first staged contribution; the row of every block it starts
         const uint16_t *sq16 = reinterpret_cast<const uint16_t *>(sq) + (m0.B0 & 1);
-        for (int bb = t; bb < m0.nb; bb += GS) {
-            const int rr = srow[bb];
-            const int q0 = sq16[bb], q1 = bb + 1 < m0.nb ? int(sq16[bb + 1]) : m0.nu * NB;
+        const int total = m0.nu * NB;
+        auto block_end = [&](int k) { return k + 1 < m0.nb ? int(sq16[k + 1]) : total; };
+        {
+            int k = int(u0.lane0 & 0xfffu), j = int((u0.lane0 >> 12) & 0xffu), rr = int(u0.lane0 >> 20);
+            const int ke = int(u0.lane1 & 0xfffu), je = int((u0.lane1 >> 12) & 0xffu);
+            if ((k != ke || j != je) && k < m0.nb) {
+                int re = srb[rr + 1];
+                int q0 = sq16[k], q1 = block_end(k), qn = block_end(k + 1);
+                while (true) {
+                    const int s = q0 + j * L, pe = min(q1, s + L);
+                    T acc[9];
+#pragma unroll
+                    for (int i = 0; i < 9; ++i) acc[i] = T(0);
+                    // two contributions in flight, added in unit order
+                    int x = s;
+                    for (; x + 2 <= pe; x += 2) {
+                        T c0[9], c1[9];
+#pragma unroll
+                        for (int i = 0; i < 9; ++i) {
+                            c0[i] = sv[x * 9 + i];
+                            c1[i] = sv[x * 9 + 9 + i];
+                        }
+#pragma unroll
+                        for (int i = 0; i < 9; ++i) acc[i] = (acc[i] + c0[i]) + c1[i];
+                    }
+                    if (x < pe)
+#pragma unroll
+                        for (int i = 0; i < 9; ++i) acc[i] += sv[x * 9 + i];
+                    if (s < pe) // (a block without contributions has an empty piece)
+#pragma unroll
+                        for (int i = 0; i < 9; ++i) sv[s * 9 + i] = acc[i];
+                    if (j == 0) srow[k] = uint8_t(rr);
+                    if (pe == q1) {
+                        ++k;
+                        j = 0;
+                    } else {
+                        ++j;
+                    }
+                    if ((k == ke && j == je) || k >= m0.nb) break;
+                    if (j == 0) {
+                        while (k >= re) re = srb[++rr + 1];
+                        q0 = q1;
+                        q1 = qn;
+                        qn = block_end(k + 1);
+                    }
+                }
+            }
+        }
+        // ---- chunk c: one thread per (row, component): the gradient, in FP64
+        // (unit order; four staged rows in flight)
+        if constexpr (GRAD) {
+            for (int it = t; it < 3 * m0.nr; it += GS) {
+                const int row = it / 3, comp = it - 3 * row, ue = sru[row + 1];
+                int u = sru[row];
+                double s = 0;
+                for (; u + 4 <= ue; u += 4) {
+                    const T g0 = sg[u * 3 + comp], g1 = sg[u * 3 + 3 + comp], g2 = sg[u * 3 + 6 + comp],
+                            g3 = sg[u * 3 + 9 + comp];
+                    s += double(g0);
+                    s += double(g1);
+                    s += double(g2);
+                    s += double(g3);
+                }
+                for (; u < ue; ++u) s += double(sg[u * 3 + comp]);
+                grad[3 * int64_t(m0.i0) + it] = s;
+            }
+        }
+        __syncthreads();
+        // ---- chunk c: one thread per block (consecutive threads, consecutive
+        // blocks: the stores of a row's run coalesce): the piece sums in piece
+        // order, into the CSR values
+        for (int k = t; k < m0.nb; k += GS) {
+            const int q0 = sq16[k], q1 = block_end(k), rr = srow[k];
+            const int rs = srb[rr], deg = srb[rr + 1] - rs;
             T acc[9];
+            if (q0 < q1) {
 #pragma unroll
-            for (int i = 0; i < 9; ++i) acc[i] = T(0);
-            for (int k = q0; k < q1; ++k)
+                for (int i = 0; i < 9; ++i) acc[i] = sv[q0 * 9 + i];
+            } else {
 #pragma unroll
-                for (int i = 0; i < 9; ++i) acc[i] += sv[k * 9 + i];
-            const int rb0 = srb[rr], deg = srb[rr + 1] - rb0;
-            T *out = values + 9 * int64_t(m0.B0 + rb0) + 3 * (bb - rb0);
+                for (int i = 0; i < 9; ++i) acc[i] = T(0); // a block without contributions
+            }
+            for (int s = q0 + L; s < q1; s += L)
+#pragma unroll
+                for (int i = 0; i < 9; ++i) acc[i] += sv[s * 9 + i];
+            T *out = values + 9 * int64_t(m0.B0 + rs) + 3 * (k - rs);
 #pragma unroll
             for (int mm = 0; mm < 3; ++mm)
 #pragma unroll
                 for (int nn = 0; nn < 3; ++nn) out[mm * 3 * deg + nn] = acc[mm * 3 + nn];
-        }
-        // one thread per row: the gradient, in FP64 (rows without units get 0)
-        if (gloc && t < m0.nr) {
-            double s0 = 0, s1 = 0, s2 = 0;
-            for (int u = sru[t]; u < sru[t + 1]; ++u) {
-                s0 += double(sg[u * 3]);
-                s1 += double(sg[u * 3 + 1]);
-                s2 += double(sg[u * 3 + 2]);
-            }
-            double *g = grad + 3 * int64_t(m0.i0 + t);
-            g[0] = s0;
-            g[1] = s1;
-            g[2] = s2;
         }
         if (c1 >= c_end) break;
         __syncthreads(); // the stage and tables are read before chunk c1 overwrites them
@@ -1637,10 +1704,13 @@ bool build_unit_plan(Handle &h, UnitPlan &p, int gs)
     // from an even index, up to one entry past a chunk's last block
     u.bq0.push_back(0);
     u.bq0.push_back(0);
+    if (u.bq0.size() % 2) u.bq0.push_back(0);
     p.bq0 = upload(u.bq0, h.stream);
+    p.n_words = int(u.bq0.size() / 2);
+    p.lanes = upload(u.lanes, h.stream);
     p.n_chunks = nc;
     p.threads = gs;
-    p.grid = 0;
+    p.grid = p.grid_grad = 0;
     return true;
 }
 
@@ -1649,28 +1719,36 @@ bool build_unit_plan(Handle &h, UnitPlan &p, int gs)
 template <typename T, int O, int Q> void launch_unit_gather(Handle &h, UnitPlan &p, const T *gloc, double *g, void *values)
 {
     constexpr int NB = NLocal<O>::value;
-    auto gath = [&]<int GS>() {
-        auto kern = k_gather_units<T, O, Q, GS>;
+    auto gath = [&]<int GS, bool GRAD>() {
+        auto kern = k_gather_units<T, O, Q, GS, GRAD>;
         const int lds = int(sizeof(T)) * GS * (NB * 9 + 3);
-        if (p.grid == 0) {
+        unsigned &grid = GRAD ? p.grid_grad : p.grid;
+        if (grid == 0) {
             PF_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(kern), hipFuncAttributeMaxDynamicSharedMemorySize, lds));
             // persistent: the workgroups resident at once, a multiple of the 8 XCDs
             if (h.n_cu == 0) PF_HIP(hipDeviceGetAttribute(&h.n_cu, hipDeviceAttributeMultiprocessorCount, h.device));
             int per_cu = 0;
             PF_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, GS, size_t(lds)));
             const int g = std::max(1, h.n_cu) * std::max(1, per_cu);
-            p.grid = std::min(xcd_grid(p.n_chunks), unsigned((g + N_XCD - 1) / N_XCD * N_XCD));
+            grid = std::min(xcd_grid(p.n_chunks), unsigned((g + N_XCD - 1) / N_XCD * N_XCD));
         }
-        kern<<<p.grid, GS, lds, h.stream>>>(p.n_chunks, p.chunk_row.as<int4>(), h.node_ptr.as<int32_t>(),
-                                            h.node_inc.as<int32_t>(), h.brow_ptr.as<int32_t>(), p.udst.as<uint16_t>(),
-                                            p.bq0.as<uint16_t>(), h.fac_rec.as<T>(), gloc, static_cast<T *>(values), g);
+        kern<<<grid, GS, lds, h.stream>>>(p.n_chunks, p.n_words, p.chunk_row.as<int4>(), h.node_ptr.as<int32_t>(),
+                                          h.node_inc.as<int32_t>(), h.brow_ptr.as<int32_t>(), p.udst.as<uint16_t>(),
+                                          p.bq0.as<uint16_t>(), p.lanes.as<uint32_t>(), h.fac_rec.as<T>(), gloc,
+                                          static_cast<T *>(values), g);
     };
-    if (p.threads == 64)
-        gath.template operator()<64>();
-    else if (p.threads == 256)
-        gath.template operator()<256>();
+    auto sized = [&]<bool GRAD>() {
+        if (p.threads == 64)
+            gath.template operator()<64, GRAD>();
+        else if (p.threads == 256)
+            gath.template operator()<256, GRAD>();
+        else
+            gath.template operator()<128, GRAD>();
+    };
+    if (gloc)
+        sized.template operator()<true>();
     else
-        gath.template operator()<128>();
+        sized.template operator()<false>();
     PF_LAUNCH_CHECK();
 }
 

@@ -2,23 +2,83 @@
 // Hessian route (the kernel is k_gather_units in plain.hip).  Host code
 // only, so that tests can build and check the plans on the CPU.
 #include <algorithm>
+#include <atomic>
 #include <thread>
 
 #include "plans.hpp"
 
 namespace pfgpu {
 
+namespace {
+
+// a piece of a chunk's block sums: block (chunk-local), piece within the
+// block, the block's row (chunk-local), contributions
+struct Piece {
+    int32_t block, piece, row, len;
+};
+// the cost of a piece in contributions read: its reads plus the staging of
+// its sum
+constexpr int PIECE_COST = 2;
+
+// greedy runs of consecutive pieces of at most bound cost: the number of
+// runs (and their first pieces)
+int piece_runs(const std::vector<Piece> &p, int bound, std::vector<int> *first)
+{
+    int n = 0, cur = 0;
+    for (size_t i = 0; i < p.size(); ++i) {
+        const int c = p[i].len + PIECE_COST;
+        if (n == 0 || cur + c > bound) {
+            ++n;
+            cur = 0;
+            if (first) first->push_back(int(i));
+        }
+        cur += c;
+    }
+    return n;
+}
+
+// the threads' runs of pieces in one chunk: the least maximal run cost for
+// which gs threads suffice, written as gs + 1 lane entries
+void chunk_lanes(const std::vector<Piece> &p, int nb, int nr, int gs, uint32_t *lanes)
+{
+    int lo = 0, hi = 0;
+    for (const Piece &x : p) {
+        lo = std::max(lo, x.len + PIECE_COST);
+        hi += x.len + PIECE_COST;
+    }
+    while (lo < hi) {
+        const int mid = lo + (hi - lo) / 2;
+        if (piece_runs(p, mid, nullptr) <= gs)
+            hi = mid;
+        else
+            lo = mid + 1;
+    }
+    std::vector<int> first;
+    piece_runs(p, lo, &first);
+    for (int t = 0; t <= gs; ++t) {
+        if (t < int(first.size())) {
+            const Piece &x = p[size_t(first[size_t(t)])];
+            lanes[t] = unit_lane(x.block, x.piece, x.row);
+        } else {
+            lanes[t] = unit_lane(nb, 0, nr);
+        }
+    }
+}
+
+} // namespace
+
 bool make_unit_plan(const Pattern &pat, int n_local, const int32_t *elem_nodes, int gs, UnitPlanHost &out)
 {
     const std::vector<int32_t> &np = pat.node_ptr, &ni = pat.node_inc, &bp = pat.brow_ptr, &bc = pat.bcol;
     const int NB = n_local, N = pat.n_nodes;
-    // chunks of whole rows: at most gs units and gs rows each
+    // chunks of whole rows: at most gs units and gs - 1 rows each (a thread
+    // per row boundary)
     std::vector<int32_t> c{0};
     int units = 0, rows = 0;
     for (int i = 0; i < N; ++i) {
         const int ui = np[size_t(i) + 1] - np[size_t(i)];
         if (ui > std::min(gs, 255)) return false;
-        if (units + ui > gs || rows + 1 > gs) {
+        if (units + ui > gs || rows + 1 > gs - 1) {
             c.push_back(i);
             units = 0;
             rows = 0;
@@ -29,11 +89,14 @@ bool make_unit_plan(const Pattern &pat, int n_local, const int32_t *elem_nodes, 
     c.push_back(N);
     const int nc = int(c.size()) - 1;
     std::vector<uint16_t> q0(size_t(pat.n_blocks)), dst(size_t(np[size_t(N)]) * NB);
+    std::vector<uint32_t> lanes(size_t(nc) * size_t(gs + 1));
+    std::atomic<bool> fits{true};
     // per chunk: every block's contributions are consecutive stage positions
     // (blocks in pattern order, contributions in unit order); a unit's block
     // b goes to the next free position of block (row, node_e(b))
     auto fill = [&](int c0, int c1) {
         std::vector<int32_t> cnt;
+        std::vector<Piece> pieces;
         for (int ci = c0; ci < c1; ++ci) {
             int run = 0;
             for (int i = c[size_t(ci)]; i < c[size_t(ci) + 1]; ++i) {
@@ -58,15 +121,33 @@ bool make_unit_plan(const Pattern &pat, int n_local, const int32_t *elem_nodes, 
                     }
                 }
             }
+            // the pieces of the chunk's block sums, in block order
+            const int i0 = c[size_t(ci)], i1 = c[size_t(ci) + 1], B0 = bp[size_t(i0)], nb = bp[size_t(i1)] - B0;
+            if (nb > 4095 || i1 - i0 > 4095) {
+                fits = false;
+                continue;
+            }
+            pieces.clear();
+            for (int i = i0; i < i1; ++i)
+                for (int32_t blk = bp[size_t(i)]; blk < bp[size_t(i) + 1]; ++blk) {
+                    const int k = blk - B0, start = q0[size_t(blk)];
+                    const int n = (k + 1 < nb ? int(q0[size_t(blk) + 1]) : run) - start;
+                    const int pc = n <= UNIT_PIECE ? 1 : (n + UNIT_PIECE - 1) / UNIT_PIECE;
+                    for (int j = 0; j < pc; ++j)
+                        pieces.push_back(Piece{k, j, i - i0, std::min(UNIT_PIECE, n - j * UNIT_PIECE)});
+                }
+            chunk_lanes(pieces, nb, i1 - i0, gs, lanes.data() + size_t(ci) * size_t(gs + 1));
         }
     };
     const int nt = int(std::max(1u, std::min(16u, std::thread::hardware_concurrency())));
     std::vector<std::thread> pool;
     for (int t = 0; t < nt; ++t) pool.emplace_back(fill, int(int64_t(nc) * t / nt), int(int64_t(nc) * (t + 1) / nt));
     for (std::thread &th : pool) th.join();
+    if (!fits) return false;
     out.chunk_row = std::move(c);
     out.udst = std::move(dst);
     out.bq0 = std::move(q0);
+    out.lanes = std::move(lanes);
     return true;
 }
 

@@ -12,7 +12,13 @@
 //     chunk are a bijection onto [0, units x n_local);
 //  3. within a block, contributions are staged in unit (element) order, and
 //     each block's count equals a brute-force count of the row's elements
-//     that contain the column node.
+//     that contain the column node;
+//  4. the threads' runs of pieces, walked exactly as the kernel walks them,
+//     read every staged contribution once, stage every non-empty piece's sum
+//     over its first contribution and record the row of every block they
+//     start, staying inside the stage and the chunk's blocks and rows; the
+//     per-block pass then finds every piece sum of every block (pieces of
+//     UNIT_PIECE, in order) where the runs left it.
 // Input: a binary mesh file (order, E, N, Nv, elem_nodes, elem_vertices,
 // vertices) written by tests/test_plans.py.
 #include <algorithm>
@@ -66,7 +72,9 @@ int main(int argc, char **argv)
         }
         const int nc = int(u.chunk_row.size()) - 1;
         CHECK(u.chunk_row.front() == 0 && u.chunk_row.back() == N, "chunk rows do not cover the nodes");
-        CHECK(u.udst.size() == size_t(np[N]) * NB && u.bq0.size() == size_t(pat.n_blocks), "plan sizes");
+        CHECK(u.udst.size() == size_t(np[N]) * NB && u.bq0.size() == size_t(pat.n_blocks) &&
+                  u.lanes.size() == size_t(nc) * size_t(gs + 1),
+              "plan sizes");
         std::vector<uint8_t> seen;
         for (int c = 0; c < nc; ++c) {
             const int i0 = u.chunk_row[c], i1 = u.chunk_row[c + 1];
@@ -116,6 +124,55 @@ int main(int argc, char **argv)
                 }
             }
             for (int d = 0; d < total; ++d) CHECK(seen[size_t(d)], "chunk %d: stage %d never written", c, d);
+            CHECK(nr <= gs - 1, "chunk %d: %d rows > %d", c, nr, gs - 1);
+            // 4. the kernel's walk over the lane entries
+            const uint32_t *ln = u.lanes.data() + size_t(c) * size_t(gs + 1);
+            std::vector<int> read(size_t(total), 0), written(size_t(nb), 0), partial(size_t(total), 0);
+            std::vector<int> srb(size_t(nr) + 1);
+            for (int r = 0; r <= nr; ++r) srb[size_t(r)] = bp[i0 + r] - B0;
+            auto bq = [&](int k) { return int(u.bq0[size_t(B0 + k)]); };
+            auto bend = [&](int k) { return k + 1 < nb ? bq(k + 1) : total; };
+            auto row_of = [&](int k) { return int(std::upper_bound(srb.begin(), srb.end(), k) - srb.begin()) - 1; };
+            std::vector<int> srow(size_t(nb), -1);
+            for (int t = 0; t < gs; ++t) {
+                int k = int(ln[t] & 0xfffu), j = int((ln[t] >> 12) & 0xffu), rr = int(ln[t] >> 20);
+                const int ke = int(ln[t + 1] & 0xfffu), je = int((ln[t + 1] >> 12) & 0xffu);
+                if (!((k != ke || j != je) && k < nb)) continue;
+                CHECK(rr >= 0 && rr < nr && rr == row_of(k), "chunk %d lane %d: row %d of block %d", c, t, rr, k);
+                int q0 = bq(k), q1 = bend(k), rend = srb[size_t(rr) + 1], steps = 0;
+                while (true) {
+                    const int s = q0 + j * UNIT_PIECE, pe = std::min(q1, s + UNIT_PIECE);
+                    CHECK(s >= 0 && pe <= total && (s < pe || (j == 0 && q0 == q1)), "chunk %d lane %d: piece [%d, %d)", c, t, s, pe);
+                    for (int x = s; x < pe && x < total; ++x) read[size_t(x)]++;
+                    if (s < pe && s < total) partial[size_t(s)]++;
+                    CHECK(rr == row_of(k), "chunk %d lane %d: block %d tracked in row %d", c, t, k, rr);
+                    if (j == 0) {
+                        CHECK(srow[size_t(k)] == -1, "chunk %d: block %d started twice", c, k);
+                        srow[size_t(k)] = rr;
+                    }
+                    if (pe == q1) {
+                        ++k;
+                        j = 0;
+                    } else {
+                        ++j;
+                    }
+                    if ((k == ke && j == je) || k >= nb || ++steps > 100000) break;
+                    if (j == 0) {
+                        while (k >= rend) rend = srb[size_t(++rr) + 1];
+                        q0 = q1;
+                        q1 = bend(k);
+                    }
+                }
+            }
+            // the per-block pass
+            for (int k = 0; k < nb; ++k) {
+                const int q0 = bq(k), q1 = bend(k);
+                CHECK(srow[size_t(k)] == row_of(k), "chunk %d: block %d has row %d", c, k, srow[size_t(k)]);
+                for (int s = q0; s < q1; s += UNIT_PIECE) CHECK(partial[size_t(s)] == 1, "chunk %d: piece %d of block %d", c, s, k);
+                written[size_t(k)]++;
+            }
+            for (int d = 0; d < total; ++d) CHECK(read[size_t(d)] == 1, "chunk %d: stage %d read %d times", c, d, read[size_t(d)]);
+            for (int k = 0; k < nb; ++k) CHECK(written[size_t(k)] == 1, "chunk %d: block %d written %d times", c, k, written[size_t(k)]);
         }
         std::printf("gs %d: %d chunks checked\n", gs, nc);
     }

@@ -14,6 +14,7 @@ import subprocess
 import numpy as np
 import pytest
 
+from meshes import star_mesh
 from polyfem_amd.mesh import kuhn_box
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -53,3 +54,16 @@ def test_plans_under_sanitizers(exe, tmp_path, grid, order):
     r = subprocess.run([exe, str(p)], capture_output=True, text=True, env=env, timeout=600)
     assert r.returncode == 0, (r.stdout[-2000:], r.stderr[-4000:])
     assert "OK (0 failures)" in r.stdout
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_plans_high_valence(exe, tmp_path, order):
+    """A centre vertex in 60 tets: blocks of up to 60 contributions, split
+    into pieces."""
+    m = star_mesh(30, order=order)
+    p = tmp_path / "mesh.bin"
+    write_mesh(p, m)
+    r = subprocess.run([exe, str(p)], capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, (r.stdout[-2000:], r.stderr[-4000:])
+    assert "OK (0 failures)" in r.stdout
+    assert "gs 64: " in r.stdout and "too many units" not in r.stdout.split("gs 64: ")[1].split("\n")[0]
