@@ -84,7 +84,7 @@ typedef enum { PFGPU_PATH_FUSED = 0, PFGPU_PATH_PLAIN = 1, PFGPU_PATH_FACTORED =
 
 typedef struct {
     int32_t order;            /* 1 (P1) or 2 (P2) */
-    int32_t quadrature_order; /* <= 0: max(2(p-1),1) (AssemblerUtils.cpp:225-228) */
+    int32_t quadrature_order; /* <= 0: max(2(p-1),1) (AssemblerUtils.cpp:225-228); 1 to 4 (3 and 4: plain path) */
     int32_t n_elements;
     int32_t n_nodes;             /* discretisation nodes; n_dofs = 3 n_nodes */
     const int32_t *elem_nodes;   /* n_elements x (4|10), PolyFEM local order */

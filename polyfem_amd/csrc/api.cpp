@@ -205,6 +205,21 @@ const char *pfgpu_last_error(void) { return g_last_error.c_str(); }
 
 namespace {
 
+// the quadrature rules of 1 and 4 points weigh every point alike; the fused
+// and factored paths take no other
+bool uniform_rule(const pfgpu_handle &h) { return h.n_quad == 1 || h.n_quad == 4; }
+
+// the fused and factored paths: NeoHookean and linear elasticity, rules of
+// 1 or 4 points
+void require_path(const pfgpu_handle &h, int32_t path)
+{
+    if (path == PFGPU_PATH_PLAIN) return;
+    if (h.material == PFGPU_SAINT_VENANT || h.material == PFGPU_FIXED_COROTATIONAL || h.material == PFGPU_MOONEY_RIVLIN)
+        fail(PFGPU_ERR_UNSUPPORTED, "the fused and factored paths cover NeoHookean and linear elasticity");
+    if (!uniform_rule(h))
+        fail(PFGPU_ERR_UNSUPPORTED, "the fused and factored paths take quadrature orders 1 and 2 (the plain path takes 3 and 4)");
+}
+
 // the handle of one mesh and material (pfgpu_create)
 std::unique_ptr<pfgpu_handle> create_handle(const pfgpu_mesh_desc *mesh, const pfgpu_material_desc *material,
                                             int32_t precision)
@@ -212,7 +227,7 @@ std::unique_ptr<pfgpu_handle> create_handle(const pfgpu_mesh_desc *mesh, const p
     require(mesh != nullptr && material != nullptr, "null mesh or material descriptor");
     const pfgpu_mesh_desc &m = *mesh;
     require(m.order == 1 || m.order == 2, "order must be 1 or 2");
-    require(m.quadrature_order <= 2, "quadrature orders above 2 are not provided");
+    require(m.quadrature_order <= 4, "quadrature orders above 4 are not provided");
     require(m.n_elements > 0 && m.n_nodes > 0 && m.n_vertices > 0, "empty mesh");
     require(m.elem_nodes && m.vertices && m.elem_vertices, "null mesh array");
     require(material->kind == PFGPU_NEOHOOKEAN || material->kind == PFGPU_LINEAR_ELASTICITY ||
@@ -226,12 +241,15 @@ std::unique_ptr<pfgpu_handle> create_handle(const pfgpu_mesh_desc *mesh, const p
     PF_HIP(hipGetDevice(&h->device));
     h->order = m.order;
     h->n_local = m.order == 1 ? 4 : 10;
+    // default max(2 (p - 1), 1) (AssemblerUtils.cpp:225-228); orders 1 and 2: 1 and 4 points, 3 and 4: the
+    // 6- and 11-point Xiao-Gimbutas rules (TetQuadrature.cpp:14-36)
     const int qo = m.quadrature_order <= 0 ? (m.order == 1 ? 1 : 2) : m.quadrature_order;
-    h->n_quad = qo == 1 ? 1 : 4;
+    h->n_quad = qo == 1 ? 1 : qo == 2 ? 4 : qo == 3 ? 6 : 11;
     h->material = material->kind;
     h->precision = precision;
-    h->path = material->kind == PFGPU_NEOHOOKEAN || material->kind == PFGPU_LINEAR_ELASTICITY ? PFGPU_PATH_FACTORED
-                                                                                             : PFGPU_PATH_PLAIN;
+    h->path = (material->kind == PFGPU_NEOHOOKEAN || material->kind == PFGPU_LINEAR_ELASTICITY) && uniform_rule(*h)
+                  ? PFGPU_PATH_FACTORED
+                  : PFGPU_PATH_PLAIN;
     h->n_elements = m.n_elements;
     h->n_nodes = m.n_nodes;
     h->n_vertices = m.n_vertices;
@@ -283,7 +301,7 @@ std::unique_ptr<pfgpu_handle> create_handle(const pfgpu_mesh_desc *mesh, const p
     h->red_out.reserve(sizeof(int) * 4);
     h->partial.reserve(sizeof(double) * size_t((int64_t(m.n_elements) + 63) / 64 + 1));
     h->loc.reserve(h->value_size() * size_t(m.n_elements) * size_t(h->n_local) * 3);
-    if (h->material == PFGPU_NEOHOOKEAN || h->material == PFGPU_LINEAR_ELASTICITY)
+    if ((h->material == PFGPU_NEOHOOKEAN || h->material == PFGPU_LINEAR_ELASTICITY) && uniform_rule(*h))
         h->fac_rec.reserve(h->value_size() * size_t(m.n_elements) * size_t((11 * h->n_quad + 6 + 3) & ~3));
     return h;
 }
@@ -348,9 +366,7 @@ pfgpu_status pfgpu_set_tuning(pfgpu_handle *h, int32_t path, int32_t gather_thre
                 "gather_threads must be 64, 128, 256 or 512");
         require(hessian_threads == 64 || hessian_threads == 128 || hessian_threads == 256,
                 "hessian_threads must be 64, 128 or 256");
-        if (path != PFGPU_PATH_PLAIN && (h->material == PFGPU_SAINT_VENANT || h->material == PFGPU_FIXED_COROTATIONAL ||
-                                         h->material == PFGPU_MOONEY_RIVLIN))
-            fail(PFGPU_ERR_UNSUPPORTED, "the fused and factored paths cover NeoHookean and linear elasticity");
+        require_path(*h, path);
         h->path = path;
         h->tune_gather_threads = gather_threads;
         h->tune_hessian_threads = hessian_threads;
@@ -460,9 +476,7 @@ pfgpu_status pfgpu_set_path(pfgpu_handle *h, int32_t path)
     return guard([&] {
         check_handle(h);
         require(path == PFGPU_PATH_FUSED || path == PFGPU_PATH_PLAIN || path == PFGPU_PATH_FACTORED, "unknown path");
-        if (path != PFGPU_PATH_PLAIN && (h->material == PFGPU_SAINT_VENANT || h->material == PFGPU_FIXED_COROTATIONAL ||
-                                         h->material == PFGPU_MOONEY_RIVLIN))
-            fail(PFGPU_ERR_UNSUPPORTED, "the fused and factored paths cover NeoHookean and linear elasticity");
+        require_path(*h, path);
         h->path = path;
     });
 }

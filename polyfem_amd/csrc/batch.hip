@@ -96,7 +96,7 @@ __global__ __launch_bounds__(BLOCK) void k_multi_energy(Groups<T> G, const doubl
             Elem<T, O, Q> el;
             load_geometry(gv.m, e, el);
             gather(gv.m, e, x + gv.dof0, el.U);
-            const T w = el.det * T(quad_weight(Q));
+            const T w = el.det * T(quad_weight(Q, 0));
             T acc = T(0);
 #pragma unroll
             for (int q = 0; q < Q; ++q) {
@@ -125,7 +125,7 @@ __global__ __launch_bounds__(BLOCK) void k_multi_gradient(Groups<T> G, const dou
         Elem<T, O, Q> el;
         load_geometry(gv.m, e, el);
         gather(gv.m, e, x + gv.dof0, el.U);
-        const T w = el.det * T(quad_weight(Q));
+        const T w = el.det * T(quad_weight(Q, 0));
         T g[NB][3];
 #pragma unroll
         for (int a = 0; a < NB; ++a) g[a][0] = g[a][1] = g[a][2] = T(0);
@@ -160,7 +160,7 @@ __global__ __launch_bounds__(BLOCK) void k_multi_hvp(Groups<T> G, const double *
         gather(gv.m, e, x + gv.dof0, el.U);
         T V[NB][3];
         gather(gv.m, e, v + gv.dof0, V);
-        const T w = el.det * T(quad_weight(Q));
+        const T w = el.det * T(quad_weight(Q, 0));
         T y[NB][3];
 #pragma unroll
         for (int a = 0; a < NB; ++a) y[a][0] = y[a][1] = y[a][2] = T(0);
@@ -195,7 +195,7 @@ __global__ __launch_bounds__(BLOCK) void k_multi_diag(Groups<T> G, const double 
         Elem<T, O, Q> el;
         load_geometry(gv.m, e, el);
         if (M != MAT_LE) gather(gv.m, e, x + gv.dof0, el.U);
-        const T w = el.det * T(quad_weight(Q));
+        const T w = el.det * T(quad_weight(Q, 0));
         T d6[NB][6];
 #pragma unroll
         for (int a = 0; a < NB; ++a)

@@ -35,13 +35,51 @@ template <int ORDER> struct NLocal { static constexpr int value = ORDER == 1 ? 4
 constexpr double HMS_A = 0.13819660112501053195188660538406111299991607666015625;
 constexpr double HMS_B = 0.58541019662496840414434018384781666100025177001953125;
 
+// Quadrature orders 3 and 4: the Xiao-Gimbutas rules of 6 and 11 points the
+// reference tabulates (autogen/auto_tetrahedron.ipp:13-24), rows (x, y, z,
+// w) in its order, weights summing to 1.
+constexpr double XG6[6][4] = {
+    {0.16200149169852446, 0.18385035049209775, 0.01271836631368145, 0.12232200275734507},
+    {0.010905212211189244, 0.2815238021235462, 0.3621268299455338, 0.12806641271074692},
+    {0.19011700243928392, 0.011403329444557169, 0.35862072046688387, 0.13256802714444524},
+    {0.170816925164989, 0.15281814309092734, 0.6384932999617267, 0.14062440966040324},
+    {0.15868516322744058, 0.5856628056552158, 0.13084716895209647, 0.22441516691755742},
+    {0.5712260521491151, 0.1469183900871696, 0.14037280579421071, 0.2520039808095023},
+};
+constexpr double XG11[11][4] = {
+    {0.04049050672759043, 0.013560701879802881, 0.7712547326953761, 0.03925109092483996},
+    {0.752508507009655, 0.06809937093820666, 0.09798720364927911, 0.0552733691559369},
+    {0.0672232948933834, 0.035183929773598716, 0.15636389323939529, 0.05539379887157637},
+    {0.419266313879513, 0.04778143555908666, 0.47961101102565507, 0.05993318514655953},
+    {0.45076587609127683, 0.05945661629943383, 0.05682401712793367, 0.06946996593763537},
+    {0.1294113737889104, 0.3301904148374645, 0.0023910074574393647, 0.07616271524555836},
+    {0.1215419913339278, 0.3064939884296903, 0.5629727601430461, 0.07942668006802531},
+    {0.09720464458758327, 0.68439041545304, 0.1118007673973831, 0.1064680341554901},
+    {0.029569495206479612, 0.3179035602133946, 0.32329398483747895, 0.11023423242849766},
+    {0.43271023904776856, 0.3538232392092971, 0.10962240533194123, 0.15497611601624608},
+    {0.24027666492807262, 0.12680172591539202, 0.32847320672203845, 0.19341081204963445},
+};
+
+// points of the rule with nq points (1, 4, 6, 11)
 __host__ __device__ constexpr double quad_point(int nq, int q, int k)
 {
-    return nq == 1 ? 0.25 : (q == 3 ? HMS_A : (k == 2 - q ? HMS_B : HMS_A));
+    return nq == 1    ? 0.25
+           : nq == 4  ? (q == 3 ? HMS_A : (k == 2 - q ? HMS_B : HMS_A))
+           : nq == 6  ? XG6[q][k]
+                      : XG11[q][k];
 }
 
 // weights sum to 1 and are divided by 6 (TetQuadrature.cpp:53)
-__host__ __device__ constexpr double quad_weight(int nq) { return nq == 1 ? 1.0 / 6.0 : 0.25 / 6.0; }
+__host__ __device__ constexpr double quad_weight(int nq, int q)
+{
+    return nq == 1 ? 1.0 / 6.0 : nq == 4 ? 0.25 / 6.0 : nq == 6 ? XG6[q][3] / 6.0 : XG11[q][3] / 6.0;
+}
+// the rules of 1 and 4 points weigh every point alike: the element kernels of
+// the factored and fused paths (and the batch) fold the weight into
+// per-element factors and take these rules only
+__host__ __device__ constexpr bool quad_uniform(int nq) { return nq == 1 || nq == 4; }
+// points of the rule of the given order (orders 1 to 4)
+constexpr int quad_points_of_order(int order) { return order <= 1 ? 1 : order == 2 ? 4 : order == 3 ? 6 : 11; }
 
 // d phi_a / d xi_k of the P1/P2 Lagrange tet (autogen/auto_p_bases.cpp:1100-1123, 1260-1330)
 template <int ORDER> __host__ __device__ constexpr double ref_grad(double x, double y, double z, int a, int k)

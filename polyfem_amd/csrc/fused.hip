@@ -113,7 +113,7 @@ __device__ __forceinline__ void setup_point(const DevMesh<T> &m, int e, const in
     for (int a = 0; a < NB; ++a)
 #pragma unroll
         for (int d = 0; d < 3; ++d) el.U[a][d] = T(x[3 * int64_t(nodes[a]) + d] - x[3 * int64_t(nodes[0]) + d]);
-    const T w = el.det * T(quad_weight(NQ));
+    const T w = el.det * T(quad_weight(NQ, 0));
     Kin<T> k;
     field_grad<T, ORDER, NQ>(Q, el.U, el.jit, k.H);
     if (MAT == MAT_NH) kinematics(k);
@@ -496,7 +496,7 @@ void ensure_fused_tables(Handle &h)
 {
     if (h.fused_ready) return;
     FusedShape s{};
-    dispatch(h, [&]<typename T, int O, int Q, int M>() { s = shape_for<T, O, Q>(); });
+    dispatch<true>(h, [&]<typename T, int O, int Q, int M>() { s = shape_for<T, O, Q>(); });
     FusedTables t = build_fused_tables(h.pat, h.n_local, h.h_elem_nodes.data(), h.n_elements, h.h_vertices.data(),
                                        h.h_elem_vertices.data(), s);
     h.patch_desc = upload(t.desc, h.stream);
@@ -538,7 +538,7 @@ void fused_energy_gradient_hessian(Handle &h, const double *x, double *energy, d
         h.loc.reserve(sizeof(double) * size_t(h.n_dofs));
         gv = h.loc.as<double>();
     }
-    dispatch(h, [&]<typename T, int O, int Q, int M>() {
+    dispatch<true>(h, [&]<typename T, int O, int Q, int M>() {
         if constexpr (M == MAT_NH || M == MAT_LE) launch_egh<T, O, Q, M>(h, x, values, gv, energy);
     });
 }

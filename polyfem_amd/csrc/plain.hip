@@ -28,28 +28,11 @@ __device__ __forceinline__ double block_sum(double v)
 }
 
 // reference tables in constant memory, one per (value type, order, points)
-__constant__ RefTab<float, 1, 1> c_rf11 = RefTab<float, 1, 1>();
-__constant__ RefTab<float, 1, 4> c_rf14 = RefTab<float, 1, 4>();
-__constant__ RefTab<float, 2, 1> c_rf21 = RefTab<float, 2, 1>();
-__constant__ RefTab<float, 2, 4> c_rf24 = RefTab<float, 2, 4>();
-__constant__ RefTab<double, 1, 1> c_rd11 = RefTab<double, 1, 1>();
-__constant__ RefTab<double, 1, 4> c_rd14 = RefTab<double, 1, 4>();
-__constant__ RefTab<double, 2, 1> c_rd21 = RefTab<double, 2, 1>();
-__constant__ RefTab<double, 2, 4> c_rd24 = RefTab<double, 2, 4>();
+template <typename T, int ORDER, int NQ> __constant__ RefTab<T, ORDER, NQ> c_ref = RefTab<T, ORDER, NQ>();
 
 template <typename T, int ORDER, int NQ> __device__ __forceinline__ const RefTab<T, ORDER, NQ> &ref_tab()
 {
-    if constexpr (sizeof(T) == 4) {
-        if constexpr (ORDER == 1 && NQ == 1) return c_rf11;
-        else if constexpr (ORDER == 1) return c_rf14;
-        else if constexpr (NQ == 1) return c_rf21;
-        else return c_rf24;
-    } else {
-        if constexpr (ORDER == 1 && NQ == 1) return c_rd11;
-        else if constexpr (ORDER == 1) return c_rd14;
-        else if constexpr (NQ == 1) return c_rd21;
-        else return c_rd24;
-    }
+    return c_ref<T, ORDER, NQ>;
 }
 
 // ------------------------------------------------------------ element kernels
@@ -83,10 +66,10 @@ __global__ __launch_bounds__(BLOCK) void k_energy(DevMesh<T> m, const double *__
         Elem<T, ORDER, NQ> el;
         load_geometry(m, e, el);
         gather(m, e, x, el.U);
-        const T w = el.det * T(quad_weight(NQ));
         T acc = T(0);
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
+            const T w = el.det * T(quad_weight(NQ, q));
             Kin<T> k;
             field_grad<T, ORDER, NQ>(q, el.U, el.jit, k.H);
             kinematics(k);
@@ -111,10 +94,10 @@ __global__ __launch_bounds__(BLOCK) void k_energy_lame_grad(DevMesh<T> m, const 
     Elem<T, ORDER, NQ> el;
     load_geometry(m, e, el);
     gather(m, e, x, el.U);
-    const T w = el.det * T(quad_weight(NQ));
     T al = T(0), am = T(0);
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
+        const T w = el.det * T(quad_weight(NQ, q));
         Kin<T> k;
         field_grad<T, ORDER, NQ>(q, el.U, el.jit, k.H);
         kinematics(k);
@@ -137,7 +120,6 @@ __global__ __launch_bounds__(BLOCK) void k_gradient(DevMesh<T> m, const double *
         Elem<T, ORDER, NQ> el;
         load_geometry(m, e, el);
         gather(m, e, x, el.U);
-        const T w = el.det * T(quad_weight(NQ));
         T g[NB][3];
 #pragma unroll
         for (int a = 0; a < NB; ++a)
@@ -146,6 +128,7 @@ __global__ __launch_bounds__(BLOCK) void k_gradient(DevMesh<T> m, const double *
         T acc = T(0);
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
+            const T w = el.det * T(quad_weight(NQ, q));
             Kin<T> k;
             field_grad<T, ORDER, NQ>(q, el.U, el.jit, k.H);
             kinematics(k);
@@ -185,7 +168,8 @@ __global__ __launch_bounds__(64) void k_grad_factors(DevMesh<T> m, const double 
         Elem<T, ORDER, NQ> el;
         load_geometry(m, e, el);
         gather(m, e, x, el.U);
-        const T w = el.det * T(quad_weight(NQ));
+        // a rule that weighs every point alike (the factored path takes no other)
+        const T w = el.det * T(quad_weight(NQ, 0));
         T g[NB][3];
 #pragma unroll
         for (int a = 0; a < NB; ++a)
@@ -251,7 +235,8 @@ __global__ __launch_bounds__(HB) void k_hessian(DevMesh<T> m, const double *__re
     Elem<T, ORDER, NQ> el;
     load_geometry(m, ee, el);
     if (MAT != MAT_LE) gather(m, ee, x, el.U);
-    const T w = el.det * T(quad_weight(NQ));
+    // the weight of the rules that weigh every point alike (isotropic term)
+    const T w0 = el.det * T(quad_weight(NQ, 0));
     // per point: M = F^{-T} J^{-T} (k_a = M RG(q,a)) and the two curvature
     // weights; the k vectors are formed per block row and column to keep the
     // register footprint small.  SaintVenant adds the S-weighted metric and
@@ -264,6 +249,7 @@ __global__ __launch_bounds__(HB) void k_hessian(DevMesh<T> m, const double *__re
     T Mq[NM][9], cl[NM], cc[NM], sg6[NX][6], ff6[NX][9];
 #pragma unroll
     for (int q = 0; q < NM; ++q) {
+        const T w = el.det * T(quad_weight(NQ, q));
         if constexpr (MAT == MAT_MR) {
 #pragma unroll
             for (int i = 0; i < 9; ++i) Mq[q][i] = T(0);
@@ -287,10 +273,25 @@ __global__ __launch_bounds__(HB) void k_hessian(DevMesh<T> m, const double *__re
     }
     T g6[6];
     metric(el.jit, g6);
-    const T cm = (MAT == MAT_FC ? T(2) : T(1)) * w * el.mu;
+    const T cm = (MAT == MAT_FC ? T(2) : T(1)) * w0 * el.mu;
     T *st = stage[wv];
     T *out = hloc + tile * NBLK * 576;
     const RefTab<T, ORDER, NQ> &tb = ref_tab<T, ORDER, NQ>();
+    // the isotropic term sum_q w_q cm' RG(q,a)^T G RG(q,b): from the summed
+    // table for the rules that weigh every point alike, point by point otherwise
+    auto iso = [&](int a, int b, const T *c) {
+        if constexpr (quad_uniform(NQ)) {
+            return cm * (g6[0] * c[0] + g6[1] * c[1] + g6[2] * c[2] + g6[3] * c[3] + g6[4] * c[4] + g6[5] * c[5]);
+        } else {
+            T s = T(0);
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) {
+                const T *ra = tb.rg + (q * NB + a) * 3, *rb = tb.rg + (q * NB + b) * 3;
+                s += (MAT == MAT_FC ? T(2) : T(1)) * el.det * T(quad_weight(NQ, q)) * el.mu * sym_form(ra, g6, rb);
+            }
+            return s;
+        }
+    };
     // a and b are uniform over the wavefront: the table reads are scalar
 #pragma unroll 1
     for (int a = 0; a < NB; ++a) {
@@ -319,9 +320,10 @@ __global__ __launch_bounds__(HB) void k_hessian(DevMesh<T> m, const double *__re
             T xr[9] = {T(0), T(0), T(0), T(0), T(0), T(0), T(0), T(0), T(0)};
             if constexpr (MAT == MAT_FC) {
                 // -2 mu da sum_q (G_a x r_m)^T K^{-1} (G_b x r_n), r_m the rows of R
-                gd = cm * (g6[0] * c[0] + g6[1] * c[1] + g6[2] * c[2] + g6[3] * c[3] + g6[4] * c[4] + g6[5] * c[5]);
+                gd = iso(a, b, c);
 #pragma unroll
                 for (int q = 0; q < NQ; ++q) {
+                    const T w = el.det * T(quad_weight(NQ, q));
                     const T *ra = tb.rg + (q * NB + a) * 3, *rb = tb.rg + (q * NB + b) * 3;
                     T Ga[3], Gb[3];
 #pragma unroll
@@ -355,6 +357,7 @@ __global__ __launch_bounds__(HB) void k_hessian(DevMesh<T> m, const double *__re
                 // kinematics of each point are formed again per block)
 #pragma unroll
                 for (int q = 0; q < NQ; ++q) {
+                    const T w = el.det * T(quad_weight(NQ, q));
                     Kin<T> k;
                     field_grad<T, ORDER, NQ>(q, el.U, el.jit, k.H);
                     kinematics(k);
@@ -387,7 +390,7 @@ __global__ __launch_bounds__(HB) void k_hessian(DevMesh<T> m, const double *__re
                     for (int i = 0; i < 6; ++i) xf[i] += gq * ff6[q][i];
                 }
             } else {
-                gd = cm * (g6[0] * c[0] + g6[1] * c[1] + g6[2] * c[2] + g6[3] * c[3] + g6[4] * c[4] + g6[5] * c[5]);
+                gd = iso(a, b, c);
             }
             const int sym[9] = {0, 3, 4, 3, 1, 5, 4, 5, 2};
 #pragma unroll
@@ -426,7 +429,6 @@ __global__ __launch_bounds__(64) void k_hvp(DevMesh<T> m, const double *__restri
         gather(m, e, x, el.U);
         T V[NB][3];
         gather(m, e, v, V);
-        const T w = el.det * T(quad_weight(NQ));
         T y[NB][3];
 #pragma unroll
         for (int a = 0; a < NB; ++a)
@@ -434,6 +436,7 @@ __global__ __launch_bounds__(64) void k_hvp(DevMesh<T> m, const double *__restri
             for (int d = 0; d < 3; ++d) y[a][d] = T(0);
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
+            const T w = el.det * T(quad_weight(NQ, q));
             Kin<T> k;
             field_grad<T, ORDER, NQ>(q, el.U, el.jit, k.H);
             if (MAT == MAT_NH || MAT == MAT_FC || MAT == MAT_MR) kinematics(k);
@@ -552,7 +555,6 @@ __global__ __launch_bounds__(BLOCK) void k_material_jvp(DevMesh<T> m, const doub
     Elem<T, ORDER, NQ> el;
     load_geometry(m, e, el);
     gather(m, e, x, el.U);
-    const T w = el.det * T(quad_weight(NQ));
     const T sl = T(dl[e]), sm = T(dm[e]);
     T y[NB][3];
 #pragma unroll
@@ -561,6 +563,7 @@ __global__ __launch_bounds__(BLOCK) void k_material_jvp(DevMesh<T> m, const doub
         for (int d = 0; d < 3; ++d) y[a][d] = T(0);
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
+        const T w = el.det * T(quad_weight(NQ, q));
         Kin<T> k;
         field_grad<T, ORDER, NQ>(q, el.U, el.jit, k.H);
         T A[9];
@@ -614,10 +617,10 @@ __global__ __launch_bounds__(BLOCK) void k_material_vjp(DevMesh<T> m, const doub
     gather(m, e, x, el.U);
     T Pn[NB][3];
     gather(m, e, p, Pn);
-    const T w = el.det * T(quad_weight(NQ));
     T tl = T(0), tm = T(0);
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
+        const T w = el.det * T(quad_weight(NQ, q));
         Kin<T> k;
         field_grad<T, ORDER, NQ>(q, el.U, el.jit, k.H);
         T Gp[9];
@@ -681,7 +684,6 @@ __global__ __launch_bounds__(BLOCK) void k_shape_vjp(DevMesh<T> m, const double 
     gather(m, e, x, el.U);
     T Pn[NB][3];
     gather(m, e, p, Pn);
-    const T w = el.det * T(quad_weight(NQ));
     T s[4][3];
 #pragma unroll
     for (int v = 0; v < 4; ++v)
@@ -689,6 +691,7 @@ __global__ __launch_bounds__(BLOCK) void k_shape_vjp(DevMesh<T> m, const double 
         for (int d = 0; d < 3; ++d) s[v][d] = T(0);
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
+        const T w = el.det * T(quad_weight(NQ, q));
         Kin<T> k;
         field_grad<T, ORDER, NQ>(q, el.U, el.jit, k.H);
         if (MAT == MAT_NH || MAT == MAT_FC || MAT == MAT_MR) kinematics(k);
@@ -935,7 +938,8 @@ __global__ __launch_bounds__(64) void k_hfactors(DevMesh<T> m, const double *__r
         Elem<T, ORDER, NQ> el;
         load_geometry(m, e, el);
         if (MAT != MAT_LE) gather(m, e, x, el.U);
-        const T w = el.det * T(quad_weight(NQ));
+        // a rule that weighs every point alike (the factored path takes no other)
+        const T w = el.det * T(quad_weight(NQ, 0));
         T r[R];
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
@@ -1077,14 +1081,14 @@ __global__ __launch_bounds__(GS) void k_gather_factored(int n_chunks, const int3
 // unit order).  The block sums are then split into pieces of at most
 // UNIT_PIECE contributions (plans.hpp); thread t sums its run of pieces,
 // lanes[c (GS + 1) + t] .. lanes[c (GS + 1) + t + 1] (balanced by the plan),
-// each piece's sum over the piece's first staged contribution.  After a
-// barrier thread t takes blocks t, t + GS, ...: it adds the block's piece sums
-// in piece order and stores the block into the CSR values (3 runs of 3 values
-// per block, consecutive blocks of a row contiguous, so the stores of
-// consecutive threads coalesce).  One thread per (row, component) sums the
-// local gradient rows of the row's units in FP64 when GRAD.  Summation orders
-// are fixed by the tables and do not depend on GS: results are bitwise
-// reproducible.
+// storing a block that is one piece straight into the CSR values (3 runs of
+// 3 values per block, consecutive blocks of a row contiguous) and the sum of
+// a piece of a split block over the piece's first staged contribution; after
+// a barrier the thread that summed the first piece of a split block adds the
+// piece sums in piece order and stores the block.  One thread per (row,
+// component) sums the local gradient rows of the row's units in FP64 when
+// GRAD.  Summation orders are fixed by the tables and do not depend on GS:
+// results are bitwise reproducible.
 //
 //   H_e(a, b) = (sum_q RG(q,a)^T G RG(q,b)) I + sum_q [cl_q k_a k_b^T + cc_q k_b k_a^T],
 //   k_a = M_q RG(q, a),  G = w mu J^{-1} J^{-T} (the record's metric tail)
@@ -1118,7 +1122,6 @@ __global__ __launch_bounds__(GS) void k_gather_units(int n_chunks, int n_words, 
     __shared__ T srg[NQ * NB * 3];
     __shared__ int32_t srb[GS + 1], sru[GS + 1];
     __shared__ uint32_t sq[NWMAX];
-    __shared__ uint8_t srow[GS * NB];
     extern __shared__ __align__(16) unsigned char smem[];
     T *sv = reinterpret_cast<T *>(smem); // GS NB staged blocks of 9 values
     T *sg = sv + GS * NB * 9;            // GS local gradient rows
@@ -1274,16 +1277,26 @@ __global__ __launch_bounds__(GS) void k_gather_units(int n_chunks, int n_words, 
                 if (t + j * GS < nw) sq[t + j * GS] = u0.qw[j];
         }
         __syncthreads();
-        // ---- chunk c: this thread's run of pieces, each piece's sum over its
-        // first staged contribution; the row of every block it starts
+        // ---- chunk c: this thread's run of pieces
         const uint16_t *sq16 = reinterpret_cast<const uint16_t *>(sq) + (m0.B0 & 1);
         const int total = m0.nu * NB;
         auto block_end = [&](int k) { return k + 1 < m0.nb ? int(sq16[k + 1]) : total; };
+        // block k of the row whose blocks are rs .. re - 1 (chunk-local)
+        auto write_block = [&](int k, int rs, int re, const T(&acc)[9]) {
+            const int deg = re - rs;
+            T *out = values + 9 * int64_t(m0.B0 + rs) + 3 * (k - rs);
+#pragma unroll
+            for (int mm = 0; mm < 3; ++mm)
+#pragma unroll
+                for (int nn = 0; nn < 3; ++nn) out[mm * 3 * deg + nn] = acc[mm * 3 + nn];
+        };
+        // first pieces of split blocks: block, its row's block range
+        int nh = 0, hk0 = 0, hs0 = 0, he0 = 0, hk1 = 0, hs1 = 0, he1 = 0;
         {
             int k = int(u0.lane0 & 0xfffu), j = int((u0.lane0 >> 12) & 0xffu), rr = int(u0.lane0 >> 20);
             const int ke = int(u0.lane1 & 0xfffu), je = int((u0.lane1 >> 12) & 0xffu);
             if ((k != ke || j != je) && k < m0.nb) {
-                int re = srb[rr + 1];
+                int rs = srb[rr], re = srb[rr + 1];
                 int q0 = sq16[k], q1 = block_end(k), qn = block_end(k + 1);
                 while (true) {
                     const int s = q0 + j * L, pe = min(q1, s + L);
@@ -1305,11 +1318,27 @@ __global__ __launch_bounds__(GS) void k_gather_units(int n_chunks, int n_words, 
                     if (x < pe)
 #pragma unroll
                         for (int i = 0; i < 9; ++i) acc[i] += sv[x * 9 + i];
-                    if (s < pe) // (a block without contributions has an empty piece)
+                    const bool whole = pe == q1;
+                    if (j == 0 && whole) {
+                        write_block(k, rs, re, acc);
+                    } else {
+                        // a piece of a split block: its sum over its first contribution
 #pragma unroll
                         for (int i = 0; i < 9; ++i) sv[s * 9 + i] = acc[i];
-                    if (j == 0) srow[k] = uint8_t(rr);
-                    if (pe == q1) {
+                        if (j == 0) {
+                            if (nh == 0) {
+                                hk0 = k;
+                                hs0 = rs;
+                                he0 = re;
+                            } else {
+                                hk1 = k;
+                                hs1 = rs;
+                                he1 = re;
+                            }
+                            ++nh;
+                        }
+                    }
+                    if (whole) {
                         ++k;
                         j = 0;
                     } else {
@@ -1317,7 +1346,10 @@ __global__ __launch_bounds__(GS) void k_gather_units(int n_chunks, int n_words, 
                     }
                     if ((k == ke && j == je) || k >= m0.nb) break;
                     if (j == 0) {
-                        while (k >= re) re = srb[++rr + 1];
+                        while (k >= re) {
+                            rs = re;
+                            re = srb[++rr + 1];
+                        }
                         q0 = q1;
                         q1 = qn;
                         qn = block_end(k + 1);
@@ -1345,28 +1377,17 @@ __global__ __launch_bounds__(GS) void k_gather_units(int n_chunks, int n_words, 
             }
         }
         __syncthreads();
-        // ---- chunk c: one thread per block (consecutive threads, consecutive
-        // blocks: the stores of a row's run coalesce): the piece sums in piece
-        // order, into the CSR values
-        for (int k = t; k < m0.nb; k += GS) {
-            const int q0 = sq16[k], q1 = block_end(k), rr = srow[k];
-            const int rs = srb[rr], deg = srb[rr + 1] - rs;
+        // ---- chunk c: split blocks, their piece sums in piece order
+        for (int hh = 0; hh < nh; ++hh) {
+            const int k = hh ? hk1 : hk0, rs = hh ? hs1 : hs0, re = hh ? he1 : he0;
+            const int q0 = sq16[k], q1 = block_end(k);
             T acc[9];
-            if (q0 < q1) {
 #pragma unroll
-                for (int i = 0; i < 9; ++i) acc[i] = sv[q0 * 9 + i];
-            } else {
-#pragma unroll
-                for (int i = 0; i < 9; ++i) acc[i] = T(0); // a block without contributions
-            }
+            for (int i = 0; i < 9; ++i) acc[i] = sv[q0 * 9 + i];
             for (int s = q0 + L; s < q1; s += L)
 #pragma unroll
                 for (int i = 0; i < 9; ++i) acc[i] += sv[s * 9 + i];
-            T *out = values + 9 * int64_t(m0.B0 + rs) + 3 * (k - rs);
-#pragma unroll
-            for (int mm = 0; mm < 3; ++mm)
-#pragma unroll
-                for (int nn = 0; nn < 3; ++nn) out[mm * 3 * deg + nn] = acc[mm * 3 + nn];
+            write_block(k, rs, re, acc);
         }
         if (c1 >= c_end) break;
         __syncthreads(); // the stage and tables are read before chunk c1 overwrites them
@@ -1764,10 +1785,9 @@ bool unit_plan_ready(Handle &h)
     if (gs > 256) return false;
     // the staged blocks must fit the 160 KB of LDS (FP64 P2 at 256 threads
     // does not): the largest fitting size at or below the tuned one
-    // dynamic stage + the static tables (row tables, block starts, block rows, reference gradients)
+    // dynamic stage + the static tables (row tables, block starts, reference gradients)
     auto lds = [&](int g) {
-        return int64_t(h.value_size()) * g * (h.n_local * 9 + 3) + 8 * (g + 1) + 4 * (g * h.n_local / 2 + 2) +
-               int64_t(g) * h.n_local + 1024;
+        return int64_t(h.value_size()) * g * (h.n_local * 9 + 3) + 8 * (g + 1) + 4 * (g * h.n_local / 2 + 2) + 1024;
     };
     while (gs > 64 && lds(gs) > 160 * 1024) gs /= 2;
     if (lds(gs) > 160 * 1024) return false;
@@ -1778,7 +1798,7 @@ void factored_hessian(Handle &h, const double *x, void *values)
 {
     const bool units = unit_plan_ready(h);
     GatherPlan *p = units ? nullptr : &factored_plan(h);
-    dispatch(h, [&]<typename T, int O, int Q, int M>() {
+    dispatch<true>(h, [&]<typename T, int O, int Q, int M>() {
         if constexpr (M == MAT_NH || M == MAT_LE) {
             h.fac_rec.reserve(sizeof(T) * size_t(h.n_elements) * fac_size(Q));
             k_hfactors<T, O, Q, M><<<grid_for(h.n_elements, 64), 64, 0, h.stream>>>(h.mesh<T>(), x, h.fac_rec.as<T>());
@@ -1800,7 +1820,7 @@ void factored_energy_gradient_hessian(Handle &h, const double *x, double *energy
     const unsigned grid = grid_for(h.n_elements, 64);
     h.loc.reserve(h.value_size() * size_t(h.n_elements) * h.n_local * 3);
     if (energy) h.partial.reserve(sizeof(double) * grid);
-    dispatch(h, [&]<typename T, int O, int Q, int M>() {
+    dispatch<true>(h, [&]<typename T, int O, int Q, int M>() {
         if constexpr (M == MAT_NH || M == MAT_LE) {
             h.fac_rec.reserve(sizeof(T) * size_t(h.n_elements) * fac_size(Q));
             k_grad_factors<T, O, Q, M><<<grid, 64, 0, h.stream>>>(
@@ -1825,7 +1845,7 @@ void factored_node_diagonal(Handle &h, const double *x, void *diag)
     if (h.material != MAT_NH && h.material != MAT_LE)
         fail(PFGPU_ERR_UNSUPPORTED, "the factored Hessian covers NeoHookean and linear elasticity");
     ensure_row_chunks(h);
-    dispatch(h, [&]<typename T, int O, int Q, int M>() {
+    dispatch<true>(h, [&]<typename T, int O, int Q, int M>() {
         if constexpr (M == MAT_NH || M == MAT_LE) {
             h.fac_rec.reserve(sizeof(T) * size_t(h.n_elements) * fac_size(Q));
             k_hfactors<T, O, Q, M><<<grid_for(h.n_elements, 64), 64, 0, h.stream>>>(h.mesh<T>(), x, h.fac_rec.as<T>());
@@ -1909,7 +1929,7 @@ void plain_hvp(Handle &h, const double *x, const double *v, double *y)
 void factored_hvp(Handle &h, const double *v, double *y)
 {
     const unsigned grid = grid_for(h.n_elements, 64);
-    dispatch(h, [&]<typename T, int O, int Q, int M>() {
+    dispatch<true>(h, [&]<typename T, int O, int Q, int M>() {
         if constexpr (M == MAT_NH || M == MAT_LE) {
             k_hvp_factored<T, O, Q><<<grid, 64, 0, h.stream>>>(h.mesh<T>(), h.fac_rec.as<T>(), v, h.loc.as<T>(), h.skip);
             PF_LAUNCH_CHECK();

@@ -12,27 +12,31 @@ namespace pfgpu {
 namespace {
 
 // a piece of a chunk's block sums: block (chunk-local), piece within the
-// block, the block's row (chunk-local), contributions
+// block, the block's row (chunk-local), contributions, first piece of a
+// split block
 struct Piece {
     int32_t block, piece, row, len;
+    bool head;
 };
-// the cost of a piece in contributions read: its reads plus the staging of
-// its sum
-constexpr int PIECE_COST = 2;
+// the cost of a piece in contributions read: its reads plus the storing of
+// its sum (one block, to the values or to the stage)
+constexpr int PIECE_COST = 3;
 
-// greedy runs of consecutive pieces of at most bound cost: the number of
-// runs (and their first pieces)
+// greedy runs of consecutive pieces of at most bound cost and at most two
+// heads each: the number of runs (and their first pieces)
 int piece_runs(const std::vector<Piece> &p, int bound, std::vector<int> *first)
 {
-    int n = 0, cur = 0;
+    int n = 0, cur = 0, heads = 0;
     for (size_t i = 0; i < p.size(); ++i) {
         const int c = p[i].len + PIECE_COST;
-        if (n == 0 || cur + c > bound) {
+        if (n == 0 || cur + c > bound || (p[i].head && heads == 2)) {
             ++n;
             cur = 0;
+            heads = 0;
             if (first) first->push_back(int(i));
         }
         cur += c;
+        heads += p[i].head ? 1 : 0;
     }
     return n;
 }
@@ -134,7 +138,7 @@ bool make_unit_plan(const Pattern &pat, int n_local, const int32_t *elem_nodes, 
                     const int n = (k + 1 < nb ? int(q0[size_t(blk) + 1]) : run) - start;
                     const int pc = n <= UNIT_PIECE ? 1 : (n + UNIT_PIECE - 1) / UNIT_PIECE;
                     for (int j = 0; j < pc; ++j)
-                        pieces.push_back(Piece{k, j, i - i0, std::min(UNIT_PIECE, n - j * UNIT_PIECE)});
+                        pieces.push_back(Piece{k, j, i - i0, std::min(UNIT_PIECE, n - j * UNIT_PIECE), j == 0 && pc > 1});
                 }
             chunk_lanes(pieces, nb, i1 - i0, gs, lanes.data() + size_t(ci) * size_t(gs + 1));
         }

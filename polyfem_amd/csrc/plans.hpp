@@ -26,7 +26,8 @@ namespace pfgpu {
 // thread t in chunk c (threads take consecutive runs of pieces, balanced by
 // contributions; entry gs is the end), packed by unit_lane(): the block
 // (chunk-local), the piece within the block and the block's row
-// (chunk-local).  False when a row has more than min(gs, 255) units.
+// (chunk-local).  A thread's run holds the first piece of at most two split
+// blocks.  False when a row has more than min(gs, 255) units.
 constexpr int UNIT_PIECE = 8;
 constexpr uint32_t unit_lane(int block, int piece, int row)
 {

@@ -80,7 +80,7 @@ static void stage_point(const Mesh &m, int e, int q, double *s)
     constexpr int NB = NLocal<ORDER>::value;
     double jit[9], det;
     geometry(m, e, jit, det);
-    const double w = det * quad_weight(NQ), lam = m.lam[e], mu = m.mu[e];
+    const double w = det * quad_weight(NQ, 0), lam = m.lam[e], mu = m.mu[e];
     double A[9] = {0};
     for (int a = 0; a < NB; ++a)
         for (int k = 0; k < 3; ++k)

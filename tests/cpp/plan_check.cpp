@@ -14,11 +14,10 @@
 //     each block's count equals a brute-force count of the row's elements
 //     that contain the column node;
 //  4. the threads' runs of pieces, walked exactly as the kernel walks them,
-//     read every staged contribution once, stage every non-empty piece's sum
-//     over its first contribution and record the row of every block they
-//     start, staying inside the stage and the chunk's blocks and rows; the
-//     per-block pass then finds every piece sum of every block (pieces of
-//     UNIT_PIECE, in order) where the runs left it.
+//     read every staged contribution once, write every block of the chunk
+//     once (whole, or through the piece sums of a split block: pieces of
+//     UNIT_PIECE, in order), stay inside the stage, the chunk's blocks and
+//     rows, and hold at most two first pieces of split blocks.
 // Input: a binary mesh file (order, E, N, Nv, elem_nodes, elem_vertices,
 // vertices) written by tests/test_plans.py.
 #include <algorithm>
@@ -133,24 +132,29 @@ int main(int argc, char **argv)
             auto bq = [&](int k) { return int(u.bq0[size_t(B0 + k)]); };
             auto bend = [&](int k) { return k + 1 < nb ? bq(k + 1) : total; };
             auto row_of = [&](int k) { return int(std::upper_bound(srb.begin(), srb.end(), k) - srb.begin()) - 1; };
-            std::vector<int> srow(size_t(nb), -1);
+            std::vector<std::pair<int, int>> heads;
             for (int t = 0; t < gs; ++t) {
                 int k = int(ln[t] & 0xfffu), j = int((ln[t] >> 12) & 0xffu), rr = int(ln[t] >> 20);
                 const int ke = int(ln[t + 1] & 0xfffu), je = int((ln[t + 1] >> 12) & 0xffu);
                 if (!((k != ke || j != je) && k < nb)) continue;
                 CHECK(rr >= 0 && rr < nr && rr == row_of(k), "chunk %d lane %d: row %d of block %d", c, t, rr, k);
-                int q0 = bq(k), q1 = bend(k), rend = srb[size_t(rr) + 1], steps = 0;
+                int q0 = bq(k), q1 = bend(k), rend = srb[size_t(rr) + 1], nh = 0, steps = 0;
                 while (true) {
                     const int s = q0 + j * UNIT_PIECE, pe = std::min(q1, s + UNIT_PIECE);
                     CHECK(s >= 0 && pe <= total && (s < pe || (j == 0 && q0 == q1)), "chunk %d lane %d: piece [%d, %d)", c, t, s, pe);
                     for (int x = s; x < pe && x < total; ++x) read[size_t(x)]++;
-                    if (s < pe && s < total) partial[size_t(s)]++;
+                    const bool whole = pe == q1;
                     CHECK(rr == row_of(k), "chunk %d lane %d: block %d tracked in row %d", c, t, k, rr);
-                    if (j == 0) {
-                        CHECK(srow[size_t(k)] == -1, "chunk %d: block %d started twice", c, k);
-                        srow[size_t(k)] = rr;
+                    if (j == 0 && whole) {
+                        written[size_t(k)]++;
+                    } else {
+                        if (s < total) partial[size_t(s)]++;
+                        if (j == 0) {
+                            heads.emplace_back(k, rr);
+                            ++nh;
+                        }
                     }
-                    if (pe == q1) {
+                    if (whole) {
                         ++k;
                         j = 0;
                     } else {
@@ -163,11 +167,11 @@ int main(int argc, char **argv)
                         q1 = bend(k);
                     }
                 }
+                CHECK(nh <= 2, "chunk %d lane %d: %d split-block heads", c, t, nh);
             }
-            // the per-block pass
-            for (int k = 0; k < nb; ++k) {
-                const int q0 = bq(k), q1 = bend(k);
-                CHECK(srow[size_t(k)] == row_of(k), "chunk %d: block %d has row %d", c, k, srow[size_t(k)]);
+            for (const auto &hd : heads) {
+                const int k = hd.first, q0 = bq(k), q1 = bend(k);
+                CHECK(q1 - q0 > UNIT_PIECE, "chunk %d: block %d of %d contributions split", c, k, q1 - q0);
                 for (int s = q0; s < q1; s += UNIT_PIECE) CHECK(partial[size_t(s)] == 1, "chunk %d: piece %d of block %d", c, s, k);
                 written[size_t(k)]++;
             }

@@ -90,9 +90,13 @@ def test_p2_nodal_interpolation():
 
 # ----------------------------------------------------------- quadrature
 
-@pytest.mark.parametrize("qorder,deg", [(1, 1), (2, 2)])
+@pytest.mark.parametrize("qorder,deg", [(1, 1), (2, 2), (3, 3), (4, 4)])
 def test_quadrature_exactness(qorder, deg):
+    """Every monomial of degree <= the order integrates exactly (closed form
+    a! b! c! / (a + b + c + 3)!); the rules of orders 3 and 4 have 6 and 11
+    points with positive weights (Xiao-Gimbutas)."""
     pts, w = po.quadrature(qorder)
+    assert len(w) == {1: 1, 2: 4, 3: 6, 4: 11}[qorder] and np.all(w > 0)
     assert abs(w.sum() - 1 / 6) < 1e-15
     assert np.all(pts >= 0) and np.all(pts.sum(axis=1) <= 1)
     for a in range(deg + 1):
@@ -101,6 +105,22 @@ def test_quadrature_exactness(qorder, deg):
                 exact = math.factorial(a) * math.factorial(b) * math.factorial(c) / math.factorial(a + b + c + 3)
                 got = np.sum(w * pts[:, 0] ** a * pts[:, 1] ** b * pts[:, 2] ** c)
                 assert abs(got - exact) < 1e-15
+
+
+def test_quadrature_order_is_sharp():
+    """Orders 3 and 4 are not exact one degree higher (a rule of a lower
+    order mistyped as these would pass the exactness test above only by
+    chance; this one checks the rule is not accidentally of higher order)."""
+    for qorder in (3, 4):
+        pts, w = po.quadrature(qorder)
+        deg = qorder + 1
+        worst = 0.0
+        for a in range(deg + 1):
+            for b in range(deg + 1 - a):
+                c = deg - a - b
+                exact = math.factorial(a) * math.factorial(b) * math.factorial(c) / math.factorial(a + b + c + 3)
+                worst = max(worst, abs(np.sum(w * pts[:, 0] ** a * pts[:, 1] ** b * pts[:, 2] ** c) - exact))
+        assert worst > 1e-9
 
 
 def test_hms_points_are_the_published_rule():
