@@ -249,6 +249,15 @@ typedef struct {
     /* also converged once ||g_f|| <= relative_grad_norm ||g_f(x0)|| (per
      * problem in a batch); 0 disables */
     double relative_grad_norm;
+    /* regularized Newton (polysolve's RegularizedNewton, with the
+     * block-diagonal scaling of Levenberg-Marquardt): when the CG meets
+     * non-positive curvature, the direction is solved again on
+     * H + delta D, D the node 3x3 diagonal blocks of H (the block-Jacobi
+     * preconditioner's), delta = regularization_min, then x10 per failed
+     * solve up to regularization_max; after an accepted step delta /= 10
+     * (0 below regularization_min).  regularization_min <= 0 disables it;
+     * it needs the block-Jacobi preconditioner.  pfgpu_newton_solve only. */
+    double regularization_min, regularization_max;
 } pfgpu_newton_params;
 
 typedef struct {
@@ -263,6 +272,10 @@ typedef struct {
      * iteration cap */
     int32_t cg_stops_tolerance, cg_stops_curvature, cg_stops_cap;
     double initial_grad_norm; /* reduced gradient norm at the initial guess */
+    /* CG solves repeated on a regularized operator, and the delta of the
+     * last accepted step's direction (0: plain Newton) */
+    int32_t regularized_solves;
+    double last_regularization;
 } pfgpu_newton_report;
 enum { PFGPU_CG_TOLERANCE = 0, PFGPU_CG_CURVATURE = 1, PFGPU_CG_ITERATION_CAP = 2 };
 

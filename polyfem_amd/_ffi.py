@@ -41,14 +41,16 @@ class Info(C.Structure):
 class NewtonParams(C.Structure):
     _fields_ = [("max_iterations", C.c_int32), ("grad_norm", C.c_double), ("cg_max_iterations", C.c_int32),
                 ("cg_tolerance", C.c_double), ("line_search", C.c_int32), ("armijo_c", C.c_double),
-                ("max_line_search", C.c_int32), ("use_graph", C.c_int32), ("relative_grad_norm", C.c_double)]
+                ("max_line_search", C.c_int32), ("use_graph", C.c_int32), ("relative_grad_norm", C.c_double),
+                ("regularization_min", C.c_double), ("regularization_max", C.c_double)]
 
 
 class NewtonReport(C.Structure):
     _fields_ = [("iterations", C.c_int32), ("cg_iterations", C.c_int32), ("line_search_halvings", C.c_int32),
                 ("converged", C.c_int32), ("energy", C.c_double), ("grad_norm", C.c_double),
                 ("last_alpha", C.c_double), ("gradient_steps", C.c_int32), ("cg_stops_tolerance", C.c_int32),
-                ("cg_stops_curvature", C.c_int32), ("cg_stops_cap", C.c_int32), ("initial_grad_norm", C.c_double)]
+                ("cg_stops_curvature", C.c_int32), ("cg_stops_cap", C.c_int32), ("initial_grad_norm", C.c_double),
+                ("regularized_solves", C.c_int32), ("last_regularization", C.c_double)]
 
 
 class TransientParams(C.Structure):

@@ -186,7 +186,7 @@ int64_t pfgpu_handle::device_bytes() const
 {
     const DevBuf *bufs[] = {&conn, &vconn, &geom, &lame, &node_ptr, &node_inc, &vert_ptr, &vert_inc,
                             &brow_ptr, &blk_out, &blk_plan.chunk, &blk_plan.slot, &blk_plan.meta, &fac_plan.chunk,
-                            &fac_plan.slot, &fac_plan.meta, &fac_plan.dst, &unit_plan.chunk_row, &unit_plan.udst, &unit_plan.bq0, &unit_plan.lanes, &row_chunk, &row_slot, &row_meta, &node_diag, &patch_desc, &patch_rows, &fac_rec,
+                            &fac_plan.slot, &fac_plan.meta, &fac_plan.dst, &unit_plan.chunk_row, &unit_plan.udst, &unit_plan.bq0, &unit_plan.lanes, &row_chunk, &row_slot, &row_meta, &node_diag, &reg_diag, &patch_desc, &patch_rows, &fac_rec,
                             &patch_row_acc, &patch_elems, &patch_cs, &patch_cg, &patch_hdr, &patch_pos, &patch_vnodes,
                             &d_outer, &d_inner, &loc, &hloc, &partial, &red_out, &sum_part,
                             &h_x, &h_v, &h_out, &h_out2, &h_vals, &sv, &h_mask, &mass, &pat_bcol, &tr_fixed, &tr_f, &tr_seg, &load};
@@ -880,6 +880,8 @@ void check_newton_params(const pfgpu_newton_params *p)
                 p->relative_grad_norm >= 0,
             "invalid tolerance");
     require(p->line_search == 0 || p->line_search == 1, "line_search must be 0 or 1");
+    require(!(p->regularization_min > 0) || p->regularization_max >= p->regularization_min,
+            "regularization_max must be at least regularization_min");
 }
 
 const uint8_t *stage_mask(pfgpu_handle &h, const uint8_t *fixed)

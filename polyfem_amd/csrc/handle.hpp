@@ -75,11 +75,19 @@ struct pfgpu_handle {
     // solver workspace: vectors, scalars, the captured CG iteration
     pfgpu::DevBuf sv, h_mask;
     void *cg_graph = nullptr; // hipGraphExec_t
-    std::array<const void *, 8> cg_graph_key{};
+    std::array<const void *, 9> cg_graph_key{};
     // CG operator: assembled block-sparse values or matrix-free, block-Jacobi preconditioner
     int solver_hessian = PFGPU_SOLVER_HESSIAN_AUTO, solver_precond = PFGPU_PRECOND_BLOCK_JACOBI;
     bool op_valid = false, op_assembled = false, op_precond = false;
     bool op_records = false; // fac_rec holds the factor records at the operator's x
+    // regularization weight delta of the operator (w H + M + delta D, D the
+    // preconditioner's diagonal blocks, kept in reg_diag as 9 floats per node)
+    double op_reg = 0.0;
+    pfgpu::DevBuf reg_diag;
+    // how prepare_operator built the preconditioner (node-diagonal blocks,
+    // mass folded into the values, the fixed-dof mask)
+    bool op_ndiag = false, op_fold = false;
+    const uint8_t *op_fixed = nullptr;
     pfgpu::SegSkip skip; // set only inside a batched CG
     pfgpu::DevBuf solver_vals, pinv, diag_pos, node_diag;
     hipStream_t own_stream = nullptr; // used when the caller's stream is the legacy default

@@ -26,6 +26,7 @@
 // beta, r.r) live in device memory, so it can be captured once in a hipGraph
 // and replayed; the host reads the residual only every CHECK iterations.
 #include <cmath>
+#include <cstring>
 
 #include "dispatch.hpp"
 
@@ -269,17 +270,19 @@ __global__ __launch_bounds__(BLOCK) void k_bsr_spmv(int n_nodes, const int32_t *
     }
 }
 
-// inverse of every node's 3x3 diagonal block of w H + M, with the rows and
-// columns of fixed dofs replaced by the identity; a block that is not
-// positive definite (an indefinite tangent) falls back to 1 / |diagonal|
+// inverse of every node's 3x3 diagonal block of scale (w H + M), with the
+// rows and columns of fixed dofs replaced by the identity; a block that is
+// not positive definite (an indefinite tangent) falls back to 1 / |diagonal|
 // The diagonal blocks come from the assembled values (vals, pattern layout)
 // or, when ndiag is given, from a node-diagonal array (9 values per node).
+// dblk, when given, receives the unscaled block of w H + M with the rows and
+// columns of fixed dofs zeroed (the regularization's D).
 template <typename T>
 __global__ __launch_bounds__(BLOCK) void k_block_jacobi(int n_nodes, const int32_t *__restrict__ brow_ptr,
                                                         const int32_t *__restrict__ diag_pos, const T *__restrict__ vals,
                                                         const T *__restrict__ ndiag, const double *__restrict__ mass,
-                                                        double w, const uint8_t *__restrict__ fixed,
-                                                        float *__restrict__ pinv)
+                                                        double w, const uint8_t *__restrict__ fixed, double scale,
+                                                        float *__restrict__ pinv, float *__restrict__ dblk)
 {
     const int i = blockIdx.x * BLOCK + threadIdx.x;
     if (i >= n_nodes) return;
@@ -295,7 +298,16 @@ __global__ __launch_bounds__(BLOCK) void k_block_jacobi(int n_nodes, const int32
     for (int d = 0; d < 3; ++d)
         if (fixed && fixed[3 * int64_t(i) + d])
 #pragma unroll
-            for (int k = 0; k < 3; ++k) B[d][k] = B[k][d] = d == k ? 1.0 : 0.0;
+            for (int k = 0; k < 3; ++k) B[d][k] = B[k][d] = 0.0;
+    if (dblk)
+#pragma unroll
+        for (int m = 0; m < 9; ++m) dblk[9 * int64_t(i) + m] = float(B[m / 3][m % 3]);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) B[d][k] *= scale;
+        if (fixed && fixed[3 * int64_t(i) + d]) B[d][d] = 1.0;
+    }
     // symmetric part (the exact block is symmetric; rounding is not)
     const double a = B[0][0], b = 0.5 * (B[0][1] + B[1][0]), c = 0.5 * (B[0][2] + B[2][0]);
     const double d = B[1][1], e = 0.5 * (B[1][2] + B[2][1]), f = B[2][2];
@@ -311,6 +323,19 @@ __global__ __launch_bounds__(BLOCK) void k_block_jacobi(int n_nodes, const int32
         auto inv = [](double v) { return std::fabs(v) > 0.0 && isfinite(v) ? 1.0 / std::fabs(v) : 1.0; };
         P[0] = float(inv(a)), P[1] = 0.0f, P[2] = 0.0f, P[3] = float(inv(d)), P[4] = 0.0f, P[5] = float(inv(f));
     }
+}
+
+// y += delta D v, D the 3x3 diagonal blocks (9 floats per node)
+__global__ __launch_bounds__(BLOCK) void k_block_diag_add(int n_nodes, const float *__restrict__ dblk, double delta,
+                                                          const double *__restrict__ v, double *__restrict__ y)
+{
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n_nodes) return;
+    const float *D = dblk + 9 * int64_t(i);
+    const double v0 = v[3 * int64_t(i)], v1 = v[3 * int64_t(i) + 1], v2 = v[3 * int64_t(i) + 2];
+#pragma unroll
+    for (int m = 0; m < 3; ++m)
+        y[3 * int64_t(i) + m] += delta * (double(D[3 * m]) * v0 + double(D[3 * m + 1]) * v1 + double(D[3 * m + 2]) * v2);
 }
 
 // q := 0 on fixed dofs
@@ -481,8 +506,15 @@ bool assembled_operator(const Handle &h)
            (h.solver_hessian == PFGPU_SOLVER_HESSIAN_AUTO && h.order == 1);
 }
 
-// y = (w H(x) + [M]) v: the block-sparse product with the values assembled by
-// prepare_operator at x, or the matrix-free product
+// the regularized operator's term: y += delta D v
+void add_regularization(Handle &h, const double *v, double *y)
+{
+    if (h.op_valid && h.op_reg > 0.0)
+        k_block_diag_add<<<grid_for(h.n_nodes), BLOCK, 0, h.stream>>>(h.n_nodes, h.reg_diag.as<float>(), h.op_reg, v, y);
+}
+
+// y = (w H(x) + [M] + [delta D]) v: the block-sparse product with the values
+// assembled by prepare_operator at x, or the matrix-free product
 void hvp_any(Handle &h, const double *x, const double *v, double *y)
 {
     if (h.op_valid && h.op_assembled) {
@@ -496,6 +528,7 @@ void hvp_any(Handle &h, const double *x, const double *v, double *y)
             spmv.template operator()<float>();
         else
             spmv.template operator()<double>();
+        add_regularization(h, v, y);
         PF_LAUNCH_CHECK();
         return;
     }
@@ -511,8 +544,11 @@ void hvp_any(Handle &h, const double *x, const double *v, double *y)
                                                                    h.obj_w, y);
     else if (h.obj_w != 1.0)
         k_combine_gradient<<<grid_for(h.n_dofs), BLOCK, 0, h.stream>>>(y, h.obj_w, nullptr, nullptr, h.n_dofs);
+    add_regularization(h, v, y);
     PF_LAUNCH_CHECK();
 }
+
+void build_preconditioner(Handle &h, double scale, bool diag);
 
 // The CG operator at x: with an assembled operator or the block-Jacobi
 // preconditioner, the Hessian values at x (plain or fused assembly) into the
@@ -524,6 +560,7 @@ void prepare_operator(Handle &h, const double *x, const uint8_t *fixed)
     h.op_assembled = assembled_operator(h);
     h.op_precond = h.solver_precond == PFGPU_PRECOND_BLOCK_JACOBI;
     h.op_records = false;
+    h.op_reg = 0.0;
     // a matrix-free operator on the factored path needs only the diagonal
     // blocks for its preconditioner: those come from the factor records
     const bool diag_only = !h.op_assembled && h.op_precond && h.path == PFGPU_PATH_FACTORED &&
@@ -574,20 +611,43 @@ void prepare_operator(Handle &h, const double *x, const uint8_t *fixed)
             }
             h.diag_pos = upload(dp, h.stream);
         }
-        h.pinv.reserve(sizeof(float) * PINV * size_t(h.n_nodes));
-        auto jac = [&]<typename T>() {
-            k_block_jacobi<T><<<grid_for(h.n_nodes), BLOCK, 0, h.stream>>>(
-                h.n_nodes, h.brow_ptr.as<int32_t>(), h.diag_pos.as<int32_t>(), h.solver_vals.as<T>(),
-                diag_only || mixed_diag ? h.node_diag.as<T>() : nullptr, fold ? nullptr : mass, fold ? 1.0 : h.obj_w, fixed,
-                h.pinv.as<float>());
-        };
-        if (h.precision == PFGPU_MIXED)
-            jac.template operator()<float>();
-        else
-            jac.template operator()<double>();
-        PF_LAUNCH_CHECK();
+        h.op_ndiag = diag_only || mixed_diag;
+        h.op_fold = fold;
+        h.op_fixed = fixed;
+        build_preconditioner(h, 1.0, false);
     }
     h.op_valid = true;
+}
+
+// the block-Jacobi preconditioner of the prepared operator scaled by scale
+// (1 + delta for the regularized operator); with diag, also the unscaled
+// diagonal blocks into reg_diag
+void build_preconditioner(Handle &h, double scale, bool diag)
+{
+    const double *mass = inertia(h) ? h.mass.as<double>() : nullptr;
+    h.pinv.reserve(sizeof(float) * PINV * size_t(h.n_nodes));
+    if (diag) h.reg_diag.reserve(sizeof(float) * 9 * size_t(h.n_nodes));
+    auto jac = [&]<typename T>() {
+        k_block_jacobi<T><<<grid_for(h.n_nodes), BLOCK, 0, h.stream>>>(
+            h.n_nodes, h.brow_ptr.as<int32_t>(), h.diag_pos.as<int32_t>(), h.solver_vals.as<T>(),
+            h.op_ndiag ? h.node_diag.as<T>() : nullptr, h.op_fold ? nullptr : mass, h.op_fold ? 1.0 : h.obj_w,
+            h.op_fixed, scale, h.pinv.as<float>(), diag ? h.reg_diag.as<float>() : nullptr);
+    };
+    if (h.precision == PFGPU_MIXED)
+        jac.template operator()<float>();
+    else
+        jac.template operator()<double>();
+    PF_LAUNCH_CHECK();
+}
+
+// the prepared operator becomes w H + M + delta D (delta = 0: unregularized);
+// needs the block-Jacobi preconditioner, whose blocks D are
+void set_regularization(Handle &h, double delta)
+{
+    if (!h.op_valid || !h.op_precond) fail(PFGPU_ERR_INVALID_STATE, "regularization needs the block-Jacobi preconditioner");
+    if (delta == h.op_reg) return;
+    build_preconditioner(h, 1.0 + delta, delta > 0.0);
+    h.op_reg = delta;
 }
 
 // objective gradient into g and its value into scal[slot] (device)
@@ -718,7 +778,10 @@ int cg_solve_ws(Handle &h, const double *x, const uint8_t *fixed, const double *
     PF_LAUNCH_CHECK();
     // the graph replays one iteration; it is rebuilt when its arguments change
     // (scratch buffers are grow-only: a reallocation by another call rebuilds the graph)
-    const std::array<const void *, 8> key{x,
+    intptr_t reg_bits = 0;
+    static_assert(sizeof(reg_bits) == sizeof(h.op_reg), "the regularization weight is keyed by its bits");
+    std::memcpy(&reg_bits, &h.op_reg, sizeof(reg_bits));
+    const std::array<const void *, 9> key{x,
                                           fixed,
                                           w.r,
                                           reinterpret_cast<const void *>(intptr_t(h.obj_generation)),
@@ -726,7 +789,8 @@ int cg_solve_ws(Handle &h, const double *x, const uint8_t *fixed, const double *
                                           reinterpret_cast<const void *>(intptr_t(h.op_assembled ? 1 : 0) |
                                                                          intptr_t(precond(h) ? 2 : 0)),
                                           h.solver_vals.get(),
-                                          h.pinv.get()};
+                                          h.pinv.get(),
+                                          reinterpret_cast<const void *>(reg_bits)};
     if (use_graph && (h.cg_graph == nullptr || h.cg_graph_key != key)) {
         release_graph(h); // (the operator prepared above stays valid)
         cg_iteration(h, w, x, fixed); // sizes every scratch buffer before capture
@@ -783,6 +847,9 @@ void newton_solve(Handle &h, double *x, const uint8_t *fixed, const pfgpu_newton
     rep = pfgpu_newton_report{};
     if (inertia(h) && !h.has_mass) fail(PFGPU_ERR_INVALID_STATE, "inertia term without a mass matrix");
     double gn0 = 0; // gradient norm at the initial guess
+    // regularized Newton: delta of H + delta D, kept across iterations
+    const bool regularize = prm.regularization_min > 0.0 && h.solver_precond == PFGPU_PRECOND_BLOCK_JACOBI;
+    double reg = 0.0;
     auto energy_at = [&](const double *xx) {
         objective_energy(h, w, xx, S_E1);
         return objective_value(h, w, S_E1);
@@ -805,14 +872,24 @@ void newton_solve(Handle &h, double *x, const uint8_t *fixed, const pfgpu_newton
             rep.converged = 1;
             return;
         }
-        // Newton direction: H_ff d = -g_f
+        // Newton direction: H_ff d = -g_f; on non-positive curvature, again
+        // on H + delta D with a growing delta (regularized Newton)
         k_neg_free<<<gr, BLOCK, 0, h.stream>>>(w.d, w.g, fixed, h.n_dofs);
         PF_LAUNCH_CHECK();
         double rel = 0;
         int why = 0;
-        const int cg_it =
-            cg_solve_ws(h, x, fixed, w.d, prm.cg_max_iterations, prm.cg_tolerance, prm.use_graph != 0, &rel, &why);
-        rep.cg_iterations += cg_it;
+        while (true) {
+            if (reg > 0.0) {
+                if (!h.op_valid) prepare_operator(h, x, fixed);
+                set_regularization(h, reg);
+            }
+            rep.cg_iterations +=
+                cg_solve_ws(h, x, fixed, w.d, prm.cg_max_iterations, prm.cg_tolerance, prm.use_graph != 0, &rel, &why);
+            if (why != PFGPU_CG_CURVATURE || !regularize || reg >= prm.regularization_max) break;
+            reg = reg > 0.0 ? std::min(10.0 * reg, prm.regularization_max) : prm.regularization_min;
+            rep.regularized_solves++;
+        }
+        const double reg_used = reg;
         if (why == PFGPU_CG_CURVATURE)
             rep.cg_stops_curvature++;
         else if (why == PFGPU_CG_ITERATION_CAP)
@@ -875,6 +952,8 @@ void newton_solve(Handle &h, double *x, const uint8_t *fixed, const pfgpu_newton
         PF_HIP(hipMemcpyAsync(x, w.xt, sizeof(double) * size_t(h.n_dofs), hipMemcpyDeviceToDevice, h.stream));
         rep.iterations = it + 1;
         rep.last_alpha = alpha;
+        rep.last_regularization = reg_used;
+        if (reg > 0.0) reg = reg / 10.0 < prm.regularization_min ? 0.0 : reg / 10.0;
     }
     // final state
     objective_gradient(h, w, x, w.g, S_E0);

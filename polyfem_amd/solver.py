@@ -70,12 +70,14 @@ class NewtonReport:
     cg_stops_curvature: int = 0
     cg_stops_cap: int = 0
     initial_grad_norm: float = 0.0
+    regularized_solves: int = 0
+    last_regularization: float = 0.0
 
 
 def _report(r) -> "NewtonReport":
     return NewtonReport(r.iterations, r.cg_iterations, r.line_search_halvings, bool(r.converged), r.energy,
                         r.grad_norm, r.last_alpha, r.gradient_steps, r.cg_stops_tolerance, r.cg_stops_curvature,
-                        r.cg_stops_cap, r.initial_grad_norm)
+                        r.cg_stops_cap, r.initial_grad_norm, r.regularized_solves, r.last_regularization)
 
 
 class NewtonSolver:
@@ -83,12 +85,17 @@ class NewtonSolver:
 
     def __init__(self, max_iterations: int = 10, grad_norm: float = 1e-8, cg_max_iterations: int = 1000,
                  cg_tolerance: float = 1e-6, line_search: str = "Backtracking", armijo_c: float = 1e-4,
-                 max_line_search: int = 30, use_graph: bool = True, relative_grad_norm: float = 0.0):
+                 max_line_search: int = 30, use_graph: bool = True, relative_grad_norm: float = 0.0,
+                 regularization=None):
+        """``regularization=(delta_min, delta_max)`` turns on the regularized
+        Newton fallback (H + delta D on non-positive curvature, D the
+        block-diagonal of H); None keeps plain truncated Newton-CG."""
         if line_search not in ("Backtracking", "None"):
             raise ValueError("line_search must be 'Backtracking' or 'None'")
+        rmin, rmax = regularization if regularization is not None else (0.0, 0.0)
         self.params = _ffi.NewtonParams(max_iterations, grad_norm, cg_max_iterations, cg_tolerance,
                                         1 if line_search == "Backtracking" else 0, armijo_c, max_line_search,
-                                        1 if use_graph else 0, relative_grad_norm)
+                                        1 if use_graph else 0, relative_grad_norm, rmin, rmax)
 
     def minimize(self, assembler, x, problem: NLProblem):
         """Returns (x*, report); x holds the initial guess on the free dofs and

@@ -305,3 +305,58 @@ def test_captured_value_survives_other_entry_points():
         s.synchronize()
         assert float(e.cpu()[0]) == ref, k
     a.set_stream(0)
+
+
+def indefinite_case(order=2, amp=0.15):
+    """A clamped box whose start has an indefinite Hessian (P2, amplitude
+    0.15: 67 negative eigenvalues of H_ff out of 741)."""
+    m, prob = clamped_box(order)
+    lam, mu = random_lame(m.n_elements, seed=BASE_SEED + 3)
+    x0 = smooth_displacement(m.nodes, amp, 1e-3, BASE_SEED + 4)
+    x0[prob.fixed] = prob.fixed_values
+    return m, prob, lam, mu, x0
+
+
+@pytest.mark.parametrize("path", ["factored", "plain"])
+def test_regularized_step_solves_the_shifted_system(path):
+    """On non-positive curvature the direction is solved again on
+    H + delta D (D the node 3x3 diagonal blocks of H, fixed rows and columns
+    zeroed); the step taken equals the sparse-direct solution of that system
+    for the delta reported."""
+    m, prob, lam, mu, x0 = indefinite_case()
+    a = NLAssembler("NeoHookean", m, lame=(lam, mu), precision="fp64", path=path)
+    x1, rep = NewtonSolver(max_iterations=1, grad_norm=0.0, cg_max_iterations=20000, cg_tolerance=1e-12,
+                           line_search="None", regularization=(1e-2, 1e6)).minimize(a, x0, prob)
+    print("regularized step:", rep, flush=True)
+    delta = rep.last_regularization
+    assert rep.regularized_solves >= 1 and delta > 0 and rep.cg_stops_tolerance == 1
+    o = po.Oracle(m, lam, mu)
+    H = o.hessian_scipy(x0).tocsr()
+    n = m.n_nodes
+    blocks = np.zeros((n, 3, 3))
+    Hd = H.toarray() if m.n_dofs <= 4000 else None
+    for i in range(n):
+        blocks[i] = Hd[3 * i:3 * i + 3, 3 * i:3 * i + 3]
+    import scipy.sparse as sp
+    D = sp.block_diag(list(blocks)).tocsr()
+    f = prob.free
+    A = (H + delta * D).tocsr()[f][:, f]
+    step = spla.spsolve(A.tocsc(), -o.gradient(x0)[f])
+    # D is held in FP32: the system differs from the FP64 one by ~1e-7 delta D
+    assert rel((x1 - x0)[f], step) < 1e-5
+    assert np.array_equal(x1[prob.fixed], prob.fixed_values)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_regularized_newton_converges_from_an_indefinite_start(precision):
+    m, prob, lam, mu, x0 = indefinite_case()
+    o = po.Oracle(m, lam, mu)
+    g0 = np.linalg.norm(o.gradient(x0)[prob.free])
+    rtol = 1e-9 if precision == "fp64" else 1e-5
+    a = NLAssembler("NeoHookean", m, lame=(lam, mu), precision=precision)
+    x, rep = NewtonSolver(max_iterations=80, grad_norm=0.0, relative_grad_norm=rtol, cg_max_iterations=4000,
+                          cg_tolerance=1e-8, regularization=(1e-2, 1e6)).minimize(a, x0, prob)
+    print(f"regularized Newton {precision}:", rep, flush=True)
+    assert rep.converged and rep.regularized_solves >= 1, rep
+    assert np.linalg.norm(o.gradient(x)[prob.free]) <= (1 if precision == "fp64" else 10) * rtol * g0
+    assert o.energy(x) < o.energy(x0)
