@@ -42,9 +42,28 @@ def algorithmic_bytes(n_dofs, n_elements, n_local, n_vertices, nnz, value_bytes,
     return reads + writes
 
 
-def cpu_baseline_sample(threads: int):
-    """The reference algorithm restated in C (threaded, per-thread storage,
-    slot-map Hessian) timed on a 20^3 x 6 P2 sample of the same workload."""
+def host_ram_bytes() -> int:
+    """Available host memory (MemAvailable), or 0 when unknown."""
+    try:
+        with open("/proc/meminfo") as f:
+            for line in f:
+                if line.startswith("MemAvailable:"):
+                    return int(line.split()[1]) * 1024
+    except OSError:
+        pass
+    return 0
+
+
+def cpu_baseline_sample(cores: int):
+    """The reference algorithm restated in C (static std::thread chunking of
+    utils/par_for.cpp:10-31, per-thread gradient vectors and full-length
+    Hessian value arrays with a serial reduction, MatrixCache slot map),
+    timed on a 20^3 x 6 P2 sample of the same workload: one warm-up, the
+    median of 5 steady-state E+G+H calls, with every host core (capped so
+    that the per-thread value arrays fit the host memory) and with one
+    thread.  The first Hessian call (triplets -> CSC + slot map) is reported
+    apart.  Bounded to a sample: the first call alone takes ~100 s at the
+    full 40^3 x 6 size."""
     import numpy as np
 
     from oracle import pyoracle as po
@@ -54,25 +73,43 @@ def cpu_baseline_sample(threads: int):
     lam, mu = random_lame(m.n_elements, seed=BASE_SEED + 1)
     x = smooth_displacement(m.nodes, 0.05, 1e-3, BASE_SEED)
     o = po.Oracle(m, lam, mu)
-    b = po.Baseline(o, threads)
-    t0 = time.perf_counter()
-    nnz = b.hessian(x)  # first call: triplets -> CSC + slot map
-    first = time.perf_counter() - t0
-    vals = np.zeros(nnz)
-    g = np.zeros(m.n_dofs)
-    times = []
-    for _ in range(3):
-        t0 = time.perf_counter()
-        b.energy(x)
-        po.lib().orc_baseline_gradient(b.b, x, g)
-        b.hessian(x, vals)
-        times.append(time.perf_counter() - t0)
-    t = statistics.median(times)
     eqp = m.n_elements * 4
-    return {"value": eqp / t, "unit": "elem*qp/s", "cores": b.threads, "kind": "port",
+
+    def timed(threads):
+        b = po.Baseline(o, threads)
+        t0 = time.perf_counter()
+        nnz = b.hessian(x)  # first call: triplets -> CSC + slot map
+        first = time.perf_counter() - t0
+        vals = np.zeros(nnz)
+        g = np.zeros(m.n_dofs)
+
+        def step():
+            t0 = time.perf_counter()
+            b.energy(x)
+            po.lib().orc_baseline_gradient(b.b, x, g)
+            b.hessian(x, vals)
+            return time.perf_counter() - t0
+        step()  # warm-up
+        t = statistics.median(step() for _ in range(5))
+        used = b.threads
+        del b
+        return t, first, used, nnz
+
+    # RAM cap: the reference's per-thread value arrays cost nnz x 8 B per thread
+    t1, first1, _, nnz = timed(1)
+    ram = host_ram_bytes()
+    cap = max(1, int(0.5 * ram // (nnz * 8))) if ram else cores
+    threads = max(1, min(cores, cap))
+    print(f"cpu baseline: {cores} host cores, RAM cap {cap} threads ({ram / 2**30:.0f} GiB available, "
+          f"{nnz * 8 / 2**20:.0f} MiB of values per thread) -> {threads} threads", file=sys.stderr, flush=True)
+    tn, firstn, used, _ = timed(threads)
+    return {"value": eqp / tn, "unit": "elem*qp/s", "cores": used, "kind": "port",
             "sample": f"20^3x6 P2 NeoHookean ({m.n_elements} tets, {eqp} elem*qp), FP64 energy+gradient+Hessian, "
-                      f"median of 3 steady-state calls; first call (pattern + slot map) {first:.2f} s",
-            "seconds_per_step": t}
+                      f"reference-algorithm restatement (oracle/oracle.c baseline), median of 5 after a warm-up; "
+                      f"first call (pattern + slot map) {firstn:.1f} s",
+            "seconds_per_step": tn,
+            "one_thread": {"value": eqp / t1, "seconds_per_step": t1, "first_call_s": first1},
+            "threads_cap_by_ram": cap}
 
 
 def extras(args, m, lam, mu, x_host, a, x, g, vals, y, eqp):
@@ -665,8 +702,9 @@ def main():
     if not args.no_extras:
         out.update(extras(args, m, lam, mu, x_host, a, x, g, vals, y, eqp))
     if not args.no_cpu_baseline:
-        threads = min(16, os.cpu_count() or 1)
-        out["cpu_baseline"] = cpu_baseline_sample(threads)
+        # every host core of this process's share (16 per GPU on the MI355X boxes)
+        cores = min(int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1), os.cpu_count() or 1)
+        out["cpu_baseline"] = cpu_baseline_sample(cores)
     print(json.dumps(out))
 
 
