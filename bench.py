@@ -230,6 +230,10 @@ def extras(args, m, lam, mu, x_host, a, x, g, vals, y, eqp):
         res["newton_cg"]["fp64"] = r64
         res["newton_cg"]["mixed_vs_fp64_solution_rel"] = float(
             np.abs((xm - xd64)[prob.free]).max() / max(np.abs((xd64[prob.free] - u0)).max(), 1e-300))
+    # the 256-problem heterogeneous batch (the suite's batch workload)
+    if args.batch_problems > 0:
+        res["batch"] = batch_run(args.batch_problems, args.batch_newton, args.batch_rtol, args.batch_cg,
+                                 lambda msg: print(msg, file=sys.stderr, flush=True))
     return res
 
 
@@ -308,21 +312,22 @@ def suite_step():
     a.close()
 
 
-def suite_batch():
+def batch_run(n_problems, newton_cap, rtol, cg_cap, log):
+    """The heterogeneous batch: n_problems box meshes with tet counts
+    log-uniform in [1k, 200k], problem i P1 if i is even, LinearElasticity if
+    i // 2 is even; per problem E log-normal (mean ln 1e6, sigma 0.5), nu
+    uniform [0.3, 0.45], bottom clamped and top pulled by 2-8 % of the height;
+    mixed precision, one batch handle (pfgpu_batch_create: every element
+    kernel and node gather one launch over all problems), graph-replayed CG.
+    Mesh generation and the handle's setup are untimed."""
     import numpy as np
-    import torch
     from polyfem_amd import BatchSolver, NewtonSolver
     from polyfem_amd.mesh import BASE_SEED, face_nodes, kuhn_box, lame_from_young
-    args, clock, log = _ARGS, _clock, _log
 
-    # ---- batch: 256 box meshes with tet counts log-uniform in [1k, 200k], problem i P1
-    # if i is even, LinearElasticity if i // 2 is even; per problem E log-normal (mean
-    # ln 1e6, sigma 0.5), nu uniform [0.3, 0.45], bottom clamped and top pulled by 2-8 %
-    # of the height; mixed precision, autotuned launch shapes, graph-replayed CG
     rng = np.random.default_rng(BASE_SEED + 21)
     probs = []
     t_setup = time.perf_counter()
-    for i in range(args.batch_problems):
+    for i in range(n_problems):
         order = 1 + (i % 2)
         material = "LinearElasticity" if (i // 2) % 2 == 0 else "NeoHookean"
         cells = np.exp(rng.uniform(np.log(1000), np.log(200000))) / 6
@@ -340,43 +345,34 @@ def suite_batch():
         probs.append(dict(mesh=mb, material=material, lame=(float(lam_i), float(mu_i)), fixed_dofs=fixed,
                           fixed_values=vals, x0=x0))
     n_tets = sum(p["mesh"].n_elements for p in probs)
-    newton = NewtonSolver(max_iterations=args.batch_newton, grad_norm=0.0, relative_grad_norm=args.batch_rtol,
-                          cg_max_iterations=args.batch_cg, cg_tolerance=1e-6)
+    newton = NewtonSolver(max_iterations=newton_cap, grad_norm=0.0, relative_grad_norm=rtol,
+                          cg_max_iterations=cg_cap, cg_tolerance=1e-6)
     bs = BatchSolver(probs, precision="mixed", newton=newton)
-    tuning, g0 = [], np.zeros(len(probs))
-    for grp in bs.groups:
-        ga = grp["a"]
-        xg = torch.zeros(ga.mesh.n_dofs, dtype=torch.float64, device="cuda")
-        tuning.append(ga.autotune_dev(xg.data_ptr()))
-        # initial reduced gradient norm of every problem of the group (untimed)
-        x0g = np.concatenate([probs[i]["x0"] for i in grp["idx"]])
-        x0g[grp["problem"].fixed] = grp["problem"].fixed_values
-        gg = ga.assemble_gradient(x0g)
-        gg[grp["problem"].fixed] = 0.0
-        for k, i in enumerate(grp["idx"]):
-            g0[i] = np.linalg.norm(gg[3 * grp["no"][k]:3 * grp["no"][k + 1]])
     t_setup = time.perf_counter() - t_setup
     log(f"batch: {len(probs)} problems, {n_tets} tets, setup {t_setup:.1f} s")
-    t0 = clock()
+    t0 = _clock()
     xs, reps = bs.solve()
-    t_batch = clock() - t0
-    red = np.array([r.grad_norm for r in reps]) / g0
-    print(json.dumps({
+    t_batch = _clock() - t0
+    red = np.array([r.grad_norm / r.initial_grad_norm if r.initial_grad_norm > 0 else 0.0 for r in reps])
+    res = {
         "workload": f"batch of {len(probs)} box meshes, {n_tets} tets in all (1k-200k each, half P1 / half P2, "
-                    "half LinearElasticity / half NeoHookean), mixed precision",
-        "newton": {"max_iterations": args.batch_newton, "relative_grad_norm": args.batch_rtol,
-                   "cg_max_iterations": args.batch_cg, "cg_tolerance": 1e-6},
+                    "half LinearElasticity / half NeoHookean), mixed precision, one batch handle",
+        "newton": {"max_iterations": newton_cap, "relative_grad_norm": rtol,
+                   "cg_max_iterations": cg_cap, "cg_tolerance": 1e-6},
         "converged": int(sum(r.converged for r in reps)), "newton_iterations_max": max(r.iterations for r in reps),
-        "solve_s": t_batch, "problems_per_s": len(probs) / t_batch,
+        "solve_s": t_batch, "problems_per_s": len(probs) / t_batch, "setup_s": t_setup,
         "cg_iterations_total": int(sum(r.cg_iterations for r in reps)),
         "grad_reduction": {"median": float(np.median(red)), "max": float(red.max()),
-                           "below_1e-6": int((red <= 1e-6).sum())},
-        "groups": [{"order": int(grp["a"].mesh.order), "material": probs[grp["idx"][0]]["material"],
-                    "problems": len(grp["idx"]), "tets": int(grp["a"].mesh.n_elements), "tuning": tn}
-                   for grp, tn in zip(bs.groups, tuning)]}), flush=True)
-    for grp in bs.groups:
-        grp["a"].close()
+                           "below_1e-6": int((red <= 1e-6).sum())}}
+    bs.close()
     del bs, probs
+    return res
+
+
+def suite_batch():
+    args = _ARGS
+    print(json.dumps(batch_run(args.batch_problems, args.batch_newton, args.batch_rtol, args.batch_cg, _log)),
+          flush=True)
 
 
 def suite_cantilever():
@@ -573,7 +569,8 @@ def main():
                          "3M-tet implicit-Euler adjoint), timed at full size")
     ap.add_argument("--suite-parts", default="step,batch,cantilever,transient",
                     help="--suite: comma-separated subset of step, batch, cantilever, transient")
-    ap.add_argument("--batch-problems", type=int, default=256, help="--suite: problems in the batch")
+    ap.add_argument("--batch-problems", type=int, default=256,
+                    help="problems in the heterogeneous batch (an extra of the default run, and --suite); 0 skips it")
     ap.add_argument("--batch-newton", type=int, default=20, help="--suite: Newton iteration cap of the batch")
     ap.add_argument("--batch-rtol", type=float, default=1e-4,
                     help="--suite: per-problem gradient reduction at which a batch problem has converged")

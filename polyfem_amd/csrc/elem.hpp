@@ -329,11 +329,17 @@ template <typename T> __device__ __forceinline__ void sv_stress(const SvKin<T> &
     for (int i = 0; i < 9; ++i) S[i] = T(2) * mu * s.E[i] + (i % 4 == 0 ? lam * s.trE : T(0));
 }
 // FixedCorotational kinematics (det F > 0): the rotation R = I + Q of the polar
-// decomposition F = R S, by Newton's iteration X <- (X + X^{-T}) / 2 from
-// X = F written for Q = X - I, Q <- (Q - X^{-T} Q^T) / 2, so that F - R = H - Q
-// keeps its precision for small strains; and K^{-1} for K = tr(S) I - S,
-// which gives the rotation's derivative dR[W] = R [K^{-1} a(W)]x with
-// a(W) the axial vector of R^T W - W^T R.
+// decomposition F = R S, by the scaled Newton iteration
+// X <- (g X + X^{-T} / g) / 2 from X = F (Higham's Frobenius-norm scaling
+// g = (|X^{-1}|_F / |X|_F)^{1/2}, which converges in a few steps for any
+// condition number; g = 1, the plain iteration, once g is within 1e-3 of
+// 1), written for Q = X - I:
+//   Q <- (g - 1)^2 / (2 g) I + (g Q - X^{-T} Q^T / g) / 2,
+// so that F - R = H - Q keeps its precision for small strains; and K^{-1}
+// for K = tr(S) I - S, which gives the rotation's derivative
+// dR[W] = R [K^{-1} a(W)]x with a(W) the axial vector of R^T W - W^T R.  A
+// rotation that has not converged (|R^T R - I|_F above a few rounding
+// units) poisons the outputs with NaN.
 constexpr int FC_POLAR_ITERATIONS = 12;
 template <typename T> struct FcKin {
     T Q[9], Ki6[6]; // Ki6: K^{-1}, entries 00 11 22 01 02 12
@@ -349,16 +355,38 @@ template <typename T> __device__ __forceinline__ void fc_kinematics(const T *H, 
                         X[2] * X[7] - X[1] * X[8], X[0] * X[8] - X[2] * X[6], X[1] * X[6] - X[0] * X[7],
                         X[1] * X[5] - X[2] * X[4], X[2] * X[3] - X[0] * X[5], X[0] * X[4] - X[1] * X[3]};
         const T id = T(1) / (X[0] * C[0] + X[1] * C[1] + X[2] * C[2]);
+        T nx = T(0), nc = T(0);
+#pragma unroll
+        for (int i = 0; i < 9; ++i) {
+            nx += X[i] * X[i];
+            nc += C[i] * C[i];
+        }
+        // |X^{-1}|_F / |X|_F = |C|_F |1/det| / |X|_F
+        T g = sqrt(sqrt(nc / nx) * fabs(id));
+        if (!(fabs(g - T(1)) >= T(1e-3))) g = T(1);
+        const T shift = (g - T(1)) * (g - T(1)) / (T(2) * g), ig = id / g;
         T N[9];
 #pragma unroll
         for (int r = 0; r < 3; ++r)
 #pragma unroll
             for (int c = 0; c < 3; ++c)
-                N[r * 3 + c] = T(0.5) * (f.Q[r * 3 + c] - id * (C[r * 3 + 0] * f.Q[c * 3 + 0] + C[r * 3 + 1] * f.Q[c * 3 + 1] +
-                                                                 C[r * 3 + 2] * f.Q[c * 3 + 2]));
+                N[r * 3 + c] = (r == c ? shift : T(0)) +
+                               T(0.5) * (g * f.Q[r * 3 + c] - ig * (C[r * 3 + 0] * f.Q[c * 3 + 0] + C[r * 3 + 1] * f.Q[c * 3 + 1] +
+                                                                    C[r * 3 + 2] * f.Q[c * 3 + 2]));
 #pragma unroll
         for (int i = 0; i < 9; ++i) f.Q[i] = N[i];
     }
+    // orthogonality of R = I + Q: R^T R - I = Q + Q^T + Q^T Q
+    T orth = T(0);
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const T d = f.Q[r * 3 + c] + f.Q[c * 3 + r] + f.Q[0 * 3 + r] * f.Q[0 * 3 + c] + f.Q[1 * 3 + r] * f.Q[1 * 3 + c] +
+                        f.Q[2 * 3 + r] * f.Q[2 * 3 + c];
+            orth += d * d;
+        }
+    const bool converged = orth <= (sizeof(T) == 4 ? T(1e-8) : T(1e-20));
     // S - I = R^T F - I = H + Q^T + Q^T H (symmetrised); K = tr(S) I - S
     T Sm[9];
 #pragma unroll
@@ -381,7 +409,7 @@ template <typename T> __device__ __forceinline__ void fc_kinematics(const T *H, 
     // non-finite values (the energy is NaN there as well)
     const T detF = (T(1) + H[0]) * ((T(1) + H[4]) * (T(1) + H[8]) - H[5] * H[7]) -
                    H[1] * (H[3] * (T(1) + H[8]) - H[5] * H[6]) + H[2] * (H[3] * H[7] - (T(1) + H[4]) * H[6]);
-    const T poison = detF > T(0) ? T(0) : T(NAN);
+    const T poison = detF > T(0) && converged ? T(0) : T(NAN);
 #pragma unroll
     for (int i = 0; i < 9; ++i) f.Q[i] += poison;
 #pragma unroll

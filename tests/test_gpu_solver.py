@@ -248,7 +248,8 @@ def test_newton_with_psd_projected_operator():
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 @pytest.mark.parametrize("order", [1, 2])
-def test_preconditioner_from_node_diagonals(order, precision):
+@pytest.mark.parametrize("name", ["NeoHookean", "LinearElasticity"])
+def test_preconditioner_from_node_diagonals(order, precision, name):
     """On the factored path a matrix-free operator takes its block-Jacobi
     blocks from the factor records (no assembly); the plain path reads them
     from the assembled values.  Fixed-count PCG iterates agree."""
@@ -257,10 +258,10 @@ def test_preconditioner_from_node_diagonals(order, precision):
     x = smooth_displacement(m.nodes, 0.02, 1e-3, BASE_SEED + 74)
     x[prob.fixed] = prob.fixed_values
     b = np.random.default_rng(BASE_SEED + 75).normal(size=m.n_dofs)
-    print(f"node-diagonal preconditioner order={order} {precision}", flush=True)
+    print(f"node-diagonal preconditioner order={order} {precision} {name}", flush=True)
     ys = {}
     for path in ("plain", "factored"):
-        a = NLAssembler("NeoHookean", m, lame=(lam, mu), precision=precision, path=path)
+        a = NLAssembler(name, m, lame=(lam, mu), precision=precision, path=path)
         a.set_solver_options("matrix_free", "block_jacobi")
         ys[path], it, _ = cg_solve(a, x, b, prob.mask, max_iterations=15, tolerance=0.0)
         assert it == 15
