@@ -92,7 +92,7 @@ struct GatherPlan {
 // its chunk.
 struct UnitPlan {
     DevBuf chunk_row, udst, bq0, lanes;
-    int n_chunks = 0, threads = 0, n_words = 0;
+    int n_chunks = 0, threads = 0, split = 1, n_words = 0;
     unsigned grid = 0, grid_grad = 0;
 };
 

@@ -1105,7 +1105,7 @@ __global__ __launch_bounds__(GS) void k_gather_factored(int n_chunks, const int3
 // hand.  ctab[c] = (first row, first unit, first pattern block) of chunk c,
 // ctab[n_chunks] the totals; n_words = 32-bit words of bq0.  gridDim must be
 // a multiple of 8.
-template <typename T, int ORDER, int NQ, int GS, bool GRAD>
+template <typename T, int ORDER, int NQ, int GS, bool GRAD, int SPLIT, bool PREFETCH>
 __global__ __launch_bounds__(GS) void k_gather_units(int n_chunks, int n_words, const int4 *__restrict__ ctab,
                                                      const int32_t *__restrict__ node_ptr,
                                                      const int32_t *__restrict__ node_inc,
@@ -1115,21 +1115,25 @@ __global__ __launch_bounds__(GS) void k_gather_units(int n_chunks, int n_words, 
                                                      const T *__restrict__ gloc, T *__restrict__ values,
                                                      double *__restrict__ grad)
 {
-    constexpr int NB = NLocal<ORDER>::value, R = fac_size(NQ), DW = NB / 2, L = UNIT_PIECE;
-    // 32-bit words of block starts per thread: a chunk has at most GS NB blocks
-    constexpr int NWMAX = GS * NB / 2 + 2, QW = (NWMAX + GS - 1) / GS;
+    // UPC units per chunk; with SPLIT = 2 each unit has two threads in
+    // different wavefronts, each forming half of its blocks
+    constexpr int NB = NLocal<ORDER>::value, R = fac_size(NQ), DW = NB / 2, L = UNIT_PIECE, UPC = GS / SPLIT;
+    static_assert(SPLIT == 1 || (SPLIT == 2 && UPC % 64 == 0), "the halves of a split unit lie in different wavefronts");
+    // 32-bit words of block starts per thread: a chunk has at most UPC NB blocks
+    constexpr int NWMAX = UPC * NB / 2 + 2, QW = (NWMAX + GS - 1) / GS;
     static_assert(NB % 2 == 0, "stage positions are read as 32-bit pairs");
     __shared__ T srg[NQ * NB * 3];
-    __shared__ int32_t srb[GS + 1], sru[GS + 1];
+    __shared__ int32_t srb[UPC + 1], sru[UPC + 1];
     __shared__ uint32_t sq[NWMAX];
     extern __shared__ __align__(16) unsigned char smem[];
-    T *sv = reinterpret_cast<T *>(smem); // GS NB staged blocks of 9 values
-    T *sg = sv + GS * NB * 9;            // GS local gradient rows
+    T *sv = reinterpret_cast<T *>(smem); // UPC NB staged blocks of 9 values
+    T *sg = sv + UPC * NB * 9;           // UPC local gradient rows
     const int per = (n_chunks + N_XCD - 1) / N_XCD, xcd = int(blockIdx.x % N_XCD);
     const int stride = int(gridDim.x / N_XCD), c_end = min(n_chunks, (xcd + 1) * per);
     int c = xcd * per + int(blockIdx.x / N_XCD);
     if (c >= c_end) return; // uniform over the workgroup
     const int t = threadIdx.x, last_unit = ctab[n_chunks].y - 1;
+    const int ut = t % UPC, hp = t / UPC; // the thread's unit and half (uniform per wavefront)
     const RefTab<T, ORDER, NQ> &tb = ref_tab<T, ORDER, NQ>();
     for (int k = t; k < NQ * NB * 3; k += GS) srg[k] = tb.rg[k];
     __syncthreads();
@@ -1144,7 +1148,7 @@ __global__ __launch_bounds__(GS) void k_gather_units(int n_chunks, int n_words, 
         return Meta{a.x, b.x - a.x, a.y, b.y - a.y, a.z, b.z - a.z};
     };
     // the unit code (e n_local + a) of thread t, some other unit's past the chunk
-    auto code_of = [&](const Meta &m) { return node_inc[min(m.u0 + t, last_unit)]; };
+    auto code_of = [&](const Meta &m) { return node_inc[min(m.u0 + ut, last_unit)]; };
     // everything else a chunk needs, in registers
     struct Unit {
         T r[R];
@@ -1159,7 +1163,7 @@ __global__ __launch_bounds__(GS) void k_gather_units(int n_chunks, int n_words, 
         const T *rp = static_cast<const T *>(__builtin_assume_aligned(rec, 16)) + int64_t(e) * R;
 #pragma unroll
         for (int i = 0; i < R; ++i) u.r[i] = rp[i];
-        const uint32_t *dp = reinterpret_cast<const uint32_t *>(udst + int64_t(min(m.u0 + t, last_unit)) * NB);
+        const uint32_t *dp = reinterpret_cast<const uint32_t *>(udst + int64_t(min(m.u0 + ut, last_unit)) * NB);
 #pragma unroll
         for (int j = 0; j < DW; ++j) u.dw[j] = dp[j];
         if constexpr (GRAD) {
@@ -1184,25 +1188,32 @@ __global__ __launch_bounds__(GS) void k_gather_units(int n_chunks, int n_words, 
     Meta m0 = meta(c);
     int code0 = code_of(m0);
     Unit u0;
-    load_unit(c, m0, code0, u0);
+    if constexpr (PREFETCH) load_unit(c, m0, code0, u0);
     int c1 = c + stride;
     Meta m1 = meta(c1);
     int code1 = code_of(m1);
     while (true) { // c, c1 and the chunk metadata are uniform over the workgroup
-        // in flight while chunk c is processed: chunk c1's data, chunk c2's codes
+        // in flight while chunk c is processed: chunk c1's data (PREFETCH;
+        // otherwise chunk c's data is loaded here, the registers of a second
+        // chunk left to more resident wavefronts), chunk c2's codes
         Unit u1;
-        load_unit(c1, m1, code1, u1);
+        if constexpr (PREFETCH)
+            load_unit(c1, m1, code1, u1);
+        else
+            load_unit(c, m0, code0, u0);
         const int c2 = c1 + stride;
         const Meta m2 = meta(c2);
         const int code2 = code_of(m2);
         // ---- chunk c: blocks of this thread's unit into the stage
-        if (t < m0.nu) {
+        if (ut < m0.nu) {
             const int a = code0 - (code0 / NB) * NB;
             const T *r = u0.r;
             if constexpr (GRAD) {
-                sg[t * 3] = u0.l0;
-                sg[t * 3 + 1] = u0.l1;
-                sg[t * 3 + 2] = u0.l2;
+                if (hp == 0) {
+                    sg[ut * 3] = u0.l0;
+                    sg[ut * 3 + 1] = u0.l1;
+                    sg[ut * 3 + 2] = u0.l2;
+                }
             }
             const T *G6 = r + 11 * NQ; // 00 11 22 01 02 12
             // per quadrature point: cl k_a, cc k_a and G RG(q, a)
@@ -1221,8 +1232,10 @@ __global__ __launch_bounds__(GS) void k_gather_units(int n_chunks, int n_words, 
                 ha[q][1] = fma_t(G6[5], ra2, fma_t(G6[1], ra1, G6[3] * ra0));
                 ha[q][2] = fma_t(G6[2], ra2, fma_t(G6[5], ra1, G6[4] * ra0));
             }
+            // the blocks b = B0 .. B1 - 1 of this unit's row (b compile-time)
+            auto form = [&]<int B0, int B1>() {
 #pragma unroll
-            for (int b = 0; b < NB; ++b) {
+            for (int b = B0; b < B1; ++b) {
                 T gd = T(0);
 #pragma unroll
                 for (int q = 0; q < NQ; ++q)
@@ -1264,6 +1277,15 @@ __global__ __launch_bounds__(GS) void k_gather_units(int n_chunks, int n_words, 
 #pragma unroll
                 for (int i = 0; i < 9; ++i) o[i] = v[i];
             }
+            };
+            if constexpr (SPLIT == 1) {
+                form.template operator()<0, NB>();
+            } else {
+                if (hp == 0)
+                    form.template operator()<0, NB / 2>();
+                else
+                    form.template operator()<NB / 2, NB>();
+            }
         }
         // ---- chunk c: row and block tables
         if (t <= m0.nr) {
@@ -1282,13 +1304,13 @@ __global__ __launch_bounds__(GS) void k_gather_units(int n_chunks, int n_words, 
         const int total = m0.nu * NB;
         auto block_end = [&](int k) { return k + 1 < m0.nb ? int(sq16[k + 1]) : total; };
         // block k of the row whose blocks are rs .. re - 1 (chunk-local)
+        // (32-bit value indices: the plan is built for fewer than 2^31 values)
         auto write_block = [&](int k, int rs, int re, const T(&acc)[9]) {
-            const int deg = re - rs;
-            T *out = values + 9 * int64_t(m0.B0 + rs) + 3 * (k - rs);
+            const int deg = re - rs, o = 9 * (m0.B0 + rs) + 3 * (k - rs);
 #pragma unroll
             for (int mm = 0; mm < 3; ++mm)
 #pragma unroll
-                for (int nn = 0; nn < 3; ++nn) out[mm * 3 * deg + nn] = acc[mm * 3 + nn];
+                for (int nn = 0; nn < 3; ++nn) values[o + mm * 3 * deg + nn] = acc[mm * 3 + nn];
         };
         // first pieces of split blocks: block, its row's block range
         int nh = 0, hk0 = 0, hs0 = 0, he0 = 0, hk1 = 0, hs1 = 0, he1 = 0;
@@ -1300,11 +1322,11 @@ __global__ __launch_bounds__(GS) void k_gather_units(int n_chunks, int n_words, 
                 int q0 = sq16[k], q1 = block_end(k), qn = block_end(k + 1);
                 while (true) {
                     const int s = q0 + j * L, pe = min(q1, s + L);
+                    // the piece's first contribution, then two in flight, in unit order
                     T acc[9];
 #pragma unroll
-                    for (int i = 0; i < 9; ++i) acc[i] = T(0);
-                    // two contributions in flight, added in unit order
-                    int x = s;
+                    for (int i = 0; i < 9; ++i) acc[i] = s < pe ? sv[s * 9 + i] : T(0);
+                    int x = s + 1;
                     for (; x + 2 <= pe; x += 2) {
                         T c0[9], c1[9];
 #pragma unroll
@@ -1393,7 +1415,7 @@ __global__ __launch_bounds__(GS) void k_gather_units(int n_chunks, int n_words, 
         __syncthreads(); // the stage and tables are read before chunk c1 overwrites them
         c = c1;
         m0 = m1;
-        u0 = u1;
+        if constexpr (PREFETCH) u0 = u1;
         code0 = code1;
         c1 = c2;
         m1 = m2;
@@ -1702,14 +1724,15 @@ template <typename T, int O, int Q> void launch_factored_gather(Handle &h, Gathe
     PF_LAUNCH_CHECK();
 }
 
-// the row-unit plan of workgroup size gs (layout at k_gather_units, built by
-// make_unit_plan); false when a node has more incident elements than one
-// workgroup holds (the per-contribution gather then serves the handle)
-bool build_unit_plan(Handle &h, UnitPlan &p, int gs)
+// the row-unit plan of workgroups of gs threads, split threads per unit
+// (layout at k_gather_units, built by make_unit_plan); false when a node has
+// more incident elements than one chunk holds (the per-contribution gather
+// then serves the handle)
+bool build_unit_plan(Handle &h, UnitPlan &p, int gs, int split)
 {
-    if (p.threads == gs) return true;
+    if (p.threads == gs && p.split == split) return true;
     UnitPlanHost u;
-    if (!make_unit_plan(h.pat, h.n_local, h.h_elem_nodes.data(), gs, u)) return false;
+    if (!make_unit_plan(h.pat, h.n_local, h.h_elem_nodes.data(), gs / split, gs, u)) return false;
     // per chunk boundary: first row, first unit, first pattern block
     const int nc = int(u.chunk_row.size()) - 1;
     std::vector<int32_t> ctab(size_t(nc + 1) * 4, 0);
@@ -1731,6 +1754,7 @@ bool build_unit_plan(Handle &h, UnitPlan &p, int gs)
     p.lanes = upload(u.lanes, h.stream);
     p.n_chunks = nc;
     p.threads = gs;
+    p.split = split;
     p.grid = p.grid_grad = 0;
     return true;
 }
@@ -1740,9 +1764,11 @@ bool build_unit_plan(Handle &h, UnitPlan &p, int gs)
 template <typename T, int O, int Q> void launch_unit_gather(Handle &h, UnitPlan &p, const T *gloc, double *g, void *values)
 {
     constexpr int NB = NLocal<O>::value;
-    auto gath = [&]<int GS, bool GRAD>() {
-        auto kern = k_gather_units<T, O, Q, GS, GRAD>;
-        const int lds = int(sizeof(T)) * GS * (NB * 9 + 3);
+    auto gath = [&]<int GS, bool GRAD, int SPLIT>() {
+        // two threads per unit: no prefetch of the next chunk (fewer
+        // registers, more resident wavefronts)
+        auto kern = k_gather_units<T, O, Q, GS, GRAD, SPLIT, SPLIT == 1>;
+        const int lds = int(sizeof(T)) * (GS / SPLIT) * (NB * 9 + 3);
         unsigned &grid = GRAD ? p.grid_grad : p.grid;
         if (grid == 0) {
             PF_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(kern), hipFuncAttributeMaxDynamicSharedMemorySize, lds));
@@ -1760,11 +1786,11 @@ template <typename T, int O, int Q> void launch_unit_gather(Handle &h, UnitPlan 
     };
     auto sized = [&]<bool GRAD>() {
         if (p.threads == 64)
-            gath.template operator()<64, GRAD>();
+            gath.template operator()<64, GRAD, 1>();
         else if (p.threads == 256)
-            gath.template operator()<256, GRAD>();
+            gath.template operator()<256, GRAD, 1>();
         else
-            gath.template operator()<128, GRAD>();
+            gath.template operator()<128, GRAD, 2>();
     };
     if (gloc)
         sized.template operator()<true>();
@@ -1781,17 +1807,21 @@ bool unit_plan_ready(Handle &h)
     if (h.material != MAT_NH && h.material != MAT_LE)
         fail(PFGPU_ERR_UNSUPPORTED, "the factored Hessian covers NeoHookean and linear elasticity");
     ensure_plain_hessian_maps(h);
+    // 64: a chunk of 64 units, one thread each; 128: 64 units, two threads
+    // each (in different wavefronts, each forming half of the blocks); 256:
+    // 256 units, one thread each
     int gs = h.tune_gather_threads;
-    if (gs > 256) return false;
-    // the staged blocks must fit the 160 KB of LDS (FP64 P2 at 256 threads
-    // does not): the largest fitting size at or below the tuned one
+    if (gs > 256 || 9 * int64_t(h.pat.n_blocks) >= (int64_t(1) << 31)) return false;
     // dynamic stage + the static tables (row tables, block starts, reference gradients)
     auto lds = [&](int g) {
-        return int64_t(h.value_size()) * g * (h.n_local * 9 + 3) + 8 * (g + 1) + 4 * (g * h.n_local / 2 + 2) + 1024;
+        const int u = g == 128 ? 64 : g;
+        return int64_t(h.value_size()) * u * (h.n_local * 9 + 3) + 8 * (u + 1) + 4 * (u * h.n_local / 2 + 2) + 1024;
     };
+    // the staged blocks must fit the 160 KB of LDS (FP64 P2 at 256 threads
+    // does not): the largest fitting size at or below the tuned one
     while (gs > 64 && lds(gs) > 160 * 1024) gs /= 2;
     if (lds(gs) > 160 * 1024) return false;
-    return build_unit_plan(h, h.unit_plan, gs);
+    return build_unit_plan(h, h.unit_plan, gs, gs == 128 ? 2 : 1);
 }
 
 void factored_hessian(Handle &h, const double *x, void *values)

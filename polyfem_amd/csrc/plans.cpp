@@ -71,7 +71,8 @@ void chunk_lanes(const std::vector<Piece> &p, int nb, int nr, int gs, uint32_t *
 
 } // namespace
 
-bool make_unit_plan(const Pattern &pat, int n_local, const int32_t *elem_nodes, int gs, UnitPlanHost &out)
+bool make_unit_plan(const Pattern &pat, int n_local, const int32_t *elem_nodes, int gs, int n_lanes,
+                    UnitPlanHost &out)
 {
     const std::vector<int32_t> &np = pat.node_ptr, &ni = pat.node_inc, &bp = pat.brow_ptr, &bc = pat.bcol;
     const int NB = n_local, N = pat.n_nodes;
@@ -93,7 +94,7 @@ bool make_unit_plan(const Pattern &pat, int n_local, const int32_t *elem_nodes, 
     c.push_back(N);
     const int nc = int(c.size()) - 1;
     std::vector<uint16_t> q0(size_t(pat.n_blocks)), dst(size_t(np[size_t(N)]) * NB);
-    std::vector<uint32_t> lanes(size_t(nc) * size_t(gs + 1));
+    std::vector<uint32_t> lanes(size_t(nc) * size_t(n_lanes + 1));
     std::atomic<bool> fits{true};
     // per chunk: every block's contributions are consecutive stage positions
     // (blocks in pattern order, contributions in unit order); a unit's block
@@ -140,7 +141,7 @@ bool make_unit_plan(const Pattern &pat, int n_local, const int32_t *elem_nodes, 
                     for (int j = 0; j < pc; ++j)
                         pieces.push_back(Piece{k, j, i - i0, std::min(UNIT_PIECE, n - j * UNIT_PIECE), j == 0 && pc > 1});
                 }
-            chunk_lanes(pieces, nb, i1 - i0, gs, lanes.data() + size_t(ci) * size_t(gs + 1));
+            chunk_lanes(pieces, nb, i1 - i0, n_lanes, lanes.data() + size_t(ci) * size_t(n_lanes + 1));
         }
     };
     const int nt = int(std::max(1u, std::min(16u, std::thread::hardware_concurrency())));

@@ -22,9 +22,10 @@ namespace pfgpu {
 // pieces of UNIT_PIECE consecutive contributions (the last shorter); a split
 // block is the sum of its piece sums in piece order, each piece summed in
 // unit order.  The pieces depend on n alone, so the summation order of a
-// block does not depend on gs.  lanes[c (gs + 1) + t] is the first piece of
-// thread t in chunk c (threads take consecutive runs of pieces, balanced by
-// contributions; entry gs is the end), packed by unit_lane(): the block
+// block does not depend on gs.  lanes[c (n_lanes + 1) + t] is the first
+// piece of thread t of the n_lanes summing threads in chunk c (threads take
+// consecutive runs of pieces, balanced by contributions; entry n_lanes is the
+// end), packed by unit_lane(): the block
 // (chunk-local), the piece within the block and the block's row
 // (chunk-local).  A thread's run holds the first piece of at most two split
 // blocks.  False when a row has more than min(gs, 255) units.
@@ -38,6 +39,7 @@ struct UnitPlanHost {
     std::vector<uint16_t> udst, bq0;
     std::vector<uint32_t> lanes;
 };
-bool make_unit_plan(const Pattern &pat, int n_local, const int32_t *elem_nodes, int gs, UnitPlanHost &out);
+bool make_unit_plan(const Pattern &pat, int n_local, const int32_t *elem_nodes, int gs, int n_lanes,
+                    UnitPlanHost &out);
 
 } // namespace pfgpu

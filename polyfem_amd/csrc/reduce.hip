@@ -5,14 +5,30 @@ namespace pfgpu {
 
 namespace {
 
+// in[i0], in[i0 + BLOCK], ... below i1, added in that order with eight loads
+// in flight (the order, and so the result, is that of the plain loop)
+__device__ __forceinline__ double strided_sum(const double *__restrict__ in, int i0, int i1)
+{
+    double v = 0;
+    int i = i0;
+    for (; i + 7 * BLOCK < i1; i += 8 * BLOCK) {
+        double a[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) a[k] = in[i + k * BLOCK];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) v += a[k];
+    }
+    for (; i < i1; i += BLOCK) v += in[i];
+    return v;
+}
+
 // Single block: thread t sums in[t], in[t+256], ... in order, then a fixed
 // tree.  The summation order depends only on n, never on scheduling.
 __global__ __launch_bounds__(BLOCK) void k_sum_doubles(const double *__restrict__ in, int n, double *__restrict__ out)
 {
     __shared__ double s[BLOCK];
     const int t = threadIdx.x;
-    double v = 0;
-    for (int i = t; i < n; i += BLOCK) v += in[i];
+    const double v = strided_sum(in, t, n);
     s[t] = v;
     __syncthreads();
     for (int w = BLOCK / 2; w > 0; w >>= 1) {
@@ -30,8 +46,7 @@ __global__ __launch_bounds__(BLOCK) void k_sum_doubles_blocks(const double *__re
     __shared__ double s[BLOCK];
     const int t = threadIdx.x;
     const int i0 = blockIdx.x * seg, i1 = min(n, i0 + seg);
-    double v = 0;
-    for (int i = i0 + t; i < i1; i += BLOCK) v += in[i];
+    const double v = strided_sum(in, i0 + t, i1);
     s[t] = v;
     __syncthreads();
     for (int w = BLOCK / 2; w > 0; w >>= 1) {

@@ -63,16 +63,20 @@ int main(int argc, char **argv)
 
     const Pattern pat = build_pattern(E, NB, en.data(), N, Nv, ev.data());
     const auto &np = pat.node_ptr, &ni = pat.node_inc, &bp = pat.brow_ptr, &bc = pat.bcol;
-    for (int gs : {64, 128, 256}) {
+    // (units per chunk, summing threads): one thread per unit, or two per
+    // unit (each forming half of its blocks)
+    const int configs[4][2] = {{64, 64}, {128, 128}, {256, 256}, {64, 128}};
+    for (const auto &cfg : configs) {
+        const int gs = cfg[0], nl = cfg[1];
         UnitPlanHost u;
-        if (!make_unit_plan(pat, NB, en.data(), gs, u)) {
+        if (!make_unit_plan(pat, NB, en.data(), gs, nl, u)) {
             std::printf("gs %d: a row has too many units (the per-contribution gather serves)\n", gs);
             continue;
         }
         const int nc = int(u.chunk_row.size()) - 1;
         CHECK(u.chunk_row.front() == 0 && u.chunk_row.back() == N, "chunk rows do not cover the nodes");
         CHECK(u.udst.size() == size_t(np[N]) * NB && u.bq0.size() == size_t(pat.n_blocks) &&
-                  u.lanes.size() == size_t(nc) * size_t(gs + 1),
+                  u.lanes.size() == size_t(nc) * size_t(nl + 1),
               "plan sizes");
         std::vector<uint8_t> seen;
         for (int c = 0; c < nc; ++c) {
@@ -125,7 +129,7 @@ int main(int argc, char **argv)
             for (int d = 0; d < total; ++d) CHECK(seen[size_t(d)], "chunk %d: stage %d never written", c, d);
             CHECK(nr <= gs - 1, "chunk %d: %d rows > %d", c, nr, gs - 1);
             // 4. the kernel's walk over the lane entries
-            const uint32_t *ln = u.lanes.data() + size_t(c) * size_t(gs + 1);
+            const uint32_t *ln = u.lanes.data() + size_t(c) * size_t(nl + 1);
             std::vector<int> read(size_t(total), 0), written(size_t(nb), 0), partial(size_t(total), 0);
             std::vector<int> srb(size_t(nr) + 1);
             for (int r = 0; r <= nr; ++r) srb[size_t(r)] = bp[i0 + r] - B0;
@@ -133,7 +137,7 @@ int main(int argc, char **argv)
             auto bend = [&](int k) { return k + 1 < nb ? bq(k + 1) : total; };
             auto row_of = [&](int k) { return int(std::upper_bound(srb.begin(), srb.end(), k) - srb.begin()) - 1; };
             std::vector<std::pair<int, int>> heads;
-            for (int t = 0; t < gs; ++t) {
+            for (int t = 0; t < nl; ++t) {
                 int k = int(ln[t] & 0xfffu), j = int((ln[t] >> 12) & 0xffu), rr = int(ln[t] >> 20);
                 const int ke = int(ln[t + 1] & 0xfffu), je = int((ln[t + 1] >> 12) & 0xffu);
                 if (!((k != ke || j != je) && k < nb)) continue;
@@ -178,7 +182,7 @@ int main(int argc, char **argv)
             for (int d = 0; d < total; ++d) CHECK(read[size_t(d)] == 1, "chunk %d: stage %d read %d times", c, d, read[size_t(d)]);
             for (int k = 0; k < nb; ++k) CHECK(written[size_t(k)] == 1, "chunk %d: block %d written %d times", c, k, written[size_t(k)]);
         }
-        std::printf("gs %d: %d chunks checked\n", gs, nc);
+        std::printf("gs %d lanes %d: %d chunks checked\n", gs, nl, nc);
     }
     // the fused kernel's tables, built for the FP32 P1/P2 launch shape (the
     // sanitizers watch the builder; fused_check verifies their content)

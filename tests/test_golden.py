@@ -17,7 +17,7 @@ from polyfem_amd.mesh import kuhn_box
 HERE = os.path.dirname(os.path.abspath(__file__))
 FIXTURES = sorted(glob.glob(os.path.join(HERE, "golden", "*.npz")))
 MAT = {"NeoHookean": po.NEOHOOKEAN, "LinearElasticity": po.LINEAR_ELASTICITY}
-MIXED_TOL = {"energy": 1e-5, "gradient": 2e-5, "hessian": 2e-5, "hvp": 2e-5, "vjp": 5e-5}
+MIXED_TOL = {"energy": 2e-7, "gradient": 1e-6, "hessian": 1e-6, "hvp": 1e-6, "vjp": 1e-6}
 
 
 def rel(a, b):

@@ -14,9 +14,11 @@ from polyfem_amd.mesh import BASE_SEED, kuhn_box, lame_from_young, random_lame, 
 pytestmark = pytest.mark.gpu
 
 FP64_TOL = 1e-10
-# FP32 element kernels: ~1e-7 per operation; sums of O(10) terms with mixed
-# signs and the conditioning of ln J leave ~1e-5 relative on the arrays.
-MIXED_TOL = {"energy": 1e-5, "gradient": 2e-5, "hessian": 2e-5, "hvp": 2e-5, "vjp": 5e-5}
+# FP32 element kernels with FP64 sums: the errors measured against the oracle
+# over every path, material and order on these states are at most 5.4e-8
+# (energy), 3.1e-7 (gradient), 2.2e-7 (Hessian), 1.9e-7 (Hv) and 3.3e-7
+# (VJPs); the bars sit about three times above.
+MIXED_TOL = {"energy": 2e-7, "gradient": 1e-6, "hessian": 1e-6, "hvp": 1e-6, "vjp": 1e-6}
 
 PATHS = ["fused", "plain", "factored"]
 CASES = [(1, "NeoHookean"), (2, "NeoHookean"), (1, "LinearElasticity"), (2, "LinearElasticity")]

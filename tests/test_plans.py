@@ -66,4 +66,4 @@ def test_plans_high_valence(exe, tmp_path, order):
     r = subprocess.run([exe, str(p)], capture_output=True, text=True, timeout=600)
     assert r.returncode == 0, (r.stdout[-2000:], r.stderr[-4000:])
     assert "OK (0 failures)" in r.stdout
-    assert "gs 64: " in r.stdout and "too many units" not in r.stdout.split("gs 64: ")[1].split("\n")[0]
+    assert "gs 64 lanes 64: " in r.stdout and "gs 64: a row has too many units" not in r.stdout
