@@ -1548,40 +1548,18 @@ void plain_gradient(Handle &h, const double *x, double *g, double *energy)
 
 // chunk starts over items 0..n-1 with contribution offsets ptr: consecutive
 // items, at most SEG contributions per chunk
-template <typename P> std::vector<int32_t> segment_chunks(const P *ptr, int64_t n, int cap = SEG)
-{
-    std::vector<int32_t> c{0};
-    int64_t k0 = ptr[0];
-    for (int64_t i = 0; i < n; ++i) {
-        if (ptr[i + 1] - ptr[i] > cap) fail(PFGPU_ERR_UNSUPPORTED, "an output gathers more contributions than a gather workgroup holds");
-        if (ptr[i + 1] - k0 > cap) {
-            c.push_back(int32_t(i));
-            k0 = ptr[i];
-        }
-    }
-    c.push_back(int32_t(n));
-    return c;
-}
-
-// the row plan of k_gather_rows_seg, from the node incidence
+// the row plan of k_gather_rows_seg, from the node incidence (make_row_plan)
 void ensure_row_chunks(Handle &h)
 {
     if (h.row_chunk.get()) return;
-    const std::vector<int32_t> &ptr = h.pat.node_ptr, &inc = h.pat.node_inc;
-    const std::vector<int32_t> c = segment_chunks(ptr.data(), h.n_nodes);
-    const int nc = int(c.size()) - 1;
-    std::vector<int32_t> slot(size_t(nc) * SEG, -1), rmeta(size_t(h.n_nodes));
-    for (int ci = 0; ci < nc; ++ci) {
-        const int32_t k0 = ptr[size_t(c[ci])];
-        for (int32_t i = c[ci]; i < c[ci + 1]; ++i) {
-            rmeta[size_t(i)] = (ptr[size_t(i)] - k0) | ((ptr[size_t(i) + 1] - ptr[size_t(i)]) << 10);
-            for (int32_t k = ptr[size_t(i)]; k < ptr[size_t(i) + 1]; ++k) slot[size_t(ci) * SEG + size_t(k - k0)] = inc[size_t(k)];
-        }
-    }
-    h.row_slot = upload(slot, h.stream);
-    h.row_meta = upload(rmeta, h.stream);
-    h.row_chunk = upload(c, h.stream);
-    h.n_row_chunks = nc;
+    static_assert(ROW_SEG == SEG, "the row plan's chunk capacity is the row gather's workgroup");
+    RowPlanHost r;
+    if (!make_row_plan(h.pat, r))
+        fail(PFGPU_ERR_UNSUPPORTED, "an output gathers more contributions than a gather workgroup holds");
+    h.row_slot = upload(r.slot, h.stream);
+    h.row_meta = upload(r.rmeta, h.stream);
+    h.row_chunk = upload(r.chunk, h.stream);
+    h.n_row_chunks = int(r.chunk.size()) - 1;
 }
 
 void gather_rows(Handle &h, const void *loc, double *out)
@@ -1623,57 +1601,28 @@ void ensure_plain_hessian_maps(Handle &h)
 }
 
 // the gather plan of workgroup size gs for slot words word(k) of the
-// contributions k (layout described at k_gather_blocks_seg); by_element:
-// each chunk's slots reordered by key(word), with their block-order
-// positions in dst (the factored gather's layout)
+// contributions k (layout described at k_gather_blocks_seg, built by
+// make_contrib_plan); by_element: each chunk's slots reordered by key(word),
+// with their block-order positions in dst (the factored gather's layout)
 template <typename W, typename K> void build_plan(Handle &h, GatherPlan &p, int gs, W &&word, bool by_element, K &&key)
 {
     if (p.threads == gs) return;
     const std::vector<int32_t> &ptr = h.h_contrib_ptr;
-    const std::vector<int32_t> c = segment_chunks(ptr.data(), h.pat.n_blocks, gs);
-    const int nc = int(c.size()) - 1;
-    std::vector<int32_t> slot(size_t(nc) * gs, -1), meta(size_t(h.pat.n_blocks));
-    for (int i = 0; i < h.n_nodes; ++i) {
-        const int32_t deg = h.pat.brow_ptr[i + 1] - h.pat.brow_ptr[i];
-        if (deg > 4095) fail(PFGPU_ERR_UNSUPPORTED, "a node has more than 4095 neighbours");
-        for (int32_t b = h.pat.brow_ptr[i]; b < h.pat.brow_ptr[i + 1]; ++b) meta[size_t(b)] = deg << 20;
+    const size_t nk = size_t(ptr[size_t(h.pat.n_blocks)]);
+    std::vector<int32_t> words(nk), keys(by_element ? nk : 0);
+    for (size_t k = 0; k < nk; ++k) {
+        words[k] = word(int32_t(k));
+        if (by_element) keys[k] = key(words[k]);
     }
-    std::vector<uint16_t> dst(by_element ? size_t(nc) * gs : 0, 0);
-    // chunks are independent: fill them on the host's threads
-    auto fill = [&](int c0, int c1) {
-        std::vector<int64_t> kv;
-        std::vector<int32_t> w;
-        for (int ci = c0; ci < c1; ++ci) {
-            const int32_t k0 = ptr[size_t(c[ci])], n = ptr[size_t(c[ci + 1])] - k0;
-            int32_t *sl = slot.data() + size_t(ci) * gs;
-            for (int32_t b = c[ci]; b < c[ci + 1]; ++b) {
-                meta[size_t(b)] |= (ptr[size_t(b)] - k0) | ((ptr[size_t(b) + 1] - ptr[size_t(b)]) << 10);
-                for (int32_t k = ptr[size_t(b)]; k < ptr[size_t(b) + 1]; ++k) sl[k - k0] = word(k);
-            }
-            if (!by_element) continue;
-            // the chunk's slots ordered by element, ties in block order, with
-            // the block-order position of every contribution; the element of
-            // word w is key(w)
-            kv.resize(size_t(n));
-            w.assign(sl, sl + n);
-            for (int32_t i = 0; i < n; ++i) kv[size_t(i)] = (int64_t(key(w[size_t(i)])) << 16) | i;
-            std::sort(kv.begin(), kv.end());
-            for (int32_t i = 0; i < n; ++i) {
-                const int32_t o = int32_t(kv[size_t(i)] & 0xffff);
-                sl[i] = w[size_t(o)];
-                dst[size_t(ci) * gs + size_t(i)] = uint16_t(o);
-            }
-        }
-    };
-    const int nt = int(std::max(1u, std::min(16u, std::thread::hardware_concurrency())));
-    std::vector<std::thread> pool;
-    for (int t = 0; t < nt; ++t) pool.emplace_back(fill, int(int64_t(nc) * t / nt), int(int64_t(nc) * (t + 1) / nt));
-    for (std::thread &th : pool) th.join();
-    if (by_element) p.dst = upload(dst, h.stream);
-    p.chunk = upload(c, h.stream);
-    p.slot = upload(slot, h.stream);
-    p.meta = upload(meta, h.stream);
-    p.n_chunks = nc;
+    ContribPlanHost c;
+    if (!make_contrib_plan(h.pat, ptr, words, by_element ? &keys : nullptr, gs, c))
+        fail(PFGPU_ERR_UNSUPPORTED, "an output gathers more contributions than a gather workgroup holds, "
+                                    "or a node has more than 4095 neighbours");
+    if (by_element) p.dst = upload(c.dst, h.stream);
+    p.chunk = upload(c.chunk, h.stream);
+    p.slot = upload(c.slot, h.stream);
+    p.meta = upload(c.meta, h.stream);
+    p.n_chunks = int(c.chunk.size()) - 1;
     p.threads = gs;
     p.grid = 0;
 }

@@ -156,4 +156,89 @@ bool make_unit_plan(const Pattern &pat, int n_local, const int32_t *elem_nodes, 
     return true;
 }
 
+bool segment_chunks(const int32_t *ptr, int64_t n, int cap, std::vector<int32_t> &c)
+{
+    c.assign(1, 0);
+    int64_t k0 = ptr[0];
+    for (int64_t i = 0; i < n; ++i) {
+        if (ptr[i + 1] - ptr[i] > cap) return false;
+        if (ptr[i + 1] - k0 > cap) {
+            c.push_back(int32_t(i));
+            k0 = ptr[i];
+        }
+    }
+    c.push_back(int32_t(n));
+    return true;
+}
+
+bool make_row_plan(const Pattern &pat, RowPlanHost &out)
+{
+    const std::vector<int32_t> &ptr = pat.node_ptr, &inc = pat.node_inc;
+    std::vector<int32_t> c;
+    if (!segment_chunks(ptr.data(), pat.n_nodes, ROW_SEG, c)) return false;
+    const int nc = int(c.size()) - 1;
+    std::vector<int32_t> slot(size_t(nc) * ROW_SEG, -1), rmeta(size_t(pat.n_nodes));
+    for (int ci = 0; ci < nc; ++ci) {
+        const int32_t k0 = ptr[size_t(c[size_t(ci)])];
+        for (int32_t i = c[size_t(ci)]; i < c[size_t(ci) + 1]; ++i) {
+            rmeta[size_t(i)] = (ptr[size_t(i)] - k0) | ((ptr[size_t(i) + 1] - ptr[size_t(i)]) << 10);
+            for (int32_t k = ptr[size_t(i)]; k < ptr[size_t(i) + 1]; ++k)
+                slot[size_t(ci) * ROW_SEG + size_t(k - k0)] = inc[size_t(k)];
+        }
+    }
+    out.chunk = std::move(c);
+    out.slot = std::move(slot);
+    out.rmeta = std::move(rmeta);
+    return true;
+}
+
+bool make_contrib_plan(const Pattern &pat, const std::vector<int32_t> &ptr, const std::vector<int32_t> &words,
+                       const std::vector<int32_t> *keys, int gs, ContribPlanHost &out)
+{
+    std::vector<int32_t> c;
+    if (!segment_chunks(ptr.data(), pat.n_blocks, gs, c)) return false;
+    const int nc = int(c.size()) - 1;
+    std::vector<int32_t> slot(size_t(nc) * gs, -1), meta(size_t(pat.n_blocks));
+    for (int i = 0; i < pat.n_nodes; ++i) {
+        const int32_t deg = pat.brow_ptr[size_t(i) + 1] - pat.brow_ptr[size_t(i)];
+        if (deg > 4095) return false;
+        for (int32_t b = pat.brow_ptr[size_t(i)]; b < pat.brow_ptr[size_t(i) + 1]; ++b) meta[size_t(b)] = deg << 20;
+    }
+    std::vector<uint16_t> dst(keys ? size_t(nc) * gs : 0, 0);
+    // chunks are independent: fill them on the host's threads
+    auto fill = [&](int c0, int c1) {
+        std::vector<int64_t> kv;
+        std::vector<int32_t> w;
+        for (int ci = c0; ci < c1; ++ci) {
+            const int32_t k0 = ptr[size_t(c[size_t(ci)])], n = ptr[size_t(c[size_t(ci) + 1])] - k0;
+            int32_t *sl = slot.data() + size_t(ci) * gs;
+            for (int32_t b = c[size_t(ci)]; b < c[size_t(ci) + 1]; ++b) {
+                meta[size_t(b)] |= (ptr[size_t(b)] - k0) | ((ptr[size_t(b) + 1] - ptr[size_t(b)]) << 10);
+                for (int32_t k = ptr[size_t(b)]; k < ptr[size_t(b) + 1]; ++k) sl[k - k0] = words[size_t(k)];
+            }
+            if (!keys) continue;
+            // the chunk's slots ordered by key (element), ties in block order,
+            // with the block-order position of every contribution
+            kv.resize(size_t(n));
+            w.assign(sl, sl + n);
+            for (int32_t i = 0; i < n; ++i) kv[size_t(i)] = (int64_t((*keys)[size_t(k0 + i)]) << 16) | i;
+            std::sort(kv.begin(), kv.end());
+            for (int32_t i = 0; i < n; ++i) {
+                const int32_t o = int32_t(kv[size_t(i)] & 0xffff);
+                sl[i] = w[size_t(o)];
+                dst[size_t(ci) * gs + size_t(i)] = uint16_t(o);
+            }
+        }
+    };
+    const int nt = int(std::max(1u, std::min(16u, std::thread::hardware_concurrency())));
+    std::vector<std::thread> pool;
+    for (int t = 0; t < nt; ++t) pool.emplace_back(fill, int(int64_t(nc) * t / nt), int(int64_t(nc) * (t + 1) / nt));
+    for (std::thread &th : pool) th.join();
+    out.chunk = std::move(c);
+    out.slot = std::move(slot);
+    out.meta = std::move(meta);
+    out.dst = std::move(dst);
+    return true;
+}
+
 } // namespace pfgpu

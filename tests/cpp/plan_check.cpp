@@ -2,9 +2,10 @@
 // AddressSanitizer / UndefinedBehaviorSanitizer.
 //
 // Builds the pattern (pattern.cpp), the row-unit plans of the factored
-// gather for 64, 128 and 256 threads (plans.cpp) and the fused kernel's
-// patch tables (fused_tables.cpp) exactly as the library does, then checks
-// every index the unit-gather kernel derives from its plan:
+// gather (64 and 256 units with one thread each, 64 units with two), the row
+// plan of the node-row gathers and the per-contribution plan (plans.cpp) and
+// the fused kernel's patch tables (fused_tables.cpp) exactly as the library
+// does, then checks every index the kernels derive from them:
 //  1. chunks cover the rows in order, with at most GS units and GS rows;
 //  2. every unit's n_local stage positions lie inside the chunk's stage
 //     (units x n_local <= GS x n_local slots) and inside the range of the
@@ -17,7 +18,12 @@
 //     read every staged contribution once, write every block of the chunk
 //     once (whole, or through the piece sums of a split block: pieces of
 //     UNIT_PIECE, in order), stay inside the stage, the chunk's blocks and
-//     rows, and hold at most two first pieces of split blocks.
+//     rows, and hold at most two first pieces of split blocks;
+//  5. the row plan of the node-row gathers lists every row's incidences in
+//     order, within ROW_SEG slots per chunk;
+//  6. the per-contribution plan of the factored gather holds every block's
+//     contributions once, in element order within a chunk, with the right
+//     block metadata.
 // Input: a binary mesh file (order, E, N, Nv, elem_nodes, elem_vertices,
 // vertices) written by tests/test_plans.py.
 #include <algorithm>
@@ -183,6 +189,69 @@ int main(int argc, char **argv)
             for (int k = 0; k < nb; ++k) CHECK(written[size_t(k)] == 1, "chunk %d: block %d written %d times", c, k, written[size_t(k)]);
         }
         std::printf("gs %d lanes %d: %d chunks checked\n", gs, nl, nc);
+    }
+    // 5. the row plan of the node-row gathers
+    {
+        RowPlanHost r;
+        CHECK(make_row_plan(pat, r), "row plan refused");
+        const int nc = int(r.chunk.size()) - 1;
+        CHECK(r.chunk.front() == 0 && r.chunk.back() == N, "row chunks do not cover the nodes");
+        for (int c = 0; c < nc; ++c) {
+            const int i0 = r.chunk[c], i1 = r.chunk[c + 1], k0 = np[i0], tot = np[i1] - k0;
+            CHECK(i1 > i0 && tot <= ROW_SEG, "row chunk %d: rows [%d, %d), %d incidences", c, i0, i1, tot);
+            for (int i = i0; i < i1; ++i) {
+                const int q0 = r.rmeta[i] & 1023, cnt = r.rmeta[i] >> 10;
+                CHECK(q0 == np[i] - k0 && cnt == np[i + 1] - np[i], "row %d: meta %d/%d", i, q0, cnt);
+                for (int j = 0; j < cnt; ++j)
+                    CHECK(r.slot[size_t(c) * ROW_SEG + q0 + j] == ni[np[i] + j], "row %d incidence %d", i, j);
+            }
+            for (int k = tot; k < ROW_SEG; ++k) CHECK(r.slot[size_t(c) * ROW_SEG + k] == -1, "row chunk %d slot %d used", c, k);
+        }
+        std::printf("row plan: %d chunks checked\n", nc);
+    }
+    // 6. the per-contribution plan of the factored gather (element-ordered slots)
+    {
+        std::vector<int64_t> ptr64;
+        std::vector<int32_t> codes, block_row;
+        build_block_contributions(pat, NB, en.data(), ptr64, codes, block_row);
+        const std::vector<int32_t> ptr(ptr64.begin(), ptr64.end());
+        std::vector<int32_t> keys(codes.size());
+        for (size_t k = 0; k < codes.size(); ++k) keys[k] = codes[k] / (NB * NB);
+        for (int gs : {128, 256, 512}) {
+            ContribPlanHost cp;
+            if (!make_contrib_plan(pat, ptr, codes, &keys, gs, cp)) {
+                std::printf("contribution plan gs %d refused\n", gs);
+                continue;
+            }
+            const int nc = int(cp.chunk.size()) - 1;
+            CHECK(cp.chunk.front() == 0 && cp.chunk.back() == pat.n_blocks, "contribution chunks do not cover the blocks");
+            for (int c = 0; c < nc; ++c) {
+                const int b0 = cp.chunk[c], b1 = cp.chunk[c + 1], k0 = ptr[b0], n = ptr[b1] - k0;
+                CHECK(b1 > b0 && n <= gs, "contribution chunk %d: %d contributions > %d", c, n, gs);
+                std::vector<int> hit(size_t(n), 0);
+                for (int i = 0; i < n; ++i) {
+                    const int o = cp.dst[size_t(c) * gs + i];
+                    CHECK(o >= 0 && o < n, "chunk %d slot %d: position %d", c, i, o);
+                    if (o < 0 || o >= n) continue;
+                    hit[size_t(o)]++;
+                    CHECK(cp.slot[size_t(c) * gs + i] == codes[size_t(k0 + o)], "chunk %d slot %d: word", c, i);
+                    if (i > 0) {
+                        const int po = cp.dst[size_t(c) * gs + i - 1];
+                        const int ka = keys[size_t(k0 + po)], kb = keys[size_t(k0 + o)];
+                        CHECK(ka < kb || (ka == kb && po < o), "chunk %d slots %d/%d not in element order", c, i - 1, i);
+                    }
+                }
+                for (int o = 0; o < n; ++o) CHECK(hit[size_t(o)] == 1, "chunk %d: position %d taken %d times", c, o, hit[size_t(o)]);
+                for (int k = n; k < gs; ++k) CHECK(cp.slot[size_t(c) * gs + k] == -1, "chunk %d slot %d used", c, k);
+                for (int b = b0; b < b1; ++b) {
+                    const int q0 = cp.meta[b] & 1023, cnt = (cp.meta[b] >> 10) & 1023, deg = int(uint32_t(cp.meta[b]) >> 20);
+                    const int row = block_row[size_t(b)];
+                    CHECK(q0 == ptr[b] - k0 && cnt == ptr[b + 1] - ptr[b] && deg == bp[row + 1] - bp[row],
+                          "block %d: meta %d/%d/%d", b, q0, cnt, deg);
+                }
+            }
+            std::printf("contribution plan gs %d: %d chunks checked\n", gs, nc);
+        }
     }
     // the fused kernel's tables, built for the FP32 P1/P2 launch shape (the
     // sanitizers watch the builder; fused_check verifies their content)
