@@ -1322,10 +1322,11 @@ __global__ __launch_bounds__(GS) void k_gather_units(int n_chunks, int n_words, 
                 int q0 = sq16[k], q1 = block_end(k), qn = block_end(k + 1);
                 while (true) {
                     const int s = q0 + j * L, pe = min(q1, s + L);
-                    // the piece's first contribution, then two in flight, in unit order
+                    // the piece's first contribution (a piece is never empty:
+                    // s < pe), then two in flight, in unit order
                     T acc[9];
 #pragma unroll
-                    for (int i = 0; i < 9; ++i) acc[i] = s < pe ? sv[s * 9 + i] : T(0);
+                    for (int i = 0; i < 9; ++i) acc[i] = sv[s * 9 + i];
                     int x = s + 1;
                     for (; x + 2 <= pe; x += 2) {
                         T c0[9], c1[9];
