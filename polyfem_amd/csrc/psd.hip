@@ -13,6 +13,17 @@
 // round-robin ordering (n*3/2 disjoint rotations per round, all applied at
 // once, n*3-1 rounds per sweep) until the off-diagonal part is at rounding
 // level, and written back when some eigenvalue is negative.
+//
+// FP32 P2 matrices take a second method built on 32x32 matrix products on
+// the matrix cores: A_+ = (A + A sign(A)) / 2, the matrix sign function from
+// Newton-Schulz iterations on X0 = A / |A|_F (every eigenvalue in [-1, 1]):
+// NS_FAST steps of the quintic X <- X (a I + b X^2 + c X^4), which lifts small
+// eigenvalues by about 3.4x per step into [0.7, 1.2] in magnitude (the
+// coefficients of K. Jordan's Muon orthogonalisation, 2024), then NS_POLISH
+// steps of the cubic X <- X (3 I - X^2) / 2, which converges quadratically to
+// sign(A) there.  An eigenvalue too small to be lifted in time contributes
+// at most its own magnitude, below 1e-6 of |A| after these steps: the result
+// agrees with the eigendecomposition's to FP32 rounding of |A|.
 #include "dispatch.hpp"
 
 namespace pfgpu {
@@ -226,6 +237,96 @@ __global__ __launch_bounds__(PSD_WAVE) void k_psd_project(int n_elements, T *__r
     }
 }
 
+// ---- Newton-Schulz projection on the matrix cores (FP32, P2)
+
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+constexpr int NS_FAST = 8, NS_POLISH = 10;
+constexpr float NS_A = 3.4445f, NS_B = -4.7750f, NS_C = 2.0315f;
+constexpr int NSS = 33; // row stride of a 32x32 matrix in LDS (conflict-free rows and columns)
+
+// P Q for 32x32 matrices in LDS, with v_mfma_f32_32x32x2f32 over k = 2s, 2s+1:
+// lane l supplies P[l % 32][2s + l / 32] and Q[2s + l / 32][l % 32]
+__device__ __forceinline__ floatx16 ns_mul(const float *P, const float *Q, int i0, int h)
+{
+    floatx16 acc = {};
+#pragma unroll
+    for (int s = 0; s < 16; ++s)
+        acc = __builtin_amdgcn_mfma_f32_32x32x2f32(P[i0 * NSS + 2 * s + h], Q[(2 * s + h) * NSS + i0], acc, 0, 0, 0);
+    return acc;
+}
+// row of accumulator r of lane half h (the column is l % 32)
+__device__ __forceinline__ int ns_row(int r, int h) { return 8 * (r / 4) + 4 * h + (r % 4); }
+__device__ __forceinline__ void ns_store(float *D, const floatx16 &v, int i0, int h)
+{
+#pragma unroll
+    for (int r = 0; r < 16; ++r) D[ns_row(r, h) * NSS + i0] = v[r];
+}
+
+// One wavefront per element (P2, 30x30 padded to 32x32 with zeros, which
+// every polynomial in X keeps zero).  The workgroup is the wavefront, so the
+// barriers only order the LDS accesses of its lanes.
+__global__ __launch_bounds__(64) void k_psd_ns(int n_elements, float *__restrict__ hloc)
+{
+    constexpr int NB = 10, NBLK = NB * (NB + 1) / 2;
+    // A, the iterate X and a temporary W (X^2, then the polynomial factor):
+    // 12.7 KB, 12 wavefronts per CU
+    __shared__ float A[32 * NSS], X[32 * NSS], W[32 * NSS];
+    const int e = blockIdx.x, lane = threadIdx.x, i0 = lane % 32, h = lane / 32;
+    if (e >= n_elements) return; // uniform over the workgroup
+    float *base = hloc + (int64_t(e) / 64) * NBLK * 576 + (e % 64) * 9;
+    const BlockPairs<NB> &bp = block_pairs<NB>();
+    for (int k = lane; k < 32 * NSS; k += 64) A[k] = 0.f;
+    __syncthreads();
+    for (int i = lane; i < NBLK * 9; i += 64) {
+        const int s = i / 9, j = i % 9, r = 3 * bp.a[s] + j / 3, c = 3 * bp.b[s] + j % 3;
+        const float v = base[int64_t(s) * 576 + j];
+        A[r * NSS + c] = v;
+        A[c * NSS + r] = v;
+    }
+    __syncthreads();
+    float ss = 0.f;
+    for (int k = lane; k < 32 * 32; k += 64) {
+        const float v = A[(k / 32) * NSS + k % 32];
+        ss = fmaf(v, v, ss);
+    }
+    ss = wave_sum(ss);
+    const float sc = ss > 0.f ? 1.f / std::sqrt(ss) : 0.f; // a zero matrix stays zero
+    for (int k = lane; k < 32 * NSS; k += 64) X[k] = A[k] * sc;
+    __syncthreads();
+    for (int it = 0; it < NS_FAST + NS_POLISH; ++it) {
+        const floatx16 x2 = ns_mul(X, X, i0, h);
+        floatx16 m;
+        if (it < NS_FAST) {
+            ns_store(W, x2, i0, h);
+            __syncthreads();
+            const floatx16 x4 = ns_mul(W, W, i0, h);
+            __syncthreads(); // every lane has read W
+#pragma unroll
+            for (int r = 0; r < 16; ++r) m[r] = fmaf(NS_C, x4[r], NS_B * x2[r]) + (ns_row(r, h) == i0 ? NS_A : 0.f);
+        } else {
+#pragma unroll
+            for (int r = 0; r < 16; ++r) m[r] = -0.5f * x2[r] + (ns_row(r, h) == i0 ? 1.5f : 0.f);
+        }
+        ns_store(W, m, i0, h);
+        __syncthreads();
+        const floatx16 xn = ns_mul(X, W, i0, h);
+        __syncthreads(); // every lane has read X
+        ns_store(X, xn, i0, h);
+        __syncthreads();
+    }
+    const floatx16 ax = ns_mul(A, X, i0, h);
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const int i = ns_row(r, h);
+        W[i * NSS + i0] = 0.5f * (A[i * NSS + i0] + ax[r]);
+    }
+    __syncthreads();
+    for (int i = lane; i < NBLK * 9; i += 64) {
+        const int s = i / 9, j = i % 9, r = 3 * bp.a[s] + j / 3, c = 3 * bp.b[s] + j % 3;
+        base[int64_t(s) * 576 + j] = W[r * NSS + c];
+    }
+}
+
 } // namespace
 
 // hloc: the element-block tiles of k_hessian for every element of the handle
@@ -235,6 +336,8 @@ void psd_project_blocks(Handle &h, void *hloc)
     auto go = [&]<typename T>() {
         if (h.order == 1)
             k_psd_project<T, 1><<<h.n_elements, PSD_WAVE, 0, h.stream>>>(h.n_elements, static_cast<T *>(hloc));
+        else if constexpr (sizeof(T) == 4)
+            k_psd_ns<<<h.n_elements, 64, 0, h.stream>>>(h.n_elements, static_cast<float *>(hloc));
         else
             k_psd_project<T, 2><<<h.n_elements, PSD_WAVE, 0, h.stream>>>(h.n_elements, static_cast<T *>(hloc));
     };
