@@ -160,17 +160,20 @@ __global__ __launch_bounds__(64) void k_grad_factors(DevMesh<T> m, const double 
                                                      double *__restrict__ partial, T *__restrict__ rec)
 {
     constexpr int NB = NLocal<ORDER>::value, R = fac_size(NQ), G = 3 * NB;
-    __shared__ T srec[64 * (R + 1)];
-    __shared__ T sgr[64 * (G + 1)];
+    // one buffer for both stagings (records, then gradients): the workgroup's
+    // LDS sets how many wavefronts a CU holds
+    constexpr int S = R > G ? R : G;
+    __shared__ T srec[64 * (S + 1)];
+    T *sgr = srec;
     const int lane = threadIdx.x, e0 = blockIdx.x * 64, e = e0 + lane;
     double val = 0;
+    T g[NB][3];
     if (e < m.n_elements) {
         Elem<T, ORDER, NQ> el;
         load_geometry(m, e, el);
         gather(m, e, x, el.U);
         // a rule that weighs every point alike (the factored path takes no other)
         const T w = el.det * T(quad_weight(NQ, 0));
-        T g[NB][3];
 #pragma unroll
         for (int a = 0; a < NB; ++a)
 #pragma unroll
@@ -192,10 +195,6 @@ __global__ __launch_bounds__(64) void k_grad_factors(DevMesh<T> m, const double 
         val = double(acc);
 #pragma unroll
         for (int i = 0; i < R; ++i) srec[lane * (R + 1) + i] = r[i];
-#pragma unroll
-        for (int a = 0; a < NB; ++a)
-#pragma unroll
-            for (int d = 0; d < 3; ++d) sgr[lane * (G + 1) + a * 3 + d] = g[a][d];
     }
     __syncthreads();
     const int ne = min(64, m.n_elements - e0);
@@ -203,6 +202,14 @@ __global__ __launch_bounds__(64) void k_grad_factors(DevMesh<T> m, const double 
         const int le = i / R;
         rec[int64_t(e0) * R + i] = srec[le * (R + 1) + (i - le * R)];
     }
+    __syncthreads(); // the records are read before the gradients overwrite them
+    if (e < m.n_elements) {
+#pragma unroll
+        for (int a = 0; a < NB; ++a)
+#pragma unroll
+            for (int d = 0; d < 3; ++d) sgr[lane * (G + 1) + a * 3 + d] = g[a][d];
+    }
+    __syncthreads();
     for (int i = lane; i < ne * G; i += 64) {
         const int le = i / G;
         gloc[int64_t(e0) * G + i] = sgr[le * (G + 1) + (i - le * G)];
