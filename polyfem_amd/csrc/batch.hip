@@ -252,13 +252,28 @@ __global__ __launch_bounds__(BLOCK) void k_multi_gather(Groups<T> G, double *__r
     const int t = threadIdx.x;
     const int32_t sl = gv.slot[int64_t(c) * SEG + t];
     const int i0 = gv.chunk[c], i1 = gv.chunk[c + 1];
+    const int64_t node0 = gv.dof0 / 3;
+    if constexpr (W == 3) {
+        // every row's problem finished: zero rows, no loads (the few problems
+        // a chunk spans are tested by every thread alike)
+        if (gv.skip.flag) {
+            bool all = true;
+            for (int s = gv.skip.node_seg[i0]; s <= gv.skip.node_seg[i1 - 1]; ++s) all = all && gv.skip.flag[s] != 0.0;
+            if (all) {
+                for (int i = i0 + t; i < i1; i += BLOCK) {
+                    double *o = out + 3 * (node0 + i);
+                    o[0] = o[1] = o[2] = 0.0;
+                }
+                return;
+            }
+        }
+    }
     if (sl >= 0) {
         const T *l = gv.loc + int64_t(sl) * W;
 #pragma unroll
         for (int k = 0; k < W; ++k) sv[t * W + k] = double(l[k]);
     }
     __syncthreads();
-    const int64_t node0 = gv.dof0 / 3;
     for (int i = i0 + t; i < i1; i += BLOCK) {
         const int32_t mt = gv.rmeta[i];
         const int q0 = mt & 1023, q1 = q0 + (mt >> 10);

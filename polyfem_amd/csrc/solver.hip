@@ -1203,13 +1203,19 @@ __device__ double block_sum(double v, double *sh)
     return sh[0];
 }
 
-// part[c] = sum over chunk c's free dofs of a b
+// part[c] = sum over chunk c's free dofs of a b; with done, 0 for the chunks
+// of finished problems (cs: chunk -> problem), whose partials nobody reads
 __global__ __launch_bounds__(BLOCK) void k_chunk_dot(const double *__restrict__ a, const double *__restrict__ b,
                                                      const uint8_t *__restrict__ mask, const int32_t *__restrict__ cb,
-                                                     double *__restrict__ part)
+                                                     double *__restrict__ part, const int32_t *__restrict__ cs = nullptr,
+                                                     const double *__restrict__ done = nullptr)
 {
     __shared__ double sh[BLOCK];
     const int c = blockIdx.x;
+    if (done && done[cs[c]] != 0.0) { // uniform over the workgroup
+        if (threadIdx.x == 0) part[c] = 0.0;
+        return;
+    }
     double acc = 0;
     for (int i = cb[c] + threadIdx.x; i < cb[c + 1]; i += BLOCK)
         if (!mask || !mask[i]) acc = fma(a[i], b[i], acc);
@@ -1326,16 +1332,22 @@ __global__ __launch_bounds__(SEGW) void k_seg_alpha(const double *__restrict__ p
 }
 
 // y += alpha p, r -= alpha q on free dofs, z = P r over chunk c's nodes;
-// partials of the new r.z and r.r
+// partials of the new r.z and r.r.  The chunks of a finished problem (done)
+// keep their vectors and give zero partials, which nobody reads.
 __global__ __launch_bounds__(BLOCK) void k_chunk_pcg_update(double *__restrict__ y, double *__restrict__ r,
                                                             double *__restrict__ z, const double *__restrict__ p,
                                                             const double *__restrict__ q, const uint8_t *__restrict__ mask,
                                                             const float *__restrict__ pinv, const int32_t *__restrict__ cb,
                                                             const int32_t *__restrict__ cs, const double *__restrict__ alpha,
-                                                            double *__restrict__ part_rz, double *__restrict__ part_rr)
+                                                            const double *__restrict__ done, double *__restrict__ part_rz,
+                                                            double *__restrict__ part_rr)
 {
     __shared__ double sh[BLOCK];
     const int c = blockIdx.x;
+    if (done[cs[c]] != 0.0) { // uniform over the workgroup
+        if (threadIdx.x == 0) part_rz[c] = part_rr[c] = 0.0;
+        return;
+    }
     const double a = alpha[cs[c]];
     double rz = 0, rr = 0;
     for (int i = cb[c] / 3 + threadIdx.x; i < cb[c + 1] / 3; i += BLOCK) {
@@ -1488,10 +1500,11 @@ void seg_cg_iteration(Handle &h, const Work &w, const double *x, const uint8_t *
                       double tol2, const SegState &st)
 {
     hvp_any(h, x, w.p, w.q);
-    k_chunk_dot<<<ch.n, BLOCK, 0, h.stream>>>(w.p, w.q, mask, ch.cb.as<int32_t>(), ch.part.as<double>());
+    k_chunk_dot<<<ch.n, BLOCK, 0, h.stream>>>(w.p, w.q, mask, ch.cb.as<int32_t>(), ch.part.as<double>(),
+                                              ch.cs.as<int32_t>(), st.done);
     k_seg_alpha<<<1, SEGW, 0, h.stream>>>(ch.part.as<double>(), ch.scp.as<int32_t>(), nseg, st);
     k_chunk_pcg_update<<<ch.n, BLOCK, 0, h.stream>>>(w.y, w.r, w.z, w.p, w.q, mask, precond(h), ch.cb.as<int32_t>(),
-                                                     ch.cs.as<int32_t>(), st.alpha, ch.part.as<double>(),
+                                                     ch.cs.as<int32_t>(), st.alpha, st.done, ch.part.as<double>(),
                                                      ch.part2.as<double>());
     k_seg_beta<<<1, SEGW, 0, h.stream>>>(ch.part.as<double>(), ch.part2.as<double>(), ch.scp.as<int32_t>(), nseg, tol2,
                                           st);
