@@ -240,7 +240,8 @@ __global__ __launch_bounds__(BLOCK) void k_multi_diag(Groups<T> G, const double 
 // chunk c covers rows chunk[c] .. chunk[c+1]-1, whose contributions sit in
 // the slots c SEG .. (c+1) SEG - 1, rmeta[i] = q0 | count << 10).  W = 3:
 // FP64 sums into out (3 per node, global dofs; rows of finished problems
-// become 0); W = 6: the diagonal blocks, in T, into diag (9 per node).
+// become 0, or are left as they are when the whole chunk's problems have
+// finished); W = 6: the diagonal blocks, in T, into diag (9 per node).
 constexpr int SEG = BLOCK;
 template <typename T, int W>
 __global__ __launch_bounds__(BLOCK) void k_multi_gather(Groups<T> G, double *__restrict__ out, T *__restrict__ diag)
@@ -250,24 +251,19 @@ __global__ __launch_bounds__(BLOCK) void k_multi_gather(Groups<T> G, double *__r
     const int c = int(blockIdx.x) - gv.chunk0;
     if (c >= gv.n_chunks) return;
     const int t = threadIdx.x;
-    const int32_t sl = gv.slot[int64_t(c) * SEG + t];
     const int i0 = gv.chunk[c], i1 = gv.chunk[c + 1];
     const int64_t node0 = gv.dof0 / 3;
     if constexpr (W == 3) {
-        // every row's problem finished: zero rows, no loads (the few problems
-        // a chunk spans are tested by every thread alike)
+        // every row's problem finished: nothing is read or written (the PCG
+        // reads no row of a finished problem; the few problems a chunk spans
+        // are tested by every thread alike)
         if (gv.skip.flag) {
             bool all = true;
             for (int s = gv.skip.node_seg[i0]; s <= gv.skip.node_seg[i1 - 1]; ++s) all = all && gv.skip.flag[s] != 0.0;
-            if (all) {
-                for (int i = i0 + t; i < i1; i += BLOCK) {
-                    double *o = out + 3 * (node0 + i);
-                    o[0] = o[1] = o[2] = 0.0;
-                }
-                return;
-            }
+            if (all) return;
         }
     }
+    const int32_t sl = gv.slot[int64_t(c) * SEG + t];
     if (sl >= 0) {
         const T *l = gv.loc + int64_t(sl) * W;
 #pragma unroll
