@@ -1313,11 +1313,19 @@ __global__ __launch_bounds__(SEGW) void k_seg_cg_start(const double *__restrict_
     if (threadIdx.x == 0) *st.active = a;
 }
 
+// Per-iteration scalar kernels: one wavefront per problem over a grid of
+// seg_grid(nseg) workgroups (problem s on wavefront s mod 16 of workgroup
+// s / 16); the count of problems still iterating is added into *active by
+// each workgroup (exact: a sum of small integers in FP64), which k_seg_alpha
+// clears first.
+inline unsigned seg_grid(int nseg) { return unsigned(std::max(1, (nseg + SEGW / WAVE - 1) / (SEGW / WAVE))); }
+
 // alpha = r.z / p.Hp per problem; a non-positive curvature ends that problem
 __global__ __launch_bounds__(SEGW) void k_seg_alpha(const double *__restrict__ part, const int32_t *__restrict__ scp,
                                                     int nseg, SegState st)
 {
-    for (int s = int(threadIdx.x / WAVE); s < nseg; s += SEGW / WAVE) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *st.active = 0.0;
+    for (int s = int(blockIdx.x * (SEGW / WAVE) + threadIdx.x / WAVE); s < nseg; s += int(gridDim.x) * (SEGW / WAVE)) {
         const double pq = seg_total(part, scp, s);
         if (threadIdx.x % WAVE != 0) continue;
         if (st.done[s] != 0.0) {
@@ -1391,7 +1399,7 @@ __global__ __launch_bounds__(SEGW) void k_seg_beta(const double *__restrict__ pa
 {
     __shared__ double sh[SEGW / WAVE];
     double act = 0;
-    for (int s = int(threadIdx.x / WAVE); s < nseg; s += SEGW / WAVE) {
+    for (int s = int(blockIdx.x * (SEGW / WAVE) + threadIdx.x / WAVE); s < nseg; s += int(gridDim.x) * (SEGW / WAVE)) {
         const double rn = seg_total(part_rr, scp, s), zn = seg_total(part_rz, scp, s);
         if (threadIdx.x % WAVE != 0) continue;
         if (!isfinite(rn) || !isfinite(zn)) act += __builtin_nan(""); // reported to the host as a failed solve
@@ -1407,7 +1415,7 @@ __global__ __launch_bounds__(SEGW) void k_seg_beta(const double *__restrict__ pa
         act += st.done[s] == 0.0 ? 1.0 : 0.0;
     }
     const double a = wave_values_sum(act, sh);
-    if (threadIdx.x == 0) *st.active = a;
+    if (threadIdx.x == 0) atomicAdd(st.active, a);
 }
 
 // p = z + beta p for the problems still iterating
@@ -1502,11 +1510,12 @@ void seg_cg_iteration(Handle &h, const Work &w, const double *x, const uint8_t *
     hvp_any(h, x, w.p, w.q);
     k_chunk_dot<<<ch.n, BLOCK, 0, h.stream>>>(w.p, w.q, mask, ch.cb.as<int32_t>(), ch.part.as<double>(),
                                               ch.cs.as<int32_t>(), st.done);
-    k_seg_alpha<<<1, SEGW, 0, h.stream>>>(ch.part.as<double>(), ch.scp.as<int32_t>(), nseg, st);
+    k_seg_alpha<<<seg_grid(nseg), SEGW, 0, h.stream>>>(ch.part.as<double>(), ch.scp.as<int32_t>(), nseg, st);
     k_chunk_pcg_update<<<ch.n, BLOCK, 0, h.stream>>>(w.y, w.r, w.z, w.p, w.q, mask, precond(h), ch.cb.as<int32_t>(),
                                                      ch.cs.as<int32_t>(), st.alpha, st.done, ch.part.as<double>(),
                                                      ch.part2.as<double>());
-    k_seg_beta<<<1, SEGW, 0, h.stream>>>(ch.part.as<double>(), ch.part2.as<double>(), ch.scp.as<int32_t>(), nseg, tol2,
+    k_seg_beta<<<seg_grid(nseg), SEGW, 0, h.stream>>>(ch.part.as<double>(), ch.part2.as<double>(), ch.scp.as<int32_t>(),
+                                                      nseg, tol2,
                                           st);
     k_chunk_cg_direction<<<ch.n, BLOCK, 0, h.stream>>>(w.p, w.z, ch.cb.as<int32_t>(), ch.cs.as<int32_t>(), st.beta,
                                                        st.done);
