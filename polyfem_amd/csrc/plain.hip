@@ -422,10 +422,17 @@ __global__ __launch_bounds__(HB) void k_hessian(DevMesh<T> m, const double *__re
 
 // local H(x) v; yloc[(e*NB + a)*3 + d].  One wavefront per workgroup: the
 // 64 elements' results are staged in LDS and stored as one contiguous run
-// (elements of finished problems compute and store nothing)
+// (elements of finished problems compute and store nothing).  FP32 with the
+// NeoHookean, linear and SaintVenant tangents fits 168 VGPRs without spills:
+// three wavefronts per SIMD instead of two.
+template <typename T, int MAT> constexpr int hvp_waves()
+{
+    return sizeof(T) == 4 && (MAT == MAT_NH || MAT == MAT_LE || MAT == MAT_SV) ? 3 : 1;
+}
 template <typename T, int ORDER, int NQ, int MAT>
-__global__ __launch_bounds__(64) void k_hvp(DevMesh<T> m, const double *__restrict__ x, const double *__restrict__ v,
-                                            T *__restrict__ yloc, SegSkip skip)
+__global__ __launch_bounds__(64, (hvp_waves<T, MAT>())) void k_hvp(DevMesh<T> m, const double *__restrict__ x,
+                                                                   const double *__restrict__ v, T *__restrict__ yloc,
+                                                                   SegSkip skip)
 {
     constexpr int NB = NLocal<ORDER>::value, G = 3 * NB;
     __shared__ T sy[64 * (G + 1)];
