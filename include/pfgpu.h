@@ -174,10 +174,10 @@ pfgpu_status pfgpu_hessian_pattern(const pfgpu_handle *h, int32_t *outer, int32_
 /* Hessian values in pattern order: double* for FP64, float* for MIXED.
  * project_to_psd != 0: every element's local Hessian is projected onto the
  * positive semidefinite cone before the sum, as NLAssembler::assemble_hessian
- * does with ipc::project_to_psd (eigenvalues <= 0 clamped to 0).  FP64 and P1:
- * by eigendecomposition (a positive definite matrix is left as it is); MIXED
- * P2: by the matrix sign function, which agrees with it to about 1e-6 of the
- * element matrix's largest entry. */
+ * does with ipc::project_to_psd (eigenvalues <= 0 clamped to 0).  FP64: by
+ * eigendecomposition (a positive definite matrix is left as it is); MIXED: by
+ * the matrix sign function, which agrees with it to about 1e-6 of the element
+ * matrix's largest entry. */
 pfgpu_status pfgpu_hessian_values(pfgpu_handle *h, const double *x, int32_t project_to_psd, void *values);
 /* energy, gradient and Hessian values from one pass over the mesh; any
  * output pointer may be NULL */

@@ -14,7 +14,7 @@
 // once, n*3-1 rounds per sweep) until the off-diagonal part is at rounding
 // level, and written back when some eigenvalue is negative.
 //
-// FP32 P2 matrices take a second method built on 32x32 matrix products on
+// FP32 matrices take a second method built on 32x32 matrix products on
 // the matrix cores: A_+ = (A + A sign(A)) / 2, the matrix sign function from
 // Newton-Schulz iterations on X0 = A / |A|_F (every eigenvalue in [-1, 1]):
 // NS_FAST steps of the quintic X <- X (a I + b X^2 + c X^4), which lifts small
@@ -262,36 +262,47 @@ __device__ __forceinline__ void ns_store(float *D, const floatx16 &v, int i0, in
     for (int r = 0; r < 16; ++r) D[ns_row(r, h) * NSS + i0] = v[r];
 }
 
-// One wavefront per element (P2, 30x30 padded to 32x32 with zeros, which
-// every polynomial in X keeps zero).  The workgroup is the wavefront, so the
-// barriers only order the LDS accesses of its lanes.
-__global__ __launch_bounds__(64) void k_psd_ns(int n_elements, float *__restrict__ hloc)
+// One wavefront per P2 element (30x30 padded to 32x32 with zeros, which
+// every polynomial in X keeps zero) or per two P1 elements (12x12 matrices
+// at rows and columns 0 and 16 of a block-diagonal 32x32, each scaled by its
+// own norm: polynomials in a block-diagonal X act on each block alone).  The
+// workgroup is the wavefront, so the barriers only order the LDS accesses of
+// its lanes.
+template <int ORDER> __global__ __launch_bounds__(64) void k_psd_ns(int n_elements, float *__restrict__ hloc)
 {
-    constexpr int NB = 10, NBLK = NB * (NB + 1) / 2;
+    constexpr int NB = ORDER == 1 ? 4 : 10, N = 3 * NB, NBLK = NB * (NB + 1) / 2, EPW = ORDER == 1 ? 2 : 1;
     // A, the iterate X and a temporary W (X^2, then the polynomial factor):
     // 12.7 KB, 12 wavefronts per CU
     __shared__ float A[32 * NSS], X[32 * NSS], W[32 * NSS];
-    const int e = blockIdx.x, lane = threadIdx.x, i0 = lane % 32, h = lane / 32;
-    if (e >= n_elements) return; // uniform over the workgroup
-    float *base = hloc + (int64_t(e) / 64) * NBLK * 576 + (e % 64) * 9;
+    const int lane = threadIdx.x, i0 = lane % 32, h = lane / 32;
+    const int e0 = int(blockIdx.x) * EPW;
+    if (e0 >= n_elements) return; // uniform over the workgroup
+    const int ne = min(EPW, n_elements - e0);
     const BlockPairs<NB> &bp = block_pairs<NB>();
+    auto base = [&](int e) { return hloc + (int64_t(e) / 64) * NBLK * 576 + (e % 64) * 9; };
     for (int k = lane; k < 32 * NSS; k += 64) A[k] = 0.f;
     __syncthreads();
-    for (int i = lane; i < NBLK * 9; i += 64) {
-        const int s = i / 9, j = i % 9, r = 3 * bp.a[s] + j / 3, c = 3 * bp.b[s] + j % 3;
-        const float v = base[int64_t(s) * 576 + j];
+    for (int i = lane; i < ne * NBLK * 9; i += 64) {
+        const int k = i / (NBLK * 9), ii = i - k * NBLK * 9, s = ii / 9, j = ii % 9;
+        const int r = 16 * k + 3 * bp.a[s] + j / 3, c = 16 * k + 3 * bp.b[s] + j % 3;
+        const float v = base(e0 + k)[int64_t(s) * 576 + j];
         A[r * NSS + c] = v;
         A[c * NSS + r] = v;
     }
     __syncthreads();
-    float ss = 0.f;
-    for (int k = lane; k < 32 * 32; k += 64) {
-        const float v = A[(k / 32) * NSS + k % 32];
-        ss = fmaf(v, v, ss);
+    // the Frobenius norm of each element's matrix
+    float sc[EPW];
+#pragma unroll
+    for (int k = 0; k < EPW; ++k) {
+        float ss = 0.f;
+        for (int q = lane; q < N * N; q += 64) {
+            const float v = A[(16 * k + q / N) * NSS + 16 * k + q % N];
+            ss = fmaf(v, v, ss);
+        }
+        ss = wave_sum(ss);
+        sc[k] = ss > 0.f ? 1.f / std::sqrt(ss) : 0.f; // a zero matrix stays zero
     }
-    ss = wave_sum(ss);
-    const float sc = ss > 0.f ? 1.f / std::sqrt(ss) : 0.f; // a zero matrix stays zero
-    for (int k = lane; k < 32 * NSS; k += 64) X[k] = A[k] * sc;
+    for (int k = lane; k < 32 * NSS; k += 64) X[k] = A[k] * (EPW == 1 || k / NSS < 16 ? sc[0] : sc[EPW - 1]);
     __syncthreads();
     for (int it = 0; it < NS_FAST + NS_POLISH; ++it) {
         const floatx16 x2 = ns_mul(X, X, i0, h);
@@ -321,9 +332,10 @@ __global__ __launch_bounds__(64) void k_psd_ns(int n_elements, float *__restrict
         W[i * NSS + i0] = 0.5f * (A[i * NSS + i0] + ax[r]);
     }
     __syncthreads();
-    for (int i = lane; i < NBLK * 9; i += 64) {
-        const int s = i / 9, j = i % 9, r = 3 * bp.a[s] + j / 3, c = 3 * bp.b[s] + j % 3;
-        base[int64_t(s) * 576 + j] = W[r * NSS + c];
+    for (int i = lane; i < ne * NBLK * 9; i += 64) {
+        const int k = i / (NBLK * 9), ii = i - k * NBLK * 9, s = ii / 9, j = ii % 9;
+        const int r = 16 * k + 3 * bp.a[s] + j / 3, c = 16 * k + 3 * bp.b[s] + j % 3;
+        base(e0 + k)[int64_t(s) * 576 + j] = W[r * NSS + c];
     }
 }
 
@@ -334,12 +346,17 @@ void psd_project_blocks(Handle &h, void *hloc)
 {
     if (h.n_elements == 0) return;
     auto go = [&]<typename T>() {
-        if (h.order == 1)
-            k_psd_project<T, 1><<<h.n_elements, PSD_WAVE, 0, h.stream>>>(h.n_elements, static_cast<T *>(hloc));
-        else if constexpr (sizeof(T) == 4)
-            k_psd_ns<<<h.n_elements, 64, 0, h.stream>>>(h.n_elements, static_cast<float *>(hloc));
-        else
-            k_psd_project<T, 2><<<h.n_elements, PSD_WAVE, 0, h.stream>>>(h.n_elements, static_cast<T *>(hloc));
+        if constexpr (sizeof(T) == 4) {
+            if (h.order == 1)
+                k_psd_ns<1><<<(h.n_elements + 1) / 2, 64, 0, h.stream>>>(h.n_elements, static_cast<float *>(hloc));
+            else
+                k_psd_ns<2><<<h.n_elements, 64, 0, h.stream>>>(h.n_elements, static_cast<float *>(hloc));
+        } else {
+            if (h.order == 1)
+                k_psd_project<T, 1><<<h.n_elements, PSD_WAVE, 0, h.stream>>>(h.n_elements, static_cast<T *>(hloc));
+            else
+                k_psd_project<T, 2><<<h.n_elements, PSD_WAVE, 0, h.stream>>>(h.n_elements, static_cast<T *>(hloc));
+        }
     };
     if (h.precision == PFGPU_MIXED)
         go.template operator()<float>();

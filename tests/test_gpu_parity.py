@@ -9,7 +9,7 @@ import pytest
 
 from oracle import pyoracle as po
 from polyfem_amd import NLAssembler, PfgpuError
-from polyfem_amd.mesh import BASE_SEED, kuhn_box, lame_from_young, random_lame, smooth_displacement
+from polyfem_amd.mesh import BASE_SEED, kuhn_box, lame_from_young, random_lame, smooth_displacement, tet_mesh
 
 pytestmark = pytest.mark.gpu
 
@@ -255,20 +255,25 @@ def test_project_to_psd(order, name, precision):
     assert rel(a.hessian_values(x), plain) < (FP64_TOL if precision == "fp64" else MIXED_TOL["hessian"])
 
 
+@pytest.mark.parametrize("order", [1, 2])
 @pytest.mark.parametrize("compress", [0.0, 0.45])
-def test_project_to_psd_p2_mixed_sign_iteration(compress):
-    """FP32 P2 element Hessians take the matrix-core projection (Newton-Schulz
-    iterations for sign(A)): against the oracle's LAPACK projection on a 4^3
-    mesh, with elements of zero Lame parameters (zero Hessians, which must
-    stay zero), at 1e-5 relative, ten times tighter than the FP32 bar of the
-    eigendecomposition test above."""
-    m = kuhn_box(4, 4, 4, order=2)
+def test_project_to_psd_mixed_sign_iteration(order, compress):
+    """FP32 element Hessians take the matrix-core projection (Newton-Schulz
+    iterations for sign(A); P1 with two elements per product, so the P1 mesh
+    has an odd element count to leave the last element alone): against the
+    oracle's LAPACK projection on a 4^3 mesh, with elements of zero Lame
+    parameters (zero Hessians, which must stay zero), at 1e-5 relative, ten
+    times tighter than the FP32 bar of the eigendecomposition test above."""
+    m = kuhn_box(4, 4, 4, order=order)
+    if order == 1:
+        m = tet_mesh(m.vertices, m.elem_vertices[:-1], order=1)
+        assert m.n_elements % 2 == 1
     lam, mu = random_lame(m.n_elements, seed=BASE_SEED + 112)
     lam[::17] = 0.0
     mu[::17] = 0.0
     x = smooth_displacement(m.nodes, 0.05, 1e-3, BASE_SEED + 113)
     x[2::3] -= compress * m.nodes[:, 2]
-    print(f"psd sign iteration compress={compress}", flush=True)
+    print(f"psd sign iteration order={order} compress={compress}", flush=True)
     a = NLAssembler("NeoHookean", m, lame=(lam, mu), precision="mixed")
     o = po.Oracle(m, lam, mu)
     _, _, ref = o.hessian_psd(x)
