@@ -479,79 +479,114 @@ __global__ __launch_bounds__(64, (hvp_waves<T, MAT>())) void k_hvp(DevMesh<T> m,
 // G the record's metric tail.  Valid while the records belong to x (inside a
 // CG solve, where x is fixed); about half the arithmetic of k_hvp and no
 // gather of x.  Layout and staging as k_hvp.
-template <typename T, int ORDER, int NQ>
-__global__ __launch_bounds__(64) void k_hvp_factored(DevMesh<T> m, const T *__restrict__ rec, const double *__restrict__ v,
-                                                     T *__restrict__ yloc, SegSkip skip)
+// PARTS = 2 (rules of an even number of points): two wavefronts per 64
+// elements, each taking half of the points and reading only their part of
+// the record; the halves are added in LDS (first + second), so twice the
+// wavefronts share the kernel's loads and latency.
+template <typename T, int ORDER, int NQ, int PARTS>
+__global__ __launch_bounds__(64 * PARTS) void k_hvp_factored(DevMesh<T> m, const T *__restrict__ rec,
+                                                             const double *__restrict__ v, T *__restrict__ yloc,
+                                                             SegSkip skip)
 {
-    constexpr int NB = NLocal<ORDER>::value, G = 3 * NB, R = fac_size(NQ);
+    constexpr int NB = NLocal<ORDER>::value, G = 3 * NB, R = fac_size(NQ), QP = NQ / PARTS;
+    static_assert(NQ % PARTS == 0, "the parts take equal numbers of points");
     __shared__ T sy[64 * (G + 1)];
-    const int lane = threadIdx.x, e0 = blockIdx.x * 64, e = e0 + lane;
-    if (e < m.n_elements && !skip.element(e)) {
+    const int lane = threadIdx.x % 64, w = threadIdx.x / 64, e0 = blockIdx.x * 64, e = e0 + lane;
+    const bool on = e < m.n_elements && !skip.element(e);
+    T y[NB][3];
+#pragma unroll
+    for (int a = 0; a < NB; ++a) y[a][0] = y[a][1] = y[a][2] = T(0);
+    if (on) {
         T V[NB][3];
         gather(m, e, v, V);
-        T r[R];
         const T *rp = static_cast<const T *>(__builtin_assume_aligned(rec, 16)) + int64_t(e) * R;
-#pragma unroll
-        for (int i = 0; i < R; ++i) r[i] = rp[i];
-        const T *G6 = r + 11 * NQ; // 00 11 22 01 02 12
+        const T *G6 = rp + 11 * NQ; // 00 11 22 01 02 12
         const T Gm[9] = {G6[0], G6[3], G6[4], G6[3], G6[1], G6[5], G6[4], G6[5], G6[2]};
-        T y[NB][3];
+        // the points q = W QP .. (W + 1) QP - 1 (W uniform over the wavefront)
+        auto points = [&]<int W>() {
 #pragma unroll
-        for (int a = 0; a < NB; ++a) y[a][0] = y[a][1] = y[a][2] = T(0);
+            for (int qq = 0; qq < QP; ++qq) {
+                const int q = W * QP + qq;
+                T M[9];
 #pragma unroll
-        for (int q = 0; q < NQ; ++q) {
-            const T *M = r + 9 * q;
-            T D[9];
+                for (int i = 0; i < 9; ++i) M[i] = rp[9 * q + i];
+                T D[9];
 #pragma unroll
-            for (int i = 0; i < 9; ++i) D[i] = T(0);
+                for (int i = 0; i < 9; ++i) D[i] = T(0);
 #pragma unroll
-            for (int b = 0; b < NB; ++b)
+                for (int b = 0; b < NB; ++b)
 #pragma unroll
-                for (int k = 0; k < 3; ++k) {
-                    const double rg = RG<ORDER, NQ>(q, b, k);
-                    if (rg != 0.0)
+                    for (int k = 0; k < 3; ++k) {
+                        const double rg = RG<ORDER, NQ>(q, b, k);
+                        if (rg != 0.0)
 #pragma unroll
-                        for (int n = 0; n < 3; ++n) D[n * 3 + k] = fma_t(V[b][n], T(rg), D[n * 3 + k]);
-                }
-            T s1 = T(0);
+                            for (int n = 0; n < 3; ++n) D[n * 3 + k] = fma_t(V[b][n], T(rg), D[n * 3 + k]);
+                    }
+                T s1 = T(0);
 #pragma unroll
-            for (int i = 0; i < 9; ++i) s1 = fma_t(M[i], D[i], s1);
-            const T cl = r[9 * NQ + q], cc = r[10 * NQ + q], ls = cl * s1;
-            // W = cc M D^T
-            T W[9];
+                for (int i = 0; i < 9; ++i) s1 = fma_t(M[i], D[i], s1);
+                const T cl = rp[9 * NQ + q], cc = rp[10 * NQ + q], ls = cl * s1;
+                // W = cc M D^T
+                T Wm[9];
 #pragma unroll
-            for (int mm = 0; mm < 3; ++mm)
+                for (int mm = 0; mm < 3; ++mm)
 #pragma unroll
-                for (int nn = 0; nn < 3; ++nn)
-                    W[mm * 3 + nn] = cc * (M[mm * 3] * D[nn * 3] + M[mm * 3 + 1] * D[nn * 3 + 1] + M[mm * 3 + 2] * D[nn * 3 + 2]);
-            T Z[9];
+                    for (int nn = 0; nn < 3; ++nn)
+                        Wm[mm * 3 + nn] =
+                            cc * (M[mm * 3] * D[nn * 3] + M[mm * 3 + 1] * D[nn * 3 + 1] + M[mm * 3 + 2] * D[nn * 3 + 2]);
+                T Z[9];
 #pragma unroll
-            for (int mm = 0; mm < 3; ++mm)
+                for (int mm = 0; mm < 3; ++mm)
 #pragma unroll
-                for (int k = 0; k < 3; ++k) {
-                    T z = ls * M[mm * 3 + k];
+                    for (int k = 0; k < 3; ++k) {
+                        T z = ls * M[mm * 3 + k];
 #pragma unroll
-                    for (int j = 0; j < 3; ++j) z = fma_t(W[mm * 3 + j], M[j * 3 + k], fma_t(D[mm * 3 + j], Gm[j * 3 + k], z));
-                    Z[mm * 3 + k] = z;
-                }
+                        for (int j = 0; j < 3; ++j)
+                            z = fma_t(Wm[mm * 3 + j], M[j * 3 + k], fma_t(D[mm * 3 + j], Gm[j * 3 + k], z));
+                        Z[mm * 3 + k] = z;
+                    }
+#pragma unroll
+                for (int a = 0; a < NB; ++a)
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) {
+                        const double rg = RG<ORDER, NQ>(q, a, k);
+                        if (rg != 0.0)
+#pragma unroll
+                            for (int mm = 0; mm < 3; ++mm) y[a][mm] = fma_t(Z[mm * 3 + k], T(rg), y[a][mm]);
+                    }
+            }
+        };
+        if constexpr (PARTS == 1) {
+            points.template operator()<0>();
+        } else {
+            if (w == 0)
+                points.template operator()<0>();
+            else
+                points.template operator()<1>();
+        }
+    }
+    if constexpr (PARTS == 2) {
+        if (w == 1 && on)
 #pragma unroll
             for (int a = 0; a < NB; ++a)
 #pragma unroll
-                for (int k = 0; k < 3; ++k) {
-                    const double rg = RG<ORDER, NQ>(q, a, k);
-                    if (rg != 0.0)
+                for (int d = 0; d < 3; ++d) sy[lane * (G + 1) + a * 3 + d] = y[a][d];
+        __syncthreads();
+        if (w == 0 && on)
 #pragma unroll
-                        for (int mm = 0; mm < 3; ++mm) y[a][mm] = fma_t(Z[mm * 3 + k], T(rg), y[a][mm]);
-                }
-        }
+            for (int a = 0; a < NB; ++a)
+#pragma unroll
+                for (int d = 0; d < 3; ++d) y[a][d] += sy[lane * (G + 1) + a * 3 + d];
+        __syncthreads(); // the second half is read before the sums overwrite it
+    }
+    if (w == 0 && on)
 #pragma unroll
         for (int a = 0; a < NB; ++a)
 #pragma unroll
             for (int d = 0; d < 3; ++d) sy[lane * (G + 1) + a * 3 + d] = y[a][d];
-    }
     __syncthreads();
     const int ne = min(64, m.n_elements - e0);
-    for (int i = lane; i < ne * G; i += 64) {
+    for (int i = int(threadIdx.x); i < ne * G; i += 64 * PARTS) {
         const int le = i / G;
         if (!skip.element(e0 + le)) yloc[int64_t(e0) * G + i] = sy[le * (G + 1) + (i - le * G)];
     }
@@ -1925,7 +1960,9 @@ void factored_hvp(Handle &h, const double *v, double *y)
     const unsigned grid = grid_for(h.n_elements, 64);
     dispatch<true>(h, [&]<typename T, int O, int Q, int M>() {
         if constexpr (M == MAT_NH || M == MAT_LE) {
-            k_hvp_factored<T, O, Q><<<grid, 64, 0, h.stream>>>(h.mesh<T>(), h.fac_rec.as<T>(), v, h.loc.as<T>(), h.skip);
+            constexpr int PARTS = Q % 2 == 0 ? 2 : 1;
+            k_hvp_factored<T, O, Q, PARTS>
+                <<<grid, 64 * PARTS, 0, h.stream>>>(h.mesh<T>(), h.fac_rec.as<T>(), v, h.loc.as<T>(), h.skip);
             PF_LAUNCH_CHECK();
             gather_rows(h, h.loc.get(), y);
         } else {
