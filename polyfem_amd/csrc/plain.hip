@@ -1253,7 +1253,13 @@ __global__ __launch_bounds__(GS) void k_gather_units(int n_chunks, int n_words, 
         const int c2 = c1 + stride;
         const Meta m2 = meta(c2);
         const int code2 = code_of(m2);
-        // ---- chunk c: blocks of this thread's unit into the stage
+        // ---- chunk c: blocks of this thread's unit into the stage.  The
+        // formation (VALU-bound) issues at a raised wave priority and the rest
+        // of the chunk (LDS-bound sums, tables, gradient) at the normal one:
+        // the workgroups of a CU drift into different phases, so one's
+        // formation overlaps another's sums (the gather ran about 4 % faster
+        // with either phase raised than with none)
+        __builtin_amdgcn_s_setprio(1);
         if (ut < m0.nu) {
             const int a = code0 - (code0 / NB) * NB;
             const T *r = u0.r;
@@ -1336,6 +1342,7 @@ __global__ __launch_bounds__(GS) void k_gather_units(int n_chunks, int n_words, 
                     form.template operator()<NB / 2, NB>();
             }
         }
+        __builtin_amdgcn_s_setprio(0);
         // ---- chunk c: row and block tables
         if (t <= m0.nr) {
             srb[t] = u0.rb0 - m0.B0;
